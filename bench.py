@@ -1,0 +1,200 @@
+#!/usr/bin/env python3
+"""Benchmark of the Rayleigh–Bénard step on one MI355X.
+
+Workload (the benchmark-scale plain configuration): 4096 x 2048 lattice, Ra = 1e8, Pr = 1,
+linear conduction profile + seeded uniform noise (1e-3) on T, u = 0; 1e5 tracers + 1e5
+inertial particles (Stokes number log-uniform in [0.01, 1], tau_p = St * t_ff with
+t_ff = (ny-1)/u_ff, zero initial slip), uniform random positions; trajectory records every
+20 steps, delivered to the host.  One "step" = the fused f/g LBM kernel + the particle
+kernel (+ the record on record steps).  Inputs are resident on the GPU before timing starts.
+
+Prints one JSON line: MLUPS of the FP32 (shifted-population) path, the HBM roofline of the
+dominant kernel (fused LBM step; HIP events around every launch in the timed region), the
+FP64 path's MLUPS on the same workload, and the CPU FP64 restatement of the reference loop
+timed on a bounded sample on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "fwd MLUPS & %HBM peak; ensemble MLUPS + adjoint/fwd cost vs plain, 1 GPU"
+HBM_PEAK_GBPS = 8000.0
+NX, NY = 4096, 2048
+RA, PR, U_FF = 1e8, 1.0, 0.05
+N_TRACER, N_INERTIAL = 100_000, 100_000
+RECORD_EVERY = 20
+
+
+def make_params(rbn, precision):
+    p = rbn.default_params(nx=NX, ny=NY, Tup=0.0, Tdown=1.0, scratch=1, nstart=0, nout=10**9,
+                           noutconfig=RECORD_EVERY, prtcl_writefull=0, particle=1,
+                           precision=precision, T0_shift=0.5, rho0_shift=1.0)
+    return rbn.params_from_ra(p, RA, PR, U_FF)
+
+
+def make_inputs(p, nx=NX, ny=NY, n_tracer=N_TRACER, n_inertial=N_INERTIAL):
+    rng = np.random.default_rng(1)
+    j = np.arange(ny)
+    T0 = np.tile(p.Tdown + (p.Tup - p.Tdown) * j / (ny - 1), (nx, 1)) + rng.uniform(-1e-3, 1e-3, (nx, ny))
+    n = n_tracer + n_inertial
+    r3 = np.random.default_rng(3)
+    x = r3.uniform(0.0, nx, n)
+    x[x == 0.0] = 0.5
+    y = r3.uniform(1.0, ny, n)
+    r4 = np.random.default_rng(4)
+    t_ff = (ny - 1) / U_FF
+    st = 10.0 ** r4.uniform(-2.0, 0.0, n_inertial)
+    tau = np.concatenate([np.zeros(n_tracer), st * t_ff])
+    ids = np.arange(n, dtype=np.int32)
+    zeros = np.zeros(n)
+    return T0, (ids, x, y, tau, zeros)
+
+
+def lbm_bytes_per_node(precision):
+    # 18 population loads + 18 stores, lagged force load + store, u and v stores.
+    return 40 * (8 if precision == 0 else 4)
+
+
+def run_gpu(rbn, precision, steps, warmup, timing=True):
+    p = make_params(rbn, precision)
+    T0, (ids, x, y, tau, zeros) = make_inputs(p)
+    s = rbn.Solver(p, 0)
+    s.init_fields(T=T0)
+    s.init_particles(ids, x, y, tau_p=tau, up=zeros, vp=zeros)
+    s.discard_records()
+    s.step(warmup)
+    s.synchronize()
+    if timing:
+        s.kernel_timing(True)
+        for k in ("lbm", "particles"):
+            s.kernel_stats(k, reset=True)
+    t0 = time.perf_counter()
+    s.step(steps)
+    s.synchronize()
+    wall = time.perf_counter() - t0
+    out = {"wall_s": wall, "mlups": NX * NY * steps / wall / 1e6, "ms_per_step": 1e3 * wall / steps}
+    if timing:
+        n_lbm, ms_lbm = s.kernel_stats("lbm")
+        n_p, ms_p = s.kernel_stats("particles")
+        s.kernel_timing(False)
+        out.update(lbm_launches=n_lbm, lbm_avg_ms=ms_lbm / max(n_lbm, 1),
+                   particles_launches=n_p, particles_avg_ms=ms_p / max(n_p, 1))
+    fl = s.fields()
+    out["finite"] = bool(np.isfinite(fl["T"]).all() and np.isfinite(fl["u"]).all())
+    s.close()
+    return out
+
+
+def run_cpu_baseline(rbn, steps):
+    """The CPU FP64 restatement of the reference loop (array-of-structs, per-direction sweeps,
+    halo copies) on one host core, same workload, bounded to a few steps."""
+    from oracle.oracle import Oracle
+
+    p = make_params(rbn, 0)
+    T0, (ids, x, y, tau, zeros) = make_inputs(p)
+    o = Oracle(p)
+    o.set_faithful(True)
+    o.init_fields(T=T0)
+    o.init_particles(ids, x, y, tau_p=tau, up=zeros, vp=zeros)
+    o.step(1)
+    t0 = time.perf_counter()
+    o.step(steps)
+    dt = time.perf_counter() - t0
+    o.close()
+    return {"value": NX * NY * steps / dt / 1e6, "unit": "MLUPS", "cores": 1, "kind": "port",
+            "sample": f"oracle/rbn_oracle.c (CPU FP64 restatement of the reference truth-run loop, "
+                      f"cost-faithful mode), {NX}x{NY} + {N_TRACER}+{N_INERTIAL} particles, "
+                      f"{steps} steps after 1 warm-up step, 1 thread; host {platform.processor() or platform.machine()}, "
+                      f"nproc {os.cpu_count()}",
+            "seconds": dt}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 companion run")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build is a single-GPU solver; the multi-GPU slab decomposition is "
+              "design only (see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        return 2
+    from lagrangianrbnudging_amd import rbn
+
+    if rbn.device_count() == 0:
+        print("bench.py: no GPU visible", file=sys.stderr)
+        return 1
+    print(f"bench: fp32 {NX}x{NY} steps={args.steps} warmup={args.warmup}", file=sys.stderr, flush=True)
+    r32 = run_gpu(rbn, rbn.RBN_FP32, args.steps, args.warmup)
+    print(f"bench: fp32 done {r32}", file=sys.stderr, flush=True)
+    r64 = None
+    if not args.no_fp64:
+        r64 = run_gpu(rbn, rbn.RBN_FP64, max(args.steps // 2, 1), max(args.warmup // 2, 1))
+        print(f"bench: fp64 done {r64}", file=sys.stderr, flush=True)
+    cpu = None
+    if not args.no_cpu:
+        cpu = run_cpu_baseline(rbn, args.cpu_steps)
+        print(f"bench: cpu done {cpu}", file=sys.stderr, flush=True)
+
+    bytes_launch = NX * NY * lbm_bytes_per_node(1)
+    achieved = bytes_launch / (r32["lbm_avg_ms"] * 1e-3) / 1e9
+    line = {
+        "metric": METRIC,
+        "value": round(r32["mlups"], 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r32["ms_per_step"], 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"plain RB step {NX}x{NY}, Ra=1e8, Pr=1, FP32 shifted populations, "
+                        f"{N_TRACER} tracers + {N_INERTIAL} inertial (St in [0.01,1]), records every {RECORD_EVERY} steps",
+            "nx": NX, "ny": NY, "Ra": RA, "Pr": PR, "particles": N_TRACER + N_INERTIAL,
+            "record_every": RECORD_EVERY,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBPS, 4),
+            "traffic": None,
+            "kernel": "k_lbm32 (fused f/g collide+stream)",
+            "bytes_per_launch": bytes_launch,
+            "avg_launch_ms": round(r32["lbm_avg_ms"], 5),
+        },
+        "cpu_baseline": cpu and {k: (round(v, 3) if isinstance(v, float) else v) for k, v in cpu.items()},
+        "particles_avg_ms": round(r32["particles_avg_ms"], 5),
+        "fp64": r64 and {
+            "mlups": round(r64["mlups"], 1),
+            "ms_per_step": round(r64["ms_per_step"], 5),
+            "lbm_avg_ms": round(r64["lbm_avg_ms"], 5),
+            "roofline_frac": round(NX * NY * lbm_bytes_per_node(0) / (r64["lbm_avg_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
+        },
+        "finite": r32["finite"] and (r64 is None or r64["finite"]),
+    }
+    print(json.dumps(line))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,236 @@
+/*
+ * rbn.h — C-ABI of librbn, the MI355X (gfx950) Rayleigh–Bénard lattice-Boltzmann step
+ * with Lagrangian particles and trajectory nudging.
+ *
+ * The reference (Lokahith-Agasthya/LagrangianRBNudging) has no library interface: its
+ * "interface" is the call sequence of two C mains over process-global state, configured by
+ * `#define` blocks.  Every entry point below replaces one step of that sequence; the
+ * comment on each names the reference code it stands in for (paths relative to the
+ * reference repository root).
+ *
+ * Conventions
+ *  - Every function returns 0 (RBN_OK) or a negative rbn_status.  Nothing calls exit().
+ *    The message for the last failure on a context is rbn_last_error(ctx); failures that
+ *    happen before a context exists are reported by rbn_last_error(NULL).
+ *  - Host arrays are caller-owned and only read/written during the call.
+ *  - Field arrays on the host use the reference's halo-free dump layout [nx][ny]
+ *    (y fastest, element (i,j) at i*ny + j for interior node (i+1, j+1)), as written by
+ *    dumpf/snapshottemp (Trajectories/code/temppart_dump.c:52-141, 1183-1262).
+ *    Population arrays are [nx][ny][9] in the reference's D2Q9 order (dellar,
+ *    Trajectories/code/temppart_dump.c:1363-1401).
+ *  - A context owns one GPU stream and all device memory of one lattice.  Calls on one
+ *    context must be serialised by the caller; distinct contexts are independent.
+ *  - Device state is FP64 or FP32 (shifted populations) per rbn_params.precision.  Host
+ *    outputs are always double.
+ */
+#ifndef RBN_H
+#define RBN_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RBN_API_VERSION 1
+
+typedef struct rbn_ctx rbn_ctx;
+
+typedef enum rbn_status {
+  RBN_OK = 0,
+  RBN_EINVAL = -1,    /* bad argument or parameter combination               */
+  RBN_EIO = -2,       /* file could not be opened/read/written                */
+  RBN_EPARSE = -3,    /* parameter file could not be parsed                   */
+  RBN_EDEVICE = -4,   /* no usable GPU, or a HIP call failed                  */
+  RBN_ESTATE = -5,    /* call out of order (e.g. step before init)            */
+  RBN_ERANGE = -6,    /* size beyond a documented limit, particle out of grid */
+  RBN_ENOMEM = -7,    /* host or device allocation failed                     */
+  RBN_ENUMERIC = -8,  /* non-finite value detected                            */
+  RBN_ENOTFOUND = -9, /* observation id missing from a record                 */
+  RBN_ECALLBACK = -10 /* a user callback returned non-zero                    */
+} rbn_status;
+
+/* Which of the reference's two programs the step restates. */
+enum {
+  RBN_VARIANT_TRAJECTORIES = 0, /* truth run: Trajectories/code/temppart_dump.c:1674-1759   */
+  RBN_VARIANT_NUDGING = 1       /* nudged run: Nudging/code/tempvel_nudge.c:2346-2411         */
+};
+enum { RBN_FP64 = 0, RBN_FP32 = 1 };
+enum { RBN_DEPOSIT_BOX7 = 0, RBN_DEPOSIT_BILINEAR = 1 };
+
+/*
+ * Parameters.  The first group carries the reference's #define names
+ * (Trajectories/code/temppart_dump.c:15-31, Trajectories/code/mympi2.h:14-21,
+ *  Nudging/code/tempvel_nudge.c:13-31) with the same meaning; rbn_params_parse reads them
+ * from "#define NAME value" lines.  The second group holds options the reference fixes
+ * in code; each defaults to the reference's behaviour.
+ */
+typedef struct rbn_params {
+  /* --- reference #define names --- */
+  int nx, ny;              /* global lattice (interior nodes)                          */
+  int scratch;             /* 1: initialise from scratch                               */
+  int contin;              /* nudged run: continue from dumps (only 0 is supported)    */
+  int nstart, nsteps;      /* first step is nstart+1                                   */
+  int nout;                /* diagnostic cadence (evolution.dat)                       */
+  int noutconfig;          /* trajectory-record cadence                                */
+  int ntempread;           /* nudged run: observation-record spacing                   */
+  double rhom1;            /* initial density                                          */
+  double relax, relaxg;    /* BGK relaxation times of f and g                          */
+  double Tup, Tdown;       /* top / bottom plate temperatures                          */
+  double gravity;          /* buoyancy: F_y = -gravity * rho * T                       */
+  double alpha;            /* nudging strength per step                                */
+  int particle;            /* 1: advance particles                                     */
+  int particle_scratch;    /* 1: particles initialised from scratch                    */
+  int nprtcl;              /* number of particles / observers                          */
+  int tracer, heavyprtcl;  /* default particle kind when no per-particle tau is given  */
+  double taulag, beta;     /* Maxey–Riley relaxation time and added-mass coefficient   */
+  int prtcl_writefull;     /* 1: records also carry dtu..dyv, fu, fv at (xo,yo)        */
+  /* --- options (reference behaviour by default) --- */
+  int variant;             /* RBN_VARIANT_*                                            */
+  int precision;           /* RBN_FP64 / RBN_FP32                                      */
+  int deposit;             /* RBN_DEPOSIT_BOX7 (reference) or _BILINEAR                */
+  int fix_q3;              /* 0: keep the reference's `xi > nx-2` wrap test (misses
+                              xi = nx-2, Nudging/code/tempvel_nudge.c:1601); 1: fixed  */
+  int ndiag;               /* energy cadence; 0 = reference (nout / 5000)              */
+  double T0_shift;         /* FP32: g stored as g_k - w_k*T0_shift                     */
+  double rho0_shift;       /* FP32: f stored as f_k - w_k*rho0_shift                   */
+  double heavy_init_factor;/* inertial initial velocity = factor * fluid (0.98 in ref)  */
+} rbn_params;
+
+/* Particle state in id order (the reference's prtcl_type, Trajectories/code/mympi2.h:2-7). */
+typedef struct rbn_particles {
+  int n;
+  int *id;
+  double *x, *y, *u, *v, *xo, *yo, *uo, *vo;
+} rbn_particles;
+
+/*
+ * One trajectory record (prtcl_write, Trajectories/code/temppart_dump.c:1467-1571, and the
+ * Writefull block, Trajectories/code/mympi2.h:441-556): positions after the step's update
+ * and before the periodic/reflecting wrap, in ascending id order.  The Writefull arrays
+ * are NULL unless prtcl_writefull is set.  Pointers are valid during the callback only.
+ */
+typedef struct rbn_record {
+  int step, n;
+  const int *id;
+  const double *x, *y, *u, *v, *xo, *yo, *uo, *vo, *temp;
+  const double *dtu, *dtv, *dxu, *dxv, *dyu, *dyv, *fu, *fv;
+} rbn_record;
+
+typedef int (*rbn_record_fn)(void *user, const rbn_record *rec);
+/* Fill id/x/y/T (length n, record storage order) with trajectory record `record_step`. */
+typedef int (*rbn_obs_fn)(void *user, int record_step, int n, int *id, double *x, double *y,
+                          double *T);
+/* Called every diagnostic step with the evolution.dat quantities. */
+typedef int (*rbn_diag_fn)(void *user, int step, double mean_energy, double mean_T);
+
+/* ---------------------------------------------------------------- library */
+const char *rbn_version(void);
+int rbn_api_version(void);
+/* Message of the last failure on ctx (or of the calling thread when ctx is NULL). */
+const char *rbn_last_error(const rbn_ctx *ctx);
+/* Number of visible GPUs (0 on a host without one). */
+int rbn_device_count(int *n);
+
+/* ---------------------------------------------------------------- parameters */
+/* Reference defaults: the #define blocks of Trajectories/code/temppart_dump.c:15-31 and
+ * Trajectories/code/mympi2.h:14-21. */
+void rbn_params_default(rbn_params *p);
+/* Read "#define NAME value" lines (C expression values) from a file, updating the fields
+ * whose names appear; may be called on several files in turn.  A file that defines
+ * `ntempread` or `alpha` selects the nudged variant unless rbn_variant is also given. */
+int rbn_params_parse(const char *path, rbn_params *p);
+/* Same as rbn_params_parse on an in-memory text. */
+int rbn_params_parse_text(const char *text, rbn_params *p);
+/* Set one field by its #define (or option) name. */
+int rbn_params_set(rbn_params *p, const char *name, double value);
+/* Rayleigh number of a parameter set (media(), Nudging/code/tempvel_nudge.c:1262-1283). */
+double rbn_params_rayleigh(const rbn_params *p);
+/* Inverse of media(): choose relax, relaxg and gravity for (Ra, Pr) with free-fall
+ * velocity u_ff = sqrt(|gravity| dT L), L = ny-1, keeping nx, ny, Tup, Tdown, rhom1. */
+int rbn_params_from_ra(rbn_params *p, double Ra, double Pr, double u_ff);
+/* Validate a parameter set (sizes, cadences, variant constraints); 0 when usable. */
+int rbn_params_check(const rbn_params *p);
+
+/* ---------------------------------------------------------------- lifetime */
+/* Allocate the lattice on GPU `device`; replaces the malloc block of main()
+ * (Trajectories/code/temppart_dump.c:1598-1640, Nudging/code/tempvel_nudge.c:2271-2306)
+ * and dellar() (:1363-1401). */
+int rbn_create(const rbn_params *p, int device, rbn_ctx **out);
+void rbn_destroy(rbn_ctx *ctx);
+int rbn_get_params(const rbn_ctx *ctx, rbn_params *out);
+
+/* ---------------------------------------------------------------- initial state */
+/* Scratch initialisation: inithydro + equilif/initpop + equilig/initpop
+ * (Trajectories/code/temppart_dump.c:680-742, 1643-1648).  NULL rho -> rhom1, NULL u/v ->
+ * 0, NULL T -> the reference's tanh front with srand(0)/rand() jitter (rank-0 stream). */
+int rbn_init_fields(rbn_ctx *ctx, const double *rho, const double *T, const double *u,
+                    const double *v);
+/* Linear conduction profile Tdown..Tup (the reference's commented-out alternative,
+ * Trajectories/code/temppart_dump.c:697) plus uniform noise in [-amp, amp] from a
+ * splitmix64 stream seeded by `seed`; u = v = 0, rho = rhom1. */
+int rbn_init_conduction(rbn_ctx *ctx, double noise_amp, unsigned long long seed);
+/* Restart: post-collision populations plus the previous step's moments (restoref/g and
+ * inithydro's restart branch, Trajectories/code/temppart_dump.c:348-668, 701-727).
+ * rho_prev/T_prev feed the lagged force, u_prev/v_prev the inertial d/dt. */
+int rbn_set_populations(rbn_ctx *ctx, const double *f, const double *g,
+                        const double *rho_prev, const double *T_prev,
+                        const double *u_prev, const double *v_prev);
+/* Post-collision populations (dumpf/dumpg layout [nx][ny][9]). */
+int rbn_get_populations(rbn_ctx *ctx, double *f, double *g);
+/* Moments of the latest hydrovar (rhopre, temppre, upre, vpre); any pointer may be NULL. */
+int rbn_get_fields(rbn_ctx *ctx, double *rho, double *T, double *u, double *v);
+/* evolution.dat quantities <rho|u|^2/2>, <T> of the latest hydrovar
+ * (Trajectories/code/temppart_dump.c:1739-1757). */
+int rbn_energy(rbn_ctx *ctx, double *mean_energy, double *mean_T);
+
+/* ---------------------------------------------------------------- particles (truth run) */
+/* init_prtcl (Trajectories/code/mympi2.h:116-290) with caller-chosen positions.
+ * tau_p[i] > 0 makes particle i inertial (Maxey–Riley, :373-433) with 1/tau_p[i] in place
+ * of onebytaulag; tau_p[i] <= 0 makes it a tracer (AB2, :358-372).  tau_p NULL -> the
+ * params' tracer/heavyprtcl/taulag; beta_p NULL -> params.beta.  up/vp NULL -> initial
+ * velocity from the fluid (tracer: I[u]; inertial: heavy_init_factor*I[u], :196-207). */
+int rbn_init_particles(rbn_ctx *ctx, int n, const int *id, const double *x, const double *y,
+                       const double *tau_p, const double *beta_p, const double *up,
+                       const double *vp);
+/* Current particle state in id order; caller provides arrays of length n. */
+int rbn_get_particles(rbn_ctx *ctx, rbn_particles *out);
+int rbn_num_particles(const rbn_ctx *ctx, int *n);
+/* Called on every record step (istep % noutconfig == 0) — prtcl_write's replacement. */
+int rbn_set_record_callback(rbn_ctx *ctx, rbn_record_fn fn, void *user);
+
+/* ---------------------------------------------------------------- nudging (nudged run) */
+/* set_nudge_vol's particle part (Nudging/code/tempvel_nudge.c:2083-2122): the observers'
+ * ids and their state at nstart (the truth run's init_prtcl{x,y,T,_id}0.in). */
+int rbn_init_observers(rbn_ctx *ctx, int n, const int *id, const double *x, const double *y,
+                       const double *T);
+/* Source of trajectory records for read_prtcl (Nudging/code/tempvel_nudge.c:1380-1476):
+ * asked for record istep+ntempread-1 on steps with istep % ntempread == 1. */
+int rbn_set_obs_provider(rbn_ctx *ctx, rbn_obs_fn fn, void *user);
+/* Nudging state of the latest step: nudge_vol [nx][ny] counts, tempreadint [nx][ny]
+ * running means, nud_term [nx][ny]; any pointer may be NULL. */
+int rbn_get_nudging(rbn_ctx *ctx, int *nudge_vol, double *tempreadint, double *nud_term);
+
+/* ---------------------------------------------------------------- stepping */
+/* Diagnostics callback (evolution.dat line), every ndiag steps. */
+int rbn_set_diag_callback(rbn_ctx *ctx, rbn_diag_fn fn, void *user);
+/* Advance nsteps time steps in the reference's order (one iteration of the main loops,
+ * Trajectories/code/temppart_dump.c:1674-1759 or Nudging/code/tempvel_nudge.c:2346-2411). */
+int rbn_step(rbn_ctx *ctx, int nsteps);
+/* Index of the last completed step (istep). */
+int rbn_get_step(const rbn_ctx *ctx, int *istep);
+/* Wait for all queued GPU work of ctx. */
+int rbn_synchronize(rbn_ctx *ctx);
+
+/* ---------------------------------------------------------------- measurement */
+/* Enable per-launch HIP-event timing of each kernel on ctx's stream (0/1). */
+int rbn_set_kernel_timing(rbn_ctx *ctx, int enable);
+/* Launch count and summed duration (ms) of the timed launches of kernel `name`
+ * ("lbm", "particles", "obs", "deposit"); resets the counters when reset != 0. */
+int rbn_kernel_stats(rbn_ctx *ctx, const char *name, int reset, long long *launches,
+                     double *total_ms);
+/* Time `nsteps` steps with HIP events on ctx's stream (inputs resident on the GPU). */
+int rbn_time_steps(rbn_ctx *ctx, int nsteps, double *elapsed_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RBN_H */

@@ -1,0 +1,911 @@
+// rbn_api.cpp — the C-ABI of librbn (declared in include/rbn.h): context lifetime, initial
+// state, the per-step launch sequence, records, observers and measurement.
+//
+// The launch sequence of rbn_step restates one iteration of the reference main loops:
+//   truth run (Trajectories/code/temppart_dump.c:1674-1759):
+//     K1 fused step  ->  K2 particles (+ record on istep % noutconfig == 0)  ->  energy
+//   nudged run (Nudging/code/tempvel_nudge.c:2346-2411):
+//     observer update + ordered deposit (read_prtcl)  ->  K1 fused step (nudged)  ->  energy
+// Particles only read the step's velocity, so running K2 after the collision (the
+// reference runs it between hydrovar and the collision) yields identical results.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <numeric>
+#include <unordered_map>
+#include <vector>
+
+#include "rbn_internal.h"
+
+using rbn::Geom;
+
+namespace {
+
+int fail(rbn_ctx *c, int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  rbn::thread_fail(code, buf);
+  return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                   \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      return fail((ctx), RBN_EDEVICE, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                  __FILE__, __LINE__);                                                      \
+  } while (0)
+
+int dalloc(rbn_ctx *c, void **p, size_t bytes) {
+  *p = nullptr;
+  if (bytes == 0) bytes = 256;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(c, RBN_ENOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  }
+  return RBN_OK;
+}
+
+void dfree(void *&p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+template <typename T>
+void dfree_t(T *&p) {
+  void *q = p;
+  dfree(q);
+  p = nullptr;
+}
+
+template <typename R>
+void fill_consts(const rbn_params &P, rbn::StepConst<R> &c) {
+  // The FP64 values are computed with the reference's expressions (T:1318-1320, 752,
+  // 896-921; N:1929-1931); the FP32 ones are those values rounded, with the plate rule
+  // expressed for the shifted g.
+  const double om = 1. / P.relax, omg = 1. / P.relaxg;
+  c.om = (R)om;
+  c.one_m_om = (R)(1. - om);
+  c.omg = (R)omg;
+  c.one_m_omg = (R)(1. - omg);
+  const double B = 1.0 - 0.5 * omg;
+  for (int k = 0; k < 9; k++) {
+    c.src[k] = (R)((1.0 - 0.5 * om) * rbn::kW[k]);
+    c.bw[k] = (R)(B * rbn::kW[k]);
+  }
+  c.g0 = (R)(-P.gravity);
+  c.alpha = (R)P.alpha;
+  const bool shifted = sizeof(R) == 4;
+  const double T0 = shifted ? P.T0_shift : 0.0;
+  c.rho0 = (R)(shifted ? P.rho0_shift : 0.0);
+  c.T0 = (R)T0;
+  const int top_k[3] = {2, 5, 6}, bot_k[3] = {4, 7, 8};
+  for (int q = 0; q < 3; q++) {
+    c.wtop[q] = (R)(2.0 * rbn::kW[top_k[q]] * (shifted ? (P.Tup - T0) : P.Tup));
+    c.wbot[q] = (R)(2.0 * rbn::kW[bot_k[q]] * (shifted ? (P.Tdown - T0) : P.Tdown));
+  }
+}
+
+const char *kKernelNames[4] = {"lbm", "particles", "obs", "deposit"};
+
+int kernel_index(const char *name) {
+  for (int k = 0; k < 4; k++)
+    if (name && strcmp(name, kKernelNames[k]) == 0) return k;
+  return -1;
+}
+
+hipEvent_t take_event(rbn_ctx *c) {
+  if (!c->event_pool.empty()) {
+    hipEvent_t e = c->event_pool.back();
+    c->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+int drain_timing(rbn_ctx *c) {
+  for (auto &p : c->pending) {
+    HIPCHK(c, hipEventSynchronize(p.b));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, p.a, p.b));
+    c->k_launches[p.kernel]++;
+    c->k_ms[p.kernel] += ms;
+    c->event_pool.push_back(p.a);
+    c->event_pool.push_back(p.b);
+  }
+  c->pending.clear();
+  return RBN_OK;
+}
+
+// Launch through fn(), bracketed with events when kernel timing is on.
+template <typename Fn>
+int timed(rbn_ctx *c, int kernel, Fn &&fn) {
+  hipEvent_t a = nullptr, b = nullptr;
+  if (c->timing) {
+    a = take_event(c);
+    b = take_event(c);
+    if (!a || !b) return fail(c, RBN_EDEVICE, "hipEventCreate failed");
+    HIPCHK(c, hipEventRecord(a, c->stream));
+  }
+  hipError_t e = fn();
+  if (e != hipSuccess) return fail(c, RBN_EDEVICE, "%s launch failed: %s", kKernelNames[kernel], hipGetErrorString(e));
+  if (c->timing) {
+    HIPCHK(c, hipEventRecord(b, c->stream));
+    c->pending.push_back({kernel, a, b});
+    if (c->pending.size() > 8192) return drain_timing(c);
+  }
+  return RBN_OK;
+}
+
+// Host [nx][ny] (y fastest) <-> device [j][i] with pitch, converting to the real type.
+template <typename R>
+void to_device_layout(const rbn_ctx *c, const double *h, std::vector<unsigned char> &buf) {
+  const Geom &g = c->geom;
+  buf.assign(g.plane * sizeof(R), 0);
+  R *d = reinterpret_cast<R *>(buf.data());
+  for (int i = 0; i < g.nx; i++)
+    for (int j = 0; j < g.ny; j++) d[(size_t)j * g.pitch + i] = (R)h[(size_t)i * g.ny + j];
+}
+
+int upload_field(rbn_ctx *c, void *dst, const double *h) {
+  std::vector<unsigned char> buf;
+  if (c->prec == RBN_FP64)
+    to_device_layout<double>(c, h, buf);
+  else
+    to_device_layout<float>(c, h, buf);
+  HIPCHK(c, hipMemcpyAsync(dst, buf.data(), buf.size(), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return RBN_OK;
+}
+
+int download_field(rbn_ctx *c, const void *src, double *h, double add = 0.0) {
+  const Geom &g = c->geom;
+  std::vector<unsigned char> buf(g.plane * c->rs);
+  HIPCHK(c, hipMemcpyAsync(buf.data(), src, buf.size(), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int i = 0; i < g.nx; i++)
+    for (int j = 0; j < g.ny; j++) {
+      size_t o = (size_t)j * g.pitch + i;
+      double v = c->prec == RBN_FP64 ? reinterpret_cast<const double *>(buf.data())[o]
+                                     : (double)reinterpret_cast<const float *>(buf.data())[o];
+      h[(size_t)i * g.ny + j] = v + add;
+    }
+  return RBN_OK;
+}
+
+int check_ctx(rbn_ctx *c) {
+  if (!c) return rbn::thread_fail(RBN_EINVAL, "null context");
+  if (hipSetDevice(c->dev) != hipSuccess) return fail(c, RBN_EDEVICE, "hipSetDevice(%d) failed", c->dev);
+  return RBN_OK;
+}
+
+uint64_t splitmix64(uint64_t &s) {
+  uint64_t z = (s += 0x9E3779B97F4A7C15ULL);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+// ---------------------------------------------------------------- particle storage
+
+enum { PX, PY, PU, PV, PXO, PYO, PUO, PVO, PINVTAU, PBETA, PNARR };
+
+rbn::PartPtrs part_ptrs(rbn_ctx *c, bool record) {
+  rbn::PartPtrs pp{};
+  const size_t n = (size_t)c->np;
+  double *b = c->d_part;
+  pp.n = c->np;
+  pp.x = b + PX * n;
+  pp.y = b + PY * n;
+  pp.u = b + PU * n;
+  pp.v = b + PV * n;
+  pp.xo = b + PXO * n;
+  pp.yo = b + PYO * n;
+  pp.uo = b + PUO * n;
+  pp.vo = b + PVO * n;
+  pp.inv_tau = b + PINVTAU * n;
+  pp.beta = b + PBETA * n;
+  pp.rec = record ? c->d_rec : nullptr;
+  pp.err = c->d_err;
+  return pp;
+}
+
+enum { OCHK, OBCY, OXIDX, OYIDX, ONI };
+enum { ODX, ODY, ODT, OXINT, OYINT, OTINT, ORX, ORY, ORT, OND };
+
+rbn::ObsPtrs obs_ptrs(rbn_ctx *c) {
+  rbn::ObsPtrs o{};
+  const size_t n = (size_t)c->nobs;
+  o.n = c->nobs;
+  o.chk = c->d_obs_i + OCHK * n;
+  o.bcy = c->d_obs_i + OBCY * n;
+  o.xidx = c->d_obs_i + OXIDX * n;
+  o.yidx = c->d_obs_i + OYIDX * n;
+  double *d = c->d_obs_d;
+  o.dx = d + ODX * n;
+  o.dy = d + ODY * n;
+  o.dT = d + ODT * n;
+  o.xint = d + OXINT * n;
+  o.yint = d + OYINT * n;
+  o.Tint = d + OTINT * n;
+  o.rx = d + ORX * n;
+  o.ry = d + ORY * n;
+  o.rT = d + ORT * n;
+  return o;
+}
+
+int check_err_flag(rbn_ctx *c) {
+  int h = 0;
+  HIPCHK(c, hipMemcpyAsync(&h, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (h) {
+    HIPCHK(c, hipMemsetAsync(c->d_err, 0, sizeof(int), c->stream));
+    return fail(c, RBN_ERANGE, "a particle left the lattice (interpolation corner out of range) by step %d", c->istep);
+  }
+  if (c->dep.overflow) {
+    HIPCHK(c, hipMemcpyAsync(&h, c->dep.overflow, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (h) return fail(c, RBN_ERANGE, "deposit list overflow at step %d", c->istep);
+  }
+  return RBN_OK;
+}
+
+int deliver_record(rbn_ctx *c, int step) {
+  const size_t n = (size_t)c->np;
+  const size_t narr = c->P.prtcl_writefull ? 17 : 9;
+  HIPCHK(c, hipMemcpyAsync(c->h_rec, c->d_rec, narr * n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  int rc = check_err_flag(c);
+  if (rc) return rc;
+  const double *r = c->h_rec;
+  rbn_record rec{};
+  rec.step = step;
+  rec.n = c->np;
+  rec.id = c->pid.data();
+  const double **dst[9] = {&rec.x, &rec.y, &rec.u, &rec.v, &rec.xo, &rec.yo, &rec.uo, &rec.vo, &rec.temp};
+  for (int q = 0; q < 9; q++) *dst[q] = r + q * n;
+  if (c->P.prtcl_writefull) {
+    const double **full[8] = {&rec.dtu, &rec.dtv, &rec.dxu, &rec.dxv, &rec.dyu, &rec.dyv, &rec.fu, &rec.fv};
+    for (int q = 0; q < 8; q++) *full[q] = r + (9 + q) * n;
+  }
+  if (c->rec_fn && c->rec_fn(c->rec_user, &rec) != 0)
+    return fail(c, RBN_ECALLBACK, "record callback failed at step %d", step);
+  return RBN_OK;
+}
+
+int run_energy(rbn_ctx *c, double out[2]) {
+  const int nb = rbn::energy_blocks(c->geom);
+  double *partial = c->d_energy, *res = c->d_energy + 2 * 1024;
+  hipError_t e = rbn::launch_energy(c->stream, c->prec, c->geom, c->rho, c->u[c->cur], c->v[c->cur],
+                                    c->T, partial, nb, res);
+  if (e != hipSuccess) return fail(c, RBN_EDEVICE, "energy launch failed: %s", hipGetErrorString(e));
+  HIPCHK(c, hipMemcpyAsync(out, res, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return RBN_OK;
+}
+
+// Fetch record `rstep` through the provider, map observer ids to record slots (the first
+// match, as the reference's search N:1480-1487) and upload it.
+int load_observation(rbn_ctx *c, int rstep) {
+  const int n = c->nobs;
+  if (!c->obs_fn) return fail(c, RBN_ESTATE, "no observation provider set (record %d needed)", rstep);
+  c->h_rid.assign((size_t)n, 0);
+  c->h_rx.assign((size_t)n, 0.0);
+  c->h_ry.assign((size_t)n, 0.0);
+  c->h_rT.assign((size_t)n, 0.0);
+  if (c->obs_fn(c->obs_user, rstep, n, c->h_rid.data(), c->h_rx.data(), c->h_ry.data(), c->h_rT.data()) != 0)
+    return fail(c, RBN_ECALLBACK, "observation provider failed for record %d", rstep);
+  std::unordered_map<int, int> first;
+  first.reserve((size_t)n * 2);
+  for (int j = 0; j < n; j++) first.emplace(c->h_rid[(size_t)j], j);
+  c->h_chk.assign((size_t)n, 0);
+  for (int i = 0; i < n; i++) {
+    auto it = first.find(c->oid[(size_t)i]);
+    if (it == first.end()) return fail(c, RBN_ENOTFOUND, "observer id %d missing from record %d", c->oid[(size_t)i], rstep);
+    c->h_chk[(size_t)i] = it->second;
+  }
+  rbn::ObsPtrs o = obs_ptrs(c);
+  const size_t b = (size_t)n * sizeof(double);
+  HIPCHK(c, hipMemcpyAsync((void *)o.rx, c->h_rx.data(), b, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync((void *)o.ry, c->h_ry.data(), b, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync((void *)o.rT, c->h_rT.data(), b, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync((void *)o.chk, c->h_chk.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  // The host vectors are reused on the next interval: wait for the copies.
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return RBN_OK;
+}
+
+int init_common_after_fields(rbn_ctx *c) {
+  c->istep = c->P.nstart;
+  c->initialised = true;
+  c->moments_valid = true;
+  if (c->nud) HIPCHK(c, hipMemsetAsync(c->nud, 0, c->geom.plane * c->rs, c->stream));
+  if (c->dep.cnt) HIPCHK(c, hipMemsetAsync(c->dep.cnt, 0, c->geom.plane * sizeof(int), c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return RBN_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *rbn_version(void) { return "librbn 0.1 (gfx950)"; }
+int rbn_api_version(void) { return RBN_API_VERSION; }
+
+const char *rbn_last_error(const rbn_ctx *ctx) {
+  if (ctx) return ctx->err.c_str();
+  return rbn::thread_error();
+}
+
+int rbn_device_count(int *n) {
+  if (!n) return rbn::thread_fail(RBN_EINVAL, "null pointer");
+  int k = 0;
+  if (hipGetDeviceCount(&k) != hipSuccess) k = 0;
+  *n = k;
+  return RBN_OK;
+}
+
+void rbn_destroy(rbn_ctx *c) {
+  if (!c) return;
+  (void)hipSetDevice(c->dev);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  void *ptrs[] = {c->fA, c->fB, c->gA, c->gB, c->F, c->u[0], c->u[1], c->v[0], c->v[1], c->rho, c->T, c->nud};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  dfree_t(c->d_part);
+  dfree_t(c->d_rec);
+  dfree_t(c->d_err);
+  dfree_t(c->d_obs_i);
+  dfree_t(c->d_obs_d);
+  dfree_t(c->d_energy);
+  dfree_t(c->dep.cnt);
+  dfree_t(c->dep.fill);
+  dfree_t(c->dep.off);
+  dfree_t(c->dep.list);
+  dfree_t(c->dep.blocksum);
+  dfree_t(c->dep.Tbar);
+  dfree_t(c->dep.overflow);
+  if (c->h_rec) (void)hipHostFree(c->h_rec);
+  for (auto &p : c->pending) {
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int rbn_create(const rbn_params *p, int device, rbn_ctx **out) {
+  if (!p || !out) return rbn::thread_fail(RBN_EINVAL, "rbn_create: null argument");
+  *out = nullptr;
+  int rc = rbn_params_check(p);
+  if (rc) return rc;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+    return rbn::thread_fail(RBN_EDEVICE, "rbn_create: no GPU visible (librbn has no CPU fallback)");
+  if (device < 0 || device >= ndev) return rbn::thread_fail(RBN_EINVAL, "rbn_create: device index out of range");
+  rbn_ctx *c = new rbn_ctx();
+  c->P = *p;
+  c->dev = device;
+  c->prec = p->precision;
+  c->rs = c->prec == RBN_FP64 ? 8 : 4;
+  if (hipSetDevice(device) != hipSuccess) {
+    rbn_destroy(c);
+    return rbn::thread_fail(RBN_EDEVICE, "hipSetDevice failed");
+  }
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    rbn_destroy(c);
+    return rbn::thread_fail(RBN_EDEVICE, "hipStreamCreate failed");
+  }
+  Geom &g = c->geom;
+  g.nx = p->nx;
+  g.ny = p->ny;
+  g.pitch = (p->nx + 63) / 64 * 64;
+  g.plane = (size_t)g.pitch * (size_t)g.ny;
+  const size_t fb = g.plane * c->rs;
+  struct A {
+    void **p;
+    size_t b;
+  } allocs[] = {{&c->fA, 9 * fb}, {&c->fB, 9 * fb}, {&c->gA, 9 * fb}, {&c->gB, 9 * fb},
+                {&c->F, fb},      {&c->u[0], fb},  {&c->u[1], fb},  {&c->v[0], fb},
+                {&c->v[1], fb},   {&c->rho, fb},   {&c->T, fb}};
+  for (auto &a : allocs) {
+    if ((rc = dalloc(c, a.p, a.b))) {
+      rbn::thread_fail(rc, c->err.c_str());
+      rbn_destroy(c);
+      return rc;
+    }
+    (void)hipMemsetAsync(*a.p, 0, a.b, c->stream);
+  }
+  void *tmp = nullptr;
+  if ((rc = dalloc(c, &tmp, sizeof(int)))) goto fail_out;
+  c->d_err = (int *)tmp;
+  (void)hipMemsetAsync(c->d_err, 0, sizeof(int), c->stream);
+  if ((rc = dalloc(c, &tmp, (2 * 1024 + 2) * sizeof(double)))) goto fail_out;
+  c->d_energy = (double *)tmp;
+  if (p->variant == RBN_VARIANT_NUDGING) {
+    if ((rc = dalloc(c, &c->nud, fb))) goto fail_out;
+    (void)hipMemsetAsync(c->nud, 0, fb, c->stream);
+    const size_t ntiles = (g.plane + 1023) / 1024;
+    struct B {
+      void **p;
+      size_t b;
+    } dallocs[] = {{(void **)&c->dep.cnt, g.plane * sizeof(int)},
+                   {(void **)&c->dep.fill, g.plane * sizeof(int)},
+                   {(void **)&c->dep.off, (g.plane + 1) * sizeof(int)},
+                   {(void **)&c->dep.blocksum, ntiles * sizeof(int)},
+                   {(void **)&c->dep.Tbar, g.plane * sizeof(double)},
+                   {(void **)&c->dep.overflow, sizeof(int)}};
+    for (auto &a : dallocs) {
+      if ((rc = dalloc(c, a.p, a.b))) goto fail_out;
+      (void)hipMemsetAsync(*a.p, 0, a.b, c->stream);
+    }
+  }
+  c->consts.resize(std::max(sizeof(rbn::StepConst<double>), sizeof(rbn::StepConst<float>)));
+  if (c->prec == RBN_FP64)
+    fill_consts<double>(*p, *reinterpret_cast<rbn::StepConst<double> *>(c->consts.data()));
+  else
+    fill_consts<float>(*p, *reinterpret_cast<rbn::StepConst<float> *>(c->consts.data()));
+  if (hipStreamSynchronize(c->stream) != hipSuccess) {
+    rc = fail(c, RBN_EDEVICE, "device initialisation failed");
+    goto fail_out;
+  }
+  c->istep = p->nstart;
+  *out = c;
+  return RBN_OK;
+fail_out:
+  rbn::thread_fail(rc, c->err.c_str());
+  rbn_destroy(c);
+  return rc;
+}
+
+int rbn_get_params(const rbn_ctx *ctx, rbn_params *out) {
+  if (!ctx || !out) return rbn::thread_fail(RBN_EINVAL, "null argument");
+  *out = ctx->P;
+  return RBN_OK;
+}
+
+int rbn_init_fields(rbn_ctx *c, const double *rho, const double *T, const double *u, const double *v) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  const Geom &g = c->geom;
+  const size_t nn = (size_t)g.nx * g.ny;
+  std::vector<double> hr(nn), hT(nn), hu(nn, 0.0), hv(nn, 0.0);
+  for (size_t k = 0; k < nn; k++) hr[k] = rho ? rho[k] : c->P.rhom1;
+  if (u) std::copy(u, u + nn, hu.begin());
+  if (v) std::copy(v, v + nn, hv.begin());
+  if (T) {
+    std::copy(T, T + nn, hT.begin());
+  } else {
+    // inithydro scratch branch (Trajectories/code/temppart_dump.c:686-700) on rank 0: the
+    // per-node rand() stream visits halo nodes too, in (i, j) order over (nx+2)(ny+2).
+    const double randnorm = 1.0 / RAND_MAX, epstan = 10.;
+    srand(0);
+    for (int i = 0; i < g.nx + 2; i++)
+      for (int j = 0; j < g.ny + 2; j++) {
+        double yyytemp = ((double)(j - 1 + (g.ny) * 0));
+        double y_c = (double)c->P.ny * (1) / 2. + (randnorm * rand() - 0.5) * 2. * epstan;
+        double t = c->P.Tdown + (c->P.Tup - c->P.Tdown) * (tanh((yyytemp - y_c) / epstan) + 1.0) / 2.0;
+        if (i >= 1 && i <= g.nx && j >= 1 && j <= g.ny) hT[(size_t)(i - 1) * g.ny + (j - 1)] = t;
+      }
+  }
+  c->cur = 0;
+  if ((rc = upload_field(c, c->rho, hr.data()))) return rc;
+  if ((rc = upload_field(c, c->T, hT.data()))) return rc;
+  if ((rc = upload_field(c, c->u[0], hu.data()))) return rc;
+  if ((rc = upload_field(c, c->v[0], hv.data()))) return rc;
+  if ((rc = upload_field(c, c->u[1], hu.data()))) return rc;
+  if ((rc = upload_field(c, c->v[1], hv.data()))) return rc;
+  hipError_t e = rbn::launch_init_eq(c->stream, c->prec, g, c->rho, c->T, c->u[0], c->v[0], c->fA,
+                                     c->gA, c->F, c->consts.data());
+  if (e != hipSuccess) return fail(c, RBN_EDEVICE, "init launch failed: %s", hipGetErrorString(e));
+  return init_common_after_fields(c);
+}
+
+int rbn_init_conduction(rbn_ctx *c, double noise_amp, unsigned long long seed) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  const Geom &g = c->geom;
+  std::vector<double> T((size_t)g.nx * g.ny);
+  uint64_t s = seed;
+  for (int i = 0; i < g.nx; i++)
+    for (int j = 0; j < g.ny; j++) {
+      // Tdown + (Tup-Tdown)*(j-1)/(ny-1), the commented-out profile of T:697.
+      double lin = c->P.Tdown + (c->P.Tup - c->P.Tdown) * ((double)j) / ((double)(c->P.ny - 1));
+      double r = (double)(splitmix64(s) >> 11) * (1.0 / 9007199254740992.0);  // [0,1)
+      T[(size_t)i * g.ny + j] = lin + noise_amp * (2.0 * r - 1.0);
+    }
+  return rbn_init_fields(c, nullptr, T.data(), nullptr, nullptr);
+}
+
+int rbn_set_populations(rbn_ctx *c, const double *f, const double *gpop, const double *rho_prev,
+                        const double *T_prev, const double *u_prev, const double *v_prev) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!f || !gpop || !rho_prev || !T_prev) return fail(c, RBN_EINVAL, "rbn_set_populations: null array");
+  const Geom &g = c->geom;
+  const size_t nn = (size_t)g.nx * g.ny;
+  const bool shifted = c->prec == RBN_FP32;
+  for (int which = 0; which < 2; which++) {
+    const double *src = which ? gpop : f;
+    void *dst = which ? c->gA : c->fA;
+    const double lvl = shifted ? (which ? c->P.T0_shift : c->P.rho0_shift) : 0.0;
+    std::vector<unsigned char> buf(9 * g.plane * c->rs, 0);
+    for (int k = 0; k < 9; k++)
+      for (int i = 0; i < g.nx; i++)
+        for (int j = 0; j < g.ny; j++) {
+          size_t o = (size_t)k * g.plane + (size_t)j * g.pitch + i;
+          double val = src[((size_t)i * g.ny + j) * 9 + k];
+          if (shifted)
+            reinterpret_cast<float *>(buf.data())[o] = (float)(val - rbn::kW[k] * lvl);
+          else
+            reinterpret_cast<double *>(buf.data())[o] = val;
+        }
+    HIPCHK(c, hipMemcpyAsync(dst, buf.data(), buf.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  // Force of the first step from the given moments (forcingconstructWW, T:752).
+  std::vector<double> F(nn), zero(nn, 0.0);
+  for (size_t k = 0; k < nn; k++) F[k] = -c->P.gravity * rho_prev[k] * (T_prev[k] - 0.0);
+  c->cur = 0;
+  if ((rc = upload_field(c, c->F, F.data()))) return rc;
+  if ((rc = upload_field(c, c->rho, rho_prev))) return rc;
+  if ((rc = upload_field(c, c->T, T_prev))) return rc;
+  if ((rc = upload_field(c, c->u[0], u_prev ? u_prev : zero.data()))) return rc;
+  if ((rc = upload_field(c, c->v[0], v_prev ? v_prev : zero.data()))) return rc;
+  return init_common_after_fields(c);
+}
+
+int rbn_get_populations(rbn_ctx *c, double *f, double *gpop) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!c->initialised) return fail(c, RBN_ESTATE, "not initialised");
+  const Geom &g = c->geom;
+  const bool shifted = c->prec == RBN_FP32;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int which = 0; which < 2; which++) {
+    double *dst = which ? gpop : f;
+    if (!dst) continue;
+    const void *src = which ? c->gA : c->fA;
+    const double lvl = shifted ? (which ? c->P.T0_shift : c->P.rho0_shift) : 0.0;
+    std::vector<unsigned char> buf(9 * g.plane * c->rs);
+    HIPCHK(c, hipMemcpy(buf.data(), src, buf.size(), hipMemcpyDeviceToHost));
+    for (int k = 0; k < 9; k++)
+      for (int i = 0; i < g.nx; i++)
+        for (int j = 0; j < g.ny; j++) {
+          size_t o = (size_t)k * g.plane + (size_t)j * g.pitch + i;
+          double val = shifted ? (double)reinterpret_cast<const float *>(buf.data())[o] + rbn::kW[k] * lvl
+                               : reinterpret_cast<const double *>(buf.data())[o];
+          dst[((size_t)i * g.ny + j) * 9 + k] = val;
+        }
+  }
+  return RBN_OK;
+}
+
+int rbn_get_fields(rbn_ctx *c, double *rho, double *T, double *u, double *v) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!c->initialised) return fail(c, RBN_ESTATE, "not initialised");
+  if ((rho || T) && !c->moments_valid) return fail(c, RBN_ESTATE, "rho/T of the latest step not available");
+  if (rho && (rc = download_field(c, c->rho, rho))) return rc;
+  if (T && (rc = download_field(c, c->T, T))) return rc;
+  if (u && (rc = download_field(c, c->u[c->cur], u))) return rc;
+  if (v && (rc = download_field(c, c->v[c->cur], v))) return rc;
+  return RBN_OK;
+}
+
+int rbn_energy(rbn_ctx *c, double *mean_energy, double *mean_T) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!c->initialised || !c->moments_valid) return fail(c, RBN_ESTATE, "moments not available");
+  double r[2];
+  if ((rc = run_energy(c, r))) return rc;
+  if (mean_energy) *mean_energy = r[0];
+  if (mean_T) *mean_T = r[1];
+  return RBN_OK;
+}
+
+int rbn_init_particles(rbn_ctx *c, int n, const int *id, const double *x, const double *y,
+                       const double *tau_p, const double *beta_p, const double *up, const double *vp) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (n < 0 || (n > 0 && (!id || !x || !y))) return fail(c, RBN_EINVAL, "rbn_init_particles: bad arguments");
+  if (!c->initialised) return fail(c, RBN_ESTATE, "initialise the fields before the particles");
+  if ((up == nullptr) != (vp == nullptr)) return fail(c, RBN_EINVAL, "give both up and vp or neither");
+  if (n > (1 << 28)) return fail(c, RBN_ERANGE, "too many particles");
+  dfree_t(c->d_part);
+  dfree_t(c->d_rec);
+  if (c->h_rec) (void)hipHostFree(c->h_rec);
+  c->h_rec = nullptr;
+  c->np = n;
+  c->pid.clear();
+  c->any_heavy = false;
+  if (n == 0) return RBN_OK;
+  std::vector<int> order((size_t)n);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return id[a] < id[b]; });
+  const size_t N = (size_t)n;
+  std::vector<double> h(PNARR * N, 0.0);
+  c->pid.resize(N);
+  for (size_t q = 0; q < N; q++) {
+    const int p = order[q];
+    c->pid[q] = id[p];
+    h[PX * N + q] = x[p];
+    h[PY * N + q] = y[p];
+    h[PXO * N + q] = x[p];
+    h[PYO * N + q] = y[p];
+    double tp = tau_p ? tau_p[p] : (c->P.heavyprtcl && !c->P.tracer ? c->P.taulag : 0.0);
+    h[PINVTAU * N + q] = tp > 0.0 ? (1 / tp) : 0.0;
+    if (tp > 0.0) c->any_heavy = true;
+    h[PBETA * N + q] = beta_p ? beta_p[p] : c->P.beta;
+    if (up) {
+      h[PU * N + q] = up[p];
+      h[PV * N + q] = vp[p];
+      h[PUO * N + q] = up[p];
+      h[PVO * N + q] = vp[p];
+    }
+  }
+  void *tmp = nullptr;
+  if ((rc = dalloc(c, &tmp, PNARR * N * sizeof(double)))) return rc;
+  c->d_part = (double *)tmp;
+  if ((rc = dalloc(c, &tmp, 17 * N * sizeof(double)))) return rc;
+  c->d_rec = (double *)tmp;
+  HIPCHK(c, hipMemsetAsync(c->d_rec, 0, 17 * N * sizeof(double), c->stream));
+  if (hipHostMalloc((void **)&c->h_rec, 17 * N * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+    c->h_rec = nullptr;
+    return fail(c, RBN_ENOMEM, "pinned record buffer allocation failed");
+  }
+  HIPCHK(c, hipMemcpyAsync(c->d_part, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if (!up) {
+    hipError_t e = rbn::launch_particle_init(c->stream, c->prec, c->geom, c->u[c->cur], c->v[c->cur],
+                                             part_ptrs(c, false), c->P.heavy_init_factor);
+    if (e != hipSuccess) return fail(c, RBN_EDEVICE, "particle init launch failed: %s", hipGetErrorString(e));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return check_err_flag(c);
+}
+
+int rbn_num_particles(const rbn_ctx *c, int *n) {
+  if (!c || !n) return rbn::thread_fail(RBN_EINVAL, "null argument");
+  *n = c->np;
+  return RBN_OK;
+}
+
+int rbn_get_particles(rbn_ctx *c, rbn_particles *out) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!out) return fail(c, RBN_EINVAL, "null output");
+  if (out->n < c->np) return fail(c, RBN_EINVAL, "output arrays hold %d particles, need %d", out->n, c->np);
+  out->n = c->np;
+  if (c->np == 0) return RBN_OK;
+  const size_t N = (size_t)c->np;
+  std::vector<double> h(8 * N);
+  HIPCHK(c, hipMemcpyAsync(h.data(), c->d_part, 8 * N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  double *dst[8] = {out->x, out->y, out->u, out->v, out->xo, out->yo, out->uo, out->vo};
+  for (int q = 0; q < 8; q++)
+    if (dst[q]) std::copy(h.begin() + q * N, h.begin() + (q + 1) * N, dst[q]);
+  if (out->id) std::copy(c->pid.begin(), c->pid.end(), out->id);
+  return RBN_OK;
+}
+
+int rbn_set_record_callback(rbn_ctx *c, rbn_record_fn fn, void *user) {
+  if (!c) return rbn::thread_fail(RBN_EINVAL, "null context");
+  c->rec_fn = fn;
+  c->rec_user = user;
+  return RBN_OK;
+}
+
+int rbn_set_diag_callback(rbn_ctx *c, rbn_diag_fn fn, void *user) {
+  if (!c) return rbn::thread_fail(RBN_EINVAL, "null context");
+  c->diag_fn = fn;
+  c->diag_user = user;
+  return RBN_OK;
+}
+
+int rbn_init_observers(rbn_ctx *c, int n, const int *id, const double *x, const double *y, const double *T) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (c->P.variant != RBN_VARIANT_NUDGING) return fail(c, RBN_ESTATE, "observers belong to the nudged variant");
+  if (n < 0 || (n > 0 && (!id || !x || !y || !T))) return fail(c, RBN_EINVAL, "rbn_init_observers: bad arguments");
+  if (n > (1 << 24)) return fail(c, RBN_ERANGE, "too many observers");
+  dfree_t(c->d_obs_i);
+  dfree_t(c->d_obs_d);
+  dfree_t(c->dep.list);
+  c->nobs = n;
+  c->oid.assign(id, id + n);
+  const size_t N = (size_t)(n ? n : 1);
+  void *tmp = nullptr;
+  if ((rc = dalloc(c, &tmp, ONI * N * sizeof(int)))) return rc;
+  c->d_obs_i = (int *)tmp;
+  if ((rc = dalloc(c, &tmp, OND * N * sizeof(double)))) return rc;
+  c->d_obs_d = (double *)tmp;
+  // Each observer reaches at most 49 nodes of its box plus 28 of one wrap window.
+  c->dep.list_cap = 77LL * (long long)N;
+  if ((rc = dalloc(c, &tmp, (size_t)c->dep.list_cap * sizeof(int)))) return rc;
+  c->dep.list = (int *)tmp;
+  HIPCHK(c, hipMemsetAsync(c->d_obs_i, 0, ONI * N * sizeof(int), c->stream));
+  std::vector<double> h(OND * N, 0.0);
+  for (int i = 0; i < n; i++) {
+    h[OXINT * N + i] = h[ORX * N + i] = x[i];
+    h[OYINT * N + i] = h[ORY * N + i] = y[i];
+    h[OTINT * N + i] = h[ORT * N + i] = T[i];
+  }
+  HIPCHK(c, hipMemcpyAsync(c->d_obs_d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  // set_nudge_vol zeroes nud_term and nudge_vol (N:2051-2056).
+  if (c->nud) HIPCHK(c, hipMemsetAsync(c->nud, 0, c->geom.plane * c->rs, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->dep.cnt, 0, c->geom.plane * sizeof(int), c->stream));
+  HIPCHK(c, hipMemsetAsync(c->dep.overflow, 0, sizeof(int), c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return RBN_OK;
+}
+
+int rbn_set_obs_provider(rbn_ctx *c, rbn_obs_fn fn, void *user) {
+  if (!c) return rbn::thread_fail(RBN_EINVAL, "null context");
+  c->obs_fn = fn;
+  c->obs_user = user;
+  return RBN_OK;
+}
+
+int rbn_get_nudging(rbn_ctx *c, int *nudge_vol, double *tempreadint, double *nud_term) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (c->P.variant != RBN_VARIANT_NUDGING) return fail(c, RBN_ESTATE, "not a nudged run");
+  const Geom &g = c->geom;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (nudge_vol) {
+    std::vector<int> h(g.plane);
+    HIPCHK(c, hipMemcpy(h.data(), c->dep.cnt, g.plane * sizeof(int), hipMemcpyDeviceToHost));
+    for (int i = 0; i < g.nx; i++)
+      for (int j = 0; j < g.ny; j++) nudge_vol[(size_t)i * g.ny + j] = h[(size_t)j * g.pitch + i];
+  }
+  if (tempreadint) {
+    std::vector<double> h(g.plane);
+    HIPCHK(c, hipMemcpy(h.data(), c->dep.Tbar, g.plane * sizeof(double), hipMemcpyDeviceToHost));
+    for (int i = 0; i < g.nx; i++)
+      for (int j = 0; j < g.ny; j++) tempreadint[(size_t)i * g.ny + j] = h[(size_t)j * g.pitch + i];
+  }
+  if (nud_term && (rc = download_field(c, c->nud, nud_term))) return rc;
+  return RBN_OK;
+}
+
+int rbn_step(rbn_ctx *c, int nsteps) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!c->initialised) return fail(c, RBN_ESTATE, "rbn_step before initialisation");
+  if (nsteps < 0) return fail(c, RBN_EINVAL, "negative step count");
+  const rbn_params &P = c->P;
+  const bool nudged = P.variant == RBN_VARIANT_NUDGING;
+  if (nudged && c->nobs == 0 && !c->d_obs_i) return fail(c, RBN_ESTATE, "nudged run without observers");
+  const bool parts = !nudged && P.particle && c->np > 0;
+  const int nd = P.ndiag > 0 ? P.ndiag : (nudged ? 5000 : P.nout);
+  const int right = P.fix_q3 ? P.nx - 3 : P.nx - 2;
+  for (int n = 0; n < nsteps; n++) {
+    const int istep = ++c->istep;
+    const bool record = parts && (istep % P.noutconfig) == 0;
+    const bool diag = nd > 0 && (istep % nd) == 0;
+    const bool moments = record || diag || n == nsteps - 1;
+    if (nudged) {
+      const int r = istep % P.ntempread;
+      if (r == 1 && (rc = load_observation(c, istep + P.ntempread - 1))) return rc;
+      const int branch = r == 1 ? 1 : (r == 0 ? 0 : 2);
+      rbn::ObsPtrs o = obs_ptrs(c);
+      if ((rc = timed(c, 2, [&] { return rbn::launch_obs_update(c->stream, branch, P.nx, P.ny, P.ntempread, o); })))
+        return rc;
+      if ((rc = timed(c, 3, [&] { return rbn::launch_deposit(c->stream, c->geom, right, o, c->dep); }))) return rc;
+    }
+    rbn::LbmPtrs lp{};
+    lp.fA = c->fA;
+    lp.gA = c->gA;
+    lp.fB = c->fB;
+    lp.gB = c->gB;
+    lp.F = c->F;
+    lp.u = c->u[c->cur ^ 1];
+    lp.v = c->v[c->cur ^ 1];
+    lp.rho = moments ? c->rho : nullptr;
+    lp.T = moments ? c->T : nullptr;
+    if (nudged) {
+      lp.nvol = c->dep.cnt;
+      lp.Tbar = c->dep.Tbar;
+      lp.nud = c->nud;
+    }
+    if ((rc = timed(c, 0, [&] { return rbn::launch_lbm(c->stream, c->prec, P.variant, c->geom, lp, c->consts.data()); })))
+      return rc;
+    std::swap(c->fA, c->fB);
+    std::swap(c->gA, c->gB);
+    c->cur ^= 1;
+    c->moments_valid = moments;
+    if (parts) {
+      const int full = record && P.prtcl_writefull;
+      rbn::PartPtrs pp = part_ptrs(c, record);
+      if ((rc = timed(c, 1, [&] {
+             return rbn::launch_particles(c->stream, c->prec, c->geom, c->u[c->cur], c->v[c->cur],
+                                          c->u[c->cur ^ 1], c->v[c->cur ^ 1], c->T, pp,
+                                          c->any_heavy || full, record, full);
+           })))
+        return rc;
+      if (record && c->rec_fn && (rc = deliver_record(c, istep))) return rc;
+    }
+    if (diag && c->diag_fn) {
+      double r[2];
+      if ((rc = run_energy(c, r))) return rc;
+      if (c->diag_fn(c->diag_user, istep, r[0], r[1]) != 0)
+        return fail(c, RBN_ECALLBACK, "diagnostics callback failed at step %d", istep);
+    }
+  }
+  if (c->timing && (rc = drain_timing(c))) return rc;
+  if (parts || nudged) return check_err_flag(c);
+  return RBN_OK;
+}
+
+int rbn_get_step(const rbn_ctx *c, int *istep) {
+  if (!c || !istep) return rbn::thread_fail(RBN_EINVAL, "null argument");
+  *istep = c->istep;
+  return RBN_OK;
+}
+
+int rbn_synchronize(rbn_ctx *c) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return RBN_OK;
+}
+
+int rbn_set_kernel_timing(rbn_ctx *c, int enable) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!enable && (rc = drain_timing(c))) return rc;
+  c->timing = enable != 0;
+  return RBN_OK;
+}
+
+int rbn_kernel_stats(rbn_ctx *c, const char *name, int reset, long long *launches, double *total_ms) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  const int k = kernel_index(name);
+  if (k < 0) return fail(c, RBN_EINVAL, "unknown kernel name");
+  if ((rc = drain_timing(c))) return rc;
+  if (launches) *launches = c->k_launches[k];
+  if (total_ms) *total_ms = c->k_ms[k];
+  if (reset) {
+    c->k_launches[k] = 0;
+    c->k_ms[k] = 0.0;
+  }
+  return RBN_OK;
+}
+
+int rbn_time_steps(rbn_ctx *c, int nsteps, double *elapsed_ms) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!elapsed_ms) return fail(c, RBN_EINVAL, "null output");
+  hipEvent_t a = nullptr, b = nullptr;
+  HIPCHK(c, hipEventCreate(&a));
+  HIPCHK(c, hipEventCreate(&b));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipEventRecord(a, c->stream));
+  rc = rbn_step(c, nsteps);
+  if (rc == RBN_OK) {
+    HIPCHK(c, hipEventRecord(b, c->stream));
+    HIPCHK(c, hipEventSynchronize(b));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, a, b));
+    *elapsed_ms = ms;
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return rc;
+}
+
+}  // extern "C"

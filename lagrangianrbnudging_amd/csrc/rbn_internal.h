@@ -1,0 +1,171 @@
+// rbn_internal.h — shared declarations of librbn's host code and HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/rbn.h"
+
+namespace rbn {
+
+const char *thread_error();
+int thread_fail(int code, const char *msg);
+
+// D2Q9 lattice of the reference (dellar, Trajectories/code/temppart_dump.c:1363-1401).
+// The reverse directions are negations of the forward ones, hence the signed zeros.
+constexpr double kW[9] = {16. / 36., 4. / 36., 4. / 36., 4. / 36., 4. / 36.,
+                          1. / 36.,  1. / 36., 1. / 36., 1. / 36.};
+constexpr double kCx[9] = {0., 1., 0., -1., -0., 1., -1., -1., 1.};
+constexpr double kCy[9] = {0., 0., 1., -0., -1., 1., 1., -1., -1.};
+
+// Device lattice geometry.  Populations are structure-of-arrays planes
+// [k][j][i] (x fastest, rows of `pitch` elements), interior nodes only: the periodic x
+// direction is an index wrap and the walls are handled in-kernel, so no halo is stored.
+struct Geom {
+  int nx, ny, pitch;
+  size_t plane;  // pitch * ny
+};
+
+// Per-step constants of the fused step, precomputed on the host with the reference's
+// expressions so that the FP64 kernel reproduces them bit for bit.
+template <typename R>
+struct StepConst {
+  R om, one_m_om;     // 1./relax, 1.-om
+  R omg, one_m_omg;   // 1./relaxg, 1.-omg
+  R src[9];           // (1.0-0.5*om)*w_k        (collisf, T:1320)
+  R bw[9];            // (1.0-0.5*omg)*w_k       (nudged collisg, N:1931-1933)
+  R g0;               // -gravity                (forcingconstructWW, T:752)
+  R wtop[3], wbot[3]; // 2.0*w_k*Tup, 2.0*w_k*Tdown for k = 2,5,6 / 4,7,8 (T:896-921)
+  R alpha;
+  R rho0, T0;         // FP32 shifts (zero in FP64)
+};
+
+struct LbmPtrs {
+  const void *fA, *gA;
+  void *fB, *gB;
+  void *F;           // lagged F_y: read F(s), write F(s+1) (one value per node)
+  void *u, *v;       // hydrovar velocity of this step
+  void *rho, *T;     // written when non-null (record/diagnostic/last steps)
+  const int *nvol;   // nudged run: nudge_vol counts (nullptr: not nudged)
+  const double *Tbar;// nudged run: tempreadint
+  void *nud;         // nudged run: persistent nud_term
+};
+
+// Particle kernel arguments (device pointers).
+struct PartPtrs {
+  int n;
+  double *x, *y, *u, *v, *xo, *yo, *uo, *vo;
+  const double *inv_tau, *beta;  // inv_tau <= 0: tracer
+  double *rec;                   // 17 * n record arrays, or nullptr
+  int *err;
+};
+
+// Observer (nudging) arrays on the device, observer order.
+struct ObsPtrs {
+  int n;
+  const int *chk;
+  int *bcy, *xidx, *yidx;
+  double *dx, *dy, *dT, *xint, *yint, *Tint;
+  const double *rx, *ry, *rT;  // current record, record order
+};
+
+// Deposit work arrays.
+struct DepPtrs {
+  int *cnt;    // per interior node: contributions (== nudge_vol), [j][i] with pitch
+  int *fill;   // per node fill cursor
+  int *off;    // per node exclusive prefix of cnt (+1 total at the end)
+  int *list;   // contributions, grouped per node
+  int *blocksum;
+  double *Tbar;
+  long long list_cap;
+  int *overflow;
+};
+
+// Launchers (defined in the .hip files).
+hipError_t launch_lbm(hipStream_t s, int prec, int variant, const Geom &g, const LbmPtrs &p,
+                      const void *step_const);
+hipError_t launch_init_eq(hipStream_t s, int prec, const Geom &g, const void *rho, const void *T,
+                          const void *u, const void *v, void *f, void *gp, void *F,
+                          const void *step_const);
+hipError_t launch_particles(hipStream_t s, int prec, const Geom &g, const void *u, const void *v,
+                            const void *up, const void *vp, const void *T, const PartPtrs &pp,
+                            int need_deriv, int record, int full);
+hipError_t launch_particle_init(hipStream_t s, int prec, const Geom &g, const void *u,
+                                const void *v, const PartPtrs &pp, double heavy_factor);
+hipError_t launch_obs_update(hipStream_t s, int branch, int nx, int ny, int ntempread,
+                             const ObsPtrs &o);
+hipError_t launch_deposit(hipStream_t s, const Geom &g, int right_limit, const ObsPtrs &o,
+                          const DepPtrs &d);
+hipError_t launch_energy(hipStream_t s, int prec, const Geom &g, const void *rho, const void *u,
+                         const void *v, const void *T, double *partial, int nblocks,
+                         double *out2);
+int energy_blocks(const Geom &g);
+
+}  // namespace rbn
+
+// Context: one lattice on one GPU.
+struct rbn_ctx {
+  rbn_params P{};
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  rbn::Geom geom{};
+  int prec = RBN_FP64;
+  size_t rs = 8;  // bytes per real
+  std::vector<unsigned char> consts;  // StepConst<R>
+
+  // lattice state
+  void *fA = nullptr, *fB = nullptr, *gA = nullptr, *gB = nullptr;
+  void *F = nullptr;
+  void *u[2] = {nullptr, nullptr}, *v[2] = {nullptr, nullptr};
+  int cur = 0;  // u[cur], v[cur]: latest hydrovar velocity; u[cur^1]: the step before
+  void *rho = nullptr, *T = nullptr;
+  void *nud = nullptr;
+  int istep = 0;
+  bool initialised = false;
+  bool moments_valid = false;  // rho/T hold the latest hydrovar
+
+  // particles
+  int np = 0;
+  std::vector<int> pid;  // ids in ascending order
+  double *d_part = nullptr;  // 10 arrays of np: x,y,u,v,xo,yo,uo,vo,inv_tau,beta
+  bool any_heavy = false;
+  double *d_rec = nullptr;   // 17 * np
+  double *h_rec = nullptr;   // pinned mirror
+  int *d_err = nullptr;
+  rbn_record_fn rec_fn = nullptr;
+  void *rec_user = nullptr;
+
+  // observers
+  int nobs = 0;
+  std::vector<int> oid;
+  int *d_obs_i = nullptr;     // chk, bcy, xidx, yidx (4 * nobs)
+  double *d_obs_d = nullptr;  // dx, dy, dT, xint, yint, Tint, rx, ry, rT (9 * nobs)
+  rbn_obs_fn obs_fn = nullptr;
+  void *obs_user = nullptr;
+  std::vector<int> h_rid;
+  std::vector<double> h_rx, h_ry, h_rT;
+  std::vector<int> h_chk;
+  rbn::DepPtrs dep{};
+
+  // diagnostics
+  rbn_diag_fn diag_fn = nullptr;
+  void *diag_user = nullptr;
+  double *d_energy = nullptr;  // partials + 2 results
+
+  // timing
+  bool timing = false;
+  struct Pending {
+    int kernel;
+    hipEvent_t a, b;
+  };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> event_pool;
+  long long k_launches[4] = {0, 0, 0, 0};
+  double k_ms[4] = {0, 0, 0, 0};
+
+  std::string err;
+};

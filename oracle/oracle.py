@@ -1,0 +1,249 @@
+"""ctypes binding of the CPU FP64 oracle (oracle/rbn_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's CPU
+baseline, where it is the checker (or the timed CPU restatement of the reference), never
+the thing measured on the GPU.  Its Python API mirrors lagrangianrbnudging_amd.rbn.Solver so
+that a scenario can run unchanged on both.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+from typing import Optional
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "_build" / "liboracle.so"
+
+_DP = C.POINTER(C.c_double)
+_IP = C.POINTER(C.c_int)
+_lib = None
+
+_SIGS = {
+    "or_create": (C.c_void_p, [C.c_void_p]),
+    "or_destroy": (None, [C.c_void_p]),
+    "or_last_error": (C.c_char_p, [C.c_void_p]),
+    "or_set_faithful": (None, [C.c_void_p, C.c_int]),
+    "or_init_fields": (C.c_int, [C.c_void_p, _DP, _DP, _DP, _DP]),
+    "or_init_scratch_reference": (C.c_int, [C.c_void_p]),
+    "or_set_populations": (C.c_int, [C.c_void_p, _DP, _DP, _DP, _DP, _DP, _DP]),
+    "or_get_populations": (C.c_int, [C.c_void_p, _DP, _DP]),
+    "or_get_fields": (C.c_int, [C.c_void_p, _DP, _DP, _DP, _DP]),
+    "or_energy": (C.c_int, [C.c_void_p, _DP, _DP]),
+    "or_init_particles": (C.c_int, [C.c_void_p, C.c_int, _IP, _DP, _DP, _DP, _DP, _DP, _DP]),
+    "or_get_particles": (C.c_int, [C.c_void_p, _IP] + [_DP] * 8),
+    "or_num_records": (C.c_int, [C.c_void_p]),
+    "or_get_record": (C.c_int, [C.c_void_p, C.c_int, _IP, _IP, _IP, _DP]),
+    "or_init_observers": (C.c_int, [C.c_void_p, C.c_int, _IP, _DP, _DP, _DP]),
+    "or_add_obs_record": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _IP, _DP, _DP, _DP]),
+    "or_get_nudging": (C.c_int, [C.c_void_p, _IP, _DP, _DP]),
+    "or_get_observer_state": (C.c_int, [C.c_void_p, _DP, _DP, _DP, _IP, _IP]),
+    "or_equilibrium": (None, [C.c_double, C.c_double, C.c_double, _DP]),
+    "or_bilinear_global": (C.c_double, [C.c_void_p, C.c_int, C.c_double, C.c_double, _IP]),
+    "or_deposit_only": (C.c_int, [C.c_void_p, C.c_int, _IP, _IP, _DP]),
+    "or_step": (C.c_int, [C.c_void_p, C.c_int]),
+    "or_get_step": (C.c_int, [C.c_void_p]),
+    "or_num_diag": (C.c_int, [C.c_void_p]),
+    "or_get_diag": (C.c_int, [C.c_void_p, C.c_int, _IP, _DP, _DP]),
+}
+
+RECORD_FIELDS = ("x", "y", "u", "v", "xo", "yo", "uo", "vo", "temp")
+FULL_FIELDS = ("dtu", "dtv", "dxu", "dxv", "dyu", "dyv", "fu", "fv")
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB_PATH
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build()
+        lib = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def _dp(a):
+    return None if a is None else a.ctypes.data_as(_DP)
+
+
+def _f64(a, shape=None):
+    if a is None:
+        return None
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a.reshape(shape) if shape is not None else a
+
+
+def equilibrium(r1: float, u: float, v: float) -> np.ndarray:
+    out = np.empty(9)
+    load().or_equilibrium(r1, u, v, _dp(out))
+    return out
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+class Oracle:
+    """Single-domain FP64 restatement of the reference step (see rbn_oracle.h)."""
+
+    def __init__(self, params):
+        self.lib = load()
+        self.params = params.copy()
+        self.sim = self.lib.or_create(C.byref(self.params))
+        if not self.sim:
+            raise OracleError("or_create failed")
+        self.nx, self.ny = self.params.nx, self.params.ny
+        self._nrec_seen = 0
+
+    def close(self):
+        if self.sim:
+            self.lib.or_destroy(self.sim)
+            self.sim = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _c(self, rc):
+        if rc != 0:
+            raise OracleError(f"{rc}: {self.lib.or_last_error(self.sim).decode()}")
+
+    def set_faithful(self, on: bool):
+        self.lib.or_set_faithful(self.sim, 1 if on else 0)
+
+    def init_fields(self, rho=None, T=None, u=None, v=None):
+        sh = (self.nx, self.ny)
+        arrs = [_f64(a, sh) for a in (rho, T, u, v)]
+        self._c(self.lib.or_init_fields(self.sim, *[_dp(a) for a in arrs]))
+
+    def init_scratch_reference(self):
+        self._c(self.lib.or_init_scratch_reference(self.sim))
+
+    def set_populations(self, f, g, rho_prev, T_prev, u_prev=None, v_prev=None):
+        sh = (self.nx, self.ny)
+        f = _f64(f, (self.nx, self.ny, 9))
+        g = _f64(g, (self.nx, self.ny, 9))
+        arrs = [_f64(a, sh) for a in (rho_prev, T_prev, u_prev, v_prev)]
+        self._c(self.lib.or_set_populations(self.sim, _dp(f), _dp(g), *[_dp(a) for a in arrs]))
+
+    def populations(self):
+        f = np.empty((self.nx, self.ny, 9))
+        g = np.empty((self.nx, self.ny, 9))
+        self._c(self.lib.or_get_populations(self.sim, _dp(f), _dp(g)))
+        return f, g
+
+    def fields(self):
+        out = {k: np.empty((self.nx, self.ny)) for k in ("rho", "T", "u", "v")}
+        self._c(self.lib.or_get_fields(self.sim, *[_dp(out[k]) for k in ("rho", "T", "u", "v")]))
+        return out
+
+    def energy(self):
+        e, t = C.c_double(), C.c_double()
+        self._c(self.lib.or_energy(self.sim, C.byref(e), C.byref(t)))
+        return e.value, t.value
+
+    def init_particles(self, ids, x, y, tau_p=None, beta_p=None, up=None, vp=None):
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        arrs = [_f64(a) for a in (x, y, tau_p, beta_p, up, vp)]
+        self._c(self.lib.or_init_particles(self.sim, len(ids), ids.ctypes.data_as(_IP), *[_dp(a) for a in arrs]))
+
+    def particles(self):
+        n = self.lib.or_get_particles(self.sim, None, *([None] * 8))
+        out = {k: np.empty(max(n, 1)) for k in ("x", "y", "u", "v", "xo", "yo", "uo", "vo")}
+        ids = np.empty(max(n, 1), dtype=np.int32)
+        self.lib.or_get_particles(self.sim, ids.ctypes.data_as(_IP),
+                                  *[_dp(out[k]) for k in ("x", "y", "u", "v", "xo", "yo", "uo", "vo")])
+        res = {k: a[:n].copy() for k, a in out.items()}
+        res["id"] = ids[:n].copy()
+        return res
+
+    @property
+    def records(self) -> list:
+        out = []
+        for r in range(self.lib.or_num_records(self.sim)):
+            step, n = C.c_int(), C.c_int()
+            self._c(self.lib.or_get_record(self.sim, r, C.byref(step), C.byref(n), None, None))
+            ids = np.empty(max(n.value, 1), dtype=np.int32)
+            a = np.empty((18, max(n.value, 1)))
+            self._c(self.lib.or_get_record(self.sim, r, None, None, ids.ctypes.data_as(_IP), _dp(a)))
+            d = {"step": step.value, "id": ids[: n.value].copy()}
+            for q, k in enumerate(RECORD_FIELDS):
+                d[k] = a[1 + q, : n.value].copy()
+            for q, k in enumerate(FULL_FIELDS):
+                d[k] = a[10 + q, : n.value].copy() if self.params.prtcl_writefull else None
+            out.append(d)
+        return out
+
+    def init_observers(self, ids, x, y, T):
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        self._c(self.lib.or_init_observers(self.sim, len(ids), ids.ctypes.data_as(_IP),
+                                           *[_dp(_f64(a)) for a in (x, y, T)]))
+
+    def add_observation(self, step: int, rec: dict):
+        ids = np.ascontiguousarray(rec["id"], dtype=np.int32)
+        self._c(self.lib.or_add_obs_record(self.sim, step, len(ids), ids.ctypes.data_as(_IP),
+                                           *[_dp(_f64(rec[k])) for k in ("x", "y", "temp")]))
+
+    def set_observation_source(self, records: dict):
+        for step, rec in records.items():
+            self.add_observation(step, rec)
+
+    def nudging(self):
+        nv = np.empty((self.nx, self.ny), dtype=np.int32)
+        tb = np.empty((self.nx, self.ny))
+        nd = np.empty((self.nx, self.ny))
+        self._c(self.lib.or_get_nudging(self.sim, nv.ctypes.data_as(_IP), _dp(tb), _dp(nd)))
+        return {"nudge_vol": nv, "tempreadint": tb, "nud_term": nd}
+
+    def observer_state(self):
+        n = self.lib.or_get_observer_state(self.sim, None, None, None, None, None)
+        out = {k: np.empty(max(n, 1)) for k in ("xint", "yint", "Tint")}
+        xi = np.empty(max(n, 1), dtype=np.int32)
+        yi = np.empty(max(n, 1), dtype=np.int32)
+        self.lib.or_get_observer_state(self.sim, _dp(out["xint"]), _dp(out["yint"]), _dp(out["Tint"]),
+                                       xi.ctypes.data_as(_IP), yi.ctypes.data_as(_IP))
+        res = {k: a[:n].copy() for k, a in out.items()}
+        res["xidx"], res["yidx"] = xi[:n].copy(), yi[:n].copy()
+        return res
+
+    def deposit_only(self, xidx, yidx, T):
+        xidx = np.ascontiguousarray(xidx, dtype=np.int32)
+        yidx = np.ascontiguousarray(yidx, dtype=np.int32)
+        self._c(self.lib.or_deposit_only(self.sim, len(xidx), xidx.ctypes.data_as(_IP),
+                                         yidx.ctypes.data_as(_IP), _dp(_f64(T))))
+
+    def bilinear(self, field: str, x: float, y: float) -> float:
+        err = C.c_int(0)
+        k = {"u": 0, "v": 1, "T": 2, "rho": 3}[field]
+        val = self.lib.or_bilinear_global(self.sim, k, x, y, C.byref(err))
+        if err.value:
+            raise OracleError("bilinear corner out of range")
+        return val
+
+    def step(self, n: int = 1):
+        self._c(self.lib.or_step(self.sim, int(n)))
+
+    @property
+    def istep(self) -> int:
+        return self.lib.or_get_step(self.sim)
+
+    @property
+    def diagnostics(self):
+        out = []
+        for k in range(self.lib.or_num_diag(self.sim)):
+            s, e, t = C.c_int(), C.c_double(), C.c_double()
+            self.lib.or_get_diag(self.sim, k, C.byref(s), C.byref(e), C.byref(t))
+            out.append((s.value, e.value, t.value))
+        return out

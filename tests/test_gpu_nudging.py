@@ -1,0 +1,139 @@
+"""Nudged run (observer interpolation + ordered deposit + nudged collision): GPU vs oracle,
+bit-exact in FP64."""
+import numpy as np
+import pytest
+
+from lagrangianrbnudging_amd import rbn
+from oracle.oracle import Oracle
+from scenarios import assert_equal_arrays, conduction_T, random_particles, rb_params, truth_records
+
+pytestmark = pytest.mark.gpu
+
+
+def _init_T_at(p, T0, x, y):
+    o = Oracle(p)
+    o.init_fields(T=T0)
+    return np.array([o.bilinear("T", a, b) for a, b in zip(x, y)])
+
+
+def _nudged_pair(pn, T0n, obs_ids, ox, oy, oT, records, steps):
+    g = rbn.Solver(pn)
+    g.init_fields(T=T0n)
+    g.init_observers(obs_ids, ox, oy, oT)
+    g.set_observation_source(records)
+    o = Oracle(pn)
+    o.init_fields(T=T0n)
+    o.init_observers(obs_ids, ox, oy, oT)
+    o.set_observation_source(records)
+    g.step(steps)
+    o.step(steps)
+    return g, o
+
+
+def _compare(g, o):
+    fg, gg = g.populations()
+    fo, go = o.populations()
+    assert_equal_arrays(fg, fo, "f")
+    assert_equal_arrays(gg, go, "g")
+    a, b = g.nudging(), o.nudging()
+    for k in ("nudge_vol", "tempreadint", "nud_term"):
+        assert_equal_arrays(a[k], b[k], k)
+    a, b = g.fields(), o.fields()
+    for k in ("rho", "T", "u", "v"):
+        assert_equal_arrays(a[k], b[k], k)
+
+
+@pytest.mark.parametrize("alpha", [0.01, 0.1, 1.0])
+def test_nudged_run_fp64_bit_exact(alpha):
+    pt = rb_params(64, 32, 1e5, noutconfig=5)
+    T0 = conduction_T(pt, 1e-2, seed=5)
+    ids, x, y, tau = random_particles(pt, 150, 0, seed=21)
+    records = truth_records(pt, T0, (ids, x, y, tau), 60)
+    # observers: the truth run's initial state in its id order, permuted
+    order = np.random.default_rng(3).permutation(len(ids))
+    oT = _init_T_at(pt, T0, x, y)
+    pn = pt.copy()
+    pn.variant = rbn.RBN_VARIANT_NUDGING
+    pn.ntempread = 5
+    pn.alpha = alpha
+    pn.ndiag = 10
+    T0n = conduction_T(pn, 1e-2, seed=6)
+    g, o = _nudged_pair(pn, T0n, ids[order], x[order], y[order], oT[order], records, 53)
+    _compare(g, o)
+    assert np.count_nonzero(g.nudging()["nudge_vol"]) > 0
+
+
+def _synthetic_records(nx, ny, n, steps, every, seed):
+    """Observers on straight lines that cross the periodic x boundary and the walls."""
+    rng = np.random.default_rng(seed)
+    x0 = np.concatenate([[0.3, 3.2, nx - 2.0, nx - 1.4, nx - 0.2, 1.9], rng.uniform(0, nx, n - 6)])
+    y0 = np.concatenate([[1.1, ny - 0.1, 2.4, 1.6, ny - 1.2, ny / 2], rng.uniform(1, ny, n - 6)])
+    vx = rng.uniform(-0.15, 0.15, n)
+    vy = rng.uniform(-0.1, 0.1, n)
+    T = rng.uniform(0, 1, n)
+    ids = np.arange(n, dtype=np.int32)
+    recs = {}
+    for s in range(every, steps + every, every):
+        xs = x0 + vx * s
+        ys = y0 + vy * s
+        xs = np.mod(xs, nx)
+        ys = 1 + np.mod(ys - 1, ny - 1)
+        perm = rng.permutation(n)
+        recs[s] = {"id": ids[perm], "x": xs[perm], "y": ys[perm], "temp": (T + 0.01 * s)[perm]}
+    return ids, x0, y0, T, recs
+
+
+@pytest.mark.parametrize("fix_q3", [0, 1])
+def test_nudged_deposit_edges_fp64(fix_q3):
+    pn = rb_params(40, 24, 1e5)
+    pn.variant = rbn.RBN_VARIANT_NUDGING
+    pn.ntempread = 4
+    pn.alpha = 0.3
+    pn.fix_q3 = fix_q3
+    ids, x0, y0, T, recs = _synthetic_records(pn.nx, pn.ny, 60, 120, 4, seed=8)
+    T0n = conduction_T(pn, 1e-2, seed=4)
+    g, o = _nudged_pair(pn, T0n, ids, x0, y0, T, recs, 101)
+    _compare(g, o)
+
+
+def test_missing_observer_id_is_an_error():
+    pn = rb_params(32, 16, 1e5)
+    pn.variant = rbn.RBN_VARIANT_NUDGING
+    pn.ntempread = 4
+    g = rbn.Solver(pn)
+    g.init_fields(T=conduction_T(pn, 1e-2, 1))
+    g.init_observers(np.array([0, 1], np.int32), np.array([3.0, 5.0]), np.array([3.0, 5.0]), np.array([0.2, 0.4]))
+    g.set_observation_source({4: {"id": np.array([0, 7], np.int32), "x": np.array([3.0, 5.0]),
+                                  "y": np.array([3.0, 5.0]), "temp": np.array([0.1, 0.2])}})
+    with pytest.raises(rbn.RbnError) as e:
+        g.step(1)
+    assert e.value.code == -9
+
+
+def test_fp32_nudged_runs():
+    pt = rb_params(64, 32, 1e5, noutconfig=5)
+    T0 = conduction_T(pt, 1e-2, seed=5)
+    ids, x, y, tau = random_particles(pt, 100, 0, seed=21)
+    records = truth_records(pt, T0, (ids, x, y, tau), 60)
+    oT = _init_T_at(pt, T0, x, y)
+    pn = pt.copy()
+    pn.variant = rbn.RBN_VARIANT_NUDGING
+    pn.ntempread = 5
+    pn.alpha = 0.1
+    p32 = pn.copy()
+    p32.precision = rbn.RBN_FP32
+    T0n = conduction_T(pn, 1e-2, seed=6)
+    g = rbn.Solver(p32)
+    g.init_fields(T=T0n)
+    g.init_observers(ids, x, y, oT)
+    g.set_observation_source(records)
+    o = Oracle(pn)
+    o.init_fields(T=T0n)
+    o.init_observers(ids, x, y, oT)
+    o.set_observation_source(records)
+    g.step(53)
+    o.step(53)
+    a, b = g.nudging(), o.nudging()
+    assert_equal_arrays(a["nudge_vol"], b["nudge_vol"], "nudge_vol")  # positions are FP64 data
+    assert np.max(np.abs(a["tempreadint"] - b["tempreadint"])) == 0.0
+    assert np.max(np.abs(g.fields()["T"] - o.fields()["T"])) < 1e-5

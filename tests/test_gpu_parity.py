@@ -1,0 +1,186 @@
+"""Parity of librbn's HIP path with the CPU FP64 oracle (through the C-ABI).
+
+FP64: bit-exact (np.array_equal) on populations, fields, particle state and trajectory
+records.  Shapes are small enough that the oracle finishes in seconds.
+"""
+import numpy as np
+import pytest
+
+from lagrangianrbnudging_amd import rbn
+from oracle.oracle import Oracle
+from scenarios import (assert_equal_arrays, conduction_T, max_rel, random_particles, rb_params,
+                       setup)
+
+pytestmark = pytest.mark.gpu
+
+
+def _run_both(p, T0, particles, steps, collect=True):
+    g = rbn.Solver(p)
+    if collect:
+        g.collect_records()
+        g.collect_diagnostics()
+    setup(g, p, T0, particles)
+    o = setup(Oracle(p), p, T0, particles)
+    g.step(steps)
+    o.step(steps)
+    return g, o
+
+
+def _compare_state(g, o, what=""):
+    fg, gg = g.populations()
+    fo, go = o.populations()
+    assert_equal_arrays(fg, fo, what + "f")
+    assert_equal_arrays(gg, go, what + "g")
+    a, b = g.fields(), o.fields()
+    for k in ("rho", "T", "u", "v"):
+        assert_equal_arrays(a[k], b[k], what + k)
+
+
+def _compare_particles(g, o):
+    a, b = g.particles(), o.particles()
+    for k in ("id", "x", "y", "u", "v", "xo", "yo", "uo", "vo"):
+        assert_equal_arrays(a[k], b[k], "particle " + k)
+
+
+def _compare_records(g, o, full):
+    ra, rb = g.records, o.records
+    assert [r["step"] for r in ra] == [r["step"] for r in rb]
+    keys = ["x", "y", "u", "v", "xo", "yo", "uo", "vo", "temp"]
+    if full:
+        keys += ["dtu", "dtv", "dxu", "dxv", "dyu", "dyv", "fu", "fv"]
+    for x, y in zip(ra, rb):
+        assert_equal_arrays(x["id"], y["id"], f"record {x['step']} id")
+        for k in keys:
+            assert_equal_arrays(x[k], y[k], f"record {x['step']} {k}")
+
+
+@pytest.mark.parametrize("writefull", [0, 1])
+def test_truth_run_fp64_bit_exact_tiny(writefull):
+    p = rb_params(32, 16, 1e5, noutconfig=7, prtcl_writefull=writefull, nout=50)
+    T0 = conduction_T(p, 1e-2, seed=11)
+    parts = random_particles(p, 24, 24, seed=12)
+    g, o = _run_both(p, T0, parts, 300)
+    _compare_state(g, o)
+    _compare_particles(g, o)
+    _compare_records(g, o, writefull)
+    assert len(g.records) == 300 // 7
+    # evolution.dat: deterministic device reduction vs sequential sum
+    dg, do = g.diagnostics, o.diagnostics
+    assert [d[0] for d in dg] == [d[0] for d in do]
+    for a, b in zip(dg, do):
+        assert abs(a[1] - b[1]) <= 1e-12 * abs(b[1]) + 1e-300
+        assert abs(a[2] - b[2]) <= 1e-12 * abs(b[2])
+
+
+def test_truth_run_fp64_bit_exact_config1_shape():
+    # 256x128, Ra = 1e6, 1000 tracers (the CPU-runnable reference case), 600 steps.
+    p = rb_params(256, 128, 1e6, noutconfig=10)
+    T0 = conduction_T(p, 1e-3, seed=1)
+    parts = random_particles(p, 1000, 0, seed=2)
+    g, o = _run_both(p, T0, parts, 600)
+    _compare_state(g, o)
+    _compare_particles(g, o)
+    _compare_records(g, o, 0)
+
+
+def test_reference_scratch_init_fp64():
+    # The reference's tanh front with srand(0)/rand() jitter, Tdown = 0.01, Tup = -0.01.
+    p = rbn.default_params(nx=48, ny=24, scratch=1, noutconfig=5, nout=1000)
+    g = rbn.Solver(p)
+    g.init_fields()
+    o = Oracle(p)
+    o.init_scratch_reference()
+    _compare_state(g, o, "init ")
+    ids = np.arange(16, dtype=np.int32)
+    x = np.linspace(1.5, 46.5, 16)
+    y = np.linspace(1.2, 23.8, 16)
+    g.init_particles(ids, x, y)  # reference kind: heavy, taulag = 2150, 0.98 * fluid velocity
+    o.init_particles(ids, x, y)
+    g.step(50)
+    o.step(50)
+    _compare_state(g, o)
+    _compare_particles(g, o)
+
+
+def test_restart_round_trip_fp64():
+    p = rb_params(40, 20, 2e5, noutconfig=1000)
+    T0 = conduction_T(p, 1e-2, seed=3)
+    a = rbn.Solver(p)
+    a.init_fields(T=T0)
+    a.step(120)
+    f, gpop = a.populations()
+    fl = a.fields()
+    a.step(80)
+    b = rbn.Solver(p)
+    b.set_populations(f, gpop, fl["rho"], fl["T"], fl["u"], fl["v"])
+    b.step(80)
+    fa, ga = a.populations()
+    fb, gb = b.populations()
+    assert_equal_arrays(fa, fb, "f")
+    assert_equal_arrays(ga, gb, "g")
+    # and the oracle restarted from the same state agrees
+    o = Oracle(p)
+    o.set_populations(f, gpop, fl["rho"], fl["T"], fl["u"], fl["v"])
+    o.step(80)
+    fo, go = o.populations()
+    assert_equal_arrays(fb, fo, "f oracle")
+    assert_equal_arrays(gb, go, "g oracle")
+
+
+def test_no_particles_and_energy():
+    p = rb_params(64, 32, 1e5, particle=1)
+    T0 = conduction_T(p, 1e-2, seed=5)
+    g = rbn.Solver(p)
+    g.init_fields(T=T0)
+    g.init_particles(np.zeros(0, np.int32), np.zeros(0), np.zeros(0))
+    o = Oracle(p)
+    o.init_fields(T=T0)
+    g.step(100)
+    o.step(100)
+    _compare_state(g, o)
+    eg, tg = g.energy()
+    eo, to = o.energy()
+    assert abs(eg - eo) <= 1e-12 * eo and abs(tg - to) <= 1e-12 * abs(to)
+
+
+def test_two_contexts_are_independent():
+    p = rb_params(32, 16, 1e5)
+    a, b = rbn.Solver(p), rbn.Solver(p)
+    a.init_fields(T=conduction_T(p, 1e-2, 7))
+    b.init_fields(T=conduction_T(p, 1e-2, 7))
+    a.step(10)
+    b.step(5)
+    b.step(5)
+    assert_equal_arrays(a.populations()[1], b.populations()[1], "g")
+
+
+def test_particle_leaving_lattice_is_an_error():
+    p = rb_params(32, 16, 1e5)
+    g = rbn.Solver(p)
+    g.init_fields(T=conduction_T(p, 1e-2, 7))
+    with pytest.raises(rbn.RbnError) as e:
+        g.init_particles(np.array([0], np.int32), np.array([-5.0]), np.array([4.0]))
+    assert e.value.code == -6
+
+
+def test_fp32_shifted_error_against_fp64():
+    # FP32 with shifted populations against the FP64 oracle after 2000 steps at 128x64,
+    # Ra = 1e6 (convective, growing perturbation).  Stated bound: max |T32 - T64| <= 2e-5
+    # and max |u32 - u64| <= 2e-5 * max|u64| + 1e-7.
+    p64 = rb_params(128, 64, 1e6)
+    p32 = p64.copy()
+    p32.precision = rbn.RBN_FP32
+    T0 = conduction_T(p64, 1e-3, seed=9)
+    g = rbn.Solver(p32)
+    g.init_fields(T=T0)
+    o = Oracle(p64)
+    o.init_fields(T=T0)
+    g.step(2000)
+    o.step(2000)
+    a, b = g.fields(), o.fields()
+    errT = np.max(np.abs(a["T"] - b["T"]))
+    erru = np.max(np.abs(a["u"] - b["u"]))
+    print(f"FP32 vs FP64: max|dT| {errT:.3e}, max|du| {erru:.3e} (max|u| {np.max(np.abs(b['u'])):.3e}), "
+          f"rel rho {max_rel(a['rho'], b['rho']):.3e}")
+    assert errT <= 2e-5
+    assert erru <= 2e-5 * np.max(np.abs(b["u"])) + 1e-7
