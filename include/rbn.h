@@ -181,6 +181,11 @@ int rbn_get_fields(rbn_ctx *ctx, double *rho, double *T, double *u, double *v);
 /* evolution.dat quantities <rho|u|^2/2>, <T> of the latest hydrovar
  * (Trajectories/code/temppart_dump.c:1739-1757). */
 int rbn_energy(rbn_ctx *ctx, double *mean_energy, double *mean_T);
+/* Bilinear interpolation (bilinear, Trajectories/code/mympi2.h:292-311, periodic ghosts of
+ * communication, :313-348) of one field of the latest hydrovar at n positions, e.g. the
+ * initial particle temperatures init_prtcl writes (:196-201).  field: 0 rho, 1 T, 2 u, 3 v.
+ * Positions must lie in (-1, nx+1) x (-1, ny+1) (RBN_ERANGE otherwise). */
+int rbn_sample(rbn_ctx *ctx, int field, int n, const double *x, const double *y, double *out);
 
 /* ---------------------------------------------------------------- particles (truth run) */
 /* init_prtcl (Trajectories/code/mympi2.h:116-290) with caller-chosen positions.

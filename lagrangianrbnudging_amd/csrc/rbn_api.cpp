@@ -617,6 +617,30 @@ int rbn_energy(rbn_ctx *c, double *mean_energy, double *mean_T) {
   return RBN_OK;
 }
 
+int rbn_sample(rbn_ctx *c, int field, int n, const double *x, const double *y, double *out) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (field < 0 || field > 3 || n < 0 || (n > 0 && (!x || !y || !out)))
+    return fail(c, RBN_EINVAL, "rbn_sample: bad arguments");
+  if (!c->initialised) return fail(c, RBN_ESTATE, "not initialised");
+  if ((field == 0 || field == 1) && !c->moments_valid) return fail(c, RBN_ESTATE, "rho/T of the latest step not available");
+  if (n == 0) return RBN_OK;
+  const void *src = field == 0 ? c->rho : field == 1 ? c->T : field == 2 ? c->u[c->cur] : c->v[c->cur];
+  void *buf = nullptr;
+  if ((rc = dalloc(c, &buf, 3 * (size_t)n * sizeof(double)))) return rc;
+  double *dx = (double *)buf, *dy = dx + n, *dout = dy + n;
+  rc = RBN_OK;
+  if (hipMemcpyAsync(dx, x, (size_t)n * sizeof(double), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+      hipMemcpyAsync(dy, y, (size_t)n * sizeof(double), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+      rbn::launch_sample(c->stream, c->prec, c->geom, src, n, dx, dy, dout, c->d_err) != hipSuccess ||
+      hipMemcpyAsync(out, dout, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess)
+    rc = fail(c, RBN_EDEVICE, "rbn_sample: device operation failed");
+  (void)hipFree(buf);
+  if (rc) return rc;
+  return check_err_flag(c);
+}
+
 int rbn_init_particles(rbn_ctx *c, int n, const int *id, const double *x, const double *y,
                        const double *tau_p, const double *beta_p, const double *up, const double *vp) {
   int rc = check_ctx(c);

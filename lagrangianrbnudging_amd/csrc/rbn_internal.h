@@ -94,6 +94,8 @@ hipError_t launch_init_eq(hipStream_t s, int prec, const Geom &g, const void *rh
 hipError_t launch_particles(hipStream_t s, int prec, const Geom &g, const void *u, const void *v,
                             const void *up, const void *vp, const void *T, const PartPtrs &pp,
                             int need_deriv, int record, int full);
+hipError_t launch_sample(hipStream_t s, int prec, const Geom &g, const void *field, int n,
+                         const double *x, const double *y, double *out, int *err);
 hipError_t launch_particle_init(hipStream_t s, int prec, const Geom &g, const void *u,
                                 const void *v, const PartPtrs &pp, double heavy_factor);
 hipError_t launch_obs_update(hipStream_t s, int branch, int nx, int ny, int ntempread,

@@ -215,6 +215,16 @@ __global__ void k_particle_init(Field<R> U, Field<R> V, PartPtrs pp, double fact
   if (err) atomicOr(pp.err, 1);
 }
 
+// Bilinear samples of one field at given positions (observation operator / init records).
+template <typename R>
+__global__ void k_sample(Field<R> F, int n, const double *x, const double *y, double *out, int *errp) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  int err = 0;
+  out[p] = interp(F, x[p], y[p], err);
+  if (err) atomicOr(errp, 1);
+}
+
 template <typename R>
 Field<R> mk(const void *a, const Geom &g) {
   return Field<R>{(const R *)a, g.nx, g.ny, g.pitch};
@@ -234,6 +244,17 @@ hipError_t launch_particles(hipStream_t s, int prec, const Geom &g, const void *
   else
     hipLaunchKernelGGL(k_particles<float>, grid, block, 0, s, mk<float>(u, g), mk<float>(v, g),
                        mk<float>(up, g), mk<float>(vp, g), mk<float>(T, g), pp, record, full);
+  return hipGetLastError();
+}
+
+hipError_t launch_sample(hipStream_t s, int prec, const Geom &g, const void *field, int n,
+                         const double *x, const double *y, double *out, int *err) {
+  if (n <= 0) return hipSuccess;
+  dim3 grid((n + kBlock - 1) / kBlock), block(kBlock);
+  if (prec == RBN_FP64)
+    hipLaunchKernelGGL(k_sample<double>, grid, block, 0, s, mk<double>(field, g), n, x, y, out, err);
+  else
+    hipLaunchKernelGGL(k_sample<float>, grid, block, 0, s, mk<float>(field, g), n, x, y, out, err);
   return hipGetLastError();
 }
 

@@ -104,6 +104,7 @@ _SIGS = {
     "rbn_get_populations": (C.c_int, [C.c_void_p, _DP, _DP]),
     "rbn_get_fields": (C.c_int, [C.c_void_p, _DP, _DP, _DP, _DP]),
     "rbn_energy": (C.c_int, [C.c_void_p, _DP, _DP]),
+    "rbn_sample": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _DP, _DP, _DP]),
     "rbn_init_particles": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), _DP, _DP, _DP, _DP, _DP, _DP]),
     "rbn_get_particles": (C.c_int, [C.c_void_p, C.POINTER(Particles)]),
     "rbn_num_particles": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
@@ -271,6 +272,14 @@ class Solver:
         e, t = C.c_double(), C.c_double()
         self._c(self.lib.rbn_energy(self.ctx, C.byref(e), C.byref(t)))
         return e.value, t.value
+
+    def sample(self, field: str, x, y) -> np.ndarray:
+        """Bilinear samples of rho/T/u/v of the latest step at positions (x, y)."""
+        k = {"rho": 0, "T": 1, "u": 2, "v": 3}[field]
+        x, y = _f64(x), _f64(y)
+        out = np.empty(len(x))
+        self._c(self.lib.rbn_sample(self.ctx, k, len(x), _dp(x), _dp(y), _dp(out)))
+        return out
 
     # -- particles
     def init_particles(self, ids, x, y, tau_p=None, beta_p=None, up=None, vp=None) -> None:

@@ -1,0 +1,391 @@
+"""Closed-form pins of the CPU oracle (the checker of the GPU path).
+
+Each test fixes a property the mathematics of the scheme determines, chosen so that a
+plausible restatement mistake (a dropped term, a wrong sign, index or streaming direction,
+a wrong wall map, a reversed fold order) fails it.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from lagrangianrbnudging_amd import rbn
+from oracle.oracle import Oracle, equilibrium
+
+W = np.array([16, 4, 4, 4, 4, 1, 1, 1, 1]) / 36.0
+CX = np.array([0, 1, 0, -1, 0, 1, -1, -1, 1])
+CY = np.array([0, 0, 1, 0, -1, 1, 1, -1, -1])
+CS2 = 1.0 / 3.0
+
+
+def params(nx=32, ny=16, **kw):
+    base = dict(nx=nx, ny=ny, Tup=0.0, Tdown=0.0, gravity=0.0, relax=0.8, relaxg=0.7,
+                noutconfig=10 ** 6, nout=10 ** 6, particle=1, prtcl_writefull=0, scratch=1)
+    base.update(kw)
+    return rbn.default_params(**base)
+
+
+# ------------------------------------------------------------------ equilibrium
+
+
+@pytest.mark.parametrize("seed", range(5))
+def test_equilibrium_moments(seed):
+    rng = np.random.default_rng(seed)
+    r, u, v = rng.uniform(0.5, 2), rng.uniform(-0.2, 0.2), rng.uniform(-0.2, 0.2)
+    q = equilibrium(r, u, v)
+    tol = 4e-16 * r
+    assert abs(q.sum() - r) <= tol
+    assert abs((CX * q).sum() - r * u) <= tol
+    assert abs((CY * q).sum() - r * v) <= tol
+    assert abs((CX * CX * q).sum() - r * (CS2 + u * u)) <= tol
+    assert abs((CY * CY * q).sum() - r * (CS2 + v * v)) <= tol
+    assert abs((CX * CY * q).sum() - r * u * v) <= tol
+
+
+def test_rest_state_is_a_fixed_point():
+    p = params(gravity=-1e-3)
+    o = Oracle(p)
+    o.init_fields(T=np.zeros((p.nx, p.ny)))
+    o.step(40)
+    f, g = o.populations()
+    # f*(1-w) + w*feq may round an ulp away from w_k, but nothing may drift or flow
+    assert np.abs(f - W).max() <= 4e-16
+    assert np.array_equal(g, np.zeros_like(g))
+    assert np.abs(o.fields()["u"]).max() < 1e-16 and np.abs(o.fields()["v"]).max() < 1e-16
+
+
+# ------------------------------------------------------------------ streaming, walls, force
+
+
+def test_mass_conservation_with_walls_and_buoyancy():
+    p = params(48, 24, Tup=0.0, Tdown=1.0, gravity=-5e-5, relax=0.56, relaxg=0.56)
+    o = Oracle(p)
+    rng = np.random.default_rng(0)
+    T = np.tile(1 - np.arange(p.ny) / (p.ny - 1), (p.nx, 1)) + rng.uniform(-0.05, 0.05, (p.nx, p.ny))
+    o.init_fields(T=T)
+    m0 = o.populations()[0].sum()
+    o.step(300)
+    f = o.populations()[0]
+    assert abs(f.sum() - m0) <= 1e-12 * m0
+    # flow actually developed (buoyancy couples in)
+    assert np.abs(o.fields()["v"]).max() > 1e-6
+
+
+def test_guo_force_adds_exactly_F_of_momentum_per_step_in_the_bulk():
+    # Uniform T0 at rest under gravity: in the bulk, Sum c_y f grows by F_y = -gravity*rho*T0
+    # every step and hydrovar reports v = (n - 1/2) F_y / rho after n steps.
+    g, T0 = -2e-4, 0.75
+    p = params(16, 64, Tup=T0, Tdown=T0, gravity=g)
+    o = Oracle(p)
+    o.init_fields(T=np.full((p.nx, p.ny), T0))
+    Fy = -g * 1.0 * T0
+    for n in range(1, 6):
+        o.step(1)
+        f = o.populations()[0]
+        mid = f[:, 20:44, :]
+        # exact up to the rounding of O(0.1) populations (a dropped F/2 would be ~1e-4 off)
+        assert np.allclose((CY * mid).sum(-1), n * Fy, rtol=0, atol=2e-17 * n)
+        assert np.allclose((CX * mid).sum(-1), 0.0, rtol=0, atol=1e-17)
+        assert np.allclose(mid.sum(-1), 1.0, rtol=0, atol=4e-16)
+        v = o.fields()["v"][:, 20:44]
+        assert np.allclose(v, (n - 0.5) * Fy, rtol=0, atol=4e-17 * n)
+
+
+def _numpy_truth_step(f, g, rho_p, T_p, p):
+    """One truth-run step written independently with array shifts (out-of-place pull)."""
+    nx, ny = p.nx, p.ny
+    fs, gs = np.empty_like(f), np.empty_like(g)
+    for k in range(9):
+        fs[:, :, k] = np.roll(np.roll(f[:, :, k], CX[k], axis=0), CY[k], axis=1)
+        gs[:, :, k] = np.roll(np.roll(g[:, :, k], CX[k], axis=0), CY[k], axis=1)
+    # bounce-back from the node's own pre-stream values
+    for a, b in ((2, 4), (5, 7), (6, 8)):
+        fs[:, 0, a] = f[:, 0, b]
+        fs[:, ny - 1, b] = f[:, ny - 1, a]
+    # plate temperatures from the post-stream opposite populations
+    for a, b in ((4, 2), (7, 5), (8, 6)):
+        gs[:, ny - 1, a] = -gs[:, ny - 1, b] + 2.0 * W[b] * p.Tup
+    for a, b in ((2, 4), (5, 7), (6, 8)):
+        gs[:, 0, a] = -gs[:, 0, b] + 2.0 * W[b] * p.Tdown
+    Fy = -p.gravity * rho_p * T_p
+    rho = fs.sum(-1)
+    T = gs.sum(-1)
+    u = (fs * CX).sum(-1) / rho
+    v = (fs * CY).sum(-1) / rho + 0.5 * Fy / rho
+    om, omg = 1 / p.relax, 1 / p.relaxg
+    usq = u * u + v * v
+    cu = CX * u[..., None] + CY * v[..., None]
+    bracket = 1 + cu / CS2 + cu ** 2 / (2 * CS2 ** 2) - usq[..., None] / (2 * CS2)
+    feq = W * rho[..., None] * bracket
+    geq = W * T[..., None] * bracket
+    fac = ((CX - u[..., None]) * 0 + (CY - v[..., None]) * Fy[..., None]) / CS2 \
+        + cu * (CY * Fy[..., None]) / CS2 ** 2
+    f2 = fs * (1 - om) + om * feq + (1 - 0.5 * om) * W * fac
+    g2 = gs * (1 - omg) + omg * geq
+    return f2, g2, rho, T, u, v
+
+
+def test_oracle_step_matches_an_independent_array_restatement():
+    p = params(20, 12, Tup=0.1, Tdown=0.9, gravity=-3e-4, relax=0.61, relaxg=0.57)
+    rng = np.random.default_rng(4)
+    f = W * (1 + 0.1 * rng.uniform(-1, 1, (p.nx, p.ny, 9)))
+    g = W * (0.5 + 0.2 * rng.uniform(-1, 1, (p.nx, p.ny, 9)))
+    rho_p = 1 + 0.01 * rng.uniform(-1, 1, (p.nx, p.ny))
+    T_p = 0.5 + 0.1 * rng.uniform(-1, 1, (p.nx, p.ny))
+    o = Oracle(p)
+    o.set_populations(f, g, rho_p, T_p)
+    for _ in range(3):
+        f, g, rho, T, u, v = _numpy_truth_step(f, g, rho_p, T_p, p)
+        rho_p, T_p = rho, T
+        o.step(1)
+        fo, go = o.populations()
+        fl = o.fields()
+        assert np.allclose(fo, f, rtol=0, atol=2e-15)
+        assert np.allclose(go, g, rtol=0, atol=2e-15)
+        for k, a in (("rho", rho), ("T", T), ("u", u), ("v", v)):
+            assert np.allclose(fl[k], a, rtol=0, atol=2e-15), k
+
+
+def test_conduction_steady_state_is_linear():
+    # No buoyancy: pure diffusion between the plates relaxes to the linear profile with the
+    # plate temperatures on the wall rows, Tdown + (Tup-Tdown)(j-1)/(ny-1) (the conduction
+    # profile of the reference's commented-out initialisation; media() uses L = ny-1).
+    p = params(8, 16, Tup=0.0, Tdown=1.0, gravity=0.0, relaxg=1.0)
+    o = Oracle(p)
+    o.init_fields(T=np.full((p.nx, p.ny), 0.5))
+    o.step(6000)
+    T = o.fields()["T"]
+    assert np.ptp(T, axis=0).max() < 1e-13  # x independent
+    j = np.arange(1, p.ny + 1)
+    assert np.abs(T[0] - (p.Tdown + (p.Tup - p.Tdown) * (j - 1) / (p.ny - 1))).max() < 1e-12
+
+
+def test_adiabatic_walls_keep_a_uniform_temperature_uniform():
+    p = params(16, 12, variant=1, ntempread=4, alpha=0.0, Tup=0.3, Tdown=0.3)
+    o = Oracle(p)
+    o.init_fields(T=np.full((p.nx, p.ny), 0.3))
+    ids = np.arange(2, dtype=np.int32)
+    o.init_observers(ids, np.array([3.0, 9.0]), np.array([3.0, 9.0]), np.array([0.3, 0.3]))
+    o.set_observation_source({s: {"id": ids, "x": np.array([3.0, 9.0]), "y": np.array([3.0, 9.0]),
+                                  "temp": np.array([0.3, 0.3])} for s in range(4, 200, 4)})
+    o.step(100)
+    T = o.fields()["T"]
+    assert np.abs(T - 0.3).max() < 1e-15
+
+
+# ------------------------------------------------------------------ bilinear
+
+
+def test_bilinear_is_exact_on_bilinear_fields_and_wraps_periodically():
+    p = params(24, 12)
+    o = Oracle(p)
+    i = np.arange(1, p.nx + 1)[:, None]
+    j = np.arange(1, p.ny + 1)[None, :]
+    a, b, c, d = 0.3, -0.02, 0.05, 0.004
+    u = a + b * i + c * j + d * i * j
+    o.init_fields(T=np.zeros((p.nx, p.ny)), u=u)
+    rng = np.random.default_rng(1)
+    for x, y in zip(rng.uniform(1, p.nx, 50), rng.uniform(1, p.ny, 50)):
+        assert abs(o.bilinear("u", x, y) - (a + b * x + c * y + d * x * y)) < 1e-14
+    # x in (0,1): the left corner is ghost column 0 = column nx; y in (0,1): row 0 = row ny
+    for x, y in ((0.25, 5.5), (3.5, 0.4), (0.7, 0.6)):
+        x1, y1 = int(x), int(y)
+        gi = lambda q: p.nx if q == 0 else q  # noqa: E731
+        gj = lambda q: p.ny if q == 0 else q  # noqa: E731
+        val = lambda q, r: a + b * gi(q) + c * gj(r) + d * gi(q) * gj(r)  # noqa: E731
+        expect = (val(x1, y1) * (x1 + 1 - x) * (y1 + 1 - y) + val(x1 + 1, y1) * (x - x1) * (y1 + 1 - y)
+                  + val(x1, y1 + 1) * (x1 + 1 - x) * (y - y1) + val(x1 + 1, y1 + 1) * (x - x1) * (y - y1))
+        assert abs(o.bilinear("u", x, y) - expect) < 1e-14
+    # C truncation for x in (-1,0): corners 0 and 1, weights extrapolate
+    x, y = -0.3, 4.0
+    expect = val(0, 4) * (1 - x) + val(1, 4) * (x - 0)
+    assert abs(o.bilinear("u", x, y) - expect) < 1e-14
+    with pytest.raises(Exception):
+        o.bilinear("u", -1.5, 4.0)
+
+
+# ------------------------------------------------------------------ particles
+
+
+def _uniform_flow(U0, nx=32, ny=64, **kw):
+    p = params(nx, ny, **kw)
+    o = Oracle(p)
+    o.init_fields(T=np.zeros((nx, ny)), u=np.full((nx, ny), U0))
+    return p, o
+
+
+def test_tracer_ab2_in_uniform_flow_and_periodic_wrap():
+    U0 = 0.01
+    p, o = _uniform_flow(U0)
+    x0 = np.array([5.3, 17.1, p.nx - 0.05])
+    y0 = np.array([30.0, 33.5, 31.0])
+    o.init_particles(np.arange(3, dtype=np.int32), x0, y0, tau_p=np.zeros(3))
+    for n in range(1, 11):
+        o.step(1)
+        st = o.particles()
+        expect = np.mod(x0 + n * U0, p.nx)
+        assert np.allclose(st["x"], expect, rtol=0, atol=1e-13)
+        assert np.allclose(st["y"], y0, rtol=0, atol=1e-15)
+    assert st["x"][2] < 1.0  # crossed x = nx and wrapped
+
+
+def test_maxey_riley_relaxes_geometrically_in_uniform_flow():
+    U0, tau, beta = 0.02, 5.0, 0.3
+    p, o = _uniform_flow(U0)
+    o.init_particles(np.array([0], np.int32), np.array([8.0]), np.array([32.0]), tau_p=np.array([tau]),
+                     beta_p=np.array([beta]), up=np.array([0.0]), vp=np.array([0.0]))
+    x = 8.0
+    for n in range(1, 16):
+        o.step(1)
+        up = U0 + (0.0 - U0) * (1 - 1 / tau) ** n
+        x += up
+        st = o.particles()
+        assert abs(st["u"][0] - up) < 1e-15
+        assert abs(st["x"][0] - x) < 1e-13
+
+
+def test_reflecting_walls():
+    p, o = _uniform_flow(0.0, 16, 20)
+    ids = np.arange(2, dtype=np.int32)
+    o.init_particles(ids, np.array([4.0, 9.0]), np.array([1.2, 19.8]), tau_p=np.array([1e15, 1e15]),
+                     up=np.array([0.0, 0.0]), vp=np.array([-0.3, 0.25]))
+    o.step(1)
+    st = o.particles()
+    # y = 1.2 - 0.3 = 0.9 -> 0.9 + 2(1 - 0.9); 19.8 + 0.25 = 20.05 -> 20.05 + 2(20 - 20.05)
+    assert abs(st["y"][0] - 1.1) < 1e-13 and st["v"][0] > 0
+    assert abs(st["y"][1] - 19.95) < 1e-13 and st["v"][1] < 0
+    assert st["yo"][0] == 1.2 and st["yo"][1] == 19.8
+
+
+def test_record_sampling_and_energy_in_uniform_flow():
+    U0, T0 = 0.015, 0.25
+    p = params(32, 64, noutconfig=3, Tup=T0, Tdown=T0)
+    o = Oracle(p)
+    o.init_fields(T=np.full((p.nx, p.ny), T0), u=np.full((p.nx, p.ny), U0))
+    o.init_particles(np.array([7, 3], np.int32), np.array([4.0, 10.0]), np.array([30.0, 31.0]),
+                     tau_p=np.zeros(2))
+    o.step(6)
+    recs = o.records
+    assert [r["step"] for r in recs] == [3, 6]
+    r = recs[-1]
+    assert list(r["id"]) == [3, 7]  # ascending id order
+    assert np.allclose(r["temp"], T0, rtol=0, atol=1e-15)
+    assert np.allclose(r["x"] - r["xo"], U0, rtol=0, atol=1e-14)
+    e, t = o.energy()
+    # walls slow the rows next to them; the bulk mean stays 0.5 U0^2 to the wall-layer share
+    assert abs(t - T0) < 1e-14
+    assert 0.5 * U0 ** 2 * 0.8 < e < 0.5 * U0 ** 2 * 1.0001
+
+
+# ------------------------------------------------------------------ nudging deposit
+
+
+def test_deposit_single_and_overlapping_observers():
+    p = params(32, 16, variant=1, ntempread=4)
+    o = Oracle(p)
+    o.init_fields(T=np.zeros((p.nx, p.ny)))
+    o.deposit_only([10], [8], [0.7])
+    nd = o.nudging()
+    box = np.zeros((p.nx, p.ny), bool)
+    box[10 - 3 - 1:10 + 3, 8 - 3 - 1:8 + 3] = True  # nodes 7..13 x 5..11 (1-based)
+    assert np.array_equal(nd["nudge_vol"] > 0, box)
+    assert np.all(nd["tempreadint"][box] == 0.7) and np.all(nd["tempreadint"][~box] == 0.0)
+    o.deposit_only([10, 12], [8, 9], [0.7, 0.2])
+    nd = o.nudging()
+    both = nd["nudge_vol"] == 2
+    assert both.sum() == 5 * 6
+    assert np.all(nd["tempreadint"][both] == (0.7 * 1.0 + 0.2) / 2.0)
+
+
+def test_deposit_running_mean_follows_observer_order():
+    # three observers on one node, values chosen so that the fold order changes the bits
+    p = params(32, 16, variant=1, ntempread=4)
+    o = Oracle(p)
+    o.init_fields(T=np.zeros((p.nx, p.ny)))
+    T = [0.1, 0.7, 0.3]
+    o.deposit_only([10, 10, 10], [8, 8, 8], T)
+    fwd = 0.0
+    for n, t in enumerate(T):
+        fwd = (fwd * n + t) / (n + 1.0)
+    rev = 0.0
+    for n, t in enumerate(T[::-1]):
+        rev = (rev * n + t) / (n + 1.0)
+    assert fwd != rev
+    assert o.nudging()["tempreadint"][9, 7] == fwd
+
+
+@pytest.mark.parametrize("xi", [0, 3, 29, 30, 31, 32])
+def test_deposit_wrap_windows(xi):
+    nx, ny = 32, 16
+    p = params(nx, ny, variant=1, ntempread=4)
+    o = Oracle(p)
+    o.init_fields(T=np.zeros((nx, ny)))
+    o.deposit_only([xi], [8], [1.0])
+    got = set(np.nonzero(o.nudging()["nudge_vol"][:, 7])[0] + 1)
+    expect = {((xi + d - 1) % nx) + 1 for d in range(-3, 4)}  # the periodic 7-wide window
+    if xi == nx - 2:
+        expect.discard(1)  # the reference's `xi > nx-2` test misses this column
+    assert got == expect
+
+
+def test_deposit_q3_fix_restores_periodicity():
+    nx, ny = 32, 16
+    p = params(nx, ny, variant=1, ntempread=4, fix_q3=1)
+    o = Oracle(p)
+    o.init_fields(T=np.zeros((nx, ny)))
+    o.deposit_only([nx - 2], [8], [1.0])
+    got = set(np.nonzero(o.nudging()["nudge_vol"][:, 7])[0] + 1)
+    assert got == {((nx - 2 + d - 1) % nx) + 1 for d in range(-3, 4)}
+
+
+def test_observer_interpolation_is_linear_in_time_and_snaps():
+    nx, ny = 40, 20
+    p = params(nx, ny, variant=1, ntempread=5, alpha=0.0)
+    o = Oracle(p)
+    o.init_fields(T=np.full((nx, ny), 0.5))
+    ids = np.array([0, 1], np.int32)
+    x0, y0, T0 = np.array([10.0, nx - 0.2]), np.array([5.0, 8.0]), np.array([0.2, 0.4])
+    x5, y5, T5 = np.array([10.7, 0.3]), np.array([5.5, 7.0]), np.array([0.3, 0.1])
+    o.init_observers(ids, x0, y0, T0)
+    o.set_observation_source({5: {"id": ids[::-1], "x": x5[::-1], "y": y5[::-1], "temp": T5[::-1]},
+                              10: {"id": ids, "x": x5, "y": y5, "temp": T5}})
+    dx = np.array([0.7, 0.5]) / 5  # observer 1 crosses x = nx: unwrapped
+    for s in range(1, 5):
+        o.step(1)
+        st = o.observer_state()
+        d = np.abs(st["xint"] - np.mod(x0 + s * dx, nx))
+        assert np.all(np.minimum(d, nx - d) < 1e-13)  # the reference keeps x = nx (wraps on > nx)
+        assert np.all((st["xint"] >= 0) & (st["xint"] <= nx))
+        assert np.allclose(st["yint"], y0 + s * (y5 - y0) / 5, rtol=0, atol=1e-13)
+        assert np.allclose(st["Tint"], T0 + s * (T5 - T0) / 5, rtol=0, atol=1e-14)
+        assert np.array_equal(st["xidx"], np.round(st["xint"]).astype(int))
+    o.step(1)  # istep % ntempread == 0: snap to the record exactly
+    st = o.observer_state()
+    assert np.array_equal(st["xint"], x5) and np.array_equal(st["Tint"], T5)
+
+
+def test_nudging_recurrence_on_a_uniform_state():
+    # Every node nudged toward T_obs = 0.5 (exact running means), u = 0, adiabatic walls.
+    # With S the population sum and Q_n = alpha (T_n - T_obs):
+    #   T_n = S_n - Q_{n-1}/2,   S_{n+1} = (1-w) S_n + w T_n - (1 - w/2) Q_n.
+    nx, ny, alpha, T_init, T_obs = 24, 12, 0.2, 1.0, 0.5
+    p = params(nx, ny, variant=1, ntempread=4, alpha=alpha, relaxg=0.8, Tup=T_init, Tdown=T_init)
+    o = Oracle(p)
+    o.init_fields(T=np.full((nx, ny), T_init))
+    gx, gy = np.meshgrid(np.arange(1, nx + 1, 3), np.arange(1, ny + 1, 3), indexing="ij")
+    ids = np.arange(gx.size, dtype=np.int32)
+    xs, ys = gx.ravel().astype(float), gy.ravel().astype(float)
+    Ts = np.full(ids.size, T_obs)
+    o.init_observers(ids, xs, ys, Ts)
+    o.set_observation_source({s: {"id": ids, "x": xs, "y": ys, "temp": Ts} for s in range(4, 100, 4)})
+    w = 1 / p.relaxg
+    S, Qp = T_init, 0.0
+    for n in range(1, 30):
+        o.step(1)
+        T = S - 0.5 * Qp
+        Q = alpha * (T - T_obs)
+        assert np.allclose(o.fields()["T"], T, rtol=0, atol=1e-14), n
+        assert np.allclose(o.nudging()["nud_term"], -Q, rtol=0, atol=1e-15)
+        S = (1 - w) * S + w * T - (1 - 0.5 * w) * Q
+        Qp = Q
+    assert abs(T - T_obs) < abs(T_init - T_obs)
