@@ -202,7 +202,7 @@ uint64_t splitmix64(uint64_t &s) {
 
 enum { PX, PY, PU, PV, PXO, PYO, PUO, PVO, PINVTAU, PBETA, PNARR };
 
-rbn::PartPtrs part_ptrs(rbn_ctx *c, bool record) {
+rbn::PartPtrs part_ptrs(rbn_ctx *c, double *rec) {
   rbn::PartPtrs pp{};
   const size_t n = (size_t)c->np;
   double *b = c->d_part;
@@ -217,7 +217,7 @@ rbn::PartPtrs part_ptrs(rbn_ctx *c, bool record) {
   pp.vo = b + PVO * n;
   pp.inv_tau = b + PINVTAU * n;
   pp.beta = b + PBETA * n;
-  pp.rec = record ? c->d_rec : nullptr;
+  pp.rec = rec;
   pp.err = c->d_err;
   return pp;
 }
@@ -262,14 +262,14 @@ int check_err_flag(rbn_ctx *c) {
   return RBN_OK;
 }
 
-int deliver_record(rbn_ctx *c, int step) {
+// Hand a copied record slot to the callback and free the slot.
+int deliver_slot(rbn_ctx *c, rbn_ctx::RecSlot &s) {
+  if (s.step < 0) return RBN_OK;
+  const int step = s.step;
+  s.step = -1;
+  HIPCHK(c, hipEventSynchronize(s.copied));
   const size_t n = (size_t)c->np;
-  const size_t narr = c->P.prtcl_writefull ? 17 : 9;
-  HIPCHK(c, hipMemcpyAsync(c->h_rec, c->d_rec, narr * n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  int rc = check_err_flag(c);
-  if (rc) return rc;
-  const double *r = c->h_rec;
+  const double *r = s.h;
   rbn_record rec{};
   rec.step = step;
   rec.n = c->np;
@@ -283,6 +283,48 @@ int deliver_record(rbn_ctx *c, int step) {
   if (c->rec_fn && c->rec_fn(c->rec_user, &rec) != 0)
     return fail(c, RBN_ECALLBACK, "record callback failed at step %d", step);
   return RBN_OK;
+}
+
+// Deliver every pending record, oldest first.
+int deliver_all(rbn_ctx *c) {
+  for (int k = 0; k < rbn_ctx::kRecSlots; k++) {
+    rbn_ctx::RecSlot &s = c->rec[(c->rec_next + k) % rbn_ctx::kRecSlots];
+    int rc = deliver_slot(c, s);
+    if (rc) return rc;
+  }
+  return RBN_OK;
+}
+
+// The slot K2 writes this record step: deliver its previous content first (the oldest
+// pending record), which also guarantees its copy has finished before it is overwritten.
+int claim_record_slot(rbn_ctx *c, rbn_ctx::RecSlot **out) {
+  rbn_ctx::RecSlot &s = c->rec[c->rec_next];
+  c->rec_next = (c->rec_next + 1) % rbn_ctx::kRecSlots;
+  int rc = deliver_slot(c, s);
+  if (rc) return rc;
+  *out = &s;
+  return RBN_OK;
+}
+
+// Queue the copy of a freshly written slot on the copy stream.
+int queue_record_copy(rbn_ctx *c, rbn_ctx::RecSlot &s, int step) {
+  const size_t narr = c->P.prtcl_writefull ? 17 : 9;
+  HIPCHK(c, hipEventRecord(s.written, c->stream));
+  HIPCHK(c, hipStreamWaitEvent(c->copy_stream, s.written, 0));
+  HIPCHK(c, hipMemcpyAsync(s.h, s.d, narr * (size_t)c->np * sizeof(double), hipMemcpyDeviceToHost, c->copy_stream));
+  HIPCHK(c, hipEventRecord(s.copied, c->copy_stream));
+  s.step = step;
+  return RBN_OK;
+}
+
+void free_record_slots(rbn_ctx *c) {
+  for (auto &s : c->rec) {
+    if (s.d) (void)hipFree(s.d);
+    if (s.h) (void)hipHostFree(s.h);
+    s.d = nullptr;
+    s.h = nullptr;
+    s.step = -1;
+  }
 }
 
 int run_energy(rbn_ctx *c, double out[2]) {
@@ -364,8 +406,13 @@ void rbn_destroy(rbn_ctx *c) {
   void *ptrs[] = {c->fA, c->fB, c->gA, c->gB, c->F, c->u[0], c->u[1], c->v[0], c->v[1], c->rho, c->T, c->nud};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
+  if (c->copy_stream) (void)hipStreamSynchronize(c->copy_stream);
   dfree_t(c->d_part);
-  dfree_t(c->d_rec);
+  free_record_slots(c);
+  for (auto &s : c->rec) {
+    if (s.written) (void)hipEventDestroy(s.written);
+    if (s.copied) (void)hipEventDestroy(s.copied);
+  }
   dfree_t(c->d_err);
   dfree_t(c->d_obs_i);
   dfree_t(c->d_obs_d);
@@ -377,12 +424,12 @@ void rbn_destroy(rbn_ctx *c) {
   dfree_t(c->dep.blocksum);
   dfree_t(c->dep.Tbar);
   dfree_t(c->dep.overflow);
-  if (c->h_rec) (void)hipHostFree(c->h_rec);
   for (auto &p : c->pending) {
     (void)hipEventDestroy(p.a);
     (void)hipEventDestroy(p.b);
   }
   for (hipEvent_t e : c->event_pool) (void)hipEventDestroy(e);
+  if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -405,9 +452,18 @@ int rbn_create(const rbn_params *p, int device, rbn_ctx **out) {
     rbn_destroy(c);
     return rbn::thread_fail(RBN_EDEVICE, "hipSetDevice failed");
   }
-  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+  // Both streams keep the default priority.
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking) != hipSuccess) {
     rbn_destroy(c);
     return rbn::thread_fail(RBN_EDEVICE, "hipStreamCreate failed");
+  }
+  for (auto &s : c->rec) {
+    if (hipEventCreateWithFlags(&s.written, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&s.copied, hipEventDisableTiming) != hipSuccess) {
+      rbn_destroy(c);
+      return rbn::thread_fail(RBN_EDEVICE, "hipEventCreate failed");
+    }
   }
   Geom &g = c->geom;
   g.nx = p->nx;
@@ -649,10 +705,9 @@ int rbn_init_particles(rbn_ctx *c, int n, const int *id, const double *x, const 
   if (!c->initialised) return fail(c, RBN_ESTATE, "initialise the fields before the particles");
   if ((up == nullptr) != (vp == nullptr)) return fail(c, RBN_EINVAL, "give both up and vp or neither");
   if (n > (1 << 28)) return fail(c, RBN_ERANGE, "too many particles");
+  HIPCHK(c, hipStreamSynchronize(c->copy_stream));
   dfree_t(c->d_part);
-  dfree_t(c->d_rec);
-  if (c->h_rec) (void)hipHostFree(c->h_rec);
-  c->h_rec = nullptr;
+  free_record_slots(c);
   c->np = n;
   c->pid.clear();
   c->any_heavy = false;
@@ -684,17 +739,19 @@ int rbn_init_particles(rbn_ctx *c, int n, const int *id, const double *x, const 
   void *tmp = nullptr;
   if ((rc = dalloc(c, &tmp, PNARR * N * sizeof(double)))) return rc;
   c->d_part = (double *)tmp;
-  if ((rc = dalloc(c, &tmp, 17 * N * sizeof(double)))) return rc;
-  c->d_rec = (double *)tmp;
-  HIPCHK(c, hipMemsetAsync(c->d_rec, 0, 17 * N * sizeof(double), c->stream));
-  if (hipHostMalloc((void **)&c->h_rec, 17 * N * sizeof(double), hipHostMallocDefault) != hipSuccess) {
-    c->h_rec = nullptr;
-    return fail(c, RBN_ENOMEM, "pinned record buffer allocation failed");
+  for (auto &s : c->rec) {
+    if ((rc = dalloc(c, &tmp, 17 * N * sizeof(double)))) return rc;
+    s.d = (double *)tmp;
+    HIPCHK(c, hipMemsetAsync(s.d, 0, 17 * N * sizeof(double), c->stream));
+    if (hipHostMalloc((void **)&s.h, 17 * N * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+      s.h = nullptr;
+      return fail(c, RBN_ENOMEM, "pinned record buffer allocation failed");
+    }
   }
   HIPCHK(c, hipMemcpyAsync(c->d_part, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
   if (!up) {
     hipError_t e = rbn::launch_particle_init(c->stream, c->prec, c->geom, c->u[c->cur], c->v[c->cur],
-                                             part_ptrs(c, false), c->P.heavy_init_factor);
+                                             part_ptrs(c, nullptr), c->P.heavy_init_factor);
     if (e != hipSuccess) return fail(c, RBN_EDEVICE, "particle init launch failed: %s", hipGetErrorString(e));
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -853,14 +910,16 @@ int rbn_step(rbn_ctx *c, int nsteps) {
     c->moments_valid = moments;
     if (parts) {
       const int full = record && P.prtcl_writefull;
-      rbn::PartPtrs pp = part_ptrs(c, record);
+      rbn_ctx::RecSlot *slot = nullptr;
+      if (record && (rc = claim_record_slot(c, &slot))) return rc;
+      rbn::PartPtrs pp = part_ptrs(c, slot ? slot->d : nullptr);
       if ((rc = timed(c, 1, [&] {
              return rbn::launch_particles(c->stream, c->prec, c->geom, c->u[c->cur], c->v[c->cur],
                                           c->u[c->cur ^ 1], c->v[c->cur ^ 1], c->T, pp,
                                           c->any_heavy || full, record, full);
            })))
         return rc;
-      if (record && c->rec_fn && (rc = deliver_record(c, istep))) return rc;
+      if (slot && c->rec_fn && (rc = queue_record_copy(c, *slot, istep))) return rc;
     }
     if (diag && c->diag_fn) {
       double r[2];
@@ -870,8 +929,12 @@ int rbn_step(rbn_ctx *c, int nsteps) {
     }
   }
   if (c->timing && (rc = drain_timing(c))) return rc;
-  if (parts || nudged) return check_err_flag(c);
-  return RBN_OK;
+  if (parts || nudged) {
+    // A particle that left the lattice invalidates the records of this call: report it
+    // before delivering them.
+    if ((rc = check_err_flag(c))) return rc;
+  }
+  return deliver_all(c);
 }
 
 int rbn_get_step(const rbn_ctx *c, int *istep) {

@@ -135,8 +135,18 @@ struct rbn_ctx {
   std::vector<int> pid;  // ids in ascending order
   double *d_part = nullptr;  // 10 arrays of np: x,y,u,v,xo,yo,uo,vo,inv_tau,beta
   bool any_heavy = false;
-  double *d_rec = nullptr;   // 17 * np
-  double *h_rec = nullptr;   // pinned mirror
+  // Records: a ring of device/pinned-host slots.  K2 writes a slot on the compute stream,
+  // the copy stream moves it to the host while later steps run, and the callback fires
+  // (in step order) when the slot is reused or rbn_step returns.
+  static constexpr int kRecSlots = 3;
+  struct RecSlot {
+    double *d = nullptr;  // 17 * np
+    double *h = nullptr;  // pinned
+    hipEvent_t written = nullptr, copied = nullptr;
+    int step = -1;        // pending record step, -1 when free
+  } rec[kRecSlots];
+  int rec_next = 0;
+  hipStream_t copy_stream = nullptr;
   int *d_err = nullptr;
   rbn_record_fn rec_fn = nullptr;
   void *rec_user = nullptr;
