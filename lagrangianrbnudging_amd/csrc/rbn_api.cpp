@@ -902,7 +902,9 @@ int rbn_step(rbn_ctx *c, int nsteps) {
       lp.Tbar = c->dep.Tbar;
       lp.nud = c->nud;
     }
-    if ((rc = timed(c, 0, [&] { return rbn::launch_lbm(c->stream, c->prec, P.variant, c->geom, lp, c->consts.data()); })))
+    if ((rc = timed(c, 0, [&] {
+           return rbn::launch_lbm(c->stream, c->prec, P.variant, c->geom, lp, c->consts.data(), c->tune);
+         })))
       return rc;
     std::swap(c->fA, c->fB);
     std::swap(c->gA, c->gB);
@@ -947,6 +949,31 @@ int rbn_synchronize(rbn_ctx *c) {
   int rc = check_ctx(c);
   if (rc) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  return RBN_OK;
+}
+
+int rbn_set_tuning(rbn_ctx *c, const char *key, int value) {
+  if (!c || !key) return rbn::thread_fail(RBN_EINVAL, "null argument");
+  if (!strcmp(key, "lbm_block")) {
+    if (value != 64 && value != 128 && value != 256 && value != 512)
+      return fail(c, RBN_EINVAL, "lbm_block must be 64, 128, 256 or 512");
+    c->tune.block = value;
+  } else if (!strcmp(key, "lbm_nt")) {
+    c->tune.nt = value != 0;
+  } else if (!strcmp(key, "lbm_xcd")) {
+    c->tune.xcd = value != 0;
+  } else {
+    return fail(c, RBN_EINVAL, "unknown tuning key %s", key);
+  }
+  return RBN_OK;
+}
+
+int rbn_get_tuning(const rbn_ctx *c, const char *key, int *value) {
+  if (!c || !key || !value) return rbn::thread_fail(RBN_EINVAL, "null argument");
+  if (!strcmp(key, "lbm_block")) *value = c->tune.block;
+  else if (!strcmp(key, "lbm_nt")) *value = c->tune.nt;
+  else if (!strcmp(key, "lbm_xcd")) *value = c->tune.xcd;
+  else return rbn::thread_fail(RBN_EINVAL, "unknown tuning key");
   return RBN_OK;
 }
 

@@ -85,9 +85,17 @@ struct DepPtrs {
   int *overflow;
 };
 
+// Launch shape of the fused step: block width (64/128/256/512 nodes), non-temporal stores of
+// the outputs, XCD-banded block order.
+struct LbmTuning {
+  int block = 256;
+  int nt = 0;
+  int xcd = 0;
+};
+
 // Launchers (defined in the .hip files).
 hipError_t launch_lbm(hipStream_t s, int prec, int variant, const Geom &g, const LbmPtrs &p,
-                      const void *step_const);
+                      const void *step_const, const LbmTuning &t);
 hipError_t launch_init_eq(hipStream_t s, int prec, const Geom &g, const void *rho, const void *T,
                           const void *u, const void *v, void *f, void *gp, void *F,
                           const void *step_const);
@@ -115,6 +123,7 @@ struct rbn_ctx {
   int dev = 0;
   hipStream_t stream = nullptr;
   rbn::Geom geom{};
+  rbn::LbmTuning tune{};
   int prec = RBN_FP64;
   size_t rs = 8;  // bytes per real
   std::vector<unsigned char> consts;  // StepConst<R>

@@ -37,7 +37,31 @@ constexpr double CS2 = 1. / 3.;
 constexpr double CS22 = 2. * CS2;
 constexpr double CSSQ = 2. / 9.;
 
-constexpr int kBlock = 256;
+constexpr int kBlock = 256;     // block width of the init kernels
+constexpr int kMaxBlock = 512;  // K1 block widths 64..512 (tuning)
+
+// Node of this thread.  The 1-D grid enumerates row segments of blockDim.x nodes, row
+// major.  With `xcd` set, block L goes to segment (L % 8) * (N/8) + L/8: the hardware deals
+// consecutive blocks round-robin to the 8 XCDs, so this gives every XCD one contiguous band
+// of rows and keeps the column halo of neighbouring segments in the same L2.
+__device__ __forceinline__ bool node_of(const Geom &g, int xcd, int &i, int &j) {
+  const int gx = (g.nx + blockDim.x - 1) / blockDim.x;
+  const int N = gridDim.x;
+  int L = blockIdx.x;
+  if (xcd && (N & 7) == 0) L = (L & 7) * (N >> 3) + (L >> 3);
+  j = L / gx;
+  i = (L - j * gx) * blockDim.x + threadIdx.x;
+  return i < g.nx;
+}
+
+// Store, optionally non-temporal (the outputs are not read again within the step).
+template <int NT, typename T>
+__device__ __forceinline__ void st(T *p, T v) {
+  if (NT)
+    __builtin_nontemporal_store(v, p);
+  else
+    *p = v;
+}
 
 template <typename R>
 struct Planes {
@@ -87,11 +111,10 @@ __device__ __forceinline__ void guo_factor64(double u, double v, double Fy, doub
   fac[8] = tm + d6;
 }
 
-template <int VARIANT>
-__global__ __launch_bounds__(kBlock) void k_lbm64(Geom g, LbmPtrs p, StepConst<double> c) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  const int j = blockIdx.y;
-  if (i >= g.nx) return;
+template <int VARIANT, int NT>
+__global__ __launch_bounds__(kMaxBlock) void k_lbm64(Geom g, LbmPtrs p, StepConst<double> c, int xcd) {
+  int i, j;
+  if (!node_of(g, xcd, i, j)) return;
   const int im = (i == 0) ? g.nx - 1 : i - 1;  // periodic x (pbc)
   const int ip = (i == g.nx - 1) ? 0 : i + 1;
   const bool bot = (j == 0), top = (j == g.ny - 1);
@@ -192,9 +215,9 @@ __global__ __launch_bounds__(kBlock) void k_lbm64(Geom g, LbmPtrs p, StepConst<d
   const double Fy = F[r0 + i];
   const double u = fl[1] - fl[3] + fl[5] - fl[6] - fl[7] + fl[8] + 0.5 * rhoi * Fx;
   const double v = fl[5] + fl[2] + fl[6] - fl[7] - fl[4] - fl[8] + 0.5 * rhoi * Fy;
-  F[r0 + i] = c.g0 * rho * (T - 0.0);
-  ((double *)p.u)[r0 + i] = u;
-  ((double *)p.v)[r0 + i] = v;
+  st<NT>(&F[r0 + i], c.g0 * rho * (T - 0.0));
+  st<NT>(&((double *)p.u)[r0 + i], u);
+  st<NT>(&((double *)p.v)[r0 + i], v);
   if (p.rho) ((double *)p.rho)[r0 + i] = rho;
   if (p.T) ((double *)p.T)[r0 + i] = T;
 
@@ -206,21 +229,21 @@ __global__ __launch_bounds__(kBlock) void k_lbm64(Geom g, LbmPtrs p, StepConst<d
 #pragma unroll
   for (int k = 0; k < 9; k++) {
     const double feq = rho * kW[k] * b[k];
-    fB[(size_t)k * g.plane + r0 + i] = f[k] * c.one_m_om + c.om * feq + c.src[k] * fac[k];
+    st<NT>(&fB[(size_t)k * g.plane + r0 + i], f[k] * c.one_m_om + c.om * feq + c.src[k] * fac[k]);
   }
   if (VARIANT == RBN_VARIANT_NUDGING && nv > 0) {
     const double Q = c.alpha * (T - p.Tbar[r0 + i]);
 #pragma unroll
     for (int k = 0; k < 9; k++) {
       const double geq = T * kW[k] * b[k];
-      gB[(size_t)k * g.plane + r0 + i] = q[k] * c.one_m_omg + c.omg * geq - c.bw[k] * Q;
+      st<NT>(&gB[(size_t)k * g.plane + r0 + i], q[k] * c.one_m_omg + c.omg * geq - c.bw[k] * Q);
     }
     ((double *)p.nud)[r0 + i] = -1. * Q;
   } else {
 #pragma unroll
     for (int k = 0; k < 9; k++) {
       const double geq = T * kW[k] * b[k];
-      gB[(size_t)k * g.plane + r0 + i] = q[k] * c.one_m_omg + c.omg * geq;
+      st<NT>(&gB[(size_t)k * g.plane + r0 + i], q[k] * c.one_m_omg + c.omg * geq);
     }
   }
 }
@@ -247,11 +270,10 @@ __device__ __forceinline__ void eq_dev32(float u, float v, float A[9]) {
   A[8] = sumsq2 + ui - vi - uv;
 }
 
-template <int VARIANT>
-__global__ __launch_bounds__(kBlock) void k_lbm32(Geom g, LbmPtrs p, StepConst<float> c) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  const int j = blockIdx.y;
-  if (i >= g.nx) return;
+template <int VARIANT, int NT>
+__global__ __launch_bounds__(kMaxBlock) void k_lbm32(Geom g, LbmPtrs p, StepConst<float> c, int xcd) {
+  int i, j;
+  if (!node_of(g, xcd, i, j)) return;
   const int im = (i == 0) ? g.nx - 1 : i - 1;
   const int ip = (i == g.nx - 1) ? 0 : i + 1;
   const bool bot = (j == 0), top = (j == g.ny - 1);
@@ -346,9 +368,9 @@ __global__ __launch_bounds__(kBlock) void k_lbm32(Geom g, LbmPtrs p, StepConst<f
   const float jy = f[2] - f[4] + f[5] + f[6] - f[7] - f[8];
   const float u = jx * rhoi;
   const float v = jy * rhoi + 0.5f * rhoi * Fy;
-  F[r0 + i] = c.g0 * rho * T;
-  ((float *)p.u)[r0 + i] = u;
-  ((float *)p.v)[r0 + i] = v;
+  st<NT>(&F[r0 + i], c.g0 * rho * T);
+  st<NT>(&((float *)p.u)[r0 + i], u);
+  st<NT>(&((float *)p.v)[r0 + i], v);
   if (p.rho) ((float *)p.rho)[r0 + i] = rho;
   if (p.T) ((float *)p.T)[r0 + i] = T;
 
@@ -363,7 +385,7 @@ __global__ __launch_bounds__(kBlock) void k_lbm32(Geom g, LbmPtrs p, StepConst<f
   for (int k = 0; k < 9; k++) {
     const float w = (float)kW[k];
     const float feq = w * (dr + rho * A[k]);
-    fB[(size_t)k * g.plane + r0 + i] = f[k] * c.one_m_om + c.om * feq + c.src[k] * fac[k];
+    st<NT>(&fB[(size_t)k * g.plane + r0 + i], f[k] * c.one_m_om + c.om * feq + c.src[k] * fac[k]);
   }
   float Q = 0.f;
   if (VARIANT == RBN_VARIANT_NUDGING && nv > 0) {
@@ -374,7 +396,7 @@ __global__ __launch_bounds__(kBlock) void k_lbm32(Geom g, LbmPtrs p, StepConst<f
   for (int k = 0; k < 9; k++) {
     const float w = (float)kW[k];
     const float geq = w * (dT + T * A[k]);
-    gB[(size_t)k * g.plane + r0 + i] = q[k] * c.one_m_omg + c.omg * geq - c.bw[k] * Q;
+    st<NT>(&gB[(size_t)k * g.plane + r0 + i], q[k] * c.one_m_omg + c.omg * geq - c.bw[k] * Q);
   }
 }
 
@@ -467,21 +489,31 @@ __global__ void k_energy_final(const double *partial, int nb, double nxny, doubl
 
 }  // namespace
 
+namespace {
+template <int V, int NT>
+void lbm_dispatch(hipStream_t s, int prec, dim3 grid, dim3 block, const Geom &g, const LbmPtrs &p,
+                  const void *step_const, int xcd) {
+  if (prec == RBN_FP64)
+    hipLaunchKernelGGL((k_lbm64<V, NT>), grid, block, 0, s, g, p,
+                       *(const StepConst<double> *)step_const, xcd);
+  else
+    hipLaunchKernelGGL((k_lbm32<V, NT>), grid, block, 0, s, g, p,
+                       *(const StepConst<float> *)step_const, xcd);
+}
+}  // namespace
+
 hipError_t launch_lbm(hipStream_t s, int prec, int variant, const Geom &g, const LbmPtrs &p,
-                      const void *step_const) {
-  dim3 grid((g.nx + kBlock - 1) / kBlock, g.ny), block(kBlock);
-  if (prec == RBN_FP64) {
-    const StepConst<double> &c = *(const StepConst<double> *)step_const;
-    if (variant == RBN_VARIANT_NUDGING)
-      hipLaunchKernelGGL(k_lbm64<RBN_VARIANT_NUDGING>, grid, block, 0, s, g, p, c);
-    else
-      hipLaunchKernelGGL(k_lbm64<RBN_VARIANT_TRAJECTORIES>, grid, block, 0, s, g, p, c);
+                      const void *step_const, const LbmTuning &t) {
+  const int bw = (t.block == 64 || t.block == 128 || t.block == 512) ? t.block : 256;
+  const unsigned gx = (unsigned)((g.nx + bw - 1) / bw);
+  dim3 grid(gx * (unsigned)g.ny), block(bw);
+  const bool nud = variant == RBN_VARIANT_NUDGING;
+  if (t.nt) {
+    if (nud) lbm_dispatch<RBN_VARIANT_NUDGING, 1>(s, prec, grid, block, g, p, step_const, t.xcd);
+    else lbm_dispatch<RBN_VARIANT_TRAJECTORIES, 1>(s, prec, grid, block, g, p, step_const, t.xcd);
   } else {
-    const StepConst<float> &c = *(const StepConst<float> *)step_const;
-    if (variant == RBN_VARIANT_NUDGING)
-      hipLaunchKernelGGL(k_lbm32<RBN_VARIANT_NUDGING>, grid, block, 0, s, g, p, c);
-    else
-      hipLaunchKernelGGL(k_lbm32<RBN_VARIANT_TRAJECTORIES>, grid, block, 0, s, g, p, c);
+    if (nud) lbm_dispatch<RBN_VARIANT_NUDGING, 0>(s, prec, grid, block, g, p, step_const, t.xcd);
+    else lbm_dispatch<RBN_VARIANT_TRAJECTORIES, 0>(s, prec, grid, block, g, p, step_const, t.xcd);
   }
   return hipGetLastError();
 }

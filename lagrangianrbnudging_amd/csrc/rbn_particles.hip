@@ -71,50 +71,86 @@ __device__ __forceinline__ double interp(const Field<R> &F, double xx, double yy
                xx, yy);
 }
 
-// The six Maxey–Riley node fields at halo node (ic, jc) (M:374-411): ghost nodes take the
-// value of the periodic interior node they copy.
+// The Maxey–Riley node fields (M:374-411) at the four corners of an interpolation cell
+// need the velocity at the two corner rows on four columns (centred x differences) and at
+// the rows below/above each corner row on the two corner columns (y differences, one-sided
+// at the walls).  Ghost corners take the values of the periodic interior node they copy,
+// so all indices are wrapped to interior nodes.  The neighbourhood is loaded once per
+// field and shared by the four corners.
+struct CellPatch {
+  int I[2], J[2];     // wrapped corner columns / rows (1-based interior indices)
+  double c[2][4];     // rows J[b], columns I0-1, I0, I1, I1+1
+  double lo[2][2];    // row J[b]-1 (unused at J[b] == 1), columns I0, I1
+  double hi[2][2];    // row J[b]+1 (unused at J[b] == ny), columns I0, I1
+};
+
 template <typename R>
-__device__ __forceinline__ void node_derivs(const Field<R> &U, const Field<R> &V,
-                                            const Field<R> &Up, const Field<R> &Vp, int ic,
-                                            int jc, double d[6]) {
-  const int nx = U.nx, ny = U.ny;
-  const int I = ic == 0 ? nx : (ic == nx + 1 ? 1 : ic);
-  const int J = jc == 0 ? ny : (jc == ny + 1 ? 1 : jc);
-  const double u0 = U.at(I, J), v0 = V.at(I, J);
-  d[0] = u0 - Up.at(I, J);                        // dtu
-  d[1] = v0 - Vp.at(I, J);                        // dtv
-  d[2] = (U.at(I + 1, J) - U.at(I - 1, J)) * 0.5;  // dxu
-  d[3] = (V.at(I + 1, J) - V.at(I - 1, J)) * 0.5;  // dxv
-  if (J == 1) {
-    d[4] = (U.at(I, 2) - u0);
-    d[5] = (V.at(I, 2) - v0);
-  } else if (J == ny) {
-    d[4] = (u0 - U.at(I, ny - 1));
-    d[5] = (v0 - V.at(I, ny - 1));
-  } else {
-    d[4] = (U.at(I, J + 1) - U.at(I, J - 1)) * 0.5;
-    d[5] = (V.at(I, J + 1) - V.at(I, J - 1)) * 0.5;
+__device__ __forceinline__ void load_patch(const Field<R> &F, int x1, int y1, CellPatch &P) {
+  const int nx = F.nx, ny = F.ny;
+  P.I[0] = x1 == 0 ? nx : x1;
+  P.I[1] = x1 + 1 == nx + 1 ? 1 : x1 + 1;
+  P.J[0] = y1 == 0 ? ny : y1;
+  P.J[1] = y1 + 1 == ny + 1 ? 1 : y1 + 1;
+  for (int b = 0; b < 2; b++) {
+    const int J = P.J[b];
+    P.c[b][0] = F.at(P.I[0] - 1, J);
+    P.c[b][1] = F.at(P.I[0], J);
+    P.c[b][2] = F.at(P.I[1], J);
+    P.c[b][3] = F.at(P.I[1] + 1, J);
+  }
+  for (int b = 0; b < 2; b++) {
+    const int J = P.J[b];
+    const int other = 1 - b;
+    for (int a = 0; a < 2; a++) {
+      // reuse the other corner row when it is the neighbour row
+      if (J > 1) P.lo[b][a] = (P.J[other] == J - 1) ? P.c[other][1 + a] : F.at(P.I[a], J - 1);
+      else P.lo[b][a] = 0.0;
+      if (J < ny) P.hi[b][a] = (P.J[other] == J + 1) ? P.c[other][1 + a] : F.at(P.I[a], J + 1);
+      else P.hi[b][a] = 0.0;
+    }
   }
 }
 
-// Interpolated dtu, dtv, dxu, dxv, dyu, dyv at (xx, yy).
+// d/dx and d/dy of the field at corner (a, b) of the patch (M:384-405).
+__device__ __forceinline__ void patch_dxdy(const CellPatch &P, int a, int b, int ny, double &dx,
+                                           double &dy) {
+  const double u0 = P.c[b][1 + a];
+  dx = (P.c[b][2 + a] - P.c[b][a]) * 0.5;
+  const int J = P.J[b];
+  if (J == 1) dy = (P.hi[b][a] - u0);
+  else if (J == ny) dy = (u0 - P.lo[b][a]);
+  else dy = (P.hi[b][a] - P.lo[b][a]) * 0.5;
+}
+
+// U, V and the six Maxey–Riley interpolants at (xx, yy) from one patch load per field.
 template <typename R>
-__device__ __forceinline__ void interp_derivs(const Field<R> &U, const Field<R> &V,
-                                              const Field<R> &Up, const Field<R> &Vp,
-                                              double xx, double yy, double out[6], int &err) {
+__device__ __forceinline__ bool interp_all(const Field<R> &U, const Field<R> &V,
+                                           const Field<R> &Up, const Field<R> &Vp, double xx,
+                                           double yy, double &up, double &vp, double D[6]) {
   int x1, y1;
-  if (!corners(xx, yy, U.nx, U.ny, x1, y1)) {
-    err = 1;
-    for (int q = 0; q < 6; q++) out[q] = 0.0;
-    return;
-  }
-  double d11[6], d12[6], d21[6], d22[6];
-  node_derivs(U, V, Up, Vp, x1, y1, d11);
-  node_derivs(U, V, Up, Vp, x1, y1 + 1, d12);
-  node_derivs(U, V, Up, Vp, x1 + 1, y1, d21);
-  node_derivs(U, V, Up, Vp, x1 + 1, y1 + 1, d22);
+  if (!corners(xx, yy, U.nx, U.ny, x1, y1)) return false;
+  CellPatch pu, pv;
+  load_patch(U, x1, y1, pu);
+  load_patch(V, x1, y1, pv);
+  // corner order of blend(): (x1,y1), (x1,y1+1), (x1+1,y1), (x1+1,y1+1)
+  const int ca[4] = {0, 0, 1, 1}, cb[4] = {0, 1, 0, 1};
+  double n[8][4];  // field q at corner k: u, v, dtu, dtv, dxu, dxv, dyu, dyv
 #pragma unroll
-  for (int q = 0; q < 6; q++) out[q] = blend(d11[q], d12[q], d21[q], d22[q], x1, y1, xx, yy);
+  for (int k = 0; k < 4; k++) {
+    const int a = ca[k], b = cb[k];
+    const double u0 = pu.c[b][1 + a], v0 = pv.c[b][1 + a];
+    n[0][k] = u0;
+    n[1][k] = v0;
+    n[2][k] = u0 - Up.at(pu.I[a], pu.J[b]);
+    n[3][k] = v0 - Vp.at(pu.I[a], pu.J[b]);
+    patch_dxdy(pu, a, b, U.ny, n[4][k], n[6][k]);
+    patch_dxdy(pv, a, b, U.ny, n[5][k], n[7][k]);
+  }
+  up = blend(n[0][0], n[0][1], n[0][2], n[0][3], x1, y1, xx, yy);
+  vp = blend(n[1][0], n[1][1], n[1][2], n[1][3], x1, y1, xx, yy);
+#pragma unroll
+  for (int q = 0; q < 6; q++) D[q] = blend(n[2 + q][0], n[2 + q][1], n[2 + q][2], n[2 + q][3], x1, y1, xx, yy);
+  return true;
 }
 
 template <typename R>
@@ -128,14 +164,18 @@ __global__ __launch_bounds__(kBlock) void k_particles(Field<R> U, Field<R> V, Fi
   const double xx = pp.x[p], yy = pp.y[p];
   const double uo = pp.u[p], vo = pp.v[p];
   double pu = uo, pv = vo;
-  const double up = interp(U, xx, yy, err);
-  const double vp = interp(V, xx, yy, err);
   const double onebytau = pp.inv_tau[p];
   const bool heavy = onebytau > 0.0;
-  double D[6];
+  double D[6] = {0, 0, 0, 0, 0, 0};
+  double up = 0.0, vp = 0.0;
+  if (heavy || (record && full)) {
+    if (!interp_all(U, V, Up, Vp, xx, yy, up, vp, D)) err = 1;
+  } else {
+    up = interp(U, xx, yy, err);
+    vp = interp(V, xx, yy, err);
+  }
   double x, y;
   if (heavy) {
-    interp_derivs(U, V, Up, Vp, xx, yy, D, err);
     const double beta = pp.beta[p];
     pu = pu + delta_t * (beta * (D[0] + up * D[2] + vp * D[4]) - onebytau * (pu - up));
     pv = pv + delta_t * (beta * (D[1] + up * D[3] + vp * D[5]) - onebytau * (pv - vp));
@@ -161,7 +201,6 @@ __global__ __launch_bounds__(kBlock) void k_particles(Field<R> U, Field<R> V, Fi
     r[8 * n + p] = interp(T, x, y, err);
     if (full) {
       // Writefull quantities at (xo, yo) = (xx, yy): the same interpolants as the update.
-      if (!heavy) interp_derivs(U, V, Up, Vp, xx, yy, D, err);
 #pragma unroll
       for (int q = 0; q < 6; q++) r[(9 + q) * n + p] = D[q];
       r[15 * n + p] = up;

@@ -116,6 +116,8 @@ _SIGS = {
     "rbn_step": (C.c_int, [C.c_void_p, C.c_int]),
     "rbn_get_step": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "rbn_synchronize": (C.c_int, [C.c_void_p]),
+    "rbn_set_tuning": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int]),
+    "rbn_get_tuning": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_int)]),
     "rbn_set_kernel_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "rbn_kernel_stats": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, C.POINTER(C.c_longlong), _DP]),
     "rbn_time_steps": (C.c_int, [C.c_void_p, C.c_int, _DP]),
@@ -384,6 +386,19 @@ class Solver:
         ms = C.c_double()
         self._c(self.lib.rbn_time_steps(self.ctx, int(n), C.byref(ms)))
         return ms.value
+
+    def set_tuning(self, **kw) -> None:
+        """Launch shape of the fused step, e.g. set_tuning(lbm_block=512, lbm_nt=1, lbm_xcd=1)."""
+        for k, v in kw.items():
+            self._c(self.lib.rbn_set_tuning(self.ctx, k.encode(), int(v)))
+
+    def tuning(self) -> dict:
+        out = {}
+        for k in ("lbm_block", "lbm_nt", "lbm_xcd"):
+            v = C.c_int()
+            self._c(self.lib.rbn_get_tuning(self.ctx, k.encode(), C.byref(v)))
+            out[k] = v.value
+        return out
 
     def kernel_timing(self, on: bool) -> None:
         self._c(self.lib.rbn_set_kernel_timing(self.ctx, 1 if on else 0))

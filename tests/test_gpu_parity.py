@@ -143,6 +143,28 @@ def test_no_particles_and_energy():
     assert abs(eg - eo) <= 1e-12 * eo and abs(tg - to) <= 1e-12 * abs(to)
 
 
+@pytest.mark.parametrize("prec", [rbn.RBN_FP64, rbn.RBN_FP32])
+def test_launch_tuning_does_not_change_results(prec):
+    p = rb_params(200, 40, 1e5)  # nx not a multiple of any block width: partial segments
+    p.precision = prec
+    T0 = conduction_T(p, 1e-2, seed=13)
+    ref = None
+    for block in (64, 128, 256, 512):
+        for nt in (0, 1):
+            for xcd in (0, 1):
+                s = rbn.Solver(p)
+                s.set_tuning(lbm_block=block, lbm_nt=nt, lbm_xcd=xcd)
+                s.init_fields(T=T0)
+                s.step(30)
+                f, g = s.populations()
+                if ref is None:
+                    ref = (f, g)
+                else:
+                    assert_equal_arrays(f, ref[0], f"f block={block} nt={nt} xcd={xcd}")
+                    assert_equal_arrays(g, ref[1], f"g block={block} nt={nt} xcd={xcd}")
+                s.close()
+
+
 def test_two_contexts_are_independent():
     p = rb_params(32, 16, 1e5)
     a, b = rbn.Solver(p), rbn.Solver(p)
