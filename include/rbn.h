@@ -229,7 +229,7 @@ int rbn_synchronize(rbn_ctx *ctx);
 /* Launch shape of the fused lattice step; results do not depend on it.  No reference
  * counterpart (the reference has no launch configuration).  Keys:
  *   "lbm_block"  nodes per workgroup along x: 64, 128, 256 (default) or 512
- *   "lbm_nt"     1: non-temporal stores of the step's outputs (default 0)
+ *   "lbm_nt"     1: non-temporal stores of the step's outputs (default 1)
  *   "lbm_xcd"    1: give each of the 8 XCDs a contiguous band of rows (default 0) */
 int rbn_set_tuning(rbn_ctx *ctx, const char *key, int value);
 int rbn_get_tuning(const rbn_ctx *ctx, const char *key, int *value);

@@ -129,20 +129,20 @@ int drain_timing(rbn_ctx *c) {
   return RBN_OK;
 }
 
-// Launch through fn(), bracketed with events when kernel timing is on.
+// Launch through fn() on stream s, bracketed with events on s when kernel timing is on.
 template <typename Fn>
-int timed(rbn_ctx *c, int kernel, Fn &&fn) {
+int timed(rbn_ctx *c, int kernel, hipStream_t s, Fn &&fn) {
   hipEvent_t a = nullptr, b = nullptr;
   if (c->timing) {
     a = take_event(c);
     b = take_event(c);
     if (!a || !b) return fail(c, RBN_EDEVICE, "hipEventCreate failed");
-    HIPCHK(c, hipEventRecord(a, c->stream));
+    HIPCHK(c, hipEventRecord(a, s));
   }
   hipError_t e = fn();
   if (e != hipSuccess) return fail(c, RBN_EDEVICE, "%s launch failed: %s", kKernelNames[kernel], hipGetErrorString(e));
   if (c->timing) {
-    HIPCHK(c, hipEventRecord(b, c->stream));
+    HIPCHK(c, hipEventRecord(b, s));
     c->pending.push_back({kernel, a, b});
     if (c->pending.size() > 8192) return drain_timing(c);
   }
@@ -309,7 +309,7 @@ int claim_record_slot(rbn_ctx *c, rbn_ctx::RecSlot **out) {
 // Queue the copy of a freshly written slot on the copy stream.
 int queue_record_copy(rbn_ctx *c, rbn_ctx::RecSlot &s, int step) {
   const size_t narr = c->P.prtcl_writefull ? 17 : 9;
-  HIPCHK(c, hipEventRecord(s.written, c->stream));
+  HIPCHK(c, hipEventRecord(s.written, c->pstream));
   HIPCHK(c, hipStreamWaitEvent(c->copy_stream, s.written, 0));
   HIPCHK(c, hipMemcpyAsync(s.h, s.d, narr * (size_t)c->np * sizeof(double), hipMemcpyDeviceToHost, c->copy_stream));
   HIPCHK(c, hipEventRecord(s.copied, c->copy_stream));
@@ -403,9 +403,18 @@ void rbn_destroy(rbn_ctx *c) {
   if (!c) return;
   (void)hipSetDevice(c->dev);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  void *ptrs[] = {c->fA, c->fB, c->gA, c->gB, c->F, c->u[0], c->u[1], c->v[0], c->v[1], c->rho, c->T, c->nud};
+  if (c->pstream) (void)hipStreamSynchronize(c->pstream);
+  void *ptrs[] = {c->fA, c->fB, c->gA, c->gB, c->F, c->rho, c->T, c->nud};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
+  for (int k = 0; k < rbn_ctx::kVel; k++) {
+    if (c->u[k]) (void)hipFree(c->u[k]);
+    if (c->v[k]) (void)hipFree(c->v[k]);
+  }
+  if (c->ev_lbm) (void)hipEventDestroy(c->ev_lbm);
+  for (hipEvent_t e : c->ev_part)
+    if (e) (void)hipEventDestroy(e);
+  if (c->pstream) (void)hipStreamDestroy(c->pstream);
   if (c->copy_stream) (void)hipStreamSynchronize(c->copy_stream);
   dfree_t(c->d_part);
   free_record_slots(c);
@@ -452,9 +461,13 @@ int rbn_create(const rbn_params *p, int device, rbn_ctx **out) {
     rbn_destroy(c);
     return rbn::thread_fail(RBN_EDEVICE, "hipSetDevice failed");
   }
-  // Both streams keep the default priority.
+  // All streams keep the default priority.
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking) != hipSuccess) {
+      hipStreamCreateWithFlags(&c->pstream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_lbm, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_part[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_part[1], hipEventDisableTiming) != hipSuccess) {
     rbn_destroy(c);
     return rbn::thread_fail(RBN_EDEVICE, "hipStreamCreate failed");
   }
@@ -475,8 +488,9 @@ int rbn_create(const rbn_params *p, int device, rbn_ctx **out) {
     void **p;
     size_t b;
   } allocs[] = {{&c->fA, 9 * fb}, {&c->fB, 9 * fb}, {&c->gA, 9 * fb}, {&c->gB, 9 * fb},
-                {&c->F, fb},      {&c->u[0], fb},  {&c->u[1], fb},  {&c->v[0], fb},
-                {&c->v[1], fb},   {&c->rho, fb},   {&c->T, fb}};
+                {&c->F, fb},      {&c->u[0], fb},  {&c->u[1], fb},  {&c->u[2], fb},
+                {&c->u[3], fb},   {&c->v[0], fb},  {&c->v[1], fb},  {&c->v[2], fb},
+                {&c->v[3], fb},   {&c->rho, fb},   {&c->T, fb}};
   for (auto &a : allocs) {
     if ((rc = dalloc(c, a.p, a.b))) {
       rbn::thread_fail(rc, c->err.c_str());
@@ -562,8 +576,6 @@ int rbn_init_fields(rbn_ctx *c, const double *rho, const double *T, const double
   if ((rc = upload_field(c, c->T, hT.data()))) return rc;
   if ((rc = upload_field(c, c->u[0], hu.data()))) return rc;
   if ((rc = upload_field(c, c->v[0], hv.data()))) return rc;
-  if ((rc = upload_field(c, c->u[1], hu.data()))) return rc;
-  if ((rc = upload_field(c, c->v[1], hv.data()))) return rc;
   hipError_t e = rbn::launch_init_eq(c->stream, c->prec, g, c->rho, c->T, c->u[0], c->v[0], c->fA,
                                      c->gA, c->F, c->consts.data());
   if (e != hipSuccess) return fail(c, RBN_EDEVICE, "init launch failed: %s", hipGetErrorString(e));
@@ -883,9 +895,18 @@ int rbn_step(rbn_ctx *c, int nsteps) {
       if (r == 1 && (rc = load_observation(c, istep + P.ntempread - 1))) return rc;
       const int branch = r == 1 ? 1 : (r == 0 ? 0 : 2);
       rbn::ObsPtrs o = obs_ptrs(c);
-      if ((rc = timed(c, 2, [&] { return rbn::launch_obs_update(c->stream, branch, P.nx, P.ny, P.ntempread, o); })))
+      if ((rc = timed(c, 2, c->stream, [&] { return rbn::launch_obs_update(c->stream, branch, P.nx, P.ny, P.ntempread, o); })))
         return rc;
-      if ((rc = timed(c, 3, [&] { return rbn::launch_deposit(c->stream, c->geom, right, o, c->dep); }))) return rc;
+      if ((rc = timed(c, 3, c->stream, [&] { return rbn::launch_deposit(c->stream, c->geom, right, o, c->dep); })))
+        return rc;
+    }
+    // K1 of step s writes the velocity buffer of step s-3: the particle update of step s-2
+    // (which reads s-2 and s-3) must have finished; on record steps the particle update
+    // also reads T, which this step overwrites when it writes moments.
+    const int nxt = (c->cur + 1) % rbn_ctx::kVel;
+    if (parts) {
+      HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_part[istep % 2], 0));
+      if (moments && c->part_record_pending) HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_part[(istep + 1) % 2], 0));
     }
     rbn::LbmPtrs lp{};
     lp.fA = c->fA;
@@ -893,8 +914,8 @@ int rbn_step(rbn_ctx *c, int nsteps) {
     lp.fB = c->fB;
     lp.gB = c->gB;
     lp.F = c->F;
-    lp.u = c->u[c->cur ^ 1];
-    lp.v = c->v[c->cur ^ 1];
+    lp.u = c->u[nxt];
+    lp.v = c->v[nxt];
     lp.rho = moments ? c->rho : nullptr;
     lp.T = moments ? c->T : nullptr;
     if (nudged) {
@@ -902,25 +923,31 @@ int rbn_step(rbn_ctx *c, int nsteps) {
       lp.Tbar = c->dep.Tbar;
       lp.nud = c->nud;
     }
-    if ((rc = timed(c, 0, [&] {
+    if ((rc = timed(c, 0, c->stream, [&] {
            return rbn::launch_lbm(c->stream, c->prec, P.variant, c->geom, lp, c->consts.data(), c->tune);
          })))
       return rc;
     std::swap(c->fA, c->fB);
     std::swap(c->gA, c->gB);
-    c->cur ^= 1;
+    const int prv = c->cur;
+    c->cur = nxt;
     c->moments_valid = moments;
     if (parts) {
+      // K2 of step s on the particle stream, overlapping K1 of step s+1.
+      HIPCHK(c, hipEventRecord(c->ev_lbm, c->stream));
+      HIPCHK(c, hipStreamWaitEvent(c->pstream, c->ev_lbm, 0));
       const int full = record && P.prtcl_writefull;
       rbn_ctx::RecSlot *slot = nullptr;
       if (record && (rc = claim_record_slot(c, &slot))) return rc;
       rbn::PartPtrs pp = part_ptrs(c, slot ? slot->d : nullptr);
-      if ((rc = timed(c, 1, [&] {
-             return rbn::launch_particles(c->stream, c->prec, c->geom, c->u[c->cur], c->v[c->cur],
-                                          c->u[c->cur ^ 1], c->v[c->cur ^ 1], c->T, pp,
-                                          c->any_heavy || full, record, full);
+      if ((rc = timed(c, 1, c->pstream, [&] {
+             return rbn::launch_particles(c->pstream, c->prec, c->geom, c->u[c->cur], c->v[c->cur],
+                                          c->u[prv], c->v[prv], c->T, pp, c->any_heavy || full, record,
+                                          full);
            })))
         return rc;
+      HIPCHK(c, hipEventRecord(c->ev_part[istep % 2], c->pstream));
+      c->part_record_pending = record;
       if (slot && c->rec_fn && (rc = queue_record_copy(c, *slot, istep))) return rc;
     }
     if (diag && c->diag_fn) {
@@ -930,6 +957,8 @@ int rbn_step(rbn_ctx *c, int nsteps) {
         return fail(c, RBN_ECALLBACK, "diagnostics callback failed at step %d", istep);
     }
   }
+  if (parts) HIPCHK(c, hipStreamSynchronize(c->pstream));
+  c->part_record_pending = false;
   if (c->timing && (rc = drain_timing(c))) return rc;
   if (parts || nudged) {
     // A particle that left the lattice invalidates the records of this call: report it

@@ -86,10 +86,11 @@ struct DepPtrs {
 };
 
 // Launch shape of the fused step: block width (64/128/256/512 nodes), non-temporal stores of
-// the outputs, XCD-banded block order.
+// the outputs, XCD-banded block order.  Defaults: the fastest measured on MI355X at
+// 4096x2048 in both precisions (256 wide, non-temporal stores, hardware block order).
 struct LbmTuning {
   int block = 256;
-  int nt = 0;
+  int nt = 1;
   int xcd = 0;
 };
 
@@ -131,9 +132,17 @@ struct rbn_ctx {
   // lattice state
   void *fA = nullptr, *fB = nullptr, *gA = nullptr, *gB = nullptr;
   void *F = nullptr;
-  void *u[2] = {nullptr, nullptr}, *v[2] = {nullptr, nullptr};
-  int cur = 0;  // u[cur], v[cur]: latest hydrovar velocity; u[cur^1]: the step before
+  // Velocity ring: u[cur] is the latest hydrovar velocity, u[(cur+3)%4] the step before.
+  // Four buffers let the particle update of step s (which reads steps s and s-1) run on
+  // its own stream while the lattice steps s+1 and s+2 write the other two.
+  static constexpr int kVel = 4;
+  void *u[kVel] = {}, *v[kVel] = {};
+  int cur = 0;
   void *rho = nullptr, *T = nullptr;
+  hipStream_t pstream = nullptr;       // particle stream (default priority)
+  hipEvent_t ev_lbm = nullptr;         // K1 of the latest step finished
+  hipEvent_t ev_part[2] = {};          // K2 of step s finished, by s % 2
+  bool part_record_pending = false;    // K2 of the latest step reads T (record step)
   void *nud = nullptr;
   int istep = 0;
   bool initialised = false;

@@ -1,41 +1,11 @@
-// rbn_lbm.hip — K1: the fused D2Q9 double-population step for gfx950, plus the
-// initial-equilibrium kernel and the deterministic energy reduction.
-//
-// One thread owns one lattice node and performs, in registers, everything the reference
-// does in separate sweeps over an array-of-structs lattice:
-//   pull-streaming of f and g (moveplusforcingconstructWW, Trajectories/code/temppart_dump.c:757-922
-//   or Nudging/code/tempvel_nudge.c:677-1032) with periodic x as an index wrap (pbc, :234-250),
-//   the f half-way bounce-back from the node's own pre-stream populations,
-//   the g wall rule (fixed temperatures, or zero gradient in the nudged run),
-//   hydrovar (:957-987; nudged form N:1069-1102),
-//   the lagged buoyancy force (forcingconstructWW, :744-755: F(s+1) = -gravity*rho(s)*T(s) is
-//   stored per node and read back next step),
-//   equilif/equilig (:1097-1180), collisf (BGK + Guo, :1314-1328) and collisg
-//   (plain :1330-1341, nudged N:1917-1945).
-// Populations live in HBM as structure-of-arrays planes [k][j][i] (x fastest); a step reads
-// the A planes and writes the B planes (A-B pattern), so every access is a coalesced row
-// segment shifted by at most one element.  Traffic per node: 18 population loads + 18
-// stores, 1 load + 1 store of the lagged force, 2 stores of (u,v) = 40 reals.
-//
-// FP64 path: every expression keeps the reference's operand order and the library is
-// compiled with -ffp-contract=off, so results equal the CPU restatement bit for bit.  Two
-// value-preserving rewrites reduce the divisions: equilif and equilig share the velocity
-// terms (they are computed from the same u, v), and the Guo factor uses that F_x = 0 and
-// the D2Q9 direction symmetry (e.g. direction 7 gives ((-u)+(-v))*(-F_y) = (u+v)*F_y
-// exactly), so 6 distinct quotients replace 18.  Both change at most the sign of a zero.
-//
-// FP32 path: populations are stored shifted, f_k - w_k*rho0 and g_k - w_k*T0, and the
-// step is written in those deviations (mass and temperature are summed from the small
-// deviations before the reference level is added back), which keeps the round-off of the
-// O(1) reference level out of the O(u^2) physics.
-#include "rbn_internal.h"
+// rbn_lbm.hip — the plain launch of K1 (one thread per node, device code in
+// rbn_lbm_node.h), the initial-equilibrium kernel and the deterministic energy reduction.
+#include "rbn_lbm_node.h"
 
 namespace rbn {
 namespace {
 
-constexpr double CS2 = 1. / 3.;
-constexpr double CS22 = 2. * CS2;
-constexpr double CSSQ = 2. / 9.;
+using namespace lbm;
 
 constexpr int kBlock = 256;     // block width of the init kernels
 constexpr int kMaxBlock = 512;  // K1 block widths 64..512 (tuning)
@@ -54,350 +24,16 @@ __device__ __forceinline__ bool node_of(const Geom &g, int xcd, int &i, int &j) 
   return i < g.nx;
 }
 
-// Store, optionally non-temporal (the outputs are not read again within the step).
-template <int NT, typename T>
-__device__ __forceinline__ void st(T *p, T v) {
-  if (NT)
-    __builtin_nontemporal_store(v, p);
-  else
-    *p = v;
-}
-
-template <typename R>
-struct Planes {
-  const R *__restrict__ a;
-  size_t P;
-  __device__ __forceinline__ R at(int k, size_t off) const { return a[(size_t)k * P + off]; }
-};
-
-// ---------------------------------------------------------------------------- FP64
-
-// Velocity part of the D2Q9 equilibrium, shared by equilif and equilig: b[k] is the
-// bracket of feq_k = r1*w_k*b[k] (Trajectories/code/temppart_dump.c:1110-1135).
-__device__ __forceinline__ void eq_brackets64(double u, double v, double b[9]) {
-  double usq = u * u, vsq = v * v;
-  double sumsq = (usq + vsq) / CS22;
-  double sumsq2 = sumsq * (1. - CS2) / CS2;
-  double u2 = usq / CSSQ, v2 = vsq / CSSQ;
-  double ui = u / CS2, vi = v / CS2;
-  double uv = ui * vi;
-  b[0] = (1. - sumsq);
-  b[1] = (1. - sumsq + u2 + ui);
-  b[2] = (1. - sumsq + v2 + vi);
-  b[3] = (1. - sumsq + u2 - ui);
-  b[4] = (1. - sumsq + v2 - vi);
-  b[5] = (1. + sumsq2 + ui + vi + uv);
-  b[6] = (1. + sumsq2 - ui + vi - uv);
-  b[7] = (1. + sumsq2 - ui - vi + uv);
-  b[8] = (1. + sumsq2 + ui - vi - uv);
-}
-
-// Guo forcing factor of collisf (T:1321-1322) for F_x = 0, grouped by direction class.
-__device__ __forceinline__ void guo_factor64(double u, double v, double Fy, double fac[9]) {
-  const double tz = ((0. - v) * Fy) / CS2;   // c_y = 0   (k = 0, 1, 3)
-  const double tp = ((1. - v) * Fy) / CS2;   // c_y = +1  (k = 2, 5, 6)
-  const double tm = ((-1. - v) * Fy) / CS2;  // c_y = -1  (k = 4, 7, 8)
-  const double d2 = (v * Fy) / (CS2 * CS2);          // k = 2, 4
-  const double d5 = ((u + v) * Fy) / (CS2 * CS2);    // k = 5, 7
-  const double d6 = ((v - u) * Fy) / (CS2 * CS2);    // k = 6, 8
-  fac[0] = tz;
-  fac[1] = tz;
-  fac[3] = tz;
-  fac[2] = tp + d2;
-  fac[4] = tm + d2;
-  fac[5] = tp + d5;
-  fac[6] = tp + d6;
-  fac[7] = tm + d5;
-  fac[8] = tm + d6;
-}
-
 template <int VARIANT, int NT>
 __global__ __launch_bounds__(kMaxBlock) void k_lbm64(Geom g, LbmPtrs p, StepConst<double> c, int xcd) {
   int i, j;
-  if (!node_of(g, xcd, i, j)) return;
-  const int im = (i == 0) ? g.nx - 1 : i - 1;  // periodic x (pbc)
-  const int ip = (i == g.nx - 1) ? 0 : i + 1;
-  const bool bot = (j == 0), top = (j == g.ny - 1);
-  const size_t r0 = (size_t)j * g.pitch;
-  const size_t rm = bot ? r0 : r0 - g.pitch;
-  const size_t rp = top ? r0 : r0 + g.pitch;
-  const Planes<double> fA{(const double *)p.fA, g.plane}, gA{(const double *)p.gA, g.plane};
-
-  double f[9], q[9];
-  // f: pull, with half-way bounce-back at the walls from the node's own pre-stream
-  // populations (T:859-894).
-  f[0] = fA.at(0, r0 + i);
-  f[1] = fA.at(1, r0 + im);
-  f[3] = fA.at(3, r0 + ip);
-  if (!bot) {
-    f[2] = fA.at(2, rm + i);
-    f[5] = fA.at(5, rm + im);
-    f[6] = fA.at(6, rm + ip);
-  } else {
-    f[2] = fA.at(4, r0 + i);
-    f[5] = fA.at(7, r0 + i);
-    f[6] = fA.at(8, r0 + i);
-  }
-  if (!top) {
-    f[4] = fA.at(4, rp + i);
-    f[7] = fA.at(7, rp + ip);
-    f[8] = fA.at(8, rp + im);
-  } else {
-    f[4] = fA.at(2, r0 + i);
-    f[7] = fA.at(5, r0 + i);
-    f[8] = fA.at(6, r0 + i);
-  }
-  if (VARIANT == RBN_VARIANT_TRAJECTORIES) {
-    q[0] = gA.at(0, r0 + i);
-    q[1] = gA.at(1, r0 + im);
-    q[3] = gA.at(3, r0 + ip);
-    if (!bot) {
-      q[2] = gA.at(2, rm + i);
-      q[5] = gA.at(5, rm + im);
-      q[6] = gA.at(6, rm + ip);
-    }
-    if (!top) {
-      q[4] = gA.at(4, rp + i);
-      q[7] = gA.at(7, rp + ip);
-      q[8] = gA.at(8, rp + im);
-    }
-    // Fixed plate temperatures from the post-stream opposite populations (T:896-921).
-    if (top) {
-      q[4] = -q[2] + c.wtop[0];
-      q[7] = -q[5] + c.wtop[1];
-      q[8] = -q[6] + c.wtop[2];
-    }
-    if (bot) {
-      q[2] = -q[4] + c.wbot[0];
-      q[5] = -q[7] + c.wbot[1];
-      q[6] = -q[8] + c.wbot[2];
-    }
-  } else {
-    // Zero-gradient walls (N:684-702, 783-962): wall rows take the pre-stream populations
-    // of their inner neighbour row and are not streamed; the rows next to them pull from
-    // that copy.
-    if (bot || top) {
-      const size_t rs = bot ? r0 + g.pitch : r0 - g.pitch;
-#pragma unroll
-      for (int k = 0; k < 9; k++) q[k] = gA.at(k, rs + i);
-    } else {
-      const size_t rsm = (j == 1) ? r0 : rm;
-      const size_t rsp = (j == g.ny - 2) ? r0 : rp;
-      q[0] = gA.at(0, r0 + i);
-      q[1] = gA.at(1, r0 + im);
-      q[3] = gA.at(3, r0 + ip);
-      q[2] = gA.at(2, rsm + i);
-      q[5] = gA.at(5, rsm + im);
-      q[6] = gA.at(6, rsm + ip);
-      q[4] = gA.at(4, rsp + i);
-      q[7] = gA.at(7, rsp + ip);
-      q[8] = gA.at(8, rsp + im);
-    }
-  }
-
-  // hydrovar
-  double rho = 0.0, T = 0.0;
-#pragma unroll
-  for (int k = 0; k < 9; k++) rho = rho + f[k];
-#pragma unroll
-  for (int k = 0; k < 9; k++) T = T + q[k];
-  int nv = 0;
-  if (VARIANT == RBN_VARIANT_NUDGING && p.nvol) {
-    nv = p.nvol[r0 + i];
-    if (nv > 0) T = (T + 0.5 * ((const double *)p.nud)[r0 + i]);
-  }
-  const double rhoi = 1. / rho;
-  double fl[9];
-#pragma unroll
-  for (int k = 0; k < 9; k++) fl[k] = f[k] * rhoi;
-  double *F = (double *)p.F;
-  const double Fx = 0.0;
-  const double Fy = F[r0 + i];
-  const double u = fl[1] - fl[3] + fl[5] - fl[6] - fl[7] + fl[8] + 0.5 * rhoi * Fx;
-  const double v = fl[5] + fl[2] + fl[6] - fl[7] - fl[4] - fl[8] + 0.5 * rhoi * Fy;
-  st<NT>(&F[r0 + i], c.g0 * rho * (T - 0.0));
-  st<NT>(&((double *)p.u)[r0 + i], u);
-  st<NT>(&((double *)p.v)[r0 + i], v);
-  if (p.rho) ((double *)p.rho)[r0 + i] = rho;
-  if (p.T) ((double *)p.T)[r0 + i] = T;
-
-  // equilibria + collisions
-  double b[9], fac[9];
-  eq_brackets64(u, v, b);
-  guo_factor64(u, v, Fy, fac);
-  double *fB = (double *)p.fB, *gB = (double *)p.gB;
-#pragma unroll
-  for (int k = 0; k < 9; k++) {
-    const double feq = rho * kW[k] * b[k];
-    st<NT>(&fB[(size_t)k * g.plane + r0 + i], f[k] * c.one_m_om + c.om * feq + c.src[k] * fac[k]);
-  }
-  if (VARIANT == RBN_VARIANT_NUDGING && nv > 0) {
-    const double Q = c.alpha * (T - p.Tbar[r0 + i]);
-#pragma unroll
-    for (int k = 0; k < 9; k++) {
-      const double geq = T * kW[k] * b[k];
-      st<NT>(&gB[(size_t)k * g.plane + r0 + i], q[k] * c.one_m_omg + c.omg * geq - c.bw[k] * Q);
-    }
-    ((double *)p.nud)[r0 + i] = -1. * Q;
-  } else {
-#pragma unroll
-    for (int k = 0; k < 9; k++) {
-      const double geq = T * kW[k] * b[k];
-      st<NT>(&gB[(size_t)k * g.plane + r0 + i], q[k] * c.one_m_omg + c.omg * geq);
-    }
-  }
-}
-
-// ---------------------------------------------------------------------------- FP32
-
-__device__ __forceinline__ void eq_dev32(float u, float v, float A[9]) {
-  // A_k = (bracket_k - 1) of the equilibrium, with 1/cs2 = 3, 1/(2cs2) = 1.5,
-  // (1-cs2)/cs2 = 2, 1/cssq = 4.5.
-  float usq = u * u, vsq = v * v;
-  float sumsq = 1.5f * (usq + vsq);
-  float sumsq2 = 2.f * sumsq;
-  float u2 = 4.5f * usq, v2 = 4.5f * vsq;
-  float ui = 3.f * u, vi = 3.f * v;
-  float uv = ui * vi;
-  A[0] = -sumsq;
-  A[1] = -sumsq + u2 + ui;
-  A[2] = -sumsq + v2 + vi;
-  A[3] = -sumsq + u2 - ui;
-  A[4] = -sumsq + v2 - vi;
-  A[5] = sumsq2 + ui + vi + uv;
-  A[6] = sumsq2 - ui + vi - uv;
-  A[7] = sumsq2 - ui - vi + uv;
-  A[8] = sumsq2 + ui - vi - uv;
+  if (node_of(g, xcd, i, j)) lbm_node64<VARIANT, NT>(g, p, c, i, j);
 }
 
 template <int VARIANT, int NT>
 __global__ __launch_bounds__(kMaxBlock) void k_lbm32(Geom g, LbmPtrs p, StepConst<float> c, int xcd) {
   int i, j;
-  if (!node_of(g, xcd, i, j)) return;
-  const int im = (i == 0) ? g.nx - 1 : i - 1;
-  const int ip = (i == g.nx - 1) ? 0 : i + 1;
-  const bool bot = (j == 0), top = (j == g.ny - 1);
-  const size_t r0 = (size_t)j * g.pitch;
-  const size_t rm = bot ? r0 : r0 - g.pitch;
-  const size_t rp = top ? r0 : r0 + g.pitch;
-  const Planes<float> fA{(const float *)p.fA, g.plane}, gA{(const float *)p.gA, g.plane};
-
-  float f[9], q[9];
-  f[0] = fA.at(0, r0 + i);
-  f[1] = fA.at(1, r0 + im);
-  f[3] = fA.at(3, r0 + ip);
-  if (!bot) {
-    f[2] = fA.at(2, rm + i);
-    f[5] = fA.at(5, rm + im);
-    f[6] = fA.at(6, rm + ip);
-  } else {
-    f[2] = fA.at(4, r0 + i);
-    f[5] = fA.at(7, r0 + i);
-    f[6] = fA.at(8, r0 + i);
-  }
-  if (!top) {
-    f[4] = fA.at(4, rp + i);
-    f[7] = fA.at(7, rp + ip);
-    f[8] = fA.at(8, rp + im);
-  } else {
-    f[4] = fA.at(2, r0 + i);
-    f[7] = fA.at(5, r0 + i);
-    f[8] = fA.at(6, r0 + i);
-  }
-  if (VARIANT == RBN_VARIANT_TRAJECTORIES) {
-    q[0] = gA.at(0, r0 + i);
-    q[1] = gA.at(1, r0 + im);
-    q[3] = gA.at(3, r0 + ip);
-    if (!bot) {
-      q[2] = gA.at(2, rm + i);
-      q[5] = gA.at(5, rm + im);
-      q[6] = gA.at(6, rm + ip);
-    }
-    if (!top) {
-      q[4] = gA.at(4, rp + i);
-      q[7] = gA.at(7, rp + ip);
-      q[8] = gA.at(8, rp + im);
-    }
-    // Shifted form of the plate rule: g~_2 = -g~_4 + 2 w_4 (Tdown - T0), etc.
-    if (top) {
-      q[4] = -q[2] + c.wtop[0];
-      q[7] = -q[5] + c.wtop[1];
-      q[8] = -q[6] + c.wtop[2];
-    }
-    if (bot) {
-      q[2] = -q[4] + c.wbot[0];
-      q[5] = -q[7] + c.wbot[1];
-      q[6] = -q[8] + c.wbot[2];
-    }
-  } else {
-    if (bot || top) {
-      const size_t rs = bot ? r0 + g.pitch : r0 - g.pitch;
-#pragma unroll
-      for (int k = 0; k < 9; k++) q[k] = gA.at(k, rs + i);
-    } else {
-      const size_t rsm = (j == 1) ? r0 : rm;
-      const size_t rsp = (j == g.ny - 2) ? r0 : rp;
-      q[0] = gA.at(0, r0 + i);
-      q[1] = gA.at(1, r0 + im);
-      q[3] = gA.at(3, r0 + ip);
-      q[2] = gA.at(2, rsm + i);
-      q[5] = gA.at(5, rsm + im);
-      q[6] = gA.at(6, rsm + ip);
-      q[4] = gA.at(4, rsp + i);
-      q[7] = gA.at(7, rsp + ip);
-      q[8] = gA.at(8, rsp + im);
-    }
-  }
-
-  float dr = 0.f, dT = 0.f;
-#pragma unroll
-  for (int k = 0; k < 9; k++) dr += f[k];
-#pragma unroll
-  for (int k = 0; k < 9; k++) dT += q[k];
-  int nv = 0;
-  if (VARIANT == RBN_VARIANT_NUDGING && p.nvol) {
-    nv = p.nvol[r0 + i];
-    if (nv > 0) dT += 0.5f * ((const float *)p.nud)[r0 + i];
-  }
-  const float rho = c.rho0 + dr;
-  const float T = c.T0 + dT;
-  const float rhoi = 1.f / rho;
-  float *F = (float *)p.F;
-  const float Fy = F[r0 + i];
-  const float jx = f[1] - f[3] + f[5] - f[6] - f[7] + f[8];
-  const float jy = f[2] - f[4] + f[5] + f[6] - f[7] - f[8];
-  const float u = jx * rhoi;
-  const float v = jy * rhoi + 0.5f * rhoi * Fy;
-  st<NT>(&F[r0 + i], c.g0 * rho * T);
-  st<NT>(&((float *)p.u)[r0 + i], u);
-  st<NT>(&((float *)p.v)[r0 + i], v);
-  if (p.rho) ((float *)p.rho)[r0 + i] = rho;
-  if (p.T) ((float *)p.T)[r0 + i] = T;
-
-  float A[9];
-  eq_dev32(u, v, A);
-  // Guo factor with F_x = 0: ((c_y - v) F_y)/cs2 + ((c.u)(c_y F_y))/cs2^2.
-  const float tz = -3.f * v * Fy, tp = 3.f * (1.f - v) * Fy, tm = 3.f * (-1.f - v) * Fy;
-  const float d2 = 9.f * v * Fy, d5 = 9.f * (u + v) * Fy, d6 = 9.f * (v - u) * Fy;
-  const float fac[9] = {tz, tz, tp + d2, tz, tm + d2, tp + d5, tp + d6, tm + d5, tm + d6};
-  float *fB = (float *)p.fB, *gB = (float *)p.gB;
-#pragma unroll
-  for (int k = 0; k < 9; k++) {
-    const float w = (float)kW[k];
-    const float feq = w * (dr + rho * A[k]);
-    st<NT>(&fB[(size_t)k * g.plane + r0 + i], f[k] * c.one_m_om + c.om * feq + c.src[k] * fac[k]);
-  }
-  float Q = 0.f;
-  if (VARIANT == RBN_VARIANT_NUDGING && nv > 0) {
-    Q = c.alpha * (T - (float)p.Tbar[r0 + i]);
-    ((float *)p.nud)[r0 + i] = -Q;
-  }
-#pragma unroll
-  for (int k = 0; k < 9; k++) {
-    const float w = (float)kW[k];
-    const float geq = w * (dT + T * A[k]);
-    st<NT>(&gB[(size_t)k * g.plane + r0 + i], q[k] * c.one_m_omg + c.omg * geq - c.bw[k] * Q);
-  }
+  if (node_of(g, xcd, i, j)) lbm_node32<VARIANT, NT>(g, p, c, i, j);
 }
 
 // ---------------------------------------------------------------------------- init
