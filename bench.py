@@ -96,6 +96,59 @@ def run_gpu(rbn, precision, steps, warmup, timing=True):
     return out
 
 
+ENS_SIZES = [(256, 128), (512, 256), (768, 384), (1024, 512)]
+ENS_MEMBERS = 64
+
+
+def run_ensemble(rbn, steps, warmup):
+    """64 FP32 truth-run members, sizes drawn from ENS_SIZES, Ra log-uniform in [1e6, 1e8],
+    1e3..1e4 tracers each, own seeds; batched ensemble (autotuned, graph-captured) against the
+    same members stepped one after another with their own launches."""
+    rng = np.random.default_rng(7)
+    members, nodes = [], 0
+    for m in range(ENS_MEMBERS):
+        nx, ny = ENS_SIZES[int(rng.integers(len(ENS_SIZES)))]
+        ra = 10.0 ** rng.uniform(6.0, 8.0)
+        p = rbn.default_params(nx=nx, ny=ny, Tup=0.0, Tdown=1.0, scratch=1, nstart=0, nout=10**9,
+                               noutconfig=10**9, prtcl_writefull=0, particle=1,
+                               precision=rbn.RBN_FP32, T0_shift=0.5, rho0_shift=1.0)
+        p = rbn.params_from_ra(p, ra, PR, U_FF)
+        s = rbn.Solver(p, 0)
+        s.init_conduction(1e-3, 100 + m)
+        n = int(rng.integers(1000, 10001))
+        x = rng.uniform(0.0, nx, n)
+        x[x == 0.0] = 0.5
+        y = rng.uniform(1.0, ny, n)
+        s.init_particles(np.arange(n, dtype=np.int32), x, y, tau_p=np.zeros(n))
+        members.append(s)
+        nodes += nx * ny
+    for s in members:
+        s.step(warmup)
+    t0 = time.perf_counter()
+    for s in members:
+        s.step(steps)
+    for s in members:
+        s.synchronize()
+    t_seq = time.perf_counter() - t0
+    e = rbn.Ensemble(members)
+    best_trial = e.autotune(8)
+    e.step(warmup - warmup % 4 + 4)
+    t0 = time.perf_counter()
+    e.step(steps)
+    t_ens = time.perf_counter() - t0
+    out = {"members": ENS_MEMBERS, "nodes": nodes, "steps": steps,
+           "mlups": round(nodes * steps / t_ens / 1e6, 1),
+           "sequential_mlups": round(nodes * steps / t_seq / 1e6, 1),
+           "speedup_vs_sequential": round(t_seq / t_ens, 2),
+           "ms_per_step": round(1e3 * t_ens / steps, 4),
+           "roofline_frac": round(nodes * lbm_bytes_per_node(1) / (t_ens / steps) / 1e9 / HBM_PEAK_GBPS, 4),
+           "tuning": e.tuning(), "autotune_best_ms": round(best_trial, 4)}
+    e.close()
+    for s in members:
+        s.close()
+    return out
+
+
 def run_cpu_baseline(rbn, steps):
     """The CPU FP64 restatement of the reference loop (array-of-structs, per-direction sweeps,
     halo copies) on one host core, same workload, bounded to a few steps."""
@@ -128,6 +181,7 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 companion run")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble run")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build is a single-GPU solver; the multi-GPU slab decomposition is "
@@ -145,6 +199,10 @@ def main() -> int:
     if not args.no_fp64:
         r64 = run_gpu(rbn, rbn.RBN_FP64, max(args.steps // 2, 1), max(args.warmup // 2, 1))
         print(f"bench: fp64 done {r64}", file=sys.stderr, flush=True)
+    ens = None
+    if not args.no_ensemble:
+        ens = run_ensemble(rbn, max(args.steps // 5, 8), 20)
+        print(f"bench: ensemble done {ens}", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
         cpu = run_cpu_baseline(rbn, args.cpu_steps)
@@ -190,6 +248,7 @@ def main() -> int:
             "lbm_avg_ms": round(r64["lbm_avg_ms"], 5),
             "roofline_frac": round(NX * NY * lbm_bytes_per_node(0) / (r64["lbm_avg_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
         },
+        "ensemble": ens,
         "finite": r32["finite"] and (r64 is None or r64["finite"]),
     }
     print(json.dumps(line))

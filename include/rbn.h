@@ -225,12 +225,38 @@ int rbn_get_step(const rbn_ctx *ctx, int *istep);
 /* Wait for all queued GPU work of ctx. */
 int rbn_synchronize(rbn_ctx *ctx);
 
+/* ---------------------------------------------------------------- batched ensemble */
+/* Many independent truth-run contexts of uneven sizes and parameters advanced together: one
+ * ensemble step is a persistent launch over a work list of every member's lattice tiles
+ * followed by one over every member's particle chunks, and four steps are captured in one
+ * hipGraph that is replayed.  No reference counterpart (the reference runs one lattice per
+ * job).  Members must share device and precision, be initialised truth-run contexts, and
+ * stay owned by the caller; results equal stepping each member with rbn_step, except that
+ * no trajectory records or diagnostics are produced during ensemble steps. */
+typedef struct rbn_ensemble rbn_ensemble;
+int rbn_ensemble_create(rbn_ctx *const *members, int n, rbn_ensemble **out);
+void rbn_ensemble_destroy(rbn_ensemble *e);
+const char *rbn_ensemble_last_error(const rbn_ensemble *e);
+/* Advance every member by nsteps (graph replays for whole four-step cycles). */
+int rbn_ensemble_step(rbn_ensemble *e, int nsteps);
+/* Tile shape (tile_w in {64,128,256} nodes, tile_w*tile_h a multiple of 256), persistent
+ * workgroups per CU (0: one workgroup per work item) and work order (0: member by member,
+ * 1: members interleaved, largest first). */
+int rbn_ensemble_set_tuning(rbn_ensemble *e, int tile_w, int tile_h, int wg_per_cu, int order);
+int rbn_ensemble_get_tuning(const rbn_ensemble *e, int *tile_w, int *tile_h, int *wg_per_cu,
+                            int *order);
+/* Time every candidate launch shape on trial_steps real steps, keep the fastest, and restore
+ * every member's state afterwards; *best_ms_per_step receives the winner's time. */
+int rbn_ensemble_autotune(rbn_ensemble *e, int trial_steps, double *best_ms_per_step);
+
 /* ---------------------------------------------------------------- launch tuning */
 /* Launch shape of the fused lattice step; results do not depend on it.  No reference
  * counterpart (the reference has no launch configuration).  Keys:
  *   "lbm_block"  nodes per workgroup along x: 64, 128, 256 (default) or 512
  *   "lbm_nt"     1: non-temporal stores of the step's outputs (default 1)
- *   "lbm_xcd"    1: give each of the 8 XCDs a contiguous band of rows (default 0) */
+ *   "lbm_xcd"    1: give each of the 8 XCDs a contiguous band of rows (default 0)
+ *   "particle_overlap"  1: run the particle update of step s on its own stream concurrently
+ *                with the lattice update of step s+1 (default 0) */
 int rbn_set_tuning(rbn_ctx *ctx, const char *key, int value);
 int rbn_get_tuning(const rbn_ctx *ctx, const char *key, int *value);
 

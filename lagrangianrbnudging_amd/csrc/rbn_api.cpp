@@ -904,9 +904,12 @@ int rbn_step(rbn_ctx *c, int nsteps) {
     // (which reads s-2 and s-3) must have finished; on record steps the particle update
     // also reads T, which this step overwrites when it writes moments.
     const int nxt = (c->cur + 1) % rbn_ctx::kVel;
+    // Without overlap (the default: both kernels are bandwidth bound, and concurrency only
+    // moves the contention) the lattice step also waits for the particle update before it.
     if (parts) {
       HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_part[istep % 2], 0));
-      if (moments && c->part_record_pending) HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_part[(istep + 1) % 2], 0));
+      if (!c->tune.overlap || (moments && c->part_record_pending))
+        HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_part[(istep + 1) % 2], 0));
     }
     rbn::LbmPtrs lp{};
     lp.fA = c->fA;
@@ -991,6 +994,8 @@ int rbn_set_tuning(rbn_ctx *c, const char *key, int value) {
     c->tune.nt = value != 0;
   } else if (!strcmp(key, "lbm_xcd")) {
     c->tune.xcd = value != 0;
+  } else if (!strcmp(key, "particle_overlap")) {
+    c->tune.overlap = value != 0;
   } else {
     return fail(c, RBN_EINVAL, "unknown tuning key %s", key);
   }
@@ -1002,6 +1007,7 @@ int rbn_get_tuning(const rbn_ctx *c, const char *key, int *value) {
   if (!strcmp(key, "lbm_block")) *value = c->tune.block;
   else if (!strcmp(key, "lbm_nt")) *value = c->tune.nt;
   else if (!strcmp(key, "lbm_xcd")) *value = c->tune.xcd;
+  else if (!strcmp(key, "particle_overlap")) *value = c->tune.overlap;
   else return rbn::thread_fail(RBN_EINVAL, "unknown tuning key");
   return RBN_OK;
 }

@@ -1,0 +1,419 @@
+// rbn_ensemble.cpp — host side of the batched ensemble (rbn_ensemble_* in include/rbn.h):
+// member validation, work lists, the four-step hipGraph, the autotuner.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "rbn_ensemble.h"
+#include "rbn_internal.h"
+
+struct rbn_ensemble {
+  int dev = 0, prec = RBN_FP32;
+  std::vector<rbn_ctx *> m;
+  hipStream_t stream = nullptr, pstream = nullptr;
+  hipEvent_t ev_lbm = nullptr, ev_part[2] = {nullptr, nullptr};
+  void *d_members = nullptr;
+  int2 *d_items = nullptr;  // LBM items, then particle items
+  int n_lbm = 0, n_part = 0;
+  int *d_counters = nullptr;  // [2 sets][2 phases]
+  int cus = 1;
+  // tuning
+  int tile_w = 256, tile_h = 1, wg_per_cu = 4, order = 0;
+  bool built = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::string err;
+};
+
+namespace {
+
+int efail(rbn_ensemble *e, int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (e) e->err = buf;
+  rbn::thread_fail(code, buf);
+  return code;
+}
+
+#define EHIP(e, expr)                                                                          \
+  do {                                                                                         \
+    hipError_t e_ = (expr);                                                                    \
+    if (e_ != hipSuccess) return efail((e), RBN_EDEVICE, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+void drop_graph(rbn_ensemble *e) {
+  if (e->exec) (void)hipGraphExecDestroy(e->exec);
+  if (e->graph) (void)hipGraphDestroy(e->graph);
+  e->exec = nullptr;
+  e->graph = nullptr;
+}
+
+template <typename R>
+void fill_member(const rbn_ctx *c, int tile_w, int tile_h, rbn::EnsMember<R> &d) {
+  memset(&d, 0, sizeof d);
+  d.g = c->geom;
+  d.c = *reinterpret_cast<const rbn::StepConst<R> *>(c->consts.data());
+  d.pf[0] = (R *)c->fA;
+  d.pf[1] = (R *)c->fB;
+  d.pg[0] = (R *)c->gA;
+  d.pg[1] = (R *)c->gB;
+  d.F = (R *)c->F;
+  for (int k = 0; k < 4; k++) {
+    d.u[k] = (R *)c->u[k];
+    d.v[k] = (R *)c->v[k];
+  }
+  d.rho = (R *)c->rho;
+  d.T = (R *)c->T;
+  d.cur0 = c->cur;
+  d.tiles_x = (c->geom.nx + tile_w - 1) / tile_w;
+  d.tiles_y = (c->geom.ny + tile_h - 1) / tile_h;
+  d.part = c->d_part;
+  d.np = (c->P.particle && c->d_part) ? c->np : 0;
+  d.err = c->d_err;
+}
+
+// Descriptors and work lists for the current tuning and member state.
+int build(rbn_ensemble *e) {
+  drop_graph(e);
+  const int n = (int)e->m.size();
+  const size_t dsz = e->prec == RBN_FP64 ? sizeof(rbn::EnsMember<double>) : sizeof(rbn::EnsMember<float>);
+  std::vector<unsigned char> desc(dsz * (size_t)n);
+  std::vector<int> tiles(n), chunks(n);
+  for (int i = 0; i < n; i++) {
+    if (e->prec == RBN_FP64) {
+      auto *d = reinterpret_cast<rbn::EnsMember<double> *>(desc.data()) + i;
+      fill_member(e->m[i], e->tile_w, e->tile_h, *d);
+      tiles[i] = d->tiles_x * d->tiles_y;
+      chunks[i] = (d->np + 255) / 256;
+    } else {
+      auto *d = reinterpret_cast<rbn::EnsMember<float> *>(desc.data()) + i;
+      fill_member(e->m[i], e->tile_w, e->tile_h, *d);
+      tiles[i] = d->tiles_x * d->tiles_y;
+      chunks[i] = (d->np + 255) / 256;
+    }
+  }
+  std::vector<int2> items;
+  // order 0: member after member; order 1: round-robin over members (small and large
+  // members interleaved, largest first).
+  auto emit = [&](const std::vector<int> &cnt) {
+    if (e->order == 0) {
+      for (int i = 0; i < n; i++)
+        for (int t = 0; t < cnt[i]; t++) items.push_back(make_int2(i, t));
+    } else {
+      std::vector<int> idx(n);
+      for (int i = 0; i < n; i++) idx[i] = i;
+      std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return cnt[a] > cnt[b]; });
+      const int mx = n ? cnt[idx[0]] : 0;
+      for (int t = 0; t < mx; t++)
+        for (int i : idx)
+          if (t < cnt[i]) items.push_back(make_int2(i, t));
+    }
+  };
+  emit(tiles);
+  e->n_lbm = (int)items.size();
+  emit(chunks);
+  e->n_part = (int)items.size() - e->n_lbm;
+  if (e->d_members) (void)hipFree(e->d_members);
+  if (e->d_items) (void)hipFree(e->d_items);
+  e->d_members = nullptr;
+  e->d_items = nullptr;
+  EHIP(e, hipMalloc(&e->d_members, std::max<size_t>(desc.size(), 64)));
+  EHIP(e, hipMalloc((void **)&e->d_items, std::max<size_t>(items.size() * sizeof(int2), 64)));
+  EHIP(e, hipMemcpy(e->d_members, desc.data(), desc.size(), hipMemcpyHostToDevice));
+  EHIP(e, hipMemcpy(e->d_items, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice));
+  e->built = true;
+  return RBN_OK;
+}
+
+int grid_size(const rbn_ensemble *e) { return e->wg_per_cu > 0 ? e->wg_per_cu * e->cus : 0; }
+
+// Enqueue step k (0..3) of a cycle on the two streams.  cross_waits: the lattice launch
+// waits for the particle launch of step k-2 (it overwrites the velocity that one reads);
+// inside a captured cycle that dependency exists only for k >= 2, across cycles the graph
+// launches are ordered by the stream.
+int enqueue_step(rbn_ensemble *e, int k, bool last, bool cross_waits) {
+  const int grid = grid_size(e);
+  if (cross_waits) EHIP(e, hipStreamWaitEvent(e->stream, e->ev_part[k & 1], 0));
+  EHIP(e, rbn::launch_ensemble_phase(e->stream, e->prec, 0, e->d_members, e->d_items, e->n_lbm,
+                                     e->d_counters, k, e->tile_w, e->tile_h, last ? 1 : 0, grid));
+  if (e->n_part > 0) {
+    EHIP(e, hipEventRecord(e->ev_lbm, e->stream));
+    EHIP(e, hipStreamWaitEvent(e->pstream, e->ev_lbm, 0));
+    EHIP(e, rbn::launch_ensemble_phase(e->pstream, e->prec, 1, e->d_members, e->d_items + e->n_lbm,
+                                       e->n_part, e->d_counters, k, e->tile_w, e->tile_h, 0, grid));
+    EHIP(e, hipEventRecord(e->ev_part[k & 1], e->pstream));
+  }
+  return RBN_OK;
+}
+
+int capture(rbn_ensemble *e) {
+  drop_graph(e);
+  EHIP(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+  int rc = RBN_OK;
+  for (int k = 0; k < 4 && !rc; k++) rc = enqueue_step(e, k, k == 3, k >= 2);
+  if (!rc && e->n_part > 0) {
+    // join the particle stream back into the origin stream
+    if (hipStreamWaitEvent(e->stream, e->ev_part[1], 0) != hipSuccess)
+      rc = efail(e, RBN_EDEVICE, "capture join failed");
+  }
+  hipGraph_t g = nullptr;
+  hipError_t ee = hipStreamEndCapture(e->stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (ee != hipSuccess) return efail(e, RBN_EDEVICE, "hipStreamEndCapture: %s", hipGetErrorString(ee));
+  e->graph = g;
+  EHIP(e, hipGraphInstantiate(&e->exec, g, nullptr, nullptr, 0));
+  return RBN_OK;
+}
+
+// Advance every member's host-side state by n steps of the ensemble.
+void advance_members(rbn_ensemble *e, int n) {
+  for (rbn_ctx *c : e->m) {
+    c->istep += n;
+    if (n & 1) {
+      std::swap(c->fA, c->fB);
+      std::swap(c->gA, c->gB);
+    }
+    c->cur = (c->cur + n) & 3;
+    c->moments_valid = true;
+  }
+}
+
+int sync_members(rbn_ensemble *e) {
+  for (rbn_ctx *c : e->m) {
+    EHIP(e, hipStreamSynchronize(c->stream));
+    EHIP(e, hipStreamSynchronize(c->pstream));
+  }
+  return RBN_OK;
+}
+
+int run_steps(rbn_ensemble *e, int nsteps) {
+  int rc = sync_members(e);
+  if (rc) return rc;
+  if (nsteps == 0) return RBN_OK;
+  if (!e->built && (rc = build(e))) return rc;
+  // Every launch leaves the counters of the next step clear, but eager steps that end
+  // mid-cycle do not leave set 0 clear for a cycle that starts over: clear both here
+  // (an ordinary stream operation, never part of the captured graph).
+  EHIP(e, hipMemsetAsync(e->d_counters, 0, 4 * sizeof(int), e->stream));
+  const int cycles = nsteps / 4, rem = nsteps % 4;
+  if (cycles > 0) {
+    if (!e->exec && (rc = capture(e))) return rc;
+    for (int c = 0; c < cycles; c++) EHIP(e, hipGraphLaunch(e->exec, e->stream));
+  }
+  for (int k = 0; k < rem; k++)
+    if ((rc = enqueue_step(e, k, k == rem - 1, k >= 2))) return rc;
+  if (rem > 0 && e->n_part > 0) EHIP(e, hipStreamWaitEvent(e->stream, e->ev_part[(rem - 1) & 1], 0));
+  EHIP(e, hipStreamSynchronize(e->stream));
+  EHIP(e, hipStreamSynchronize(e->pstream));
+  advance_members(e, nsteps);
+  if (rem > 0) {  // member parity changed: descriptors and graph must be rebuilt
+    e->built = false;
+    drop_graph(e);
+  }
+  return RBN_OK;
+}
+
+// Member state that the autotuner's trial steps change, saved on the device.
+struct Snapshot {
+  std::vector<void *> bufs;
+  std::vector<std::pair<void *, size_t>> src;
+  std::vector<int> istep, cur;
+  std::vector<void *> fA, gA;
+};
+
+int take_snapshot(rbn_ensemble *e, Snapshot &s) {
+  for (rbn_ctx *c : e->m) {
+    const size_t fb = c->geom.plane * c->rs;
+    std::vector<std::pair<void *, size_t>> parts = {{c->fA, 9 * fb}, {c->gA, 9 * fb}, {c->F, fb}};
+    for (int k = 0; k < 4; k++) {
+      parts.push_back({c->u[k], fb});
+      parts.push_back({c->v[k], fb});
+    }
+    if (c->d_part && c->np > 0) parts.push_back({c->d_part, 10 * (size_t)c->np * sizeof(double)});
+    for (auto &p : parts) {
+      void *b = nullptr;
+      EHIP(e, hipMalloc(&b, p.second));
+      EHIP(e, hipMemcpy(b, p.first, p.second, hipMemcpyDeviceToDevice));
+      s.bufs.push_back(b);
+      s.src.push_back(p);
+    }
+    s.istep.push_back(c->istep);
+    s.cur.push_back(c->cur);
+    s.fA.push_back(c->fA);
+    s.gA.push_back(c->gA);
+  }
+  return RBN_OK;
+}
+
+int restore_snapshot(rbn_ensemble *e, Snapshot &s) {
+  for (size_t i = 0; i < e->m.size(); i++) {
+    rbn_ctx *c = e->m[i];
+    if (c->fA != s.fA[i]) {
+      std::swap(c->fA, c->fB);
+      std::swap(c->gA, c->gB);
+    }
+    c->istep = s.istep[i];
+    c->cur = s.cur[i];
+  }
+  for (size_t k = 0; k < s.bufs.size(); k++) {
+    EHIP(e, hipMemcpy(s.src[k].first, s.bufs[k], s.src[k].second, hipMemcpyDeviceToDevice));
+    (void)hipFree(s.bufs[k]);
+  }
+  s.bufs.clear();
+  e->built = false;
+  return RBN_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void rbn_ensemble_destroy(rbn_ensemble *e) {
+  if (!e) return;
+  (void)hipSetDevice(e->dev);
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  if (e->pstream) (void)hipStreamSynchronize(e->pstream);
+  drop_graph(e);
+  if (e->d_members) (void)hipFree(e->d_members);
+  if (e->d_items) (void)hipFree(e->d_items);
+  if (e->d_counters) (void)hipFree(e->d_counters);
+  if (e->ev_lbm) (void)hipEventDestroy(e->ev_lbm);
+  for (hipEvent_t ev : e->ev_part)
+    if (ev) (void)hipEventDestroy(ev);
+  if (e->pstream) (void)hipStreamDestroy(e->pstream);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  delete e;
+}
+
+int rbn_ensemble_create(rbn_ctx *const *members, int n, rbn_ensemble **out) {
+  if (!members || n < 1 || !out) return rbn::thread_fail(RBN_EINVAL, "rbn_ensemble_create: bad arguments");
+  *out = nullptr;
+  for (int i = 0; i < n; i++) {
+    const rbn_ctx *c = members[i];
+    if (!c) return rbn::thread_fail(RBN_EINVAL, "null member");
+    if (!c->initialised) return rbn::thread_fail(RBN_ESTATE, "member not initialised");
+    if (c->P.variant != RBN_VARIANT_TRAJECTORIES)
+      return rbn::thread_fail(RBN_EINVAL, "ensemble members must be truth-run (trajectories) contexts");
+    if (c->dev != members[0]->dev || c->prec != members[0]->prec)
+      return rbn::thread_fail(RBN_EINVAL, "ensemble members must share device and precision");
+    for (int j = 0; j < i; j++)
+      if (members[j] == c) return rbn::thread_fail(RBN_EINVAL, "a context appears twice in the ensemble");
+  }
+  rbn_ensemble *e = new rbn_ensemble();
+  e->dev = members[0]->dev;
+  e->prec = members[0]->prec;
+  e->m.assign(members, members + n);
+  if (hipSetDevice(e->dev) != hipSuccess ||
+      hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&e->pstream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&e->ev_lbm, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&e->ev_part[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&e->ev_part[1], hipEventDisableTiming) != hipSuccess ||
+      hipMalloc((void **)&e->d_counters, 4 * sizeof(int)) != hipSuccess ||
+      hipMemset(e->d_counters, 0, 4 * sizeof(int)) != hipSuccess ||
+      hipDeviceGetAttribute(&e->cus, hipDeviceAttributeMultiprocessorCount, e->dev) != hipSuccess) {
+    rbn_ensemble_destroy(e);
+    return rbn::thread_fail(RBN_EDEVICE, "ensemble device set-up failed");
+  }
+  *out = e;
+  return RBN_OK;
+}
+
+const char *rbn_ensemble_last_error(const rbn_ensemble *e) { return e ? e->err.c_str() : rbn::thread_error(); }
+
+int rbn_ensemble_set_tuning(rbn_ensemble *e, int tile_w, int tile_h, int wg_per_cu, int order) {
+  if (!e) return rbn::thread_fail(RBN_EINVAL, "null ensemble");
+  if (!(tile_w == 64 || tile_w == 128 || tile_w == 256) || tile_h < 1 || tile_h > 64 ||
+      (tile_w * tile_h) % 256 != 0 || wg_per_cu < 0 || wg_per_cu > 32 || order < 0 || order > 1)
+    return efail(e, RBN_EINVAL, "tile must be 64/128/256 wide with a multiple of 256 nodes; wg_per_cu 0..32; order 0/1");
+  e->tile_w = tile_w;
+  e->tile_h = tile_h;
+  e->wg_per_cu = wg_per_cu;
+  e->order = order;
+  e->built = false;
+  drop_graph(e);
+  return RBN_OK;
+}
+
+int rbn_ensemble_get_tuning(const rbn_ensemble *e, int *tile_w, int *tile_h, int *wg_per_cu, int *order) {
+  if (!e) return rbn::thread_fail(RBN_EINVAL, "null ensemble");
+  if (tile_w) *tile_w = e->tile_w;
+  if (tile_h) *tile_h = e->tile_h;
+  if (wg_per_cu) *wg_per_cu = e->wg_per_cu;
+  if (order) *order = e->order;
+  return RBN_OK;
+}
+
+int rbn_ensemble_step(rbn_ensemble *e, int nsteps) {
+  if (!e || nsteps < 0) return rbn::thread_fail(RBN_EINVAL, "rbn_ensemble_step: bad arguments");
+  if (hipSetDevice(e->dev) != hipSuccess) return efail(e, RBN_EDEVICE, "hipSetDevice failed");
+  int rc = run_steps(e, nsteps);
+  if (rc) return rc;
+  for (rbn_ctx *c : e->m) {
+    int h = 0;
+    EHIP(e, hipMemcpy(&h, c->d_err, sizeof(int), hipMemcpyDeviceToHost));
+    if (h) return efail(e, RBN_ERANGE, "a particle of an ensemble member left its lattice");
+  }
+  return RBN_OK;
+}
+
+int rbn_ensemble_autotune(rbn_ensemble *e, int trial_steps, double *best_ms_per_step) {
+  if (!e) return rbn::thread_fail(RBN_EINVAL, "null ensemble");
+  if (hipSetDevice(e->dev) != hipSuccess) return efail(e, RBN_EDEVICE, "hipSetDevice failed");
+  const int trial = std::max(4, (trial_steps + 3) / 4 * 4);
+  struct Cand {
+    int tw, th, wpc, order;
+  };
+  std::vector<Cand> cands;
+  const int shapes[][2] = {{256, 1}, {256, 2}, {256, 4}, {128, 2}, {128, 4}, {64, 4}, {64, 8}};
+  const int wpcs[] = {0, 2, 4, 8};
+  for (auto &sh : shapes)
+    for (int w : wpcs)
+      for (int o = 0; o < 2; o++) cands.push_back({sh[0], sh[1], w, o});
+  int rc = sync_members(e);
+  if (rc) return rc;
+  Snapshot snap;
+  if ((rc = take_snapshot(e, snap))) return rc;
+  hipEvent_t a, b;
+  EHIP(e, hipEventCreate(&a));
+  EHIP(e, hipEventCreate(&b));
+  double best = 1e300;
+  Cand bestc = cands[0];
+  for (const Cand &c : cands) {
+    if ((rc = rbn_ensemble_set_tuning(e, c.tw, c.th, c.wpc, c.order))) break;
+    if ((rc = run_steps(e, 4))) break;  // build, capture, warm
+    EHIP(e, hipEventRecord(a, e->stream));
+    for (int s = 0; s < trial / 4; s++) EHIP(e, hipGraphLaunch(e->exec, e->stream));
+    EHIP(e, hipEventRecord(b, e->stream));
+    EHIP(e, hipEventSynchronize(b));
+    EHIP(e, hipStreamSynchronize(e->pstream));
+    advance_members(e, trial);
+    float ms = 0.f;
+    EHIP(e, hipEventElapsedTime(&ms, a, b));
+    if (ms / trial < best) {
+      best = ms / trial;
+      bestc = c;
+    }
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  int rc2 = restore_snapshot(e, snap);
+  if (rc) return rc;
+  if (rc2) return rc2;
+  if ((rc = rbn_ensemble_set_tuning(e, bestc.tw, bestc.th, bestc.wpc, bestc.order))) return rc;
+  if (best_ms_per_step) *best_ms_per_step = best;
+  return RBN_OK;
+}
+
+}  // extern "C"

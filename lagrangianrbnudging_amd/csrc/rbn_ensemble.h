@@ -1,0 +1,29 @@
+// rbn_ensemble.h — device-side description of one ensemble member (see rbn_ensemble.hip).
+#pragma once
+
+#include "rbn_internal.h"
+
+namespace rbn {
+
+// Pointers and constants of one member as of the start of a four-step cycle: pf[0]/pg[0] are
+// the f/g planes read by the first step, u[cur0] the latest velocity.
+template <typename R>
+struct EnsMember {
+  Geom g;
+  StepConst<R> c;
+  R *pf[2], *pg[2];
+  R *F;
+  R *u[4], *v[4];
+  R *rho, *T;
+  int cur0;
+  int tiles_x, tiles_y;
+  double *part;  // 10 arrays of np (x, y, u, v, xo, yo, uo, vo, 1/tau, beta)
+  int np;
+  int *err;
+};
+
+hipError_t launch_ensemble_phase(hipStream_t s, int prec, int phase, const void *members,
+                                 const int2 *items, int n_items, int *counters, int k, int tile_w,
+                                 int tile_h, int write_moments, int grid);
+
+}  // namespace rbn

@@ -92,6 +92,7 @@ struct LbmTuning {
   int block = 256;
   int nt = 1;
   int xcd = 0;
+  int overlap = 0;  // particle update of step s concurrent with the lattice step s+1
 };
 
 // Launchers (defined in the .hip files).

@@ -116,6 +116,13 @@ _SIGS = {
     "rbn_step": (C.c_int, [C.c_void_p, C.c_int]),
     "rbn_get_step": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "rbn_synchronize": (C.c_int, [C.c_void_p]),
+    "rbn_ensemble_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p)]),
+    "rbn_ensemble_destroy": (None, [C.c_void_p]),
+    "rbn_ensemble_last_error": (C.c_char_p, [C.c_void_p]),
+    "rbn_ensemble_step": (C.c_int, [C.c_void_p, C.c_int]),
+    "rbn_ensemble_set_tuning": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "rbn_ensemble_get_tuning": (C.c_int, [C.c_void_p] + [C.POINTER(C.c_int)] * 4),
+    "rbn_ensemble_autotune": (C.c_int, [C.c_void_p, C.c_int, _DP]),
     "rbn_set_tuning": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int]),
     "rbn_get_tuning": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_int)]),
     "rbn_set_kernel_timing": (C.c_int, [C.c_void_p, C.c_int]),
@@ -394,7 +401,7 @@ class Solver:
 
     def tuning(self) -> dict:
         out = {}
-        for k in ("lbm_block", "lbm_nt", "lbm_xcd"):
+        for k in ("lbm_block", "lbm_nt", "lbm_xcd", "particle_overlap"):
             v = C.c_int()
             self._c(self.lib.rbn_get_tuning(self.ctx, k.encode(), C.byref(v)))
             out[k] = v.value
@@ -407,3 +414,48 @@ class Solver:
         n, ms = C.c_longlong(), C.c_double()
         self._c(self.lib.rbn_kernel_stats(self.ctx, name.encode(), 1 if reset else 0, C.byref(n), C.byref(ms)))
         return n.value, ms.value
+
+
+class Ensemble:
+    """Several truth-run Solvers advanced together (rbn_ensemble_*)."""
+
+    def __init__(self, members: list):
+        self.lib = load()
+        self.members = list(members)
+        arr = (C.c_void_p * len(self.members))(*[m.ctx for m in self.members])
+        e = C.c_void_p()
+        rc = self.lib.rbn_ensemble_create(arr, len(self.members), C.byref(e))
+        if rc != 0:
+            raise RbnError(rc, self.lib.rbn_last_error(None).decode())
+        self.e = e
+
+    def _c(self, rc: int) -> None:
+        if rc != 0:
+            raise RbnError(rc, self.lib.rbn_ensemble_last_error(self.e).decode())
+
+    def close(self) -> None:
+        if self.e:
+            self.lib.rbn_ensemble_destroy(self.e)
+            self.e = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, n: int) -> None:
+        self._c(self.lib.rbn_ensemble_step(self.e, int(n)))
+
+    def set_tuning(self, tile_w=256, tile_h=1, wg_per_cu=4, order=0) -> None:
+        self._c(self.lib.rbn_ensemble_set_tuning(self.e, tile_w, tile_h, wg_per_cu, order))
+
+    def tuning(self) -> dict:
+        v = [C.c_int() for _ in range(4)]
+        self._c(self.lib.rbn_ensemble_get_tuning(self.e, *[C.byref(x) for x in v]))
+        return dict(zip(("tile_w", "tile_h", "wg_per_cu", "order"), (x.value for x in v)))
+
+    def autotune(self, trial_steps: int = 8) -> float:
+        ms = C.c_double()
+        self._c(self.lib.rbn_ensemble_autotune(self.e, int(trial_steps), C.byref(ms)))
+        return ms.value
