@@ -249,6 +249,22 @@ int rbn_ensemble_get_tuning(const rbn_ensemble *e, int *tile_w, int *tile_h, int
  * every member's state afterwards; *best_ms_per_step receives the winner's time. */
 int rbn_ensemble_autotune(rbn_ensemble *e, int trial_steps, double *best_ms_per_step);
 
+/* ---------------------------------------------------------------- adjoint window */
+/* Gradient of the trajectory-temperature misfit of a nudged run.  No reference counterpart
+ * (the reference has no gradient).  The window starts from the state rbn_init_fields set
+ * (no step since) and runs nsteps nudged steps exactly as rbn_step does; the misfit is
+ *   J = sum over steps s with s % ntempread == 0, over observers k, of (T_s(x_k,y_k) - T_k)^2
+ * with (x_k, y_k, T_k) the record observer k snaps to at step s and T_s(x, y) the bilinear
+ * interpolation of the step's temperature.  The state entering every ckpt_every-th step is
+ * kept on the device (20 lattice planes per checkpoint) with every fetched record.
+ * rbn_adj_forward leaves the context at the end of the window and stores J. */
+int rbn_adj_forward(rbn_ctx *ctx, int nsteps, int ckpt_every, double *J);
+/* Reverse sweep of the last rbn_adj_forward: recomputes each checkpoint segment (ckpt_every
+ * steps of populations held on the device) and runs the adjoint step over it backwards.
+ * dJ_dT0 (nx*ny, [i][j] as the T argument of rbn_init_fields) and dJ_dalpha may be NULL.
+ * The context's lattice and observer state are not modified. */
+int rbn_adj_backward(rbn_ctx *ctx, double *dJ_dT0, double *dJ_dalpha);
+
 /* ---------------------------------------------------------------- launch tuning */
 /* Launch shape of the fused lattice step; results do not depend on it.  No reference
  * counterpart (the reference has no launch configuration).  Keys:
@@ -264,7 +280,7 @@ int rbn_get_tuning(const rbn_ctx *ctx, const char *key, int *value);
 /* Enable per-launch HIP-event timing of each kernel on ctx's stream (0/1). */
 int rbn_set_kernel_timing(rbn_ctx *ctx, int enable);
 /* Launch count and summed duration (ms) of the timed launches of kernel `name`
- * ("lbm", "particles", "obs", "deposit"); resets the counters when reset != 0. */
+ * ("lbm", "particles", "obs", "deposit", "adjoint"); resets the counters when reset != 0. */
 int rbn_kernel_stats(rbn_ctx *ctx, const char *name, int reset, long long *launches,
                      double *total_ms);
 /* Time `nsteps` steps with HIP events on ctx's stream (inputs resident on the GPU). */

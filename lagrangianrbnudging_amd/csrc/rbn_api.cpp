@@ -20,11 +20,13 @@
 #include <unordered_map>
 #include <vector>
 
+#include "rbn_host.h"
 #include "rbn_internal.h"
 
 using rbn::Geom;
 
-namespace {
+namespace rbn {
+namespace host {
 
 int fail(rbn_ctx *c, int code, const char *fmt, ...) {
   char buf[512];
@@ -36,14 +38,6 @@ int fail(rbn_ctx *c, int code, const char *fmt, ...) {
   rbn::thread_fail(code, buf);
   return code;
 }
-
-#define HIPCHK(ctx, expr)                                                                   \
-  do {                                                                                      \
-    hipError_t e_ = (expr);                                                                 \
-    if (e_ != hipSuccess)                                                                   \
-      return fail((ctx), RBN_EDEVICE, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
-                  __FILE__, __LINE__);                                                      \
-  } while (0)
 
 int dalloc(rbn_ctx *c, void **p, size_t bytes) {
   *p = nullptr;
@@ -58,13 +52,6 @@ int dalloc(rbn_ctx *c, void **p, size_t bytes) {
 
 void dfree(void *&p) {
   if (p) (void)hipFree(p);
-  p = nullptr;
-}
-
-template <typename T>
-void dfree_t(T *&p) {
-  void *q = p;
-  dfree(q);
   p = nullptr;
 }
 
@@ -96,10 +83,10 @@ void fill_consts(const rbn_params &P, rbn::StepConst<R> &c) {
   }
 }
 
-const char *kKernelNames[4] = {"lbm", "particles", "obs", "deposit"};
+const char *const kKernelNames[KT_COUNT] = {"lbm", "particles", "obs", "deposit", "adjoint"};
 
 int kernel_index(const char *name) {
-  for (int k = 0; k < 4; k++)
+  for (int k = 0; k < KT_COUNT; k++)
     if (name && strcmp(name, kKernelNames[k]) == 0) return k;
   return -1;
 }
@@ -129,25 +116,6 @@ int drain_timing(rbn_ctx *c) {
   return RBN_OK;
 }
 
-// Launch through fn() on stream s, bracketed with events on s when kernel timing is on.
-template <typename Fn>
-int timed(rbn_ctx *c, int kernel, hipStream_t s, Fn &&fn) {
-  hipEvent_t a = nullptr, b = nullptr;
-  if (c->timing) {
-    a = take_event(c);
-    b = take_event(c);
-    if (!a || !b) return fail(c, RBN_EDEVICE, "hipEventCreate failed");
-    HIPCHK(c, hipEventRecord(a, s));
-  }
-  hipError_t e = fn();
-  if (e != hipSuccess) return fail(c, RBN_EDEVICE, "%s launch failed: %s", kKernelNames[kernel], hipGetErrorString(e));
-  if (c->timing) {
-    HIPCHK(c, hipEventRecord(b, s));
-    c->pending.push_back({kernel, a, b});
-    if (c->pending.size() > 8192) return drain_timing(c);
-  }
-  return RBN_OK;
-}
 
 // Host [nx][ny] (y fastest) <-> device [j][i] with pitch, converting to the real type.
 template <typename R>
@@ -170,7 +138,7 @@ int upload_field(rbn_ctx *c, void *dst, const double *h) {
   return RBN_OK;
 }
 
-int download_field(rbn_ctx *c, const void *src, double *h, double add = 0.0) {
+int download_field(rbn_ctx *c, const void *src, double *h, double add) {
   const Geom &g = c->geom;
   std::vector<unsigned char> buf(g.plane * c->rs);
   HIPCHK(c, hipMemcpyAsync(buf.data(), src, buf.size(), hipMemcpyDeviceToHost, c->stream));
@@ -222,18 +190,14 @@ rbn::PartPtrs part_ptrs(rbn_ctx *c, double *rec) {
   return pp;
 }
 
-enum { OCHK, OBCY, OXIDX, OYIDX, ONI };
-enum { ODX, ODY, ODT, OXINT, OYINT, OTINT, ORX, ORY, ORT, OND };
-
-rbn::ObsPtrs obs_ptrs(rbn_ctx *c) {
+rbn::ObsPtrs obs_view(int nobs, int *ints, double *d) {
   rbn::ObsPtrs o{};
-  const size_t n = (size_t)c->nobs;
-  o.n = c->nobs;
-  o.chk = c->d_obs_i + OCHK * n;
-  o.bcy = c->d_obs_i + OBCY * n;
-  o.xidx = c->d_obs_i + OXIDX * n;
-  o.yidx = c->d_obs_i + OYIDX * n;
-  double *d = c->d_obs_d;
+  const size_t n = (size_t)nobs;
+  o.n = nobs;
+  o.chk = ints + OCHK * n;
+  o.bcy = ints + OBCY * n;
+  o.xidx = ints + OXIDX * n;
+  o.yidx = ints + OYIDX * n;
   o.dx = d + ODX * n;
   o.dy = d + ODY * n;
   o.dT = d + ODT * n;
@@ -245,6 +209,8 @@ rbn::ObsPtrs obs_ptrs(rbn_ctx *c) {
   o.rT = d + ORT * n;
   return o;
 }
+
+rbn::ObsPtrs obs_ptrs(rbn_ctx *c) { return obs_view(c->nobs, c->d_obs_i, c->d_obs_d); }
 
 int check_err_flag(rbn_ctx *c) {
   int h = 0;
@@ -306,10 +272,10 @@ int claim_record_slot(rbn_ctx *c, rbn_ctx::RecSlot **out) {
   return RBN_OK;
 }
 
-// Queue the copy of a freshly written slot on the copy stream.
-int queue_record_copy(rbn_ctx *c, rbn_ctx::RecSlot &s, int step) {
+// Queue the copy of a freshly written slot (written by K2 on stream ks) on the copy stream.
+int queue_record_copy(rbn_ctx *c, rbn_ctx::RecSlot &s, int step, hipStream_t ks) {
   const size_t narr = c->P.prtcl_writefull ? 17 : 9;
-  HIPCHK(c, hipEventRecord(s.written, c->pstream));
+  HIPCHK(c, hipEventRecord(s.written, ks));
   HIPCHK(c, hipStreamWaitEvent(c->copy_stream, s.written, 0));
   HIPCHK(c, hipMemcpyAsync(s.h, s.d, narr * (size_t)c->np * sizeof(double), hipMemcpyDeviceToHost, c->copy_stream));
   HIPCHK(c, hipEventRecord(s.copied, c->copy_stream));
@@ -371,6 +337,7 @@ int load_observation(rbn_ctx *c, int rstep) {
 
 int init_common_after_fields(rbn_ctx *c) {
   c->istep = c->P.nstart;
+  c->at_init = false;
   c->initialised = true;
   c->moments_valid = true;
   if (c->nud) HIPCHK(c, hipMemsetAsync(c->nud, 0, c->geom.plane * c->rs, c->stream));
@@ -379,7 +346,10 @@ int init_common_after_fields(rbn_ctx *c) {
   return RBN_OK;
 }
 
-}  // namespace
+}  // namespace host
+}  // namespace rbn
+
+using namespace rbn::host;
 
 extern "C" {
 
@@ -404,6 +374,7 @@ void rbn_destroy(rbn_ctx *c) {
   (void)hipSetDevice(c->dev);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->pstream) (void)hipStreamSynchronize(c->pstream);
+  rbn::adj_free(c);
   void *ptrs[] = {c->fA, c->fB, c->gA, c->gB, c->F, c->rho, c->T, c->nud};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -579,7 +550,9 @@ int rbn_init_fields(rbn_ctx *c, const double *rho, const double *T, const double
   hipError_t e = rbn::launch_init_eq(c->stream, c->prec, g, c->rho, c->T, c->u[0], c->v[0], c->fA,
                                      c->gA, c->F, c->consts.data());
   if (e != hipSuccess) return fail(c, RBN_EDEVICE, "init launch failed: %s", hipGetErrorString(e));
-  return init_common_after_fields(c);
+  if ((rc = init_common_after_fields(c))) return rc;
+  c->at_init = true;
+  return RBN_OK;
 }
 
 int rbn_init_conduction(rbn_ctx *c, double noise_amp, unsigned long long seed) {
@@ -885,6 +858,7 @@ int rbn_step(rbn_ctx *c, int nsteps) {
   const bool parts = !nudged && P.particle && c->np > 0;
   const int nd = P.ndiag > 0 ? P.ndiag : (nudged ? 5000 : P.nout);
   const int right = P.fix_q3 ? P.nx - 3 : P.nx - 2;
+  if (nsteps > 0) c->at_init = false;
   for (int n = 0; n < nsteps; n++) {
     const int istep = ++c->istep;
     const bool record = parts && (istep % P.noutconfig) == 0;
@@ -905,10 +879,13 @@ int rbn_step(rbn_ctx *c, int nsteps) {
     // also reads T, which this step overwrites when it writes moments.
     const int nxt = (c->cur + 1) % rbn_ctx::kVel;
     // Without overlap (the default: both kernels are bandwidth bound, and concurrency only
-    // moves the contention) the lattice step also waits for the particle update before it.
-    if (parts) {
+    // moves the contention) K2 runs on the lattice stream itself, ordered without events (a
+    // cross-stream event wait costs about 20 us per hand-over).
+    const bool overlap = c->tune.overlap != 0;
+    hipStream_t ks = overlap ? c->pstream : c->stream;
+    if (parts && overlap) {
       HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_part[istep % 2], 0));
-      if (!c->tune.overlap || (moments && c->part_record_pending))
+      if (moments && c->part_record_pending)
         HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_part[(istep + 1) % 2], 0));
     }
     rbn::LbmPtrs lp{};
@@ -936,22 +913,24 @@ int rbn_step(rbn_ctx *c, int nsteps) {
     c->cur = nxt;
     c->moments_valid = moments;
     if (parts) {
-      // K2 of step s on the particle stream, overlapping K1 of step s+1.
-      HIPCHK(c, hipEventRecord(c->ev_lbm, c->stream));
-      HIPCHK(c, hipStreamWaitEvent(c->pstream, c->ev_lbm, 0));
+      // K2 of step s; with overlap, on the particle stream alongside K1 of step s+1.
+      if (overlap) {
+        HIPCHK(c, hipEventRecord(c->ev_lbm, c->stream));
+        HIPCHK(c, hipStreamWaitEvent(c->pstream, c->ev_lbm, 0));
+      }
       const int full = record && P.prtcl_writefull;
       rbn_ctx::RecSlot *slot = nullptr;
       if (record && (rc = claim_record_slot(c, &slot))) return rc;
       rbn::PartPtrs pp = part_ptrs(c, slot ? slot->d : nullptr);
-      if ((rc = timed(c, 1, c->pstream, [&] {
-             return rbn::launch_particles(c->pstream, c->prec, c->geom, c->u[c->cur], c->v[c->cur],
+      if ((rc = timed(c, 1, ks, [&] {
+             return rbn::launch_particles(ks, c->prec, c->geom, c->u[c->cur], c->v[c->cur],
                                           c->u[prv], c->v[prv], c->T, pp, c->any_heavy || full, record,
                                           full);
            })))
         return rc;
-      HIPCHK(c, hipEventRecord(c->ev_part[istep % 2], c->pstream));
-      c->part_record_pending = record;
-      if (slot && c->rec_fn && (rc = queue_record_copy(c, *slot, istep))) return rc;
+      if (overlap) HIPCHK(c, hipEventRecord(c->ev_part[istep % 2], c->pstream));
+      c->part_record_pending = overlap && record;
+      if (slot && c->rec_fn && (rc = queue_record_copy(c, *slot, istep, ks))) return rc;
     }
     if (diag && c->diag_fn) {
       double r[2];

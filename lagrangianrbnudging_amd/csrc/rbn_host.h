@@ -1,0 +1,83 @@
+// rbn_host.h — host-side helpers shared by the C-ABI translation units (rbn_api.cpp,
+// rbn_adjoint.cpp): error reporting, device allocation, per-launch timing, observer array
+// views and the observation-record upload.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+
+#include "rbn_internal.h"
+
+namespace rbn {
+namespace host {
+
+// Record an error on the context (and the calling thread) and return code.
+int fail(rbn_ctx *c, int code, const char *fmt, ...);
+
+#define HIPCHK(ctx, expr)                                                                   \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      return ::rbn::host::fail((ctx), RBN_EDEVICE, "%s failed: %s (%s:%d)", #expr,          \
+                               hipGetErrorString(e_), __FILE__, __LINE__);                  \
+  } while (0)
+
+int dalloc(rbn_ctx *c, void **p, size_t bytes);
+void dfree(void *&p);
+
+template <typename T>
+void dfree_t(T *&p) {
+  void *q = p;
+  dfree(q);
+  p = nullptr;
+}
+
+int check_ctx(rbn_ctx *c);
+
+// Kernel classes of the per-launch timing (rbn_set_kernel_timing / rbn_kernel_stats).
+enum { KT_LBM, KT_PARTICLES, KT_OBS, KT_DEPOSIT, KT_ADJOINT, KT_COUNT };
+extern const char *const kKernelNames[KT_COUNT];
+hipEvent_t take_event(rbn_ctx *c);
+int drain_timing(rbn_ctx *c);
+
+// Launch through fn() on stream s, bracketed with events on s when kernel timing is on.
+template <typename Fn>
+int timed(rbn_ctx *c, int kernel, hipStream_t s, Fn &&fn) {
+  hipEvent_t a = nullptr, b = nullptr;
+  if (c->timing) {
+    a = take_event(c);
+    b = take_event(c);
+    if (!a || !b) return fail(c, RBN_EDEVICE, "hipEventCreate failed");
+    HIPCHK(c, hipEventRecord(a, s));
+  }
+  hipError_t e = fn();
+  if (e != hipSuccess) return fail(c, RBN_EDEVICE, "%s launch failed: %s", kKernelNames[kernel], hipGetErrorString(e));
+  if (c->timing) {
+    HIPCHK(c, hipEventRecord(b, s));
+    c->pending.push_back({kernel, a, b});
+    if (c->pending.size() > 8192) return drain_timing(c);
+  }
+  return RBN_OK;
+}
+
+// Device field <-> host array in the reference's [nx][ny] order (y fastest).
+int upload_field(rbn_ctx *c, void *dst, const double *h);
+int download_field(rbn_ctx *c, const void *src, double *h, double add = 0.0);
+
+// Observer arrays: ints chk, bcy, xidx, yidx; doubles dx, dy, dT, xint, yint, Tint and the
+// current record rx, ry, rT (record order).  Each array holds nobs entries.
+enum { OCHK, OBCY, OXIDX, OYIDX, ONI };
+enum { ODX, ODY, ODT, OXINT, OYINT, OTINT, ORX, ORY, ORT, OND };
+ObsPtrs obs_view(int n, int *ints, double *dbls);
+ObsPtrs obs_ptrs(rbn_ctx *c);
+
+// Fetch record rstep from the observation provider, map observers to record entries by id
+// and upload it as the current record.
+int load_observation(rbn_ctx *c, int rstep);
+
+// Report (and clear) a particle / observer range error or a deposit overflow.
+int check_err_flag(rbn_ctx *c);
+
+}  // namespace host
+}  // namespace rbn

@@ -95,6 +95,9 @@ struct LbmTuning {
   int overlap = 0;  // particle update of step s concurrent with the lattice step s+1
 };
 
+struct AdjState;  // rbn_adjoint.cpp
+void adj_free(rbn_ctx *c);
+
 // Launchers (defined in the .hip files).
 hipError_t launch_lbm(hipStream_t s, int prec, int variant, const Geom &g, const LbmPtrs &p,
                       const void *step_const, const LbmTuning &t);
@@ -195,8 +198,12 @@ struct rbn_ctx {
   };
   std::vector<Pending> pending;
   std::vector<hipEvent_t> event_pool;
-  long long k_launches[4] = {0, 0, 0, 0};
-  double k_ms[4] = {0, 0, 0, 0};
+  long long k_launches[5] = {0, 0, 0, 0, 0};
+  double k_ms[5] = {0, 0, 0, 0, 0};
+
+  // adjoint window (rbn_adj_forward / rbn_adj_backward)
+  bool at_init = false;             // state is the one rbn_init_fields set (no step since)
+  rbn::AdjState *adj = nullptr;
 
   std::string err;
 };

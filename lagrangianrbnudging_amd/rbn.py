@@ -123,6 +123,8 @@ _SIGS = {
     "rbn_ensemble_set_tuning": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]),
     "rbn_ensemble_get_tuning": (C.c_int, [C.c_void_p] + [C.POINTER(C.c_int)] * 4),
     "rbn_ensemble_autotune": (C.c_int, [C.c_void_p, C.c_int, _DP]),
+    "rbn_adj_forward": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _DP]),
+    "rbn_adj_backward": (C.c_int, [C.c_void_p, _DP, _DP]),
     "rbn_set_tuning": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int]),
     "rbn_get_tuning": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_int)]),
     "rbn_set_kernel_timing": (C.c_int, [C.c_void_p, C.c_int]),
@@ -388,6 +390,20 @@ class Solver:
 
     def synchronize(self) -> None:
         self._c(self.lib.rbn_synchronize(self.ctx))
+
+    # -- adjoint window (nudged run)
+    def adj_forward(self, nsteps: int, ckpt_every: int = 100) -> float:
+        """Run nsteps nudged steps from the initial state and return the misfit J."""
+        J = C.c_double()
+        self._c(self.lib.rbn_adj_forward(self.ctx, int(nsteps), int(ckpt_every), C.byref(J)))
+        return J.value
+
+    def adj_backward(self) -> tuple[np.ndarray, float]:
+        """Gradient of the last adj_forward's J: (dJ/dT0 as [nx][ny], dJ/dalpha)."""
+        dT0 = np.empty((self.nx, self.ny))
+        da = C.c_double()
+        self._c(self.lib.rbn_adj_backward(self.ctx, _dp(dT0), C.byref(da)))
+        return dT0, da.value
 
     def time_steps(self, n: int) -> float:
         ms = C.c_double()

@@ -60,6 +60,26 @@ def truth_records(p, T0, particles, steps, device=0):
     return recs
 
 
+def synthetic_observations(nx, ny, n, steps, every, seed):
+    """Observers on straight lines that cross the periodic x boundary and the walls."""
+    rng = np.random.default_rng(seed)
+    x0 = np.concatenate([[0.3, 3.2, nx - 2.0, nx - 1.4, nx - 0.2, 1.9], rng.uniform(0, nx, n - 6)])
+    y0 = np.concatenate([[1.1, ny - 0.1, 2.4, 1.6, ny - 1.2, ny / 2], rng.uniform(1, ny, n - 6)])
+    vx = rng.uniform(-0.15, 0.15, n)
+    vy = rng.uniform(-0.1, 0.1, n)
+    T = rng.uniform(0, 1, n)
+    ids = np.arange(n, dtype=np.int32)
+    recs = {}
+    for s in range(every, steps + every, every):
+        xs = x0 + vx * s
+        ys = y0 + vy * s
+        xs = np.mod(xs, nx)
+        ys = 1 + np.mod(ys - 1, ny - 1)
+        perm = rng.permutation(n)
+        recs[s] = {"id": ids[perm], "x": xs[perm], "y": ys[perm], "temp": (T + 0.01 * s)[perm]}
+    return ids, x0, y0, T, recs
+
+
 def assert_equal_arrays(a, b, what):
     a = np.asarray(a)
     b = np.asarray(b)

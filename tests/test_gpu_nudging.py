@@ -5,7 +5,8 @@ import pytest
 
 from lagrangianrbnudging_amd import rbn
 from oracle.oracle import Oracle
-from scenarios import assert_equal_arrays, conduction_T, random_particles, rb_params, truth_records
+from scenarios import (assert_equal_arrays, conduction_T, random_particles, rb_params, synthetic_observations,
+                       truth_records)
 
 pytestmark = pytest.mark.gpu
 
@@ -63,26 +64,6 @@ def test_nudged_run_fp64_bit_exact(alpha):
     assert np.count_nonzero(g.nudging()["nudge_vol"]) > 0
 
 
-def _synthetic_records(nx, ny, n, steps, every, seed):
-    """Observers on straight lines that cross the periodic x boundary and the walls."""
-    rng = np.random.default_rng(seed)
-    x0 = np.concatenate([[0.3, 3.2, nx - 2.0, nx - 1.4, nx - 0.2, 1.9], rng.uniform(0, nx, n - 6)])
-    y0 = np.concatenate([[1.1, ny - 0.1, 2.4, 1.6, ny - 1.2, ny / 2], rng.uniform(1, ny, n - 6)])
-    vx = rng.uniform(-0.15, 0.15, n)
-    vy = rng.uniform(-0.1, 0.1, n)
-    T = rng.uniform(0, 1, n)
-    ids = np.arange(n, dtype=np.int32)
-    recs = {}
-    for s in range(every, steps + every, every):
-        xs = x0 + vx * s
-        ys = y0 + vy * s
-        xs = np.mod(xs, nx)
-        ys = 1 + np.mod(ys - 1, ny - 1)
-        perm = rng.permutation(n)
-        recs[s] = {"id": ids[perm], "x": xs[perm], "y": ys[perm], "temp": (T + 0.01 * s)[perm]}
-    return ids, x0, y0, T, recs
-
-
 @pytest.mark.parametrize("fix_q3", [0, 1])
 def test_nudged_deposit_edges_fp64(fix_q3):
     pn = rb_params(40, 24, 1e5)
@@ -90,7 +71,7 @@ def test_nudged_deposit_edges_fp64(fix_q3):
     pn.ntempread = 4
     pn.alpha = 0.3
     pn.fix_q3 = fix_q3
-    ids, x0, y0, T, recs = _synthetic_records(pn.nx, pn.ny, 60, 120, 4, seed=8)
+    ids, x0, y0, T, recs = synthetic_observations(pn.nx, pn.ny, 60, 120, 4, seed=8)
     T0n = conduction_T(pn, 1e-2, seed=4)
     g, o = _nudged_pair(pn, T0n, ids, x0, y0, T, recs, 101)
     _compare(g, o)

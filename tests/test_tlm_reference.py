@@ -1,0 +1,92 @@
+"""The complex-step tangent-linear reference (oracle/tlm_nudged.py), pinned on the CPU before
+it is used to check the GPU adjoint: its real part against the C oracle's nudged run, its
+misfit against the oracle's own interpolation, and its derivative against central
+differences and linearity."""
+import numpy as np
+import pytest
+
+from lagrangianrbnudging_amd import rbn
+from oracle import tlm_nudged as tlm
+from scenarios import conduction_T, rb_params, synthetic_observations
+
+
+def _scenario(nx=40, ny=24, ntr=4, alpha=0.3, nobs=60, steps=41, seed=8):
+    p = rb_params(nx, ny, 1e5)
+    p.variant = rbn.RBN_VARIANT_NUDGING
+    p.ntempread = ntr
+    p.alpha = alpha
+    ids, x0, y0, T, recs = synthetic_observations(nx, ny, nobs, steps + ntr, ntr, seed=seed)
+    T0 = conduction_T(p, 1e-2, seed=4)
+    data, pops = tlm.nudging_data(p, T0, (ids, x0, y0, T), recs, steps)
+    return p, T0, data, pops
+
+
+@pytest.fixture(scope="module")
+def scenario():
+    return _scenario()
+
+
+def test_restatement_matches_oracle_populations(scenario):
+    p, T0, data, (f_or, g_or) = scenario
+    J, (f, g, F, nud) = tlm.misfit(p, T0, p.alpha, data)
+    f = np.moveaxis(f, 0, -1)
+    g = np.moveaxis(g, 0, -1)
+    # same arithmetic up to operand order: agreement to a few ulps of O(1) populations
+    assert np.max(np.abs(f - f_or)) < 1e-13
+    assert np.max(np.abs(g - g_or)) < 1e-13
+    assert np.isfinite(J) and J > 0
+
+
+def test_misfit_matches_oracle_interpolation(scenario):
+    p, T0, data, _ = scenario
+    J, _ = tlm.misfit(p, T0, p.alpha, data)
+    Jo = 0.0
+    nobs_steps = 0
+    for d in data:
+        if "obs" in d:
+            nobs_steps += 1
+            x, y, To = d["obs"]
+            Jo += float(np.sum((tlm.bilinear(d["T"], x, y) - To) ** 2))
+    assert nobs_steps == len(data) // p.ntempread
+    assert abs(J - Jo) <= 1e-12 * Jo
+
+
+def test_bilinear_reproduces_linear_fields_and_wraps():
+    nx, ny = 12, 8
+    i = np.arange(1, nx + 1)[:, None]
+    j = np.arange(1, ny + 1)[None, :]
+    T = 0.5 + 0.25 * i + 0.125 * j + 0 * i * j
+    x = np.array([1.0, 2.5, 7.25, 11.9])
+    y = np.array([1.0, 3.5, 2.75, 7.9])
+    assert np.allclose(tlm.bilinear(T, x, y), 0.5 + 0.25 * x + 0.125 * y, atol=1e-14)
+    # the ghost column 0 copies column nx, ghost row ny+1 copies row 1
+    Tw = np.random.default_rng(1).uniform(size=(nx, ny))
+    assert np.isclose(tlm.bilinear(Tw, np.array([0.0]), np.array([2.0]))[0], Tw[nx - 1, 1])
+    assert np.isclose(tlm.bilinear(Tw, np.array([3.0]), np.array([ny + 0.99999999]))[0], Tw[2, 0], atol=1e-7)
+
+
+def test_complex_step_matches_central_differences(scenario):
+    p, T0, data, _ = scenario
+    rng = np.random.default_rng(11)
+    for _ in range(3):
+        d = rng.standard_normal(T0.shape)
+        _, dJ = tlm.directional_derivative(p, T0, p.alpha, data, d)
+        eps = 1e-5
+        Jp, _ = tlm.misfit(p, T0 + eps * d, p.alpha, data)
+        Jm, _ = tlm.misfit(p, T0 - eps * d, p.alpha, data)
+        fd = (Jp - Jm) / (2 * eps)
+        assert abs(fd - dJ) <= 1e-6 * abs(dJ)
+    _, dJa = tlm.directional_derivative(p, T0, p.alpha, data, np.zeros_like(T0), 1.0)
+    eps = 1e-6
+    fd = (tlm.misfit(p, T0, p.alpha + eps, data)[0] - tlm.misfit(p, T0, p.alpha - eps, data)[0]) / (2 * eps)
+    assert abs(fd - dJa) <= 1e-6 * abs(dJa)
+
+
+def test_directional_derivative_is_linear(scenario):
+    p, T0, data, _ = scenario
+    rng = np.random.default_rng(12)
+    a, b = rng.standard_normal(T0.shape), rng.standard_normal(T0.shape)
+    _, da = tlm.directional_derivative(p, T0, p.alpha, data, a)
+    _, db = tlm.directional_derivative(p, T0, p.alpha, data, b)
+    _, dab = tlm.directional_derivative(p, T0, p.alpha, data, 2.0 * a - 3.0 * b)
+    assert abs(dab - (2.0 * da - 3.0 * db)) <= 1e-12 * (abs(da) + abs(db))
