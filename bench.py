@@ -9,9 +9,12 @@ t_ff = (ny-1)/u_ff, zero initial slip), uniform random positions; trajectory rec
 kernel (+ the record on record steps).  Inputs are resident on the GPU before timing starts.
 
 Prints one JSON line: MLUPS of the FP32 (shifted-population) path, the HBM roofline of the
-dominant kernel (fused LBM step; HIP events around every launch in the timed region), the
-FP64 path's MLUPS on the same workload, and the CPU FP64 restatement of the reference loop
-timed on a bounded sample on this host.
+dominant kernel (fused LBM step; HIP events around every launch in the timed region; the
+HBM traffic per launch comes from the committed rocprofv3 counter summary under profiles/),
+the FP64 path's MLUPS on the same workload, the batched ensemble (64 uneven members) against
+the same members stepped one by one, the adjoint window's cost against the plain nudged
+forward (512 x 256, 2000 steps, checkpoint every 100), and the CPU FP64 restatement of the
+reference loop timed on a bounded sample on this host.
 """
 from __future__ import annotations
 
@@ -149,6 +152,109 @@ def run_ensemble(rbn, steps, warmup):
     return out
 
 
+ADJ_NX, ADJ_NY, ADJ_RA = 512, 256, 1e7
+ADJ_OBS, ADJ_EVERY, ADJ_ALPHA = 10_000, 10, 0.1
+
+
+def adjoint_bytes_per_node(precision):
+    # populations entering the step (18), lagged force, nud_term, pulled adjoints of the next
+    # step (18), force and nud_term adjoints: loads; pulled adjoints (18) + 2: stores; plus
+    # the deposit count (int) and mean (double) of the step.
+    r = 8 if precision == 0 else 4
+    return (18 + 2 + 18 + 2) * r + (18 + 2) * r + 4 + 8
+
+
+def run_adjoint(rbn, window, ckpt):
+    """Adjoint configuration: 512 x 256, Ra = 1e7, FP32.  A truth run with 1e4 tracers
+    recorded every 10 steps provides the observations; the nudged run starts from another
+    initial temperature.  Timed: the plain nudged forward (rbn_step) over the window, then
+    the checkpointed forward (rbn_adj_forward) and the reverse sweep (rbn_adj_backward)."""
+    pt = rbn.default_params(nx=ADJ_NX, ny=ADJ_NY, Tup=0.0, Tdown=1.0, scratch=1, nstart=0, nout=10**9,
+                            noutconfig=ADJ_EVERY, prtcl_writefull=0, particle=1,
+                            precision=rbn.RBN_FP32, T0_shift=0.5, rho0_shift=1.0)
+    pt = rbn.params_from_ra(pt, ADJ_RA, PR, U_FF)
+    rng = np.random.default_rng(21)
+    lin = np.tile(pt.Tdown + (pt.Tup - pt.Tdown) * np.arange(ADJ_NY) / (ADJ_NY - 1), (ADJ_NX, 1))
+    T_truth = lin + rng.uniform(-1e-2, 1e-2, lin.shape)
+    T_first = lin + rng.uniform(-1e-2, 1e-2, lin.shape)
+    ids = np.arange(ADJ_OBS, dtype=np.int32)
+    x = rng.uniform(0.0, ADJ_NX, ADJ_OBS)
+    x[x == 0.0] = 0.5
+    y = rng.uniform(1.0, ADJ_NY, ADJ_OBS)
+    truth = rbn.Solver(pt, 0)
+    truth.init_fields(T=T_truth)
+    truth.init_particles(ids, x, y, tau_p=np.zeros(ADJ_OBS))
+    T_at = truth.sample("T", x, y)
+    truth.collect_records()
+    truth.step(window + ADJ_EVERY)
+    recs = {r["step"]: {"id": r["id"], "x": r["x"], "y": r["y"], "temp": r["temp"]} for r in truth.records}
+    truth.close()
+    pn = pt.copy()
+    pn.variant = rbn.RBN_VARIANT_NUDGING
+    pn.ntempread = ADJ_EVERY
+    pn.alpha = ADJ_ALPHA
+
+    def fresh():
+        s = rbn.Solver(pn, 0)
+        s.set_observation_source(recs)
+        s.init_fields(T=T_first)
+        s.init_observers(ids, x, y, T_at)
+        return s
+
+    s = fresh()
+    s.step(window // 10)  # warm-up
+    s.close()
+    s = fresh()
+    t0 = time.perf_counter()
+    s.step(window)
+    s.synchronize()
+    t_plain = time.perf_counter() - t0
+    s.close()
+    a = fresh()
+    a.adj_forward(window, ckpt)  # warm-up (allocations)
+    a.adj_backward()
+    a.init_fields(T=T_first)
+    a.init_observers(ids, x, y, T_at)
+    t0 = time.perf_counter()
+    J = a.adj_forward(window, ckpt)
+    t1 = time.perf_counter()
+    g, ga = a.adj_backward()
+    t2 = time.perf_counter()
+    a.kernel_timing(True)
+    a.kernel_stats("adjoint", reset=True)
+    a.adj_backward()
+    n_adj, ms_adj = a.kernel_stats("adjoint")
+    a.kernel_timing(False)
+    a.close()
+    avg = ms_adj / max(n_adj, 1)
+    nodes = ADJ_NX * ADJ_NY
+    return {"config": f"{ADJ_NX}x{ADJ_NY}, Ra={ADJ_RA:g}, FP32, window {window} steps, checkpoint every {ckpt}, "
+                      f"{ADJ_OBS} tracer observations every {ADJ_EVERY} steps, alpha={ADJ_ALPHA}",
+            "ms_per_step_plain": round(1e3 * t_plain / window, 4),
+            "ms_per_step_forward_ckpt": round(1e3 * (t1 - t0) / window, 4),
+            "ms_per_step_backward": round(1e3 * (t2 - t1) / window, 4),
+            "cost_vs_plain_forward": round((t2 - t0) / t_plain, 3),
+            "adjoint_kernel_avg_ms": round(avg, 5),
+            "adjoint_kernel_gbps": round(nodes * adjoint_bytes_per_node(1) / (avg * 1e-3) / 1e9, 1),
+            "J": J, "max_abs_dJ_dT0": float(np.max(np.abs(g))), "dJ_dalpha": ga,
+            "finite": bool(np.isfinite(g).all() and np.isfinite(ga))}
+
+
+TRAFFIC_PROFILE = ROOT / "profiles" / "k_lbm32_4096x2048_pmc.txt"
+
+
+def measured_traffic():
+    """HBM bytes per K1 launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+    (counters scaled by a copy kernel of known size), or None."""
+    try:
+        for line in TRAFFIC_PROFILE.read_text().splitlines():
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def run_cpu_baseline(rbn, steps):
     """The CPU FP64 restatement of the reference loop (array-of-structs, per-direction sweeps,
     halo copies) on one host core, same workload, bounded to a few steps."""
@@ -182,6 +288,9 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 companion run")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble run")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-window run")
+    ap.add_argument("--adj-window", type=int, default=2000)
+    ap.add_argument("--adj-ckpt", type=int, default=100)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build is a single-GPU solver; the multi-GPU slab decomposition is "
@@ -203,6 +312,10 @@ def main() -> int:
     if not args.no_ensemble:
         ens = run_ensemble(rbn, max(args.steps // 5, 8), 20)
         print(f"bench: ensemble done {ens}", file=sys.stderr, flush=True)
+    adj = None
+    if not args.no_adjoint:
+        adj = run_adjoint(rbn, args.adj_window, args.adj_ckpt)
+        print(f"bench: adjoint done {adj}", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
         cpu = run_cpu_baseline(rbn, args.cpu_steps)
@@ -235,7 +348,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(),
             "kernel": "k_lbm32 (fused f/g collide+stream)",
             "bytes_per_launch": bytes_launch,
             "avg_launch_ms": round(r32["lbm_avg_ms"], 5),
@@ -249,7 +362,8 @@ def main() -> int:
             "roofline_frac": round(NX * NY * lbm_bytes_per_node(0) / (r64["lbm_avg_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
         },
         "ensemble": ens,
-        "finite": r32["finite"] and (r64 is None or r64["finite"]),
+        "adjoint": adj,
+        "finite": r32["finite"] and (r64 is None or r64["finite"]) and (adj is None or adj["finite"]),
     }
     print(json.dumps(line))
     return 0
