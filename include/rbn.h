@@ -213,6 +213,12 @@ int rbn_set_obs_provider(rbn_ctx *ctx, rbn_obs_fn fn, void *user);
 /* Nudging state of the latest step: nudge_vol [nx][ny] counts, tempreadint [nx][ny]
  * running means, nud_term [nx][ny]; any pointer may be NULL. */
 int rbn_get_nudging(rbn_ctx *ctx, int *nudge_vol, double *tempreadint, double *nud_term);
+/* Per-node deposit weight W [nx][ny] of the latest step's bilinear deposit
+ * (RBN_DEPOSIT_BILINEAR: each observer gives the four corners of its cell its bilinear
+ * interpolation weights, Tbar = sum w T / W, and the nudged collision uses
+ * Q = alpha * (W * (T - Tbar))); zero for the reference 7x7 deposit.  No reference
+ * counterpart: the bilinear deposit is an option beside the reference's running mean. */
+int rbn_get_nudging_weights(rbn_ctx *ctx, double *weight);
 
 /* ---------------------------------------------------------------- stepping */
 /* Diagnostics callback (evolution.dat line), every ndiag steps. */

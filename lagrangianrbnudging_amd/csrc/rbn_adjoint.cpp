@@ -56,6 +56,7 @@ struct AdjState {
   unsigned char *segFN = nullptr;     // K x 2 planes (F_in, nud_in)
   int *seg_nvol = nullptr;            // K x plane ints
   double *seg_tbar = nullptr;         // K x plane doubles
+  double *seg_w = nullptr;            // K x plane doubles (bilinear deposit weights)
   unsigned char *work = nullptr;      // F, nud, u, v planes
   int *rep_oi = nullptr;              // replayed observer state
   double *rep_od = nullptr;
@@ -69,7 +70,7 @@ void adj_free(rbn_ctx *c) {
   AdjState *a = c->adj;
   if (!a) return;
   void *ptrs[] = {a->ckpt, a->ckpt_oi, a->ckpt_od, a->rec, a->rec_chk, a->obs, a->maxbits, a->part,
-                  a->acc, a->init, a->seg, a->segFN, a->seg_nvol, a->seg_tbar, a->work, a->rep_oi,
+                  a->acc, a->init, a->seg, a->segFN, a->seg_nvol, a->seg_tbar, a->seg_w, a->work, a->rep_oi,
                   a->rep_od, a->lam, a->lamFN, a->lamT, a->dT0};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -169,9 +170,10 @@ int ensure_backward_storage(rbn_ctx *c) {
       (rc = alloc_n(c, &a->work, 4 * pb)) || (rc = alloc_n(c, &a->rep_oi, ONI * n)) ||
       (rc = alloc_n(c, &a->rep_od, OND * n)) || (rc = alloc_n(c, &a->lam, 2 * 18 * pb)) ||
       (rc = alloc_n(c, &a->lamFN, 2 * pb)) || (rc = alloc_n(c, &a->lamT, g.plane)) ||
-      (rc = alloc_n(c, &a->dT0, g.plane))) {
+      (rc = alloc_n(c, &a->dT0, g.plane)) ||
+      (c->dep.mode == RBN_DEPOSIT_BILINEAR && (rc = alloc_n(c, &a->seg_w, K * g.plane)))) {
     // keep the forward products; free the partial workspace
-    void **ws[] = {(void **)&a->seg, (void **)&a->segFN, (void **)&a->seg_nvol, (void **)&a->seg_tbar,
+    void **ws[] = {(void **)&a->seg, (void **)&a->segFN, (void **)&a->seg_nvol, (void **)&a->seg_tbar, (void **)&a->seg_w,
                    (void **)&a->work, (void **)&a->rep_oi, (void **)&a->rep_od, (void **)&a->lam,
                    (void **)&a->lamFN, (void **)&a->lamT, (void **)&a->dT0};
     for (void **p : ws) dfree(*p);
@@ -245,6 +247,7 @@ int rbn_adj_forward(rbn_ctx *c, int nsteps, int ckpt_every, double *J) {
     lp.nvol = c->dep.cnt;
     lp.Tbar = c->dep.Tbar;
     lp.nud = c->nud;
+    lp.wq = c->dep.mode == RBN_DEPOSIT_BILINEAR ? c->dep.wq : nullptr;
     if ((rc = timed(c, KT_LBM, s, [&] { return rbn::launch_lbm(s, c->prec, P.variant, g, lp, c->consts.data(), c->tune); })))
       return rc;
     std::swap(c->fA, c->fB);
@@ -314,6 +317,7 @@ int rbn_adj_backward(rbn_ctx *c, double *dJ_dT0, double *dJ_dalpha) {
       rbn::DepPtrs d = c->dep;
       d.cnt = a->seg_nvol + (size_t)t * g.plane;
       d.Tbar = a->seg_tbar + (size_t)t * g.plane;
+      d.wq = a->seg_w ? a->seg_w + (size_t)t * g.plane : nullptr;
       if ((rc = launch_step_kernels(c, istep, ro, d))) return rc;
       unsigned char *fn = plane_at(a->segFN, pb, 2 * (size_t)t);
       HIPCHK(c, hipMemcpyAsync(fn, wF, pb, hipMemcpyDeviceToDevice, s));
@@ -331,6 +335,7 @@ int rbn_adj_backward(rbn_ctx *c, double *dJ_dT0, double *dJ_dalpha) {
       lp.nvol = d.cnt;
       lp.Tbar = d.Tbar;
       lp.nud = wN;
+      lp.wq = d.wq;
       if ((rc = timed(c, KT_LBM, s, [&] { return rbn::launch_lbm(s, c->prec, P.variant, g, lp, c->consts.data(), c->tune); })))
         return rc;
     }
@@ -351,6 +356,7 @@ int rbn_adj_backward(rbn_ctx *c, double *dJ_dT0, double *dJ_dalpha) {
       args.nudin = fn + pb;
       args.nvol = a->seg_nvol + (size_t)t * g.plane;
       args.Tbar = a->seg_tbar + (size_t)t * g.plane;
+      args.wq = a->seg_w ? a->seg_w + (size_t)t * g.plane : nullptr;
       args.lam_next = have_next ? plane_at(a->lam, pb, 18 * (size_t)cur) : nullptr;
       args.lam = plane_at(a->lam, pb, 18 * (size_t)(1 - cur));
       args.lamF = lamF;

@@ -13,6 +13,7 @@ struct AdjStepArgs {
   const void *Fin, *nudin;  // lagged force and nud_term entering the step
   const int *nvol;          // deposit counts of the step
   const double *Tbar;       // deposit means of the step
+  const double *wq;         // bilinear deposit weights of the step (nullptr: weight 1)
   // Adjoint state.
   const void *lam_next;     // pulled-population adjoints of step s+1 (f: planes 0-8, g: 9-17),
                             // nullptr after the last step of the window

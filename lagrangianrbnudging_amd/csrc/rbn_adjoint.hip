@@ -2,10 +2,11 @@
 // observation operator.  No reference counterpart: the reference has no gradient.
 //
 // Forward map of step s at node n of the nudged run (F_x = 0, zero-gradient g walls; the
-// deposit counts nv and means Tbar are functions of the observation records only):
+// deposit counts nv, means Tbar and weights W -- 1 for the 7x7 deposit -- are functions of
+// the observation records only):
 //   pull f, g                                    rho = sum f_k,  S = sum g_k
 //   T = S + [nv>0] nud_in / 2                    u = j_x / rho,  v = (j_y + F_y/2) / rho
-//   F_out = g0 rho T                             Q = alpha (T - Tbar)
+//   F_out = g0 rho T                             Q = alpha W (T - Tbar)
 //   f'_k = (1-om) f_k + om w_k rho b_k + src_k fac_k
 //   g'_k = (1-omg) g_k + omg w_k T b_k - [nv>0] bw_k Q
 //   nud_out = [nv>0] ? -Q : nud_in
@@ -190,8 +191,9 @@ __global__ __launch_bounds__(kT) void k_adjoint(const AdjStepArgs a) {
       lu += lfac * (9.0 * CX[k] * CY[k]) * Fy;
     }
     if (nv > 0) {
-      lT += a.alpha * lQ;
-      dal = (T - a.Tbar[o]) * lQ;
+      const double W = a.wq ? a.wq[o] : 1.0;
+      lT += a.alpha * W * lQ;
+      dal = W * (T - a.Tbar[o]) * lQ;
     }
     const double ljx = ri * lu, ljy = ri * lv;
     lFy += 0.5 * ri * lv;

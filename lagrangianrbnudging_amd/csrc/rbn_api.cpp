@@ -403,6 +403,7 @@ void rbn_destroy(rbn_ctx *c) {
   dfree_t(c->dep.list);
   dfree_t(c->dep.blocksum);
   dfree_t(c->dep.Tbar);
+  dfree_t(c->dep.wq);
   dfree_t(c->dep.overflow);
   for (auto &p : c->pending) {
     (void)hipEventDestroy(p.a);
@@ -488,11 +489,13 @@ int rbn_create(const rbn_params *p, int device, rbn_ctx **out) {
                    {(void **)&c->dep.off, (g.plane + 1) * sizeof(int)},
                    {(void **)&c->dep.blocksum, ntiles * sizeof(int)},
                    {(void **)&c->dep.Tbar, g.plane * sizeof(double)},
+                   {(void **)&c->dep.wq, g.plane * sizeof(double)},
                    {(void **)&c->dep.overflow, sizeof(int)}};
     for (auto &a : dallocs) {
       if ((rc = dalloc(c, a.p, a.b))) goto fail_out;
       (void)hipMemsetAsync(*a.p, 0, a.b, c->stream);
     }
+    c->dep.mode = p->deposit;
   }
   c->consts.resize(std::max(sizeof(rbn::StepConst<double>), sizeof(rbn::StepConst<float>)));
   if (c->prec == RBN_FP64)
@@ -825,6 +828,21 @@ int rbn_set_obs_provider(rbn_ctx *c, rbn_obs_fn fn, void *user) {
   return RBN_OK;
 }
 
+int rbn_get_nudging_weights(rbn_ctx *c, double *weight) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (c->P.variant != RBN_VARIANT_NUDGING) return fail(c, RBN_ESTATE, "not a nudged run");
+  if (!weight) return fail(c, RBN_EINVAL, "null output");
+  const Geom &g = c->geom;
+  std::vector<double> h(g.plane, 0.0);
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (c->dep.mode == RBN_DEPOSIT_BILINEAR)
+    HIPCHK(c, hipMemcpy(h.data(), c->dep.wq, g.plane * sizeof(double), hipMemcpyDeviceToHost));
+  for (int i = 0; i < g.nx; i++)
+    for (int j = 0; j < g.ny; j++) weight[(size_t)i * g.ny + j] = h[(size_t)j * g.pitch + i];
+  return RBN_OK;
+}
+
 int rbn_get_nudging(rbn_ctx *c, int *nudge_vol, double *tempreadint, double *nud_term) {
   int rc = check_ctx(c);
   if (rc) return rc;
@@ -901,6 +919,7 @@ int rbn_step(rbn_ctx *c, int nsteps) {
     if (nudged) {
       lp.nvol = c->dep.cnt;
       lp.Tbar = c->dep.Tbar;
+      lp.wq = c->dep.mode == RBN_DEPOSIT_BILINEAR ? c->dep.wq : nullptr;
       lp.nud = c->nud;
     }
     if ((rc = timed(c, 0, c->stream, [&] {

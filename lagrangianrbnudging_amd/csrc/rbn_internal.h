@@ -53,6 +53,7 @@ struct LbmPtrs {
   const int *nvol;   // nudged run: nudge_vol counts (nullptr: not nudged)
   const double *Tbar;// nudged run: tempreadint
   void *nud;         // nudged run: persistent nud_term
+  const double *wq;  // nudged run, bilinear deposit: per-node weight W (Q = alpha*(W*(T-Tbar)))
 };
 
 // Particle kernel arguments (device pointers).
@@ -81,6 +82,8 @@ struct DepPtrs {
   int *list;   // contributions, grouped per node
   int *blocksum;
   double *Tbar;
+  double *wq;  // bilinear deposit: per-node weight sum (nullptr for the 7x7 deposit)
+  int mode;    // RBN_DEPOSIT_BOX7 or RBN_DEPOSIT_BILINEAR
   long long list_cap;
   int *overflow;
 };

@@ -16,7 +16,15 @@
 //   4. per node, sort its observer indices ascending and fold the running mean in that
 //      order — the reference's order, so the result is bit-identical.
 // Each observer contributes at most once per node when nx >= 8, so observer order is the
-// complete per-node order.
+// complete per-node order.  Count and fill run one thread per (observer, candidate node), so
+// the atomics of one observer are independent; the scan's tile offsets are added where the
+// ranges are used.
+//
+// Bilinear deposit (RBN_DEPOSIT_BILINEAR; no reference counterpart, an option beside the
+// reference's running mean): each observer gives the four corners of its interpolation cell
+// the weights the record interpolation uses, so the deposit is the transpose of that
+// operator; per node W = sum w_k and S = sum w_k T_k in observer order (same four passes),
+// Tbar = S / W, and the nudged collision uses Q = alpha * (W * (T - Tbar)).
 #include <algorithm>
 
 #include "rbn_internal.h"
@@ -112,25 +120,52 @@ __global__ void k_obs_update(ObsPtrs o, int branch, int nx, int ny, int ntemprea
 
 // ---------------------------------------------------------------- deposit
 
-// Visit the nodes observer k deposits into (N:1568-1616, global form), calling fn(offset)
-// for each interior node.
-template <typename Fn>
-__device__ __forceinline__ void visit(const Geom &g, int right, int xi, int yi, Fn &&fn) {
-  auto cell = [&](int i, int j) {
-    if ((i > 0) && (i <= g.nx) && (j > 0) && (j <= g.ny)) fn((size_t)(j - 1) * g.pitch + (i - 1));
-  };
-  for (int i1 = -3; i1 < 4; i1++)
-    for (int j1 = -3; j1 < 4; j1++) cell(xi + i1, yi + j1);
-  if (xi < 4) {
-    const int t = g.nx + xi;
-    for (int i1 = -3; i1 < 1; i1++)
-      for (int j1 = -3; j1 < 4; j1++) cell(t + i1, yi + j1);
+// The nodes observer k deposits into (N:1568-1616, global form): the 7x7 box at its rounded
+// position, plus 4 x 7 nodes of the x-wrap window when xi < 4 (columns nx+xi-3 .. nx+xi) or
+// xi > right (columns xi-nx .. xi-nx+3).  With nx >= 8 at most one window applies, so an
+// observer has at most kCells = 77 candidate nodes, and it reaches each node at most once.
+// cell_of(c) maps candidate c to the node offset, or -1 when the candidate is outside the
+// lattice or its window is inactive.  Work is spread as one thread per (observer, candidate)
+// so that the atomics of one observer are independent and their latencies overlap.
+constexpr int kCells = 77;
+
+__device__ __forceinline__ long long cell_of(const Geom &g, int right, int xi, int yi, int c) {
+  int i, j;
+  if (c < 49) {
+    i = xi + c / 7 - 3;
+    j = yi + c % 7 - 3;
+  } else {
+    const int w = c - 49, col = w / 7;
+    j = yi + w % 7 - 3;
+    if (xi < 4)
+      i = g.nx + xi + col - 3;
+    else if (xi > right)
+      i = xi - g.nx + col;
+    else
+      return -1;
   }
-  if (xi > right) {
-    const int t = xi - g.nx;
-    for (int i1 = 0; i1 < 4; i1++)
-      for (int j1 = -3; j1 < 4; j1++) cell(t + i1, yi + j1);
-  }
+  if ((i > 0) && (i <= g.nx) && (j > 0) && (j <= g.ny)) return (long long)(j - 1) * g.pitch + (i - 1);
+  return -1;
+}
+
+// Bilinear deposit (RBN_DEPOSIT_BILINEAR, no reference counterpart): observer k gives each
+// corner c of its interpolation cell the record-interpolation weight (M:292-311, same
+// operand order), ghost corners folded onto the interior node they copy; zero weights are
+// not contributions.  Returns the node offset and the weight, or -1.  The list key of a
+// contribution is 4k + c, so sorting keys sorts by observer.
+constexpr int kBilCells = 4;
+
+__device__ __forceinline__ long long bil_cell(const Geom &g, double xx, double yy, int c, double &w) {
+  if (!(xx > -1.0 && xx < (double)g.nx + 1.0 && yy > -1.0 && yy < (double)g.ny + 1.0)) return -1;
+  const int x1 = (int)xx, y1 = (int)yy, x2 = x1 + 1, y2 = y1 + 1;
+  const int ic = (c & 1) ? x2 : x1, jc = (c & 2) ? y2 : y1;
+  const double wx = (c & 1) ? (xx - (double)x1) : ((double)x2 - xx);
+  const double wy = (c & 2) ? (yy - (double)y1) : ((double)y2 - yy);
+  w = wx * wy;
+  if (w == 0.0) return -1;
+  const int I = ic == 0 ? g.nx : (ic == g.nx + 1 ? 1 : ic);
+  const int J = jc == 0 ? g.ny : (jc == g.ny + 1 ? 1 : jc);
+  return (long long)(J - 1) * g.pitch + (I - 1);
 }
 
 __global__ void k_dep_zero(size_t n, int *cnt, int *fill) {
@@ -141,10 +176,27 @@ __global__ void k_dep_zero(size_t n, int *cnt, int *fill) {
   }
 }
 
+// Node of candidate c of observer k for deposit MODE (-1: none); *key is the list key.
+template <int MODE>
+__device__ __forceinline__ long long candidate(const Geom &g, int right, const ObsPtrs &o, int k, int c,
+                                               int *key) {
+  if (MODE == RBN_DEPOSIT_BILINEAR) {
+    double w;
+    *key = 4 * k + c;
+    return bil_cell(g, o.xint[k], o.yint[k], c, w);
+  }
+  *key = k;
+  return cell_of(g, right, o.xidx[k], o.yidx[k], c);
+}
+
+template <int MODE>
 __global__ void k_dep_count(Geom g, int right, ObsPtrs o, int *cnt) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= o.n) return;
-  visit(g, right, o.xidx[k], o.yidx[k], [&](size_t off) { atomicAdd(&cnt[off], 1); });
+  constexpr int NC = MODE == RBN_DEPOSIT_BILINEAR ? kBilCells : kCells;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)o.n * NC) return;
+  int key;
+  const long long off = candidate<MODE>(g, right, o, (int)(t / NC), (int)(t % NC), &key);
+  if (off >= 0) atomicAdd(&cnt[off], 1);
 }
 
 // Exclusive scan, pass 1: tile-local scans and tile totals.
@@ -200,50 +252,100 @@ __global__ __launch_bounds__(kBlock) void k_scan_sums(int ntiles, int *tile_sum,
   if (threadIdx.x == 0) *total = carry;
 }
 
-__global__ void k_scan_add(size_t n, int *out, const int *tile_sum) {
-  const size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (o < n) out[o] += tile_sum[o / kScanTile];
+// Start of node c's slot range: tile-local exclusive prefix + the tile's exclusive prefix.
+__device__ __forceinline__ long long seg_start(const int *off, const int *tile_sum, size_t c) {
+  return (long long)off[c] + tile_sum[c / kScanTile];
 }
 
-__global__ void k_dep_fill(Geom g, int right, ObsPtrs o, const int *off, int *fill, int *list,
-                           long long cap, int *overflow) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= o.n) return;
-  visit(g, right, o.xidx[k], o.yidx[k], [&](size_t c) {
-    const long long slot = (long long)off[c] + atomicAdd(&fill[c], 1);
-    if (slot < cap)
-      list[slot] = k;
-    else
-      atomicOr(overflow, 1);
-  });
+template <int MODE>
+__global__ void k_dep_fill(Geom g, int right, ObsPtrs o, const int *off, const int *tile_sum,
+                           int *fill, int *list, long long cap, int *overflow) {
+  constexpr int NC = MODE == RBN_DEPOSIT_BILINEAR ? kBilCells : kCells;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)o.n * NC) return;
+  int key;
+  const long long node = candidate<MODE>(g, right, o, (int)(t / NC), (int)(t % NC), &key);
+  if (node < 0) return;
+  const long long slot = seg_start(off, tile_sum, (size_t)node) + atomicAdd(&fill[node], 1);
+  if (slot < cap)
+    list[slot] = key;
+  else
+    atomicOr(overflow, 1);
 }
 
-// Per node: order the contributing observers and fold the running mean (N:1578-1579).
-__global__ void k_dep_fold(Geom g, const int *cnt, const int *off, int *list, const double *Tk,
-                           int nobs, double *Tbar, long long cap) {
+// Per node: order the contributions by observer and fold them in that order, the
+// reference's order.  7x7 deposit: the running mean (N:1578-1579).  Bilinear deposit:
+// W = sum w_k, S = sum w_k T_k, Tbar = S / W.  Lists of up to kSortRegs entries are sorted in
+// registers (odd-even transposition network, static indices); longer ones in place.
+constexpr int kSortRegs = 16;
+
+template <int MODE>
+struct Fold {
+  const Geom &g;
+  const ObsPtrs &o;
+  double t = 0.0, S = 0.0, W = 0.0;
+  __device__ __forceinline__ void take(int a, int key) {
+    if (MODE == RBN_DEPOSIT_BILINEAR) {
+      const int k = key >> 2;
+      double w;
+      if (k < o.n && bil_cell(g, o.xint[k], o.yint[k], key & 3, w) >= 0) {
+        W = W + w;
+        S = S + w * o.Tint[k];
+      }
+    } else if (key >= 0 && key < o.n) {
+      t = (t * (double)a + o.Tint[key]) / ((double)a + 1.0);
+    }
+  }
+};
+
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_dep_fold(Geom g, const int *cnt, const int *off,
+                                                     const int *tile_sum, int *list, ObsPtrs o,
+                                                     double *Tbar, double *wq, long long cap) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y;
   if (i >= g.nx) return;
   const size_t c = (size_t)j * g.pitch + i;
   const int n = cnt[c];
-  double t = 0.0;
-  if (n > 0 && (long long)off[c] + n <= cap) {
-    int *seg = list + off[c];
-    for (int a = 1; a < n; a++) {  // insertion sort: segments are short
-      const int key = seg[a];
-      int b = a - 1;
-      while (b >= 0 && seg[b] > key) {
-        seg[b + 1] = seg[b];
-        b--;
+  Fold<MODE> f{g, o};
+  const long long base = n > 0 ? seg_start(off, tile_sum, c) : 0;
+  if (n > 0 && base + n <= cap) {
+    int *seg = list + base;
+    if (n <= kSortRegs) {
+      int key[kSortRegs];
+#pragma unroll
+      for (int a = 0; a < kSortRegs; a++) key[a] = a < n ? seg[a] : 0x7fffffff;
+#pragma unroll
+      for (int pass = 0; pass < kSortRegs; pass++) {
+#pragma unroll
+        for (int a = pass & 1; a + 1 < kSortRegs; a += 2) {
+          const int lo = min(key[a], key[a + 1]), hi = max(key[a], key[a + 1]);
+          key[a] = lo;
+          key[a + 1] = hi;
+        }
       }
-      seg[b + 1] = key;
-    }
-    for (int a = 0; a < n; a++) {
-      const int k = seg[a];
-      if (k >= 0 && k < nobs) t = (t * (double)a + Tk[k]) / ((double)a + 1.0);
+#pragma unroll
+      for (int a = 0; a < kSortRegs; a++)
+        if (a < n) f.take(a, key[a]);
+    } else {
+      for (int a = 1; a < n; a++) {  // insertion sort in place
+        const int key = seg[a];
+        int b = a - 1;
+        while (b >= 0 && seg[b] > key) {
+          seg[b + 1] = seg[b];
+          b--;
+        }
+        seg[b + 1] = key;
+      }
+      for (int a = 0; a < n; a++) f.take(a, seg[a]);
     }
   }
-  Tbar[c] = t;
+  if (MODE == RBN_DEPOSIT_BILINEAR) {
+    Tbar[c] = f.W != 0.0 ? f.S / f.W : 0.0;
+    wq[c] = f.W;
+  } else {
+    Tbar[c] = f.t;
+  }
 }
 
 }  // namespace
@@ -262,17 +364,32 @@ hipError_t launch_deposit(hipStream_t s, const Geom &g, int right_limit, const O
   const int ntiles = (int)((n + kScanTile - 1) / kScanTile);
   const int zb = (int)std::min<size_t>((n + kBlock - 1) / kBlock, 4096);
   hipLaunchKernelGGL(k_dep_zero, dim3(zb), dim3(kBlock), 0, s, n, d.cnt, d.fill);
-  const dim3 ob((o.n + kBlock - 1) / kBlock > 0 ? (o.n + kBlock - 1) / kBlock : 1);
-  if (o.n > 0) hipLaunchKernelGGL(k_dep_count, ob, dim3(kBlock), 0, s, g, right_limit, o, d.cnt);
+  const bool bil = d.mode == RBN_DEPOSIT_BILINEAR;
+  const long long work = (long long)o.n * (bil ? kBilCells : kCells);
+  const dim3 ob((unsigned)((work + kBlock - 1) / kBlock > 0 ? (work + kBlock - 1) / kBlock : 1));
+  const dim3 nb((g.nx + kBlock - 1) / kBlock, g.ny);
+  if (o.n > 0) {
+    if (bil)
+      hipLaunchKernelGGL(k_dep_count<RBN_DEPOSIT_BILINEAR>, ob, dim3(kBlock), 0, s, g, right_limit, o, d.cnt);
+    else
+      hipLaunchKernelGGL(k_dep_count<RBN_DEPOSIT_BOX7>, ob, dim3(kBlock), 0, s, g, right_limit, o, d.cnt);
+  }
   hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(kBlock), 0, s, n, d.cnt, d.off, d.blocksum);
   hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kBlock), 0, s, ntiles, d.blocksum, d.off + n);
-  hipLaunchKernelGGL(k_scan_add, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
-                     n, d.off, d.blocksum);
-  if (o.n > 0)
-    hipLaunchKernelGGL(k_dep_fill, ob, dim3(kBlock), 0, s, g, right_limit, o, d.off, d.fill,
-                       d.list, d.list_cap, d.overflow);
-  hipLaunchKernelGGL(k_dep_fold, dim3((g.nx + kBlock - 1) / kBlock, g.ny), dim3(kBlock), 0, s, g,
-                     d.cnt, d.off, d.list, o.Tint, o.n, d.Tbar, d.list_cap);
+  if (o.n > 0) {
+    if (bil)
+      hipLaunchKernelGGL(k_dep_fill<RBN_DEPOSIT_BILINEAR>, ob, dim3(kBlock), 0, s, g, right_limit, o, d.off,
+                         d.blocksum, d.fill, d.list, d.list_cap, d.overflow);
+    else
+      hipLaunchKernelGGL(k_dep_fill<RBN_DEPOSIT_BOX7>, ob, dim3(kBlock), 0, s, g, right_limit, o, d.off,
+                         d.blocksum, d.fill, d.list, d.list_cap, d.overflow);
+  }
+  if (bil)
+    hipLaunchKernelGGL(k_dep_fold<RBN_DEPOSIT_BILINEAR>, nb, dim3(kBlock), 0, s, g, d.cnt, d.off, d.blocksum,
+                       d.list, o, d.Tbar, d.wq, d.list_cap);
+  else
+    hipLaunchKernelGGL(k_dep_fold<RBN_DEPOSIT_BOX7>, nb, dim3(kBlock), 0, s, g, d.cnt, d.off, d.blocksum,
+                       d.list, o, d.Tbar, d.wq, d.list_cap);
   return hipGetLastError();
 }
 

@@ -400,8 +400,8 @@ int rbn_params_check(const rbn_params *p) {
   if (p->variant != RBN_VARIANT_TRAJECTORIES && p->variant != RBN_VARIANT_NUDGING)
     return set_err(RBN_EINVAL, "unknown variant");
   if (p->noutconfig < 1) return set_err(RBN_EINVAL, "noutconfig must be >= 1");
-  if (p->deposit != RBN_DEPOSIT_BOX7)
-    return set_err(RBN_EINVAL, "only the reference 7x7 deposit (RBN_DEPOSIT_BOX7) is implemented");
+  if (p->deposit != RBN_DEPOSIT_BOX7 && p->deposit != RBN_DEPOSIT_BILINEAR)
+    return set_err(RBN_EINVAL, "deposit must be RBN_DEPOSIT_BOX7 or RBN_DEPOSIT_BILINEAR");
   if (p->variant == RBN_VARIANT_NUDGING) {
     if (p->ntempread < 2) return set_err(RBN_EINVAL, "ntempread must be >= 2");
     if (p->contin != 0) return set_err(RBN_EINVAL, "contin = 1 (restart of a nudged run) is not supported");

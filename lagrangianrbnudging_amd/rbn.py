@@ -19,6 +19,8 @@ RBN_VARIANT_TRAJECTORIES = 0
 RBN_VARIANT_NUDGING = 1
 RBN_FP64 = 0
 RBN_FP32 = 1
+RBN_DEPOSIT_BOX7 = 0
+RBN_DEPOSIT_BILINEAR = 1
 
 STATUS = {
     0: "RBN_OK", -1: "RBN_EINVAL", -2: "RBN_EIO", -3: "RBN_EPARSE", -4: "RBN_EDEVICE",
@@ -112,6 +114,7 @@ _SIGS = {
     "rbn_init_observers": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), _DP, _DP, _DP]),
     "rbn_set_obs_provider": (C.c_int, [C.c_void_p, OBS_FN, C.c_void_p]),
     "rbn_get_nudging": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), _DP, _DP]),
+    "rbn_get_nudging_weights": (C.c_int, [C.c_void_p, _DP]),
     "rbn_set_diag_callback": (C.c_int, [C.c_void_p, DIAG_FN, C.c_void_p]),
     "rbn_step": (C.c_int, [C.c_void_p, C.c_int]),
     "rbn_get_step": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
@@ -376,7 +379,9 @@ class Solver:
         tb = np.empty((self.nx, self.ny))
         nd = np.empty((self.nx, self.ny))
         self._c(self.lib.rbn_get_nudging(self.ctx, nv.ctypes.data_as(C.POINTER(C.c_int)), _dp(tb), _dp(nd)))
-        return {"nudge_vol": nv, "tempreadint": tb, "nud_term": nd}
+        w = np.empty((self.nx, self.ny))
+        self._c(self.lib.rbn_get_nudging_weights(self.ctx, _dp(w)))
+        return {"nudge_vol": nv, "tempreadint": tb, "nud_term": nd, "weight": w}
 
     # -- stepping
     def step(self, n: int = 1) -> None:

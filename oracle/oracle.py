@@ -43,6 +43,8 @@ _SIGS = {
     "or_equilibrium": (None, [C.c_double, C.c_double, C.c_double, _DP]),
     "or_bilinear_global": (C.c_double, [C.c_void_p, C.c_int, C.c_double, C.c_double, _IP]),
     "or_deposit_only": (C.c_int, [C.c_void_p, C.c_int, _IP, _IP, _DP]),
+    "or_deposit_bilinear_only": (C.c_int, [C.c_void_p, C.c_int, _DP, _DP, _DP]),
+    "or_get_nudging_weights": (C.c_int, [C.c_void_p, _DP]),
     "or_step": (C.c_int, [C.c_void_p, C.c_int]),
     "or_get_step": (C.c_int, [C.c_void_p]),
     "or_num_diag": (C.c_int, [C.c_void_p]),
@@ -205,7 +207,9 @@ class Oracle:
         tb = np.empty((self.nx, self.ny))
         nd = np.empty((self.nx, self.ny))
         self._c(self.lib.or_get_nudging(self.sim, nv.ctypes.data_as(_IP), _dp(tb), _dp(nd)))
-        return {"nudge_vol": nv, "tempreadint": tb, "nud_term": nd}
+        w = np.empty((self.nx, self.ny))
+        self._c(self.lib.or_get_nudging_weights(self.sim, _dp(w)))
+        return {"nudge_vol": nv, "tempreadint": tb, "nud_term": nd, "weight": w}
 
     def observer_state(self):
         n = self.lib.or_get_observer_state(self.sim, None, None, None, None, None)
@@ -223,6 +227,10 @@ class Oracle:
         yidx = np.ascontiguousarray(yidx, dtype=np.int32)
         self._c(self.lib.or_deposit_only(self.sim, len(xidx), xidx.ctypes.data_as(_IP),
                                          yidx.ctypes.data_as(_IP), _dp(_f64(T))))
+
+    def deposit_bilinear_only(self, x, y, T):
+        x, y, T = (_f64(a) for a in (x, y, T))
+        self._c(self.lib.or_deposit_bilinear_only(self.sim, len(x), _dp(x), _dp(y), _dp(T)))
 
     def bilinear(self, field: str, x: float, y: float) -> float:
         err = C.c_int(0)

@@ -70,6 +70,7 @@ struct or_sim {
   /* nudging */
   int *nudge_vol;
   double *tempreadint; /* [NX*NY], idx (j-1)+NY*(i-1) as N:1563 */
+  double *nudge_w;     /* [NX*NY]: bilinear deposit weight per node (RBN_DEPOSIT_BILINEAR) */
   double *nud_term;
   int nobs;
   int *oid, *chk, *bcy, *xidx, *yidx;
@@ -121,8 +122,9 @@ or_sim *or_create(const rbn_params *p) {
   for (size_t k = 0; k < sizeof fields / sizeof fields[0]; k++) *fields[k] = (double *)calloc(nn, sizeof(double));
   s->nudge_vol = (int *)calloc(nn, sizeof(int));
   s->tempreadint = (double *)calloc((size_t)p->nx * (size_t)p->ny, sizeof(double));
+  s->nudge_w = (double *)calloc((size_t)p->nx * (size_t)p->ny, sizeof(double));
   if (!s->f || !s->g || !s->feq || !s->geq || !s->memup1 || !s->memdown1 || !s->nud_term ||
-      !s->nudge_vol || !s->tempreadint) {
+      !s->nudge_vol || !s->tempreadint || !s->nudge_w) {
     or_destroy(s);
     return NULL;
   }
@@ -155,7 +157,7 @@ void or_destroy(or_sim *s) {
                   s->ubef,  s->vbef,  s->dtu,     s->dtv,   s->dxu,    s->dxv,
                   s->dyu,   s->dyv,   s->upost,   s->vpost, s->rhopost, s->Tpost,
                   s->utot,  s->vtot,  s->tau,     s->taug,  s->nud_term, s->nudge_vol,
-                  s->tempreadint, s->dstep, s->de, s->dt};
+                  s->tempreadint, s->nudge_w, s->dstep, s->de, s->dt};
   for (size_t k = 0; k < sizeof ptrs / sizeof ptrs[0]; k++) free(ptrs[k]);
   free_particles(s);
   free_observers(s);
@@ -367,7 +369,9 @@ static void or_collide_g(or_sim *s, int nudged) {
       if (nudged && s->nudge_vol[id] > 0) {
         size_t ir = (size_t)(j - 1) + (size_t)s->NY * (size_t)(i - 1);
         double B = 1.0 - 0.5 * om;
-        double Q = alpha * (s->T[id] - s->tempreadint[ir]);
+        double Q = s->P.deposit == RBN_DEPOSIT_BILINEAR
+                       ? alpha * (s->nudge_w[ir] * (s->T[id] - s->tempreadint[ir]))
+                       : alpha * (s->T[id] - s->tempreadint[ir]);
         for (int k = 0; k < NPOP; k++)
           s->g[id].p[k] = s->g[id].p[k] * (1. - om) + om * s->geq[id].p[k] - B * W[k] * Q;
         s->nud_term[id] = -1. * Q;
@@ -578,6 +582,54 @@ int or_deposit_only(or_sim *s, int n, const int *xidx, const int *yidx, const do
   return 0;
 }
 
+/* Bilinear deposit (RBN_DEPOSIT_BILINEAR; no reference counterpart): observer k at the
+ * continuous position (x_k, y_k) gives each of the four corners of its interpolation cell the
+ * bilinear weight that the record interpolation (M:292-311) gives that corner, with ghost
+ * corners folded onto the interior node they copy (periodic in x and y, as communication).
+ * It is the transpose of the interpolation operator.  Per node, in observer order:
+ *   W = sum w_k,  S = sum w_k T_k,  tempreadint = S / W,  nudge_vol = number of observers
+ * with w_k != 0.  The nudged collision then uses Q = alpha * (W * (T - tempreadint)), i.e.
+ * sum_k alpha w_k (T - T_k).  Corners are visited in the order (x1,y1), (x2,y1), (x1,y2),
+ * (x2,y2); an observer reaches a node at most once, so the per-node order is observer order. */
+static void or_deposit_bilinear(or_sim *s, int n, const double *x, const double *y, const double *Tk) {
+  const int NX = s->NX, NY = s->NY;
+  memset(s->nudge_vol, 0, s->nn * sizeof(int));
+  memset(s->tempreadint, 0, (size_t)NX * (size_t)NY * sizeof(double));
+  memset(s->nudge_w, 0, (size_t)NX * (size_t)NY * sizeof(double));
+  for (int k = 0; k < n; k++) {
+    const double xx = x[k], yy = y[k];
+    if (!(xx > -1.0 && xx < (double)NX + 1.0 && yy > -1.0 && yy < (double)NY + 1.0)) continue;
+    const int x1 = (int)xx, y1 = (int)yy, x2 = x1 + 1, y2 = y1 + 1;
+    const int ci[4] = {x1, x2, x1, x2}, cj[4] = {y1, y1, y2, y2};
+    const double w[4] = {((double)x2 - xx) * ((double)y2 - yy), (xx - (double)x1) * ((double)y2 - yy),
+                         ((double)x2 - xx) * (yy - (double)y1), (xx - (double)x1) * (yy - (double)y1)};
+    for (int c = 0; c < 4; c++) {
+      if (w[c] == 0.0) continue;
+      const int i = ci[c] == 0 ? NX : (ci[c] == NX + 1 ? 1 : ci[c]);
+      const int j = cj[c] == 0 ? NY : (cj[c] == NY + 1 ? 1 : cj[c]);
+      const size_t id = IX(s, i, j), ir = (size_t)(j - 1) + (size_t)NY * (size_t)(i - 1);
+      s->nudge_w[ir] = s->nudge_w[ir] + w[c];
+      s->tempreadint[ir] = s->tempreadint[ir] + w[c] * Tk[k];
+      s->nudge_vol[id] += 1;
+    }
+  }
+  for (size_t ir = 0; ir < (size_t)NX * (size_t)NY; ir++)
+    if (s->nudge_w[ir] != 0.0) s->tempreadint[ir] = s->tempreadint[ir] / s->nudge_w[ir];
+}
+
+int or_deposit_bilinear_only(or_sim *s, int n, const double *x, const double *y, const double *T) {
+  or_deposit_bilinear(s, n, x, y, T);
+  return 0;
+}
+
+/* The deposit of the configured kind for the current observer state. */
+static void or_deposit_current(or_sim *s) {
+  if (s->P.deposit == RBN_DEPOSIT_BILINEAR)
+    or_deposit_bilinear(s, s->nobs, s->xint, s->yint, s->Tint);
+  else
+    or_deposit(s, s->nobs, s->xidx, s->yidx, s->Tint);
+}
+
 static const or_obsrec *find_obs(const or_sim *s, int step) {
   for (int r = 0; r < s->norec; r++)
     if (s->orecs[r].step == step) return &s->orecs[r];
@@ -666,7 +718,7 @@ static int or_read_obs(or_sim *s) {
       s->dT[i] = (Tread - s->Tint[i]) / ((double)ntr);
       advance_y_T(s, i, rec->T);
     }
-    or_deposit(s, n, s->xidx, s->yidx, s->Tint);
+    or_deposit_current(s);
     for (int k = 0; k < n; k++) {
       s->prx[k] = rec->x[k];
       s->pry[k] = rec->y[k];
@@ -680,13 +732,13 @@ static int or_read_obs(or_sim *s) {
       s->xidx[i] = (int)round(s->prx[s->chk[i]]);
       s->yidx[i] = (int)round(s->pry[s->chk[i]]);
     }
-    or_deposit(s, n, s->xidx, s->yidx, s->Tint);
+    or_deposit_current(s);
   } else {
     for (int i = 0; i < n; i++) {
       advance_x(s, i);
       advance_y_T(s, i, s->prT);
     }
-    or_deposit(s, n, s->xidx, s->yidx, s->Tint);
+    or_deposit_current(s);
   }
   return 0;
 }
@@ -938,6 +990,13 @@ int or_add_obs_record(or_sim *s, int step, int n, const int *id, const double *x
   memcpy(r->x, x, (size_t)n * sizeof(double));
   memcpy(r->y, y, (size_t)n * sizeof(double));
   memcpy(r->T, T, (size_t)n * sizeof(double));
+  return 0;
+}
+
+int or_get_nudging_weights(const or_sim *s, double *w) {
+  for (int i = 1; i < s->NX + 1; i++)
+    for (int j = 1; j < s->NY + 1; j++)
+      w[(size_t)(i - 1) * s->NY + (size_t)(j - 1)] = s->nudge_w[(size_t)(j - 1) + (size_t)s->NY * (size_t)(i - 1)];
   return 0;
 }
 

@@ -75,6 +75,10 @@ void or_equilibrium(double r1, double u, double v, double out9[9]);
 double or_bilinear_global(const or_sim *s, int field, double x, double y, int *err);
 /* Run only the deposit for given rounded positions and temperatures (observer order). */
 int or_deposit_only(or_sim *s, int n, const int *xidx, const int *yidx, const double *T);
+/* Run only the bilinear deposit (RBN_DEPOSIT_BILINEAR) for given continuous positions. */
+int or_deposit_bilinear_only(or_sim *s, int n, const double *x, const double *y, const double *T);
+/* Per-node bilinear deposit weights W ([nx][ny]; zero for the 7x7 deposit). */
+int or_get_nudging_weights(const or_sim *s, double *w);
 
 int or_step(or_sim *s, int nsteps);
 int or_get_step(const or_sim *s);

@@ -31,7 +31,8 @@ OPP = [0, 3, 4, 1, 2, 7, 8, 5, 6]
 
 
 def nudging_data(p, T0, obs, records, nsteps):
-    """Per step of a nudged oracle run: deposit counts/means and, at observation steps
+    """Per step of a nudged oracle run: deposit counts/means (and weights of the bilinear
+    deposit) and, at observation steps
     (istep % ntempread == 0), the observer positions and temperatures (observer order) and
     the oracle's temperature field; also the oracle's final populations."""
     o = Oracle(p)
@@ -43,7 +44,8 @@ def nudging_data(p, T0, obs, records, nsteps):
         for _ in range(nsteps):
             o.step(1)
             nd = o.nudging()
-            d = {"nvol": nd["nudge_vol"].copy(), "tbar": nd["tempreadint"].copy()}
+            d = {"nvol": nd["nudge_vol"].copy(), "tbar": nd["tempreadint"].copy(),
+                 "w": nd["weight"].copy() if p.deposit == 1 else None}
             if o.istep % p.ntempread == 0:
                 s = o.observer_state()
                 d["obs"] = (s["xint"], s["yint"], s["Tint"])
@@ -129,7 +131,8 @@ class NudgedStep:
         v = (jy + 0.5 * F) / rho
         usq = u * u + v * v
         Fn = self.g0 * rho * T
-        Q = alpha * (T - data["tbar"])
+        w = data.get("w")
+        Q = alpha * (T - data["tbar"]) if w is None else alpha * (w * (T - data["tbar"]))
         fo, go = np.empty_like(fp), np.empty_like(gp)
         for k in range(9):
             cu = CX[k] * u + CY[k] * v

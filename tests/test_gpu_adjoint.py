@@ -15,26 +15,31 @@ pytestmark = pytest.mark.gpu
 NX, NY, NTR, NOBS, W, K = 48, 24, 4, 60, 41, 8
 
 
-def _params(prec=rbn.RBN_FP64, alpha=0.3):
+def _params(prec=rbn.RBN_FP64, alpha=0.3, deposit=rbn.RBN_DEPOSIT_BOX7):
     p = rb_params(NX, NY, 1e5)
     p.variant = rbn.RBN_VARIANT_NUDGING
     p.ntempread = NTR
     p.alpha = alpha
     p.precision = prec
+    p.deposit = deposit
     return p
 
 
-@pytest.fixture(scope="module")
-def case():
-    p = _params()
+def _case(deposit):
+    p = _params(deposit=deposit)
     ids, x0, y0, T, recs = synthetic_observations(NX, NY, NOBS, W + NTR, NTR, seed=8)
     T0 = conduction_T(p, 1e-2, seed=4)
     data, _ = tlm.nudging_data(p, T0, (ids, x0, y0, T), recs, W)
     return {"p": p, "obs": (ids, x0, y0, T), "recs": recs, "T0": T0, "data": data}
 
 
+@pytest.fixture(scope="module")
+def case():
+    return _case(rbn.RBN_DEPOSIT_BOX7)
+
+
 def _solver(case, prec=rbn.RBN_FP64, alpha=None):
-    p = _params(prec, case["p"].alpha if alpha is None else alpha)
+    p = _params(prec, case["p"].alpha if alpha is None else alpha, case["p"].deposit)
     s = rbn.Solver(p)
     s.set_observation_source(case["recs"])
     return s
@@ -83,6 +88,22 @@ def test_gradient_matches_tangent_linear_reference(case):
         assert abs(g[i, j] - ref) <= 1e-10 * scale, ((i, j), g[i, j], ref)
     _, ref_a = tlm.directional_derivative(case["p"], case["T0"], case["p"].alpha, case["data"],
                                           np.zeros_like(g), 1.0)
+    assert abs(ga - ref_a) <= 1e-10 * abs(ref_a), (ga, ref_a)
+
+
+def test_bilinear_deposit_gradient_matches_tangent_linear_reference():
+    c = _case(rbn.RBN_DEPOSIT_BILINEAR)
+    s = _solver(c)
+    J = _J(s, c, c["T0"])
+    Jt, _ = tlm.misfit(c["p"], c["T0"], c["p"].alpha, c["data"])
+    assert abs(J - Jt) <= 1e-12 * Jt
+    g, ga = s.adj_backward()
+    rng = np.random.default_rng(15)
+    for _ in range(3):
+        d = rng.standard_normal(g.shape)
+        _, ref = tlm.directional_derivative(c["p"], c["T0"], c["p"].alpha, c["data"], d)
+        assert abs(float(np.sum(g * d)) - ref) <= 1e-10 * abs(ref), (float(np.sum(g * d)), ref)
+    _, ref_a = tlm.directional_derivative(c["p"], c["T0"], c["p"].alpha, c["data"], np.zeros_like(g), 1.0)
     assert abs(ga - ref_a) <= 1e-10 * abs(ref_a), (ga, ref_a)
 
 

@@ -77,6 +77,45 @@ def test_nudged_deposit_edges_fp64(fix_q3):
     _compare(g, o)
 
 
+@pytest.mark.parametrize("alpha", [0.1, 1.0])
+def test_bilinear_deposit_run_fp64_bit_exact(alpha):
+    pn = rb_params(40, 24, 1e5)
+    pn.variant = rbn.RBN_VARIANT_NUDGING
+    pn.ntempread = 4
+    pn.alpha = alpha
+    pn.deposit = rbn.RBN_DEPOSIT_BILINEAR
+    ids, x0, y0, T, recs = synthetic_observations(pn.nx, pn.ny, 80, 120, 4, seed=9)
+    T0n = conduction_T(pn, 1e-2, seed=4)
+    g, o = _nudged_pair(pn, T0n, ids, x0, y0, T, recs, 97)
+    _compare(g, o)
+    a, b = g.nudging(), o.nudging()
+    assert_equal_arrays(a["weight"], b["weight"], "weight")
+    assert np.count_nonzero(a["weight"]) > 0 and np.all(a["weight"][a["nudge_vol"] == 0] == 0)
+
+
+def test_bilinear_deposit_fp32_against_fp64():
+    pn = rb_params(40, 24, 1e5)
+    pn.variant = rbn.RBN_VARIANT_NUDGING
+    pn.ntempread = 4
+    pn.alpha = 0.3
+    pn.deposit = rbn.RBN_DEPOSIT_BILINEAR
+    ids, x0, y0, T, recs = synthetic_observations(pn.nx, pn.ny, 80, 120, 4, seed=9)
+    T0n = conduction_T(pn, 1e-2, seed=4)
+    out = {}
+    for prec in (rbn.RBN_FP64, rbn.RBN_FP32):
+        q = pn.copy()
+        q.precision = prec
+        s = rbn.Solver(q)
+        s.init_fields(T=T0n)
+        s.init_observers(ids, x0, y0, T)
+        s.set_observation_source(recs)
+        s.step(97)
+        out[prec] = (s.fields()["T"], s.nudging())
+    assert np.max(np.abs(out[rbn.RBN_FP32][0] - out[rbn.RBN_FP64][0])) < 1e-5
+    # observer positions are FP64 data in both modes: identical deposits
+    assert_equal_arrays(out[rbn.RBN_FP32][1]["weight"], out[rbn.RBN_FP64][1]["weight"], "weight")
+
+
 def test_missing_observer_id_is_an_error():
     pn = rb_params(32, 16, 1e5)
     pn.variant = rbn.RBN_VARIANT_NUDGING

@@ -338,6 +338,96 @@ def test_deposit_q3_fix_restores_periodicity():
     assert got == {((nx - 2 + d - 1) % nx) + 1 for d in range(-3, 4)}
 
 
+def _bil_oracle(nx=32, ny=16, **kw):
+    p = params(nx, ny, variant=1, ntempread=4, deposit=rbn.RBN_DEPOSIT_BILINEAR, **kw)
+    o = Oracle(p)
+    o.init_fields(T=np.zeros((nx, ny)))
+    return p, o
+
+
+def test_bilinear_deposit_single_observer_gives_the_interpolation_weights():
+    p, o = _bil_oracle()
+    x, y, T = 10.3, 8.6, 0.7
+    o.deposit_bilinear_only([x], [y], [T])
+    nd = o.nudging()
+    w = np.zeros((p.nx, p.ny))
+    # 1-based corners (10,8), (11,8), (10,9), (11,9) with the record-interpolation weights
+    w[9, 7], w[10, 7] = (11 - x) * (9 - y), (x - 10) * (9 - y)
+    w[9, 8], w[10, 8] = (11 - x) * (y - 8), (x - 10) * (y - 8)
+    assert np.array_equal(nd["weight"], w)
+    assert np.array_equal(nd["nudge_vol"], (w > 0).astype(np.int32))
+    assert np.max(np.abs(nd["tempreadint"][w > 0] - T)) <= 1e-15
+    assert np.all(nd["tempreadint"][w == 0] == 0.0)
+    # a node on a lattice line: zero weights are not contributions
+    o.deposit_bilinear_only([12.0], [5.0], [0.4])
+    nd = o.nudging()
+    assert nd["nudge_vol"].sum() == 1 and nd["weight"][11, 4] == 1.0 and nd["tempreadint"][11, 4] == 0.4
+
+
+def test_bilinear_deposit_folds_ghost_corners_periodically():
+    p, o = _bil_oracle()
+    x, y = 0.25, 15.5  # corner column 0 is the ghost copy of column nx
+    o.deposit_bilinear_only([x], [y], [1.0])
+    w = o.nudging()["weight"]
+    assert w[p.nx - 1, 14] == (1 - x) * (16 - y) and w[0, 14] == x * (16 - y)
+    assert w[p.nx - 1, 15] == (1 - x) * (y - 15) and w[0, 15] == x * (y - 15)
+    assert np.count_nonzero(w) == 4
+
+
+def test_bilinear_deposit_conserves_weight_and_heat():
+    p, o = _bil_oracle(40, 24)
+    rng = np.random.default_rng(3)
+    n = 200
+    x, y, T = rng.uniform(0, p.nx, n), rng.uniform(1, p.ny, n), rng.uniform(0, 1, n)
+    o.deposit_bilinear_only(x, y, T)
+    nd = o.nudging()
+    assert abs(nd["weight"].sum() - n) < 1e-11
+    assert abs(np.sum(nd["weight"] * nd["tempreadint"]) - T.sum()) < 1e-11
+
+
+def test_bilinear_deposit_sums_in_observer_order():
+    p, o = _bil_oracle()
+    x = np.array([10.2, 10.9, 10.55])
+    y = np.array([8.3, 8.8, 8.05])
+    T = np.array([0.1, 0.7, 0.3])
+    o.deposit_bilinear_only(x, y, T)
+    nd = o.nudging()
+    S = W_ = 0.0
+    for k in range(3):  # node (11, 9) is corner (x2, y2) of every observer
+        w = (x[k] - 10) * (y[k] - 8)
+        W_ = W_ + w
+        S = S + w * T[k]
+    assert nd["weight"][10, 8] == W_ and nd["tempreadint"][10, 8] == S / W_
+
+
+def test_bilinear_nudging_recurrence_with_quarter_weights():
+    # Observers at the centres of alternate cells give every node exactly one contribution of
+    # weight 1/4, so Q_n = alpha * (1/4) * (T_n - T_obs) everywhere.
+    nx, ny, alpha, T_init, T_obs = 24, 12, 0.2, 1.0, 0.5
+    p = params(nx, ny, variant=1, ntempread=4, alpha=alpha, relaxg=0.8, Tup=T_init, Tdown=T_init,
+               deposit=rbn.RBN_DEPOSIT_BILINEAR)
+    o = Oracle(p)
+    o.init_fields(T=np.full((nx, ny), T_init))
+    gx, gy = np.meshgrid(np.arange(1, nx + 1, 2) + 0.5, np.arange(1, ny + 1, 2) + 0.5, indexing="ij")
+    ids = np.arange(gx.size, dtype=np.int32)
+    xs, ys = gx.ravel(), gy.ravel()
+    Ts = np.full(ids.size, T_obs)
+    o.init_observers(ids, xs, ys, Ts)
+    o.set_observation_source({s: {"id": ids, "x": xs, "y": ys, "temp": Ts} for s in range(4, 100, 4)})
+    w = 1 / p.relaxg
+    S, Qp = T_init, 0.0
+    for n in range(1, 30):
+        o.step(1)
+        nd = o.nudging()
+        assert np.all(nd["weight"] == 0.25) and np.all(nd["nudge_vol"] == 1)
+        T = S - 0.5 * Qp
+        Q = alpha * (0.25 * (T - T_obs))
+        assert np.allclose(o.fields()["T"], T, rtol=0, atol=1e-14), n
+        assert np.allclose(nd["nud_term"], -Q, rtol=0, atol=1e-15)
+        S = (1 - w) * S + w * T - (1 - 0.5 * w) * Q
+        Qp = Q
+
+
 def test_observer_interpolation_is_linear_in_time_and_snaps():
     nx, ny = 40, 20
     p = params(nx, ny, variant=1, ntempread=5, alpha=0.0)

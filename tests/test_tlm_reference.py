@@ -10,11 +10,12 @@ from oracle import tlm_nudged as tlm
 from scenarios import conduction_T, rb_params, synthetic_observations
 
 
-def _scenario(nx=40, ny=24, ntr=4, alpha=0.3, nobs=60, steps=41, seed=8):
+def _scenario(nx=40, ny=24, ntr=4, alpha=0.3, nobs=60, steps=41, seed=8, deposit=rbn.RBN_DEPOSIT_BOX7):
     p = rb_params(nx, ny, 1e5)
     p.variant = rbn.RBN_VARIANT_NUDGING
     p.ntempread = ntr
     p.alpha = alpha
+    p.deposit = deposit
     ids, x0, y0, T, recs = synthetic_observations(nx, ny, nobs, steps + ntr, ntr, seed=seed)
     T0 = conduction_T(p, 1e-2, seed=4)
     data, pops = tlm.nudging_data(p, T0, (ids, x0, y0, T), recs, steps)
@@ -35,6 +36,14 @@ def test_restatement_matches_oracle_populations(scenario):
     assert np.max(np.abs(f - f_or)) < 1e-13
     assert np.max(np.abs(g - g_or)) < 1e-13
     assert np.isfinite(J) and J > 0
+
+
+def test_restatement_matches_oracle_with_the_bilinear_deposit():
+    p, T0, data, (f_or, g_or) = _scenario(deposit=rbn.RBN_DEPOSIT_BILINEAR)
+    assert any(d["w"] is not None and np.any((d["w"] > 0) & (d["w"] < 1)) for d in data)
+    J, (f, g, F, nud) = tlm.misfit(p, T0, p.alpha, data)
+    assert np.max(np.abs(np.moveaxis(f, 0, -1) - f_or)) < 1e-13
+    assert np.max(np.abs(np.moveaxis(g, 0, -1) - g_or)) < 1e-13
 
 
 def test_misfit_matches_oracle_interpolation(scenario):
