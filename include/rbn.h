@@ -276,7 +276,10 @@ int rbn_adj_backward(rbn_ctx *ctx, double *dJ_dT0, double *dJ_dalpha);
  * counterpart (the reference has no launch configuration).  Keys:
  *   "lbm_block"  nodes per workgroup along x: 64, 128, 256 (default) or 512
  *   "lbm_nt"     1: non-temporal stores of the step's outputs (default 1)
+ *   "lbm_ntload" 1: non-temporal loads of the step's inputs (default 0)
  *   "lbm_xcd"    1: give each of the 8 XCDs a contiguous band of rows (default 0)
+ *   "lbm_reverse" 1: sweep every odd step last row first, so that it starts on the rows the
+ *                previous step wrote last (default 1)
  *   "particle_overlap"  1: run the particle update of step s on its own stream concurrently
  *                with the lattice update of step s+1 (default 0) */
 int rbn_set_tuning(rbn_ctx *ctx, const char *key, int value);

@@ -923,7 +923,9 @@ int rbn_step(rbn_ctx *c, int nsteps) {
       lp.nud = c->nud;
     }
     if ((rc = timed(c, 0, c->stream, [&] {
-           return rbn::launch_lbm(c->stream, c->prec, P.variant, c->geom, lp, c->consts.data(), c->tune);
+           rbn::LbmTuning t = c->tune;
+           t.flip = t.reverse && (istep & 1);
+           return rbn::launch_lbm(c->stream, c->prec, P.variant, c->geom, lp, c->consts.data(), t);
          })))
       return rc;
     std::swap(c->fA, c->fB);
@@ -990,8 +992,12 @@ int rbn_set_tuning(rbn_ctx *c, const char *key, int value) {
     c->tune.block = value;
   } else if (!strcmp(key, "lbm_nt")) {
     c->tune.nt = value != 0;
+  } else if (!strcmp(key, "lbm_ntload")) {
+    c->tune.ntload = value != 0;
   } else if (!strcmp(key, "lbm_xcd")) {
     c->tune.xcd = value != 0;
+  } else if (!strcmp(key, "lbm_reverse")) {
+    c->tune.reverse = value != 0;
   } else if (!strcmp(key, "particle_overlap")) {
     c->tune.overlap = value != 0;
   } else {
@@ -1004,7 +1010,9 @@ int rbn_get_tuning(const rbn_ctx *c, const char *key, int *value) {
   if (!c || !key || !value) return rbn::thread_fail(RBN_EINVAL, "null argument");
   if (!strcmp(key, "lbm_block")) *value = c->tune.block;
   else if (!strcmp(key, "lbm_nt")) *value = c->tune.nt;
+  else if (!strcmp(key, "lbm_ntload")) *value = c->tune.ntload;
   else if (!strcmp(key, "lbm_xcd")) *value = c->tune.xcd;
+  else if (!strcmp(key, "lbm_reverse")) *value = c->tune.reverse;
   else if (!strcmp(key, "particle_overlap")) *value = c->tune.overlap;
   else return rbn::thread_fail(RBN_EINVAL, "unknown tuning key");
   return RBN_OK;

@@ -11,29 +11,32 @@ constexpr int kBlock = 256;     // block width of the init kernels
 constexpr int kMaxBlock = 512;  // K1 block widths 64..512 (tuning)
 
 // Node of this thread.  The 1-D grid enumerates row segments of blockDim.x nodes, row
-// major.  With `xcd` set, block L goes to segment (L % 8) * (N/8) + L/8: the hardware deals
-// consecutive blocks round-robin to the 8 XCDs, so this gives every XCD one contiguous band
-// of rows and keeps the column halo of neighbouring segments in the same L2.
-__device__ __forceinline__ bool node_of(const Geom &g, int xcd, int &i, int &j) {
+// major.  order bit 0 (XCD banding): block L goes to segment (L % 8) * (N/8) + L/8: the
+// hardware deals consecutive blocks round-robin to the 8 XCDs, so this gives every XCD one
+// contiguous band of rows and keeps the column halo of neighbouring segments in the same L2.
+// order bit 1 (reversed sweep): segments are visited last row first, so that a step that
+// follows a forward sweep starts on the rows its predecessor wrote last.
+__device__ __forceinline__ bool node_of(const Geom &g, int order, int &i, int &j) {
   const int gx = (g.nx + blockDim.x - 1) / blockDim.x;
   const int N = gridDim.x;
   int L = blockIdx.x;
-  if (xcd && (N & 7) == 0) L = (L & 7) * (N >> 3) + (L >> 3);
+  if ((order & 1) && (N & 7) == 0) L = (L & 7) * (N >> 3) + (L >> 3);
+  if (order & 2) L = N - 1 - L;
   j = L / gx;
   i = (L - j * gx) * blockDim.x + threadIdx.x;
   return i < g.nx;
 }
 
 template <int VARIANT, int NT>
-__global__ __launch_bounds__(kMaxBlock) void k_lbm64(Geom g, LbmPtrs p, StepConst<double> c, int xcd) {
+__global__ __launch_bounds__(kMaxBlock) void k_lbm64(Geom g, LbmPtrs p, StepConst<double> c, int order) {
   int i, j;
-  if (node_of(g, xcd, i, j)) lbm_node64<VARIANT, NT>(g, p, c, i, j);
+  if (node_of(g, order, i, j)) lbm_node64<VARIANT, NT>(g, p, c, i, j);
 }
 
 template <int VARIANT, int NT>
-__global__ __launch_bounds__(kMaxBlock) void k_lbm32(Geom g, LbmPtrs p, StepConst<float> c, int xcd) {
+__global__ __launch_bounds__(kMaxBlock) void k_lbm32(Geom g, LbmPtrs p, StepConst<float> c, int order) {
   int i, j;
-  if (node_of(g, xcd, i, j)) lbm_node32<VARIANT, NT>(g, p, c, i, j);
+  if (node_of(g, order, i, j)) lbm_node32<VARIANT, NT>(g, p, c, i, j);
 }
 
 // ---------------------------------------------------------------------------- init
@@ -128,13 +131,24 @@ __global__ void k_energy_final(const double *partial, int nb, double nxny, doubl
 namespace {
 template <int V, int NT>
 void lbm_dispatch(hipStream_t s, int prec, dim3 grid, dim3 block, const Geom &g, const LbmPtrs &p,
-                  const void *step_const, int xcd) {
+                  const void *step_const, int order) {
   if (prec == RBN_FP64)
     hipLaunchKernelGGL((k_lbm64<V, NT>), grid, block, 0, s, g, p,
-                       *(const StepConst<double> *)step_const, xcd);
+                       *(const StepConst<double> *)step_const, order);
   else
     hipLaunchKernelGGL((k_lbm32<V, NT>), grid, block, 0, s, g, p,
-                       *(const StepConst<float> *)step_const, xcd);
+                       *(const StepConst<float> *)step_const, order);
+}
+
+template <int V>
+void lbm_dispatch_nt(int nt, hipStream_t s, int prec, dim3 grid, dim3 block, const Geom &g,
+                     const LbmPtrs &p, const void *step_const, int order) {
+  switch (nt & 3) {
+    case 0: lbm_dispatch<V, 0>(s, prec, grid, block, g, p, step_const, order); break;
+    case 1: lbm_dispatch<V, 1>(s, prec, grid, block, g, p, step_const, order); break;
+    case 2: lbm_dispatch<V, 2>(s, prec, grid, block, g, p, step_const, order); break;
+    default: lbm_dispatch<V, 3>(s, prec, grid, block, g, p, step_const, order); break;
+  }
 }
 }  // namespace
 
@@ -143,14 +157,12 @@ hipError_t launch_lbm(hipStream_t s, int prec, int variant, const Geom &g, const
   const int bw = (t.block == 64 || t.block == 128 || t.block == 512) ? t.block : 256;
   const unsigned gx = (unsigned)((g.nx + bw - 1) / bw);
   dim3 grid(gx * (unsigned)g.ny), block(bw);
-  const bool nud = variant == RBN_VARIANT_NUDGING;
-  if (t.nt) {
-    if (nud) lbm_dispatch<RBN_VARIANT_NUDGING, 1>(s, prec, grid, block, g, p, step_const, t.xcd);
-    else lbm_dispatch<RBN_VARIANT_TRAJECTORIES, 1>(s, prec, grid, block, g, p, step_const, t.xcd);
-  } else {
-    if (nud) lbm_dispatch<RBN_VARIANT_NUDGING, 0>(s, prec, grid, block, g, p, step_const, t.xcd);
-    else lbm_dispatch<RBN_VARIANT_TRAJECTORIES, 0>(s, prec, grid, block, g, p, step_const, t.xcd);
-  }
+  const int nt = (t.nt ? 1 : 0) | (t.ntload ? 2 : 0);
+  const int order = (t.xcd ? 1 : 0) | (t.flip ? 2 : 0);
+  if (variant == RBN_VARIANT_NUDGING)
+    lbm_dispatch_nt<RBN_VARIANT_NUDGING>(nt, s, prec, grid, block, g, p, step_const, order);
+  else
+    lbm_dispatch_nt<RBN_VARIANT_TRAJECTORIES>(nt, s, prec, grid, block, g, p, step_const, order);
   return hipGetLastError();
 }
 

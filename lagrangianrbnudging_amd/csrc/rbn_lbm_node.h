@@ -40,21 +40,31 @@ constexpr double CS2 = 1. / 3.;
 constexpr double CS22 = 2. * CS2;
 constexpr double CSSQ = 2. / 9.;
 
-// Store, optionally non-temporal (the outputs are not read again within the step).
+// NT flags of the fused step: bit 0 non-temporal stores of the outputs (not read again
+// within the step), bit 1 non-temporal loads of the inputs (read once per step).
 template <int NT, typename T>
 __device__ __forceinline__ void st(T *p, T v) {
-  if (NT)
+  if (NT & 1)
     __builtin_nontemporal_store(v, p);
   else
     *p = v;
 }
 
-template <typename R>
+template <typename R, int NT = 0>
 struct Planes {
   const R *__restrict__ a;
   size_t P;
-  __device__ __forceinline__ R at(int k, size_t off) const { return a[(size_t)k * P + off]; }
+  __device__ __forceinline__ R at(int k, size_t off) const {
+    if (NT & 2) return __builtin_nontemporal_load(&a[(size_t)k * P + off]);
+    return a[(size_t)k * P + off];
+  }
 };
+
+template <int NT, typename T>
+__device__ __forceinline__ T ld(const T *p) {
+  if (NT & 2) return __builtin_nontemporal_load(p);
+  return *p;
+}
 
 // ---------------------------------------------------------------------------- FP64
 
@@ -107,7 +117,7 @@ __device__ __forceinline__ void lbm_node64(const Geom &g, const LbmPtrs &p, cons
   const size_t r0 = (size_t)j * g.pitch;
   const size_t rm = bot ? r0 : r0 - g.pitch;
   const size_t rp = top ? r0 : r0 + g.pitch;
-  const Planes<double> fA{(const double *)p.fA, g.plane}, gA{(const double *)p.gA, g.plane};
+  const Planes<double, NT> fA{(const double *)p.fA, g.plane}, gA{(const double *)p.gA, g.plane};
 
   double f[9], q[9];
   // f: pull, with half-way bounce-back at the walls from the node's own pre-stream
@@ -198,7 +208,7 @@ __device__ __forceinline__ void lbm_node64(const Geom &g, const LbmPtrs &p, cons
   for (int k = 0; k < 9; k++) fl[k] = f[k] * rhoi;
   double *F = (double *)p.F;
   const double Fx = 0.0;
-  const double Fy = F[r0 + i];
+  const double Fy = ld<NT>(&F[r0 + i]);
   const double u = fl[1] - fl[3] + fl[5] - fl[6] - fl[7] + fl[8] + 0.5 * rhoi * Fx;
   const double v = fl[5] + fl[2] + fl[6] - fl[7] - fl[4] - fl[8] + 0.5 * rhoi * Fy;
   st<NT>(&F[r0 + i], c.g0 * rho * (T - 0.0));
@@ -266,7 +276,7 @@ __device__ __forceinline__ void lbm_node32(const Geom &g, const LbmPtrs &p, cons
   const size_t r0 = (size_t)j * g.pitch;
   const size_t rm = bot ? r0 : r0 - g.pitch;
   const size_t rp = top ? r0 : r0 + g.pitch;
-  const Planes<float> fA{(const float *)p.fA, g.plane}, gA{(const float *)p.gA, g.plane};
+  const Planes<float, NT> fA{(const float *)p.fA, g.plane}, gA{(const float *)p.gA, g.plane};
 
   float f[9], q[9];
   f[0] = fA.at(0, r0 + i);
@@ -349,7 +359,7 @@ __device__ __forceinline__ void lbm_node32(const Geom &g, const LbmPtrs &p, cons
   const float T = c.T0 + dT;
   const float rhoi = 1.f / rho;
   float *F = (float *)p.F;
-  const float Fy = F[r0 + i];
+  const float Fy = ld<NT>(&F[r0 + i]);
   const float jx = f[1] - f[3] + f[5] - f[6] - f[7] + f[8];
   const float jy = f[2] - f[4] + f[5] + f[6] - f[7] - f[8];
   const float u = jx * rhoi;

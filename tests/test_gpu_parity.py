@@ -149,20 +149,21 @@ def test_launch_tuning_does_not_change_results(prec):
     p.precision = prec
     T0 = conduction_T(p, 1e-2, seed=13)
     ref = None
-    for block in (64, 128, 256, 512):
-        for nt in (0, 1):
-            for xcd in (0, 1):
-                s = rbn.Solver(p)
-                s.set_tuning(lbm_block=block, lbm_nt=nt, lbm_xcd=xcd)
-                s.init_fields(T=T0)
-                s.step(30)
-                f, g = s.populations()
-                if ref is None:
-                    ref = (f, g)
-                else:
-                    assert_equal_arrays(f, ref[0], f"f block={block} nt={nt} xcd={xcd}")
-                    assert_equal_arrays(g, ref[1], f"g block={block} nt={nt} xcd={xcd}")
-                s.close()
+    shapes = [(b, nt, 0, x, 0) for b in (64, 128, 256, 512) for nt in (0, 1) for x in (0, 1)]
+    shapes += [(256, 1, 1, 0, 0), (256, 1, 0, 0, 1), (512, 1, 1, 1, 1), (64, 0, 1, 0, 1)]
+    for block, nt, ntload, xcd, rev in shapes:
+        s = rbn.Solver(p)
+        s.set_tuning(lbm_block=block, lbm_nt=nt, lbm_ntload=ntload, lbm_xcd=xcd, lbm_reverse=rev)
+        s.init_fields(T=T0)
+        s.step(30)
+        f, g = s.populations()
+        what = f"block={block} nt={nt} ntload={ntload} xcd={xcd} reverse={rev}"
+        if ref is None:
+            ref = (f, g)
+        else:
+            assert_equal_arrays(f, ref[0], f"f {what}")
+            assert_equal_arrays(g, ref[1], f"g {what}")
+        s.close()
 
 
 def test_two_contexts_are_independent():
