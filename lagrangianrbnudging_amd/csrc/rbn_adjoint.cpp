@@ -50,14 +50,14 @@ struct AdjState {
   unsigned long long *maxbits = nullptr;  // nobs_steps
   double *part = nullptr;             // block partials
   double *acc = nullptr;              // [0] J, [1] dJ/dalpha
-  unsigned char *init = nullptr;      // rho0, u0, v0 planes
+  unsigned char *init = nullptr;      // rho0 plane, then (u0, v0) interleaved (2 planes)
   // backward workspace
   unsigned char *seg = nullptr;       // K x 18 planes (populations entering each step)
   unsigned char *segFN = nullptr;     // K x 2 planes (F_in, nud_in)
   int *seg_nvol = nullptr;            // K x plane ints
   double *seg_tbar = nullptr;         // K x plane doubles
   double *seg_w = nullptr;            // K x plane doubles (bilinear deposit weights)
-  unsigned char *work = nullptr;      // F, nud, u, v planes
+  unsigned char *work = nullptr;      // F, nud planes, then a velocity slot (2 planes)
   int *rep_oi = nullptr;              // replayed observer state
   double *rep_od = nullptr;
   unsigned char *lam = nullptr;       // 2 x 18 planes
@@ -205,8 +205,7 @@ int rbn_adj_forward(rbn_ctx *c, int nsteps, int ckpt_every, double *J) {
   hipStream_t s = c->stream;
   // initial moments (rbn_init_fields left rho0 in rho and (u0, v0) in the first ring slot)
   HIPCHK(c, hipMemcpyAsync(plane_at(a->init, pb, 0), c->rho, pb, hipMemcpyDeviceToDevice, s));
-  HIPCHK(c, hipMemcpyAsync(plane_at(a->init, pb, 1), c->u[c->cur], pb, hipMemcpyDeviceToDevice, s));
-  HIPCHK(c, hipMemcpyAsync(plane_at(a->init, pb, 2), c->v[c->cur], pb, hipMemcpyDeviceToDevice, s));
+  HIPCHK(c, hipMemcpyAsync(plane_at(a->init, pb, 1), c->uv[c->cur], 2 * pb, hipMemcpyDeviceToDevice, s));
   HIPCHK(c, hipMemsetAsync(a->acc, 0, 2 * sizeof(double), s));
   HIPCHK(c, hipMemsetAsync(a->maxbits, 0, (size_t)std::max(a->nobs_steps, 1) * sizeof(unsigned long long), s));
   c->at_init = false;
@@ -240,8 +239,7 @@ int rbn_adj_forward(rbn_ctx *c, int nsteps, int ckpt_every, double *J) {
     lp.fB = c->fB;
     lp.gB = c->gB;
     lp.F = c->F;
-    lp.u = c->u[nxt];
-    lp.v = c->v[nxt];
+    lp.uv = c->uv[nxt];
     lp.rho = moments ? c->rho : nullptr;
     lp.T = moments ? c->T : nullptr;
     lp.nvol = c->dep.cnt;
@@ -285,7 +283,7 @@ int rbn_adj_backward(rbn_ctx *c, double *dJ_dT0, double *dJ_dalpha) {
   hipStream_t s = c->stream;
   unsigned char *lamF = a->lamFN, *lamN = a->lamFN + pb;
   unsigned char *wF = plane_at(a->work, pb, 0), *wN = plane_at(a->work, pb, 1);
-  unsigned char *wu = plane_at(a->work, pb, 2), *wv = plane_at(a->work, pb, 3);
+  unsigned char *wuv = plane_at(a->work, pb, 2);
   HIPCHK(c, hipMemsetAsync(a->lamFN, 0, 2 * pb, s));
   HIPCHK(c, hipMemsetAsync(a->lamT, 0, g.plane * sizeof(unsigned long long), s));
   HIPCHK(c, hipMemsetAsync(a->acc + 1, 0, sizeof(double), s));
@@ -330,8 +328,7 @@ int rbn_adj_backward(rbn_ctx *c, double *dJ_dT0, double *dJ_dalpha) {
       lp.fB = out;
       lp.gB = out + 9 * pb;
       lp.F = wF;
-      lp.u = wu;
-      lp.v = wv;
+      lp.uv = wuv;
       lp.nvol = d.cnt;
       lp.Tbar = d.Tbar;
       lp.nud = wN;
@@ -370,7 +367,7 @@ int rbn_adj_backward(rbn_ctx *c, double *dJ_dT0, double *dJ_dalpha) {
     }
   }
   HIPCHK(c, rbn::launch_adj_init(s, c->prec, g, plane_at(a->lam, pb, 18 * (size_t)cur), lamF, plane_at(a->init, pb, 0),
-                                 plane_at(a->init, pb, 1), plane_at(a->init, pb, 2), -P.gravity, a->dT0));
+                                 plane_at(a->init, pb, 1), -P.gravity, a->dT0));
   if (c->timing && (rc = drain_timing(c))) return rc;
   double h[2];
   std::vector<double> buf(dJ_dT0 ? g.plane : 0);

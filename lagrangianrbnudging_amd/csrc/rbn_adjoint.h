@@ -38,7 +38,7 @@ hipError_t launch_sum_partials(hipStream_t s, const double *part, int n, double 
 // dJ/dT0 (FP64, device layout) from the pulled adjoints of the first step of the window and
 // the adjoint of the first lagged force, for the equilibrium initial state of rbn_init_fields.
 hipError_t launch_adj_init(hipStream_t s, int prec, const Geom &g, const void *lam, const void *lamF,
-                           const void *rho0, const void *u0, const void *v0, double g0, double *dT0);
+                           const void *rho0, const void *uv0, double g0, double *dT0);
 // Misfit of one observation: residuals r_k = T(x_k, y_k) - T_k (bilinear, observer order),
 // per-block sums of r_k^2 and the max |r_k| (as ordered bits).
 hipError_t launch_misfit(hipStream_t s, int prec, const Geom &g, const void *T, int n, const double *x,

@@ -220,13 +220,13 @@ __global__ __launch_bounds__(kT) void k_sum_partials(const double *part, int n, 
 }
 
 template <typename R>
-__global__ void k_adj_init(Geom g, const R *lam, const R *lamF, const R *rho0, const R *u0, const R *v0,
-                           double g0, double *dT0) {
+__global__ void k_adj_init(Geom g, const R *lam, const R *lamF, const R *rho0, const R *uv0, double g0,
+                           double *dT0) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y;
   if (i >= g.nx) return;
   const size_t P = g.plane, o = (size_t)j * g.pitch + i;
-  const double u = u0[o], v = v0[o], usq = u * u + v * v;
+  const double u = uv0[2 * o], v = uv0[2 * o + 1], usq = u * u + v * v;
   double s = 0.0;
   if (j >= 1 && j <= g.ny - 2) {
     for (int k = 0; k < 9; k++) {
@@ -253,7 +253,7 @@ __global__ __launch_bounds__(kT) void k_misfit(Geom g, const R *T, int n, const 
   const int p = blockIdx.x * kT + threadIdx.x;
   double r2 = 0.0;
   if (p < n) {
-    const part::Field<R> F{T, g.nx, g.ny, g.pitch};
+    const part::Field<R> F = part::scalar_of<R>(T, g.nx, g.ny, g.pitch);
     int e = 0;
     const double val = part::interp(F, x[p], y[p], e);
     if (e) atomicOr(err, 1);
@@ -308,14 +308,14 @@ hipError_t launch_sum_partials(hipStream_t s, const double *part, int n, double 
 }
 
 hipError_t launch_adj_init(hipStream_t s, int prec, const Geom &g, const void *lam, const void *lamF,
-                           const void *rho0, const void *u0, const void *v0, double g0, double *dT0) {
+                           const void *rho0, const void *uv0, double g0, double *dT0) {
   const dim3 grid((g.nx + kT - 1) / kT, g.ny), block(kT);
   if (prec == RBN_FP64)
     hipLaunchKernelGGL(k_adj_init<double>, grid, block, 0, s, g, (const double *)lam, (const double *)lamF,
-                       (const double *)rho0, (const double *)u0, (const double *)v0, g0, dT0);
+                       (const double *)rho0, (const double *)uv0, g0, dT0);
   else
     hipLaunchKernelGGL(k_adj_init<float>, grid, block, 0, s, g, (const float *)lam, (const float *)lamF,
-                       (const float *)rho0, (const float *)u0, (const float *)v0, g0, dT0);
+                       (const float *)rho0, (const float *)uv0, g0, dT0);
   return hipGetLastError();
 }
 

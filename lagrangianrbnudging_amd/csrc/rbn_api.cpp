@@ -138,14 +138,34 @@ int upload_field(rbn_ctx *c, void *dst, const double *h) {
   return RBN_OK;
 }
 
-int download_field(rbn_ctx *c, const void *src, double *h, double add) {
+int upload_uv(rbn_ctx *c, void *dst, const double *hu, const double *hv) {
   const Geom &g = c->geom;
-  std::vector<unsigned char> buf(g.plane * c->rs);
+  std::vector<unsigned char> buf(2 * g.plane * c->rs, 0);
+  for (int i = 0; i < g.nx; i++)
+    for (int j = 0; j < g.ny; j++) {
+      const size_t o = 2 * ((size_t)j * g.pitch + i), h = (size_t)i * g.ny + j;
+      if (c->prec == RBN_FP64) {
+        reinterpret_cast<double *>(buf.data())[o] = hu[h];
+        reinterpret_cast<double *>(buf.data())[o + 1] = hv[h];
+      } else {
+        reinterpret_cast<float *>(buf.data())[o] = (float)hu[h];
+        reinterpret_cast<float *>(buf.data())[o + 1] = (float)hv[h];
+      }
+    }
+  HIPCHK(c, hipMemcpyAsync(dst, buf.data(), buf.size(), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return RBN_OK;
+}
+
+int download_field(rbn_ctx *c, const void *src, double *h, double add, int cs) {
+  const Geom &g = c->geom;
+  // the last node's value is element (plane - 1) * cs of src: copy no further
+  std::vector<unsigned char> buf(((g.plane - 1) * (size_t)cs + 1) * c->rs);
   HIPCHK(c, hipMemcpyAsync(buf.data(), src, buf.size(), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   for (int i = 0; i < g.nx; i++)
     for (int j = 0; j < g.ny; j++) {
-      size_t o = (size_t)j * g.pitch + i;
+      size_t o = ((size_t)j * g.pitch + i) * cs;
       double v = c->prec == RBN_FP64 ? reinterpret_cast<const double *>(buf.data())[o]
                                      : (double)reinterpret_cast<const float *>(buf.data())[o];
       h[(size_t)i * g.ny + j] = v + add;
@@ -296,7 +316,7 @@ void free_record_slots(rbn_ctx *c) {
 int run_energy(rbn_ctx *c, double out[2]) {
   const int nb = rbn::energy_blocks(c->geom);
   double *partial = c->d_energy, *res = c->d_energy + 2 * 1024;
-  hipError_t e = rbn::launch_energy(c->stream, c->prec, c->geom, c->rho, c->u[c->cur], c->v[c->cur],
+  hipError_t e = rbn::launch_energy(c->stream, c->prec, c->geom, c->rho, c->uv[c->cur],
                                     c->T, partial, nb, res);
   if (e != hipSuccess) return fail(c, RBN_EDEVICE, "energy launch failed: %s", hipGetErrorString(e));
   HIPCHK(c, hipMemcpyAsync(out, res, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -378,10 +398,8 @@ void rbn_destroy(rbn_ctx *c) {
   void *ptrs[] = {c->fA, c->fB, c->gA, c->gB, c->F, c->rho, c->T, c->nud};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
-  for (int k = 0; k < rbn_ctx::kVel; k++) {
-    if (c->u[k]) (void)hipFree(c->u[k]);
-    if (c->v[k]) (void)hipFree(c->v[k]);
-  }
+  for (int k = 0; k < rbn_ctx::kVel; k++)
+    if (c->uv[k]) (void)hipFree(c->uv[k]);
   if (c->ev_lbm) (void)hipEventDestroy(c->ev_lbm);
   for (hipEvent_t e : c->ev_part)
     if (e) (void)hipEventDestroy(e);
@@ -460,9 +478,8 @@ int rbn_create(const rbn_params *p, int device, rbn_ctx **out) {
     void **p;
     size_t b;
   } allocs[] = {{&c->fA, 9 * fb}, {&c->fB, 9 * fb}, {&c->gA, 9 * fb}, {&c->gB, 9 * fb},
-                {&c->F, fb},      {&c->u[0], fb},  {&c->u[1], fb},  {&c->u[2], fb},
-                {&c->u[3], fb},   {&c->v[0], fb},  {&c->v[1], fb},  {&c->v[2], fb},
-                {&c->v[3], fb},   {&c->rho, fb},   {&c->T, fb}};
+                {&c->F, fb},      {&c->uv[0], 2 * fb}, {&c->uv[1], 2 * fb}, {&c->uv[2], 2 * fb},
+                {&c->uv[3], 2 * fb}, {&c->rho, fb},   {&c->T, fb}};
   for (auto &a : allocs) {
     if ((rc = dalloc(c, a.p, a.b))) {
       rbn::thread_fail(rc, c->err.c_str());
@@ -548,9 +565,8 @@ int rbn_init_fields(rbn_ctx *c, const double *rho, const double *T, const double
   c->cur = 0;
   if ((rc = upload_field(c, c->rho, hr.data()))) return rc;
   if ((rc = upload_field(c, c->T, hT.data()))) return rc;
-  if ((rc = upload_field(c, c->u[0], hu.data()))) return rc;
-  if ((rc = upload_field(c, c->v[0], hv.data()))) return rc;
-  hipError_t e = rbn::launch_init_eq(c->stream, c->prec, g, c->rho, c->T, c->u[0], c->v[0], c->fA,
+  if ((rc = upload_uv(c, c->uv[0], hu.data(), hv.data()))) return rc;
+  hipError_t e = rbn::launch_init_eq(c->stream, c->prec, g, c->rho, c->T, c->uv[0], c->fA,
                                      c->gA, c->F, c->consts.data());
   if (e != hipSuccess) return fail(c, RBN_EDEVICE, "init launch failed: %s", hipGetErrorString(e));
   if ((rc = init_common_after_fields(c))) return rc;
@@ -607,8 +623,7 @@ int rbn_set_populations(rbn_ctx *c, const double *f, const double *gpop, const d
   if ((rc = upload_field(c, c->F, F.data()))) return rc;
   if ((rc = upload_field(c, c->rho, rho_prev))) return rc;
   if ((rc = upload_field(c, c->T, T_prev))) return rc;
-  if ((rc = upload_field(c, c->u[0], u_prev ? u_prev : zero.data()))) return rc;
-  if ((rc = upload_field(c, c->v[0], v_prev ? v_prev : zero.data()))) return rc;
+  if ((rc = upload_uv(c, c->uv[0], u_prev ? u_prev : zero.data(), v_prev ? v_prev : zero.data()))) return rc;
   return init_common_after_fields(c);
 }
 
@@ -645,8 +660,8 @@ int rbn_get_fields(rbn_ctx *c, double *rho, double *T, double *u, double *v) {
   if ((rho || T) && !c->moments_valid) return fail(c, RBN_ESTATE, "rho/T of the latest step not available");
   if (rho && (rc = download_field(c, c->rho, rho))) return rc;
   if (T && (rc = download_field(c, c->T, T))) return rc;
-  if (u && (rc = download_field(c, c->u[c->cur], u))) return rc;
-  if (v && (rc = download_field(c, c->v[c->cur], v))) return rc;
+  if (u && (rc = download_field(c, c->uv[c->cur], u, 0.0, 2))) return rc;
+  if (v && (rc = download_field(c, (const char *)c->uv[c->cur] + c->rs, v, 0.0, 2))) return rc;
   return RBN_OK;
 }
 
@@ -669,14 +684,18 @@ int rbn_sample(rbn_ctx *c, int field, int n, const double *x, const double *y, d
   if (!c->initialised) return fail(c, RBN_ESTATE, "not initialised");
   if ((field == 0 || field == 1) && !c->moments_valid) return fail(c, RBN_ESTATE, "rho/T of the latest step not available");
   if (n == 0) return RBN_OK;
-  const void *src = field == 0 ? c->rho : field == 1 ? c->T : field == 2 ? c->u[c->cur] : c->v[c->cur];
+  const void *src = field == 0   ? c->rho
+                    : field == 1 ? c->T
+                    : field == 2 ? c->uv[c->cur]
+                                 : (const void *)((const char *)c->uv[c->cur] + c->rs);
+  const int cs = field >= 2 ? 2 : 1;
   void *buf = nullptr;
   if ((rc = dalloc(c, &buf, 3 * (size_t)n * sizeof(double)))) return rc;
   double *dx = (double *)buf, *dy = dx + n, *dout = dy + n;
   rc = RBN_OK;
   if (hipMemcpyAsync(dx, x, (size_t)n * sizeof(double), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
       hipMemcpyAsync(dy, y, (size_t)n * sizeof(double), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
-      rbn::launch_sample(c->stream, c->prec, c->geom, src, n, dx, dy, dout, c->d_err) != hipSuccess ||
+      rbn::launch_sample(c->stream, c->prec, c->geom, src, cs, n, dx, dy, dout, c->d_err) != hipSuccess ||
       hipMemcpyAsync(out, dout, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
       hipStreamSynchronize(c->stream) != hipSuccess)
     rc = fail(c, RBN_EDEVICE, "rbn_sample: device operation failed");
@@ -738,7 +757,7 @@ int rbn_init_particles(rbn_ctx *c, int n, const int *id, const double *x, const 
   }
   HIPCHK(c, hipMemcpyAsync(c->d_part, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
   if (!up) {
-    hipError_t e = rbn::launch_particle_init(c->stream, c->prec, c->geom, c->u[c->cur], c->v[c->cur],
+    hipError_t e = rbn::launch_particle_init(c->stream, c->prec, c->geom, c->uv[c->cur],
                                              part_ptrs(c, nullptr), c->P.heavy_init_factor);
     if (e != hipSuccess) return fail(c, RBN_EDEVICE, "particle init launch failed: %s", hipGetErrorString(e));
   }
@@ -912,8 +931,7 @@ int rbn_step(rbn_ctx *c, int nsteps) {
     lp.fB = c->fB;
     lp.gB = c->gB;
     lp.F = c->F;
-    lp.u = c->u[nxt];
-    lp.v = c->v[nxt];
+    lp.uv = c->uv[nxt];
     lp.rho = moments ? c->rho : nullptr;
     lp.T = moments ? c->T : nullptr;
     if (nudged) {
@@ -944,8 +962,8 @@ int rbn_step(rbn_ctx *c, int nsteps) {
       if (record && (rc = claim_record_slot(c, &slot))) return rc;
       rbn::PartPtrs pp = part_ptrs(c, slot ? slot->d : nullptr);
       if ((rc = timed(c, 1, ks, [&] {
-             return rbn::launch_particles(ks, c->prec, c->geom, c->u[c->cur], c->v[c->cur],
-                                          c->u[prv], c->v[prv], c->T, pp, c->any_heavy || full, record,
+             return rbn::launch_particles(ks, c->prec, c->geom, c->uv[c->cur], c->uv[prv], c->T, pp,
+                                          c->any_heavy || full, record,
                                           full);
            })))
         return rc;

@@ -66,10 +66,7 @@ void fill_member(const rbn_ctx *c, int tile_w, int tile_h, rbn::EnsMember<R> &d)
   d.pg[0] = (R *)c->gA;
   d.pg[1] = (R *)c->gB;
   d.F = (R *)c->F;
-  for (int k = 0; k < 4; k++) {
-    d.u[k] = (R *)c->u[k];
-    d.v[k] = (R *)c->v[k];
-  }
+  for (int k = 0; k < 4; k++) d.uv[k] = (R *)c->uv[k];
   d.rho = (R *)c->rho;
   d.T = (R *)c->T;
   d.cur0 = c->cur;
@@ -236,10 +233,7 @@ int take_snapshot(rbn_ensemble *e, Snapshot &s) {
   for (rbn_ctx *c : e->m) {
     const size_t fb = c->geom.plane * c->rs;
     std::vector<std::pair<void *, size_t>> parts = {{c->fA, 9 * fb}, {c->gA, 9 * fb}, {c->F, fb}};
-    for (int k = 0; k < 4; k++) {
-      parts.push_back({c->u[k], fb});
-      parts.push_back({c->v[k], fb});
-    }
+    for (int k = 0; k < 4; k++) parts.push_back({c->uv[k], 2 * fb});
     if (c->d_part && c->np > 0) parts.push_back({c->d_part, 10 * (size_t)c->np * sizeof(double)});
     for (auto &p : parts) {
       void *b = nullptr;

@@ -6,14 +6,14 @@
 namespace rbn {
 
 // Pointers and constants of one member as of the start of a four-step cycle: pf[0]/pg[0] are
-// the f/g planes read by the first step, u[cur0] the latest velocity.
+// the f/g planes read by the first step, uv[cur0] the latest velocity ((u, v) interleaved).
 template <typename R>
 struct EnsMember {
   Geom g;
   StepConst<R> c;
   R *pf[2], *pg[2];
   R *F;
-  R *u[4], *v[4];
+  R *uv[4];
   R *rho, *T;
   int cur0;
   int tiles_x, tiles_y;

@@ -35,8 +35,7 @@ __device__ __forceinline__ void lbm_item(const EnsMember<R> &m, int tile, int k,
   p.gB = par ? m.pg[0] : m.pg[1];
   p.F = m.F;
   const int out = (m.cur0 + 1 + k) & 3;
-  p.u = m.u[out];
-  p.v = m.v[out];
+  p.uv = m.uv[out];
   p.rho = write_moments ? m.rho : nullptr;
   p.T = write_moments ? m.T : nullptr;
   const int col = threadIdx.x % tile_w;
@@ -56,9 +55,10 @@ __device__ __forceinline__ void particle_item(const EnsMember<R> &m, int chunk, 
   const int p = chunk * kThreads + threadIdx.x;
   if (p >= m.np) return;
   const int cur = (m.cur0 + 1 + k) & 3, prv = (m.cur0 + k) & 3;
-  const part::Field<R> U{m.u[cur], m.g.nx, m.g.ny, m.g.pitch}, V{m.v[cur], m.g.nx, m.g.ny, m.g.pitch};
-  const part::Field<R> Up{m.u[prv], m.g.nx, m.g.ny, m.g.pitch}, Vp{m.v[prv], m.g.nx, m.g.ny, m.g.pitch};
-  const part::Field<R> T{m.T, m.g.nx, m.g.ny, m.g.pitch};
+  const int nx = m.g.nx, ny = m.g.ny, pitch = m.g.pitch;
+  const part::Field<R> U = part::u_of<R>(m.uv[cur], nx, ny, pitch), V = part::v_of<R>(m.uv[cur], nx, ny, pitch);
+  const part::Field<R> Up = part::u_of<R>(m.uv[prv], nx, ny, pitch), Vp = part::v_of<R>(m.uv[prv], nx, ny, pitch);
+  const part::Field<R> T = part::scalar_of<R>(m.T, nx, ny, pitch);
   PartPtrs pp{};
   const size_t n = (size_t)m.np;
   double *b = m.part;

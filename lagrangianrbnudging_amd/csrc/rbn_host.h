@@ -63,7 +63,10 @@ int timed(rbn_ctx *c, int kernel, hipStream_t s, Fn &&fn) {
 
 // Device field <-> host array in the reference's [nx][ny] order (y fastest).
 int upload_field(rbn_ctx *c, void *dst, const double *h);
-int download_field(rbn_ctx *c, const void *src, double *h, double add = 0.0);
+// (u, v) interleaved into one velocity slot.
+int upload_uv(rbn_ctx *c, void *dst, const double *hu, const double *hv);
+// Node o of the source is src[o * cs] (cs = 2 for one component of a velocity slot).
+int download_field(rbn_ctx *c, const void *src, double *h, double add = 0.0, int cs = 1);
 
 // Observer arrays: ints chk, bcy, xidx, yidx; doubles dx, dy, dT, xint, yint, Tint and the
 // current record rx, ry, rT (record order).  Each array holds nobs entries.

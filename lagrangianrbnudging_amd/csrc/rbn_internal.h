@@ -48,7 +48,7 @@ struct LbmPtrs {
   const void *fA, *gA;
   void *fB, *gB;
   void *F;           // lagged F_y: read F(s), write F(s+1) (one value per node)
-  void *u, *v;       // hydrovar velocity of this step
+  void *uv;          // hydrovar velocity of this step, (u, v) interleaved per node
   void *rho, *T;     // written when non-null (record/diagnostic/last steps)
   const int *nvol;   // nudged run: nudge_vol counts (nullptr: not nudged)
   const double *Tbar;// nudged run: tempreadint
@@ -111,22 +111,21 @@ void adj_free(rbn_ctx *c);
 hipError_t launch_lbm(hipStream_t s, int prec, int variant, const Geom &g, const LbmPtrs &p,
                       const void *step_const, const LbmTuning &t);
 hipError_t launch_init_eq(hipStream_t s, int prec, const Geom &g, const void *rho, const void *T,
-                          const void *u, const void *v, void *f, void *gp, void *F,
-                          const void *step_const);
-hipError_t launch_particles(hipStream_t s, int prec, const Geom &g, const void *u, const void *v,
-                            const void *up, const void *vp, const void *T, const PartPtrs &pp,
-                            int need_deriv, int record, int full);
-hipError_t launch_sample(hipStream_t s, int prec, const Geom &g, const void *field, int n,
+                          const void *uv, void *f, void *gp, void *F, const void *step_const);
+hipError_t launch_particles(hipStream_t s, int prec, const Geom &g, const void *uv, const void *uvp,
+                            const void *T, const PartPtrs &pp, int need_deriv, int record, int full);
+// Bilinear samples of a field whose node o holds its value at field[o * cs] (cs = 1 for
+// scalar planes, 2 for the u or v component of an interleaved velocity slot).
+hipError_t launch_sample(hipStream_t s, int prec, const Geom &g, const void *field, int cs, int n,
                          const double *x, const double *y, double *out, int *err);
-hipError_t launch_particle_init(hipStream_t s, int prec, const Geom &g, const void *u,
-                                const void *v, const PartPtrs &pp, double heavy_factor);
+hipError_t launch_particle_init(hipStream_t s, int prec, const Geom &g, const void *uv,
+                                const PartPtrs &pp, double heavy_factor);
 hipError_t launch_obs_update(hipStream_t s, int branch, int nx, int ny, int ntempread,
                              const ObsPtrs &o);
 hipError_t launch_deposit(hipStream_t s, const Geom &g, int right_limit, const ObsPtrs &o,
                           const DepPtrs &d);
-hipError_t launch_energy(hipStream_t s, int prec, const Geom &g, const void *rho, const void *u,
-                         const void *v, const void *T, double *partial, int nblocks,
-                         double *out2);
+hipError_t launch_energy(hipStream_t s, int prec, const Geom &g, const void *rho, const void *uv,
+                         const void *T, double *partial, int nblocks, double *out2);
 int energy_blocks(const Geom &g);
 
 }  // namespace rbn
@@ -145,11 +144,13 @@ struct rbn_ctx {
   // lattice state
   void *fA = nullptr, *fB = nullptr, *gA = nullptr, *gB = nullptr;
   void *F = nullptr;
-  // Velocity ring: u[cur] is the latest hydrovar velocity, u[(cur+3)%4] the step before.
-  // Four buffers let the particle update of step s (which reads steps s and s-1) run on
-  // its own stream while the lattice steps s+1 and s+2 write the other two.
+  // Velocity ring: uv[cur] is the latest hydrovar velocity, uv[(cur+3)%4] the step before.
+  // Each slot holds (u, v) interleaved per node (2 * plane reals), so that the particle
+  // update fetches one cache line per patch row for both components.  Four slots let the
+  // particle update of step s (which reads steps s and s-1) run on its own stream while the
+  // lattice steps s+1 and s+2 write the other two.
   static constexpr int kVel = 4;
-  void *u[kVel] = {}, *v[kVel] = {};
+  void *uv[kVel] = {};
   int cur = 0;
   void *rho = nullptr, *T = nullptr;
   hipStream_t pstream = nullptr;       // particle stream (default priority)

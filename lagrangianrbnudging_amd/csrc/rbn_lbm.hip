@@ -43,14 +43,14 @@ __global__ __launch_bounds__(kMaxBlock) void k_lbm32(Geom g, LbmPtrs p, StepCons
 
 // f = feq(rho, u, v), g = geq(T, u, v) (inithydro + initpop, T:1643-1648) and the force of
 // the first step from the initial moments.
-__global__ void k_init64(Geom g, const double *rho, const double *T, const double *u,
-                         const double *v, double *f, double *gp, double *F, double g0) {
+__global__ void k_init64(Geom g, const double *rho, const double *T, const double *uv, double *f,
+                         double *gp, double *F, double g0) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y;
   if (i >= g.nx) return;
   const size_t o = (size_t)j * g.pitch + i;
   double b[9];
-  eq_brackets64(u[o], v[o], b);
+  eq_brackets64(uv[2 * o], uv[2 * o + 1], b);
 #pragma unroll
   for (int k = 0; k < 9; k++) {
     f[(size_t)k * g.plane + o] = rho[o] * kW[k] * b[k];
@@ -59,15 +59,14 @@ __global__ void k_init64(Geom g, const double *rho, const double *T, const doubl
   F[o] = g0 * rho[o] * (T[o] - 0.0);
 }
 
-__global__ void k_init32(Geom g, const float *rho, const float *T, const float *u,
-                         const float *v, float *f, float *gp, float *F, float g0, float rho0,
-                         float T0) {
+__global__ void k_init32(Geom g, const float *rho, const float *T, const float *uv, float *f,
+                         float *gp, float *F, float g0, float rho0, float T0) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int j = blockIdx.y;
   if (i >= g.nx) return;
   const size_t o = (size_t)j * g.pitch + i;
   float A[9];
-  eq_dev32(u[o], v[o], A);
+  eq_dev32(uv[2 * o], uv[2 * o + 1], A);
   const float r = rho[o], t = T[o];
 #pragma unroll
   for (int k = 0; k < 9; k++) {
@@ -85,9 +84,8 @@ constexpr int kRedBlock = 256;
 // Per-block partial sums of 0.5*rho*(u^2+v^2) and T over interior rows (evolution.dat,
 // T:1739-1757), accumulated in FP64 in a fixed order: the result is deterministic.
 template <typename R>
-__global__ __launch_bounds__(kRedBlock) void k_energy_partial(Geom g, const R *rho, const R *u,
-                                                              const R *v, const R *T,
-                                                              double *partial) {
+__global__ __launch_bounds__(kRedBlock) void k_energy_partial(Geom g, const R *rho, const R *uv,
+                                                              const R *T, double *partial) {
   __shared__ double se[kRedBlock], st[kRedBlock];
   const long long n = (long long)g.nx * g.ny;
   double e = 0.0, t = 0.0;
@@ -95,7 +93,7 @@ __global__ __launch_bounds__(kRedBlock) void k_energy_partial(Geom g, const R *r
        idx += (long long)gridDim.x * kRedBlock) {
     const int j = (int)(idx / g.nx), i = (int)(idx - (long long)j * g.nx);
     const size_t o = (size_t)j * g.pitch + i;
-    const double r = rho[o], uu = u[o], vv = v[o];
+    const double r = rho[o], uu = uv[2 * o], vv = uv[2 * o + 1];
     e += 0.5 * r * (uu * uu + vv * vv);
     t += (double)T[o];
   }
@@ -167,19 +165,16 @@ hipError_t launch_lbm(hipStream_t s, int prec, int variant, const Geom &g, const
 }
 
 hipError_t launch_init_eq(hipStream_t s, int prec, const Geom &g, const void *rho, const void *T,
-                          const void *u, const void *v, void *f, void *gp, void *F,
-                          const void *step_const) {
+                          const void *uv, void *f, void *gp, void *F, const void *step_const) {
   dim3 grid((g.nx + kBlock - 1) / kBlock, g.ny), block(kBlock);
   if (prec == RBN_FP64) {
     const StepConst<double> &c = *(const StepConst<double> *)step_const;
     hipLaunchKernelGGL(k_init64, grid, block, 0, s, g, (const double *)rho, (const double *)T,
-                       (const double *)u, (const double *)v, (double *)f, (double *)gp,
-                       (double *)F, c.g0);
+                       (const double *)uv, (double *)f, (double *)gp, (double *)F, c.g0);
   } else {
     const StepConst<float> &c = *(const StepConst<float> *)step_const;
     hipLaunchKernelGGL(k_init32, grid, block, 0, s, g, (const float *)rho, (const float *)T,
-                       (const float *)u, (const float *)v, (float *)f, (float *)gp, (float *)F,
-                       c.g0, c.rho0, c.T0);
+                       (const float *)uv, (float *)f, (float *)gp, (float *)F, c.g0, c.rho0, c.T0);
   }
   return hipGetLastError();
 }
@@ -190,17 +185,14 @@ int energy_blocks(const Geom &g) {
   return (int)(nb < 1024 ? nb : 1024);
 }
 
-hipError_t launch_energy(hipStream_t s, int prec, const Geom &g, const void *rho, const void *u,
-                         const void *v, const void *T, double *partial, int nblocks,
-                         double *out2) {
+hipError_t launch_energy(hipStream_t s, int prec, const Geom &g, const void *rho, const void *uv,
+                         const void *T, double *partial, int nblocks, double *out2) {
   if (prec == RBN_FP64)
     hipLaunchKernelGGL(k_energy_partial<double>, dim3(nblocks), dim3(kRedBlock), 0, s, g,
-                       (const double *)rho, (const double *)u, (const double *)v,
-                       (const double *)T, partial);
+                       (const double *)rho, (const double *)uv, (const double *)T, partial);
   else
     hipLaunchKernelGGL(k_energy_partial<float>, dim3(nblocks), dim3(kRedBlock), 0, s, g,
-                       (const float *)rho, (const float *)u, (const float *)v,
-                       (const float *)T, partial);
+                       (const float *)rho, (const float *)uv, (const float *)T, partial);
   hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(64), 0, s, partial, nblocks,
                      (double)g.nx * (double)g.ny, out2);
   return hipGetLastError();

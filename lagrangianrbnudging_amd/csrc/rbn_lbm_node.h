@@ -60,6 +60,16 @@ struct Planes {
   }
 };
 
+// Store of the interleaved (u, v) pair of one node as one vector.
+template <int NT, typename T>
+__device__ __forceinline__ void st_uv(void *uv, size_t o, T u, T v) {
+  typedef T T2 __attribute__((ext_vector_type(2)));
+  T2 w;
+  w.x = u;
+  w.y = v;
+  st<NT>(&((T2 *)uv)[o], w);
+}
+
 template <int NT, typename T>
 __device__ __forceinline__ T ld(const T *p) {
   if (NT & 2) return __builtin_nontemporal_load(p);
@@ -212,8 +222,7 @@ __device__ __forceinline__ void lbm_node64(const Geom &g, const LbmPtrs &p, cons
   const double u = fl[1] - fl[3] + fl[5] - fl[6] - fl[7] + fl[8] + 0.5 * rhoi * Fx;
   const double v = fl[5] + fl[2] + fl[6] - fl[7] - fl[4] - fl[8] + 0.5 * rhoi * Fy;
   st<NT>(&F[r0 + i], c.g0 * rho * (T - 0.0));
-  st<NT>(&((double *)p.u)[r0 + i], u);
-  st<NT>(&((double *)p.v)[r0 + i], v);
+  st_uv<NT, double>(p.uv, r0 + i, u, v);
   if (p.rho) ((double *)p.rho)[r0 + i] = rho;
   if (p.T) ((double *)p.T)[r0 + i] = T;
 
@@ -365,8 +374,7 @@ __device__ __forceinline__ void lbm_node32(const Geom &g, const LbmPtrs &p, cons
   const float u = jx * rhoi;
   const float v = jy * rhoi + 0.5f * rhoi * Fy;
   st<NT>(&F[r0 + i], c.g0 * rho * T);
-  st<NT>(&((float *)p.u)[r0 + i], u);
-  st<NT>(&((float *)p.v)[r0 + i], v);
+  st_uv<NT, float>(p.uv, r0 + i, u, v);
   if (p.rho) ((float *)p.rho)[r0 + i] = rho;
   if (p.T) ((float *)p.T)[r0 + i] = T;
 

@@ -24,18 +24,34 @@
 namespace rbn {
 namespace part {
 
+// A node field: node o's value is a[o * cs] (cs = 1 for scalar planes; 2 for the u or v
+// component of an interleaved velocity slot, a pointing at the component).
 template <typename R>
 struct Field {
   const R *__restrict__ a;
-  int nx, ny, pitch;
+  int nx, ny, pitch, cs;
   // Value at halo coordinates ic in [0, nx+1], jc in [0, ny+1]: ghosts are periodic copies
   // (communication on one rank, periodic in both directions).
   __device__ __forceinline__ double at(int ic, int jc) const {
     const int i = ic == 0 ? nx : (ic == nx + 1 ? 1 : ic);
     const int j = jc == 0 ? ny : (jc == ny + 1 ? 1 : jc);
-    return (double)a[(size_t)(j - 1) * pitch + (i - 1)];
+    return (double)a[((size_t)(j - 1) * pitch + (i - 1)) * cs];
   }
 };
+
+// The u and v fields of an interleaved velocity slot, and a scalar plane.
+template <typename R>
+__device__ __host__ __forceinline__ Field<R> u_of(const void *uv, int nx, int ny, int pitch) {
+  return Field<R>{(const R *)uv, nx, ny, pitch, 2};
+}
+template <typename R>
+__device__ __host__ __forceinline__ Field<R> v_of(const void *uv, int nx, int ny, int pitch) {
+  return Field<R>{(const R *)uv + 1, nx, ny, pitch, 2};
+}
+template <typename R>
+__device__ __host__ __forceinline__ Field<R> scalar_of(const void *a, int nx, int ny, int pitch) {
+  return Field<R>{(const R *)a, nx, ny, pitch, 1};
+}
 
 // Lower corner of the interpolation cell (C truncation, M:293-296) and whether all four
 // corners lie on the halo-extended lattice, i.e. x in (-1, nx+1), y in (-1, ny+1).  The

@@ -50,48 +50,55 @@ __global__ void k_sample(Field<R> F, int n, const double *x, const double *y, do
 }
 
 template <typename R>
-Field<R> mk(const void *a, const Geom &g) {
-  return Field<R>{(const R *)a, g.nx, g.ny, g.pitch};
+Field<R> mk(const void *a, const Geom &g, int cs = 1) {
+  return Field<R>{(const R *)a, g.nx, g.ny, g.pitch, cs};
+}
+template <typename R>
+Field<R> mk_u(const void *uv, const Geom &g) {
+  return u_of<R>(uv, g.nx, g.ny, g.pitch);
+}
+template <typename R>
+Field<R> mk_v(const void *uv, const Geom &g) {
+  return v_of<R>(uv, g.nx, g.ny, g.pitch);
 }
 
 }  // namespace
 
-hipError_t launch_particles(hipStream_t s, int prec, const Geom &g, const void *u, const void *v,
-                            const void *up, const void *vp, const void *T, const PartPtrs &pp,
-                            int need_deriv, int record, int full) {
+hipError_t launch_particles(hipStream_t s, int prec, const Geom &g, const void *uv, const void *uvp,
+                            const void *T, const PartPtrs &pp, int need_deriv, int record, int full) {
   (void)need_deriv;
   if (pp.n <= 0) return hipSuccess;
   dim3 grid((pp.n + kBlock - 1) / kBlock), block(kBlock);
   if (prec == RBN_FP64)
-    hipLaunchKernelGGL(k_particles<double>, grid, block, 0, s, mk<double>(u, g), mk<double>(v, g),
-                       mk<double>(up, g), mk<double>(vp, g), mk<double>(T, g), pp, record, full);
+    hipLaunchKernelGGL(k_particles<double>, grid, block, 0, s, mk_u<double>(uv, g), mk_v<double>(uv, g),
+                       mk_u<double>(uvp, g), mk_v<double>(uvp, g), mk<double>(T, g), pp, record, full);
   else
-    hipLaunchKernelGGL(k_particles<float>, grid, block, 0, s, mk<float>(u, g), mk<float>(v, g),
-                       mk<float>(up, g), mk<float>(vp, g), mk<float>(T, g), pp, record, full);
+    hipLaunchKernelGGL(k_particles<float>, grid, block, 0, s, mk_u<float>(uv, g), mk_v<float>(uv, g),
+                       mk_u<float>(uvp, g), mk_v<float>(uvp, g), mk<float>(T, g), pp, record, full);
   return hipGetLastError();
 }
 
-hipError_t launch_sample(hipStream_t s, int prec, const Geom &g, const void *field, int n,
+hipError_t launch_sample(hipStream_t s, int prec, const Geom &g, const void *field, int cs, int n,
                          const double *x, const double *y, double *out, int *err) {
   if (n <= 0) return hipSuccess;
   dim3 grid((n + kBlock - 1) / kBlock), block(kBlock);
   if (prec == RBN_FP64)
-    hipLaunchKernelGGL(k_sample<double>, grid, block, 0, s, mk<double>(field, g), n, x, y, out, err);
+    hipLaunchKernelGGL(k_sample<double>, grid, block, 0, s, mk<double>(field, g, cs), n, x, y, out, err);
   else
-    hipLaunchKernelGGL(k_sample<float>, grid, block, 0, s, mk<float>(field, g), n, x, y, out, err);
+    hipLaunchKernelGGL(k_sample<float>, grid, block, 0, s, mk<float>(field, g, cs), n, x, y, out, err);
   return hipGetLastError();
 }
 
-hipError_t launch_particle_init(hipStream_t s, int prec, const Geom &g, const void *u,
-                                const void *v, const PartPtrs &pp, double heavy_factor) {
+hipError_t launch_particle_init(hipStream_t s, int prec, const Geom &g, const void *uv,
+                                const PartPtrs &pp, double heavy_factor) {
   if (pp.n <= 0) return hipSuccess;
   dim3 grid((pp.n + kBlock - 1) / kBlock), block(kBlock);
   if (prec == RBN_FP64)
-    hipLaunchKernelGGL(k_particle_init<double>, grid, block, 0, s, mk<double>(u, g),
-                       mk<double>(v, g), pp, heavy_factor);
+    hipLaunchKernelGGL(k_particle_init<double>, grid, block, 0, s, mk_u<double>(uv, g),
+                       mk_v<double>(uv, g), pp, heavy_factor);
   else
-    hipLaunchKernelGGL(k_particle_init<float>, grid, block, 0, s, mk<float>(u, g),
-                       mk<float>(v, g), pp, heavy_factor);
+    hipLaunchKernelGGL(k_particle_init<float>, grid, block, 0, s, mk_u<float>(uv, g),
+                       mk_v<float>(uv, g), pp, heavy_factor);
   return hipGetLastError();
 }
 
