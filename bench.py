@@ -152,6 +152,100 @@ def run_ensemble(rbn, steps, warmup):
     return out
 
 
+NUD_NX, NUD_NY, NUD_RA = 1024, 512, 1e7
+NUD_OBS, NUD_EVERY, NUD_ALPHA = 10_000, 5, 0.1
+
+
+def run_nudging(rbn, steps, spinup, chunks=10):
+    """Nudging configuration: a truth run at 1024 x 512, Ra = 1e7, spun up from the
+    conduction profile with 1e-2 noise (seed 5) into developed convection, then followed for
+    `steps` steps with 1e4 tracers recorded every 5 steps; nudged runs start from the
+    conduction profile with another noise seed (6) and are nudged toward the recorded
+    temperatures along the same trajectories (alpha = 0.1).  Reported: the nudged FP32 step
+    rate (bilinear deposit), the L2 temperature error against the truth every steps/chunks
+    steps for the bilinear and the 7x7 deposit and for a free (alpha = 0) run, and the
+    FP32-vs-FP64 temperature difference of the bilinear nudged run at the end."""
+    pt = rbn.default_params(nx=NUD_NX, ny=NUD_NY, Tup=0.0, Tdown=1.0, scratch=1, nstart=0, nout=10**9,
+                            noutconfig=NUD_EVERY, prtcl_writefull=0, particle=1,
+                            precision=rbn.RBN_FP32, T0_shift=0.5, rho0_shift=1.0)
+    pt = rbn.params_from_ra(pt, NUD_RA, PR, U_FF)
+    lin = np.tile(pt.Tdown + (pt.Tup - pt.Tdown) * np.arange(NUD_NY) / (NUD_NY - 1), (NUD_NX, 1))
+    T_truth0 = lin + np.random.default_rng(5).uniform(-1e-2, 1e-2, lin.shape)
+    T_first = lin + np.random.default_rng(6).uniform(-1e-2, 1e-2, lin.shape)
+    rng = np.random.default_rng(7)
+    ids = np.arange(NUD_OBS, dtype=np.int32)
+    x = rng.uniform(0.0, NUD_NX, NUD_OBS)
+    x[x == 0.0] = 0.5
+    y = rng.uniform(1.0, NUD_NY, NUD_OBS)
+    chunk = max(steps // chunks, NUD_EVERY)
+    truth = rbn.Solver(pt, 0)
+    truth.init_fields(T=T_truth0)
+    truth.step(spinup)
+    e_spin = truth.energy()
+    T_start = truth.fields()["T"]
+    truth.init_particles(ids, x, y, tau_p=np.zeros(NUD_OBS))
+    T_at = truth.sample("T", x, y)
+    truth.collect_records()
+    T_ref = []
+    for _ in range(chunks):
+        truth.step(chunk)
+        T_ref.append(truth.fields()["T"])
+    truth.step(NUD_EVERY)  # the record the last interval reads
+    recs = {r["step"] - spinup: {"id": r["id"], "x": r["x"], "y": r["y"], "temp": r["temp"]}
+            for r in truth.records}
+    truth.close()
+
+    def nudged(prec, alpha, deposit):
+        q = pt.copy()
+        q.variant = rbn.RBN_VARIANT_NUDGING
+        q.ntempread = NUD_EVERY
+        q.alpha = alpha
+        q.precision = prec
+        q.deposit = deposit
+        s = rbn.Solver(q, 0)
+        s.set_observation_source(recs)
+        s.init_fields(T=T_first)
+        s.init_observers(ids, x, y, T_at)
+        return s
+
+    def l2_curve(prec, alpha, deposit):
+        s = nudged(prec, alpha, deposit)
+        out, T = [], None
+        for k in range(chunks):
+            s.step(chunk)
+            T = s.fields()["T"]
+            out.append(float(f"{np.sqrt(np.mean((T - T_ref[k]) ** 2)):.4g}"))
+        s.close()
+        return out, T
+
+    bil = rbn.RBN_DEPOSIT_BILINEAR
+    s = nudged(rbn.RBN_FP32, NUD_ALPHA, bil)
+    s.step(NUD_EVERY * 4)
+    s.synchronize()
+    n = chunk * chunks - NUD_EVERY * 4
+    t0 = time.perf_counter()
+    s.step(n)
+    s.synchronize()
+    dt = time.perf_counter() - t0
+    s.close()
+    e32, T32 = l2_curve(rbn.RBN_FP32, NUD_ALPHA, bil)
+    e64, T64 = l2_curve(rbn.RBN_FP64, NUD_ALPHA, bil)
+    ebox, _ = l2_curve(rbn.RBN_FP32, NUD_ALPHA, rbn.RBN_DEPOSIT_BOX7)
+    efree, _ = l2_curve(rbn.RBN_FP32, 0.0, bil)
+    return {"config": f"{NUD_NX}x{NUD_NY}, Ra={NUD_RA:g}, truth spun up {spinup} steps, {NUD_OBS} tracer "
+                      f"observations every {NUD_EVERY} steps, alpha={NUD_ALPHA}, {chunk * chunks} steps",
+            "mlups_fp32_bilinear": round(NUD_NX * NUD_NY * n / dt / 1e6, 1),
+            "ms_per_step_fp32": round(1e3 * dt / n, 4),
+            "truth_energy_T_at_start": [float(f"{v:.4g}") for v in e_spin],
+            "l2_T_error_initial": float(f"{np.sqrt(np.mean((T_first - T_start) ** 2)):.4g}"),
+            "every_steps": chunk,
+            "l2_T_error_bilinear_fp32": e32,
+            "l2_T_error_bilinear_fp64": e64,
+            "l2_T_error_box7_fp32": ebox,
+            "l2_T_error_free_fp32": efree,
+            "fp32_vs_fp64_max_dT": float(f"{np.max(np.abs(T32 - T64)):.3g}")}
+
+
 ADJ_NX, ADJ_NY, ADJ_RA = 512, 256, 1e7
 ADJ_OBS, ADJ_EVERY, ADJ_ALPHA = 10_000, 10, 0.1
 
@@ -289,6 +383,9 @@ def main() -> int:
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 companion run")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble run")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-window run")
+    ap.add_argument("--no-nudging", action="store_true", help="skip the nudging run")
+    ap.add_argument("--nud-steps", type=int, default=2000)
+    ap.add_argument("--nud-spinup", type=int, default=30000)
     ap.add_argument("--adj-window", type=int, default=2000)
     ap.add_argument("--adj-ckpt", type=int, default=100)
     args = ap.parse_args()
@@ -312,6 +409,10 @@ def main() -> int:
     if not args.no_ensemble:
         ens = run_ensemble(rbn, max(args.steps // 5, 8), 20)
         print(f"bench: ensemble done {ens}", file=sys.stderr, flush=True)
+    nud = None
+    if not args.no_nudging:
+        nud = run_nudging(rbn, args.nud_steps, args.nud_spinup)
+        print(f"bench: nudging done {nud}", file=sys.stderr, flush=True)
     adj = None
     if not args.no_adjoint:
         adj = run_adjoint(rbn, args.adj_window, args.adj_ckpt)
@@ -362,6 +463,7 @@ def main() -> int:
             "roofline_frac": round(NX * NY * lbm_bytes_per_node(0) / (r64["lbm_avg_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
         },
         "ensemble": ens,
+        "nudging": nud,
         "adjoint": adj,
         "finite": r32["finite"] and (r64 is None or r64["finite"]) and (adj is None or adj["finite"]),
     }

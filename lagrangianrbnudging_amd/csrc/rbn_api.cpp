@@ -329,29 +329,63 @@ int run_energy(rbn_ctx *c, double out[2]) {
 int load_observation(rbn_ctx *c, int rstep) {
   const int n = c->nobs;
   if (!c->obs_fn) return fail(c, RBN_ESTATE, "no observation provider set (record %d needed)", rstep);
-  c->h_rid.assign((size_t)n, 0);
-  c->h_rx.assign((size_t)n, 0.0);
-  c->h_ry.assign((size_t)n, 0.0);
-  c->h_rT.assign((size_t)n, 0.0);
-  if (c->obs_fn(c->obs_user, rstep, n, c->h_rid.data(), c->h_rx.data(), c->h_ry.data(), c->h_rT.data()) != 0)
+  rbn_ctx::ObsSlot &s = c->obs_slot[c->obs_next];
+  c->obs_next ^= 1;
+  const size_t N = (size_t)(n > 0 ? n : 1);
+  const size_t bytes = 3 * N * sizeof(double) + 2 * N * sizeof(int);
+  if (!s.copied) HIPCHK(c, hipEventCreateWithFlags(&s.copied, hipEventDisableTiming));
+  HIPCHK(c, hipEventSynchronize(s.copied));  // the slot's previous upload has finished
+  if (s.cap < N) {
+    if (s.h) HIPCHK(c, hipHostFree(s.h));
+    s.h = nullptr;
+    s.cap = 0;
+    if (hipHostMalloc((void **)&s.h, bytes, hipHostMallocDefault) != hipSuccess) {
+      s.h = nullptr;
+      return fail(c, RBN_ENOMEM, "pinned observation buffer allocation failed");
+    }
+    s.cap = N;
+  }
+  double *rx = (double *)s.h, *ry = rx + N, *rT = ry + N;
+  int *rid = (int *)(rT + N), *chk = rid + N;
+  if (c->obs_fn(c->obs_user, rstep, n, rid, rx, ry, rT) != 0)
     return fail(c, RBN_ECALLBACK, "observation provider failed for record %d", rstep);
-  std::unordered_map<int, int> first;
-  first.reserve((size_t)n * 2);
-  for (int j = 0; j < n; j++) first.emplace(c->h_rid[(size_t)j], j);
-  c->h_chk.assign((size_t)n, 0);
-  for (int i = 0; i < n; i++) {
-    auto it = first.find(c->oid[(size_t)i]);
-    if (it == first.end()) return fail(c, RBN_ENOTFOUND, "observer id %d missing from record %d", c->oid[(size_t)i], rstep);
-    c->h_chk[(size_t)i] = it->second;
+  // chk[i] = the first record entry with observer i's id (N:1480-1487).
+  if (c->prev_rid.size() == (size_t)n && std::equal(rid, rid + n, c->prev_rid.begin())) {
+    std::copy(c->prev_chk.begin(), c->prev_chk.end(), chk);
+  } else {
+    int lo = 0, hi = -1;
+    if (n > 0) {
+      lo = *std::min_element(rid, rid + n);
+      hi = *std::max_element(rid, rid + n);
+    }
+    const long long range = (long long)hi - (long long)lo + 1;
+    if (n > 0 && range <= 4LL * n + 1024) {  // dense ids: a direct table
+      std::vector<int> first((size_t)range, -1);
+      for (int j = n - 1; j >= 0; j--) first[(size_t)(rid[j] - lo)] = j;
+      for (int i = 0; i < n; i++) {
+        const long long id = c->oid[(size_t)i];
+        const int j = (id >= lo && id <= hi) ? first[(size_t)(id - lo)] : -1;
+        if (j < 0) return fail(c, RBN_ENOTFOUND, "observer id %d missing from record %d", c->oid[(size_t)i], rstep);
+        chk[i] = j;
+      }
+    } else {
+      std::unordered_map<int, int> first;
+      first.reserve((size_t)n * 2);
+      for (int j = 0; j < n; j++) first.emplace(rid[j], j);
+      for (int i = 0; i < n; i++) {
+        auto it = first.find(c->oid[(size_t)i]);
+        if (it == first.end()) return fail(c, RBN_ENOTFOUND, "observer id %d missing from record %d", c->oid[(size_t)i], rstep);
+        chk[i] = it->second;
+      }
+    }
+    c->prev_rid.assign(rid, rid + n);
+    c->prev_chk.assign(chk, chk + n);
   }
   rbn::ObsPtrs o = obs_ptrs(c);
-  const size_t b = (size_t)n * sizeof(double);
-  HIPCHK(c, hipMemcpyAsync((void *)o.rx, c->h_rx.data(), b, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, hipMemcpyAsync((void *)o.ry, c->h_ry.data(), b, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, hipMemcpyAsync((void *)o.rT, c->h_rT.data(), b, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, hipMemcpyAsync((void *)o.chk, c->h_chk.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice, c->stream));
-  // The host vectors are reused on the next interval: wait for the copies.
-  HIPCHK(c, hipStreamSynchronize(c->stream));
+  // rx, ry, rT are consecutive arrays on the device and in the slot
+  HIPCHK(c, hipMemcpyAsync((void *)o.rx, rx, 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync((void *)o.chk, chk, (size_t)n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipEventRecord(s.copied, c->stream));
   return RBN_OK;
 }
 
@@ -405,6 +439,10 @@ void rbn_destroy(rbn_ctx *c) {
     if (e) (void)hipEventDestroy(e);
   if (c->pstream) (void)hipStreamDestroy(c->pstream);
   if (c->copy_stream) (void)hipStreamSynchronize(c->copy_stream);
+  for (auto &s : c->obs_slot) {
+    if (s.h) (void)hipHostFree(s.h);
+    if (s.copied) (void)hipEventDestroy(s.copied);
+  }
   dfree_t(c->d_part);
   free_record_slots(c);
   for (auto &s : c->rec) {
@@ -814,6 +852,8 @@ int rbn_init_observers(rbn_ctx *c, int n, const int *id, const double *x, const 
   dfree_t(c->dep.list);
   c->nobs = n;
   c->oid.assign(id, id + n);
+  c->prev_rid.clear();
+  c->prev_chk.clear();
   const size_t N = (size_t)(n ? n : 1);
   void *tmp = nullptr;
   if ((rc = dalloc(c, &tmp, ONI * N * sizeof(int)))) return rc;

@@ -190,9 +190,17 @@ struct rbn_ctx {
   double *d_obs_d = nullptr;  // dx, dy, dT, xint, yint, Tint, rx, ry, rT (9 * nobs)
   rbn_obs_fn obs_fn = nullptr;
   void *obs_user = nullptr;
-  std::vector<int> h_rid;
-  std::vector<double> h_rx, h_ry, h_rT;
-  std::vector<int> h_chk;
+  // Record staging: two pinned slots (x, y, T as doubles, then ids and the observer map as
+  // ints) so that the upload of a record is queued on the stream without waiting for it;
+  // a slot is refilled only after the event of its previous upload.  The observer -> record
+  // map is rebuilt only when a record's ids differ from the previous record's.
+  struct ObsSlot {
+    unsigned char *h = nullptr;
+    size_t cap = 0;
+    hipEvent_t copied = nullptr;
+  } obs_slot[2];
+  int obs_next = 0;
+  std::vector<int> prev_rid, prev_chk;
   rbn::DepPtrs dep{};
 
   // diagnostics

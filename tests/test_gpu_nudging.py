@@ -116,6 +116,26 @@ def test_bilinear_deposit_fp32_against_fp64():
     assert_equal_arrays(out[rbn.RBN_FP32][1]["weight"], out[rbn.RBN_FP64][1]["weight"], "weight")
 
 
+def test_sparse_observer_ids_fp64_bit_exact():
+    # ids far apart (the hash-map match), consecutive records in the same order (the previous
+    # map is reused) and in a new order (the map is rebuilt)
+    pn = rb_params(40, 24, 1e5)
+    pn.variant = rbn.RBN_VARIANT_NUDGING
+    pn.ntempread = 4
+    pn.alpha = 0.3
+    ids, x0, y0, T, recs = synthetic_observations(pn.nx, pn.ny, 50, 64, 4, seed=11)
+    big = (ids.astype(np.int64) * 1000003 % 2000000011).astype(np.int32)
+    for k, (st, r) in enumerate(sorted(recs.items())):
+        r["id"] = big[r["id"]]
+        if k % 3 != 2:  # sorted by id: two consecutive records share one order
+            order = np.argsort(r["id"])
+            for key in ("id", "x", "y", "temp"):
+                r[key] = r[key][order]
+    T0n = conduction_T(pn, 1e-2, seed=4)
+    g, o = _nudged_pair(pn, T0n, big, x0, y0, T, recs, 61)
+    _compare(g, o)
+
+
 def test_missing_observer_id_is_an_error():
     pn = rb_params(32, 16, 1e5)
     pn.variant = rbn.RBN_VARIANT_NUDGING
