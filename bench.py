@@ -158,9 +158,10 @@ NUD_OBS, NUD_EVERY, NUD_ALPHA = 10_000, 5, 0.1
 
 def run_nudging(rbn, steps, spinup, chunks=10):
     """Nudging configuration: a truth run at 1024 x 512, Ra = 1e7, spun up from the
-    conduction profile with 1e-2 noise (seed 5) into developed convection, then followed for
+    conduction profile with four seeded convection rolls and 1e-3 noise (seed 5) into
+    developed convection, then followed for
     `steps` steps with 1e4 tracers recorded every 5 steps; nudged runs start from the
-    conduction profile with another noise seed (6) and are nudged toward the recorded
+    conduction profile with independent noise (seed 6) and are nudged toward the recorded
     temperatures along the same trajectories (alpha = 0.1).  Reported: the nudged FP32 step
     rate (bilinear deposit), the L2 temperature error against the truth every steps/chunks
     steps for the bilinear and the 7x7 deposit and for a free (alpha = 0) run, and the
@@ -170,8 +171,11 @@ def run_nudging(rbn, steps, spinup, chunks=10):
                             precision=rbn.RBN_FP32, T0_shift=0.5, rho0_shift=1.0)
     pt = rbn.params_from_ra(pt, NUD_RA, PR, U_FF)
     lin = np.tile(pt.Tdown + (pt.Tup - pt.Tdown) * np.arange(NUD_NY) / (NUD_NY - 1), (NUD_NX, 1))
-    T_truth0 = lin + np.random.default_rng(5).uniform(-1e-2, 1e-2, lin.shape)
-    T_first = lin + np.random.default_rng(6).uniform(-1e-2, 1e-2, lin.shape)
+    # four convection rolls seeded in the truth so that the spin-up reaches developed flow
+    ii, jj = np.meshgrid(np.arange(NUD_NX), np.arange(NUD_NY), indexing="ij")
+    rolls = 0.05 * np.sin(np.pi * jj / (NUD_NY - 1)) * np.cos(2 * np.pi * 2 * ii / NUD_NX)
+    T_truth0 = lin + rolls + np.random.default_rng(5).uniform(-1e-3, 1e-3, lin.shape)
+    T_first = lin + np.random.default_rng(6).uniform(-1e-3, 1e-3, lin.shape)
     rng = np.random.default_rng(7)
     ids = np.arange(NUD_OBS, dtype=np.int32)
     x = rng.uniform(0.0, NUD_NX, NUD_OBS)

@@ -246,8 +246,9 @@ const char *rbn_ensemble_last_error(const rbn_ensemble *e);
 /* Advance every member by nsteps (graph replays for whole four-step cycles). */
 int rbn_ensemble_step(rbn_ensemble *e, int nsteps);
 /* Tile shape (tile_w in {64,128,256} nodes, tile_w*tile_h a multiple of 256), persistent
- * workgroups per CU (0: one workgroup per work item) and work order (0: member by member,
- * 1: members interleaved, largest first). */
+ * workgroups per CU (0: one workgroup per work item) and work order (bit 0 clear: member by
+ * member, set: members interleaved, largest first; bit 1: odd steps visit the lattice work
+ * items last first). */
 int rbn_ensemble_set_tuning(rbn_ensemble *e, int tile_w, int tile_h, int wg_per_cu, int order);
 int rbn_ensemble_get_tuning(const rbn_ensemble *e, int *tile_w, int *tile_h, int *wg_per_cu,
                             int *order);
@@ -275,6 +276,8 @@ int rbn_adj_backward(rbn_ctx *ctx, double *dJ_dT0, double *dJ_dalpha);
 /* Launch shape of the fused lattice step; results do not depend on it.  No reference
  * counterpart (the reference has no launch configuration).  Keys:
  *   "lbm_block"  nodes per workgroup along x: 64, 128, 256 (default) or 512
+ *   "lbm_npt"    nodes per thread along x: 1 (default), 2, 4 or 8 (row segments of
+ *                lbm_block * lbm_npt nodes per workgroup)
  *   "lbm_nt"     1: non-temporal stores of the step's outputs (default 1)
  *   "lbm_ntload" 1: non-temporal loads of the step's inputs (default 0)
  *   "lbm_xcd"    1: give each of the 8 XCDs a contiguous band of rows (default 0)

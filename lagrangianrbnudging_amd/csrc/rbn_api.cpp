@@ -1050,6 +1050,10 @@ int rbn_set_tuning(rbn_ctx *c, const char *key, int value) {
     c->tune.block = value;
   } else if (!strcmp(key, "lbm_nt")) {
     c->tune.nt = value != 0;
+  } else if (!strcmp(key, "lbm_npt")) {
+    if (value != 1 && value != 2 && value != 4 && value != 8)
+      return fail(c, RBN_EINVAL, "lbm_npt must be 1, 2, 4 or 8");
+    c->tune.npt = value;
   } else if (!strcmp(key, "lbm_ntload")) {
     c->tune.ntload = value != 0;
   } else if (!strcmp(key, "lbm_xcd")) {
@@ -1068,6 +1072,7 @@ int rbn_get_tuning(const rbn_ctx *c, const char *key, int *value) {
   if (!c || !key || !value) return rbn::thread_fail(RBN_EINVAL, "null argument");
   if (!strcmp(key, "lbm_block")) *value = c->tune.block;
   else if (!strcmp(key, "lbm_nt")) *value = c->tune.nt;
+  else if (!strcmp(key, "lbm_npt")) *value = c->tune.npt;
   else if (!strcmp(key, "lbm_ntload")) *value = c->tune.ntload;
   else if (!strcmp(key, "lbm_xcd")) *value = c->tune.xcd;
   else if (!strcmp(key, "lbm_reverse")) *value = c->tune.reverse;

@@ -98,10 +98,11 @@ int build(rbn_ensemble *e) {
     }
   }
   std::vector<int2> items;
-  // order 0: member after member; order 1: round-robin over members (small and large
-  // members interleaved, largest first).
+  // order bit 0 clear: member after member; set: round-robin over members (small and large
+  // members interleaved, largest first).  Bit 1: odd steps visit the lattice items in
+  // reverse (applied at launch).
   auto emit = [&](const std::vector<int> &cnt) {
-    if (e->order == 0) {
+    if ((e->order & 1) == 0) {
       for (int i = 0; i < n; i++)
         for (int t = 0; t < cnt[i]; t++) items.push_back(make_int2(i, t));
     } else {
@@ -140,12 +141,13 @@ int enqueue_step(rbn_ensemble *e, int k, bool last, bool cross_waits) {
   const int grid = grid_size(e);
   if (cross_waits) EHIP(e, hipStreamWaitEvent(e->stream, e->ev_part[k & 1], 0));
   EHIP(e, rbn::launch_ensemble_phase(e->stream, e->prec, 0, e->d_members, e->d_items, e->n_lbm,
-                                     e->d_counters, k, e->tile_w, e->tile_h, last ? 1 : 0, grid));
+                                     e->d_counters, k, e->tile_w, e->tile_h, last ? 1 : 0, grid,
+                                     (e->order & 2) && (k & 1)));
   if (e->n_part > 0) {
     EHIP(e, hipEventRecord(e->ev_lbm, e->stream));
     EHIP(e, hipStreamWaitEvent(e->pstream, e->ev_lbm, 0));
     EHIP(e, rbn::launch_ensemble_phase(e->pstream, e->prec, 1, e->d_members, e->d_items + e->n_lbm,
-                                       e->n_part, e->d_counters, k, e->tile_w, e->tile_h, 0, grid));
+                                       e->n_part, e->d_counters, k, e->tile_w, e->tile_h, 0, grid, 0));
     EHIP(e, hipEventRecord(e->ev_part[k & 1], e->pstream));
   }
   return RBN_OK;
@@ -329,8 +331,8 @@ const char *rbn_ensemble_last_error(const rbn_ensemble *e) { return e ? e->err.c
 int rbn_ensemble_set_tuning(rbn_ensemble *e, int tile_w, int tile_h, int wg_per_cu, int order) {
   if (!e) return rbn::thread_fail(RBN_EINVAL, "null ensemble");
   if (!(tile_w == 64 || tile_w == 128 || tile_w == 256) || tile_h < 1 || tile_h > 64 ||
-      (tile_w * tile_h) % 256 != 0 || wg_per_cu < 0 || wg_per_cu > 32 || order < 0 || order > 1)
-    return efail(e, RBN_EINVAL, "tile must be 64/128/256 wide with a multiple of 256 nodes; wg_per_cu 0..32; order 0/1");
+      (tile_w * tile_h) % 256 != 0 || wg_per_cu < 0 || wg_per_cu > 32 || order < 0 || order > 3)
+    return efail(e, RBN_EINVAL, "tile must be 64/128/256 wide with a multiple of 256 nodes; wg_per_cu 0..32; order 0..3");
   e->tile_w = tile_w;
   e->tile_h = tile_h;
   e->wg_per_cu = wg_per_cu;
@@ -374,7 +376,7 @@ int rbn_ensemble_autotune(rbn_ensemble *e, int trial_steps, double *best_ms_per_
   const int wpcs[] = {0, 2, 4, 8};
   for (auto &sh : shapes)
     for (int w : wpcs)
-      for (int o = 0; o < 2; o++) cands.push_back({sh[0], sh[1], w, o});
+      for (int o = 0; o < 4; o++) cands.push_back({sh[0], sh[1], w, o});
   int rc = sync_members(e);
   if (rc) return rc;
   Snapshot snap;

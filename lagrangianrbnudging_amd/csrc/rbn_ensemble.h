@@ -24,6 +24,6 @@ struct EnsMember {
 
 hipError_t launch_ensemble_phase(hipStream_t s, int prec, int phase, const void *members,
                                  const int2 *items, int n_items, int *counters, int k, int tile_w,
-                                 int tile_h, int write_moments, int grid);
+                                 int tile_h, int write_moments, int grid, int reverse);
 
 }  // namespace rbn

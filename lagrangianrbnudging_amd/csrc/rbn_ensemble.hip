@@ -85,16 +85,18 @@ __device__ __forceinline__ void clear_next(int *counters, int k, int phase) {
   if (blockIdx.x == 0 && threadIdx.x == 0) counters[((k + 1) & 1) * 2 + phase] = 0;
 }
 
+// reverse: visit the items last first (lattice phase on odd steps, so that a step starts on
+// the tiles its predecessor wrote last).
 template <typename R, int PHASE>
 __global__ __launch_bounds__(kThreads) void k_ensemble(const EnsMember<R> *__restrict__ M,
                                                        const int2 *__restrict__ items, int n_items,
                                                        int *counters, int k, int tile_w, int tile_h,
-                                                       int write_moments, int persistent) {
+                                                       int write_moments, int persistent, int reverse) {
   clear_next(counters, k, PHASE);
   if (!persistent) {
     const int it = blockIdx.x;
     if (it >= n_items) return;
-    const int2 w = items[it];
+    const int2 w = items[reverse ? n_items - 1 - it : it];
     if (PHASE == 0)
       lbm_item(M[w.x], w.y, k, tile_w, tile_h, write_moments);
     else
@@ -109,7 +111,7 @@ __global__ __launch_bounds__(kThreads) void k_ensemble(const EnsMember<R> *__res
     const int it = item;
     __syncthreads();
     if (it >= n_items) break;
-    const int2 w = items[it];
+    const int2 w = items[reverse ? n_items - 1 - it : it];
     if (PHASE == 0)
       lbm_item(M[w.x], w.y, k, tile_w, tile_h, write_moments);
     else
@@ -121,7 +123,7 @@ __global__ __launch_bounds__(kThreads) void k_ensemble(const EnsMember<R> *__res
 
 hipError_t launch_ensemble_phase(hipStream_t s, int prec, int phase, const void *members,
                                  const int2 *items, int n_items, int *counters, int k, int tile_w,
-                                 int tile_h, int write_moments, int grid) {
+                                 int tile_h, int write_moments, int grid, int reverse) {
   if (n_items <= 0) return hipSuccess;
   const int persistent = grid > 0;
   const dim3 g(persistent ? (unsigned)grid : (unsigned)n_items), b(kThreads);
@@ -129,18 +131,18 @@ hipError_t launch_ensemble_phase(hipStream_t s, int prec, int phase, const void 
     const auto *M = (const EnsMember<double> *)members;
     if (phase == 0)
       hipLaunchKernelGGL((k_ensemble<double, 0>), g, b, 0, s, M, items, n_items, counters, k, tile_w,
-                         tile_h, write_moments, persistent);
+                         tile_h, write_moments, persistent, reverse);
     else
       hipLaunchKernelGGL((k_ensemble<double, 1>), g, b, 0, s, M, items, n_items, counters, k, tile_w,
-                         tile_h, write_moments, persistent);
+                         tile_h, write_moments, persistent, reverse);
   } else {
     const auto *M = (const EnsMember<float> *)members;
     if (phase == 0)
       hipLaunchKernelGGL((k_ensemble<float, 0>), g, b, 0, s, M, items, n_items, counters, k, tile_w,
-                         tile_h, write_moments, persistent);
+                         tile_h, write_moments, persistent, reverse);
     else
       hipLaunchKernelGGL((k_ensemble<float, 1>), g, b, 0, s, M, items, n_items, counters, k, tile_w,
-                         tile_h, write_moments, persistent);
+                         tile_h, write_moments, persistent, reverse);
   }
   return hipGetLastError();
 }

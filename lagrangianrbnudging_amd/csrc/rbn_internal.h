@@ -96,6 +96,7 @@ struct DepPtrs {
 // update in FP32).
 struct LbmTuning {
   int block = 256;
+  int npt = 1;      // nodes per thread along x (1, 2, 4, 8): row segments of block * npt nodes
   int nt = 1;
   int ntload = 0;   // non-temporal loads of the step's inputs
   int xcd = 0;

@@ -40,7 +40,7 @@ def _compare(a, b):
 
 
 @pytest.mark.parametrize("prec", [rbn.RBN_FP64, rbn.RBN_FP32])
-@pytest.mark.parametrize("tuning", [(256, 1, 0, 0), (256, 2, 4, 1), (64, 8, 2, 0), (128, 4, 8, 1)])
+@pytest.mark.parametrize("tuning", [(256, 1, 0, 0), (256, 2, 4, 1), (64, 8, 2, 2), (128, 4, 8, 3), (256, 1, 0, 2)])
 def test_ensemble_equals_individual_steps(prec, tuning):
     ens_members = _members(prec)
     ref_members = _members(prec)

@@ -151,13 +151,14 @@ def test_launch_tuning_does_not_change_results(prec):
     ref = None
     shapes = [(b, nt, 0, x, 0) for b in (64, 128, 256, 512) for nt in (0, 1) for x in (0, 1)]
     shapes += [(256, 1, 1, 0, 0), (256, 1, 0, 0, 1), (512, 1, 1, 1, 1), (64, 0, 1, 0, 1)]
-    for block, nt, ntload, xcd, rev in shapes:
+    shapes = [s + (1,) for s in shapes] + [(256, 1, 0, 0, 1, 2), (128, 1, 0, 1, 1, 4), (64, 1, 0, 0, 0, 8)]
+    for block, nt, ntload, xcd, rev, npt in shapes:
         s = rbn.Solver(p)
-        s.set_tuning(lbm_block=block, lbm_nt=nt, lbm_ntload=ntload, lbm_xcd=xcd, lbm_reverse=rev)
+        s.set_tuning(lbm_block=block, lbm_nt=nt, lbm_ntload=ntload, lbm_xcd=xcd, lbm_reverse=rev, lbm_npt=npt)
         s.init_fields(T=T0)
         s.step(30)
         f, g = s.populations()
-        what = f"block={block} nt={nt} ntload={ntload} xcd={xcd} reverse={rev}"
+        what = f"block={block} nt={nt} ntload={ntload} xcd={xcd} reverse={rev} npt={npt}"
         if ref is None:
             ref = (f, g)
         else:
