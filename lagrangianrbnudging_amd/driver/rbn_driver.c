@@ -26,6 +26,11 @@
  *   --raw            write raw records even when HDF5 is available
  *   --steps N        override nsteps
  *   --verbose        print "Loop <istep> particles propagated" every step, as the reference
+ *   --restart-dir DIR  scratch 0: read the checkpoint of step nstart from DIR (default --out)
+ *                    instead of the reference's fixed absolute paths
+ *   --dump-every N   checkpoint every N steps (default 2000000, as the reference) and after
+ *                    the last step: confnew/congnew/confpre/congpre<istep>.h5 (/f, /g
+ *                    [nx][ny][9]) and prtcl_dump<istep>.in (+ prtcl_dump_id<istep>.in)
  */
 #define _POSIX_C_SOURCE 200809L
 #include <dlfcn.h>
@@ -112,6 +117,16 @@ static int h5_write_1d(h5id file, const char *name, int is_int, const void *data
 static int h5_write_2d(h5id file, const char *name, const double *data, int nx, int ny) {
   h5size dims[2] = {(h5size)nx, (h5size)ny};
   h5id space = H5.screate_simple(2, dims, NULL);
+  h5id ds = H5.dcreate2(file, name, *H5.t_double, space, H5P_DEFAULT_, H5P_DEFAULT_, H5P_DEFAULT_);
+  int rc = (space < 0 || ds < 0 || H5.dwrite(ds, *H5.t_double, H5S_ALL_, H5S_ALL_, H5P_DEFAULT_, data) < 0) ? -1 : 0;
+  if (ds >= 0) H5.dclose(ds);
+  if (space >= 0) H5.sclose(space);
+  return rc;
+}
+
+/* A double dataset of rank 1..3 with the given extents. */
+static int h5_write_nd(h5id file, const char *name, const double *data, int rank, const h5size *dims) {
+  h5id space = H5.screate_simple(rank, dims, NULL);
   h5id ds = H5.dcreate2(file, name, *H5.t_double, space, H5P_DEFAULT_, H5P_DEFAULT_, H5P_DEFAULT_);
   int rc = (space < 0 || ds < 0 || H5.dwrite(ds, *H5.t_double, H5S_ALL_, H5S_ALL_, H5P_DEFAULT_, data) < 0) ? -1 : 0;
   if (ds >= 0) H5.dclose(ds);
@@ -286,6 +301,244 @@ static void snapshot(rbn_ctx *ctx, const rbn_params *p, int istep) {
   free(v);
 }
 
+/* ---------------------------------------------------------------- checkpoint / restart */
+
+/* A checkpoint file: HDF5 <name><istep>.h5 with the given datasets, or raw
+ * <name><istep>.bin holding the same arrays back to back. */
+static int write_ckpt(const char *dir, const char *name, int istep, int nd, const char **dsn,
+                      const double **data, const size_t *count, int rank3, const rbn_params *p) {
+  char path[4200];
+  if (H5.ok && !g_raw) {
+    snprintf(path, sizeof path, "%s/%s%d.h5", dir, name, istep);
+    h5id f = H5.fcreate(path, H5F_ACC_TRUNC_, H5P_DEFAULT_, H5P_DEFAULT_);
+    if (f < 0) return -1;
+    int rc = 0;
+    for (int d = 0; d < nd && !rc; d++) {
+      const h5size dims3[3] = {(h5size)p->nx, (h5size)p->ny, 9}, dims2[2] = {(h5size)p->nx, (h5size)p->ny};
+      rc = (rank3 && d == 0) ? h5_write_nd(f, dsn[d], data[d], 3, dims3) : h5_write_nd(f, dsn[d], data[d], 2, dims2);
+    }
+    H5.fclose(f);
+    return rc;
+  }
+  snprintf(path, sizeof path, "%s/%s%d.bin", dir, name, istep);
+  FILE *f = fopen(path, "wb");
+  if (!f) return -1;
+  int ok = 1;
+  for (int d = 0; d < nd && ok; d++) ok = fwrite(data[d], sizeof(double), count[d], f) == count[d];
+  fclose(f);
+  return ok ? 0 : -1;
+}
+
+/* Reads dataset d of checkpoint file <name><istep> (<name> alone when istep < 0) into
+ * data[d]; returns how many datasets were read (optional trailing datasets may be absent). */
+static int read_ckpt(const char *dir, const char *name, int istep, int nd, const char **dsn, double **data,
+                     const size_t *count) {
+  char path[4200], num[32] = "";
+  int got = 0;
+  if (istep >= 0) snprintf(num, sizeof num, "%d", istep);
+  snprintf(path, sizeof path, "%s/%s%s.h5", dir, name, num);
+  if (H5.ok) {
+    h5id f = H5.fopen(path, H5F_ACC_RDONLY_, H5P_DEFAULT_);
+    if (f >= 0) {
+      for (int d = 0; d < nd; d++) {
+        if (h5_read_1d(f, dsn[d], 0, data[d], (long long)count[d]) != (long long)count[d]) break;
+        got++;
+      }
+      H5.fclose(f);
+      return got;
+    }
+  }
+  snprintf(path, sizeof path, "%s/%s%s.bin", dir, name, num);
+  FILE *f = fopen(path, "rb");
+  if (!f) return 0;
+  for (int d = 0; d < nd; d++) {
+    if (fread(data[d], sizeof(double), count[d], f) != count[d]) break;
+    got++;
+  }
+  fclose(f);
+  return got;
+}
+
+static const double kW9[9] = {16. / 36., 4. / 36., 4. / 36., 4. / 36., 4. / 36., 1. / 36., 1. / 36., 1. / 36., 1. / 36.};
+static const double kCx9[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+static const double kCy9[9] = {0, 0, 1, 0, -1, 1, 1, -1, -1};
+
+/* dumpf / dumpg / dumpfpre / dumpgpre (T:43-230) and prtcl_dump (T:1403-1455):
+ *   confnew<istep>  /f   [nx][ny][9]  populations after the step's collision
+ *   congnew<istep>  /g
+ *   confpre<istep>  /f   populations whose moments, by the restart formula of inithydro
+ *                        (T:705-727: v carries + 0.5*(-gravity)*T), are the step's moments:
+ *                        the library does not keep the pre-collision populations, so these are
+ *                        the equilibria of (rho, u, v - 0.5*(-gravity)*T); the exact moments
+ *                        follow as /rho /T /u /v, which the reference does not read
+ *   congpre<istep>  /g
+ *   prtcl_dump<istep>.in     8 doubles per particle (x y u v xo yo uo vo), ascending id
+ *   prtcl_dump_id<istep>.in  the ids (the reference takes them from the record file) */
+static void dump_state(rbn_ctx *ctx, const rbn_params *p, int istep) {
+  const size_t nn = (size_t)p->nx * (size_t)p->ny;
+  double *f = malloc(9 * nn * sizeof(double)), *g = malloc(9 * nn * sizeof(double));
+  double *fp = malloc(9 * nn * sizeof(double)), *gp = malloc(9 * nn * sizeof(double));
+  double *rho = malloc(nn * sizeof(double)), *T = malloc(nn * sizeof(double));
+  double *u = malloc(nn * sizeof(double)), *v = malloc(nn * sizeof(double));
+  CHECK(rbn_get_populations(ctx, f, g), ctx);
+  CHECK(rbn_get_fields(ctx, rho, T, u, v), ctx);
+  for (size_t o = 0; o < nn; o++) {
+    const double vh = v[o] - 0.5 * (-p->gravity) * T[o];
+    const double usq = u[o] * u[o] + vh * vh;
+    for (int k = 0; k < 9; k++) {
+      const double cu = kCx9[k] * u[o] + kCy9[k] * vh;
+      const double b = 1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * usq;
+      fp[9 * o + k] = rho[o] * kW9[k] * b;
+      gp[9 * o + k] = T[o] * kW9[k] * b;
+    }
+  }
+  const size_t c9 = 9 * nn;
+  const char *nf[] = {"/f"}, *ng[] = {"/g"}, *npre[] = {"/f", "/rho", "/T", "/u", "/v"};
+  const double *df[] = {f}, *dg[] = {g}, *dpre[] = {fp, rho, T, u, v}, *dgp[] = {gp};
+  const size_t cf[] = {c9}, cpre[] = {c9, nn, nn, nn, nn};
+  if (write_ckpt(g_out, "confnew", istep, 1, nf, df, cf, 1, p) || write_ckpt(g_out, "congnew", istep, 1, ng, dg, cf, 1, p) ||
+      write_ckpt(g_out, "confpre", istep, 5, npre, dpre, cpre, 1, p) ||
+      write_ckpt(g_out, "congpre", istep, 1, ng, dgp, cf, 1, p)) {
+    fprintf(stderr, "rbn_driver: cannot write the checkpoint of step %d\n", istep);
+    exit(1);
+  }
+  free(f);
+  free(g);
+  free(fp);
+  free(gp);
+  free(rho);
+  free(T);
+  free(u);
+  free(v);
+  int n = 0;
+  CHECK(rbn_num_particles(ctx, &n), ctx);
+  if (p->variant == RBN_VARIANT_TRAJECTORIES && p->particle && n > 0) {
+    double *a[8];
+    for (int q = 0; q < 8; q++) a[q] = malloc(sizeof(double) * (size_t)n);
+    int *id = malloc(sizeof(int) * (size_t)n);
+    rbn_particles st = {n, id, a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7]};
+    CHECK(rbn_get_particles(ctx, &st), ctx);
+    double *rec = malloc(8 * sizeof(double) * (size_t)n);
+    for (int k = 0; k < n; k++)
+      for (int q = 0; q < 8; q++) rec[8 * (size_t)k + q] = a[q][k];
+    char path[4200];
+    snprintf(path, sizeof path, "%s/prtcl_dump%d.in", g_out, istep);
+    int rc = write_raw(path, rec, 8 * sizeof(double) * (size_t)n);
+    snprintf(path, sizeof path, "%s/prtcl_dump_id%d.in", g_out, istep);
+    rc = rc || write_raw(path, id, sizeof(int) * (size_t)n);
+    if (rc) {
+      fprintf(stderr, "rbn_driver: cannot write %s\n", path);
+      exit(1);
+    }
+    for (int q = 0; q < 8; q++) free(a[q]);
+    free(id);
+    free(rec);
+  }
+}
+
+/* restoref / restoreg / restorefpre / restoregpre + inithydro's restart branch
+ * (T:351-680, 705-727) and restore_prtcl (T:251-349), reading the checkpoint of step nstart
+ * from `dir` instead of the reference's fixed absolute paths.  The moments of the step come
+ * from confpre's /rho /T /u /v when present (bit-exact continuation), else from its
+ * populations by the reference's formula. */
+static void restore_state(rbn_ctx *ctx, const rbn_params *p, const char *dir) {
+  const size_t nn = (size_t)p->nx * (size_t)p->ny, c9 = 9 * nn;
+  double *f = malloc(c9 * sizeof(double)), *g = malloc(c9 * sizeof(double));
+  double *fp = malloc(c9 * sizeof(double)), *gp = malloc(c9 * sizeof(double));
+  double *rho = malloc(nn * sizeof(double)), *T = malloc(nn * sizeof(double));
+  double *u = malloc(nn * sizeof(double)), *v = malloc(nn * sizeof(double));
+  const char *nf[] = {"/f"}, *ng[] = {"/g"}, *npre[] = {"/f", "/rho", "/T", "/u", "/v"};
+  double *df[] = {f}, *dg[] = {g}, *dpre[] = {fp, rho, T, u, v}, *dgp[] = {gp};
+  const size_t cf[] = {c9}, cpre[] = {c9, nn, nn, nn, nn};
+  const int st = p->nstart;
+  /* this driver's numbered files, else the reference's unnumbered conf.h5 / cong.h5 /
+   * confpre.h5 / congpre.h5 (T:375, 455, 535, 615) */
+  int ok = read_ckpt(dir, "confnew", st, 1, nf, df, cf) == 1 && read_ckpt(dir, "congnew", st, 1, ng, dg, cf) == 1;
+  if (!ok) ok = read_ckpt(dir, "conf", -1, 1, nf, df, cf) == 1 && read_ckpt(dir, "cong", -1, 1, ng, dg, cf) == 1;
+  if (!ok) {
+    fprintf(stderr, "rbn_driver: conf/cong checkpoint of step %d not found under %s\n", st, dir);
+    exit(1);
+  }
+  int got = read_ckpt(dir, "confpre", st, 5, npre, dpre, cpre);
+  int gotg = got >= 1 ? read_ckpt(dir, "congpre", st, 1, ng, dgp, cf) : 0;
+  if (got < 1 || gotg != 1) {
+    got = read_ckpt(dir, "confpre", -1, 5, npre, dpre, cpre);
+    gotg = got >= 1 ? read_ckpt(dir, "congpre", -1, 1, ng, dgp, cf) : 0;
+  }
+  if (got < 1 || gotg != 1) {
+    fprintf(stderr, "rbn_driver: confpre/congpre checkpoint of step %d not found under %s\n", st, dir);
+    exit(1);
+  }
+  if (got < 5) {
+    for (size_t o = 0; o < nn; o++) {
+      double r = 0.0, t = 0.0, fl[9];
+      for (int k = 0; k < 9; k++) r = r + fp[9 * o + k];
+      for (int k = 0; k < 9; k++) t = t + gp[9 * o + k];
+      const double ri = 1. / r;
+      for (int k = 0; k < 9; k++) fl[k] = fp[9 * o + k] * ri;
+      rho[o] = r;
+      T[o] = t;
+      u[o] = fl[1] - fl[3] + fl[5] - fl[6] - fl[7] + fl[8];
+      v[o] = fl[5] + fl[2] + fl[6] - fl[7] - fl[4] - fl[8] + (0.5 * (-p->gravity) * t);
+    }
+  }
+  CHECK(rbn_set_populations(ctx, f, g, rho, T, u, v), ctx);
+  free(f);
+  free(g);
+  free(fp);
+  free(gp);
+  free(rho);
+  free(T);
+  free(u);
+  free(v);
+  if (p->variant == RBN_VARIANT_TRAJECTORIES && p->particle && !p->particle_scratch) {
+    char path[4200];
+    snprintf(path, sizeof path, "%s/prtcl_dump%d.in", dir, st);
+    size_t cap = (size_t)1 << 30;
+    double *rec = malloc(cap);
+    long bytes = read_raw(path, rec, cap);
+    if (bytes <= 0 || bytes % 64) {
+      fprintf(stderr, "rbn_driver: cannot read %s\n", path);
+      exit(1);
+    }
+    const int n = (int)(bytes / 64);
+    int *id = malloc(sizeof(int) * (size_t)n);
+    snprintf(path, sizeof path, "%s/prtcl_dump_id%d.in", dir, st);
+    if (read_raw(path, id, sizeof(int) * (size_t)n) != (long)(sizeof(int) * (size_t)n)) {
+      /* as the reference: the ids of the record written at nstart */
+      int ok = 0;
+      snprintf(path, sizeof path, "%s/Prtcl_Veldata1/Particles%d.h5", dir, st);
+      if (H5.ok) {
+        h5id fh = H5.fopen(path, H5F_ACC_RDONLY_, H5P_DEFAULT_);
+        if (fh >= 0) {
+          ok = h5_read_1d(fh, "/id", 1, id, n) == n;
+          H5.fclose(fh);
+        }
+      }
+      if (!ok) {
+        fprintf(stderr, "rbn_driver: particle ids of step %d not found under %s\n", st, dir);
+        exit(1);
+      }
+    }
+    double *x = malloc(sizeof(double) * (size_t)n), *y = malloc(sizeof(double) * (size_t)n);
+    double *up = malloc(sizeof(double) * (size_t)n), *vp = malloc(sizeof(double) * (size_t)n);
+    for (int k = 0; k < n; k++) {
+      x[k] = rec[8 * (size_t)k];
+      y[k] = rec[8 * (size_t)k + 1];
+      up[k] = rec[8 * (size_t)k + 2];
+      vp[k] = rec[8 * (size_t)k + 3];
+    }
+    CHECK(rbn_init_particles(ctx, n, id, x, y, NULL, NULL, up, vp), ctx);
+    CHECK(rbn_set_record_callback(ctx, on_record, NULL), ctx);
+    free(rec);
+    free(id);
+    free(x);
+    free(y);
+    free(up);
+    free(vp);
+  }
+}
+
 /* ---------------------------------------------------------------- particle placement */
 
 /* init_prtcl's 40x40 grid of square centres (M:123-168) on one rank: node (i, j) of the
@@ -322,8 +575,8 @@ static int reference_grid(const rbn_params *p, double **px, double **py) {
 int main(int argc, char **argv) {
   rbn_params P;
   rbn_params_default(&P);
-  int device = 0, nfiles = 0, steps_override = -1;
-  const char *init = "ref", *pfile = NULL;
+  int device = 0, nfiles = 0, steps_override = -1, dump_every = 2000000;
+  const char *init = "ref", *pfile = NULL, *restart_dir = NULL;
   for (int a = 1; a < argc; a++) {
     if (!strcmp(argv[a], "--out") && a + 1 < argc) g_out = argv[++a];
     else if (!strcmp(argv[a], "--obs") && a + 1 < argc) g_obs = argv[++a];
@@ -331,6 +584,8 @@ int main(int argc, char **argv) {
     else if (!strcmp(argv[a], "--init") && a + 1 < argc) init = argv[++a];
     else if (!strcmp(argv[a], "--particles") && a + 1 < argc) pfile = argv[++a];
     else if (!strcmp(argv[a], "--steps") && a + 1 < argc) steps_override = atoi(argv[++a]);
+    else if (!strcmp(argv[a], "--restart-dir") && a + 1 < argc) restart_dir = argv[++a];
+    else if (!strcmp(argv[a], "--dump-every") && a + 1 < argc) dump_every = atoi(argv[++a]);
     else if (!strcmp(argv[a], "--raw")) g_raw = 1;
     else if (!strcmp(argv[a], "--verbose")) g_verbose = 1;
     else if (argv[a][0] == '-') {
@@ -343,22 +598,21 @@ int main(int argc, char **argv) {
   }
   if (!nfiles) {
     fprintf(stderr, "usage: rbn_driver [--out DIR] [--obs DIR] [--device N] [--init ref|conduction:AMP:SEED]\n"
-                    "                  [--particles FILE] [--raw] [--steps N] [--verbose] params-file...\n");
+                    "                  [--particles FILE] [--raw] [--steps N] [--verbose]\n"
+                    "                  [--restart-dir DIR] [--dump-every N] params-file...\n");
     return 2;
   }
   if (steps_override >= 0) P.nsteps = steps_override;
-  if (!P.scratch) {
-    fprintf(stderr, "rbn_driver: restart from dumps (scratch = 0) is not supported; set scratch 1\n");
-    return 2;
-  }
   h5_load(g_raw);
   mkdir_p(g_out);
   rbn_ctx *ctx = NULL;
   CHECK(rbn_create(&P, device, &ctx), NULL);
   const int nudged = P.variant == RBN_VARIANT_NUDGING;
 
-  /* inithydro + equilif/initpop + equilig/initpop */
-  if (!strncmp(init, "conduction:", 11)) {
+  /* inithydro + equilif/initpop + equilig/initpop, or the restart from a checkpoint */
+  if (!P.scratch) {
+    restore_state(ctx, &P, restart_dir ? restart_dir : g_out);
+  } else if (!strncmp(init, "conduction:", 11)) {
     double amp = 0.0;
     unsigned long long seed = 0;
     if (sscanf(init + 11, "%lf:%llu", &amp, &seed) != 2) {
@@ -464,11 +718,17 @@ int main(int argc, char **argv) {
       int to_snap = P.noutconfig - istep % P.noutconfig;
       if (to_snap < chunk) chunk = to_snap;
     }
+    if (dump_every > 0) {
+      int to_dump = dump_every - istep % dump_every;
+      if (to_dump < chunk) chunk = to_dump;
+    }
     CHECK(rbn_step(ctx, chunk), ctx);
     istep += chunk;
     if (g_verbose) printf("Loop %d particles propagated \n", istep);
     if (nudged && P.noutconfig > 0 && istep % P.noutconfig == 0) snapshot(ctx, &P, istep);
+    if (dump_every > 0 && istep % dump_every == 0 && istep < last) dump_state(ctx, &P, istep);
   }
+  dump_state(ctx, &P, istep); /* after the main loop (T:1762-1766) */
   if (g_evolution) fclose(g_evolution);
   rbn_destroy(ctx);
   return 0;

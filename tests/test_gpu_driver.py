@@ -105,6 +105,50 @@ def test_driver_truth_and_nudged_runs(tmp_path, raw):
     (tmp_path / f"done_{ext}").write_text((n / "evolution.dat").read_text())
 
 
+@pytest.mark.parametrize("raw", [True, False])
+def test_driver_checkpoint_restart_continues_exactly(tmp_path, raw):
+    # FP64 truth run with particles: 40 steps in one go against 24 steps, a checkpoint, and a
+    # restart (scratch 0, Particle_scratch 0) for the remaining 16 steps.
+    base = _params_text()
+    (tmp_path / "full.h").write_text(base)
+    (tmp_path / "first.h").write_text(base + "#define nsteps 24\n")
+    (tmp_path / "second.h").write_text(base + "#define scratch 0\n#define nstart 24\n#define nsteps 16\n"
+                                              "#define Particle_scratch 0\n")
+    flag = ["--raw"] if raw else []
+    _run([*flag, "--out", "full", "--init", "conduction:0.01:5", "full.h"], tmp_path)
+    _run([*flag, "--out", "first", "--init", "conduction:0.01:5", "first.h"], tmp_path)
+    ext = "bin" if raw else "h5"
+    if not (tmp_path / "first" / f"confnew24.{ext}").exists():
+        pytest.skip("HDF5 runtime library not available on this host")
+    for name in ("confnew", "congnew", "confpre", "congpre"):
+        assert (tmp_path / "first" / f"{name}24.{ext}").stat().st_size > 0
+    _run([*flag, "--out", "second", "--restart-dir", "first", "second.h"], tmp_path)
+    full, second = tmp_path / "full", tmp_path / "second"
+    if raw:
+        for name in ("confnew40.bin", "congnew40.bin"):
+            a = np.fromfile(full / name)
+            b = np.fromfile(second / name)
+            assert a.size == 64 * 32 * 9 and np.array_equal(a, b), name
+        a = np.fromfile(full / "confpre40.bin")
+        b = np.fromfile(second / "confpre40.bin")
+        assert np.array_equal(a, b)  # includes the step's exact moments
+        ids_a, rec_a = _read_bin_record(full / "Prtcl_Veldata1" / "Particles40.bin")
+        ids_b, rec_b = _read_bin_record(second / "Prtcl_Veldata1" / "Particles40.bin")
+        assert np.array_equal(ids_a, ids_b) and np.array_equal(rec_a, rec_b)
+    assert (full / "prtcl_dump40.in").read_bytes() == (second / "prtcl_dump40.in").read_bytes()
+    assert (full / "prtcl_dump_id40.in").read_bytes() == (second / "prtcl_dump_id40.in").read_bytes()
+    # the evolution lines after the restart continue the single run's
+    la = (full / "evolution.dat").read_text().splitlines()
+    lb = (second / "evolution.dat").read_text().splitlines()
+    assert lb == [l for l in la if int(l.split()[0]) > 24]
+
+
+def test_driver_restart_without_checkpoint_is_an_error(tmp_path):
+    (tmp_path / "r.h").write_text(_params_text() + "#define scratch 0\n#define nstart 10\n")
+    out = subprocess.run([str(DRIVER), "--raw", "r.h"], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert out.returncode != 0 and "checkpoint" in out.stderr
+
+
 def test_driver_rejects_bad_input(tmp_path):
     (tmp_path / "bad.h").write_text("#define nx 4\n#define ny 32\n#define scratch 1\n")
     out = subprocess.run([str(DRIVER), "bad.h"], cwd=tmp_path, capture_output=True, text=True, timeout=60)
