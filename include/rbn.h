@@ -181,6 +181,12 @@ int rbn_get_fields(rbn_ctx *ctx, double *rho, double *T, double *u, double *v);
 /* evolution.dat quantities <rho|u|^2/2>, <T> of the latest hydrovar
  * (Trajectories/code/temppart_dump.c:1739-1757). */
 int rbn_energy(rbn_ctx *ctx, double *mean_energy, double *mean_T);
+/* Row profiles of the latest step (profile, Nudging/code/tempvel_nudge.c:1104-1250):
+ * out[(j-1)*9 + q] for rows j = 1..ny, q = x averages of rho, u, v, u^2, v^2, T, vT,
+ * dzT = T(j+1) - T(j) (row ny reads row 1, the periodic ghost row), vorticity^2 (computed
+ * but not printed by the reference; periodic x, one-sided y differences at the walls).
+ * Sums in a fixed order on the device. */
+int rbn_profiles(rbn_ctx *ctx, double *out);
 /* Bilinear interpolation (bilinear, Trajectories/code/mympi2.h:292-311, periodic ghosts of
  * communication, :313-348) of one field of the latest hydrovar at n positions, e.g. the
  * initial particle temperatures init_prtcl writes (:196-201).  field: 0 rho, 1 T, 2 u, 3 v.

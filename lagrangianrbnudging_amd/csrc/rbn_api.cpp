@@ -714,6 +714,23 @@ int rbn_energy(rbn_ctx *c, double *mean_energy, double *mean_T) {
   return RBN_OK;
 }
 
+int rbn_profiles(rbn_ctx *c, double *out) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!out) return fail(c, RBN_EINVAL, "null output");
+  if (!c->initialised || !c->moments_valid) return fail(c, RBN_ESTATE, "moments of the latest step not available");
+  const size_t bytes = (size_t)c->geom.ny * 9 * sizeof(double);
+  void *buf = nullptr;
+  if ((rc = dalloc(c, &buf, bytes))) return rc;
+  rc = RBN_OK;
+  if (rbn::launch_profiles(c->stream, c->prec, c->geom, c->rho, c->T, c->uv[c->cur], (double *)buf) != hipSuccess ||
+      hipMemcpyAsync(out, buf, bytes, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess)
+    rc = fail(c, RBN_EDEVICE, "rbn_profiles: device operation failed");
+  (void)hipFree(buf);
+  return rc;
+}
+
 int rbn_sample(rbn_ctx *c, int field, int n, const double *x, const double *y, double *out) {
   int rc = check_ctx(c);
   if (rc) return rc;

@@ -128,6 +128,9 @@ hipError_t launch_deposit(hipStream_t s, const Geom &g, int right_limit, const O
 hipError_t launch_energy(hipStream_t s, int prec, const Geom &g, const void *rho, const void *uv,
                          const void *T, double *partial, int nblocks, double *out2);
 int energy_blocks(const Geom &g);
+// Row profiles of the latest moments into out[ny * 9] (device, FP64).
+hipError_t launch_profiles(hipStream_t s, int prec, const Geom &g, const void *rho, const void *T,
+                           const void *uv, double *out);
 
 }  // namespace rbn
 

@@ -135,6 +135,50 @@ __global__ void k_energy_final(const double *partial, int nb, double nxny, doubl
   out2[1] = t / nxny;
 }
 
+// ---------------------------------------------------------------------------- profiles
+
+// One block per row j: x averages of rho, u, v, u^2, v^2, T, vT, dzT, vorticity^2
+// (profile, N:1104-1250).  dzT = T(j+1) - T(j) with the top row reading row 1 (the periodic
+// ghost row); vorticity = dv/dx - du/dy with periodic x and one-sided y differences at the
+// walls.  Each thread sums a fixed column subset in order, then a fixed tree over the block:
+// the result does not depend on scheduling.
+template <typename R>
+__global__ __launch_bounds__(kRedBlock) void k_profiles(Geom g, const R *rho, const R *T, const R *uv,
+                                                        double *out) {
+  __shared__ double red[9][kRedBlock];
+  const int j = blockIdx.x;
+  const size_t r0 = (size_t)j * g.pitch;
+  const size_t ru = (size_t)(j == g.ny - 1 ? 0 : j + 1) * g.pitch;
+  const int jn = j == g.ny - 1 ? j : j + 1, js = j == 0 ? 0 : j - 1;
+  const double yscale = (j == 0 || j == g.ny - 1) ? 1.0 : 0.5;
+  double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int i = threadIdx.x; i < g.nx; i += kRedBlock) {
+    const size_t o = r0 + i;
+    const int il = i == 0 ? g.nx - 1 : i - 1, ir = i == g.nx - 1 ? 0 : i + 1;
+    const double u = uv[2 * o], v = uv[2 * o + 1], t = T[o];
+    const double dvbydx = 0.5 * ((double)uv[2 * (r0 + ir) + 1] - (double)uv[2 * (r0 + il) + 1]);
+    const double dubydy = yscale * ((double)uv[2 * ((size_t)jn * g.pitch + i)] - (double)uv[2 * ((size_t)js * g.pitch + i)]);
+    const double vor = dvbydx - dubydy;
+    acc[0] += (double)rho[o];
+    acc[1] += u;
+    acc[2] += v;
+    acc[3] += u * u;
+    acc[4] += v * v;
+    acc[5] += t;
+    acc[6] += v * t;
+    acc[7] += (double)T[ru + i] - t;
+    acc[8] += vor * vor;
+  }
+  for (int q = 0; q < 9; q++) red[q][threadIdx.x] = acc[q];
+  __syncthreads();
+  for (int s = kRedBlock / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+      for (int q = 0; q < 9; q++) red[q][threadIdx.x] += red[q][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x < 9) out[(size_t)j * 9 + threadIdx.x] = red[threadIdx.x][0] * (1 / (double)g.nx);
+}
+
 }  // namespace
 
 namespace {
@@ -188,6 +232,17 @@ hipError_t launch_init_eq(hipStream_t s, int prec, const Geom &g, const void *rh
     hipLaunchKernelGGL(k_init32, grid, block, 0, s, g, (const float *)rho, (const float *)T,
                        (const float *)uv, (float *)f, (float *)gp, (float *)F, c.g0, c.rho0, c.T0);
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_profiles(hipStream_t s, int prec, const Geom &g, const void *rho, const void *T,
+                           const void *uv, double *out) {
+  if (prec == RBN_FP64)
+    hipLaunchKernelGGL(k_profiles<double>, dim3(g.ny), dim3(kRedBlock), 0, s, g, (const double *)rho,
+                       (const double *)T, (const double *)uv, out);
+  else
+    hipLaunchKernelGGL(k_profiles<float>, dim3(g.ny), dim3(kRedBlock), 0, s, g, (const float *)rho,
+                       (const float *)T, (const float *)uv, out);
   return hipGetLastError();
 }
 

@@ -11,7 +11,7 @@
  *   nudged run (Nudging/code/tempvel_nudge.c:2254-2419): observers from the truth run's
  *     init_prtcl*0.in, records read from <obs>/Prtcl_Veldata1/Particles<step>.h5,
  *     Tempfolder/snaptemp<istep>.h5 (/T /u /v, [nx][ny]) every noutconfig steps,
- *     evolution.dat every 5000 steps.
+ *     profiles.<istep>.dat every nout steps, evolution.dat every 5000 steps.
  * HDF5 is loaded at run time; without it records are raw files Particles<step>.bin:
  *   int32 n, int32 id[n], then double arrays in the dataset order above.
  *
@@ -539,6 +539,27 @@ static void restore_state(rbn_ctx *ctx, const rbn_params *p, const char *dir) {
   }
 }
 
+/* profile (N:1104-1250): profiles.<istep>.dat, one line per row j = 1..ny with the x
+ * averages of rho, u, v, u^2, v^2, T, vT, dzT and the vorticity squared (the reference's
+ * format has that ninth %e but passes no argument for it). */
+static void write_profiles(rbn_ctx *ctx, const rbn_params *p, int istep) {
+  double *pr = malloc(sizeof(double) * 9 * (size_t)p->ny);
+  CHECK(rbn_profiles(ctx, pr), ctx);
+  char path[4200];
+  snprintf(path, sizeof path, "%s/profiles.%d.dat", g_out, istep);
+  FILE *f = fopen(path, "w");
+  if (!f) {
+    fprintf(stderr, "rbn_driver: cannot write %s\n", path);
+    exit(1);
+  }
+  for (int j = 1; j <= p->ny; j++) {
+    const double *r = pr + (size_t)(j - 1) * 9;
+    fprintf(f, "%d %e %e %e %e %e %e %e %e %e\n", j, r[0], r[1], r[2], r[3], r[4], r[5], r[6], r[7], r[8]);
+  }
+  fclose(f);
+  free(pr);
+}
+
 /* ---------------------------------------------------------------- particle placement */
 
 /* init_prtcl's 40x40 grid of square centres (M:123-168) on one rank: node (i, j) of the
@@ -722,10 +743,15 @@ int main(int argc, char **argv) {
       int to_dump = dump_every - istep % dump_every;
       if (to_dump < chunk) chunk = to_dump;
     }
+    if (nudged && P.nout > 0) {
+      int to_prof = P.nout - istep % P.nout;
+      if (to_prof < chunk) chunk = to_prof;
+    }
     CHECK(rbn_step(ctx, chunk), ctx);
     istep += chunk;
     if (g_verbose) printf("Loop %d particles propagated \n", istep);
     if (nudged && P.noutconfig > 0 && istep % P.noutconfig == 0) snapshot(ctx, &P, istep);
+    if (nudged && P.nout > 0 && istep % P.nout == 0) write_profiles(ctx, &P, istep); /* N:2382 */
     if (dump_every > 0 && istep % dump_every == 0 && istep < last) dump_state(ctx, &P, istep);
   }
   dump_state(ctx, &P, istep); /* after the main loop (T:1762-1766) */

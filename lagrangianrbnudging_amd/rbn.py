@@ -106,6 +106,7 @@ _SIGS = {
     "rbn_get_populations": (C.c_int, [C.c_void_p, _DP, _DP]),
     "rbn_get_fields": (C.c_int, [C.c_void_p, _DP, _DP, _DP, _DP]),
     "rbn_energy": (C.c_int, [C.c_void_p, _DP, _DP]),
+    "rbn_profiles": (C.c_int, [C.c_void_p, _DP]),
     "rbn_sample": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _DP, _DP, _DP]),
     "rbn_init_particles": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), _DP, _DP, _DP, _DP, _DP, _DP]),
     "rbn_get_particles": (C.c_int, [C.c_void_p, C.POINTER(Particles)]),
@@ -286,6 +287,12 @@ class Solver:
         e, t = C.c_double(), C.c_double()
         self._c(self.lib.rbn_energy(self.ctx, C.byref(e), C.byref(t)))
         return e.value, t.value
+
+    def profiles(self) -> np.ndarray:
+        """[ny][9]: x averages of rho, u, v, u^2, v^2, T, vT, dzT, vorticity^2 per row."""
+        out = np.empty((self.ny, 9))
+        self._c(self.lib.rbn_profiles(self.ctx, _dp(out)))
+        return out
 
     def sample(self, field: str, x, y) -> np.ndarray:
         """Bilinear samples of rho/T/u/v of the latest step at positions (x, y)."""

@@ -45,6 +45,7 @@ _SIGS = {
     "or_deposit_only": (C.c_int, [C.c_void_p, C.c_int, _IP, _IP, _DP]),
     "or_deposit_bilinear_only": (C.c_int, [C.c_void_p, C.c_int, _DP, _DP, _DP]),
     "or_get_nudging_weights": (C.c_int, [C.c_void_p, _DP]),
+    "or_profiles": (C.c_int, [C.c_void_p, _DP]),
     "or_step": (C.c_int, [C.c_void_p, C.c_int]),
     "or_get_step": (C.c_int, [C.c_void_p]),
     "or_num_diag": (C.c_int, [C.c_void_p]),
@@ -227,6 +228,12 @@ class Oracle:
         yidx = np.ascontiguousarray(yidx, dtype=np.int32)
         self._c(self.lib.or_deposit_only(self.sim, len(xidx), xidx.ctypes.data_as(_IP),
                                          yidx.ctypes.data_as(_IP), _dp(_f64(T))))
+
+    def profiles(self) -> np.ndarray:
+        """[ny][9]: x averages of rho, u, v, u^2, v^2, T, vT, dzT, vorticity^2 per row."""
+        out = np.empty((self.ny, 9))
+        self._c(self.lib.or_profiles(self.sim, _dp(out)))
+        return out
 
     def deposit_bilinear_only(self, x, y, T):
         x, y, T = (_f64(a) for a in (x, y, T))

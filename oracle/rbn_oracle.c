@@ -862,6 +862,44 @@ int or_energy(const or_sim *s, double *mean_energy, double *mean_T) {
   return 0;
 }
 
+/* profile (N:1104-1250; the same routine, unused, at T:990-1080): per row j = 1..ny the x
+ * averages (sums times 1/nx) of rho, u, v, u^2, v^2, T, vT, dzT = T(j+1) - T(j) (the top row
+ * reads the ghost row, filled by pbc + hydrovar), and the vorticity squared, which the
+ * reference computes but does not print (its format has one more %e than arguments); here
+ * with the periodic interior neighbours in x and one-sided y differences at the walls.
+ * out[(j-1)*9 + q]. */
+int or_profiles(const or_sim *s, double *out) {
+  const int NX = s->NX, NY = s->NY;
+  const double norma = 1 / ((double)(s->P.nx));
+  for (int j = 1; j <= NY; j++) {
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int i = 1; i < NX + 1; i++) {
+      const size_t id = IX(s, i, j), up = IX(s, i, j + 1);
+      acc[7] += (s->T[up] - s->T[id]);
+    }
+    for (int i = 1; i < NX + 1; i++) {
+      const size_t id = IX(s, i, j);
+      const int il = i == 1 ? NX : i - 1, ir = i == NX ? 1 : i + 1;
+      const double dvbydx = 0.5 * (s->v[IX(s, ir, j)] - s->v[IX(s, il, j)]);
+      double dubydy;
+      if (j == 1) dubydy = (s->u[IX(s, i, 2)] - s->u[id]);
+      else if (j == NY) dubydy = (s->u[id] - s->u[IX(s, i, NY - 1)]);
+      else dubydy = 0.5 * (s->u[IX(s, i, j + 1)] - s->u[IX(s, i, j - 1)]);
+      const double vor = dvbydx - dubydy;
+      acc[1] += (s->u[id]);
+      acc[2] += (s->v[id]);
+      acc[3] += (s->u[id] * s->u[id]);
+      acc[4] += (s->v[id] * s->v[id]);
+      acc[5] += s->T[id];
+      acc[0] += s->rho[id];
+      acc[6] += (s->v[id] * s->T[id]);
+      acc[8] += (vor * vor);
+    }
+    for (int q = 0; q < 9; q++) out[(size_t)(j - 1) * 9 + q] = acc[q] * norma;
+  }
+  return 0;
+}
+
 /* ------------------------------------------------------------------ particle set-up */
 
 static const int *g_sort_ids;

@@ -48,6 +48,9 @@ int or_set_populations(or_sim *s, const double *f, const double *g, const double
 int or_get_populations(const or_sim *s, double *f, double *g);
 int or_get_fields(const or_sim *s, double *rho, double *T, double *u, double *v);
 int or_energy(const or_sim *s, double *mean_energy, double *mean_T);
+/* Row profiles of the latest step: out[(j-1)*9 + q], q = rho, u, v, u^2, v^2, T, vT, dzT,
+ * vorticity^2 (x averages). */
+int or_profiles(const or_sim *s, double *out);
 
 int or_init_particles(or_sim *s, int n, const int *id, const double *x, const double *y,
                       const double *tau_p, const double *beta_p, const double *up,

@@ -99,6 +99,10 @@ def test_driver_truth_and_nudged_runs(tmp_path, raw):
     assert [s.name for s in snaps] == sorted(f"snaptemp{k}.{ext}" for k in (10, 20, 30))
     lines = (n / "evolution.dat").read_text().splitlines()
     assert [int(l.split()[0]) for l in lines] == [10, 20, 30]
+    for k in (10, 20, 30):  # profile() every nout steps (N:2382)
+        rows = (n / f"profiles.{k}.dat").read_text().splitlines()
+        assert len(rows) == 32 and all(len(r.split()) == 10 for r in rows)
+        assert [int(r.split()[0]) for r in rows] == list(range(1, 33))
     if raw:
         T30 = np.fromfile(n / "Tempfolder" / "snaptemp30.bin")[: 64 * 32]
         assert np.isfinite(T30).all()

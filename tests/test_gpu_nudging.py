@@ -75,6 +75,14 @@ def test_nudged_deposit_edges_fp64(fix_q3):
     T0n = conduction_T(pn, 1e-2, seed=4)
     g, o = _nudged_pair(pn, T0n, ids, x0, y0, T, recs, 101)
     _compare(g, o)
+    # row profiles: the top row's dzT reads the ghost row, which in the reference holds row 1's
+    # population sum without the nudging term; the library uses row 1's temperature
+    a, b = g.profiles(), o.profiles()
+    err = np.abs(a - b)
+    err[-1, 7] = 0.0
+    assert np.max(err) <= 1e-13 * max(1.0, np.max(np.abs(b)))
+    Tg = g.fields()["T"]
+    assert abs(a[-1, 7] - np.mean(Tg[:, 0] - Tg[:, -1])) <= 1e-15
 
 
 @pytest.mark.parametrize("alpha", [0.1, 1.0])

@@ -167,6 +167,31 @@ def test_launch_tuning_does_not_change_results(prec):
         s.close()
 
 
+@pytest.mark.parametrize("prec", [rbn.RBN_FP64, rbn.RBN_FP32])
+def test_row_profiles_match_the_oracle(prec):
+    p = rb_params(96, 48, 1e6)
+    T0 = conduction_T(p, 5e-2, seed=17)
+    q = p.copy()
+    q.precision = prec
+    g = rbn.Solver(q)
+    g.init_fields(T=T0)
+    o = Oracle(p)
+    o.init_fields(T=T0)
+    g.step(120)
+    o.step(120)
+    a, b = g.profiles(), o.profiles()
+    assert a.shape == (48, 9)
+    err = np.max(np.abs(a - b), axis=0)
+    scale = np.max(np.abs(b), axis=0)
+    if prec == rbn.RBN_FP64:
+        # the same values summed in another order
+        assert np.all(err <= 1e-13 * scale + 1e-300), err
+    else:
+        # FP32 fields against FP64: the stated field error (1e-4 on T, 1e-2 relative on u, v)
+        assert np.all(err <= 2e-2 * scale + 2e-4), err
+    assert np.max(np.abs(b[:, 8])) > 0  # vorticity developed
+
+
 def test_two_contexts_are_independent():
     p = rb_params(32, 16, 1e5)
     a, b = rbn.Solver(p), rbn.Solver(p)

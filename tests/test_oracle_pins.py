@@ -277,6 +277,55 @@ def test_record_sampling_and_energy_in_uniform_flow():
     assert 0.5 * U0 ** 2 * 0.8 < e < 0.5 * U0 ** 2 * 1.0001
 
 
+# ------------------------------------------------------------------ row profiles
+
+
+def _numpy_profiles(fl, Tghost_top):
+    """Independent restatement of the row profiles from the oracle's [nx][ny] fields."""
+    rho, T, u, v = fl["rho"], fl["T"], fl["u"], fl["v"]
+    nx, ny = T.shape
+    Tup = np.concatenate([T[:, 1:], Tghost_top[:, None]], axis=1)
+    dvdx = 0.5 * (np.roll(v, -1, axis=0) - np.roll(v, 1, axis=0))
+    dudy = np.empty_like(u)
+    dudy[:, 1:-1] = 0.5 * (u[:, 2:] - u[:, :-2])
+    dudy[:, 0] = u[:, 1] - u[:, 0]
+    dudy[:, -1] = u[:, -1] - u[:, -2]
+    vor = dvdx - dudy
+    cols = [rho, u, v, u * u, v * v, T, v * T, Tup - T, vor * vor]
+    return np.stack([c.mean(axis=0) for c in cols], axis=1)
+
+
+def test_profiles_of_a_flow_match_an_independent_restatement():
+    p = params(40, 20, gravity=-1e-4, Tup=0.0, Tdown=1.0)
+    o = Oracle(p)
+    rng = np.random.default_rng(4)
+    T0 = np.tile(1.0 - np.arange(p.ny) / (p.ny - 1), (p.nx, 1)) + rng.uniform(-0.05, 0.05, (p.nx, p.ny))
+    o.init_fields(T=T0)
+    o.step(30)
+    pr = o.profiles()
+    fl = o.fields()
+    ref = _numpy_profiles(fl, fl["T"][:, 0])  # the top ghost row is the periodic copy of row 1
+    assert pr.shape == (p.ny, 9)
+    assert np.max(np.abs(pr - ref)) < 1e-15
+    assert np.max(np.abs(pr[:, 8])) > 0 and np.max(np.abs(pr[:, 1])) > 0
+
+
+def test_profiles_of_a_linear_rest_state():
+    # conduction between the plates: T linear in j, no flow; dzT is the slope except on the
+    # top row, which reads the ghost copy of row 1
+    p = params(24, 12, gravity=0.0, Tup=0.0, Tdown=1.0)
+    o = Oracle(p)
+    T0 = np.tile(1.0 - np.arange(p.ny) / (p.ny - 1), (p.nx, 1))
+    o.init_fields(T=T0)
+    o.step(1)
+    pr = o.profiles()
+    slope = -1.0 / (p.ny - 1)
+    assert np.allclose(pr[:, 0], 1.0, atol=1e-15) and np.allclose(pr[:, 1:5], 0.0, atol=1e-15)
+    assert np.allclose(pr[:, 5], T0[0], atol=1e-14)
+    assert np.allclose(pr[:-1, 7], slope, atol=1e-14) and np.isclose(pr[-1, 7], 1.0, atol=1e-14)
+    assert np.allclose(pr[:, 6], 0.0, atol=1e-15) and np.allclose(pr[:, 8], 0.0, atol=1e-20)
+
+
 # ------------------------------------------------------------------ nudging deposit
 
 
