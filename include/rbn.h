@@ -290,7 +290,11 @@ int rbn_adj_backward(rbn_ctx *ctx, double *dJ_dT0, double *dJ_dalpha);
  *   "lbm_reverse" 1: sweep every odd step last row first, so that it starts on the rows the
  *                previous step wrote last (default 1)
  *   "particle_overlap"  1: run the particle update of step s on its own stream concurrently
- *                with the lattice update of step s+1 (default 0) */
+ *                with the lattice update of step s+1 (default 0)
+ *   "adj_store_deposit" 1: rbn_adj_forward keeps every step's deposit (counts, means, weights:
+ *                12-20 B per node and step) when that fits in a quarter of the free device
+ *                memory, so that rbn_adj_backward does not replay the observers (default 1);
+ *                0: always replay */
 int rbn_set_tuning(rbn_ctx *ctx, const char *key, int value);
 int rbn_get_tuning(const rbn_ctx *ctx, const char *key, int *value);
 

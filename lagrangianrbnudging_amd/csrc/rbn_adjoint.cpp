@@ -13,11 +13,15 @@
 // misfit after the observation steps; every K steps the state entering the step (f, g,
 // lagged force, nud_term and the observer state) is copied into a checkpoint, and every
 // record fetched from the observation provider is kept on the device.
+// The deposit (counts, means, bilinear weights) depends only on the records.  When all W
+// steps of it fit in a quarter of the free device memory (12-20 B per node and step) the
+// forward writes each step's deposit into its own slot and keeps it; otherwise the backward
+// replays the observer state machine and the deposit from the checkpointed observer state.
 // Backward: segments in reverse order.  A segment is recomputed from its checkpoint into a
-// K-slot buffer (populations, force and nud_term entering each step, deposit counts and
-// means of each step; the lattice step writes its output straight into the next slot),
-// then swept backwards with one adjoint launch per step (rbn_adjoint.hip).  Finally the
-// pulled adjoints of the first step are projected on the initial temperature.
+// K-slot buffer (populations, force and nud_term entering each step; the lattice step writes
+// its output straight into the next slot), then swept backwards with one adjoint launch per
+// step (rbn_adjoint.hip).  Finally the pulled adjoints of the first step are projected on
+// the initial temperature.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -57,6 +61,12 @@ struct AdjState {
   int *seg_nvol = nullptr;            // K x plane ints
   double *seg_tbar = nullptr;         // K x plane doubles
   double *seg_w = nullptr;            // K x plane doubles (bilinear deposit weights)
+  // every step's deposit, kept by the forward when it fits (the backward then does not
+  // replay the observers)
+  bool store_dep = false;
+  int *all_nvol = nullptr;            // W x plane
+  double *all_tbar = nullptr;         // W x plane
+  double *all_w = nullptr;            // W x plane (bilinear deposit)
   unsigned char *work = nullptr;      // F, nud planes, then a velocity slot (2 planes)
   int *rep_oi = nullptr;              // replayed observer state
   double *rep_od = nullptr;
@@ -71,7 +81,7 @@ void adj_free(rbn_ctx *c) {
   if (!a) return;
   void *ptrs[] = {a->ckpt, a->ckpt_oi, a->ckpt_od, a->rec, a->rec_chk, a->obs, a->maxbits, a->part,
                   a->acc, a->init, a->seg, a->segFN, a->seg_nvol, a->seg_tbar, a->seg_w, a->work, a->rep_oi,
-                  a->rep_od, a->lam, a->lamFN, a->lamT, a->dT0};
+                  a->rep_od, a->lam, a->lamFN, a->lamT, a->dT0, a->all_nvol, a->all_tbar, a->all_w};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   delete a;
@@ -130,8 +140,16 @@ int ensure_forward_storage(rbn_ctx *c, int W, int K) {
   const int nseg = (W + K - 1) / K;
   const int ntr = c->P.ntempread;
   const int nrec = (W + ntr - 1) / ntr, nos = W / ntr;
+  const bool bil = c->dep.mode == RBN_DEPOSIT_BILINEAR;
+  const size_t dep_bytes = (size_t)W * g.plane * (sizeof(int) + sizeof(double) + (bil ? sizeof(double) : 0));
+  bool want_store = false;
+  if (c->adj_store_deposit) {
+    size_t free_b = 0, total_b = 0;
+    want_store = hipMemGetInfo(&free_b, &total_b) == hipSuccess && dep_bytes <= free_b / 4;
+  }
   AdjState *a = c->adj;
-  if (a && a->W == W && a->K == K && a->nobs == c->nobs && a->prec == c->prec && a->plane_bytes == pb) {
+  if (a && a->W == W && a->K == K && a->nobs == c->nobs && a->prec == c->prec && a->plane_bytes == pb &&
+      (a->all_nvol != nullptr) == want_store && (a->all_w != nullptr) == (want_store && bil)) {
     a->ready = false;
     return RBN_OK;
   }
@@ -156,6 +174,22 @@ int ensure_forward_storage(rbn_ctx *c, int W, int K) {
     rbn::adj_free(c);
     return rc;
   }
+  if (want_store) {
+    void *p1 = nullptr, *p2 = nullptr, *p3 = nullptr;
+    const size_t np = (size_t)W * g.plane;
+    bool ok = hipMalloc(&p1, np * sizeof(int)) == hipSuccess && hipMalloc(&p2, np * sizeof(double)) == hipSuccess &&
+              (!bil || hipMalloc(&p3, np * sizeof(double)) == hipSuccess);
+    if (ok) {
+      a->all_nvol = (int *)p1;
+      a->all_tbar = (double *)p2;
+      a->all_w = (double *)p3;
+      a->store_dep = true;
+    } else {  // not enough memory after all: replay instead
+      for (void *p : {p1, p2, p3})
+        if (p) (void)hipFree(p);
+      (void)hipGetLastError();
+    }
+  }
   return RBN_OK;
 }
 
@@ -165,13 +199,15 @@ int ensure_backward_storage(rbn_ctx *c) {
   const Geom &g = c->geom;
   const size_t pb = a->plane_bytes, K = (size_t)a->K, n = (size_t)a->nobs;
   int rc;
+  const bool replay = !a->store_dep;
   if ((rc = alloc_n(c, &a->seg, K * 18 * pb)) || (rc = alloc_n(c, &a->segFN, K * 2 * pb)) ||
-      (rc = alloc_n(c, &a->seg_nvol, K * g.plane)) || (rc = alloc_n(c, &a->seg_tbar, K * g.plane)) ||
-      (rc = alloc_n(c, &a->work, 4 * pb)) || (rc = alloc_n(c, &a->rep_oi, ONI * n)) ||
-      (rc = alloc_n(c, &a->rep_od, OND * n)) || (rc = alloc_n(c, &a->lam, 2 * 18 * pb)) ||
+      (replay && (rc = alloc_n(c, &a->seg_nvol, K * g.plane))) ||
+      (replay && (rc = alloc_n(c, &a->seg_tbar, K * g.plane))) ||
+      (rc = alloc_n(c, &a->work, 4 * pb)) || (replay && (rc = alloc_n(c, &a->rep_oi, ONI * n))) ||
+      (replay && (rc = alloc_n(c, &a->rep_od, OND * n))) || (rc = alloc_n(c, &a->lam, 2 * 18 * pb)) ||
       (rc = alloc_n(c, &a->lamFN, 2 * pb)) || (rc = alloc_n(c, &a->lamT, g.plane)) ||
       (rc = alloc_n(c, &a->dT0, g.plane)) ||
-      (c->dep.mode == RBN_DEPOSIT_BILINEAR && (rc = alloc_n(c, &a->seg_w, K * g.plane)))) {
+      (replay && c->dep.mode == RBN_DEPOSIT_BILINEAR && (rc = alloc_n(c, &a->seg_w, K * g.plane)))) {
     // keep the forward products; free the partial workspace
     void **ws[] = {(void **)&a->seg, (void **)&a->segFN, (void **)&a->seg_nvol, (void **)&a->seg_tbar, (void **)&a->seg_w,
                    (void **)&a->work, (void **)&a->rep_oi, (void **)&a->rep_od, (void **)&a->lam,
@@ -229,7 +265,13 @@ int rbn_adj_forward(rbn_ctx *c, int nsteps, int ckpt_every, double *J) {
       HIPCHK(c, hipMemcpyAsync(a->rec + e * 3 * n, (const void *)o.rx, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, s));
       HIPCHK(c, hipMemcpyAsync(a->rec_chk + e * n, o.chk, n * sizeof(int), hipMemcpyDeviceToDevice, s));
     }
-    if ((rc = launch_step_kernels(c, istep, o, c->dep))) return rc;
+    rbn::DepPtrs d = c->dep;
+    if (a->store_dep) {
+      d.cnt = a->all_nvol + (size_t)t * g.plane;
+      d.Tbar = a->all_tbar + (size_t)t * g.plane;
+      d.wq = a->all_w ? a->all_w + (size_t)t * g.plane : nullptr;
+    }
+    if ((rc = launch_step_kernels(c, istep, o, d))) return rc;
     const bool obs_step = r == 0;
     const bool moments = obs_step || t == nsteps - 1;
     const int nxt = (c->cur + 1) % rbn_ctx::kVel;
@@ -242,10 +284,10 @@ int rbn_adj_forward(rbn_ctx *c, int nsteps, int ckpt_every, double *J) {
     lp.uv = c->uv[nxt];
     lp.rho = moments ? c->rho : nullptr;
     lp.T = moments ? c->T : nullptr;
-    lp.nvol = c->dep.cnt;
-    lp.Tbar = c->dep.Tbar;
+    lp.nvol = d.cnt;
+    lp.Tbar = d.Tbar;
     lp.nud = c->nud;
-    lp.wq = c->dep.mode == RBN_DEPOSIT_BILINEAR ? c->dep.wq : nullptr;
+    lp.wq = c->dep.mode == RBN_DEPOSIT_BILINEAR ? d.wq : nullptr;
     if ((rc = timed(c, KT_LBM, s, [&] { return rbn::launch_lbm(s, c->prec, P.variant, g, lp, c->consts.data(), c->tune); })))
       return rc;
     std::swap(c->fA, c->fB);
@@ -260,6 +302,13 @@ int rbn_adj_forward(rbn_ctx *c, int nsteps, int ckpt_every, double *J) {
                                    a->maxbits + m, c->d_err));
       HIPCHK(c, rbn::launch_sum_partials(s, a->part, rbn::misfit_blocks(c->nobs), a->acc));
     }
+  }
+  if (a->store_dep) {  // the context's nudging state is the last step's
+    const size_t last = (size_t)(nsteps - 1) * g.plane;
+    HIPCHK(c, hipMemcpyAsync(c->dep.cnt, a->all_nvol + last, g.plane * sizeof(int), hipMemcpyDeviceToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(c->dep.Tbar, a->all_tbar + last, g.plane * sizeof(double), hipMemcpyDeviceToDevice, s));
+    if (a->all_w)
+      HIPCHK(c, hipMemcpyAsync(c->dep.wq, a->all_w + last, g.plane * sizeof(double), hipMemcpyDeviceToDevice, s));
   }
   if (c->timing && (rc = drain_timing(c))) return rc;
   if ((rc = check_err_flag(c))) return rc;
@@ -287,7 +336,7 @@ int rbn_adj_backward(rbn_ctx *c, double *dJ_dT0, double *dJ_dalpha) {
   HIPCHK(c, hipMemsetAsync(a->lamFN, 0, 2 * pb, s));
   HIPCHK(c, hipMemsetAsync(a->lamT, 0, g.plane * sizeof(unsigned long long), s));
   HIPCHK(c, hipMemsetAsync(a->acc + 1, 0, sizeof(double), s));
-  const rbn::ObsPtrs ro = obs_view(a->nobs, a->rep_oi, a->rep_od);
+  const rbn::ObsPtrs ro = a->store_dep ? rbn::ObsPtrs{} : obs_view(a->nobs, a->rep_oi, a->rep_od);
   rbn::AdjStepArgs args{};
   args.g = g;
   args.nobs = a->nobs;
@@ -302,21 +351,36 @@ int rbn_adj_backward(rbn_ctx *c, double *dJ_dT0, double *dJ_dalpha) {
     HIPCHK(c, hipMemcpyAsync(a->seg, ck, 18 * pb, hipMemcpyDeviceToDevice, s));
     HIPCHK(c, hipMemcpyAsync(wF, ck + 18 * pb, pb, hipMemcpyDeviceToDevice, s));
     HIPCHK(c, hipMemcpyAsync(wN, ck + 19 * pb, pb, hipMemcpyDeviceToDevice, s));
-    HIPCHK(c, hipMemcpyAsync(a->rep_oi, a->ckpt_oi + (size_t)sg * ONI * n, ONI * n * sizeof(int), hipMemcpyDeviceToDevice, s));
-    HIPCHK(c, hipMemcpyAsync(a->rep_od, a->ckpt_od + (size_t)sg * OND * n, OND * n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    if (!a->store_dep) {
+      HIPCHK(c, hipMemcpyAsync(a->rep_oi, a->ckpt_oi + (size_t)sg * ONI * n, ONI * n * sizeof(int), hipMemcpyDeviceToDevice, s));
+      HIPCHK(c, hipMemcpyAsync(a->rep_od, a->ckpt_od + (size_t)sg * OND * n, OND * n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    }
+    // the deposit of window step w: kept by the forward, or replayed into segment slot t
+    auto dep_of = [&](int w, int t) {
+      rbn::DepPtrs d = c->dep;
+      if (a->store_dep) {
+        d.cnt = a->all_nvol + (size_t)w * g.plane;
+        d.Tbar = a->all_tbar + (size_t)w * g.plane;
+        d.wq = a->all_w ? a->all_w + (size_t)w * g.plane : nullptr;
+      } else {
+        d.cnt = a->seg_nvol + (size_t)t * g.plane;
+        d.Tbar = a->seg_tbar + (size_t)t * g.plane;
+        d.wq = a->seg_w ? a->seg_w + (size_t)t * g.plane : nullptr;
+      }
+      return d;
+    };
     // recompute the segment
     for (int t = 0; t < L; t++) {
       const int w = t0 + t, istep = a->start + w + 1;
-      if (istep % ntr == 1) {
-        const size_t e = (size_t)(w / ntr);
-        HIPCHK(c, hipMemcpyAsync((void *)ro.rx, a->rec + e * 3 * n, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, s));
-        HIPCHK(c, hipMemcpyAsync((void *)ro.chk, a->rec_chk + e * n, n * sizeof(int), hipMemcpyDeviceToDevice, s));
+      const rbn::DepPtrs d = dep_of(w, t);
+      if (!a->store_dep) {
+        if (istep % ntr == 1) {
+          const size_t e = (size_t)(w / ntr);
+          HIPCHK(c, hipMemcpyAsync((void *)ro.rx, a->rec + e * 3 * n, 3 * n * sizeof(double), hipMemcpyDeviceToDevice, s));
+          HIPCHK(c, hipMemcpyAsync((void *)ro.chk, a->rec_chk + e * n, n * sizeof(int), hipMemcpyDeviceToDevice, s));
+        }
+        if ((rc = launch_step_kernels(c, istep, ro, d))) return rc;
       }
-      rbn::DepPtrs d = c->dep;
-      d.cnt = a->seg_nvol + (size_t)t * g.plane;
-      d.Tbar = a->seg_tbar + (size_t)t * g.plane;
-      d.wq = a->seg_w ? a->seg_w + (size_t)t * g.plane : nullptr;
-      if ((rc = launch_step_kernels(c, istep, ro, d))) return rc;
       unsigned char *fn = plane_at(a->segFN, pb, 2 * (size_t)t);
       HIPCHK(c, hipMemcpyAsync(fn, wF, pb, hipMemcpyDeviceToDevice, s));
       HIPCHK(c, hipMemcpyAsync(fn + pb, wN, pb, hipMemcpyDeviceToDevice, s));
@@ -351,9 +415,10 @@ int rbn_adj_backward(rbn_ctx *c, double *dJ_dT0, double *dJ_dalpha) {
       args.gA = in + 9 * pb;
       args.Fin = fn;
       args.nudin = fn + pb;
-      args.nvol = a->seg_nvol + (size_t)t * g.plane;
-      args.Tbar = a->seg_tbar + (size_t)t * g.plane;
-      args.wq = a->seg_w ? a->seg_w + (size_t)t * g.plane : nullptr;
+      const rbn::DepPtrs d = dep_of(w, t);
+      args.nvol = d.cnt;
+      args.Tbar = d.Tbar;
+      args.wq = d.wq;
       args.lam_next = have_next ? plane_at(a->lam, pb, 18 * (size_t)cur) : nullptr;
       args.lam = plane_at(a->lam, pb, 18 * (size_t)(1 - cur));
       args.lamF = lamF;

@@ -1079,6 +1079,8 @@ int rbn_set_tuning(rbn_ctx *c, const char *key, int value) {
     c->tune.reverse = value != 0;
   } else if (!strcmp(key, "particle_overlap")) {
     c->tune.overlap = value != 0;
+  } else if (!strcmp(key, "adj_store_deposit")) {
+    c->adj_store_deposit = value != 0;
   } else {
     return fail(c, RBN_EINVAL, "unknown tuning key %s", key);
   }
@@ -1094,6 +1096,7 @@ int rbn_get_tuning(const rbn_ctx *c, const char *key, int *value) {
   else if (!strcmp(key, "lbm_xcd")) *value = c->tune.xcd;
   else if (!strcmp(key, "lbm_reverse")) *value = c->tune.reverse;
   else if (!strcmp(key, "particle_overlap")) *value = c->tune.overlap;
+  else if (!strcmp(key, "adj_store_deposit")) *value = c->adj_store_deposit;
   else return rbn::thread_fail(RBN_EINVAL, "unknown tuning key");
   return RBN_OK;
 }

@@ -225,6 +225,7 @@ struct rbn_ctx {
 
   // adjoint window (rbn_adj_forward / rbn_adj_backward)
   bool at_init = false;             // state is the one rbn_init_fields set (no step since)
+  int adj_store_deposit = 1;        // keep every step's deposit when it fits (else replay it)
   rbn::AdjState *adj = nullptr;
 
   std::string err;

@@ -140,6 +140,23 @@ def test_gradient_does_not_depend_on_checkpoint_interval(case, ckpt):
     assert a1 == a0
 
 
+@pytest.mark.parametrize("deposit", [rbn.RBN_DEPOSIT_BOX7, rbn.RBN_DEPOSIT_BILINEAR])
+def test_kept_and_replayed_deposits_give_the_same_gradient(deposit):
+    c = _case(deposit)
+    out = []
+    for store in (1, 0):
+        s = _solver(c)
+        s.set_tuning(adj_store_deposit=store)
+        J = _J(s, c, c["T0"], ckpt=7)
+        g, ga = s.adj_backward()
+        out.append((J, g, ga, s.nudging()))
+    (J1, g1, a1, n1), (J0, g0, a0, n0) = out
+    assert J1 == J0 and a1 == a0
+    assert_equal_arrays(g1, g0, "dJ/dT0")
+    for k in ("nudge_vol", "tempreadint", "nud_term", "weight"):
+        assert_equal_arrays(n1[k], n0[k], k)
+
+
 def test_backward_repeats_and_leaves_the_state(case):
     s = _solver(case)
     _J(s, case, case["T0"])
