@@ -277,6 +277,50 @@ def test_record_sampling_and_energy_in_uniform_flow():
     assert 0.5 * U0 ** 2 * 0.8 < e < 0.5 * U0 ** 2 * 1.0001
 
 
+def test_writefull_fields_match_an_independent_restatement():
+    # Writefull (Trajectories/code/mympi2.h:440-502): the node fields dtu = u(s) - u(s-1), centred x differences
+    # (periodic) and centred y differences (one-sided on the wall rows), interpolated at the
+    # particle's position before the update, together with u and v there.
+    from oracle.tlm_nudged import bilinear
+
+    p = params(24, 16, noutconfig=5, prtcl_writefull=1, gravity=-2e-4, Tup=0.0, Tdown=1.0,
+               relax=0.6, relaxg=0.6)
+    o = Oracle(p)
+    rng = np.random.default_rng(8)
+    T0 = np.tile(1.0 - np.arange(p.ny) / (p.ny - 1), (p.nx, 1)) + rng.uniform(-0.05, 0.05, (p.nx, p.ny))
+    o.init_fields(T=T0)
+    n = 12
+    x = np.concatenate([[0.3, 23.7, 5.5], rng.uniform(0.5, 23.5, n - 3)])
+    y = np.concatenate([[1.2, 15.6, 8.0], rng.uniform(1.0, 15.9, n - 3)])
+    o.init_particles(np.arange(n, dtype=np.int32), x, y, tau_p=np.zeros(n))
+    o.step(4)
+    prev = o.fields()
+    o.step(1)
+    cur = o.fields()
+    r = o.records[-1]
+    assert r["step"] == 5 and r["dtu"] is not None
+
+    def ddx(a):
+        return 0.5 * (np.roll(a, -1, axis=0) - np.roll(a, 1, axis=0))
+
+    def ddy(a):
+        d = np.empty_like(a)
+        d[:, 1:-1] = 0.5 * (a[:, 2:] - a[:, :-2])
+        d[:, 0] = a[:, 1] - a[:, 0]
+        d[:, -1] = a[:, -1] - a[:, -2]
+        return d
+
+    u, v = cur["u"], cur["v"]
+    expect = {"dtu": u - prev["u"], "dtv": v - prev["v"], "dxu": ddx(u), "dxv": ddx(v),
+              "dyu": ddy(u), "dyv": ddy(v), "fu": u, "fv": v}
+    for k, field in expect.items():
+        got = r[k]
+        ref = bilinear(field, r["xo"], r["yo"])
+        scale = max(np.max(np.abs(ref)), 1e-12)
+        assert np.max(np.abs(got - ref)) <= 1e-13 * scale, k
+    assert np.max(np.abs(expect["dyu"])) > 0 and np.max(np.abs(expect["dtv"])) > 0
+
+
 # ------------------------------------------------------------------ row profiles
 
 
