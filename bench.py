@@ -5,12 +5,16 @@ Workload (the benchmark-scale plain configuration): 4096 x 2048 lattice, Ra = 1e
 linear conduction profile + seeded uniform noise (1e-3) on T, u = 0; 1e5 tracers + 1e5
 inertial particles (Stokes number log-uniform in [0.01, 1], tau_p = St * t_ff with
 t_ff = (ny-1)/u_ff, zero initial slip), uniform random positions; trajectory records every
-20 steps, delivered to the host.  One "step" = the fused f/g LBM kernel + the particle
-kernel (+ the record on record steps).  Inputs are resident on the GPU before timing starts.
+20 steps, delivered to the host.  One "step" = the fused f/g LBM update + the particle
+update (+ the record on record steps); by default the particle update of a step runs as
+extra workgroups of the next step's lattice launch.  Inputs are resident on the GPU before
+timing starts.
 
 Prints one JSON line: MLUPS of the FP32 (shifted-population) path, the HBM roofline of the
-dominant kernel (fused LBM step; HIP events around every launch in the timed region; the
-HBM traffic per launch comes from the committed rocprofv3 counter summary under profiles/),
+dominant kernel (the lattice launch carrying the particle update; HIP events around every
+launch in the timed region; the HBM traffic per launch comes from the committed rocprofv3
+counter summary under profiles/), the same step with the particle update as a launch of its
+own (the lattice kernel's roofline alone),
 the FP64 path's MLUPS on the same workload, the batched ensemble (64 uneven members) against
 the same members stepped one by one, the adjoint window's cost against the plain nudged
 forward (512 x 256, 2000 steps, checkpoint every 100), and the CPU FP64 restatement of the
@@ -69,10 +73,24 @@ def lbm_bytes_per_node(precision):
     return 40 * (8 if precision == 0 else 4)
 
 
-def run_gpu(rbn, precision, steps, warmup, timing=True):
+def particle_bytes_per_step(precision, n_tracer=N_TRACER, n_inertial=N_INERTIAL):
+    """Algorithmic bytes of one particle update, record writes excluded.  Every particle reads
+    x, y, u, v, 1/tau_p (FP64) and writes x, y, u, v, xo, yo, uo, vo; a tracer reads (u, v) at
+    the 4 corners of its cell, an inertial particle also beta, (u, v) at 12 nodes (the two
+    corner rows on four columns, the rows below and above on the two corner columns) and the
+    previous step's (u, v) at the 4 corners."""
+    r = 8 if precision == 0 else 4
+    state = 5 * 8 + 8 * 8
+    tracer = state + 4 * 2 * r
+    inertial = state + 8 + 16 * 2 * r
+    return n_tracer * tracer + n_inertial * inertial
+
+
+def run_gpu(rbn, precision, steps, warmup, timing=True, **tune):
     p = make_params(rbn, precision)
     T0, (ids, x, y, tau, zeros) = make_inputs(p)
     s = rbn.Solver(p, 0)
+    s.set_tuning(**tune)
     s.init_fields(T=T0)
     s.init_particles(ids, x, y, tau_p=tau, up=zeros, vp=zeros)
     s.discard_records()
@@ -91,8 +109,10 @@ def run_gpu(rbn, precision, steps, warmup, timing=True):
         n_lbm, ms_lbm = s.kernel_stats("lbm")
         n_p, ms_p = s.kernel_stats("particles")
         s.kernel_timing(False)
-        out.update(lbm_launches=n_lbm, lbm_avg_ms=ms_lbm / max(n_lbm, 1),
-                   particles_launches=n_p, particles_avg_ms=ms_p / max(n_p, 1))
+        out.update(lbm_launches=n_lbm, lbm_ms=ms_lbm, lbm_avg_ms=ms_lbm / max(n_lbm, 1),
+                   particles_launches=n_p, particles_avg_ms=ms_p / max(n_p, 1),
+                   # particle updates carried by lattice launches (the rest ran on their own)
+                   carried=steps - n_p)
     fl = s.fields()
     out["finite"] = bool(np.isfinite(fl["T"]).all() and np.isfinite(fl["u"]).all())
     s.close()
@@ -338,12 +358,20 @@ def run_adjoint(rbn, window, ckpt):
             "finite": bool(np.isfinite(g).all() and np.isfinite(ga))}
 
 
-TRAFFIC_PROFILE = ROOT / "profiles" / "k_lbm32_4096x2048_pmc.txt"
+def lattice_launch_bytes(r, precision):
+    """Algorithmic bytes of the lattice launches of a timed run and the rate they reach: every
+    launch moves the lattice bytes, and those that carry the previous step's particle update
+    also that update's bytes."""
+    total = r["lbm_launches"] * NX * NY * lbm_bytes_per_node(precision) + r["carried"] * particle_bytes_per_step(precision)
+    return total, total / (r["lbm_ms"] * 1e-3) / 1e9
+
+
+TRAFFIC_PROFILE = ROOT / "profiles" / "k_lbm_fused32_4096x2048_pmc.txt"
 
 
 def measured_traffic():
-    """HBM bytes per K1 launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
-    (counters scaled by a copy kernel of known size), or None."""
+    """HBM bytes per lattice launch carrying a particle update, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (counters scaled by a copy kernel of known size), or None."""
     try:
         for line in TRAFFIC_PROFILE.read_text().splitlines():
             if line.startswith("traffic_bytes_per_launch:"):
@@ -405,6 +433,8 @@ def main() -> int:
     print(f"bench: fp32 {NX}x{NY} steps={args.steps} warmup={args.warmup}", file=sys.stderr, flush=True)
     r32 = run_gpu(rbn, rbn.RBN_FP32, args.steps, args.warmup)
     print(f"bench: fp32 done {r32}", file=sys.stderr, flush=True)
+    sep = run_gpu(rbn, rbn.RBN_FP32, max(args.steps // 2, 1), max(args.warmup // 2, 1), particle_fuse=0)
+    print(f"bench: fp32 separate particle launches done {sep}", file=sys.stderr, flush=True)
     r64 = None
     if not args.no_fp64:
         r64 = run_gpu(rbn, rbn.RBN_FP64, max(args.steps // 2, 1), max(args.warmup // 2, 1))
@@ -426,8 +456,10 @@ def main() -> int:
         cpu = run_cpu_baseline(rbn, args.cpu_steps)
         print(f"bench: cpu done {cpu}", file=sys.stderr, flush=True)
 
-    bytes_launch = NX * NY * lbm_bytes_per_node(1)
-    achieved = bytes_launch / (r32["lbm_avg_ms"] * 1e-3) / 1e9
+    lat_bytes = NX * NY * lbm_bytes_per_node(1)
+    total_bytes, achieved = lattice_launch_bytes(r32, 1)
+    bytes_launch = total_bytes / max(r32["lbm_launches"], 1)
+    sep_achieved = lat_bytes / (sep["lbm_avg_ms"] * 1e-3) / 1e9
     line = {
         "metric": METRIC,
         "value": round(r32["mlups"], 1),
@@ -454,17 +486,27 @@ def main() -> int:
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 4),
             "traffic": measured_traffic(),
-            "kernel": "k_lbm32 (fused f/g collide+stream)",
-            "bytes_per_launch": bytes_launch,
+            "kernel": "k_lbm_fused32 (fused f/g collide+stream + the previous step's particle update)",
+            "bytes_per_launch": round(bytes_launch),
             "avg_launch_ms": round(r32["lbm_avg_ms"], 5),
+            "launches": r32["lbm_launches"],
+            "carrying_particles": r32["carried"],
         },
         "cpu_baseline": cpu and {k: (round(v, 3) if isinstance(v, float) else v) for k, v in cpu.items()},
-        "particles_avg_ms": round(r32["particles_avg_ms"], 5),
+        "separate_particle_launches": {
+            "mlups": round(sep["mlups"], 1),
+            "ms_per_step": round(sep["ms_per_step"], 5),
+            "lattice_kernel": "k_lbm32",
+            "lattice_avg_ms": round(sep["lbm_avg_ms"], 5),
+            "lattice_bytes_per_launch": lat_bytes,
+            "lattice_roofline_frac": round(sep_achieved / HBM_PEAK_GBPS, 4),
+            "particles_avg_ms": round(sep["particles_avg_ms"], 5),
+        },
         "fp64": r64 and {
             "mlups": round(r64["mlups"], 1),
             "ms_per_step": round(r64["ms_per_step"], 5),
             "lbm_avg_ms": round(r64["lbm_avg_ms"], 5),
-            "roofline_frac": round(NX * NY * lbm_bytes_per_node(0) / (r64["lbm_avg_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
+            "roofline_frac": round(lattice_launch_bytes(r64, 0)[1] / HBM_PEAK_GBPS, 4),
         },
         "ensemble": ens,
         "nudging": nud,

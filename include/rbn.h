@@ -286,11 +286,17 @@ int rbn_adj_backward(rbn_ctx *ctx, double *dJ_dT0, double *dJ_dalpha);
  *                lbm_block * lbm_npt nodes per workgroup)
  *   "lbm_nt"     1: non-temporal stores of the step's outputs (default 1)
  *   "lbm_ntload" 1: non-temporal loads of the step's inputs (default 0)
+ *   "lbm_uv_keep" 1: ordinary (cached) stores of the velocity under lbm_nt, for the particle
+ *                update that gathers from it next (default 1)
  *   "lbm_xcd"    1: give each of the 8 XCDs a contiguous band of rows (default 0)
  *   "lbm_reverse" 1: sweep every odd step last row first, so that it starts on the rows the
  *                previous step wrote last (default 1)
  *   "particle_overlap"  1: run the particle update of step s on its own stream concurrently
  *                with the lattice update of step s+1 (default 0)
+ *   "particle_fuse" 1: run the particle update of step s as extra workgroups of the lattice
+ *                launch of step s+1 (the last step of an rbn_step call, and a record step
+ *                followed by a step that rewrites the moments, launch it on its own); ignored
+ *                under particle_overlap (default 1)
  *   "adj_store_deposit" 1: rbn_adj_forward keeps every step's deposit (counts, means, weights:
  *                12-20 B per node and step) when that fits in a quarter of the free device
  *                memory, so that rbn_adj_backward does not replay the observers (default 1);

@@ -941,14 +941,47 @@ int rbn_get_nudging(rbn_ctx *c, int *nudge_vol, double *tempreadint, double *nud
   return RBN_OK;
 }
 
-int rbn_step(rbn_ctx *c, int nsteps) {
-  int rc = check_ctx(c);
-  if (rc) return rc;
-  if (!c->initialised) return fail(c, RBN_ESTATE, "rbn_step before initialisation");
-  if (nsteps < 0) return fail(c, RBN_EINVAL, "negative step count");
+namespace {
+
+rbn::PartJob pending_job(rbn_ctx *c) {
+  const rbn_ctx::PendingParticles &j = c->pend;
+  rbn::PartJob job;
+  job.uv = c->uv[j.cur];
+  job.uvp = c->uv[j.prv];
+  job.T = c->T;
+  job.pp = part_ptrs(c, j.slot ? j.slot->d : nullptr);
+  job.record = j.record;
+  job.full = j.full;
+  return job;
+}
+
+// The pending particle update has been launched (on the lattice stream): queue its record.
+int pending_launched(rbn_ctx *c) {
+  rbn_ctx::PendingParticles j = c->pend;
+  c->pend.on = false;
+  if (j.slot && c->rec_fn) return queue_record_copy(c, *j.slot, j.istep, c->stream);
+  return RBN_OK;
+}
+
+// Run the pending particle update as a launch of its own.
+int flush_particles(rbn_ctx *c) {
+  if (!c->pend.on) return RBN_OK;
+  const rbn::PartJob job = pending_job(c);
+  int rc = timed(c, 1, c->stream, [&] {
+    return rbn::launch_particles(c->stream, c->prec, c->geom, job.uv, job.uvp, job.T, job.pp,
+                                 c->any_heavy || job.full, job.record, job.full);
+  });
+  if (rc) {
+    c->pend.on = false;
+    return rc;
+  }
+  return pending_launched(c);
+}
+
+int step_loop(rbn_ctx *c, int nsteps) {
+  int rc = RBN_OK;
   const rbn_params &P = c->P;
   const bool nudged = P.variant == RBN_VARIANT_NUDGING;
-  if (nudged && c->nobs == 0 && !c->d_obs_i) return fail(c, RBN_ESTATE, "nudged run without observers");
   const bool parts = !nudged && P.particle && c->np > 0;
   const int nd = P.ndiag > 0 ? P.ndiag : (nudged ? 5000 : P.nout);
   const int right = P.fix_q3 ? P.nx - 3 : P.nx - 2;
@@ -976,7 +1009,18 @@ int rbn_step(rbn_ctx *c, int nsteps) {
     // moves the contention) K2 runs on the lattice stream itself, ordered without events (a
     // cross-stream event wait costs about 20 us per hand-over).
     const bool overlap = c->tune.overlap != 0;
+    // particle_fuse: K2 of step s rides in the grid of K1 of step s+1, unless step s is a
+    // record step (its update reads T) and step s+1 rewrites the moments.
+    const bool fuse = parts && !overlap && c->tune.fuse != 0;
     hipStream_t ks = overlap ? c->pstream : c->stream;
+    bool carry = false;
+    if (c->pend.on) {
+      if (c->pend.record && moments) {
+        if ((rc = flush_particles(c))) return rc;
+      } else {
+        carry = true;
+      }
+    }
     if (parts && overlap) {
       HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_part[istep % 2], 0));
       if (moments && c->part_record_pending)
@@ -1000,15 +1044,30 @@ int rbn_step(rbn_ctx *c, int nsteps) {
     if ((rc = timed(c, 0, c->stream, [&] {
            rbn::LbmTuning t = c->tune;
            t.flip = t.reverse && (istep & 1);
+           if (carry)
+             return rbn::launch_lbm_fused(c->stream, c->prec, c->geom, lp, c->consts.data(), t, pending_job(c));
            return rbn::launch_lbm(c->stream, c->prec, P.variant, c->geom, lp, c->consts.data(), t);
-         })))
+         }))) {
+      c->pend.on = false;
       return rc;
+    }
+    if (carry && (rc = pending_launched(c))) return rc;
     std::swap(c->fA, c->fB);
     std::swap(c->gA, c->gB);
     const int prv = c->cur;
     c->cur = nxt;
     c->moments_valid = moments;
-    if (parts) {
+    if (fuse) {
+      rbn_ctx::RecSlot *slot = nullptr;
+      if (record && (rc = claim_record_slot(c, &slot))) return rc;
+      c->pend.on = true;
+      c->pend.istep = istep;
+      c->pend.cur = c->cur;
+      c->pend.prv = prv;
+      c->pend.record = record;
+      c->pend.full = record && P.prtcl_writefull;
+      c->pend.slot = slot;
+    } else if (parts) {
       // K2 of step s; with overlap, on the particle stream alongside K1 of step s+1.
       if (overlap) {
         HIPCHK(c, hipEventRecord(c->ev_lbm, c->stream));
@@ -1035,6 +1094,25 @@ int rbn_step(rbn_ctx *c, int nsteps) {
         return fail(c, RBN_ECALLBACK, "diagnostics callback failed at step %d", istep);
     }
   }
+  return RBN_OK;
+}
+
+}  // namespace
+
+int rbn_step(rbn_ctx *c, int nsteps) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!c->initialised) return fail(c, RBN_ESTATE, "rbn_step before initialisation");
+  if (nsteps < 0) return fail(c, RBN_EINVAL, "negative step count");
+  const rbn_params &P = c->P;
+  const bool nudged = P.variant == RBN_VARIANT_NUDGING;
+  if (nudged && c->nobs == 0 && !c->d_obs_i) return fail(c, RBN_ESTATE, "nudged run without observers");
+  const bool parts = !nudged && P.particle && c->np > 0;
+  rc = step_loop(c, nsteps);
+  // The last step's particle update (particle_fuse) has no lattice launch to ride in.
+  const int rf = flush_particles(c);
+  if (rc) return rc;
+  if (rf) return rf;
   if (parts) HIPCHK(c, hipStreamSynchronize(c->pstream));
   c->part_record_pending = false;
   if (c->timing && (rc = drain_timing(c))) return rc;
@@ -1073,12 +1151,16 @@ int rbn_set_tuning(rbn_ctx *c, const char *key, int value) {
     c->tune.npt = value;
   } else if (!strcmp(key, "lbm_ntload")) {
     c->tune.ntload = value != 0;
+  } else if (!strcmp(key, "lbm_uv_keep")) {
+    c->tune.uv_keep = value != 0;
   } else if (!strcmp(key, "lbm_xcd")) {
     c->tune.xcd = value != 0;
   } else if (!strcmp(key, "lbm_reverse")) {
     c->tune.reverse = value != 0;
   } else if (!strcmp(key, "particle_overlap")) {
     c->tune.overlap = value != 0;
+  } else if (!strcmp(key, "particle_fuse")) {
+    c->tune.fuse = value != 0;
   } else if (!strcmp(key, "adj_store_deposit")) {
     c->adj_store_deposit = value != 0;
   } else {
@@ -1093,9 +1175,11 @@ int rbn_get_tuning(const rbn_ctx *c, const char *key, int *value) {
   else if (!strcmp(key, "lbm_nt")) *value = c->tune.nt;
   else if (!strcmp(key, "lbm_npt")) *value = c->tune.npt;
   else if (!strcmp(key, "lbm_ntload")) *value = c->tune.ntload;
+  else if (!strcmp(key, "lbm_uv_keep")) *value = c->tune.uv_keep;
   else if (!strcmp(key, "lbm_xcd")) *value = c->tune.xcd;
   else if (!strcmp(key, "lbm_reverse")) *value = c->tune.reverse;
   else if (!strcmp(key, "particle_overlap")) *value = c->tune.overlap;
+  else if (!strcmp(key, "particle_fuse")) *value = c->tune.fuse;
   else if (!strcmp(key, "adj_store_deposit")) *value = c->adj_store_deposit;
   else return rbn::thread_fail(RBN_EINVAL, "unknown tuning key");
   return RBN_OK;

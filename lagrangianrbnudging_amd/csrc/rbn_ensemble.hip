@@ -56,8 +56,8 @@ __device__ __forceinline__ void particle_item(const EnsMember<R> &m, int chunk, 
   if (p >= m.np) return;
   const int cur = (m.cur0 + 1 + k) & 3, prv = (m.cur0 + k) & 3;
   const int nx = m.g.nx, ny = m.g.ny, pitch = m.g.pitch;
-  const part::Field<R> U = part::u_of<R>(m.uv[cur], nx, ny, pitch), V = part::v_of<R>(m.uv[cur], nx, ny, pitch);
-  const part::Field<R> Up = part::u_of<R>(m.uv[prv], nx, ny, pitch), Vp = part::v_of<R>(m.uv[prv], nx, ny, pitch);
+  const part::VelField<R> UV = part::vel_of<R>(m.uv[cur], nx, ny, pitch);
+  const part::VelField<R> UVp = part::vel_of<R>(m.uv[prv], nx, ny, pitch);
   const part::Field<R> T = part::scalar_of<R>(m.T, nx, ny, pitch);
   PartPtrs pp{};
   const size_t n = (size_t)m.np;
@@ -75,7 +75,7 @@ __device__ __forceinline__ void particle_item(const EnsMember<R> &m, int chunk, 
   pp.beta = b + 9 * n;
   pp.rec = nullptr;
   pp.err = m.err;
-  part::particle_update(U, V, Up, Vp, T, pp, p, 0, 0);
+  part::particle_update(UV, UVp, T, pp, p, 0, 0);
 }
 
 // Counters: [set][phase], set = step % 2.  A launch of phase P at step k clears only

@@ -99,10 +99,20 @@ struct LbmTuning {
   int npt = 1;      // nodes per thread along x (1, 2, 4, 8): row segments of block * npt nodes
   int nt = 1;
   int ntload = 0;   // non-temporal loads of the step's inputs
+  int uv_keep = 1;  // ordinary (cached) velocity stores under non-temporal stores
   int xcd = 0;
   int reverse = 1;  // sweep every other step last row first
   int flip = 0;     // this launch sweeps last row first (set per launch from reverse)
   int overlap = 0;  // particle update of step s concurrent with the lattice step s+1
+  int fuse = 1;     // particle update of step s in the grid of the lattice step s+1
+};
+
+// The particle update of one step, carried by the next lattice launch (launch_lbm_fused):
+// uv / uvp are the velocity slots of that step and the one before, T its temperature.
+struct PartJob {
+  const void *uv = nullptr, *uvp = nullptr, *T = nullptr;
+  PartPtrs pp{};
+  int record = 0, full = 0;
 };
 
 struct AdjState;  // rbn_adjoint.cpp
@@ -111,6 +121,10 @@ void adj_free(rbn_ctx *c);
 // Launchers (defined in the .hip files).
 hipError_t launch_lbm(hipStream_t s, int prec, int variant, const Geom &g, const LbmPtrs &p,
                       const void *step_const, const LbmTuning &t);
+// K1 of a truth-run step with the particle update of the previous step in the same grid
+// (its own workgroups, ahead of the lattice ones).
+hipError_t launch_lbm_fused(hipStream_t s, int prec, const Geom &g, const LbmPtrs &p,
+                            const void *step_const, const LbmTuning &t, const PartJob &job);
 hipError_t launch_init_eq(hipStream_t s, int prec, const Geom &g, const void *rho, const void *T,
                           const void *uv, void *f, void *gp, void *F, const void *step_const);
 hipError_t launch_particles(hipStream_t s, int prec, const Geom &g, const void *uv, const void *uvp,
@@ -182,6 +196,13 @@ struct rbn_ctx {
     int step = -1;        // pending record step, -1 when free
   } rec[kRecSlots];
   int rec_next = 0;
+  // The particle update of the latest step when it waits to ride in the next lattice
+  // launch (particle_fuse); never pending when rbn_step returns.
+  struct PendingParticles {
+    bool on = false;
+    int istep = 0, cur = 0, prv = 0, record = 0, full = 0;
+    RecSlot *slot = nullptr;
+  } pend;
   hipStream_t copy_stream = nullptr;
   int *d_err = nullptr;
   rbn_record_fn rec_fn = nullptr;

@@ -10,29 +10,11 @@ using namespace lbm;
 constexpr int kBlock = 256;     // block width of the init kernels
 constexpr int kMaxBlock = 512;  // K1 block widths 64..512 (tuning)
 
-// First node of this thread.  The 1-D grid enumerates row segments of blockDim.x * npt
-// nodes, row major; the thread owns nodes i, i + blockDim.x, ..., npt of them, so the
-// x-neighbour lines at the seams between its block's sub-segments stay inside one CU.  order bit 0 (XCD banding): block L goes to segment (L % 8) * (N/8) + L/8: the
-// hardware deals consecutive blocks round-robin to the 8 XCDs, so this gives every XCD one
-// contiguous band of rows and keeps the column halo of neighbouring segments in the same L2.
-// order bit 1 (reversed sweep): segments are visited last row first, so that a step that
-// follows a forward sweep starts on the rows its predecessor wrote last.
-__device__ __forceinline__ void node_of(const Geom &g, int order, int npt, int &i, int &j) {
-  const int seg = blockDim.x * npt;
-  const int gx = (g.nx + seg - 1) / seg;
-  const int N = gridDim.x;
-  int L = blockIdx.x;
-  if ((order & 1) && (N & 7) == 0) L = (L & 7) * (N >> 3) + (L >> 3);
-  if (order & 2) L = N - 1 - L;
-  j = L / gx;
-  i = (L - j * gx) * seg + threadIdx.x;
-}
-
 template <int VARIANT, int NT>
 __global__ __launch_bounds__(kMaxBlock) void k_lbm64(Geom g, LbmPtrs p, StepConst<double> c, int order,
                                                      int npt) {
   int i0, j;
-  node_of(g, order, npt, i0, j);
+  segment_node(g, blockIdx.x, gridDim.x, order, npt, i0, j);
   for (int k = 0; k < npt; k++) {
     const int i = i0 + k * (int)blockDim.x;
     if (i < g.nx) lbm_node64<VARIANT, NT>(g, p, c, i, j);
@@ -43,7 +25,7 @@ template <int VARIANT, int NT>
 __global__ __launch_bounds__(kMaxBlock) void k_lbm32(Geom g, LbmPtrs p, StepConst<float> c, int order,
                                                      int npt) {
   int i0, j;
-  node_of(g, order, npt, i0, j);
+  segment_node(g, blockIdx.x, gridDim.x, order, npt, i0, j);
   for (int k = 0; k < npt; k++) {
     const int i = i0 + k * (int)blockDim.x;
     if (i < g.nx) lbm_node32<VARIANT, NT>(g, p, c, i, j);
@@ -196,11 +178,14 @@ void lbm_dispatch(hipStream_t s, int prec, dim3 grid, dim3 block, const Geom &g,
 template <int V>
 void lbm_dispatch_nt(int nt, hipStream_t s, int prec, dim3 grid, dim3 block, const Geom &g,
                      const LbmPtrs &p, const void *step_const, int order, int npt) {
-  switch (nt & 3) {
-    case 0: lbm_dispatch<V, 0>(s, prec, grid, block, g, p, step_const, order, npt); break;
+  // Bit 2 only changes the velocity store when bit 0 is set (4 and 6 are 0 and 2).
+  switch (nt & 7) {
+    case 0: case 4: lbm_dispatch<V, 0>(s, prec, grid, block, g, p, step_const, order, npt); break;
     case 1: lbm_dispatch<V, 1>(s, prec, grid, block, g, p, step_const, order, npt); break;
-    case 2: lbm_dispatch<V, 2>(s, prec, grid, block, g, p, step_const, order, npt); break;
-    default: lbm_dispatch<V, 3>(s, prec, grid, block, g, p, step_const, order, npt); break;
+    case 2: case 6: lbm_dispatch<V, 2>(s, prec, grid, block, g, p, step_const, order, npt); break;
+    case 3: lbm_dispatch<V, 3>(s, prec, grid, block, g, p, step_const, order, npt); break;
+    case 5: lbm_dispatch<V, 5>(s, prec, grid, block, g, p, step_const, order, npt); break;
+    default: lbm_dispatch<V, 7>(s, prec, grid, block, g, p, step_const, order, npt); break;
   }
 }
 }  // namespace
@@ -211,7 +196,7 @@ hipError_t launch_lbm(hipStream_t s, int prec, int variant, const Geom &g, const
   const int npt = (t.npt == 2 || t.npt == 4 || t.npt == 8) ? t.npt : 1;
   const unsigned gx = (unsigned)((g.nx + bw * npt - 1) / (bw * npt));
   dim3 grid(gx * (unsigned)g.ny), block(bw);
-  const int nt = (t.nt ? 1 : 0) | (t.ntload ? 2 : 0);
+  const int nt = (t.nt ? 1 : 0) | (t.ntload ? 2 : 0) | (t.uv_keep ? 4 : 0);
   const int order = (t.xcd ? 1 : 0) | (t.flip ? 2 : 0);
   if (variant == RBN_VARIANT_NUDGING)
     lbm_dispatch_nt<RBN_VARIANT_NUDGING>(nt, s, prec, grid, block, g, p, step_const, order, npt);

@@ -41,7 +41,9 @@ constexpr double CS22 = 2. * CS2;
 constexpr double CSSQ = 2. / 9.;
 
 // NT flags of the fused step: bit 0 non-temporal stores of the outputs (not read again
-// within the step), bit 1 non-temporal loads of the inputs (read once per step).
+// within the step), bit 1 non-temporal loads of the inputs (read once per step), bit 2
+// ordinary stores of the velocity even under bit 0 (the particle update gathers from it
+// right after the step, so it is left to the caches).
 template <int NT, typename T>
 __device__ __forceinline__ void st(T *p, T v) {
   if (NT & 1)
@@ -67,7 +69,7 @@ __device__ __forceinline__ void st_uv(void *uv, size_t o, T u, T v) {
   T2 w;
   w.x = u;
   w.y = v;
-  st<NT>(&((T2 *)uv)[o], w);
+  st<(NT & 4) ? 0 : NT>(&((T2 *)uv)[o], w);
 }
 
 template <int NT, typename T>
@@ -402,6 +404,24 @@ __device__ __forceinline__ void lbm_node32(const Geom &g, const LbmPtrs &p, cons
     const float geq = w * (dT + T * A[k]);
     st<NT>(&gB[(size_t)k * g.plane + r0 + i], q[k] * c.one_m_omg + c.omg * geq - c.bw[k] * Q);
   }
+}
+
+// First node of this thread in lattice block L of N.  The lattice blocks enumerate row
+// segments of blockDim.x * npt nodes, row major; the thread owns nodes i, i + blockDim.x,
+// ..., npt of them, so the x-neighbour lines at the seams between its block's sub-segments
+// stay inside one CU.  order bit 0 (XCD banding): block L goes to segment (L % 8) * (N/8) +
+// L/8: the hardware deals consecutive blocks round-robin to the 8 XCDs, so this gives every
+// XCD one contiguous band of rows and keeps the column halo of neighbouring segments in the
+// same L2.  order bit 1 (reversed sweep): segments are visited last row first, so that a
+// step that follows a forward sweep starts on the rows its predecessor wrote last.
+__device__ __forceinline__ void segment_node(const Geom &g, int L, int N, int order, int npt, int &i,
+                                             int &j) {
+  const int seg = blockDim.x * npt;
+  const int gx = (g.nx + seg - 1) / seg;
+  if ((order & 1) && (N & 7) == 0) L = (L & 7) * (N >> 3) + (L >> 3);
+  if (order & 2) L = N - 1 - L;
+  j = L / gx;
+  i = (L - j * gx) * seg + threadIdx.x;
 }
 
 }  // namespace lbm

@@ -39,14 +39,26 @@ struct Field {
   }
 };
 
-// The u and v fields of an interleaved velocity slot, and a scalar plane.
+// An interleaved velocity slot: node o holds (u, v) at uv[2o], uv[2o + 1], fetched with one
+// vector load per node.
 template <typename R>
-__device__ __host__ __forceinline__ Field<R> u_of(const void *uv, int nx, int ny, int pitch) {
-  return Field<R>{(const R *)uv, nx, ny, pitch, 2};
-}
+struct VelField {
+  const R *__restrict__ uv;
+  int nx, ny, pitch;
+  template <typename D>
+  __device__ __forceinline__ void at(int ic, int jc, D &u, D &v) const {
+    typedef R R2 __attribute__((ext_vector_type(2)));
+    const int i = ic == 0 ? nx : (ic == nx + 1 ? 1 : ic);
+    const int j = jc == 0 ? ny : (jc == ny + 1 ? 1 : jc);
+    const R2 w = ((const R2 *)uv)[(size_t)(j - 1) * pitch + (i - 1)];
+    u = (D)w.x;
+    v = (D)w.y;
+  }
+};
+
 template <typename R>
-__device__ __host__ __forceinline__ Field<R> v_of(const void *uv, int nx, int ny, int pitch) {
-  return Field<R>{(const R *)uv + 1, nx, ny, pitch, 2};
+__device__ __host__ __forceinline__ VelField<R> vel_of(const void *uv, int nx, int ny, int pitch) {
+  return VelField<R>{(const R *)uv, nx, ny, pitch};
 }
 template <typename R>
 __device__ __host__ __forceinline__ Field<R> scalar_of(const void *a, int nx, int ny, int pitch) {
@@ -89,93 +101,147 @@ __device__ __forceinline__ double interp(const Field<R> &F, double xx, double yy
                xx, yy);
 }
 
+// bilinear of u and v from the same four corner loads.
+template <typename R>
+__device__ __forceinline__ void interp_uv(const VelField<R> &F, double xx, double yy, double &up,
+                                          double &vp, int &err) {
+  int x1, y1;
+  if (!corners(xx, yy, F.nx, F.ny, x1, y1)) {
+    err = 1;
+    up = vp = 0.0;
+    return;
+  }
+  double u11, v11, u12, v12, u21, v21, u22, v22;
+  F.at(x1, y1, u11, v11);
+  F.at(x1, y1 + 1, u12, v12);
+  F.at(x1 + 1, y1, u21, v21);
+  F.at(x1 + 1, y1 + 1, u22, v22);
+  up = blend(u11, u12, u21, u22, x1, y1, xx, yy);
+  vp = blend(v11, v12, v21, v22, x1, y1, xx, yy);
+}
+
 // The Maxey–Riley node fields (M:374-411) at the four corners of an interpolation cell
 // need the velocity at the two corner rows on four columns (centred x differences) and at
 // the rows below/above each corner row on the two corner columns (y differences, one-sided
 // at the walls).  Ghost corners take the values of the periodic interior node they copy,
-// so all indices are wrapped to interior nodes.  The neighbourhood is loaded once per
-// field and shared by the four corners.
+// so all indices are wrapped to interior nodes.  The neighbourhood is loaded once, u and v
+// of a node together, and shared by the four corners.  It is held in the field's own
+// precision (every value is widened to FP64 exactly where it is used).
+template <typename R>
 struct CellPatch {
-  int I[2], J[2];     // wrapped corner columns / rows (1-based interior indices)
-  double c[2][4];     // rows J[b], columns I0-1, I0, I1, I1+1
-  double lo[2][2];    // row J[b]-1 (unused at J[b] == 1), columns I0, I1
-  double hi[2][2];    // row J[b]+1 (unused at J[b] == ny), columns I0, I1
+  int I[2], J[2];  // wrapped corner columns / rows (1-based interior indices)
+  R c[2][4];       // rows J[b], columns I0-1, I0, I1, I1+1
+  R lo[2][2];      // row J[b]-1 (unused at J[b] == 1), columns I0, I1
+  R hi[2][2];      // row J[b]+1 (unused at J[b] == ny), columns I0, I1
 };
 
 template <typename R>
-__device__ __forceinline__ void load_patch(const Field<R> &F, int x1, int y1, CellPatch &P) {
+__device__ __forceinline__ void load_patch(const VelField<R> &F, int x1, int y1, CellPatch<R> &Pu,
+                                           CellPatch<R> &Pv) {
   const int nx = F.nx, ny = F.ny;
-  P.I[0] = x1 == 0 ? nx : x1;
-  P.I[1] = x1 + 1 == nx + 1 ? 1 : x1 + 1;
-  P.J[0] = y1 == 0 ? ny : y1;
-  P.J[1] = y1 + 1 == ny + 1 ? 1 : y1 + 1;
-  for (int b = 0; b < 2; b++) {
-    const int J = P.J[b];
-    P.c[b][0] = F.at(P.I[0] - 1, J);
-    P.c[b][1] = F.at(P.I[0], J);
-    P.c[b][2] = F.at(P.I[1], J);
-    P.c[b][3] = F.at(P.I[1] + 1, J);
+  Pu.I[0] = x1 == 0 ? nx : x1;
+  Pu.I[1] = x1 + 1 == nx + 1 ? 1 : x1 + 1;
+  Pu.J[0] = y1 == 0 ? ny : y1;
+  Pu.J[1] = y1 + 1 == ny + 1 ? 1 : y1 + 1;
+  for (int q = 0; q < 2; q++) {
+    Pv.I[q] = Pu.I[q];
+    Pv.J[q] = Pu.J[q];
   }
   for (int b = 0; b < 2; b++) {
-    const int J = P.J[b];
+    const int J = Pu.J[b];
+    F.at(Pu.I[0] - 1, J, Pu.c[b][0], Pv.c[b][0]);
+    F.at(Pu.I[0], J, Pu.c[b][1], Pv.c[b][1]);
+    F.at(Pu.I[1], J, Pu.c[b][2], Pv.c[b][2]);
+    F.at(Pu.I[1] + 1, J, Pu.c[b][3], Pv.c[b][3]);
+  }
+  for (int b = 0; b < 2; b++) {
+    const int J = Pu.J[b];
     const int other = 1 - b;
     for (int a = 0; a < 2; a++) {
       // reuse the other corner row when it is the neighbour row
-      if (J > 1) P.lo[b][a] = (P.J[other] == J - 1) ? P.c[other][1 + a] : F.at(P.I[a], J - 1);
-      else P.lo[b][a] = 0.0;
-      if (J < ny) P.hi[b][a] = (P.J[other] == J + 1) ? P.c[other][1 + a] : F.at(P.I[a], J + 1);
-      else P.hi[b][a] = 0.0;
+      if (J > 1) {
+        if (Pu.J[other] == J - 1) {
+          Pu.lo[b][a] = Pu.c[other][1 + a];
+          Pv.lo[b][a] = Pv.c[other][1 + a];
+        } else {
+          F.at(Pu.I[a], J - 1, Pu.lo[b][a], Pv.lo[b][a]);
+        }
+      } else {
+        Pu.lo[b][a] = Pv.lo[b][a] = R(0);
+      }
+      if (J < ny) {
+        if (Pu.J[other] == J + 1) {
+          Pu.hi[b][a] = Pu.c[other][1 + a];
+          Pv.hi[b][a] = Pv.c[other][1 + a];
+        } else {
+          F.at(Pu.I[a], J + 1, Pu.hi[b][a], Pv.hi[b][a]);
+        }
+      } else {
+        Pu.hi[b][a] = Pv.hi[b][a] = R(0);
+      }
     }
   }
 }
 
 // d/dx and d/dy of the field at corner (a, b) of the patch (M:384-405).
-__device__ __forceinline__ void patch_dxdy(const CellPatch &P, int a, int b, int ny, double &dx,
+template <typename R>
+__device__ __forceinline__ void patch_dxdy(const CellPatch<R> &P, int a, int b, int ny, double &dx,
                                            double &dy) {
-  const double u0 = P.c[b][1 + a];
-  dx = (P.c[b][2 + a] - P.c[b][a]) * 0.5;
+  const double u0 = (double)P.c[b][1 + a];
+  dx = ((double)P.c[b][2 + a] - (double)P.c[b][a]) * 0.5;
   const int J = P.J[b];
-  if (J == 1) dy = (P.hi[b][a] - u0);
-  else if (J == ny) dy = (u0 - P.lo[b][a]);
-  else dy = (P.hi[b][a] - P.lo[b][a]) * 0.5;
+  if (J == 1) dy = ((double)P.hi[b][a] - u0);
+  else if (J == ny) dy = (u0 - (double)P.lo[b][a]);
+  else dy = ((double)P.hi[b][a] - (double)P.lo[b][a]) * 0.5;
 }
 
 // U, V and the six Maxey–Riley interpolants at (xx, yy) from one patch load per field.
 template <typename R>
-__device__ __forceinline__ bool interp_all(const Field<R> &U, const Field<R> &V,
-                                           const Field<R> &Up, const Field<R> &Vp, double xx,
+__device__ __forceinline__ bool interp_all(const VelField<R> &UV, const VelField<R> &UVp, double xx,
                                            double yy, double &up, double &vp, double D[6]) {
   int x1, y1;
-  if (!corners(xx, yy, U.nx, U.ny, x1, y1)) return false;
-  CellPatch pu, pv;
-  load_patch(U, x1, y1, pu);
-  load_patch(V, x1, y1, pv);
-  // corner order of blend(): (x1,y1), (x1,y1+1), (x1+1,y1), (x1+1,y1+1)
-  const int ca[4] = {0, 0, 1, 1}, cb[4] = {0, 1, 0, 1};
-  double n[8][4];  // field q at corner k: u, v, dtu, dtv, dxu, dxv, dyu, dyv
+  if (!corners(xx, yy, UV.nx, UV.ny, x1, y1)) return false;
+  CellPatch<R> pu, pv;
+  load_patch(UV, x1, y1, pu, pv);
+  // blend() of all eight node fields, corner by corner in its summation order (x1,y1),
+  // (x1+1,y1), (x1,y1+1), (x1+1,y1+1), each term (value * x weight) * y weight: the same
+  // operations as eight blend() calls, with eight running sums live instead of 32 corner
+  // values.
+  const double wx[2] = {(double)(x1 + 1) - xx, xx - (double)x1};
+  const double wy[2] = {(double)(y1 + 1) - yy, yy - (double)y1};
+  const int ca[4] = {0, 1, 0, 1}, cb[4] = {0, 0, 1, 1};
+  double acc[8];  // u, v, dtu, dtv, dxu, dxv, dyu, dyv
 #pragma unroll
   for (int k = 0; k < 4; k++) {
     const int a = ca[k], b = cb[k];
-    const double u0 = pu.c[b][1 + a], v0 = pv.c[b][1 + a];
-    n[0][k] = u0;
-    n[1][k] = v0;
-    n[2][k] = u0 - Up.at(pu.I[a], pu.J[b]);
-    n[3][k] = v0 - Vp.at(pu.I[a], pu.J[b]);
-    patch_dxdy(pu, a, b, U.ny, n[4][k], n[6][k]);
-    patch_dxdy(pv, a, b, U.ny, n[5][k], n[7][k]);
-  }
-  up = blend(n[0][0], n[0][1], n[0][2], n[0][3], x1, y1, xx, yy);
-  vp = blend(n[1][0], n[1][1], n[1][2], n[1][3], x1, y1, xx, yy);
+    const double u0 = (double)pu.c[b][1 + a], v0 = (double)pv.c[b][1 + a];
+    double ub, vb;
+    UVp.at(pu.I[a], pu.J[b], ub, vb);
+    double n[8];
+    n[0] = u0;
+    n[1] = v0;
+    n[2] = u0 - ub;
+    n[3] = v0 - vb;
+    patch_dxdy(pu, a, b, UV.ny, n[4], n[6]);
+    patch_dxdy(pv, a, b, UV.ny, n[5], n[7]);
 #pragma unroll
-  for (int q = 0; q < 6; q++) D[q] = blend(n[2 + q][0], n[2 + q][1], n[2 + q][2], n[2 + q][3], x1, y1, xx, yy);
+    for (int q = 0; q < 8; q++) {
+      const double t = n[q] * wx[a] * wy[b];
+      acc[q] = k == 0 ? t : acc[q] + t;
+    }
+  }
+  up = acc[0];
+  vp = acc[1];
+#pragma unroll
+  for (int q = 0; q < 6; q++) D[q] = acc[2 + q];
   return true;
 }
 
 // One particle's step (index p < pp.n).
 template <typename R>
-__device__ __forceinline__ void particle_update(const Field<R> &U, const Field<R> &V, const Field<R> &Up,
-                                                const Field<R> &Vp, const Field<R> &T, const PartPtrs &pp,
-                                                int p, int record, int full) {
+__device__ __forceinline__ void particle_update(const VelField<R> &UV, const VelField<R> &UVp,
+                                                const Field<R> &T, const PartPtrs &pp, int p, int record,
+                                                int full) {
   const double delta_t = 1.0;
   int err = 0;
   const double xx = pp.x[p], yy = pp.y[p];
@@ -186,10 +252,9 @@ __device__ __forceinline__ void particle_update(const Field<R> &U, const Field<R
   double D[6] = {0, 0, 0, 0, 0, 0};
   double up = 0.0, vp = 0.0;
   if (heavy || (record && full)) {
-    if (!interp_all(U, V, Up, Vp, xx, yy, up, vp, D)) err = 1;
+    if (!interp_all(UV, UVp, xx, yy, up, vp, D)) err = 1;
   } else {
-    up = interp(U, xx, yy, err);
-    vp = interp(V, xx, yy, err);
+    interp_uv(UV, xx, yy, up, vp, err);
   }
   double x, y;
   if (heavy) {
@@ -225,7 +290,7 @@ __device__ __forceinline__ void particle_update(const Field<R> &U, const Field<R
     }
   }
   // toggle_prtcl on one rank (M:598-635, 694-717).
-  const double NXd = (double)U.nx, NYd = (double)U.ny;
+  const double NXd = (double)UV.nx, NYd = (double)UV.ny;
   if (x >= NXd) {
     x = x - NXd;
   } else if (x <= 0.0) {

@@ -10,21 +10,21 @@ using namespace part;
 constexpr int kBlock = 256;
 
 template <typename R>
-__global__ __launch_bounds__(kBlock) void k_particles(Field<R> U, Field<R> V, Field<R> Up,
-                                                      Field<R> Vp, Field<R> T, PartPtrs pp,
-                                                      int record, int full) {
+__global__ __launch_bounds__(kBlock) void k_particles(VelField<R> UV, VelField<R> UVp, Field<R> T,
+                                                      PartPtrs pp, int record, int full) {
   const int p = blockIdx.x * kBlock + threadIdx.x;
-  if (p < pp.n) particle_update(U, V, Up, Vp, T, pp, p, record, full);
+  if (p < pp.n) particle_update(UV, UVp, T, pp, p, record, full);
 }
 
 // Initial particle velocity from the fluid (init_prtcl, M:181-208).
 template <typename R>
-__global__ void k_particle_init(Field<R> U, Field<R> V, PartPtrs pp, double factor) {
+__global__ void k_particle_init(VelField<R> UV, PartPtrs pp, double factor) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= pp.n) return;
   int err = 0;
   const double xx = pp.x[p], yy = pp.y[p];
-  const double tu = interp(U, xx, yy, err), tv = interp(V, xx, yy, err);
+  double tu, tv;
+  interp_uv(UV, xx, yy, tu, tv, err);
   double pu = tu, pv = tv;
   if (pp.inv_tau[p] > 0.0) {
     pu = factor * tu;
@@ -54,12 +54,8 @@ Field<R> mk(const void *a, const Geom &g, int cs = 1) {
   return Field<R>{(const R *)a, g.nx, g.ny, g.pitch, cs};
 }
 template <typename R>
-Field<R> mk_u(const void *uv, const Geom &g) {
-  return u_of<R>(uv, g.nx, g.ny, g.pitch);
-}
-template <typename R>
-Field<R> mk_v(const void *uv, const Geom &g) {
-  return v_of<R>(uv, g.nx, g.ny, g.pitch);
+VelField<R> mk_uv(const void *uv, const Geom &g) {
+  return vel_of<R>(uv, g.nx, g.ny, g.pitch);
 }
 
 }  // namespace
@@ -70,11 +66,11 @@ hipError_t launch_particles(hipStream_t s, int prec, const Geom &g, const void *
   if (pp.n <= 0) return hipSuccess;
   dim3 grid((pp.n + kBlock - 1) / kBlock), block(kBlock);
   if (prec == RBN_FP64)
-    hipLaunchKernelGGL(k_particles<double>, grid, block, 0, s, mk_u<double>(uv, g), mk_v<double>(uv, g),
-                       mk_u<double>(uvp, g), mk_v<double>(uvp, g), mk<double>(T, g), pp, record, full);
+    hipLaunchKernelGGL(k_particles<double>, grid, block, 0, s, mk_uv<double>(uv, g), mk_uv<double>(uvp, g),
+                       mk<double>(T, g), pp, record, full);
   else
-    hipLaunchKernelGGL(k_particles<float>, grid, block, 0, s, mk_u<float>(uv, g), mk_v<float>(uv, g),
-                       mk_u<float>(uvp, g), mk_v<float>(uvp, g), mk<float>(T, g), pp, record, full);
+    hipLaunchKernelGGL(k_particles<float>, grid, block, 0, s, mk_uv<float>(uv, g), mk_uv<float>(uvp, g),
+                       mk<float>(T, g), pp, record, full);
   return hipGetLastError();
 }
 
@@ -94,11 +90,11 @@ hipError_t launch_particle_init(hipStream_t s, int prec, const Geom &g, const vo
   if (pp.n <= 0) return hipSuccess;
   dim3 grid((pp.n + kBlock - 1) / kBlock), block(kBlock);
   if (prec == RBN_FP64)
-    hipLaunchKernelGGL(k_particle_init<double>, grid, block, 0, s, mk_u<double>(uv, g),
-                       mk_v<double>(uv, g), pp, heavy_factor);
+    hipLaunchKernelGGL(k_particle_init<double>, grid, block, 0, s, mk_uv<double>(uv, g), pp,
+                       heavy_factor);
   else
-    hipLaunchKernelGGL(k_particle_init<float>, grid, block, 0, s, mk_u<float>(uv, g),
-                       mk_v<float>(uv, g), pp, heavy_factor);
+    hipLaunchKernelGGL(k_particle_init<float>, grid, block, 0, s, mk_uv<float>(uv, g), pp,
+                       heavy_factor);
   return hipGetLastError();
 }
 

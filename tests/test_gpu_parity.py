@@ -14,8 +14,9 @@ from scenarios import (assert_equal_arrays, conduction_T, max_rel, random_partic
 pytestmark = pytest.mark.gpu
 
 
-def _run_both(p, T0, particles, steps, collect=True):
+def _run_both(p, T0, particles, steps, collect=True, **tune):
     g = rbn.Solver(p)
+    g.set_tuning(**tune)
     if collect:
         g.collect_records()
         g.collect_diagnostics()
@@ -54,12 +55,13 @@ def _compare_records(g, o, full):
             assert_equal_arrays(x[k], y[k], f"record {x['step']} {k}")
 
 
+@pytest.mark.parametrize("fuse", [1, 0])
 @pytest.mark.parametrize("writefull", [0, 1])
-def test_truth_run_fp64_bit_exact_tiny(writefull):
+def test_truth_run_fp64_bit_exact_tiny(writefull, fuse):
     p = rb_params(32, 16, 1e5, noutconfig=7, prtcl_writefull=writefull, nout=50)
     T0 = conduction_T(p, 1e-2, seed=11)
     parts = random_particles(p, 24, 24, seed=12)
-    g, o = _run_both(p, T0, parts, 300)
+    g, o = _run_both(p, T0, parts, 300, particle_fuse=fuse)
     _compare_state(g, o)
     _compare_particles(g, o)
     _compare_records(g, o, writefull)
@@ -70,6 +72,69 @@ def test_truth_run_fp64_bit_exact_tiny(writefull):
     for a, b in zip(dg, do):
         assert abs(a[1] - b[1]) <= 1e-12 * abs(b[1]) + 1e-300
         assert abs(a[2] - b[2]) <= 1e-12 * abs(b[2])
+
+
+@pytest.mark.parametrize("writefull", [0, 1])
+def test_particle_fuse_fp64_bit_exact(writefull):
+    # The particle update riding in the next lattice launch, including the cases that fall
+    # back to a launch of its own: the last step of a call (chunks of 13, 2, 50, 35 steps)
+    # and a record step followed by a diagnostics step (records every 7, energy every 5:
+    # steps 14/15, 49/50, ...).
+    p = rb_params(48, 24, 1e5, noutconfig=7, prtcl_writefull=writefull, nout=5)
+    T0 = conduction_T(p, 1e-2, seed=21)
+    parts = random_particles(p, 40, 40, seed=22)
+    g = rbn.Solver(p)
+    g.set_tuning(particle_fuse=1)
+    g.collect_records()
+    g.collect_diagnostics()
+    setup(g, p, T0, parts)
+    o = setup(Oracle(p), p, T0, parts)
+    for n in (13, 2, 50, 35):
+        g.step(n)
+    o.step(100)
+    _compare_state(g, o)
+    _compare_particles(g, o)
+    _compare_records(g, o, writefull)
+    assert len(g.records) == 100 // 7
+    assert [d[0] for d in g.diagnostics] == [d[0] for d in o.diagnostics]
+
+
+@pytest.mark.parametrize("writefull", [0, 1])
+def test_particle_fuse_fp32_matches_separate_launches(writefull):
+    p = rb_params(200, 64, 1e6, noutconfig=10, prtcl_writefull=writefull, nout=11)
+    p.precision = rbn.RBN_FP32
+    T0 = conduction_T(p, 1e-2, seed=23)
+    parts = random_particles(p, 300, 300, seed=24)
+    runs = []
+    for fuse in (0, 1):
+        g = rbn.Solver(p)
+        g.set_tuning(particle_fuse=fuse)
+        g.collect_records()
+        setup(g, p, T0, parts)
+        for n in (40, 1, 59):
+            g.step(n)
+        runs.append(g)
+    _compare_state_pair(*runs)
+    a, b = runs[0].particles(), runs[1].particles()
+    for k in ("x", "y", "u", "v", "xo", "yo", "uo", "vo"):
+        assert_equal_arrays(a[k], b[k], "particle " + k)
+    assert [r["step"] for r in runs[0].records] == [r["step"] for r in runs[1].records]
+    for x, y in zip(runs[0].records, runs[1].records):
+        for k in x:
+            if isinstance(x[k], np.ndarray):
+                assert_equal_arrays(x[k], y[k], f"record {x['step']} {k}")
+    for r in runs:
+        r.close()
+
+
+def _compare_state_pair(a, b):
+    fa, ga = a.populations()
+    fb, gb = b.populations()
+    assert_equal_arrays(fa, fb, "f")
+    assert_equal_arrays(ga, gb, "g")
+    x, y = a.fields(), b.fields()
+    for k in ("rho", "T", "u", "v"):
+        assert_equal_arrays(x[k], y[k], k)
 
 
 def test_truth_run_fp64_bit_exact_config1_shape():
@@ -151,19 +216,24 @@ def test_launch_tuning_does_not_change_results(prec):
     ref = None
     shapes = [(b, nt, 0, x, 0) for b in (64, 128, 256, 512) for nt in (0, 1) for x in (0, 1)]
     shapes += [(256, 1, 1, 0, 0), (256, 1, 0, 0, 1), (512, 1, 1, 1, 1), (64, 0, 1, 0, 1)]
-    shapes = [s + (1,) for s in shapes] + [(256, 1, 0, 0, 1, 2), (128, 1, 0, 1, 1, 4), (64, 1, 0, 0, 0, 8)]
-    for block, nt, ntload, xcd, rev, npt in shapes:
+    shapes = [s + (1, 0) for s in shapes] + [(256, 1, 0, 0, 1, 2, 0), (128, 1, 0, 1, 1, 4, 0), (64, 1, 0, 0, 0, 8, 0)]
+    shapes += [(256, 1, 0, 0, 1, 1, 1), (256, 1, 1, 0, 1, 1, 1), (256, 0, 0, 0, 1, 1, 1)]
+    for block, nt, ntload, xcd, rev, npt, keep in shapes:
         s = rbn.Solver(p)
-        s.set_tuning(lbm_block=block, lbm_nt=nt, lbm_ntload=ntload, lbm_xcd=xcd, lbm_reverse=rev, lbm_npt=npt)
+        s.set_tuning(lbm_block=block, lbm_nt=nt, lbm_ntload=ntload, lbm_xcd=xcd, lbm_reverse=rev, lbm_npt=npt,
+                     lbm_uv_keep=keep)
         s.init_fields(T=T0)
         s.step(30)
         f, g = s.populations()
-        what = f"block={block} nt={nt} ntload={ntload} xcd={xcd} reverse={rev} npt={npt}"
+        fl = s.fields()
+        what = f"block={block} nt={nt} ntload={ntload} xcd={xcd} reverse={rev} npt={npt} uv_keep={keep}"
         if ref is None:
-            ref = (f, g)
+            ref = (f, g, fl)
         else:
             assert_equal_arrays(f, ref[0], f"f {what}")
             assert_equal_arrays(g, ref[1], f"g {what}")
+            for k in ("u", "v", "T", "rho"):
+                assert_equal_arrays(fl[k], ref[2][k], f"{k} {what}")
         s.close()
 
 
