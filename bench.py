@@ -74,16 +74,26 @@ def lbm_bytes_per_node(precision):
 
 
 def particle_bytes_per_step(precision, n_tracer=N_TRACER, n_inertial=N_INERTIAL):
-    """Algorithmic bytes of one particle update, record writes excluded.  Every particle reads
-    x, y, u, v, 1/tau_p (FP64) and writes x, y, u, v, xo, yo, uo, vo; a tracer reads (u, v) at
-    the 4 corners of its cell, an inertial particle also beta, (u, v) at 12 nodes (the two
-    corner rows on four columns, the rows below and above on the two corner columns) and the
-    previous step's (u, v) at the 4 corners."""
+    """Algorithmic bytes of one particle update (record writes excluded): FP64 state, 8 values
+    read and 8 written (128 B), plus the fluid values of the interpolation stencil, 8 reals
+    for a tracer ((u, v) at the 4 cell corners) and 48 for an inertial particle (the stencil
+    of u, v and their time and space differences).  The update itself reads fewer: 5 state
+    values (xo, yo, uo, vo are only written), and for an inertial particle (u, v) at 12 nodes
+    plus the previous step's at the 4 corners (32 reals)."""
     r = 8 if precision == 0 else 4
-    state = 5 * 8 + 8 * 8
-    tracer = state + 4 * 2 * r
-    inertial = state + 8 + 16 * 2 * r
-    return n_tracer * tracer + n_inertial * inertial
+    return n_tracer * (16 * 8 + 8 * r) + n_inertial * (16 * 8 + 48 * r)
+
+
+def record_steps(warmup, steps):
+    """Record steps (istep % RECORD_EVERY == 0) among the timed steps warmup+1 .. warmup+steps."""
+    return (warmup + steps) // RECORD_EVERY - warmup // RECORD_EVERY
+
+
+# Every TIMING_EVERY-th launch of each kernel class is bracketed with HIP events in the timed
+# region (events between two launches widen the gap between them by a few microseconds).
+# 7 is coprime with the record interval, so the sampled launches include record steps in
+# proportion.
+TIMING_EVERY = 7
 
 
 def run_gpu(rbn, precision, steps, warmup, timing=True, **tune):
@@ -97,8 +107,8 @@ def run_gpu(rbn, precision, steps, warmup, timing=True, **tune):
     s.step(warmup)
     s.synchronize()
     if timing:
-        s.kernel_timing(True)
-        for k in ("lbm", "particles"):
+        s.kernel_timing(True, every=TIMING_EVERY)
+        for k in ("lbm", "lbm_fused", "particles"):
             s.kernel_stats(k, reset=True)
     t0 = time.perf_counter()
     s.step(steps)
@@ -106,13 +116,15 @@ def run_gpu(rbn, precision, steps, warmup, timing=True, **tune):
     wall = time.perf_counter() - t0
     out = {"wall_s": wall, "mlups": NX * NY * steps / wall / 1e6, "ms_per_step": 1e3 * wall / steps}
     if timing:
+        # sampled launch counts and summed durations per kernel class
         n_lbm, ms_lbm = s.kernel_stats("lbm")
+        n_f, ms_f = s.kernel_stats("lbm_fused")
         n_p, ms_p = s.kernel_stats("particles")
         s.kernel_timing(False)
-        out.update(lbm_launches=n_lbm, lbm_ms=ms_lbm, lbm_avg_ms=ms_lbm / max(n_lbm, 1),
-                   particles_launches=n_p, particles_avg_ms=ms_p / max(n_p, 1),
-                   # particle updates carried by lattice launches (the rest ran on their own)
-                   carried=steps - n_p)
+        out.update(lbm_timed=n_lbm, lbm_avg_ms=ms_lbm / max(n_lbm, 1),
+                   fused_timed=n_f, fused_avg_ms=ms_f / max(n_f, 1),
+                   particles_timed=n_p, particles_avg_ms=ms_p / max(n_p, 1),
+                   record_fraction=record_steps(warmup, steps) / steps)
     fl = s.fields()
     out["finite"] = bool(np.isfinite(fl["T"]).all() and np.isfinite(fl["u"]).all())
     s.close()
@@ -358,12 +370,12 @@ def run_adjoint(rbn, window, ckpt):
             "finite": bool(np.isfinite(g).all() and np.isfinite(ga))}
 
 
-def lattice_launch_bytes(r, precision):
-    """Algorithmic bytes of the lattice launches of a timed run and the rate they reach: every
-    launch moves the lattice bytes, and those that carry the previous step's particle update
-    also that update's bytes."""
-    total = r["lbm_launches"] * NX * NY * lbm_bytes_per_node(precision) + r["carried"] * particle_bytes_per_step(precision)
-    return total, total / (r["lbm_ms"] * 1e-3) / 1e9
+def launch_bytes(precision, record_fraction, particles):
+    """Algorithmic bytes of one lattice launch, averaged over the steps: the lattice bytes, T on
+    record steps, and with particles=True the carried particle update."""
+    s = 8 if precision == 0 else 4
+    b = NX * NY * (lbm_bytes_per_node(precision) + record_fraction * s)
+    return b + (particle_bytes_per_step(precision) if particles else 0)
 
 
 TRAFFIC_PROFILE = ROOT / "profiles" / "k_lbm_fused32_4096x2048_pmc.txt"
@@ -381,28 +393,69 @@ def measured_traffic():
     return None
 
 
-def run_cpu_baseline(rbn, steps):
+def cpu_threads():
+    """Host cores this process may use, at most 16 (the host share of one GPU)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def run_cpu_baseline(rbn, steps, threads):
     """The CPU FP64 restatement of the reference loop (array-of-structs, per-direction sweeps,
-    halo copies) on one host core, same workload, bounded to a few steps."""
+    halo copies) on `threads` host cores, decomposed like the reference's MPI run: x-slabs of
+    nx/threads columns, one per thread, each an independent restatement instance stepping its
+    slab and the particles inside it (the oracle releases the GIL inside each step).  The
+    slabs wrap periodically onto themselves instead of exchanging halos, so this leaves out
+    the reference's halo traffic: an upper bound on its throughput on this host."""
+    import threading
+
     from oracle.oracle import Oracle
 
     p = make_params(rbn, 0)
-    T0, (ids, x, y, tau, zeros) = make_inputs(p)
-    o = Oracle(p)
-    o.set_faithful(True)
-    o.init_fields(T=T0)
-    o.init_particles(ids, x, y, tau_p=tau, up=zeros, vp=zeros)
-    o.step(1)
+    T0, (ids, x, y, tau, zeros) = make_inputs(p, NX, NY, N_TRACER, N_INERTIAL)
+    w = NX // threads
+    sims = []
+    placed = 0
+    for k in range(threads):
+        q = p.copy()
+        q.nx = w
+        o = Oracle(q)
+        o.set_faithful(True)
+        o.init_fields(T=np.ascontiguousarray(T0[k * w:(k + 1) * w]))
+        m = (x >= k * w) & (x < (k + 1) * w)
+        xl = x[m] - k * w
+        xl[xl == 0.0] = 0.5
+        o.init_particles(ids[m], xl, y[m], tau_p=tau[m], up=zeros[m], vp=zeros[m])
+        placed += int(m.sum())
+        sims.append(o)
+    start = threading.Barrier(threads + 1)
+    done = [0.0] * threads
+
+    def work(k):
+        sims[k].step(1)
+        start.wait()
+        sims[k].step(steps)
+        done[k] = time.perf_counter()
+
+    ts = [threading.Thread(target=work, args=(k,)) for k in range(threads)]
+    for t in ts:
+        t.start()
+    start.wait()
     t0 = time.perf_counter()
-    o.step(steps)
-    dt = time.perf_counter() - t0
-    o.close()
-    return {"value": NX * NY * steps / dt / 1e6, "unit": "MLUPS", "cores": 1, "kind": "port",
+    for t in ts:
+        t.join()
+    dt = max(done) - t0
+    for o in sims:
+        o.close()
+    return {"value": w * threads * NY * steps / dt / 1e6, "unit": "MLUPS", "cores": threads, "kind": "port",
             "sample": f"oracle/rbn_oracle.c (CPU FP64 restatement of the reference truth-run loop, "
-                      f"cost-faithful mode), {NX}x{NY} + {N_TRACER}+{N_INERTIAL} particles, "
-                      f"{steps} steps after 1 warm-up step, 1 thread; host {platform.processor() or platform.machine()}, "
-                      f"nproc {os.cpu_count()}",
-            "seconds": dt}
+                      f"cost-faithful mode) on {threads} threads, one x-slab of {w}x{NY} each with the "
+                      f"particles inside it ({NX}x{NY} + {N_TRACER}+{N_INERTIAL} particles in total), "
+                      f"{steps} steps after 1 warm-up step, slabs periodic on themselves (no halo exchange); "
+                      f"host {platform.processor() or platform.machine()}, nproc {os.cpu_count()}",
+            "seconds": dt, "particles": placed}
 
 
 def main() -> int:
@@ -410,7 +463,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--cpu-threads", type=int, default=0, help="CPU baseline threads (default: host cores, at most 16)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 companion run")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble run")
@@ -422,8 +476,8 @@ def main() -> int:
     ap.add_argument("--adj-ckpt", type=int, default=100)
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build is a single-GPU solver; the multi-GPU slab decomposition is "
-              "design only (see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py: the solver runs on one GPU; an x-slab decomposition over several GPUs "
+              "is not implemented. Run with --gpus 1.", file=sys.stderr)
         return 2
     from lagrangianrbnudging_amd import rbn
 
@@ -453,12 +507,12 @@ def main() -> int:
         print(f"bench: adjoint done {adj}", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu:
-        cpu = run_cpu_baseline(rbn, args.cpu_steps)
+        cpu = run_cpu_baseline(rbn, args.cpu_steps, args.cpu_threads or cpu_threads())
         print(f"bench: cpu done {cpu}", file=sys.stderr, flush=True)
 
-    lat_bytes = NX * NY * lbm_bytes_per_node(1)
-    total_bytes, achieved = lattice_launch_bytes(r32, 1)
-    bytes_launch = total_bytes / max(r32["lbm_launches"], 1)
+    bytes_launch = launch_bytes(1, r32["record_fraction"], True)
+    achieved = bytes_launch / (r32["fused_avg_ms"] * 1e-3) / 1e9
+    lat_bytes = launch_bytes(1, sep["record_fraction"], False)
     sep_achieved = lat_bytes / (sep["lbm_avg_ms"] * 1e-3) / 1e9
     line = {
         "metric": METRIC,
@@ -488,9 +542,9 @@ def main() -> int:
             "traffic": measured_traffic(),
             "kernel": "k_lbm_fused32 (fused f/g collide+stream + the previous step's particle update)",
             "bytes_per_launch": round(bytes_launch),
-            "avg_launch_ms": round(r32["lbm_avg_ms"], 5),
-            "launches": r32["lbm_launches"],
-            "carrying_particles": r32["carried"],
+            "avg_launch_ms": round(r32["fused_avg_ms"], 5),
+            "launches_timed": r32["fused_timed"],
+            "timed_every": TIMING_EVERY,
         },
         "cpu_baseline": cpu and {k: (round(v, 3) if isinstance(v, float) else v) for k, v in cpu.items()},
         "separate_particle_launches": {
@@ -498,15 +552,15 @@ def main() -> int:
             "ms_per_step": round(sep["ms_per_step"], 5),
             "lattice_kernel": "k_lbm32",
             "lattice_avg_ms": round(sep["lbm_avg_ms"], 5),
-            "lattice_bytes_per_launch": lat_bytes,
+            "lattice_bytes_per_launch": round(lat_bytes),
             "lattice_roofline_frac": round(sep_achieved / HBM_PEAK_GBPS, 4),
             "particles_avg_ms": round(sep["particles_avg_ms"], 5),
         },
         "fp64": r64 and {
             "mlups": round(r64["mlups"], 1),
             "ms_per_step": round(r64["ms_per_step"], 5),
-            "lbm_avg_ms": round(r64["lbm_avg_ms"], 5),
-            "roofline_frac": round(lattice_launch_bytes(r64, 0)[1] / HBM_PEAK_GBPS, 4),
+            "fused_avg_ms": round(r64["fused_avg_ms"], 5),
+            "roofline_frac": round(launch_bytes(0, r64["record_fraction"], True) / (r64["fused_avg_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
         },
         "ensemble": ens,
         "nudging": nud,

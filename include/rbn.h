@@ -305,10 +305,13 @@ int rbn_set_tuning(rbn_ctx *ctx, const char *key, int value);
 int rbn_get_tuning(const rbn_ctx *ctx, const char *key, int *value);
 
 /* ---------------------------------------------------------------- measurement */
-/* Enable per-launch HIP-event timing of each kernel on ctx's stream (0/1). */
-int rbn_set_kernel_timing(rbn_ctx *ctx, int enable);
+/* Per-launch HIP-event timing on ctx's stream: every = 0 off, 1 every launch, n > 1 every
+ * n-th launch of each kernel class (the events between two launches widen the gap between
+ * them by a few microseconds; sampling keeps that out of a timed run). */
+int rbn_set_kernel_timing(rbn_ctx *ctx, int every);
 /* Launch count and summed duration (ms) of the timed launches of kernel `name`
- * ("lbm", "particles", "obs", "deposit", "adjoint"); resets the counters when reset != 0. */
+ * ("lbm", "particles", "obs", "deposit", "adjoint", or "lbm_fused": a lattice launch
+ * carrying the previous step's particle update); resets the counters when reset != 0. */
 int rbn_kernel_stats(rbn_ctx *ctx, const char *name, int reset, long long *launches,
                      double *total_ms);
 /* Time `nsteps` steps with HIP events on ctx's stream (inputs resident on the GPU). */

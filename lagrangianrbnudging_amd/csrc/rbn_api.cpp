@@ -83,7 +83,7 @@ void fill_consts(const rbn_params &P, rbn::StepConst<R> &c) {
   }
 }
 
-const char *const kKernelNames[KT_COUNT] = {"lbm", "particles", "obs", "deposit", "adjoint"};
+const char *const kKernelNames[KT_COUNT] = {"lbm", "particles", "obs", "deposit", "adjoint", "lbm_fused"};
 
 int kernel_index(const char *name) {
   for (int k = 0; k < KT_COUNT; k++)
@@ -967,7 +967,7 @@ int pending_launched(rbn_ctx *c) {
 int flush_particles(rbn_ctx *c) {
   if (!c->pend.on) return RBN_OK;
   const rbn::PartJob job = pending_job(c);
-  int rc = timed(c, 1, c->stream, [&] {
+  int rc = timed(c, KT_PARTICLES, c->stream, [&] {
     return rbn::launch_particles(c->stream, c->prec, c->geom, job.uv, job.uvp, job.T, job.pp,
                                  c->any_heavy || job.full, job.record, job.full);
   });
@@ -996,9 +996,9 @@ int step_loop(rbn_ctx *c, int nsteps) {
       if (r == 1 && (rc = load_observation(c, istep + P.ntempread - 1))) return rc;
       const int branch = r == 1 ? 1 : (r == 0 ? 0 : 2);
       rbn::ObsPtrs o = obs_ptrs(c);
-      if ((rc = timed(c, 2, c->stream, [&] { return rbn::launch_obs_update(c->stream, branch, P.nx, P.ny, P.ntempread, o); })))
+      if ((rc = timed(c, KT_OBS, c->stream, [&] { return rbn::launch_obs_update(c->stream, branch, P.nx, P.ny, P.ntempread, o); })))
         return rc;
-      if ((rc = timed(c, 3, c->stream, [&] { return rbn::launch_deposit(c->stream, c->geom, right, o, c->dep); })))
+      if ((rc = timed(c, KT_DEPOSIT, c->stream, [&] { return rbn::launch_deposit(c->stream, c->geom, right, o, c->dep); })))
         return rc;
     }
     // K1 of step s writes the velocity buffer of step s-3: the particle update of step s-2
@@ -1041,7 +1041,7 @@ int step_loop(rbn_ctx *c, int nsteps) {
       lp.wq = c->dep.mode == RBN_DEPOSIT_BILINEAR ? c->dep.wq : nullptr;
       lp.nud = c->nud;
     }
-    if ((rc = timed(c, 0, c->stream, [&] {
+    if ((rc = timed(c, carry ? KT_LBM_FUSED : KT_LBM, c->stream, [&] {
            rbn::LbmTuning t = c->tune;
            t.flip = t.reverse && (istep & 1);
            if (carry)
@@ -1077,7 +1077,7 @@ int step_loop(rbn_ctx *c, int nsteps) {
       rbn_ctx::RecSlot *slot = nullptr;
       if (record && (rc = claim_record_slot(c, &slot))) return rc;
       rbn::PartPtrs pp = part_ptrs(c, slot ? slot->d : nullptr);
-      if ((rc = timed(c, 1, ks, [&] {
+      if ((rc = timed(c, KT_PARTICLES, ks, [&] {
              return rbn::launch_particles(ks, c->prec, c->geom, c->uv[c->cur], c->uv[prv], c->T, pp,
                                           c->any_heavy || full, record,
                                           full);
@@ -1185,11 +1185,14 @@ int rbn_get_tuning(const rbn_ctx *c, const char *key, int *value) {
   return RBN_OK;
 }
 
-int rbn_set_kernel_timing(rbn_ctx *c, int enable) {
+int rbn_set_kernel_timing(rbn_ctx *c, int every) {
   int rc = check_ctx(c);
   if (rc) return rc;
-  if (!enable && (rc = drain_timing(c))) return rc;
-  c->timing = enable != 0;
+  if (every < 0) return fail(c, RBN_EINVAL, "negative timing interval");
+  if ((rc = drain_timing(c))) return rc;
+  c->timing = every != 0;
+  c->timing_every = every > 0 ? every : 1;
+  for (long long &n : c->k_seen) n = 0;
   return RBN_OK;
 }
 

@@ -36,16 +36,20 @@ void dfree_t(T *&p) {
 int check_ctx(rbn_ctx *c);
 
 // Kernel classes of the per-launch timing (rbn_set_kernel_timing / rbn_kernel_stats).
-enum { KT_LBM, KT_PARTICLES, KT_OBS, KT_DEPOSIT, KT_ADJOINT, KT_COUNT };
+// KT_LBM_FUSED: a lattice launch carrying the previous step's particle update.
+enum { KT_LBM, KT_PARTICLES, KT_OBS, KT_DEPOSIT, KT_ADJOINT, KT_LBM_FUSED, KT_COUNT };
+static_assert(KT_COUNT == rbn_ctx::kKernelClasses, "kernel classes");
 extern const char *const kKernelNames[KT_COUNT];
 hipEvent_t take_event(rbn_ctx *c);
 int drain_timing(rbn_ctx *c);
 
-// Launch through fn() on stream s, bracketed with events on s when kernel timing is on.
+// Launch through fn() on stream s, bracketed with events on s when kernel timing is on
+// (every timing_every-th launch of the class).
 template <typename Fn>
 int timed(rbn_ctx *c, int kernel, hipStream_t s, Fn &&fn) {
   hipEvent_t a = nullptr, b = nullptr;
-  if (c->timing) {
+  const bool sample = c->timing && (c->k_seen[kernel]++ % c->timing_every) == 0;
+  if (sample) {
     a = take_event(c);
     b = take_event(c);
     if (!a || !b) return fail(c, RBN_EDEVICE, "hipEventCreate failed");
@@ -53,7 +57,7 @@ int timed(rbn_ctx *c, int kernel, hipStream_t s, Fn &&fn) {
   }
   hipError_t e = fn();
   if (e != hipSuccess) return fail(c, RBN_EDEVICE, "%s launch failed: %s", kKernelNames[kernel], hipGetErrorString(e));
-  if (c->timing) {
+  if (sample) {
     HIPCHK(c, hipEventRecord(b, s));
     c->pending.push_back({kernel, a, b});
     if (c->pending.size() > 8192) return drain_timing(c);

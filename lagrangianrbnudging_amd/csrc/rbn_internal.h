@@ -233,16 +233,19 @@ struct rbn_ctx {
   void *diag_user = nullptr;
   double *d_energy = nullptr;  // partials + 2 results
 
-  // timing
+  // timing: every timing_every-th launch of each kernel class is bracketed with events
+  static constexpr int kKernelClasses = 6;
   bool timing = false;
+  int timing_every = 1;
+  long long k_seen[kKernelClasses] = {};
   struct Pending {
     int kernel;
     hipEvent_t a, b;
   };
   std::vector<Pending> pending;
   std::vector<hipEvent_t> event_pool;
-  long long k_launches[5] = {0, 0, 0, 0, 0};
-  double k_ms[5] = {0, 0, 0, 0, 0};
+  long long k_launches[kKernelClasses] = {};
+  double k_ms[kKernelClasses] = {};
 
   // adjoint window (rbn_adj_forward / rbn_adj_backward)
   bool at_init = false;             // state is the one rbn_init_fields set (no step since)

@@ -436,8 +436,9 @@ class Solver:
             out[k] = v.value
         return out
 
-    def kernel_timing(self, on: bool) -> None:
-        self._c(self.lib.rbn_set_kernel_timing(self.ctx, 1 if on else 0))
+    def kernel_timing(self, on, every: int = 1) -> None:
+        """HIP-event timing of every `every`-th launch of each kernel class (off when on is false)."""
+        self._c(self.lib.rbn_set_kernel_timing(self.ctx, int(every) if on else 0))
 
     def kernel_stats(self, name: str, reset: bool = False) -> tuple[int, float]:
         n, ms = C.c_longlong(), C.c_double()

@@ -127,6 +127,27 @@ def test_particle_fuse_fp32_matches_separate_launches(writefull):
         r.close()
 
 
+def test_sampled_kernel_timing_counts():
+    # every = 3 over one call of 31 steps: the first lattice launch carries nothing, the
+    # next 30 carry the previous step's particle update, the last step's update runs alone.
+    p = rb_params(64, 32, 1e5, noutconfig=1000)
+    g = rbn.Solver(p)
+    setup(g, p, conduction_T(p, 1e-2, seed=25), random_particles(p, 50, 50, seed=26))
+    g.kernel_timing(True, every=3)
+    g.step(31)
+    assert g.kernel_stats("lbm")[0] == 1
+    assert g.kernel_stats("lbm_fused")[0] == 10
+    assert g.kernel_stats("particles")[0] == 1
+    n, ms = g.kernel_stats("lbm_fused", reset=True)
+    assert ms > 0 and g.kernel_stats("lbm_fused")[0] == 0
+    g.kernel_timing(False)
+    g.step(5)
+    assert g.kernel_stats("lbm_fused")[0] == 0
+    with pytest.raises(rbn.RbnError):
+        g.kernel_stats("nope")
+    g.close()
+
+
 def _compare_state_pair(a, b):
     fa, ga = a.populations()
     fb, gb = b.populations()
