@@ -239,8 +239,8 @@ int rbn_synchronize(rbn_ctx *ctx);
 
 /* ---------------------------------------------------------------- batched ensemble */
 /* Many independent truth-run contexts of uneven sizes and parameters advanced together: one
- * ensemble step is a persistent launch over a work list of every member's lattice tiles
- * followed by one over every member's particle chunks, and four steps are captured in one
+ * ensemble step is a persistent launch over a work list of every member's particle chunks
+ * (the update of the previous step) and lattice tiles, and four steps are captured in one
  * hipGraph that is replayed.  No reference counterpart (the reference runs one lattice per
  * job).  Members must share device and precision, be initialised truth-run contexts, and
  * stay owned by the caller; results equal stepping each member with rbn_step, except that
@@ -249,12 +249,14 @@ typedef struct rbn_ensemble rbn_ensemble;
 int rbn_ensemble_create(rbn_ctx *const *members, int n, rbn_ensemble **out);
 void rbn_ensemble_destroy(rbn_ensemble *e);
 const char *rbn_ensemble_last_error(const rbn_ensemble *e);
-/* Advance every member by nsteps (graph replays for whole four-step cycles). */
+/* Advance every member by nsteps (graph replays for whole four-step cycles; the first and
+ * last steps of a call, and the last step's particle update, are launched eagerly). */
 int rbn_ensemble_step(rbn_ensemble *e, int nsteps);
 /* Tile shape (tile_w in {64,128,256} nodes, tile_w*tile_h a multiple of 256), persistent
  * workgroups per CU (0: one workgroup per work item) and work order (bit 0 clear: member by
  * member, set: members interleaved, largest first; bit 1: odd steps visit the lattice work
- * items last first). */
+ * items last first; bit 2: a lattice launch and a particle launch per step, the particle
+ * launch on a second stream beside the next lattice launch, instead of one launch). */
 int rbn_ensemble_set_tuning(rbn_ensemble *e, int tile_w, int tile_h, int wg_per_cu, int order);
 int rbn_ensemble_get_tuning(const rbn_ensemble *e, int *tile_w, int *tile_h, int *wg_per_cu,
                             int *order);

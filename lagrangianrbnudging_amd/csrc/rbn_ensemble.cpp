@@ -133,32 +133,52 @@ int build(rbn_ensemble *e) {
 
 int grid_size(const rbn_ensemble *e) { return e->wg_per_cu > 0 ? e->wg_per_cu * e->cus : 0; }
 
-// Enqueue step k (0..3) of a cycle on the two streams.  cross_waits: the lattice launch
-// waits for the particle launch of step k-2 (it overwrites the velocity that one reads);
-// inside a captured cycle that dependency exists only for k >= 2, across cycles the graph
-// launches are ordered by the stream.
+// order bit 2: separate lattice and particle launches (the particle launch on a second
+// stream beside the next lattice launch) instead of one launch per step.
+bool separate(const rbn_ensemble *e) { return (e->order & 4) != 0; }
+
+int launch(rbn_ensemble *e, hipStream_t s, int phase, int k, bool moments) {
+  EHIP(e, rbn::launch_ensemble_phase(s, e->prec, phase, e->d_members, e->d_items, e->n_lbm, e->n_part,
+                                     e->d_counters, k, e->tile_w, e->tile_h, moments ? 1 : 0,
+                                     grid_size(e), phase != 1 && (e->order & 2) && (k & 1)));
+  return RBN_OK;
+}
+
+// Separate phases: enqueue step k (0..3) of a cycle on the two streams.  cross_waits: the
+// lattice launch waits for the particle launch of step k-2 (it overwrites the velocity that
+// one reads); inside a captured cycle that dependency exists only for k >= 2, across cycles
+// the graph launches are ordered by the stream.
 int enqueue_step(rbn_ensemble *e, int k, bool last, bool cross_waits) {
-  const int grid = grid_size(e);
+  int rc;
   if (cross_waits) EHIP(e, hipStreamWaitEvent(e->stream, e->ev_part[k & 1], 0));
-  EHIP(e, rbn::launch_ensemble_phase(e->stream, e->prec, 0, e->d_members, e->d_items, e->n_lbm,
-                                     e->d_counters, k, e->tile_w, e->tile_h, last ? 1 : 0, grid,
-                                     (e->order & 2) && (k & 1)));
+  if ((rc = launch(e, e->stream, 0, k, last))) return rc;
   if (e->n_part > 0) {
     EHIP(e, hipEventRecord(e->ev_lbm, e->stream));
     EHIP(e, hipStreamWaitEvent(e->pstream, e->ev_lbm, 0));
-    EHIP(e, rbn::launch_ensemble_phase(e->pstream, e->prec, 1, e->d_members, e->d_items + e->n_lbm,
-                                       e->n_part, e->d_counters, k, e->tile_w, e->tile_h, 0, grid, 0));
+    if ((rc = launch(e, e->pstream, 1, k, false))) return rc;
     EHIP(e, hipEventRecord(e->ev_part[k & 1], e->pstream));
   }
   return RBN_OK;
 }
 
+// One launch per step: step t (counted from the descriptors' state) carries the particle
+// update of step t-1; step 0 has none to carry.
+int enqueue_fused(rbn_ensemble *e, int t, bool last) {
+  return launch(e, e->stream, t > 0 && e->n_part > 0 ? 2 : 0, t & 3, last);
+}
+
+// Captured cycle: separate phases, steps 0..3 (moments on the last); one launch per step,
+// steps 1..4 (no moments: the last step of a call runs eagerly and writes them).
 int capture(rbn_ensemble *e) {
   drop_graph(e);
   EHIP(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
   int rc = RBN_OK;
-  for (int k = 0; k < 4 && !rc; k++) rc = enqueue_step(e, k, k == 3, k >= 2);
-  if (!rc && e->n_part > 0) {
+  if (separate(e)) {
+    for (int k = 0; k < 4 && !rc; k++) rc = enqueue_step(e, k, k == 3, k >= 2);
+  } else {
+    for (int t = 1; t <= 4 && !rc; t++) rc = enqueue_fused(e, t, false);
+  }
+  if (!rc && separate(e) && e->n_part > 0) {
     // join the particle stream back into the origin stream
     if (hipStreamWaitEvent(e->stream, e->ev_part[1], 0) != hipSuccess)
       rc = efail(e, RBN_EDEVICE, "capture join failed");
@@ -205,18 +225,33 @@ int run_steps(rbn_ensemble *e, int nsteps) {
   // mid-cycle do not leave set 0 clear for a cycle that starts over: clear both here
   // (an ordinary stream operation, never part of the captured graph).
   EHIP(e, hipMemsetAsync(e->d_counters, 0, 4 * sizeof(int), e->stream));
-  const int cycles = nsteps / 4, rem = nsteps % 4;
-  if (cycles > 0) {
-    if (!e->exec && (rc = capture(e))) return rc;
-    for (int c = 0; c < cycles; c++) EHIP(e, hipGraphLaunch(e->exec, e->stream));
+  if (separate(e)) {
+    const int cycles = nsteps / 4, rem = nsteps % 4;
+    if (cycles > 0) {
+      if (!e->exec && (rc = capture(e))) return rc;
+      for (int c = 0; c < cycles; c++) EHIP(e, hipGraphLaunch(e->exec, e->stream));
+    }
+    for (int k = 0; k < rem; k++)
+      if ((rc = enqueue_step(e, k, k == rem - 1, k >= 2))) return rc;
+    if (rem > 0 && e->n_part > 0) EHIP(e, hipStreamWaitEvent(e->stream, e->ev_part[(rem - 1) & 1], 0));
+  } else {
+    // step 0 eagerly, whole cycles of steps 1..4c from the graph, the rest (at least one
+    // step when nsteps >= 2, the last writing the moments) eagerly, then the particle
+    // update of the last step on its own.
+    if ((rc = enqueue_fused(e, 0, nsteps == 1))) return rc;
+    const int cycles = nsteps >= 2 ? (nsteps - 2) / 4 : 0;
+    if (cycles > 0) {
+      if (!e->exec && (rc = capture(e))) return rc;
+      for (int c = 0; c < cycles; c++) EHIP(e, hipGraphLaunch(e->exec, e->stream));
+    }
+    for (int t = 1 + 4 * cycles; t < nsteps; t++)
+      if ((rc = enqueue_fused(e, t, t == nsteps - 1))) return rc;
+    if (e->n_part > 0 && (rc = launch(e, e->stream, 1, (nsteps - 1) & 3, false))) return rc;
   }
-  for (int k = 0; k < rem; k++)
-    if ((rc = enqueue_step(e, k, k == rem - 1, k >= 2))) return rc;
-  if (rem > 0 && e->n_part > 0) EHIP(e, hipStreamWaitEvent(e->stream, e->ev_part[(rem - 1) & 1], 0));
   EHIP(e, hipStreamSynchronize(e->stream));
   EHIP(e, hipStreamSynchronize(e->pstream));
   advance_members(e, nsteps);
-  if (rem > 0) {  // member parity changed: descriptors and graph must be rebuilt
+  if (nsteps % 4) {  // member parity changed: descriptors and graph must be rebuilt
     e->built = false;
     drop_graph(e);
   }
@@ -331,8 +366,8 @@ const char *rbn_ensemble_last_error(const rbn_ensemble *e) { return e ? e->err.c
 int rbn_ensemble_set_tuning(rbn_ensemble *e, int tile_w, int tile_h, int wg_per_cu, int order) {
   if (!e) return rbn::thread_fail(RBN_EINVAL, "null ensemble");
   if (!(tile_w == 64 || tile_w == 128 || tile_w == 256) || tile_h < 1 || tile_h > 64 ||
-      (tile_w * tile_h) % 256 != 0 || wg_per_cu < 0 || wg_per_cu > 32 || order < 0 || order > 3)
-    return efail(e, RBN_EINVAL, "tile must be 64/128/256 wide with a multiple of 256 nodes; wg_per_cu 0..32; order 0..3");
+      (tile_w * tile_h) % 256 != 0 || wg_per_cu < 0 || wg_per_cu > 32 || order < 0 || order > 7)
+    return efail(e, RBN_EINVAL, "tile must be 64/128/256 wide with a multiple of 256 nodes; wg_per_cu 0..32; order 0..7");
   e->tile_w = tile_w;
   e->tile_h = tile_h;
   e->wg_per_cu = wg_per_cu;
@@ -376,7 +411,7 @@ int rbn_ensemble_autotune(rbn_ensemble *e, int trial_steps, double *best_ms_per_
   const int wpcs[] = {0, 2, 4, 8};
   for (auto &sh : shapes)
     for (int w : wpcs)
-      for (int o = 0; o < 4; o++) cands.push_back({sh[0], sh[1], w, o});
+      for (int o = 0; o < 8; o++) cands.push_back({sh[0], sh[1], w, o});
   int rc = sync_members(e);
   if (rc) return rc;
   Snapshot snap;
@@ -388,7 +423,11 @@ int rbn_ensemble_autotune(rbn_ensemble *e, int trial_steps, double *best_ms_per_
   Cand bestc = cands[0];
   for (const Cand &c : cands) {
     if ((rc = rbn_ensemble_set_tuning(e, c.tw, c.th, c.wpc, c.order))) break;
-    if ((rc = run_steps(e, 4))) break;  // build, capture, warm
+    if ((rc = run_steps(e, 4))) break;  // build, warm
+    if (!e->exec && (rc = capture(e))) break;
+    // The captured cycle of one launch per step starts at step 1 (counter set 1), which the
+    // eager steps above leave used: clear the counters before replaying.
+    EHIP(e, hipMemsetAsync(e->d_counters, 0, 4 * sizeof(int), e->stream));
     EHIP(e, hipEventRecord(a, e->stream));
     for (int s = 0; s < trial / 4; s++) EHIP(e, hipGraphLaunch(e->exec, e->stream));
     EHIP(e, hipEventRecord(b, e->stream));

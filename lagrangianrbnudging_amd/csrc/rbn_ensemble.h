@@ -22,8 +22,11 @@ struct EnsMember {
   int *err;
 };
 
+// items: the n_lbm lattice items, then the n_part particle items.  phase 0: the lattice items
+// of step k; 1: the particle items of step k; 2: the particle items of step k-1 followed by
+// the lattice items of step k in one launch.
 hipError_t launch_ensemble_phase(hipStream_t s, int prec, int phase, const void *members,
-                                 const int2 *items, int n_items, int *counters, int k, int tile_w,
-                                 int tile_h, int write_moments, int grid, int reverse);
+                                 const int2 *items, int n_lbm, int n_part, int *counters, int k,
+                                 int tile_w, int tile_h, int write_moments, int grid, int reverse);
 
 }  // namespace rbn

@@ -2,18 +2,28 @@
 // and Rayleigh numbers advanced together.  No reference counterpart: the reference runs one
 // lattice per MPI job.
 //
-// One ensemble step is two persistent launches over a work list:
-//   E1: every (member, tile) item of the fused LBM step (device code of K1, rbn_lbm_node.h);
-//   E2: every (member, 256-particle chunk) item of the particle step (K2, rbn_particle_node.h).
+// Work items:
+//   lattice: every (member, tile) item of the fused LBM step (device code of K1,
+//            rbn_lbm_node.h);
+//   particle: every (member, 256-particle chunk) item of the particle step (K2,
+//            rbn_particle_node.h).
+// By default one persistent launch per step runs the particle items of the previous step
+// followed by the lattice items of this step (the particle update of step k-1 reads the
+// velocity slots of steps k-1 and k-2 and writes only particle state; the lattice step k
+// writes slot k), so the latency-bound gathers overlap the lattice streaming.  The first
+// step of a call runs the lattice items alone and a last launch runs the final step's
+// particle items.  With separate phases (order bit 2) each step is a lattice launch and a
+// particle launch on a second stream beside the next lattice launch.
 // A fixed pool of workgroups (wg_per_cu per CU, chosen by the autotuner; 0 = one workgroup
 // per item) pulls items from an atomic counter, so small and large members share the GPU
 // instead of each small lattice leaving most CUs idle.  The host captures four consecutive
 // steps (after four steps the A/B population planes and the four-deep velocity ring of
 // every member are back where they started) into one hipGraph and replays it.
 //
-// Work counters: step k of the four-step cycle uses counter set k % 2 and clears set
-// (k+1) % 2 for the next step, which only starts after this launch has finished; the
-// counters are therefore zero at every launch without any memset node in the graph.
+// Work counters: a launch at step k uses counter set k % 2 (lattice or fused launches one
+// counter, particle launches the other) and clears its counter of set (k+1) % 2 for the
+// next step, which only starts after this launch has finished; the counters are therefore
+// zero at every launch without any memset node in the graph.
 #include "rbn_lbm_node.h"
 #include "rbn_particle_node.h"
 #include "rbn_ensemble.h"
@@ -78,72 +88,98 @@ __device__ __forceinline__ void particle_item(const EnsMember<R> &m, int chunk, 
   part::particle_update(UV, UVp, T, pp, p, 0, 0);
 }
 
-// Counters: [set][phase], set = step % 2.  A launch of phase P at step k clears only
-// counter [(k+1)%2][P]: the next launch of the same phase is on the same stream and starts
-// after this one ends, while the other phase may be running concurrently.
-__device__ __forceinline__ void clear_next(int *counters, int k, int phase) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) counters[((k + 1) & 1) * 2 + phase] = 0;
+// Counters: [set][slot], set = step % 2, slot 0 for lattice and fused launches, 1 for
+// particle launches.  A launch at step k clears only its counter of set (k+1) % 2: the next
+// launch with the same slot is on the same stream and starts after this one ends, while a
+// launch with the other slot may be running concurrently.
+__device__ __forceinline__ void clear_next(int *counters, int k, int slot) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) counters[((k + 1) & 1) * 2 + slot] = 0;
 }
 
-// reverse: visit the items last first (lattice phase on odd steps, so that a step starts on
-// the tiles its predecessor wrote last).
+// Item `it` of a launch of PHASE: the particle items come first (phases 1, 2; phase 2 runs
+// them for step k-1), then the lattice items (phases 0, 2), visited last first when reverse
+// is set (odd steps, so that a step starts on the tiles its predecessor wrote last).
+template <typename R, int PHASE>
+__device__ __forceinline__ void run_item(const EnsMember<R> *__restrict__ M, const int2 *__restrict__ items,
+                                         int it, int n_lbm, int n_first, int k, int tile_w, int tile_h,
+                                         int write_moments, int reverse) {
+  if (it < n_first) {
+    const int2 w = items[n_lbm + it];
+    particle_item(M[w.x], w.y, PHASE == 2 ? k - 1 : k);
+    return;
+  }
+  const int l = it - n_first;
+  const int2 w = items[reverse ? n_lbm - 1 - l : l];
+  lbm_item(M[w.x], w.y, k, tile_w, tile_h, write_moments);
+}
+
 template <typename R, int PHASE>
 __global__ __launch_bounds__(kThreads) void k_ensemble(const EnsMember<R> *__restrict__ M,
-                                                       const int2 *__restrict__ items, int n_items,
-                                                       int *counters, int k, int tile_w, int tile_h,
-                                                       int write_moments, int persistent, int reverse) {
-  clear_next(counters, k, PHASE);
+                                                       const int2 *__restrict__ items, int n_lbm,
+                                                       int n_part, int *counters, int k, int tile_w,
+                                                       int tile_h, int write_moments, int persistent,
+                                                       int reverse) {
+  constexpr int slot = PHASE == 1 ? 1 : 0;
+  const int n_first = PHASE == 0 ? 0 : n_part;
+  const int n_items = n_first + (PHASE == 1 ? 0 : n_lbm);
+  clear_next(counters, k, slot);
   if (!persistent) {
     const int it = blockIdx.x;
-    if (it >= n_items) return;
-    const int2 w = items[reverse ? n_items - 1 - it : it];
-    if (PHASE == 0)
-      lbm_item(M[w.x], w.y, k, tile_w, tile_h, write_moments);
-    else
-      particle_item(M[w.x], w.y, k);
+    if (it < n_items) run_item<R, PHASE>(M, items, it, n_lbm, n_first, k, tile_w, tile_h, write_moments, reverse);
     return;
   }
   __shared__ int item;
-  int *ctr = &counters[(k & 1) * 2 + PHASE];
+  int *ctr = &counters[(k & 1) * 2 + slot];
   for (;;) {
     if (threadIdx.x == 0) item = atomicAdd(ctr, 1);
     __syncthreads();
     const int it = item;
     __syncthreads();
     if (it >= n_items) break;
-    const int2 w = items[reverse ? n_items - 1 - it : it];
-    if (PHASE == 0)
-      lbm_item(M[w.x], w.y, k, tile_w, tile_h, write_moments);
-    else
-      particle_item(M[w.x], w.y, k);
+    run_item<R, PHASE>(M, items, it, n_lbm, n_first, k, tile_w, tile_h, write_moments, reverse);
   }
 }
 
 }  // namespace
 
+namespace {
+template <typename R, int PHASE>
+void launch_phase(hipStream_t s, dim3 g, const void *members, const int2 *items, int n_lbm, int n_part,
+                  int *counters, int k, int tile_w, int tile_h, int write_moments, int persistent,
+                  int reverse) {
+  hipLaunchKernelGGL((k_ensemble<R, PHASE>), g, dim3(kThreads), 0, s, (const EnsMember<R> *)members, items,
+                     n_lbm, n_part, counters, k, tile_w, tile_h, write_moments, persistent, reverse);
+}
+
+template <typename R>
+void launch_prec(hipStream_t s, int phase, dim3 g, const void *members, const int2 *items, int n_lbm,
+                 int n_part, int *counters, int k, int tile_w, int tile_h, int write_moments,
+                 int persistent, int reverse) {
+  if (phase == 0)
+    launch_phase<R, 0>(s, g, members, items, n_lbm, n_part, counters, k, tile_w, tile_h, write_moments,
+                       persistent, reverse);
+  else if (phase == 1)
+    launch_phase<R, 1>(s, g, members, items, n_lbm, n_part, counters, k, tile_w, tile_h, write_moments,
+                       persistent, reverse);
+  else
+    launch_phase<R, 2>(s, g, members, items, n_lbm, n_part, counters, k, tile_w, tile_h, write_moments,
+                       persistent, reverse);
+}
+}  // namespace
+
 hipError_t launch_ensemble_phase(hipStream_t s, int prec, int phase, const void *members,
-                                 const int2 *items, int n_items, int *counters, int k, int tile_w,
-                                 int tile_h, int write_moments, int grid, int reverse) {
+                                 const int2 *items, int n_lbm, int n_part, int *counters, int k,
+                                 int tile_w, int tile_h, int write_moments, int grid, int reverse) {
+  const int n_items = (phase == 0 ? 0 : n_part) + (phase == 1 ? 0 : n_lbm);
   if (n_items <= 0) return hipSuccess;
   const int persistent = grid > 0;
-  const dim3 g(persistent ? (unsigned)grid : (unsigned)n_items), b(kThreads);
-  if (prec == RBN_FP64) {
-    const auto *M = (const EnsMember<double> *)members;
-    if (phase == 0)
-      hipLaunchKernelGGL((k_ensemble<double, 0>), g, b, 0, s, M, items, n_items, counters, k, tile_w,
-                         tile_h, write_moments, persistent, reverse);
-    else
-      hipLaunchKernelGGL((k_ensemble<double, 1>), g, b, 0, s, M, items, n_items, counters, k, tile_w,
-                         tile_h, write_moments, persistent, reverse);
-  } else {
-    const auto *M = (const EnsMember<float> *)members;
-    if (phase == 0)
-      hipLaunchKernelGGL((k_ensemble<float, 0>), g, b, 0, s, M, items, n_items, counters, k, tile_w,
-                         tile_h, write_moments, persistent, reverse);
-    else
-      hipLaunchKernelGGL((k_ensemble<float, 1>), g, b, 0, s, M, items, n_items, counters, k, tile_w,
-                         tile_h, write_moments, persistent, reverse);
-  }
+  const dim3 g(persistent ? (unsigned)grid : (unsigned)n_items);
+  if (prec == RBN_FP64)
+    launch_prec<double>(s, phase, g, members, items, n_lbm, n_part, counters, k, tile_w, tile_h,
+                        write_moments, persistent, reverse);
+  else
+    launch_prec<float>(s, phase, g, members, items, n_lbm, n_part, counters, k, tile_w, tile_h,
+                       write_moments, persistent, reverse);
   return hipGetLastError();
 }
 

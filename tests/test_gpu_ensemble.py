@@ -1,5 +1,6 @@
-"""Batched ensemble: one persistent launch per phase over all members, hipGraph-captured,
-must give exactly what stepping every member on its own gives."""
+"""Batched ensemble: persistent launches over all members (one per step carrying the previous
+step's particle update, or a lattice and a particle launch per step), hipGraph-captured, must
+give exactly what stepping every member on its own gives."""
 import numpy as np
 import pytest
 
@@ -40,16 +41,19 @@ def _compare(a, b):
 
 
 @pytest.mark.parametrize("prec", [rbn.RBN_FP64, rbn.RBN_FP32])
-@pytest.mark.parametrize("tuning", [(256, 1, 0, 0), (256, 2, 4, 1), (64, 8, 2, 2), (128, 4, 8, 3), (256, 1, 0, 2)])
+@pytest.mark.parametrize("tuning", [(256, 1, 0, 0), (256, 2, 4, 1), (64, 8, 2, 2), (128, 4, 8, 3), (256, 1, 0, 2),
+                                    (256, 1, 4, 4), (128, 4, 2, 7)])
 def test_ensemble_equals_individual_steps(prec, tuning):
     ens_members = _members(prec)
     ref_members = _members(prec)
     e = rbn.Ensemble(ens_members)
     e.set_tuning(*tuning)
-    e.step(9)   # two graph cycles + one eager step (parity change, rebuild)
-    e.step(13)  # three cycles + one eager step from the shifted state
+    # Calls of 9, 1, 2, 6, 13 steps: graph cycles and eager steps, calls without a cycle,
+    # parity changes (rebuilds) between calls.
+    for n in (9, 1, 2, 6, 13):
+        e.step(n)
     for s in ref_members:
-        s.step(22)
+        s.step(31)
     _compare(ens_members, ref_members)
     e.close()
 
