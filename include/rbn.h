@@ -290,7 +290,8 @@ int rbn_adj_backward(rbn_ctx *ctx, double *dJ_dT0, double *dJ_dalpha);
  *   "lbm_ntload" 1: non-temporal loads of the step's inputs (default 0)
  *   "lbm_uv_keep" 1: ordinary (cached) stores of the velocity under lbm_nt, for the particle
  *                update that gathers from it next (default 1)
- *   "lbm_xcd"    1: give each of the 8 XCDs a contiguous band of rows (default 0)
+ *   "lbm_xcd"    1: give each of the 8 XCDs a contiguous band of rows; 2: give each XCD one
+ *                row in eight, its segments visited in order (default 0)
  *   "lbm_reverse" 1: sweep every odd step last row first, so that it starts on the rows the
  *                previous step wrote last (default 1)
  *   "particle_overlap"  1: run the particle update of step s on its own stream concurrently

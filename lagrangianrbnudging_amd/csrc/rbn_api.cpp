@@ -1154,7 +1154,8 @@ int rbn_set_tuning(rbn_ctx *c, const char *key, int value) {
   } else if (!strcmp(key, "lbm_uv_keep")) {
     c->tune.uv_keep = value != 0;
   } else if (!strcmp(key, "lbm_xcd")) {
-    c->tune.xcd = value != 0;
+    if (value < 0 || value > 2) return fail(c, RBN_EINVAL, "lbm_xcd must be 0, 1 or 2");
+    c->tune.xcd = value;
   } else if (!strcmp(key, "lbm_reverse")) {
     c->tune.reverse = value != 0;
   } else if (!strcmp(key, "particle_overlap")) {

@@ -100,7 +100,7 @@ struct LbmTuning {
   int nt = 1;
   int ntload = 0;   // non-temporal loads of the step's inputs
   int uv_keep = 1;  // ordinary (cached) velocity stores under non-temporal stores
-  int xcd = 0;
+  int xcd = 0;      // 1: a contiguous band of rows per XCD; 2: one row in eight per XCD
   int reverse = 1;  // sweep every other step last row first
   int flip = 0;     // this launch sweeps last row first (set per launch from reverse)
   int overlap = 0;  // particle update of step s concurrent with the lattice step s+1

@@ -197,7 +197,7 @@ hipError_t launch_lbm(hipStream_t s, int prec, int variant, const Geom &g, const
   const unsigned gx = (unsigned)((g.nx + bw * npt - 1) / (bw * npt));
   dim3 grid(gx * (unsigned)g.ny), block(bw);
   const int nt = (t.nt ? 1 : 0) | (t.ntload ? 2 : 0) | (t.uv_keep ? 4 : 0);
-  const int order = (t.xcd ? 1 : 0) | (t.flip ? 2 : 0);
+  const int order = (t.xcd == 1 ? 1 : 0) | (t.flip ? 2 : 0) | (t.xcd == 2 ? 4 : 0);
   if (variant == RBN_VARIANT_NUDGING)
     lbm_dispatch_nt<RBN_VARIANT_NUDGING>(nt, s, prec, grid, block, g, p, step_const, order, npt);
   else

@@ -93,7 +93,7 @@ hipError_t launch_lbm_fused(hipStream_t s, int prec, const Geom &g, const LbmPtr
   const int npt = (t.npt == 2 || t.npt == 4 || t.npt == 8) ? t.npt : 1;
   const unsigned lattice_blocks = (unsigned)((g.nx + bw * npt - 1) / (bw * npt)) * (unsigned)g.ny;
   const int nt = (t.nt ? 1 : 0) | (t.ntload ? 2 : 0) | (t.uv_keep ? 4 : 0);
-  const int order = (t.xcd ? 1 : 0) | (t.flip ? 2 : 0);
+  const int order = (t.xcd == 1 ? 1 : 0) | (t.flip ? 2 : 0) | (t.xcd == 2 ? 4 : 0);
   // Bit 2 only changes the velocity store when bit 0 is set (4 and 6 are 0 and 2).
   switch (nt & 7) {
     case 0: case 4: fused_dispatch<0>(s, prec, lattice_blocks, bw, g, p, step_const, order, npt, job); break;

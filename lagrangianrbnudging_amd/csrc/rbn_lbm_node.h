@@ -413,13 +413,23 @@ __device__ __forceinline__ void lbm_node32(const Geom &g, const LbmPtrs &p, cons
 // L/8: the hardware deals consecutive blocks round-robin to the 8 XCDs, so this gives every
 // XCD one contiguous band of rows and keeps the column halo of neighbouring segments in the
 // same L2.  order bit 1 (reversed sweep): segments are visited last row first, so that a
-// step that follows a forward sweep starts on the rows its predecessor wrote last.
+// step that follows a forward sweep starts on the rows its predecessor wrote last.  order
+// bit 2 (rows per XCD, when ny is a multiple of 8): block L sweeps segment (L/8) % gx of row
+// 8 * (L/8 / gx) + L % 8, so every XCD owns one row in eight and visits the segments of its
+// row in order: the x-neighbour lines at a seam stay in one L2 while all XCDs still work on
+// the same eight rows.
 __device__ __forceinline__ void segment_node(const Geom &g, int L, int N, int order, int npt, int &i,
                                              int &j) {
   const int seg = blockDim.x * npt;
   const int gx = (g.nx + seg - 1) / seg;
   if ((order & 1) && (N & 7) == 0) L = (L & 7) * (N >> 3) + (L >> 3);
   if (order & 2) L = N - 1 - L;
+  if ((order & 4) && (g.ny & 7) == 0) {
+    const int q = L >> 3;
+    j = (q / gx) * 8 + (L & 7);
+    i = (q - (q / gx) * gx) * seg + threadIdx.x;
+    return;
+  }
   j = L / gx;
   i = (L - j * gx) * seg + threadIdx.x;
 }

@@ -108,7 +108,7 @@ def test_particle_fuse_fp32_matches_separate_launches(writefull):
     runs = []
     for fuse in (0, 1):
         g = rbn.Solver(p)
-        g.set_tuning(particle_fuse=fuse)
+        g.set_tuning(particle_fuse=fuse, lbm_xcd=2 * fuse)
         g.collect_records()
         setup(g, p, T0, parts)
         for n in (40, 1, 59):
@@ -239,6 +239,7 @@ def test_launch_tuning_does_not_change_results(prec):
     shapes += [(256, 1, 1, 0, 0), (256, 1, 0, 0, 1), (512, 1, 1, 1, 1), (64, 0, 1, 0, 1)]
     shapes = [s + (1, 0) for s in shapes] + [(256, 1, 0, 0, 1, 2, 0), (128, 1, 0, 1, 1, 4, 0), (64, 1, 0, 0, 0, 8, 0)]
     shapes += [(256, 1, 0, 0, 1, 1, 1), (256, 1, 1, 0, 1, 1, 1), (256, 0, 0, 0, 1, 1, 1)]
+    shapes += [(256, 1, 0, 2, 1, 1, 1), (128, 1, 0, 2, 0, 2, 0), (512, 1, 0, 2, 1, 1, 1)]
     for block, nt, ntload, xcd, rev, npt, keep in shapes:
         s = rbn.Solver(p)
         s.set_tuning(lbm_block=block, lbm_nt=nt, lbm_ntload=ntload, lbm_xcd=xcd, lbm_reverse=rev, lbm_npt=npt,
