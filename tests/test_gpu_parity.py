@@ -106,23 +106,24 @@ def test_particle_fuse_fp32_matches_separate_launches(writefull):
     T0 = conduction_T(p, 1e-2, seed=23)
     parts = random_particles(p, 300, 300, seed=24)
     runs = []
-    for fuse in (0, 1):
+    for tune in (dict(particle_fuse=0), dict(particle_fuse=1), dict(particle_fuse=1, lbm_xcd=2)):
         g = rbn.Solver(p)
-        g.set_tuning(particle_fuse=fuse, lbm_xcd=2 * fuse)
+        g.set_tuning(**tune)
         g.collect_records()
         setup(g, p, T0, parts)
         for n in (40, 1, 59):
             g.step(n)
         runs.append(g)
-    _compare_state_pair(*runs)
-    a, b = runs[0].particles(), runs[1].particles()
-    for k in ("x", "y", "u", "v", "xo", "yo", "uo", "vo"):
-        assert_equal_arrays(a[k], b[k], "particle " + k)
-    assert [r["step"] for r in runs[0].records] == [r["step"] for r in runs[1].records]
-    for x, y in zip(runs[0].records, runs[1].records):
-        for k in x:
-            if isinstance(x[k], np.ndarray):
-                assert_equal_arrays(x[k], y[k], f"record {x['step']} {k}")
+    for other in runs[1:]:
+        _compare_state_pair(runs[0], other)
+        a, b = runs[0].particles(), other.particles()
+        for k in ("x", "y", "u", "v", "xo", "yo", "uo", "vo"):
+            assert_equal_arrays(a[k], b[k], "particle " + k)
+        assert [r["step"] for r in runs[0].records] == [r["step"] for r in other.records]
+        for x, y in zip(runs[0].records, other.records):
+            for k in x:
+                if isinstance(x[k], np.ndarray):
+                    assert_equal_arrays(x[k], y[k], f"record {x['step']} {k}")
     for r in runs:
         r.close()
 
