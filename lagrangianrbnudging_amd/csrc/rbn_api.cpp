@@ -1001,16 +1001,18 @@ int step_loop(rbn_ctx *c, int nsteps) {
       if ((rc = timed(c, KT_DEPOSIT, c->stream, [&] { return rbn::launch_deposit(c->stream, c->geom, right, o, c->dep); })))
         return rc;
     }
-    // K1 of step s writes the velocity buffer of step s-3: the particle update of step s-2
-    // (which reads s-2 and s-3) must have finished; on record steps the particle update
-    // also reads T, which this step overwrites when it writes moments.
+    // K1 of step s overwrites the velocity slot of step s-4, which the particle update of
+    // step s-3 reads; on record steps the particle update also reads T, which K1 overwrites
+    // when it writes moments.
     const int nxt = (c->cur + 1) % rbn_ctx::kVel;
-    // Without overlap (the default: both kernels are bandwidth bound, and concurrency only
-    // moves the contention) K2 runs on the lattice stream itself, ordered without events (a
-    // cross-stream event wait costs about 20 us per hand-over).
+    // particle_fuse (default): K2 of step s rides in the grid of K1 of step s+1, unless step
+    // s is a record step and step s+1 rewrites the moments.  Otherwise K2 runs on the lattice
+    // stream after K1, ordered without events (a cross-stream event wait costs about 20 us
+    // per hand-over), or with particle_overlap on its own stream beside the next K1 (both
+    // kernels are bandwidth bound, so concurrency only moves the contention).  With overlap,
+    // K1 of step s waits for the particle update of step s-2 (the latest event of the same
+    // parity, which covers step s-3).
     const bool overlap = c->tune.overlap != 0;
-    // particle_fuse: K2 of step s rides in the grid of K1 of step s+1, unless step s is a
-    // record step (its update reads T) and step s+1 rewrites the moments.
     const bool fuse = parts && !overlap && c->tune.fuse != 0;
     hipStream_t ks = overlap ? c->pstream : c->stream;
     bool carry = false;
