@@ -16,7 +16,9 @@ launch in the timed region; the HBM traffic per launch comes from the committed 
 counter summary under profiles/), the same step with the particle update as a launch of its
 own (the lattice kernel's roofline alone),
 the FP64 path's MLUPS on the same workload, the batched ensemble (64 uneven members) against
-the same members stepped one by one, the adjoint window's cost against the plain nudged
+the same members stepped one by one, the nudging configuration (1024 x 512: the L2
+temperature error of nudged runs against the truth over 20,000 steps, and a sweep over the
+number of observed tracers and alpha), the adjoint window's cost against the plain nudged
 forward (512 x 256, 2000 steps, checkpoint every 100), and the CPU FP64 restatement of the
 reference loop timed on a bounded sample on this host.
 """
@@ -186,6 +188,84 @@ def run_ensemble(rbn, steps, warmup):
 
 NUD_NX, NUD_NY, NUD_RA = 1024, 512, 1e7
 NUD_OBS, NUD_EVERY, NUD_ALPHA = 10_000, 5, 0.1
+NUD_SWEEP_OBS, NUD_SWEEP_ALPHA = (1_000, 10_000, 100_000), (0.01, 0.1, 1.0)
+
+
+def nudging_truth(rbn, spinup):
+    """The spun-up truth run of the nudging configuration and the nudged runs' first state."""
+    pt = rbn.default_params(nx=NUD_NX, ny=NUD_NY, Tup=0.0, Tdown=1.0, scratch=1, nstart=0, nout=10**9,
+                            noutconfig=NUD_EVERY, prtcl_writefull=0, particle=1,
+                            precision=rbn.RBN_FP32, T0_shift=0.5, rho0_shift=1.0)
+    pt = rbn.params_from_ra(pt, NUD_RA, PR, U_FF)
+    lin = np.tile(pt.Tdown + (pt.Tup - pt.Tdown) * np.arange(NUD_NY) / (NUD_NY - 1), (NUD_NX, 1))
+    # four convection rolls seeded in the truth so that the spin-up reaches developed flow
+    ii, jj = np.meshgrid(np.arange(NUD_NX), np.arange(NUD_NY), indexing="ij")
+    rolls = 0.05 * np.sin(np.pi * jj / (NUD_NY - 1)) * np.cos(2 * np.pi * 2 * ii / NUD_NX)
+    T_truth0 = lin + rolls + np.random.default_rng(5).uniform(-1e-3, 1e-3, lin.shape)
+    T_first = lin + np.random.default_rng(6).uniform(-1e-3, 1e-3, lin.shape)
+    truth = rbn.Solver(pt, 0)
+    truth.init_fields(T=T_truth0)
+    truth.step(spinup)
+    return pt, truth, T_first
+
+
+def nudged_solver(rbn, pt, recs, ids, x, y, T_at, T_first, prec, alpha, deposit):
+    q = pt.copy()
+    q.variant = rbn.RBN_VARIANT_NUDGING
+    q.ntempread = NUD_EVERY
+    q.alpha = alpha
+    q.precision = prec
+    q.deposit = deposit
+    s = rbn.Solver(q, 0)
+    s.set_observation_source(recs)
+    s.init_fields(T=T_first)
+    s.init_observers(ids, x, y, T_at)
+    return s
+
+
+def run_nudging_sweep(rbn, steps, spinup):
+    """The nudging configuration's parameter sweep: N_p in {1e3, 1e4, 1e5} tracer observations
+    times alpha in {0.01, 0.1, 1}, FP32, bilinear deposit, `steps` steps from the same spun-up
+    truth.  The observers of the smaller sets are the first ids of one 1e5-tracer truth run
+    (tracers do not act on the flow).  Reported: the final L2 temperature error of every run,
+    the initial error and the nudged step time per N_p at alpha = 0.1."""
+    pt, truth, T_first = nudging_truth(rbn, spinup)
+    T_start = truth.fields()["T"]
+    n_max = NUD_SWEEP_OBS[-1]
+    rng = np.random.default_rng(8)
+    ids = np.arange(n_max, dtype=np.int32)
+    x = rng.uniform(0.0, NUD_NX, n_max)
+    x[x == 0.0] = 0.5
+    y = rng.uniform(1.0, NUD_NY, n_max)
+    truth.init_particles(ids, x, y, tau_p=np.zeros(n_max))
+    T_at = truth.sample("T", x, y)
+    truth.collect_records()
+    truth.step(steps)
+    T_end = truth.fields()["T"]
+    truth.step(NUD_EVERY)  # the record the last interval reads
+    recs = {r["step"] - spinup: {"id": r["id"], "x": r["x"], "y": r["y"], "temp": r["temp"]}
+            for r in truth.records}
+    truth.close()
+    l2, ms = [], []
+    for n in NUD_SWEEP_OBS:
+        sub = {k: {f: v[f][:n] for f in v} for k, v in recs.items()}
+        row = []
+        for alpha in NUD_SWEEP_ALPHA:
+            s = nudged_solver(rbn, pt, sub, ids[:n], x[:n], y[:n], T_at[:n], T_first, rbn.RBN_FP32, alpha,
+                              rbn.RBN_DEPOSIT_BILINEAR)
+            s.synchronize()
+            t0 = time.perf_counter()
+            s.step(steps)
+            s.synchronize()
+            dt = time.perf_counter() - t0
+            row.append(float(f"{np.sqrt(np.mean((s.fields()['T'] - T_end) ** 2)):.4g}"))
+            if alpha == NUD_ALPHA:
+                ms.append(round(1e3 * dt / steps, 4))
+            s.close()
+        l2.append(row)
+    return {"steps": steps, "n_obs": list(NUD_SWEEP_OBS), "alpha": list(NUD_SWEEP_ALPHA),
+            "l2_T_error_initial": float(f"{np.sqrt(np.mean((T_first - T_start) ** 2)):.4g}"),
+            "l2_T_error_final": l2, "ms_per_step_alpha_0.1": ms}
 
 
 def run_nudging(rbn, steps, spinup, chunks=10):
@@ -198,25 +278,13 @@ def run_nudging(rbn, steps, spinup, chunks=10):
     rate (bilinear deposit), the L2 temperature error against the truth every steps/chunks
     steps for the bilinear and the 7x7 deposit and for a free (alpha = 0) run, and the
     FP32-vs-FP64 temperature difference of the bilinear nudged run at the end."""
-    pt = rbn.default_params(nx=NUD_NX, ny=NUD_NY, Tup=0.0, Tdown=1.0, scratch=1, nstart=0, nout=10**9,
-                            noutconfig=NUD_EVERY, prtcl_writefull=0, particle=1,
-                            precision=rbn.RBN_FP32, T0_shift=0.5, rho0_shift=1.0)
-    pt = rbn.params_from_ra(pt, NUD_RA, PR, U_FF)
-    lin = np.tile(pt.Tdown + (pt.Tup - pt.Tdown) * np.arange(NUD_NY) / (NUD_NY - 1), (NUD_NX, 1))
-    # four convection rolls seeded in the truth so that the spin-up reaches developed flow
-    ii, jj = np.meshgrid(np.arange(NUD_NX), np.arange(NUD_NY), indexing="ij")
-    rolls = 0.05 * np.sin(np.pi * jj / (NUD_NY - 1)) * np.cos(2 * np.pi * 2 * ii / NUD_NX)
-    T_truth0 = lin + rolls + np.random.default_rng(5).uniform(-1e-3, 1e-3, lin.shape)
-    T_first = lin + np.random.default_rng(6).uniform(-1e-3, 1e-3, lin.shape)
+    pt, truth, T_first = nudging_truth(rbn, spinup)
     rng = np.random.default_rng(7)
     ids = np.arange(NUD_OBS, dtype=np.int32)
     x = rng.uniform(0.0, NUD_NX, NUD_OBS)
     x[x == 0.0] = 0.5
     y = rng.uniform(1.0, NUD_NY, NUD_OBS)
     chunk = max(steps // chunks, NUD_EVERY)
-    truth = rbn.Solver(pt, 0)
-    truth.init_fields(T=T_truth0)
-    truth.step(spinup)
     e_spin = truth.energy()
     T_start = truth.fields()["T"]
     truth.init_particles(ids, x, y, tau_p=np.zeros(NUD_OBS))
@@ -232,17 +300,7 @@ def run_nudging(rbn, steps, spinup, chunks=10):
     truth.close()
 
     def nudged(prec, alpha, deposit):
-        q = pt.copy()
-        q.variant = rbn.RBN_VARIANT_NUDGING
-        q.ntempread = NUD_EVERY
-        q.alpha = alpha
-        q.precision = prec
-        q.deposit = deposit
-        s = rbn.Solver(q, 0)
-        s.set_observation_source(recs)
-        s.init_fields(T=T_first)
-        s.init_observers(ids, x, y, T_at)
-        return s
+        return nudged_solver(rbn, pt, recs, ids, x, y, T_at, T_first, prec, alpha, deposit)
 
     def l2_curve(prec, alpha, deposit):
         s = nudged(prec, alpha, deposit)
@@ -458,6 +516,18 @@ def run_cpu_baseline(rbn, steps, threads):
             "seconds": dt, "particles": placed}
 
 
+def finite_or_null(v):
+    """The result with every non-finite float replaced by null (a diverged run), so that the
+    line stays strict JSON."""
+    if isinstance(v, dict):
+        return {k: finite_or_null(x) for k, x in v.items()}
+    if isinstance(v, (list, tuple)):
+        return [finite_or_null(x) for x in v]
+    if isinstance(v, float) and not np.isfinite(v):
+        return None
+    return v
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -470,7 +540,8 @@ def main() -> int:
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble run")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-window run")
     ap.add_argument("--no-nudging", action="store_true", help="skip the nudging run")
-    ap.add_argument("--nud-steps", type=int, default=2000)
+    ap.add_argument("--nud-steps", type=int, default=20000)
+    ap.add_argument("--nud-sweep-steps", type=int, default=2000, help="steps of each N_p x alpha sweep run (0: skip)")
     ap.add_argument("--nud-spinup", type=int, default=30000)
     ap.add_argument("--adj-window", type=int, default=2000)
     ap.add_argument("--adj-ckpt", type=int, default=100)
@@ -501,6 +572,9 @@ def main() -> int:
     if not args.no_nudging:
         nud = run_nudging(rbn, args.nud_steps, args.nud_spinup)
         print(f"bench: nudging done {nud}", file=sys.stderr, flush=True)
+        if args.nud_sweep_steps > 0:
+            nud["sweep"] = run_nudging_sweep(rbn, args.nud_sweep_steps, args.nud_spinup)
+            print(f"bench: nudging sweep done {nud['sweep']}", file=sys.stderr, flush=True)
     adj = None
     if not args.no_adjoint:
         adj = run_adjoint(rbn, args.adj_window, args.adj_ckpt)
@@ -567,7 +641,7 @@ def main() -> int:
         "adjoint": adj,
         "finite": r32["finite"] and (r64 is None or r64["finite"]) and (adj is None or adj["finite"]),
     }
-    print(json.dumps(line))
+    print(json.dumps(finite_or_null(line), allow_nan=False))
     return 0
 
 

@@ -222,7 +222,7 @@ int rbn_get_nudging(rbn_ctx *ctx, int *nudge_vol, double *tempreadint, double *n
 /* Per-node deposit weight W [nx][ny] of the latest step's bilinear deposit
  * (RBN_DEPOSIT_BILINEAR: each observer gives the four corners of its cell its bilinear
  * interpolation weights, Tbar = sum w T / W, and the nudged collision uses
- * Q = alpha * (W * (T - Tbar))); zero for the reference 7x7 deposit.  No reference
+ * Q = alpha * (min(W, 1) * (T - Tbar))); zero for the reference 7x7 deposit.  No reference
  * counterpart: the bilinear deposit is an option beside the reference's running mean. */
 int rbn_get_nudging_weights(rbn_ctx *ctx, double *weight);
 

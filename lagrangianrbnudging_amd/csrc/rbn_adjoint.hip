@@ -191,7 +191,7 @@ __global__ __launch_bounds__(kT) void k_adjoint(const AdjStepArgs a) {
       lu += lfac * (9.0 * CX[k] * CY[k]) * Fy;
     }
     if (nv > 0) {
-      const double W = a.wq ? a.wq[o] : 1.0;
+      const double W = a.wq ? nudge_weight(a.wq[o]) : 1.0;
       lT += a.alpha * W * lQ;
       dal = W * (T - a.Tbar[o]) * lQ;
     }

@@ -115,6 +115,10 @@ struct PartJob {
   int record = 0, full = 0;
 };
 
+// Strength of the bilinear deposit's nudging at a node whose observer weights sum to W:
+// min(W, 1), so that alpha <= 1 never over-relaxes a node that several observers reach.
+__host__ __device__ inline double nudge_weight(double W) { return W < 1.0 ? W : 1.0; }
+
 struct AdjState;  // rbn_adjoint.cpp
 void adj_free(rbn_ctx *c);
 

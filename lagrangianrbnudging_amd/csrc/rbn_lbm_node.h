@@ -239,7 +239,8 @@ __device__ __forceinline__ void lbm_node64(const Geom &g, const LbmPtrs &p, cons
     st<NT>(&fB[(size_t)k * g.plane + r0 + i], f[k] * c.one_m_om + c.om * feq + c.src[k] * fac[k]);
   }
   if (VARIANT == RBN_VARIANT_NUDGING && nv > 0) {
-    const double Q = p.wq ? c.alpha * (p.wq[r0 + i] * (T - p.Tbar[r0 + i])) : c.alpha * (T - p.Tbar[r0 + i]);
+    const double Q = p.wq ? c.alpha * (nudge_weight(p.wq[r0 + i]) * (T - p.Tbar[r0 + i]))
+                          : c.alpha * (T - p.Tbar[r0 + i]);
 #pragma unroll
     for (int k = 0; k < 9; k++) {
       const double geq = T * kW[k] * b[k];
@@ -395,7 +396,8 @@ __device__ __forceinline__ void lbm_node32(const Geom &g, const LbmPtrs &p, cons
   }
   float Q = 0.f;
   if (VARIANT == RBN_VARIANT_NUDGING && nv > 0) {
-    Q = p.wq ? c.alpha * ((float)p.wq[r0 + i] * (T - (float)p.Tbar[r0 + i])) : c.alpha * (T - (float)p.Tbar[r0 + i]);
+    Q = p.wq ? c.alpha * ((float)nudge_weight(p.wq[r0 + i]) * (T - (float)p.Tbar[r0 + i]))
+             : c.alpha * (T - (float)p.Tbar[r0 + i]);
     ((float *)p.nud)[r0 + i] = -Q;
   }
 #pragma unroll

@@ -369,9 +369,9 @@ static void or_collide_g(or_sim *s, int nudged) {
       if (nudged && s->nudge_vol[id] > 0) {
         size_t ir = (size_t)(j - 1) + (size_t)s->NY * (size_t)(i - 1);
         double B = 1.0 - 0.5 * om;
-        double Q = s->P.deposit == RBN_DEPOSIT_BILINEAR
-                       ? alpha * (s->nudge_w[ir] * (s->T[id] - s->tempreadint[ir]))
-                       : alpha * (s->T[id] - s->tempreadint[ir]);
+        const double Wn = s->nudge_w[ir] < 1.0 ? s->nudge_w[ir] : 1.0;  /* min(W, 1) */
+        double Q = s->P.deposit == RBN_DEPOSIT_BILINEAR ? alpha * (Wn * (s->T[id] - s->tempreadint[ir]))
+                                                        : alpha * (s->T[id] - s->tempreadint[ir]);
         for (int k = 0; k < NPOP; k++)
           s->g[id].p[k] = s->g[id].p[k] * (1. - om) + om * s->geq[id].p[k] - B * W[k] * Q;
         s->nud_term[id] = -1. * Q;
@@ -588,8 +588,10 @@ int or_deposit_only(or_sim *s, int n, const int *xidx, const int *yidx, const do
  * corners folded onto the interior node they copy (periodic in x and y, as communication).
  * It is the transpose of the interpolation operator.  Per node, in observer order:
  *   W = sum w_k,  S = sum w_k T_k,  tempreadint = S / W,  nudge_vol = number of observers
- * with w_k != 0.  The nudged collision then uses Q = alpha * (W * (T - tempreadint)), i.e.
- * sum_k alpha w_k (T - T_k).  Corners are visited in the order (x1,y1), (x2,y1), (x1,y2),
+ * with w_k != 0.  The nudged collision then uses Q = alpha * (min(W, 1) * (T - tempreadint)):
+ * sum_k alpha w_k (T - T_k) where the observers' weights at the node sum to at most 1, and a
+ * full relaxation toward the weighted mean where several observers reach the node (W > 1),
+ * so that alpha <= 1 never over-relaxes.  Corners are visited in the order (x1,y1), (x2,y1), (x1,y2),
  * (x2,y2); an observer reaches a node at most once, so the per-node order is observer order. */
 static void or_deposit_bilinear(or_sim *s, int n, const double *x, const double *y, const double *Tk) {
   const int NX = s->NX, NY = s->NY;

@@ -132,7 +132,7 @@ class NudgedStep:
         usq = u * u + v * v
         Fn = self.g0 * rho * T
         w = data.get("w")
-        Q = alpha * (T - data["tbar"]) if w is None else alpha * (w * (T - data["tbar"]))
+        Q = alpha * (T - data["tbar"]) if w is None else alpha * (np.minimum(w, 1.0) * (T - data["tbar"]))
         fo, go = np.empty_like(fp), np.empty_like(gp)
         for k in range(9):
             cu = CX[k] * u + CY[k] * v

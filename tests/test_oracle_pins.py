@@ -493,17 +493,19 @@ def test_bilinear_deposit_sums_in_observer_order():
     assert nd["weight"][10, 8] == W_ and nd["tempreadint"][10, 8] == S / W_
 
 
-def test_bilinear_nudging_recurrence_with_quarter_weights():
+@pytest.mark.parametrize("copies", [1, 8])
+def test_bilinear_nudging_recurrence_with_quarter_weights(copies):
     # Observers at the centres of alternate cells give every node exactly one contribution of
-    # weight 1/4, so Q_n = alpha * (1/4) * (T_n - T_obs) everywhere.
+    # weight 1/4 per copy, so W = copies / 4 and Q_n = alpha * min(W, 1) * (T_n - T_obs)
+    # everywhere: 1/4 for one copy, saturated at 1 for eight (W = 2).
     nx, ny, alpha, T_init, T_obs = 24, 12, 0.2, 1.0, 0.5
     p = params(nx, ny, variant=1, ntempread=4, alpha=alpha, relaxg=0.8, Tup=T_init, Tdown=T_init,
                deposit=rbn.RBN_DEPOSIT_BILINEAR)
     o = Oracle(p)
     o.init_fields(T=np.full((nx, ny), T_init))
     gx, gy = np.meshgrid(np.arange(1, nx + 1, 2) + 0.5, np.arange(1, ny + 1, 2) + 0.5, indexing="ij")
-    ids = np.arange(gx.size, dtype=np.int32)
-    xs, ys = gx.ravel(), gy.ravel()
+    ids = np.arange(gx.size * copies, dtype=np.int32)
+    xs, ys = np.repeat(gx.ravel(), copies), np.repeat(gy.ravel(), copies)
     Ts = np.full(ids.size, T_obs)
     o.init_observers(ids, xs, ys, Ts)
     o.set_observation_source({s: {"id": ids, "x": xs, "y": ys, "temp": Ts} for s in range(4, 100, 4)})
@@ -512,9 +514,9 @@ def test_bilinear_nudging_recurrence_with_quarter_weights():
     for n in range(1, 30):
         o.step(1)
         nd = o.nudging()
-        assert np.all(nd["weight"] == 0.25) and np.all(nd["nudge_vol"] == 1)
+        assert np.all(nd["weight"] == 0.25 * copies) and np.all(nd["nudge_vol"] == copies)
         T = S - 0.5 * Qp
-        Q = alpha * (0.25 * (T - T_obs))
+        Q = alpha * (min(0.25 * copies, 1.0) * (T - T_obs))
         assert np.allclose(o.fields()["T"], T, rtol=0, atol=1e-14), n
         assert np.allclose(nd["nud_term"], -Q, rtol=0, atol=1e-15)
         S = (1 - w) * S + w * T - (1 - 0.5 * w) * Q
