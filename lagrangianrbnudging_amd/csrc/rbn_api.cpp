@@ -772,7 +772,6 @@ int rbn_init_particles(rbn_ctx *c, int n, const int *id, const double *x, const 
   free_record_slots(c);
   c->np = n;
   c->pid.clear();
-  c->any_heavy = false;
   if (n == 0) return RBN_OK;
   std::vector<int> order((size_t)n);
   std::iota(order.begin(), order.end(), 0);
@@ -789,7 +788,6 @@ int rbn_init_particles(rbn_ctx *c, int n, const int *id, const double *x, const 
     h[PYO * N + q] = y[p];
     double tp = tau_p ? tau_p[p] : (c->P.heavyprtcl && !c->P.tracer ? c->P.taulag : 0.0);
     h[PINVTAU * N + q] = tp > 0.0 ? (1 / tp) : 0.0;
-    if (tp > 0.0) c->any_heavy = true;
     h[PBETA * N + q] = beta_p ? beta_p[p] : c->P.beta;
     if (up) {
       h[PU * N + q] = up[p];
@@ -968,8 +966,8 @@ int flush_particles(rbn_ctx *c) {
   if (!c->pend.on) return RBN_OK;
   const rbn::PartJob job = pending_job(c);
   int rc = timed(c, KT_PARTICLES, c->stream, [&] {
-    return rbn::launch_particles(c->stream, c->prec, c->geom, job.uv, job.uvp, job.T, job.pp,
-                                 c->any_heavy || job.full, job.record, job.full);
+    return rbn::launch_particles(c->stream, c->prec, c->geom, job.uv, job.uvp, job.T, job.pp, job.record,
+                                 job.full);
   });
   if (rc) {
     c->pend.on = false;
@@ -1080,8 +1078,7 @@ int step_loop(rbn_ctx *c, int nsteps) {
       if (record && (rc = claim_record_slot(c, &slot))) return rc;
       rbn::PartPtrs pp = part_ptrs(c, slot ? slot->d : nullptr);
       if ((rc = timed(c, KT_PARTICLES, ks, [&] {
-             return rbn::launch_particles(ks, c->prec, c->geom, c->uv[c->cur], c->uv[prv], c->T, pp,
-                                          c->any_heavy || full, record,
+             return rbn::launch_particles(ks, c->prec, c->geom, c->uv[c->cur], c->uv[prv], c->T, pp, record,
                                           full);
            })))
         return rc;

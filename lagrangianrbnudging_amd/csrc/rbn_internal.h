@@ -132,7 +132,7 @@ hipError_t launch_lbm_fused(hipStream_t s, int prec, const Geom &g, const LbmPtr
 hipError_t launch_init_eq(hipStream_t s, int prec, const Geom &g, const void *rho, const void *T,
                           const void *uv, void *f, void *gp, void *F, const void *step_const);
 hipError_t launch_particles(hipStream_t s, int prec, const Geom &g, const void *uv, const void *uvp,
-                            const void *T, const PartPtrs &pp, int need_deriv, int record, int full);
+                            const void *T, const PartPtrs &pp, int record, int full);
 // Bilinear samples of a field whose node o holds its value at field[o * cs] (cs = 1 for
 // scalar planes, 2 for the u or v component of an interleaved velocity slot).
 hipError_t launch_sample(hipStream_t s, int prec, const Geom &g, const void *field, int cs, int n,
@@ -188,7 +188,6 @@ struct rbn_ctx {
   int np = 0;
   std::vector<int> pid;  // ids in ascending order
   double *d_part = nullptr;  // 10 arrays of np: x,y,u,v,xo,yo,uo,vo,inv_tau,beta
-  bool any_heavy = false;
   // Records: a ring of device/pinned-host slots.  K2 writes a slot on the compute stream,
   // the copy stream moves it to the host while later steps run, and the callback fires
   // (in step order) when the slot is reused or rbn_step returns.

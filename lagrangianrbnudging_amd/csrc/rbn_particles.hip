@@ -61,8 +61,7 @@ VelField<R> mk_uv(const void *uv, const Geom &g) {
 }  // namespace
 
 hipError_t launch_particles(hipStream_t s, int prec, const Geom &g, const void *uv, const void *uvp,
-                            const void *T, const PartPtrs &pp, int need_deriv, int record, int full) {
-  (void)need_deriv;
+                            const void *T, const PartPtrs &pp, int record, int full) {
   if (pp.n <= 0) return hipSuccess;
   dim3 grid((pp.n + kBlock - 1) / kBlock), block(kBlock);
   if (prec == RBN_FP64)
