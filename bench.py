@@ -7,15 +7,15 @@ inertial particles (Stokes number log-uniform in [0.01, 1], tau_p = St * t_ff wi
 t_ff = (ny-1)/u_ff, zero initial slip), uniform random positions; trajectory records every
 20 steps, written to device memory (the rate with every record also copied to the host over
 PCIe is reported beside it).  One "step" = the fused f/g LBM update + the particle update
-(+ the record on record steps); by default the particle update of a step runs as extra
-workgroups of the next step's lattice launch.  Inputs are resident on the GPU before timing
-starts.
+(+ the record on record steps); by default one lattice launch advances two steps (the first
+step's populations kept in LDS) and carries the previous two steps' particle updates as
+extra workgroups.  Inputs are resident on the GPU before timing starts.
 
 Prints one JSON line: MLUPS of the FP32 (shifted-population) path, the HBM roofline of the
-dominant kernel (the lattice launch carrying the particle update; HIP events around every
-launch in the timed region; the HBM traffic per launch comes from the committed rocprofv3
-counter summary under profiles/), the same step with the particle update as a launch of its
-own (the lattice kernel's roofline alone),
+dominant kernel (the two-step lattice launch; HIP events around launches in the timed
+region; the HBM traffic per launch comes from the committed rocprofv3 counter summary under
+profiles/), the same workload at one step per launch and with the particle update as a
+launch of its own (the one-step lattice kernel's roofline alone),
 the FP64 path's MLUPS on the same workload, the batched ensemble (64 uneven members) against
 the same members stepped one by one, the nudging configuration (1024 x 512: the L2
 temperature error of nudged runs against the truth over 20,000 steps, and a sweep over the
@@ -115,7 +115,7 @@ def run_gpu(rbn, precision, steps, warmup, timing=True, records_to_host=False, *
     s.synchronize()
     if timing:
         s.kernel_timing(True, every=TIMING_EVERY)
-        for k in ("lbm", "lbm_fused", "particles"):
+        for k in ("lbm", "lbm_fused", "lbm2", "particles"):
             s.kernel_stats(k, reset=True)
     t0 = time.perf_counter()
     s.step(steps)
@@ -126,10 +126,12 @@ def run_gpu(rbn, precision, steps, warmup, timing=True, records_to_host=False, *
         # sampled launch counts and summed durations per kernel class
         n_lbm, ms_lbm = s.kernel_stats("lbm")
         n_f, ms_f = s.kernel_stats("lbm_fused")
+        n_2, ms_2 = s.kernel_stats("lbm2")
         n_p, ms_p = s.kernel_stats("particles")
         s.kernel_timing(False)
         out.update(lbm_timed=n_lbm, lbm_avg_ms=ms_lbm / max(n_lbm, 1),
                    fused_timed=n_f, fused_avg_ms=ms_f / max(n_f, 1),
+                   lbm2_timed=n_2, lbm2_avg_ms=ms_2 / max(n_2, 1),
                    particles_timed=n_p, particles_avg_ms=ms_p / max(n_p, 1),
                    record_fraction=record_steps(warmup, steps) / steps)
     fl = s.fields()
@@ -433,15 +435,17 @@ def run_adjoint(rbn, window, ckpt):
             "finite": bool(np.isfinite(g).all() and np.isfinite(ga))}
 
 
-def launch_bytes(precision, record_fraction, particles):
-    """Algorithmic bytes of one lattice launch, averaged over the steps: the lattice bytes, T on
-    record steps, and with particles=True the carried particle update."""
+def launch_bytes(precision, record_fraction, particles, steps=1):
+    """Algorithmic bytes of one lattice launch of `steps` steps, averaged over the steps: the
+    lattice bytes of every step, T on record steps, and with particles=True as many carried
+    particle updates.  A two-step launch keeps the first step's populations in LDS, so it
+    moves fewer bytes through HBM than these (its measured traffic is reported beside)."""
     s = 8 if precision == 0 else 4
     b = NX * NY * (lbm_bytes_per_node(precision) + record_fraction * s)
-    return b + (particle_bytes_per_step(precision) if particles else 0)
+    return steps * (b + (particle_bytes_per_step(precision) if particles else 0))
 
 
-TRAFFIC_PROFILE = ROOT / "profiles" / "k_lbm_fused32_4096x2048_pmc.txt"
+TRAFFIC_PROFILE = ROOT / "profiles" / "k_lbm2_32_4096x2048_pmc.txt"
 
 
 def measured_traffic():
@@ -563,6 +567,8 @@ def main() -> int:
     print(f"bench: fp32 {NX}x{NY} steps={args.steps} warmup={args.warmup}", file=sys.stderr, flush=True)
     r32 = run_gpu(rbn, rbn.RBN_FP32, args.steps, args.warmup)
     print(f"bench: fp32 done {r32}", file=sys.stderr, flush=True)
+    one = run_gpu(rbn, rbn.RBN_FP32, max(args.steps // 2, 1), max(args.warmup // 2, 1), lbm_steps=1)
+    print(f"bench: fp32 one step per launch done {one}", file=sys.stderr, flush=True)
     sep = run_gpu(rbn, rbn.RBN_FP32, max(args.steps // 2, 1), max(args.warmup // 2, 1), particle_fuse=0)
     print(f"bench: fp32 separate particle launches done {sep}", file=sys.stderr, flush=True)
     host = run_gpu(rbn, rbn.RBN_FP32, args.steps, args.warmup, timing=False, records_to_host=True)
@@ -591,8 +597,11 @@ def main() -> int:
         cpu = run_cpu_baseline(rbn, args.cpu_steps, args.cpu_threads or cpu_threads())
         print(f"bench: cpu done {cpu}", file=sys.stderr, flush=True)
 
-    bytes_launch = launch_bytes(1, r32["record_fraction"], True)
-    achieved = bytes_launch / (r32["fused_avg_ms"] * 1e-3) / 1e9
+    bytes_launch = launch_bytes(1, r32["record_fraction"], True, steps=2)
+    achieved = bytes_launch / (r32["lbm2_avg_ms"] * 1e-3) / 1e9
+    traffic = measured_traffic()
+    one_bytes = launch_bytes(1, one["record_fraction"], True)
+    one_achieved = one_bytes / (one["fused_avg_ms"] * 1e-3) / 1e9
     lat_bytes = launch_bytes(1, sep["record_fraction"], False)
     sep_achieved = lat_bytes / (sep["lbm_avg_ms"] * 1e-3) / 1e9
     line = {
@@ -620,11 +629,14 @@ def main() -> int:
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 4),
-            "traffic": measured_traffic(),
-            "kernel": "k_lbm_fused32 (fused f/g collide+stream + the previous step's particle update)",
+            "traffic": traffic,
+            "traffic_frac": traffic and round(traffic / (r32["lbm2_avg_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
+            "kernel": "k_lbm2<float, 8> (two fused f/g collide+stream steps, the first step's populations in "
+                      "LDS, + the two previous steps' particle updates)",
+            "steps_per_launch": 2,
             "bytes_per_launch": round(bytes_launch),
-            "avg_launch_ms": round(r32["fused_avg_ms"], 5),
-            "launches_timed": r32["fused_timed"],
+            "avg_launch_ms": round(r32["lbm2_avg_ms"], 5),
+            "launches_timed": r32["lbm2_timed"],
             "timed_every": TIMING_EVERY,
         },
         "cpu_baseline": cpu and {k: (round(v, 3) if isinstance(v, float) else v) for k, v in cpu.items()},
@@ -632,6 +644,14 @@ def main() -> int:
             "mlups": round(host["mlups"], 1),
             "ms_per_step": round(host["ms_per_step"], 5),
             "note": "PCIe-inclusive: every record also copied to pinned host memory and handed to a callback",
+        },
+        "one_step_per_launch": {
+            "mlups": round(one["mlups"], 1),
+            "ms_per_step": round(one["ms_per_step"], 5),
+            "lattice_kernel": "k_lbm_fused32 (one step + the previous step's particle update)",
+            "avg_launch_ms": round(one["fused_avg_ms"], 5),
+            "bytes_per_launch": round(one_bytes),
+            "roofline_frac": round(one_achieved / HBM_PEAK_GBPS, 4),
         },
         "separate_particle_launches": {
             "mlups": round(sep["mlups"], 1),
@@ -645,8 +665,8 @@ def main() -> int:
         "fp64": r64 and {
             "mlups": round(r64["mlups"], 1),
             "ms_per_step": round(r64["ms_per_step"], 5),
-            "fused_avg_ms": round(r64["fused_avg_ms"], 5),
-            "roofline_frac": round(launch_bytes(0, r64["record_fraction"], True) / (r64["fused_avg_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
+            "lbm2_avg_ms": round(r64["lbm2_avg_ms"], 5),
+            "roofline_frac": round(launch_bytes(0, r64["record_fraction"], True, steps=2) / (r64["lbm2_avg_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
         },
         "ensemble": ens,
         "nudging": nud,

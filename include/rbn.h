@@ -240,7 +240,7 @@ int rbn_synchronize(rbn_ctx *ctx);
 /* ---------------------------------------------------------------- batched ensemble */
 /* Many independent truth-run contexts of uneven sizes and parameters advanced together: one
  * ensemble step is a persistent launch over a work list of every member's particle chunks
- * (the update of the previous step) and lattice tiles, and four steps are captured in one
+ * (the update of the previous step) and lattice tiles, and six steps are captured in one
  * hipGraph that is replayed.  No reference counterpart (the reference runs one lattice per
  * job).  Members must share device and precision, be initialised truth-run contexts, and
  * stay owned by the caller; results equal stepping each member with rbn_step, except that
@@ -249,7 +249,7 @@ typedef struct rbn_ensemble rbn_ensemble;
 int rbn_ensemble_create(rbn_ctx *const *members, int n, rbn_ensemble **out);
 void rbn_ensemble_destroy(rbn_ensemble *e);
 const char *rbn_ensemble_last_error(const rbn_ensemble *e);
-/* Advance every member by nsteps (graph replays for whole four-step cycles; the first and
+/* Advance every member by nsteps (graph replays for whole six-step cycles; the first and
  * last steps of a call, and the last step's particle update, are launched eagerly). */
 int rbn_ensemble_step(rbn_ensemble *e, int nsteps);
 /* Tile shape (tile_w in {64,128,256} nodes, tile_w*tile_h a multiple of 256), persistent
@@ -294,6 +294,12 @@ int rbn_adj_backward(rbn_ctx *ctx, double *dJ_dT0, double *dJ_dalpha);
  *                row in eight, its segments visited in order (default 0)
  *   "lbm_reverse" 1: sweep every odd step last row first, so that it starts on the rows the
  *                previous step wrote last (default 1)
+ *   "lbm_steps"  2: truth runs advance two steps per lattice launch, the first step's
+ *                populations kept in LDS (temporal blocking), for every step that writes no
+ *                moments and is followed by another step of the same rbn_step call; with
+ *                particles only under particle_fuse (default 2); 1: one step per launch
+ *   "lbm2_tile_h" rows of a two-step tile of 64 columns: 4, 8 or 16 (FP32 only), 0 for the
+ *                default (8 in FP32, 4 in FP64)
  *   "particle_overlap"  1: run the particle update of step s on its own stream concurrently
  *                with the lattice update of step s+1 (default 0)
  *   "particle_fuse" 1: run the particle update of step s as extra workgroups of the lattice
@@ -313,8 +319,9 @@ int rbn_get_tuning(const rbn_ctx *ctx, const char *key, int *value);
  * them by a few microseconds; sampling keeps that out of a timed run). */
 int rbn_set_kernel_timing(rbn_ctx *ctx, int every);
 /* Launch count and summed duration (ms) of the timed launches of kernel `name`
- * ("lbm", "particles", "obs", "deposit", "adjoint", or "lbm_fused": a lattice launch
- * carrying the previous step's particle update); resets the counters when reset != 0. */
+ * ("lbm", "particles", "obs", "deposit", "adjoint", "lbm_fused": a one-step lattice launch
+ * carrying earlier steps' particle updates, "lbm2": a two-step lattice launch); resets the
+ * counters when reset != 0. */
 int rbn_kernel_stats(rbn_ctx *ctx, const char *name, int reset, long long *launches,
                      double *total_ms);
 /* Time `nsteps` steps with HIP events on ctx's stream (inputs resident on the GPU). */

@@ -83,7 +83,7 @@ void fill_consts(const rbn_params &P, rbn::StepConst<R> &c) {
   }
 }
 
-const char *const kKernelNames[KT_COUNT] = {"lbm", "particles", "obs", "deposit", "adjoint", "lbm_fused"};
+const char *const kKernelNames[KT_COUNT] = {"lbm", "particles", "obs", "deposit", "adjoint", "lbm_fused", "lbm2"};
 
 int kernel_index(const char *name) {
   for (int k = 0; k < KT_COUNT; k++)
@@ -429,7 +429,7 @@ void rbn_destroy(rbn_ctx *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->pstream) (void)hipStreamSynchronize(c->pstream);
   rbn::adj_free(c);
-  void *ptrs[] = {c->fA, c->fB, c->gA, c->gB, c->F, c->rho, c->T, c->nud};
+  void *ptrs[] = {c->fA, c->fB, c->gA, c->gB, c->F, c->F2, c->rho, c->T, c->nud};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (int k = 0; k < rbn_ctx::kVel; k++)
@@ -516,8 +516,8 @@ int rbn_create(const rbn_params *p, int device, rbn_ctx **out) {
     void **p;
     size_t b;
   } allocs[] = {{&c->fA, 9 * fb}, {&c->fB, 9 * fb}, {&c->gA, 9 * fb}, {&c->gB, 9 * fb},
-                {&c->F, fb},      {&c->uv[0], 2 * fb}, {&c->uv[1], 2 * fb}, {&c->uv[2], 2 * fb},
-                {&c->uv[3], 2 * fb}, {&c->rho, fb},   {&c->T, fb}};
+                {&c->F, fb},      {&c->F2, fb},        {&c->uv[0], 2 * fb}, {&c->uv[1], 2 * fb}, {&c->uv[2], 2 * fb},
+                {&c->uv[3], 2 * fb}, {&c->uv[4], 2 * fb}, {&c->uv[5], 2 * fb}, {&c->rho, fb}, {&c->T, fb}};
   for (auto &a : allocs) {
     if ((rc = dalloc(c, a.p, a.b))) {
       rbn::thread_fail(rc, c->err.c_str());
@@ -941,39 +941,82 @@ int rbn_get_nudging(rbn_ctx *c, int *nudge_vol, double *tempreadint, double *nud
 
 namespace {
 
+// ---------------------------------------------------------------- pending particle updates
+// With particle_fuse the particle update of a step waits for the next lattice launch and
+// runs inside it; after a two-step launch the updates of both steps wait.  None is pending
+// when rbn_step returns.
+
+void add_pending(rbn_ctx *c, int istep, int cur, int prv, bool record, bool full, rbn_ctx::RecSlot *slot) {
+  rbn_ctx::PendingParticles &j = c->pend[c->npend++];
+  j.istep = istep;
+  j.cur = cur;
+  j.prv = prv;
+  j.record = record;
+  j.full = full;
+  j.slot = slot;
+}
+
+bool pending_reads_T(const rbn_ctx *c) {
+  for (int u = 0; u < c->npend; u++)
+    if (c->pend[u].record) return true;
+  return false;
+}
+
 rbn::PartJob pending_job(rbn_ctx *c) {
-  const rbn_ctx::PendingParticles &j = c->pend;
   rbn::PartJob job;
-  job.uv = c->uv[j.cur];
-  job.uvp = c->uv[j.prv];
+  job.n = c->npend;
+  for (int u = 0; u < c->npend; u++) {
+    const rbn_ctx::PendingParticles &j = c->pend[u];
+    job.uv[u] = c->uv[j.cur];
+    job.uvp[u] = c->uv[j.prv];
+    job.rec[u] = j.slot ? j.slot->d : nullptr;
+    job.record[u] = j.record;
+    job.full[u] = j.full;
+  }
   job.T = c->T;
-  job.pp = part_ptrs(c, j.slot ? j.slot->d : nullptr);
-  job.record = j.record;
-  job.full = j.full;
+  job.pp = part_ptrs(c, nullptr);
   return job;
 }
 
-// The pending particle update has been launched (on the lattice stream): queue its record.
+// The pending particle updates have been launched (on the lattice stream): queue their
+// records, in step order.
 int pending_launched(rbn_ctx *c) {
-  rbn_ctx::PendingParticles j = c->pend;
-  c->pend.on = false;
-  if (j.slot && c->rec_fn) return queue_record_copy(c, *j.slot, j.istep, c->stream);
+  const int n = c->npend;
+  c->npend = 0;
+  for (int u = 0; u < n; u++) {
+    const rbn_ctx::PendingParticles &j = c->pend[u];
+    if (j.slot && c->rec_fn) {
+      int rc = queue_record_copy(c, *j.slot, j.istep, c->stream);
+      if (rc) return rc;
+    }
+  }
   return RBN_OK;
 }
 
-// Run the pending particle update as a launch of its own.
+// Run the pending particle updates as launches of their own, in step order.
 int flush_particles(rbn_ctx *c) {
-  if (!c->pend.on) return RBN_OK;
-  const rbn::PartJob job = pending_job(c);
-  int rc = timed(c, KT_PARTICLES, c->stream, [&] {
-    return rbn::launch_particles(c->stream, c->prec, c->geom, job.uv, job.uvp, job.T, job.pp, job.record,
-                                 job.full);
-  });
-  if (rc) {
-    c->pend.on = false;
-    return rc;
+  for (int u = 0; u < c->npend; u++) {
+    const rbn_ctx::PendingParticles &j = c->pend[u];
+    const rbn::PartPtrs pp = part_ptrs(c, j.slot ? j.slot->d : nullptr);
+    int rc = timed(c, KT_PARTICLES, c->stream, [&] {
+      return rbn::launch_particles(c->stream, c->prec, c->geom, c->uv[j.cur], c->uv[j.prv], c->T, pp, j.record,
+                                   j.full);
+    });
+    if (rc) {
+      c->npend = 0;
+      return rc;
+    }
   }
   return pending_launched(c);
+}
+
+int report_energy(rbn_ctx *c, int istep) {
+  double r[2];
+  int rc = run_energy(c, r);
+  if (rc) return rc;
+  if (c->diag_fn(c->diag_user, istep, r[0], r[1]) != 0)
+    return fail(c, RBN_ECALLBACK, "diagnostics callback failed at step %d", istep);
+  return RBN_OK;
 }
 
 int step_loop(rbn_ctx *c, int nsteps) {
@@ -983,12 +1026,36 @@ int step_loop(rbn_ctx *c, int nsteps) {
   const bool parts = !nudged && P.particle && c->np > 0;
   const int nd = P.ndiag > 0 ? P.ndiag : (nudged ? 5000 : P.nout);
   const int right = P.fix_q3 ? P.nx - 3 : P.nx - 2;
+  // Particle schedules.  particle_fuse (default): K2 of a step rides in the grid of the next
+  // lattice launch, unless it is a record step's (it reads T) and that launch rewrites the
+  // moments.  Otherwise K2 runs on the lattice stream after K1, ordered without events (a
+  // cross-stream event wait costs about 20 us per hand-over), or with particle_overlap on
+  // its own stream beside the next K1 (both kernels are bandwidth bound, so concurrency only
+  // moves the contention); with overlap, K1 of step s waits for the particle update of step
+  // s-2 (the latest event of the same parity).
+  const bool overlap = c->tune.overlap != 0;
+  const bool fuse = parts && !overlap && c->tune.fuse != 0;
+  // lbm_steps 2: two truth-run steps per lattice launch, the first of which writes no
+  // moments; with particles only when their updates ride in lattice launches.
+  const bool pairs = c->tune.steps2 && !nudged && (!parts || fuse);
+  hipStream_t ks = overlap ? c->pstream : c->stream;
   if (nsteps > 0) c->at_init = false;
   for (int n = 0; n < nsteps; n++) {
     const int istep = ++c->istep;
     const bool record = parts && (istep % P.noutconfig) == 0;
     const bool diag = nd > 0 && (istep % nd) == 0;
     const bool moments = record || diag || n == nsteps - 1;
+    const bool pair = pairs && !moments && n + 1 < nsteps;
+    int istep2 = 0;
+    bool record2 = false, diag2 = false, moments2 = false;
+    if (pair) {
+      istep2 = ++c->istep;
+      n++;
+      record2 = parts && (istep2 % P.noutconfig) == 0;
+      diag2 = nd > 0 && (istep2 % nd) == 0;
+      moments2 = record2 || diag2 || n == nsteps - 1;
+    }
+    const bool writes_moments = pair ? moments2 : moments;
     if (nudged) {
       const int r = istep % P.ntempread;
       if (r == 1 && (rc = load_observation(c, istep + P.ntempread - 1))) return rc;
@@ -999,23 +1066,11 @@ int step_loop(rbn_ctx *c, int nsteps) {
       if ((rc = timed(c, KT_DEPOSIT, c->stream, [&] { return rbn::launch_deposit(c->stream, c->geom, right, o, c->dep); })))
         return rc;
     }
-    // K1 of step s overwrites the velocity slot of step s-4, which the particle update of
-    // step s-3 reads; on record steps the particle update also reads T, which K1 overwrites
-    // when it writes moments.
-    const int nxt = (c->cur + 1) % rbn_ctx::kVel;
-    // particle_fuse (default): K2 of step s rides in the grid of K1 of step s+1, unless step
-    // s is a record step and step s+1 rewrites the moments.  Otherwise K2 runs on the lattice
-    // stream after K1, ordered without events (a cross-stream event wait costs about 20 us
-    // per hand-over), or with particle_overlap on its own stream beside the next K1 (both
-    // kernels are bandwidth bound, so concurrency only moves the contention).  With overlap,
-    // K1 of step s waits for the particle update of step s-2 (the latest event of the same
-    // parity, which covers step s-3).
-    const bool overlap = c->tune.overlap != 0;
-    const bool fuse = parts && !overlap && c->tune.fuse != 0;
-    hipStream_t ks = overlap ? c->pstream : c->stream;
+    // A launch writes the velocity slots after cur: K1 of step s overwrites the slot of step
+    // s-6, a two-step launch also that of s-5; pending updates read slots s-3 .. s-1.
     bool carry = false;
-    if (c->pend.on) {
-      if (c->pend.record && moments) {
+    if (c->npend > 0) {
+      if (pending_reads_T(c) && writes_moments) {
         if ((rc = flush_particles(c))) return rc;
       } else {
         carry = true;
@@ -1026,47 +1081,54 @@ int step_loop(rbn_ctx *c, int nsteps) {
       if (moments && c->part_record_pending)
         HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_part[(istep + 1) % 2], 0));
     }
+    const int s1 = (c->cur + 1) % rbn_ctx::kVel, s2 = (c->cur + 2) % rbn_ctx::kVel;
     rbn::LbmPtrs lp{};
     lp.fA = c->fA;
     lp.gA = c->gA;
     lp.fB = c->fB;
     lp.gB = c->gB;
     lp.F = c->F;
-    lp.uv = c->uv[nxt];
-    lp.rho = moments ? c->rho : nullptr;
-    lp.T = moments ? c->T : nullptr;
+    lp.uv = c->uv[s1];
+    lp.rho = !pair && moments ? c->rho : nullptr;
+    lp.T = !pair && moments ? c->T : nullptr;
     if (nudged) {
       lp.nvol = c->dep.cnt;
       lp.Tbar = c->dep.Tbar;
       lp.wq = c->dep.mode == RBN_DEPOSIT_BILINEAR ? c->dep.wq : nullptr;
       lp.nud = c->nud;
     }
-    if ((rc = timed(c, carry ? KT_LBM_FUSED : KT_LBM, c->stream, [&] {
+    const int kclass = pair ? KT_LBM2 : (carry ? KT_LBM_FUSED : KT_LBM);
+    if ((rc = timed(c, kclass, c->stream, [&] {
            rbn::LbmTuning t = c->tune;
+           if (pair) {
+             const rbn::PartJob job = carry ? pending_job(c) : rbn::PartJob{};
+             return rbn::launch_lbm2(c->stream, c->prec, c->geom, lp, c->F2, c->uv[s2], moments2 ? c->rho : nullptr,
+                                     moments2 ? c->T : nullptr, c->consts.data(), t.tile_h,
+                                     t.reverse && ((istep >> 1) & 1), carry ? &job : nullptr);
+           }
            t.flip = t.reverse && (istep & 1);
            if (carry)
              return rbn::launch_lbm_fused(c->stream, c->prec, c->geom, lp, c->consts.data(), t, pending_job(c));
            return rbn::launch_lbm(c->stream, c->prec, P.variant, c->geom, lp, c->consts.data(), t);
          }))) {
-      c->pend.on = false;
+      c->npend = 0;
       return rc;
     }
     if (carry && (rc = pending_launched(c))) return rc;
     std::swap(c->fA, c->fB);
     std::swap(c->gA, c->gB);
+    if (pair) std::swap(c->F, c->F2);
     const int prv = c->cur;
-    c->cur = nxt;
-    c->moments_valid = moments;
+    c->cur = pair ? s2 : s1;
+    c->moments_valid = writes_moments;
     if (fuse) {
+      // this launch's particle updates wait for the next launch
+      const int rstep = pair ? istep2 : istep;
+      const bool rec = pair ? record2 : record;
       rbn_ctx::RecSlot *slot = nullptr;
-      if (record && (rc = claim_record_slot(c, &slot))) return rc;
-      c->pend.on = true;
-      c->pend.istep = istep;
-      c->pend.cur = c->cur;
-      c->pend.prv = prv;
-      c->pend.record = record;
-      c->pend.full = record && P.prtcl_writefull;
-      c->pend.slot = slot;
+      if (rec && (rc = claim_record_slot(c, &slot))) return rc;
+      if (pair) add_pending(c, istep, s1, prv, false, false, nullptr);
+      add_pending(c, rstep, c->cur, pair ? s1 : prv, rec, rec && P.prtcl_writefull, slot);
     } else if (parts) {
       // K2 of step s; with overlap, on the particle stream alongside K1 of step s+1.
       if (overlap) {
@@ -1086,12 +1148,8 @@ int step_loop(rbn_ctx *c, int nsteps) {
       c->part_record_pending = overlap && record;
       if (slot && c->rec_fn && (rc = queue_record_copy(c, *slot, istep, ks))) return rc;
     }
-    if (diag && c->diag_fn) {
-      double r[2];
-      if ((rc = run_energy(c, r))) return rc;
-      if (c->diag_fn(c->diag_user, istep, r[0], r[1]) != 0)
-        return fail(c, RBN_ECALLBACK, "diagnostics callback failed at step %d", istep);
-    }
+    const bool dstep = pair ? diag2 : diag;
+    if (dstep && c->diag_fn && (rc = report_energy(c, pair ? istep2 : istep))) return rc;
   }
   return RBN_OK;
 }
@@ -1157,6 +1215,13 @@ int rbn_set_tuning(rbn_ctx *c, const char *key, int value) {
     c->tune.xcd = value;
   } else if (!strcmp(key, "lbm_reverse")) {
     c->tune.reverse = value != 0;
+  } else if (!strcmp(key, "lbm_steps")) {
+    if (value != 1 && value != 2) return fail(c, RBN_EINVAL, "lbm_steps must be 1 or 2");
+    c->tune.steps2 = value == 2;
+  } else if (!strcmp(key, "lbm2_tile_h")) {
+    if (value != 0 && value != 4 && value != 8 && value != 16)
+      return fail(c, RBN_EINVAL, "lbm2_tile_h must be 0, 4, 8 or 16");
+    c->tune.tile_h = value;
   } else if (!strcmp(key, "particle_overlap")) {
     c->tune.overlap = value != 0;
   } else if (!strcmp(key, "particle_fuse")) {
@@ -1178,6 +1243,8 @@ int rbn_get_tuning(const rbn_ctx *c, const char *key, int *value) {
   else if (!strcmp(key, "lbm_uv_keep")) *value = c->tune.uv_keep;
   else if (!strcmp(key, "lbm_xcd")) *value = c->tune.xcd;
   else if (!strcmp(key, "lbm_reverse")) *value = c->tune.reverse;
+  else if (!strcmp(key, "lbm_steps")) *value = c->tune.steps2 ? 2 : 1;
+  else if (!strcmp(key, "lbm2_tile_h")) *value = c->tune.tile_h;
   else if (!strcmp(key, "particle_overlap")) *value = c->tune.overlap;
   else if (!strcmp(key, "particle_fuse")) *value = c->tune.fuse;
   else if (!strcmp(key, "adj_store_deposit")) *value = c->adj_store_deposit;

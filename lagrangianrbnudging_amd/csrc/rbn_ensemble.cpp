@@ -1,5 +1,5 @@
 // rbn_ensemble.cpp — host side of the batched ensemble (rbn_ensemble_* in include/rbn.h):
-// member validation, work lists, the four-step hipGraph, the autotuner.
+// member validation, work lists, the captured cycle of kCycle steps, the autotuner.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -66,7 +66,7 @@ void fill_member(const rbn_ctx *c, int tile_w, int tile_h, rbn::EnsMember<R> &d)
   d.pg[0] = (R *)c->gA;
   d.pg[1] = (R *)c->gB;
   d.F = (R *)c->F;
-  for (int k = 0; k < 4; k++) d.uv[k] = (R *)c->uv[k];
+  for (int k = 0; k < rbn::kCycle; k++) d.uv[k] = (R *)c->uv[k];
   d.rho = (R *)c->rho;
   d.T = (R *)c->T;
   d.cur0 = c->cur;
@@ -144,7 +144,7 @@ int launch(rbn_ensemble *e, hipStream_t s, int phase, int k, bool moments) {
   return RBN_OK;
 }
 
-// Separate phases: enqueue step k (0..3) of a cycle on the two streams.  cross_waits: the
+// Separate phases: enqueue step k (0 .. kCycle-1) of a cycle on the two streams.  cross_waits: the
 // lattice launch waits for the particle launch of step k-2 (it overwrites the velocity that
 // one reads); inside a captured cycle that dependency exists only for k >= 2, across cycles
 // the graph launches are ordered by the stream.
@@ -164,19 +164,19 @@ int enqueue_step(rbn_ensemble *e, int k, bool last, bool cross_waits) {
 // One launch per step: step t (counted from the descriptors' state) carries the particle
 // update of step t-1; step 0 has none to carry.
 int enqueue_fused(rbn_ensemble *e, int t, bool last) {
-  return launch(e, e->stream, t > 0 && e->n_part > 0 ? 2 : 0, t & 3, last);
+  return launch(e, e->stream, t > 0 && e->n_part > 0 ? 2 : 0, t % rbn::kCycle, last);
 }
 
-// Captured cycle: separate phases, steps 0..3 (moments on the last); one launch per step,
-// steps 1..4 (no moments: the last step of a call runs eagerly and writes them).
+// Captured cycle: separate phases, steps 0 .. kCycle-1 (moments on the last); one launch per
+// step, steps 1 .. kCycle (no moments: the last step of a call runs eagerly and writes them).
 int capture(rbn_ensemble *e) {
   drop_graph(e);
   EHIP(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
   int rc = RBN_OK;
   if (separate(e)) {
-    for (int k = 0; k < 4 && !rc; k++) rc = enqueue_step(e, k, k == 3, k >= 2);
+    for (int k = 0; k < rbn::kCycle && !rc; k++) rc = enqueue_step(e, k, k == rbn::kCycle - 1, k >= 2);
   } else {
-    for (int t = 1; t <= 4 && !rc; t++) rc = enqueue_fused(e, t, false);
+    for (int t = 1; t <= rbn::kCycle && !rc; t++) rc = enqueue_fused(e, t, false);
   }
   if (!rc && separate(e) && e->n_part > 0) {
     // join the particle stream back into the origin stream
@@ -203,7 +203,7 @@ void advance_members(rbn_ensemble *e, int n) {
       std::swap(c->fA, c->fB);
       std::swap(c->gA, c->gB);
     }
-    c->cur = (c->cur + n) & 3;
+    c->cur = (c->cur + n) % rbn_ctx::kVel;
     c->moments_valid = true;
   }
 }
@@ -226,7 +226,7 @@ int run_steps(rbn_ensemble *e, int nsteps) {
   // (an ordinary stream operation, never part of the captured graph).
   EHIP(e, hipMemsetAsync(e->d_counters, 0, 4 * sizeof(int), e->stream));
   if (separate(e)) {
-    const int cycles = nsteps / 4, rem = nsteps % 4;
+    const int cycles = nsteps / rbn::kCycle, rem = nsteps % rbn::kCycle;
     if (cycles > 0) {
       if (!e->exec && (rc = capture(e))) return rc;
       for (int c = 0; c < cycles; c++) EHIP(e, hipGraphLaunch(e->exec, e->stream));
@@ -235,23 +235,23 @@ int run_steps(rbn_ensemble *e, int nsteps) {
       if ((rc = enqueue_step(e, k, k == rem - 1, k >= 2))) return rc;
     if (rem > 0 && e->n_part > 0) EHIP(e, hipStreamWaitEvent(e->stream, e->ev_part[(rem - 1) & 1], 0));
   } else {
-    // step 0 eagerly, whole cycles of steps 1..4c from the graph, the rest (at least one
+    // step 0 eagerly, whole cycles of steps 1..kCycle*c from the graph, the rest (at least one
     // step when nsteps >= 2, the last writing the moments) eagerly, then the particle
     // update of the last step on its own.
     if ((rc = enqueue_fused(e, 0, nsteps == 1))) return rc;
-    const int cycles = nsteps >= 2 ? (nsteps - 2) / 4 : 0;
+    const int cycles = nsteps >= 2 ? (nsteps - 2) / rbn::kCycle : 0;
     if (cycles > 0) {
       if (!e->exec && (rc = capture(e))) return rc;
       for (int c = 0; c < cycles; c++) EHIP(e, hipGraphLaunch(e->exec, e->stream));
     }
-    for (int t = 1 + 4 * cycles; t < nsteps; t++)
+    for (int t = 1 + rbn::kCycle * cycles; t < nsteps; t++)
       if ((rc = enqueue_fused(e, t, t == nsteps - 1))) return rc;
-    if (e->n_part > 0 && (rc = launch(e, e->stream, 1, (nsteps - 1) & 3, false))) return rc;
+    if (e->n_part > 0 && (rc = launch(e, e->stream, 1, (nsteps - 1) % rbn::kCycle, false))) return rc;
   }
   EHIP(e, hipStreamSynchronize(e->stream));
   EHIP(e, hipStreamSynchronize(e->pstream));
   advance_members(e, nsteps);
-  if (nsteps % 4) {  // member parity changed: descriptors and graph must be rebuilt
+  if (nsteps % rbn::kCycle) {  // member parity or ring position changed: rebuild descriptors and graph
     e->built = false;
     drop_graph(e);
   }
@@ -270,7 +270,7 @@ int take_snapshot(rbn_ensemble *e, Snapshot &s) {
   for (rbn_ctx *c : e->m) {
     const size_t fb = c->geom.plane * c->rs;
     std::vector<std::pair<void *, size_t>> parts = {{c->fA, 9 * fb}, {c->gA, 9 * fb}, {c->F, fb}};
-    for (int k = 0; k < 4; k++) parts.push_back({c->uv[k], 2 * fb});
+    for (int k = 0; k < rbn_ctx::kVel; k++) parts.push_back({c->uv[k], 2 * fb});
     if (c->d_part && c->np > 0) parts.push_back({c->d_part, 10 * (size_t)c->np * sizeof(double)});
     for (auto &p : parts) {
       void *b = nullptr;
@@ -402,7 +402,7 @@ int rbn_ensemble_step(rbn_ensemble *e, int nsteps) {
 int rbn_ensemble_autotune(rbn_ensemble *e, int trial_steps, double *best_ms_per_step) {
   if (!e) return rbn::thread_fail(RBN_EINVAL, "null ensemble");
   if (hipSetDevice(e->dev) != hipSuccess) return efail(e, RBN_EDEVICE, "hipSetDevice failed");
-  const int trial = std::max(4, (trial_steps + 3) / 4 * 4);
+  const int trial = std::max(rbn::kCycle, (trial_steps + rbn::kCycle - 1) / rbn::kCycle * rbn::kCycle);
   struct Cand {
     int tw, th, wpc, order;
   };
@@ -423,13 +423,13 @@ int rbn_ensemble_autotune(rbn_ensemble *e, int trial_steps, double *best_ms_per_
   Cand bestc = cands[0];
   for (const Cand &c : cands) {
     if ((rc = rbn_ensemble_set_tuning(e, c.tw, c.th, c.wpc, c.order))) break;
-    if ((rc = run_steps(e, 4))) break;  // build, warm
+    if ((rc = run_steps(e, rbn::kCycle))) break;  // build, warm (a whole cycle: descriptors stay valid)
     if (!e->exec && (rc = capture(e))) break;
     // The captured cycle of one launch per step starts at step 1 (counter set 1), which the
     // eager steps above leave used: clear the counters before replaying.
     EHIP(e, hipMemsetAsync(e->d_counters, 0, 4 * sizeof(int), e->stream));
     EHIP(e, hipEventRecord(a, e->stream));
-    for (int s = 0; s < trial / 4; s++) EHIP(e, hipGraphLaunch(e->exec, e->stream));
+    for (int s = 0; s < trial / rbn::kCycle; s++) EHIP(e, hipGraphLaunch(e->exec, e->stream));
     EHIP(e, hipEventRecord(b, e->stream));
     EHIP(e, hipEventSynchronize(b));
     EHIP(e, hipStreamSynchronize(e->pstream));

@@ -5,15 +5,20 @@
 
 namespace rbn {
 
-// Pointers and constants of one member as of the start of a four-step cycle: pf[0]/pg[0] are
-// the f/g planes read by the first step, uv[cur0] the latest velocity ((u, v) interleaved).
+// Steps of a replayed cycle: after kCycle steps the A/B population planes and the velocity
+// ring of every member are back where they started.
+constexpr int kCycle = rbn_ctx::kVel;
+static_assert(kCycle % 2 == 0, "a cycle must return the A/B planes");
+
+// Pointers and constants of one member as of the start of a cycle: pf[0]/pg[0] are the f/g
+// planes read by the first step, uv[cur0] the latest velocity ((u, v) interleaved).
 template <typename R>
 struct EnsMember {
   Geom g;
   StepConst<R> c;
   R *pf[2], *pg[2];
   R *F;
-  R *uv[4];
+  R *uv[kCycle];
   R *rho, *T;
   int cur0;
   int tiles_x, tiles_y;

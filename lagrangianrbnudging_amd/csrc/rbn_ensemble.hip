@@ -16,9 +16,9 @@
 // particle launch on a second stream beside the next lattice launch.
 // A fixed pool of workgroups (wg_per_cu per CU, chosen by the autotuner; 0 = one workgroup
 // per item) pulls items from an atomic counter, so small and large members share the GPU
-// instead of each small lattice leaving most CUs idle.  The host captures four consecutive
-// steps (after four steps the A/B population planes and the four-deep velocity ring of
-// every member are back where they started) into one hipGraph and replays it.
+// instead of each small lattice leaving most CUs idle.  The host captures kCycle = 6
+// consecutive steps (after six steps the A/B population planes and the six-deep velocity
+// ring of every member are back where they started) into one hipGraph and replays it.
 //
 // Work counters: a launch at step k uses counter set k % 2 (lattice or fused launches one
 // counter, particle launches the other) and clears its counter of set (k+1) % 2 for the
@@ -44,7 +44,7 @@ __device__ __forceinline__ void lbm_item(const EnsMember<R> &m, int tile, int k,
   p.gA = par ? m.pg[1] : m.pg[0];
   p.gB = par ? m.pg[0] : m.pg[1];
   p.F = m.F;
-  const int out = (m.cur0 + 1 + k) & 3;
+  const int out = (m.cur0 + 1 + k) % kCycle;
   p.uv = m.uv[out];
   p.rho = write_moments ? m.rho : nullptr;
   p.T = write_moments ? m.T : nullptr;
@@ -64,7 +64,8 @@ template <typename R>
 __device__ __forceinline__ void particle_item(const EnsMember<R> &m, int chunk, int k) {
   const int p = chunk * kThreads + threadIdx.x;
   if (p >= m.np) return;
-  const int cur = (m.cur0 + 1 + k) & 3, prv = (m.cur0 + k) & 3;
+  // k is -1 for the previous cycle's last step (carried by step 0)
+  const int cur = (m.cur0 + 1 + k + kCycle) % kCycle, prv = (m.cur0 + k + kCycle) % kCycle;
   const int nx = m.g.nx, ny = m.g.ny, pitch = m.g.pitch;
   const part::VelField<R> UV = part::vel_of<R>(m.uv[cur], nx, ny, pitch);
   const part::VelField<R> UVp = part::vel_of<R>(m.uv[prv], nx, ny, pitch);

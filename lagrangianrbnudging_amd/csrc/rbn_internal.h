@@ -105,14 +105,21 @@ struct LbmTuning {
   int flip = 0;     // this launch sweeps last row first (set per launch from reverse)
   int overlap = 0;  // particle update of step s concurrent with the lattice step s+1
   int fuse = 1;     // particle update of step s in the grid of the lattice step s+1
+  int steps2 = 1;   // truth run: two steps per lattice launch (temporal blocking in LDS)
+  int tile_h = 0;   // rows of a two-step tile (4, 8, 16; 0: 8 in FP32, 4 in FP64)
 };
 
-// The particle update of one step, carried by the next lattice launch (launch_lbm_fused):
-// uv / uvp are the velocity slots of that step and the one before, T its temperature.
+// The particle updates of one or two consecutive steps, in step order, carried by the next
+// lattice launch: uv[u] / uvp[u] are the velocity slots of update u's step and the one
+// before, rec[u] its record slot (or null), T the temperature of the one record step among
+// them (a record update is never carried past a launch that rewrites the moments).
 struct PartJob {
-  const void *uv = nullptr, *uvp = nullptr, *T = nullptr;
+  int n = 0;
+  const void *uv[2] = {}, *uvp[2] = {};
+  double *rec[2] = {};
+  int record[2] = {}, full[2] = {};
+  const void *T = nullptr;
   PartPtrs pp{};
-  int record = 0, full = 0;
 };
 
 // Strength of the bilinear deposit's nudging at a node whose observer weights sum to W:
@@ -125,6 +132,13 @@ void adj_free(rbn_ctx *c);
 // Launchers (defined in the .hip files).
 hipError_t launch_lbm(hipStream_t s, int prec, int variant, const Geom &g, const LbmPtrs &p,
                       const void *step_const, const LbmTuning &t);
+// Two truth-run steps of K1 in one launch (rbn_lbm2.hip): p holds the first step's inputs and
+// the second step's outputs (fB, gB), p.F = F(s), p.uv = the first step's velocity slot;
+// Fout receives F(s+2), uv2 the second step's velocity, rho2 / T2 (or null) its moments.
+// job (or null): particle updates carried as extra workgroups ahead of the tiles.
+hipError_t launch_lbm2(hipStream_t s, int prec, const Geom &g, const LbmPtrs &p, void *Fout, void *uv2,
+                       void *rho2, void *T2, const void *step_const, int tile_h, int flip,
+                       const PartJob *job);
 // K1 of a truth-run step with the particle update of the previous step in the same grid
 // (its own workgroups, ahead of the lattice ones).
 hipError_t launch_lbm_fused(hipStream_t s, int prec, const Geom &g, const LbmPtrs &p,
@@ -166,12 +180,13 @@ struct rbn_ctx {
   // lattice state
   void *fA = nullptr, *fB = nullptr, *gA = nullptr, *gB = nullptr;
   void *F = nullptr;
-  // Velocity ring: uv[cur] is the latest hydrovar velocity, uv[(cur+3)%4] the step before.
+  void *F2 = nullptr;  // second force plane: a two-step launch reads F and writes F2, then swaps
+  // Velocity ring: uv[cur] is the latest hydrovar velocity, uv[(cur+5)%6] the step before.
   // Each slot holds (u, v) interleaved per node (2 * plane reals), so that the particle
-  // update fetches one cache line per patch row for both components.  Four slots let the
-  // particle update of step s (which reads steps s and s-1) run on its own stream while the
-  // lattice steps s+1 and s+2 write the other two.
-  static constexpr int kVel = 4;
+  // update fetches one cache line per patch row for both components.  Six slots let a
+  // two-step lattice launch (steps s, s+1) carry the particle updates of steps s-2 and s-1,
+  // which read the slots of steps s-3 .. s-1.
+  static constexpr int kVel = 6;
   void *uv[kVel] = {};
   int cur = 0;
   void *rho = nullptr, *T = nullptr;
@@ -199,13 +214,14 @@ struct rbn_ctx {
     int step = -1;        // pending record step, -1 when free
   } rec[kRecSlots];
   int rec_next = 0;
-  // The particle update of the latest step when it waits to ride in the next lattice
-  // launch (particle_fuse); never pending when rbn_step returns.
+  // The particle updates waiting to ride in the next lattice launch (particle_fuse): the
+  // latest step's, or the latest two steps' after a two-step launch, in step order; never
+  // pending when rbn_step returns.
   struct PendingParticles {
-    bool on = false;
     int istep = 0, cur = 0, prv = 0, record = 0, full = 0;
     RecSlot *slot = nullptr;
-  } pend;
+  } pend[2];
+  int npend = 0;
   hipStream_t copy_stream = nullptr;
   int *d_err = nullptr;
   rbn_record_fn rec_fn = nullptr;
@@ -237,7 +253,7 @@ struct rbn_ctx {
   double *d_energy = nullptr;  // partials + 2 results
 
   // timing: every timing_every-th launch of each kernel class is bracketed with events
-  static constexpr int kKernelClasses = 6;
+  static constexpr int kKernelClasses = 7;
   bool timing = false;
   int timing_every = 1;
   long long k_seen[kKernelClasses] = {};

@@ -6,8 +6,10 @@
 // particle update can ride in the launch of the next lattice step: its workgroups come
 // first in the grid, and their gathers (latency bound, a few cache lines per particle)
 // overlap the streaming of the lattice workgroups instead of running as a launch of their
-// own between two lattice launches.  The caller keeps the one conflict out: a record step's
-// update reads T, so it is not carried by a step that rewrites the moments.
+// own between two lattice launches.  After a two-step launch two updates are pending; a
+// thread runs both for its particle, in step order.  The caller keeps the one conflict out:
+// a record step's update reads T, so it is not carried by a launch that rewrites the
+// moments.
 #include "rbn_lbm_node.h"
 #include "rbn_particle_node.h"
 
@@ -18,20 +20,11 @@ using namespace lbm;
 
 constexpr int kMaxBlock = 512;
 
-template <typename R>
-struct PartArgs {
-  part::VelField<R> uv, uvp;
-  part::Field<R> T;
-  PartPtrs pp;
-  int record, full, nblocks;
-};
-
 template <int NT>
 __global__ __launch_bounds__(kMaxBlock) void k_lbm_fused64(Geom g, LbmPtrs p, StepConst<double> c, int order,
-                                                           int npt, PartArgs<double> a) {
+                                                           int npt, part::PartSteps<double> a) {
   if ((int)blockIdx.x < a.nblocks) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < a.pp.n) part::particle_update(a.uv, a.uvp, a.T, a.pp, q, a.record, a.full);
+    part::run_part_steps(a, blockIdx.x * blockDim.x + threadIdx.x);
     return;
   }
   int i0, j;
@@ -44,10 +37,9 @@ __global__ __launch_bounds__(kMaxBlock) void k_lbm_fused64(Geom g, LbmPtrs p, St
 
 template <int NT>
 __global__ __launch_bounds__(kMaxBlock) void k_lbm_fused32(Geom g, LbmPtrs p, StepConst<float> c, int order,
-                                                           int npt, PartArgs<float> a) {
+                                                           int npt, part::PartSteps<float> a) {
   if ((int)blockIdx.x < a.nblocks) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q < a.pp.n) part::particle_update(a.uv, a.uvp, a.T, a.pp, q, a.record, a.full);
+    part::run_part_steps(a, blockIdx.x * blockDim.x + threadIdx.x);
     return;
   }
   int i0, j;
@@ -58,28 +50,15 @@ __global__ __launch_bounds__(kMaxBlock) void k_lbm_fused32(Geom g, LbmPtrs p, St
   }
 }
 
-template <typename R>
-PartArgs<R> part_args(const Geom &g, const PartJob &job, int bw) {
-  PartArgs<R> a;
-  a.uv = part::vel_of<R>(job.uv, g.nx, g.ny, g.pitch);
-  a.uvp = part::vel_of<R>(job.uvp, g.nx, g.ny, g.pitch);
-  a.T = part::scalar_of<R>(job.T, g.nx, g.ny, g.pitch);
-  a.pp = job.pp;
-  a.record = job.record;
-  a.full = job.full;
-  a.nblocks = (job.pp.n + bw - 1) / bw;
-  return a;
-}
-
 template <int NT>
 void fused_dispatch(hipStream_t s, int prec, unsigned lattice_blocks, int bw, const Geom &g,
                     const LbmPtrs &p, const void *step_const, int order, int npt, const PartJob &job) {
   if (prec == RBN_FP64) {
-    const PartArgs<double> a = part_args<double>(g, job, bw);
+    const part::PartSteps<double> a = part::part_steps<double>(g, &job, bw);
     hipLaunchKernelGGL((k_lbm_fused64<NT>), dim3(lattice_blocks + (unsigned)a.nblocks), dim3(bw), 0, s, g, p,
                        *(const StepConst<double> *)step_const, order, npt, a);
   } else {
-    const PartArgs<float> a = part_args<float>(g, job, bw);
+    const part::PartSteps<float> a = part::part_steps<float>(g, &job, bw);
     hipLaunchKernelGGL((k_lbm_fused32<NT>), dim3(lattice_blocks + (unsigned)a.nblocks), dim3(bw), 0, s, g, p,
                        *(const StepConst<float> *)step_const, order, npt, a);
   }

@@ -314,5 +314,45 @@ __device__ __forceinline__ void particle_update(const VelField<R> &UV, const Vel
   if (err) atomicOr(pp.err, 1);
 }
 
+// The particle updates a lattice launch carries (PartJob), as kernel arguments: nblocks
+// workgroups of the launch, one thread per particle, run them in step order.
+template <typename R>
+struct PartSteps {
+  VelField<R> uv[2], uvp[2];
+  Field<R> T;
+  PartPtrs pp;
+  double *rec[2];
+  int record[2], full[2];
+  int n, nblocks;
+};
+
+template <typename R>
+inline PartSteps<R> part_steps(const Geom &g, const PartJob *job, int bw) {
+  PartSteps<R> a{};
+  if (!job || job->n <= 0 || job->pp.n <= 0) return a;
+  for (int u = 0; u < job->n; u++) {
+    a.uv[u] = vel_of<R>(job->uv[u], g.nx, g.ny, g.pitch);
+    a.uvp[u] = vel_of<R>(job->uvp[u], g.nx, g.ny, g.pitch);
+    a.rec[u] = job->rec[u];
+    a.record[u] = job->record[u];
+    a.full[u] = job->full[u];
+  }
+  a.T = scalar_of<R>(job->T, g.nx, g.ny, g.pitch);
+  a.pp = job->pp;
+  a.n = job->n;
+  a.nblocks = (job->pp.n + bw - 1) / bw;
+  return a;
+}
+
+template <typename R>
+__device__ __forceinline__ void run_part_steps(const PartSteps<R> &a, int q) {
+  if (q >= a.pp.n) return;
+  PartPtrs pp = a.pp;
+  for (int u = 0; u < a.n; u++) {
+    pp.rec = a.rec[u];
+    particle_update(a.uv[u], a.uvp[u], a.T, pp, q, a.record[u], a.full[u]);
+  }
+}
+
 }  // namespace part
 }  // namespace rbn

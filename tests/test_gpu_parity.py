@@ -55,13 +55,13 @@ def _compare_records(g, o, full):
             assert_equal_arrays(x[k], y[k], f"record {x['step']} {k}")
 
 
-@pytest.mark.parametrize("fuse", [1, 0])
+@pytest.mark.parametrize("fuse,steps", [(1, 1), (0, 1), (1, 2)])
 @pytest.mark.parametrize("writefull", [0, 1])
-def test_truth_run_fp64_bit_exact_tiny(writefull, fuse):
+def test_truth_run_fp64_bit_exact_tiny(writefull, fuse, steps):
     p = rb_params(32, 16, 1e5, noutconfig=7, prtcl_writefull=writefull, nout=50)
     T0 = conduction_T(p, 1e-2, seed=11)
     parts = random_particles(p, 24, 24, seed=12)
-    g, o = _run_both(p, T0, parts, 300, particle_fuse=fuse)
+    g, o = _run_both(p, T0, parts, 300, particle_fuse=fuse, lbm_steps=steps)
     _compare_state(g, o)
     _compare_particles(g, o)
     _compare_records(g, o, writefull)
@@ -74,8 +74,9 @@ def test_truth_run_fp64_bit_exact_tiny(writefull, fuse):
         assert abs(a[2] - b[2]) <= 1e-12 * abs(b[2])
 
 
+@pytest.mark.parametrize("steps", [1, 2])
 @pytest.mark.parametrize("writefull", [0, 1])
-def test_particle_fuse_fp64_bit_exact(writefull):
+def test_particle_fuse_fp64_bit_exact(writefull, steps):
     # The particle update riding in the next lattice launch, including the cases that fall
     # back to a launch of its own: the last step of a call (chunks of 13, 2, 50, 35 steps)
     # and a record step followed by a diagnostics step (records every 7, energy every 5:
@@ -84,7 +85,7 @@ def test_particle_fuse_fp64_bit_exact(writefull):
     T0 = conduction_T(p, 1e-2, seed=21)
     parts = random_particles(p, 40, 40, seed=22)
     g = rbn.Solver(p)
-    g.set_tuning(particle_fuse=1)
+    g.set_tuning(particle_fuse=1, lbm_steps=steps)
     g.collect_records()
     g.collect_diagnostics()
     setup(g, p, T0, parts)
@@ -106,7 +107,9 @@ def test_particle_fuse_fp32_matches_separate_launches(writefull):
     T0 = conduction_T(p, 1e-2, seed=23)
     parts = random_particles(p, 300, 300, seed=24)
     runs = []
-    for tune in (dict(particle_fuse=0), dict(particle_fuse=1), dict(particle_fuse=1, lbm_xcd=2)):
+    for tune in (dict(particle_fuse=0, lbm_steps=1), dict(particle_fuse=1, lbm_steps=1),
+                 dict(particle_fuse=1, lbm_steps=1, lbm_xcd=2), dict(particle_fuse=1, lbm_steps=2),
+                 dict(particle_fuse=1, lbm_steps=2, lbm2_tile_h=4), dict(particle_fuse=0, lbm_steps=2)):
         g = rbn.Solver(p)
         g.set_tuning(**tune)
         g.collect_records()
@@ -128,11 +131,43 @@ def test_particle_fuse_fp32_matches_separate_launches(writefull):
         r.close()
 
 
+@pytest.mark.parametrize("prec", [rbn.RBN_FP64, rbn.RBN_FP32])
+@pytest.mark.parametrize("shape,tile_h", [((200, 44), 4), ((200, 44), 8), ((200, 44), 16), ((40, 20), 8),
+                                          ((128, 16), 16)])
+def test_two_step_launch_is_two_steps(prec, shape, tile_h):
+    # Partial tiles in x and y, a lattice narrower than one tile (the ring wraps onto the
+    # tile itself), energy diagnostics every 7 steps (a diagnostics step never starts a pair),
+    # calls of odd and even lengths.
+    nx, ny = shape
+    p = rb_params(nx, ny, 1e5, particle=0, nout=7)
+    p.precision = prec
+    T0 = conduction_T(p, 1e-2, seed=27)
+    runs = []
+    for steps in (1, 2):
+        s = rbn.Solver(p)
+        s.set_tuning(lbm_steps=steps, lbm2_tile_h=tile_h)
+        s.collect_diagnostics()
+        s.init_fields(T=T0)
+        for n in (23, 8, 1, 2, 9):
+            s.step(n)
+        runs.append(s)
+    _compare_state_pair(*runs)
+    assert runs[0].diagnostics == runs[1].diagnostics
+    if prec == rbn.RBN_FP64:
+        o = Oracle(p)
+        o.init_fields(T=T0)
+        o.step(43)
+        _compare_state(runs[1], o)
+    for s in runs:
+        s.close()
+
+
 def test_sampled_kernel_timing_counts():
     # every = 3 over one call of 31 steps: the first lattice launch carries nothing, the
     # next 30 carry the previous step's particle update, the last step's update runs alone.
     p = rb_params(64, 32, 1e5, noutconfig=1000)
     g = rbn.Solver(p)
+    g.set_tuning(lbm_steps=1)
     setup(g, p, conduction_T(p, 1e-2, seed=25), random_particles(p, 50, 50, seed=26))
     g.kernel_timing(True, every=3)
     g.step(31)
@@ -146,6 +181,15 @@ def test_sampled_kernel_timing_counts():
     assert g.kernel_stats("lbm_fused")[0] == 0
     with pytest.raises(rbn.RbnError):
         g.kernel_stats("nope")
+    g.close()
+    # Two steps per launch (the default): steps 1..30 in 15 two-step launches, the last
+    # step (it writes the moments) alone, carrying the updates of steps 29 and 30; the last
+    # step's update on its own.
+    g = rbn.Solver(p)
+    setup(g, p, conduction_T(p, 1e-2, seed=25), random_particles(p, 50, 50, seed=26))
+    g.kernel_timing(True, every=3)
+    g.step(31)
+    assert [g.kernel_stats(k)[0] for k in ("lbm", "lbm2", "lbm_fused", "particles")] == [0, 5, 1, 1]
     g.close()
 
 
