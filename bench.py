@@ -187,6 +187,8 @@ def run_ensemble(rbn, steps, warmup):
            "ms_per_step": round(1e3 * t_ens / steps, 4),
            "roofline_frac": round(nodes * lbm_bytes_per_node(1) / (t_ens / steps) / 1e9 / HBM_PEAK_GBPS, 4),
            "tuning": e.tuning(), "autotune_best_ms": round(best_trial, 4)}
+    order = out["tuning"]["order"]
+    out["two_steps_per_launch"] = bool(order & 8) and not order & 4
     e.close()
     for s in members:
         s.close()

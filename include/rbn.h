@@ -249,14 +249,16 @@ typedef struct rbn_ensemble rbn_ensemble;
 int rbn_ensemble_create(rbn_ctx *const *members, int n, rbn_ensemble **out);
 void rbn_ensemble_destroy(rbn_ensemble *e);
 const char *rbn_ensemble_last_error(const rbn_ensemble *e);
-/* Advance every member by nsteps (graph replays for whole six-step cycles; the first and
- * last steps of a call, and the last step's particle update, are launched eagerly). */
+/* Advance every member by nsteps (graph replays for whole cycles of six launches; the first
+ * and last launches of a call, and the last launch's particle updates, run eagerly). */
 int rbn_ensemble_step(rbn_ensemble *e, int nsteps);
 /* Tile shape (tile_w in {64,128,256} nodes, tile_w*tile_h a multiple of 256), persistent
  * workgroups per CU (0: one workgroup per work item) and work order (bit 0 clear: member by
  * member, set: members interleaved, largest first; bit 1: odd steps visit the lattice work
  * items last first; bit 2: a lattice launch and a particle launch per step, the particle
- * launch on a second stream beside the next lattice launch, instead of one launch). */
+ * launch on a second stream beside the next lattice launch, instead of one launch; bit 3
+ * without bit 2: two steps per launch on 64-node-wide tiles of 8 rows in FP32, 4 in FP64,
+ * whatever tile_w and tile_h say, except for a single step at the end of a call). */
 int rbn_ensemble_set_tuning(rbn_ensemble *e, int tile_w, int tile_h, int wg_per_cu, int order);
 int rbn_ensemble_get_tuning(const rbn_ensemble *e, int *tile_w, int *tile_h, int *wg_per_cu,
                             int *order);

@@ -18,8 +18,8 @@ struct rbn_ensemble {
   hipStream_t stream = nullptr, pstream = nullptr;
   hipEvent_t ev_lbm = nullptr, ev_part[2] = {nullptr, nullptr};
   void *d_members = nullptr;
-  int2 *d_items = nullptr;  // LBM items, then particle items
-  int n_lbm = 0, n_part = 0;
+  int2 *d_items = nullptr;  // one-step tiles, particle chunks, two-step tiles
+  int n_lbm = 0, n_part = 0, n_lbm2 = 0;
   int *d_counters = nullptr;  // [2 sets][2 phases]
   int cus = 1;
   // tuning
@@ -65,13 +65,17 @@ void fill_member(const rbn_ctx *c, int tile_w, int tile_h, rbn::EnsMember<R> &d)
   d.pf[1] = (R *)c->fB;
   d.pg[0] = (R *)c->gA;
   d.pg[1] = (R *)c->gB;
-  d.F = (R *)c->F;
+  d.F[0] = (R *)c->F;
+  d.F[1] = (R *)c->F2;
   for (int k = 0; k < rbn::kCycle; k++) d.uv[k] = (R *)c->uv[k];
   d.rho = (R *)c->rho;
   d.T = (R *)c->T;
   d.cur0 = c->cur;
   d.tiles_x = (c->geom.nx + tile_w - 1) / tile_w;
   d.tiles_y = (c->geom.ny + tile_h - 1) / tile_h;
+  const int th2 = sizeof(R) == 8 ? rbn::kEnsTH64 : rbn::kEnsTH32;
+  d.t2x = (c->geom.nx + 63) / 64;
+  d.t2y = (c->geom.ny + th2 - 1) / th2;
   d.part = c->d_part;
   d.np = (c->P.particle && c->d_part) ? c->np : 0;
   d.err = c->d_err;
@@ -83,17 +87,19 @@ int build(rbn_ensemble *e) {
   const int n = (int)e->m.size();
   const size_t dsz = e->prec == RBN_FP64 ? sizeof(rbn::EnsMember<double>) : sizeof(rbn::EnsMember<float>);
   std::vector<unsigned char> desc(dsz * (size_t)n);
-  std::vector<int> tiles(n), chunks(n);
+  std::vector<int> tiles(n), chunks(n), tiles2(n);
   for (int i = 0; i < n; i++) {
     if (e->prec == RBN_FP64) {
       auto *d = reinterpret_cast<rbn::EnsMember<double> *>(desc.data()) + i;
       fill_member(e->m[i], e->tile_w, e->tile_h, *d);
       tiles[i] = d->tiles_x * d->tiles_y;
+      tiles2[i] = d->t2x * d->t2y;
       chunks[i] = (d->np + 255) / 256;
     } else {
       auto *d = reinterpret_cast<rbn::EnsMember<float> *>(desc.data()) + i;
       fill_member(e->m[i], e->tile_w, e->tile_h, *d);
       tiles[i] = d->tiles_x * d->tiles_y;
+      tiles2[i] = d->t2x * d->t2y;
       chunks[i] = (d->np + 255) / 256;
     }
   }
@@ -119,6 +125,8 @@ int build(rbn_ensemble *e) {
   e->n_lbm = (int)items.size();
   emit(chunks);
   e->n_part = (int)items.size() - e->n_lbm;
+  emit(tiles2);
+  e->n_lbm2 = (int)items.size() - e->n_lbm - e->n_part;
   if (e->d_members) (void)hipFree(e->d_members);
   if (e->d_items) (void)hipFree(e->d_items);
   e->d_members = nullptr;
@@ -136,6 +144,47 @@ int grid_size(const rbn_ensemble *e) { return e->wg_per_cu > 0 ? e->wg_per_cu * 
 // order bit 2: separate lattice and particle launches (the particle launch on a second
 // stream beside the next lattice launch) instead of one launch per step.
 bool separate(const rbn_ensemble *e) { return (e->order & 4) != 0; }
+
+// order bit 3 (without bit 2): two steps per launch on two-step tiles, the previous launch's
+// particle updates carried.
+bool pairs(const rbn_ensemble *e) { return (e->order & 8) != 0 && !separate(e); }
+
+// Steps of one replay of the captured graph: the velocity ring's period, doubled in the
+// two-step mode so that the A/B and F planes (flipped once per launch) also return.
+int graph_steps(const rbn_ensemble *e) { return pairs(e) ? 2 * rbn::kCycle : rbn::kCycle; }
+
+// Two-step mode: where a launch stands (steps done t, launches L, two-step launches P, particle
+// updates waiting).
+struct PairState {
+  int t = 0, L = 0, P = 0, pending = 0;
+};
+
+int launch_pairs(rbn_ensemble *e, const rbn::EnsStep &st) {
+  EHIP(e, rbn::launch_ensemble_pairs(e->stream, e->prec, e->d_members, e->d_items, e->n_lbm, e->n_part, e->n_lbm2,
+                                     e->d_counters, st, e->tile_w, e->tile_h, grid_size(e)));
+  return RBN_OK;
+}
+
+// The next launch of the two-step mode: two steps when two remain, carrying what is pending.
+int enqueue_pair(rbn_ensemble *e, PairState &ps, int nsteps, bool moments) {
+  const bool two = ps.t + 1 < nsteps;
+  rbn::EnsStep st;
+  st.k = ps.t % rbn::kCycle;
+  st.lpar = ps.L & 1;
+  st.fpar = ps.P & 1;
+  st.ncarry = e->n_part > 0 ? ps.pending : 0;
+  st.lat = two ? 2 : 1;
+  st.moments = moments ? 1 : 0;
+  st.reverse = (e->order & 2) && (ps.L & 1);
+  st.cset = ps.L & 1;
+  int rc = launch_pairs(e, st);
+  if (rc) return rc;
+  ps.pending = two ? 2 : 1;
+  ps.t += two ? 2 : 1;
+  ps.L++;
+  if (two) ps.P++;
+  return RBN_OK;
+}
 
 int launch(rbn_ensemble *e, hipStream_t s, int phase, int k, bool moments) {
   EHIP(e, rbn::launch_ensemble_phase(s, e->prec, phase, e->d_members, e->d_items, e->n_lbm, e->n_part,
@@ -175,6 +224,14 @@ int capture(rbn_ensemble *e) {
   int rc = RBN_OK;
   if (separate(e)) {
     for (int k = 0; k < rbn::kCycle && !rc; k++) rc = enqueue_step(e, k, k == rbn::kCycle - 1, k >= 2);
+  } else if (pairs(e)) {
+    // launches 1 .. kCycle (steps 2 .. 2 kCycle + 1), each of two steps carrying two updates
+    PairState ps;
+    ps.t = 2;
+    ps.L = 1;
+    ps.P = 1;
+    ps.pending = 2;
+    for (int l = 0; l < rbn::kCycle && !rc; l++) rc = enqueue_pair(e, ps, 1 << 30, false);
   } else {
     for (int t = 1; t <= rbn::kCycle && !rc; t++) rc = enqueue_fused(e, t, false);
   }
@@ -195,14 +252,17 @@ int capture(rbn_ensemble *e) {
   return RBN_OK;
 }
 
-// Advance every member's host-side state by n steps of the ensemble.
-void advance_members(rbn_ensemble *e, int n) {
+// Advance every member's host-side state by n steps made of `launches` lattice launches, of
+// which `pairs` advanced two steps (each flips the A/B planes once, each two-step launch
+// the F planes once).
+void advance_members(rbn_ensemble *e, int n, int launches, int pairs) {
   for (rbn_ctx *c : e->m) {
     c->istep += n;
-    if (n & 1) {
+    if (launches & 1) {
       std::swap(c->fA, c->fB);
       std::swap(c->gA, c->gB);
     }
+    if (pairs & 1) std::swap(c->F, c->F2);
     c->cur = (c->cur + n) % rbn_ctx::kVel;
     c->moments_valid = true;
   }
@@ -225,7 +285,37 @@ int run_steps(rbn_ensemble *e, int nsteps) {
   // mid-cycle do not leave set 0 clear for a cycle that starts over: clear both here
   // (an ordinary stream operation, never part of the captured graph).
   EHIP(e, hipMemsetAsync(e->d_counters, 0, 4 * sizeof(int), e->stream));
-  if (separate(e)) {
+  int launches = nsteps, npairs = 0;
+  if (pairs(e)) {
+    // launch 0 eagerly, whole cycles of kCycle two-step launches from the graph, the rest
+    // eagerly (at least one launch, the last writing the moments), then the particle
+    // updates of the last launch on their own.
+    PairState ps;
+    if ((rc = enqueue_pair(e, ps, nsteps, nsteps <= 2))) return rc;
+    const int span = 2 * rbn::kCycle;
+    const int cycles = nsteps - ps.t >= 1 ? (nsteps - ps.t - 1) / span : 0;
+    if (cycles > 0) {
+      if (!e->exec && (rc = capture(e))) return rc;
+      for (int c = 0; c < cycles; c++) EHIP(e, hipGraphLaunch(e->exec, e->stream));
+      ps.t += cycles * span;
+      ps.L += cycles * rbn::kCycle;
+      ps.P += cycles * rbn::kCycle;
+    }
+    while (ps.t < nsteps) {
+      const bool last = ps.t + 2 >= nsteps;
+      if ((rc = enqueue_pair(e, ps, nsteps, last))) return rc;
+    }
+    if (e->n_part > 0) {
+      rbn::EnsStep st{};
+      st.k = ps.t % rbn::kCycle;
+      st.ncarry = ps.pending;
+      st.lat = 0;
+      st.cset = ps.L & 1;
+      if ((rc = launch_pairs(e, st))) return rc;
+    }
+    launches = ps.L;
+    npairs = ps.P;
+  } else if (separate(e)) {
     const int cycles = nsteps / rbn::kCycle, rem = nsteps % rbn::kCycle;
     if (cycles > 0) {
       if (!e->exec && (rc = capture(e))) return rc;
@@ -250,8 +340,9 @@ int run_steps(rbn_ensemble *e, int nsteps) {
   }
   EHIP(e, hipStreamSynchronize(e->stream));
   EHIP(e, hipStreamSynchronize(e->pstream));
-  advance_members(e, nsteps);
-  if (nsteps % rbn::kCycle) {  // member parity or ring position changed: rebuild descriptors and graph
+  advance_members(e, nsteps, launches, npairs);
+  if (nsteps % rbn::kCycle || (launches & 1) || (npairs & 1)) {
+    // member parity or ring position changed: rebuild descriptors and graph
     e->built = false;
     drop_graph(e);
   }
@@ -263,13 +354,13 @@ struct Snapshot {
   std::vector<void *> bufs;
   std::vector<std::pair<void *, size_t>> src;
   std::vector<int> istep, cur;
-  std::vector<void *> fA, gA;
+  std::vector<void *> fA, gA, F;
 };
 
 int take_snapshot(rbn_ensemble *e, Snapshot &s) {
   for (rbn_ctx *c : e->m) {
     const size_t fb = c->geom.plane * c->rs;
-    std::vector<std::pair<void *, size_t>> parts = {{c->fA, 9 * fb}, {c->gA, 9 * fb}, {c->F, fb}};
+    std::vector<std::pair<void *, size_t>> parts = {{c->fA, 9 * fb}, {c->gA, 9 * fb}, {c->F, fb}, {c->F2, fb}};
     for (int k = 0; k < rbn_ctx::kVel; k++) parts.push_back({c->uv[k], 2 * fb});
     if (c->d_part && c->np > 0) parts.push_back({c->d_part, 10 * (size_t)c->np * sizeof(double)});
     for (auto &p : parts) {
@@ -283,6 +374,7 @@ int take_snapshot(rbn_ensemble *e, Snapshot &s) {
     s.cur.push_back(c->cur);
     s.fA.push_back(c->fA);
     s.gA.push_back(c->gA);
+    s.F.push_back(c->F);
   }
   return RBN_OK;
 }
@@ -294,6 +386,7 @@ int restore_snapshot(rbn_ensemble *e, Snapshot &s) {
       std::swap(c->fA, c->fB);
       std::swap(c->gA, c->gB);
     }
+    if (c->F != s.F[i]) std::swap(c->F, c->F2);
     c->istep = s.istep[i];
     c->cur = s.cur[i];
   }
@@ -366,8 +459,8 @@ const char *rbn_ensemble_last_error(const rbn_ensemble *e) { return e ? e->err.c
 int rbn_ensemble_set_tuning(rbn_ensemble *e, int tile_w, int tile_h, int wg_per_cu, int order) {
   if (!e) return rbn::thread_fail(RBN_EINVAL, "null ensemble");
   if (!(tile_w == 64 || tile_w == 128 || tile_w == 256) || tile_h < 1 || tile_h > 64 ||
-      (tile_w * tile_h) % 256 != 0 || wg_per_cu < 0 || wg_per_cu > 32 || order < 0 || order > 7)
-    return efail(e, RBN_EINVAL, "tile must be 64/128/256 wide with a multiple of 256 nodes; wg_per_cu 0..32; order 0..7");
+      (tile_w * tile_h) % 256 != 0 || wg_per_cu < 0 || wg_per_cu > 32 || order < 0 || order > 15)
+    return efail(e, RBN_EINVAL, "tile must be 64/128/256 wide with a multiple of 256 nodes; wg_per_cu 0..32; order 0..15");
   e->tile_w = tile_w;
   e->tile_h = tile_h;
   e->wg_per_cu = wg_per_cu;
@@ -402,7 +495,6 @@ int rbn_ensemble_step(rbn_ensemble *e, int nsteps) {
 int rbn_ensemble_autotune(rbn_ensemble *e, int trial_steps, double *best_ms_per_step) {
   if (!e) return rbn::thread_fail(RBN_EINVAL, "null ensemble");
   if (hipSetDevice(e->dev) != hipSuccess) return efail(e, RBN_EDEVICE, "hipSetDevice failed");
-  const int trial = std::max(rbn::kCycle, (trial_steps + rbn::kCycle - 1) / rbn::kCycle * rbn::kCycle);
   struct Cand {
     int tw, th, wpc, order;
   };
@@ -412,6 +504,9 @@ int rbn_ensemble_autotune(rbn_ensemble *e, int trial_steps, double *best_ms_per_
   for (auto &sh : shapes)
     for (int w : wpcs)
       for (int o = 0; o < 8; o++) cands.push_back({sh[0], sh[1], w, o});
+  // two steps per launch: the tile is fixed (64 x 8 in FP32, 64 x 4 in FP64)
+  for (int w : wpcs)
+    for (int o = 8; o < 12; o++) cands.push_back({256, 1, w, o});
   int rc = sync_members(e);
   if (rc) return rc;
   Snapshot snap;
@@ -423,17 +518,23 @@ int rbn_ensemble_autotune(rbn_ensemble *e, int trial_steps, double *best_ms_per_
   Cand bestc = cands[0];
   for (const Cand &c : cands) {
     if ((rc = rbn_ensemble_set_tuning(e, c.tw, c.th, c.wpc, c.order))) break;
-    if ((rc = run_steps(e, rbn::kCycle))) break;  // build, warm (a whole cycle: descriptors stay valid)
+    const int gs = graph_steps(e);
+    const int reps = std::max(1, (trial_steps + gs - 1) / gs), trial = reps * gs;
+    if ((rc = run_steps(e, gs))) break;  // build, warm (a whole cycle: descriptors stay valid)
+    if (!e->built && (rc = build(e))) break;
     if (!e->exec && (rc = capture(e))) break;
-    // The captured cycle of one launch per step starts at step 1 (counter set 1), which the
-    // eager steps above leave used: clear the counters before replaying.
+    // The captured cycle starts past the first launch of a call (counter set 1), which the
+    // eager launches above leave used: clear the counters before replaying.
     EHIP(e, hipMemsetAsync(e->d_counters, 0, 4 * sizeof(int), e->stream));
     EHIP(e, hipEventRecord(a, e->stream));
-    for (int s = 0; s < trial / rbn::kCycle; s++) EHIP(e, hipGraphLaunch(e->exec, e->stream));
+    for (int s = 0; s < reps; s++) EHIP(e, hipGraphLaunch(e->exec, e->stream));
     EHIP(e, hipEventRecord(b, e->stream));
     EHIP(e, hipEventSynchronize(b));
     EHIP(e, hipStreamSynchronize(e->pstream));
-    advance_members(e, trial);
+    if (pairs(e))
+      advance_members(e, trial, trial / 2, trial / 2);
+    else
+      advance_members(e, trial, trial, 0);
     float ms = 0.f;
     EHIP(e, hipEventElapsedTime(&ms, a, b));
     if (ms / trial < best) {

@@ -10,18 +10,23 @@ namespace rbn {
 constexpr int kCycle = rbn_ctx::kVel;
 static_assert(kCycle % 2 == 0, "a cycle must return the A/B planes");
 
-// Pointers and constants of one member as of the start of a cycle: pf[0]/pg[0] are the f/g
-// planes read by the first step, uv[cur0] the latest velocity ((u, v) interleaved).
+// Rows of a two-step tile (64 columns) in the ensemble: the single-lattice defaults.
+constexpr int kEnsTH32 = 8, kEnsTH64 = 4;
+
+// Pointers and constants of one member as of the descriptors' base state: pf[0]/pg[0] are the
+// f/g planes read by the first launch, F[0] its force plane (F[1] the second plane a two-step
+// launch writes), uv[cur0] the latest velocity ((u, v) interleaved).
 template <typename R>
 struct EnsMember {
   Geom g;
   StepConst<R> c;
   R *pf[2], *pg[2];
-  R *F;
+  R *F[2];
   R *uv[kCycle];
   R *rho, *T;
   int cur0;
-  int tiles_x, tiles_y;
+  int tiles_x, tiles_y;  // one-step tiles
+  int t2x, t2y;          // two-step tiles
   double *part;  // 10 arrays of np (x, y, u, v, xo, yo, uo, vo, 1/tau, beta)
   int np;
   int *err;
@@ -33,5 +38,16 @@ struct EnsMember {
 hipError_t launch_ensemble_phase(hipStream_t s, int prec, int phase, const void *members,
                                  const int2 *items, int n_lbm, int n_part, int *counters, int k,
                                  int tile_w, int tile_h, int write_moments, int grid, int reverse);
+
+// One launch of the two-step mode: the particle updates of steps k-ncarry .. k-1 (items
+// n_lbm .. n_lbm+n_part), then the lattice of step k as one-step tiles (lat 1, items 0 ..
+// n_lbm) or of steps k, k+1 as two-step tiles (lat 2, items n_lbm+n_part ..).  lpar: which
+// of pf/pg is the input; fpar: which of F is the input force; cset: the counter set.
+struct EnsStep {
+  int k, lpar, fpar, ncarry, lat, moments, reverse, cset;
+};
+hipError_t launch_ensemble_pairs(hipStream_t s, int prec, const void *members, const int2 *items, int n_lbm,
+                                 int n_part, int n_lbm2, int *counters, const EnsStep &st, int tile_w,
+                                 int tile_h, int grid);
 
 }  // namespace rbn

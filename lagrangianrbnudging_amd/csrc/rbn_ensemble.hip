@@ -24,6 +24,7 @@
 // counter, particle launches the other) and clears its counter of set (k+1) % 2 for the
 // next step, which only starts after this launch has finished; the counters are therefore
 // zero at every launch without any memset node in the graph.
+#include "rbn_lbm2_tile.h"
 #include "rbn_lbm_node.h"
 #include "rbn_particle_node.h"
 #include "rbn_ensemble.h"
@@ -32,18 +33,20 @@ namespace rbn {
 namespace {
 
 constexpr int kThreads = 256;
+static_assert(kThreads == lbm2::kTileThreads, "two-step tiles run on whole workgroups");
 
+// One-step tile of step k: input planes pf[par] / pg[par], force F (read and rewritten in
+// place).
 template <typename R>
-__device__ __forceinline__ void lbm_item(const EnsMember<R> &m, int tile, int k, int tile_w,
+__device__ __forceinline__ void lbm_item(const EnsMember<R> &m, int tile, int k, int par, R *F, int tile_w,
                                          int tile_h, int write_moments) {
   const int tx = tile % m.tiles_x, ty = tile / m.tiles_x;
-  const int par = k & 1;
   LbmPtrs p{};
   p.fA = par ? m.pf[1] : m.pf[0];
   p.fB = par ? m.pf[0] : m.pf[1];
   p.gA = par ? m.pg[1] : m.pg[0];
   p.gB = par ? m.pg[0] : m.pg[1];
-  p.F = m.F;
+  p.F = F;
   const int out = (m.cur0 + 1 + k) % kCycle;
   p.uv = m.uv[out];
   p.rho = write_moments ? m.rho : nullptr;
@@ -111,7 +114,7 @@ __device__ __forceinline__ void run_item(const EnsMember<R> *__restrict__ M, con
   }
   const int l = it - n_first;
   const int2 w = items[reverse ? n_lbm - 1 - l : l];
-  lbm_item(M[w.x], w.y, k, tile_w, tile_h, write_moments);
+  lbm_item(M[w.x], w.y, k, k & 1, M[w.x].F[0], tile_w, tile_h, write_moments);
 }
 
 template <typename R, int PHASE>
@@ -166,6 +169,73 @@ void launch_prec(hipStream_t s, int phase, dim3 g, const void *members, const in
     launch_phase<R, 2>(s, g, members, items, n_lbm, n_part, counters, k, tile_w, tile_h, write_moments,
                        persistent, reverse);
 }
+// Two-step tile of steps k, k+1 (64 x TH nodes, rbn_lbm2_tile.h).
+template <typename R, int TH>
+__device__ __forceinline__ void pair_item(const EnsMember<R> &m, int tile, const EnsStep &st,
+                                          lbm2::TileLds<R, TH> &L) {
+  const int tx = tile % m.t2x, ty = tile / m.t2x;
+  lbm2::TwoStepPtrs p;
+  p.fA = m.pf[st.lpar];
+  p.gA = m.pg[st.lpar];
+  p.fB = m.pf[1 - st.lpar];
+  p.gB = m.pg[1 - st.lpar];
+  p.Fin = m.F[st.fpar];
+  p.Fout = m.F[1 - st.fpar];
+  p.uv1 = m.uv[(m.cur0 + 1 + st.k) % kCycle];
+  p.uv2 = m.uv[(m.cur0 + 2 + st.k) % kCycle];
+  p.rho2 = st.moments ? m.rho : nullptr;
+  p.T2 = st.moments ? m.T : nullptr;
+  lbm2::two_step_tile<R, TH>(m.g, p, m.c, tx, ty, L);
+}
+
+// Item `it` of a two-step-mode launch: carried particle items first, then lattice items.
+template <typename R, int TH>
+__device__ __forceinline__ void run_item2(const EnsMember<R> *__restrict__ M, const int2 *__restrict__ items, int it,
+                                          int n_lbm, int n_part, int n_first, int n_lat, const EnsStep &st,
+                                          int tile_w, int tile_h, lbm2::TileLds<R, TH> &L) {
+  if (it < n_first) {
+    const int2 w = items[n_lbm + it];
+    for (int u = 0; u < st.ncarry; u++) particle_item(M[w.x], w.y, st.k - st.ncarry + u);
+    return;
+  }
+  const int l = it - n_first, lr = st.reverse ? n_lat - 1 - l : l;
+  if (st.lat == 2) {
+    const int2 w = items[n_lbm + n_part + lr];
+    pair_item<R, TH>(M[w.x], w.y, st, L);
+  } else {
+    const int2 w = items[lr];
+    lbm_item(M[w.x], w.y, st.k, st.lpar, M[w.x].F[st.fpar], tile_w, tile_h, st.moments);
+  }
+}
+
+template <typename R, int TH>
+__global__ __launch_bounds__(kThreads) void k_ensemble2(const EnsMember<R> *__restrict__ M,
+                                                        const int2 *__restrict__ items, int n_lbm, int n_part,
+                                                        int n_lbm2, int *counters, EnsStep st, int tile_w,
+                                                        int tile_h, int persistent) {
+  __shared__ lbm2::TileLds<R, TH> L;
+  const int slot = st.lat ? 0 : 1;
+  const int n_first = st.ncarry > 0 ? n_part : 0;
+  const int n_lat = st.lat == 2 ? n_lbm2 : (st.lat == 1 ? n_lbm : 0);
+  const int n_items = n_first + n_lat;
+  clear_next(counters, st.cset, slot);
+  if (!persistent) {
+    const int it = blockIdx.x;
+    if (it < n_items) run_item2<R, TH>(M, items, it, n_lbm, n_part, n_first, n_lat, st, tile_w, tile_h, L);
+    return;
+  }
+  __shared__ int item;
+  int *ctr = &counters[(st.cset & 1) * 2 + slot];
+  for (;;) {
+    if (threadIdx.x == 0) item = atomicAdd(ctr, 1);
+    __syncthreads();
+    const int it = item;
+    __syncthreads();
+    if (it >= n_items) break;
+    run_item2<R, TH>(M, items, it, n_lbm, n_part, n_first, n_lat, st, tile_w, tile_h, L);
+  }
+}
+
 }  // namespace
 
 hipError_t launch_ensemble_phase(hipStream_t s, int prec, int phase, const void *members,
@@ -181,6 +251,22 @@ hipError_t launch_ensemble_phase(hipStream_t s, int prec, int phase, const void 
   else
     launch_prec<float>(s, phase, g, members, items, n_lbm, n_part, counters, k, tile_w, tile_h,
                        write_moments, persistent, reverse);
+  return hipGetLastError();
+}
+
+hipError_t launch_ensemble_pairs(hipStream_t s, int prec, const void *members, const int2 *items, int n_lbm,
+                                 int n_part, int n_lbm2, int *counters, const EnsStep &st, int tile_w,
+                                 int tile_h, int grid) {
+  const int n_items = (st.ncarry > 0 ? n_part : 0) + (st.lat == 2 ? n_lbm2 : (st.lat == 1 ? n_lbm : 0));
+  if (n_items <= 0) return hipSuccess;
+  const int persistent = grid > 0;
+  const dim3 g(persistent ? (unsigned)grid : (unsigned)n_items), b(kThreads);
+  if (prec == RBN_FP64)
+    hipLaunchKernelGGL((k_ensemble2<double, kEnsTH64>), g, b, 0, s, (const EnsMember<double> *)members, items,
+                       n_lbm, n_part, n_lbm2, counters, st, tile_w, tile_h, persistent);
+  else
+    hipLaunchKernelGGL((k_ensemble2<float, kEnsTH32>), g, b, 0, s, (const EnsMember<float> *)members, items,
+                       n_lbm, n_part, n_lbm2, counters, st, tile_w, tile_h, persistent);
   return hipGetLastError();
 }
 

@@ -1,5 +1,6 @@
 """Batched ensemble: persistent launches over all members (one per step carrying the previous
-step's particle update, or a lattice and a particle launch per step), hipGraph-captured, must
+step's particle update, a lattice and a particle launch per step, or one per two steps on
+two-step tiles carrying the previous launch's two particle updates), hipGraph-captured, must
 give exactly what stepping every member on its own gives."""
 import numpy as np
 import pytest
@@ -42,18 +43,22 @@ def _compare(a, b):
 
 @pytest.mark.parametrize("prec", [rbn.RBN_FP64, rbn.RBN_FP32])
 @pytest.mark.parametrize("tuning", [(256, 1, 0, 0), (256, 2, 4, 1), (64, 8, 2, 2), (128, 4, 8, 3), (256, 1, 0, 2),
-                                    (256, 1, 4, 4), (128, 4, 2, 7)])
+                                    (256, 1, 4, 4), (128, 4, 2, 7), (256, 1, 0, 8), (128, 4, 4, 11), (64, 8, 2, 9),
+                                    (256, 2, 8, 10), (128, 4, 2, 15)])
 def test_ensemble_equals_individual_steps(prec, tuning):
     ens_members = _members(prec)
     ref_members = _members(prec)
     e = rbn.Ensemble(ens_members)
     e.set_tuning(*tuning)
-    # Calls of 9, 1, 2, 6, 13 steps: graph cycles and eager steps, calls without a cycle,
-    # parity changes (rebuilds) between calls.
-    for n in (9, 1, 2, 6, 13):
+    # Calls of 9, 1, 2, 6, 13, 27, 12, 15 steps: graph cycles (one replay covers six steps
+    # with one step per launch, twelve with two) and eager launches, calls without a cycle,
+    # parity changes (rebuilds) between calls, a call (12 steps) after which the captured
+    # graph is replayed again by the next call.
+    calls = (9, 1, 2, 6, 13, 27, 12, 15)
+    for n in calls:
         e.step(n)
     for s in ref_members:
-        s.step(31)
+        s.step(sum(calls))
     _compare(ens_members, ref_members)
     e.close()
 
