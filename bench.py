@@ -144,10 +144,11 @@ ENS_SIZES = [(256, 128), (512, 256), (768, 384), (1024, 512)]
 ENS_MEMBERS = 64
 
 
-def run_ensemble(rbn, steps, warmup):
+def run_ensemble(rbn, steps, warmup, tuning=None):
     """64 FP32 truth-run members, sizes drawn from ENS_SIZES, Ra log-uniform in [1e6, 1e8],
-    1e3..1e4 tracers each, own seeds; batched ensemble (autotuned, graph-captured) against the
-    same members stepped one after another with their own launches."""
+    1e3..1e4 tracers each, own seeds; batched ensemble (autotuned, or the given launch shape
+    (tile_w, tile_h, wg_per_cu, order); graph-captured) against the same members stepped one
+    after another with their own launches."""
     rng = np.random.default_rng(7)
     members, nodes = [], 0
     for m in range(ENS_MEMBERS):
@@ -175,7 +176,11 @@ def run_ensemble(rbn, steps, warmup):
         s.synchronize()
     t_seq = time.perf_counter() - t0
     e = rbn.Ensemble(members)
-    best_trial = e.autotune(8)
+    if tuning is None:
+        best_trial = e.autotune(8)
+    else:
+        e.set_tuning(*tuning)
+        best_trial = float("nan")
     e.step(warmup - warmup % 4 + 4)
     t0 = time.perf_counter()
     e.step(steps)
@@ -186,7 +191,7 @@ def run_ensemble(rbn, steps, warmup):
            "speedup_vs_sequential": round(t_seq / t_ens, 2),
            "ms_per_step": round(1e3 * t_ens / steps, 4),
            "roofline_frac": round(nodes * lbm_bytes_per_node(1) / (t_ens / steps) / 1e9 / HBM_PEAK_GBPS, 4),
-           "tuning": e.tuning(), "autotune_best_ms": round(best_trial, 4)}
+           "tuning": e.tuning(), "autotune_best_ms": finite_or_null(round(best_trial, 4))}
     order = out["tuning"]["order"]
     out["two_steps_per_launch"] = bool(order & 8) and not order & 4
     e.close()
@@ -549,6 +554,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 companion run")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble run")
+    ap.add_argument("--ensemble-tuning", type=str, default="",
+                    help="tile_w,tile_h,wg_per_cu,order for the ensemble instead of autotuning")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-window run")
     ap.add_argument("--no-nudging", action="store_true", help="skip the nudging run")
     ap.add_argument("--nud-steps", type=int, default=20000)
@@ -581,7 +588,8 @@ def main() -> int:
         print(f"bench: fp64 done {r64}", file=sys.stderr, flush=True)
     ens = None
     if not args.no_ensemble:
-        ens = run_ensemble(rbn, max(args.steps // 5, 8), 20)
+        tun = tuple(int(v) for v in args.ensemble_tuning.split(",")) if args.ensemble_tuning else None
+        ens = run_ensemble(rbn, max(args.steps // 5, 8), 20, tun)
         print(f"bench: ensemble done {ens}", file=sys.stderr, flush=True)
     nud = None
     if not args.no_nudging:
