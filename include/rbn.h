@@ -300,8 +300,15 @@ int rbn_adj_backward(rbn_ctx *ctx, double *dJ_dT0, double *dJ_dalpha);
  *                populations kept in LDS (temporal blocking), for every step that writes no
  *                moments and is followed by another step of the same rbn_step call; with
  *                particles only under particle_fuse (default 2); 1: one step per launch
- *   "lbm2_tile_h" rows of a two-step tile of 64 columns: 4, 8 or 16 (FP32 only), 0 for the
- *                default (8 in FP32, 4 in FP64)
+ *   "lbm2_seg"   8..64: two steps per launch by row sweeps, every workgroup sweeping a strip
+ *                of 256 columns over a segment of lbm2_seg rows, the first step's
+ *                sideways-moving populations passed between its threads through LDS; 0: by
+ *                tiles of 64 columns and lbm2_tile_h rows (default 0)
+ *   "lbm2_xcd"   1: the row-sweep workgroups one XCD receives take consecutive strips and
+ *                segments (default 1)
+ *   "lbm2_variant" 1: the row sweep issues the loads of a row one row ahead (default 0)
+ *   "lbm2_tile_h" rows of a two-step tile of 64 columns (lbm2_seg 0): 4, 8 or 16 (FP32
+ *                only), 0 for the default (8 in FP32, 4 in FP64)
  *   "particle_overlap"  1: run the particle update of step s on its own stream concurrently
  *                with the lattice update of step s+1 (default 0)
  *   "particle_fuse" 1: run the particle update of step s as extra workgroups of the lattice

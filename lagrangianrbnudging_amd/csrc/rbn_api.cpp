@@ -1103,7 +1103,7 @@ int step_loop(rbn_ctx *c, int nsteps) {
            if (pair) {
              const rbn::PartJob job = carry ? pending_job(c) : rbn::PartJob{};
              return rbn::launch_lbm2(c->stream, c->prec, c->geom, lp, c->F2, c->uv[s2], moments2 ? c->rho : nullptr,
-                                     moments2 ? c->T : nullptr, c->consts.data(), t.tile_h,
+                                     moments2 ? c->T : nullptr, c->consts.data(), t.tile_h, t.seg, t.xcd2, t.variant,
                                      t.reverse && ((istep >> 1) & 1), carry ? &job : nullptr);
            }
            t.flip = t.reverse && (istep & 1);
@@ -1222,6 +1222,14 @@ int rbn_set_tuning(rbn_ctx *c, const char *key, int value) {
     if (value != 0 && value != 4 && value != 8 && value != 16)
       return fail(c, RBN_EINVAL, "lbm2_tile_h must be 0, 4, 8 or 16");
     c->tune.tile_h = value;
+  } else if (!strcmp(key, "lbm2_seg")) {
+    if (value != 0 && (value < 8 || value > 64)) return fail(c, RBN_EINVAL, "lbm2_seg must be 0 or 8..64");
+    c->tune.seg = value;
+  } else if (!strcmp(key, "lbm2_xcd")) {
+    c->tune.xcd2 = value != 0;
+  } else if (!strcmp(key, "lbm2_variant")) {
+    if (value < 0 || value > 1) return fail(c, RBN_EINVAL, "lbm2_variant must be 0 or 1");
+    c->tune.variant = value;
   } else if (!strcmp(key, "particle_overlap")) {
     c->tune.overlap = value != 0;
   } else if (!strcmp(key, "particle_fuse")) {
@@ -1245,6 +1253,9 @@ int rbn_get_tuning(const rbn_ctx *c, const char *key, int *value) {
   else if (!strcmp(key, "lbm_reverse")) *value = c->tune.reverse;
   else if (!strcmp(key, "lbm_steps")) *value = c->tune.steps2 ? 2 : 1;
   else if (!strcmp(key, "lbm2_tile_h")) *value = c->tune.tile_h;
+  else if (!strcmp(key, "lbm2_seg")) *value = c->tune.seg;
+  else if (!strcmp(key, "lbm2_xcd")) *value = c->tune.xcd2;
+  else if (!strcmp(key, "lbm2_variant")) *value = c->tune.variant;
   else if (!strcmp(key, "particle_overlap")) *value = c->tune.overlap;
   else if (!strcmp(key, "particle_fuse")) *value = c->tune.fuse;
   else if (!strcmp(key, "adj_store_deposit")) *value = c->adj_store_deposit;

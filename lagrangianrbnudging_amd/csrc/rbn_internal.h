@@ -107,6 +107,9 @@ struct LbmTuning {
   int fuse = 1;     // particle update of step s in the grid of the lattice step s+1
   int steps2 = 1;   // truth run: two steps per lattice launch (temporal blocking in LDS)
   int tile_h = 0;   // rows of a two-step tile (4, 8, 16; 0: 8 in FP32, 4 in FP64)
+  int seg = 0;      // two steps by row sweeps over segments of seg rows (8..64); 0: by tiles
+  int xcd2 = 1;     // row sweeps: consecutive strips and segments on one XCD
+  int variant = 0;  // row sweeps: 1 pulls one row ahead
 };
 
 // The particle updates of one or two consecutive steps, in step order, carried by the next
@@ -137,8 +140,8 @@ hipError_t launch_lbm(hipStream_t s, int prec, int variant, const Geom &g, const
 // Fout receives F(s+2), uv2 the second step's velocity, rho2 / T2 (or null) its moments.
 // job (or null): particle updates carried as extra workgroups ahead of the tiles.
 hipError_t launch_lbm2(hipStream_t s, int prec, const Geom &g, const LbmPtrs &p, void *Fout, void *uv2,
-                       void *rho2, void *T2, const void *step_const, int tile_h, int flip,
-                       const PartJob *job);
+                       void *rho2, void *T2, const void *step_const, int tile_h, int seg, int xcd, int variant,
+                       int flip, const PartJob *job);
 // K1 of a truth-run step with the particle update of the previous step in the same grid
 // (its own workgroups, ahead of the lattice ones).
 hipError_t launch_lbm_fused(hipStream_t s, int prec, const Geom &g, const LbmPtrs &p,

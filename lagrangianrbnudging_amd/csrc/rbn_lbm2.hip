@@ -1,9 +1,10 @@
-// rbn_lbm2.hip — two truth-run steps of K1 in one launch (the tile code, rbn_lbm2_tile.h,
-// keeps the first step's populations in LDS).  A step that must write its moments is never
-// the first of a pair (the host launches it alone).  Particle updates of earlier steps ride
-// in the launch as workgroups ahead of the tiles (rbn_lbm_fused.hip explains why that is
-// safe).
+// rbn_lbm2.hip — two truth-run steps of K1 in one launch: by row sweeps over column strips
+// (rbn_lbm2_wave.h) or by tiles (rbn_lbm2_tile.h), the first step's populations passed
+// through LDS either way.  A step that must write its moments is never the first of a pair
+// (the host launches it alone).  Particle updates of earlier steps ride in the launch as
+// workgroups ahead of the lattice ones (rbn_lbm_fused.hip explains why that is safe).
 #include "rbn_lbm2_tile.h"
+#include "rbn_lbm2_wave.h"
 #include "rbn_particle_node.h"
 
 namespace rbn {
@@ -27,6 +28,48 @@ __global__ __launch_bounds__(kTileThreads) void k_lbm2(Geom g, Lbm2Args a, StepC
   const int t = a.flip ? a.ntiles - 1 - b : b;
   const int ty = t / a.tiles_x, tx = t - ty * a.tiles_x;
   two_step_tile<R, TH>(g, a.p, c, tx, ty, L);
+}
+
+struct WaveArgs {
+  TwoStepPtrs p;
+  int strips, seg, nwg, flip, xcd;
+};
+
+// Dispatch index -> work index so that the workgroups one XCD receives (every eighth in
+// dispatch order) take consecutive strips and segments, whose edge rows and columns they
+// share in that XCD's L2.
+__device__ __forceinline__ int xcd_order(int b, int n) {
+  const int x = b & 7, k = b >> 3, per = n >> 3, rem = n & 7;
+  return x * per + min(x, rem) + k;
+}
+
+template <typename R, int AHEAD, int SW>
+__global__ __launch_bounds__(SW) void k_lbm2w(Geom g, WaveArgs a, StepConst<R> c, part::PartSteps<R> ps) {
+  __shared__ WaveLds<R, SW> L;
+  if ((int)blockIdx.x < ps.nblocks) {
+    part::run_part_steps(ps, blockIdx.x * SW + threadIdx.x);
+    return;
+  }
+  int b = (int)blockIdx.x - ps.nblocks;
+  if (a.xcd) b = xcd_order(b, a.nwg);
+  const int w = a.flip ? a.nwg - 1 - b : b;
+  const int sy = w / a.strips, sx = w - sy * a.strips;
+  const int ja = sy * a.seg;
+  two_step_strip<R, AHEAD, SW>(g, a.p, c, sx, ja, min(ja + a.seg, g.ny), L);
+}
+
+constexpr int kStripW = 256;
+
+template <typename R>
+void dispatch_wave(hipStream_t s, const Geom &g, const WaveArgs &a, int variant, const void *step_const,
+                   const PartJob *job) {
+  const StepConst<R> &c = *(const StepConst<R> *)step_const;
+  const part::PartSteps<R> ps = part::part_steps<R>(g, job, kStripW);
+  const dim3 grid((unsigned)(a.nwg + ps.nblocks)), block(kStripW);
+  if (variant & 1)
+    hipLaunchKernelGGL((k_lbm2w<R, 1, kStripW>), grid, block, 0, s, g, a, c, ps);
+  else
+    hipLaunchKernelGGL((k_lbm2w<R, 0, kStripW>), grid, block, 0, s, g, a, c, ps);
 }
 
 // Tile heights: 4, 8 or 16 rows (LDS 29.5 / 49.6 / 89.6 KB in FP32); FP64 at most 8 rows
@@ -66,8 +109,31 @@ void dispatch64(hipStream_t s, int th, const Geom &g, const Lbm2Args &a, const v
 }  // namespace
 
 hipError_t launch_lbm2(hipStream_t s, int prec, const Geom &g, const LbmPtrs &p, void *Fout, void *uv2,
-                       void *rho2, void *T2, const void *step_const, int tile_h, int flip,
-                       const PartJob *job) {
+                       void *rho2, void *T2, const void *step_const, int tile_h, int seg, int xcd, int variant,
+                       int flip, const PartJob *job) {
+  if (seg > 0) {
+    WaveArgs w;
+    w.p.fA = p.fA;
+    w.p.gA = p.gA;
+    w.p.fB = p.fB;
+    w.p.gB = p.gB;
+    w.p.Fin = p.F;
+    w.p.Fout = Fout;
+    w.p.uv1 = p.uv;
+    w.p.uv2 = uv2;
+    w.p.rho2 = rho2;
+    w.p.T2 = T2;
+    w.seg = min(seg, kMaxSeg);
+    w.strips = (g.nx + kStripW - 1) / kStripW;
+    w.nwg = w.strips * ((g.ny + w.seg - 1) / w.seg);
+    w.flip = flip;
+    w.xcd = xcd;
+    if (prec == RBN_FP64)
+      dispatch_wave<double>(s, g, w, variant, step_const, job);
+    else
+      dispatch_wave<float>(s, g, w, variant, step_const, job);
+    return hipGetLastError();
+  }
   const int th = tile_rows(prec, tile_h);
   Lbm2Args a;
   a.p.fA = p.fA;

@@ -109,7 +109,8 @@ def test_particle_fuse_fp32_matches_separate_launches(writefull):
     runs = []
     for tune in (dict(particle_fuse=0, lbm_steps=1), dict(particle_fuse=1, lbm_steps=1),
                  dict(particle_fuse=1, lbm_steps=1, lbm_xcd=2), dict(particle_fuse=1, lbm_steps=2),
-                 dict(particle_fuse=1, lbm_steps=2, lbm2_tile_h=4), dict(particle_fuse=0, lbm_steps=2)):
+                 dict(particle_fuse=1, lbm_steps=2, lbm2_seg=0, lbm2_tile_h=4),
+                 dict(particle_fuse=1, lbm_steps=2, lbm2_seg=16, lbm2_xcd=0), dict(particle_fuse=0, lbm_steps=2)):
         g = rbn.Solver(p)
         g.set_tuning(**tune)
         g.collect_records()
@@ -132,12 +133,16 @@ def test_particle_fuse_fp32_matches_separate_launches(writefull):
 
 
 @pytest.mark.parametrize("prec", [rbn.RBN_FP64, rbn.RBN_FP32])
-@pytest.mark.parametrize("shape,tile_h", [((200, 44), 4), ((200, 44), 8), ((200, 44), 16), ((40, 20), 8),
-                                          ((128, 16), 16)])
-def test_two_step_launch_is_two_steps(prec, shape, tile_h):
-    # Partial tiles in x and y, a lattice narrower than one tile (the ring wraps onto the
-    # tile itself), energy diagnostics every 7 steps (a diagnostics step never starts a pair),
-    # calls of odd and even lengths.
+@pytest.mark.parametrize("shape,tile_h,seg,variant", [((200, 44), 4, 0, 0), ((200, 44), 8, 0, 0), ((200, 44), 16, 0, 0),
+                                                      ((40, 20), 8, 0, 0), ((128, 16), 16, 0, 0), ((200, 44), 0, 8, 0),
+                                                      ((300, 37), 0, 13, 1), ((40, 20), 0, 64, 0), ((520, 16), 0, 32, 1),
+                                                      ((257, 24), 0, 8, 0), ((64, 70), 0, 64, 1), ((200, 44), 0, 43, 1)])
+def test_two_step_launch_is_two_steps(prec, shape, tile_h, seg, variant):
+    # Tiles (seg 0): partial tiles in x and y, a lattice narrower than one tile (the ring
+    # wraps onto the tile itself).  Row sweeps: partial strips (down to one column, whose
+    # halo columns are its periodic neighbours), a lattice narrower than a strip, segments
+    # that do not divide the rows, a single segment.  Energy diagnostics every 7 steps (a
+    # diagnostics step never starts a pair), calls of odd and even lengths.
     nx, ny = shape
     p = rb_params(nx, ny, 1e5, particle=0, nout=7)
     p.precision = prec
@@ -145,7 +150,7 @@ def test_two_step_launch_is_two_steps(prec, shape, tile_h):
     runs = []
     for steps in (1, 2):
         s = rbn.Solver(p)
-        s.set_tuning(lbm_steps=steps, lbm2_tile_h=tile_h)
+        s.set_tuning(lbm_steps=steps, lbm2_tile_h=tile_h, lbm2_seg=seg, lbm2_variant=variant)
         s.collect_diagnostics()
         s.init_fields(T=T0)
         for n in (23, 8, 1, 2, 9):
