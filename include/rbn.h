@@ -216,6 +216,13 @@ int rbn_init_observers(rbn_ctx *ctx, int n, const int *id, const double *x, cons
 /* Source of trajectory records for read_prtcl (Nudging/code/tempvel_nudge.c:1380-1476):
  * asked for record istep+ntempread-1 on steps with istep % ntempread == 1. */
 int rbn_set_obs_provider(rbn_ctx *ctx, rbn_obs_fn fn, void *user);
+/* The same records from caller-owned host arrays, served without a callback (the
+ * reference reads them from Prtcl_Veldata1/Particles%d.h5): record r of nrec is for step
+ * steps[r] (ascending) and holds n entries id[r*n + k], x[r*n + k], y[r*n + k], T[r*n + k]
+ * in record storage order.  The arrays must stay valid while the context steps; replaces
+ * the provider. */
+int rbn_set_obs_table(rbn_ctx *ctx, int nrec, int n, const int *steps, const int *id, const double *x,
+                      const double *y, const double *T);
 /* Nudging state of the latest step: nudge_vol [nx][ny] counts, tempreadint [nx][ny]
  * running means, nud_term [nx][ny]; any pointer may be NULL. */
 int rbn_get_nudging(rbn_ctx *ctx, int *nudge_vol, double *tempreadint, double *nud_term);
@@ -238,13 +245,16 @@ int rbn_get_step(const rbn_ctx *ctx, int *istep);
 int rbn_synchronize(rbn_ctx *ctx);
 
 /* ---------------------------------------------------------------- batched ensemble */
-/* Many independent truth-run contexts of uneven sizes and parameters advanced together: one
- * ensemble step is a persistent launch over a work list of every member's particle chunks
- * (the update of the previous step) and lattice tiles, and six steps are captured in one
- * hipGraph that is replayed.  No reference counterpart (the reference runs one lattice per
- * job).  Members must share device and precision, be initialised truth-run contexts, and
- * stay owned by the caller; results equal stepping each member with rbn_step, except that
- * no trajectory records or diagnostics are produced during ensemble steps. */
+/* Many independent contexts of uneven sizes and parameters advanced together: one ensemble
+ * step is a persistent launch over a work list of every member's particle chunks (the update
+ * of the previous step) and lattice tiles, and cycles of launches are captured in one
+ * hipGraph that is replayed.  Nudged members add, per step, their record loads (from their
+ * providers, between launches) and one launch per observer/deposit pass over all of them;
+ * such ensembles step eagerly, one launch per step.  No reference counterpart (the
+ * reference runs one lattice per job).  Members must share device and precision, be
+ * initialised, and stay owned by the caller; results equal stepping each member with
+ * rbn_step, except that no trajectory records or diagnostics are produced during ensemble
+ * steps. */
 typedef struct rbn_ensemble rbn_ensemble;
 int rbn_ensemble_create(rbn_ctx *const *members, int n, rbn_ensemble **out);
 void rbn_ensemble_destroy(rbn_ensemble *e);
@@ -263,7 +273,9 @@ int rbn_ensemble_set_tuning(rbn_ensemble *e, int tile_w, int tile_h, int wg_per_
 int rbn_ensemble_get_tuning(const rbn_ensemble *e, int *tile_w, int *tile_h, int *wg_per_cu,
                             int *order);
 /* Time every candidate launch shape on trial_steps real steps, keep the fastest, and restore
- * every member's state afterwards; *best_ms_per_step receives the winner's time. */
+ * every member's state afterwards; *best_ms_per_step receives the winner's time.  With
+ * nudged members every candidate steps from the starting state (their records must cover
+ * 2 * max(trial_steps, 6) steps from it). */
 int rbn_ensemble_autotune(rbn_ensemble *e, int trial_steps, double *best_ms_per_step);
 
 /* ---------------------------------------------------------------- adjoint window */

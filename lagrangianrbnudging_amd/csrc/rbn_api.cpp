@@ -325,8 +325,8 @@ int run_energy(rbn_ctx *c, double out[2]) {
 }
 
 // Fetch record `rstep` through the provider, map observer ids to record slots (the first
-// match, as the reference's search N:1480-1487) and upload it.
-int load_observation(rbn_ctx *c, int rstep) {
+// match, as the reference's search N:1480-1487) and upload it on stream s.
+int load_observation(rbn_ctx *c, int rstep, hipStream_t s_up) {
   const int n = c->nobs;
   if (!c->obs_fn) return fail(c, RBN_ESTATE, "no observation provider set (record %d needed)", rstep);
   rbn_ctx::ObsSlot &s = c->obs_slot[c->obs_next];
@@ -383,11 +383,13 @@ int load_observation(rbn_ctx *c, int rstep) {
   }
   rbn::ObsPtrs o = obs_ptrs(c);
   // rx, ry, rT are consecutive arrays on the device and in the slot
-  HIPCHK(c, hipMemcpyAsync((void *)o.rx, rx, 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, hipMemcpyAsync((void *)o.chk, chk, (size_t)n * sizeof(int), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, hipEventRecord(s.copied, c->stream));
+  HIPCHK(c, hipMemcpyAsync((void *)o.rx, rx, 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice, s_up));
+  HIPCHK(c, hipMemcpyAsync((void *)o.chk, chk, (size_t)n * sizeof(int), hipMemcpyHostToDevice, s_up));
+  HIPCHK(c, hipEventRecord(s.copied, s_up));
   return RBN_OK;
 }
+
+int load_observation(rbn_ctx *c, int rstep) { return load_observation(c, rstep, c->stream); }
 
 int init_common_after_fields(rbn_ctx *c) {
   c->istep = c->P.nstart;
@@ -899,6 +901,38 @@ int rbn_set_obs_provider(rbn_ctx *c, rbn_obs_fn fn, void *user) {
   if (!c) return rbn::thread_fail(RBN_EINVAL, "null context");
   c->obs_fn = fn;
   c->obs_user = user;
+  return RBN_OK;
+}
+
+namespace {
+
+// The provider over an observation table: record `step` by binary search of the ascending
+// steps, copied out as stored.
+int table_provider(void *user, int step, int n, int *id, double *x, double *y, double *T) {
+  const rbn_ctx::ObsTable &t = *(const rbn_ctx::ObsTable *)user;
+  if (n != t.n) return 1;
+  const int *end = t.steps + t.nrec, *it = std::lower_bound(t.steps, end, step);
+  if (it == end || *it != step) return 1;
+  const size_t r = (size_t)(it - t.steps) * (size_t)n;
+  std::copy(t.id + r, t.id + r + n, id);
+  std::copy(t.x + r, t.x + r + n, x);
+  std::copy(t.y + r, t.y + r + n, y);
+  std::copy(t.T + r, t.T + r + n, T);
+  return 0;
+}
+
+}  // namespace
+
+int rbn_set_obs_table(rbn_ctx *c, int nrec, int n, const int *steps, const int *id, const double *x,
+                      const double *y, const double *T) {
+  if (!c) return rbn::thread_fail(RBN_EINVAL, "null context");
+  if (nrec < 0 || n < 0 || (nrec > 0 && (!steps || (n > 0 && (!id || !x || !y || !T)))))
+    return fail(c, RBN_EINVAL, "rbn_set_obs_table: bad arguments");
+  for (int r = 1; r < nrec; r++)
+    if (steps[r] <= steps[r - 1]) return fail(c, RBN_EINVAL, "rbn_set_obs_table: record steps must ascend");
+  c->obs_table = {nrec, n, steps, id, x, y, T};
+  c->obs_fn = table_provider;
+  c->obs_user = &c->obs_table;
   return RBN_OK;
 }
 

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "rbn_ensemble.h"
+#include "rbn_host.h"
 #include "rbn_internal.h"
 
 struct rbn_ensemble {
@@ -22,6 +23,11 @@ struct rbn_ensemble {
   int n_lbm = 0, n_part = 0, n_lbm2 = 0;
   int *d_counters = nullptr;  // [2 sets][2 phases]
   int cus = 1;
+  // nudged members: their observer/deposit descriptors and per-pass block prefixes
+  std::vector<int> nudged;  // member indices
+  void *d_nmembers = nullptr;
+  int *d_npre = nullptr;
+  int ntot[rbn::kNudgeBatches] = {};
   // tuning
   int tile_w = 256, tile_h = 1, wg_per_cu = 4, order = 0;
   bool built = false;
@@ -77,8 +83,44 @@ void fill_member(const rbn_ctx *c, int tile_w, int tile_h, rbn::EnsMember<R> &d)
   d.t2x = (c->geom.nx + 63) / 64;
   d.t2y = (c->geom.ny + th2 - 1) / th2;
   d.part = c->d_part;
-  d.np = (c->P.particle && c->d_part) ? c->np : 0;
+  d.np = (c->P.particle && c->d_part && c->P.variant != RBN_VARIANT_NUDGING) ? c->np : 0;
   d.err = c->d_err;
+  d.variant = c->P.variant;
+  if (c->P.variant == RBN_VARIANT_NUDGING) {
+    d.nvol = c->dep.cnt;
+    d.Tbar = c->dep.Tbar;
+    d.wq = c->dep.mode == RBN_DEPOSIT_BILINEAR ? c->dep.wq : nullptr;
+    d.nud = (R *)c->nud;
+  }
+}
+
+// Observer/deposit descriptors of the nudged members as of their current step, and the block
+// prefixes of the batched passes.
+int build_nudged(rbn_ensemble *e) {
+  const int nm = (int)e->nudged.size();
+  if (nm == 0) return RBN_OK;
+  std::vector<rbn::NudgeMember> d((size_t)nm);
+  std::vector<int> pre((size_t)rbn::kNudgeBatches * (nm + 1), 0);
+  for (int q = 0; q < nm; q++) {
+    rbn_ctx *c = e->m[(size_t)e->nudged[(size_t)q]];
+    rbn::NudgeMember &x = d[(size_t)q];
+    x.g = c->geom;
+    x.right = c->P.fix_q3 ? c->P.nx - 3 : c->P.nx - 2;
+    x.ntempread = c->P.ntempread;
+    x.istep0 = c->istep;
+    x.o = rbn::host::obs_ptrs(c);
+    x.d = c->dep;
+    int b[rbn::kNudgeBatches];
+    rbn::nudge_batch_blocks(x, b);
+    for (int p = 0; p < rbn::kNudgeBatches; p++)
+      pre[(size_t)p * (nm + 1) + q + 1] = pre[(size_t)p * (nm + 1) + q] + b[p];
+  }
+  for (int p = 0; p < rbn::kNudgeBatches; p++) e->ntot[p] = pre[(size_t)p * (nm + 1) + nm];
+  if (!e->d_nmembers) EHIP(e, hipMalloc(&e->d_nmembers, d.size() * sizeof(rbn::NudgeMember)));
+  if (!e->d_npre) EHIP(e, hipMalloc((void **)&e->d_npre, pre.size() * sizeof(int)));
+  EHIP(e, hipMemcpy(e->d_nmembers, d.data(), d.size() * sizeof(rbn::NudgeMember), hipMemcpyHostToDevice));
+  EHIP(e, hipMemcpy(e->d_npre, pre.data(), pre.size() * sizeof(int), hipMemcpyHostToDevice));
+  return RBN_OK;
 }
 
 // Descriptors and work lists for the current tuning and member state.
@@ -135,6 +177,8 @@ int build(rbn_ensemble *e) {
   EHIP(e, hipMalloc((void **)&e->d_items, std::max<size_t>(items.size() * sizeof(int2), 64)));
   EHIP(e, hipMemcpy(e->d_members, desc.data(), desc.size(), hipMemcpyHostToDevice));
   EHIP(e, hipMemcpy(e->d_items, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice));
+  int rc = build_nudged(e);
+  if (rc) return rc;
   e->built = true;
   return RBN_OK;
 }
@@ -142,12 +186,13 @@ int build(rbn_ensemble *e) {
 int grid_size(const rbn_ensemble *e) { return e->wg_per_cu > 0 ? e->wg_per_cu * e->cus : 0; }
 
 // order bit 2: separate lattice and particle launches (the particle launch on a second
-// stream beside the next lattice launch) instead of one launch per step.
-bool separate(const rbn_ensemble *e) { return (e->order & 4) != 0; }
+// stream beside the next lattice launch) instead of one launch per step.  Not with nudged
+// members (one launch per step).
+bool separate(const rbn_ensemble *e) { return (e->order & 4) != 0 && e->nudged.empty(); }
 
 // order bit 3 (without bit 2): two steps per launch on two-step tiles, the previous launch's
-// particle updates carried.
-bool pairs(const rbn_ensemble *e) { return (e->order & 8) != 0 && !separate(e); }
+// particle updates carried.  Not with nudged members (a nudged step has no two-step form).
+bool pairs(const rbn_ensemble *e) { return (e->order & 8) != 0 && !separate(e) && e->nudged.empty(); }
 
 // Steps of one replay of the captured graph: the velocity ring's period, doubled in the
 // two-step mode so that the A/B and F planes (flipped once per launch) also return.
@@ -258,6 +303,7 @@ int capture(rbn_ensemble *e) {
 void advance_members(rbn_ensemble *e, int n, int launches, int pairs) {
   for (rbn_ctx *c : e->m) {
     c->istep += n;
+    if (n > 0) c->at_init = false;
     if (launches & 1) {
       std::swap(c->fA, c->fB);
       std::swap(c->gA, c->gB);
@@ -285,6 +331,30 @@ int run_steps(rbn_ensemble *e, int nsteps) {
   // mid-cycle do not leave set 0 clear for a cycle that starts over: clear both here
   // (an ordinary stream operation, never part of the captured graph).
   EHIP(e, hipMemsetAsync(e->d_counters, 0, 4 * sizeof(int), e->stream));
+  for (int q : e->nudged)
+    if (!e->m[(size_t)q]->d_obs_i) return efail(e, RBN_ESTATE, "member %d: nudged run without observers", q);
+  if (!e->nudged.empty()) {
+    // With nudged members every step runs eagerly: the record loads of the members whose
+    // record interval starts (host provider, upload on this stream), the batched observer
+    // update and deposit, then the lattice launch carrying the previous step's particle
+    // updates; the last step's updates run on their own.
+    for (int t = 0; t < nsteps; t++) {
+      for (int q : e->nudged) {
+        rbn_ctx *c = e->m[(size_t)q];
+        const int istep = c->istep + t + 1, ntr = c->P.ntempread;
+        if (istep % ntr == 1 && (rc = rbn::host::load_observation(c, istep + ntr - 1, e->stream)))
+          return efail(e, rc, "member %d: %s", q, rbn_last_error(c));
+      }
+      EHIP(e, rbn::launch_nudge_batch(e->stream, (const rbn::NudgeMember *)e->d_nmembers, e->d_npre, e->ntot,
+                                      (int)e->nudged.size(), t));
+      if ((rc = enqueue_fused(e, t, t == nsteps - 1))) return rc;
+    }
+    if (e->n_part > 0 && (rc = launch(e, e->stream, 1, (nsteps - 1) % rbn::kCycle, false))) return rc;
+    EHIP(e, hipStreamSynchronize(e->stream));
+    advance_members(e, nsteps, nsteps, 0);
+    e->built = false;  // the observer descriptors hold the members' step counts
+    return RBN_OK;
+  }
   int launches = nsteps, npairs = 0;
   if (pairs(e)) {
     // launch 0 eagerly, whole cycles of kCycle two-step launches from the graph, the rest
@@ -363,6 +433,18 @@ int take_snapshot(rbn_ensemble *e, Snapshot &s) {
     std::vector<std::pair<void *, size_t>> parts = {{c->fA, 9 * fb}, {c->gA, 9 * fb}, {c->F, fb}, {c->F2, fb}};
     for (int k = 0; k < rbn_ctx::kVel; k++) parts.push_back({c->uv[k], 2 * fb});
     if (c->d_part && c->np > 0) parts.push_back({c->d_part, 10 * (size_t)c->np * sizeof(double)});
+    if (c->P.variant == RBN_VARIANT_NUDGING) {
+      // observers, nudging term and the latest deposit
+      const size_t no = (size_t)c->nobs, plane = c->geom.plane;
+      if (no > 0) {
+        parts.push_back({c->d_obs_i, rbn::host::ONI * no * sizeof(int)});
+        parts.push_back({c->d_obs_d, rbn::host::OND * no * sizeof(double)});
+      }
+      parts.push_back({c->nud, fb});
+      parts.push_back({c->dep.cnt, plane * sizeof(int)});
+      parts.push_back({c->dep.Tbar, plane * sizeof(double)});
+      if (c->dep.wq) parts.push_back({c->dep.wq, plane * sizeof(double)});
+    }
     for (auto &p : parts) {
       void *b = nullptr;
       EHIP(e, hipMalloc(&b, p.second));
@@ -379,7 +461,8 @@ int take_snapshot(rbn_ensemble *e, Snapshot &s) {
   return RBN_OK;
 }
 
-int restore_snapshot(rbn_ensemble *e, Snapshot &s) {
+// Put the saved state back; keep: the snapshot stays for another restore.
+int restore_snapshot(rbn_ensemble *e, Snapshot &s, bool keep = false) {
   for (size_t i = 0; i < e->m.size(); i++) {
     rbn_ctx *c = e->m[i];
     if (c->fA != s.fA[i]) {
@@ -392,9 +475,9 @@ int restore_snapshot(rbn_ensemble *e, Snapshot &s) {
   }
   for (size_t k = 0; k < s.bufs.size(); k++) {
     EHIP(e, hipMemcpy(s.src[k].first, s.bufs[k], s.src[k].second, hipMemcpyDeviceToDevice));
-    (void)hipFree(s.bufs[k]);
+    if (!keep) (void)hipFree(s.bufs[k]);
   }
-  s.bufs.clear();
+  if (!keep) s.bufs.clear();
   e->built = false;
   return RBN_OK;
 }
@@ -412,6 +495,8 @@ void rbn_ensemble_destroy(rbn_ensemble *e) {
   if (e->d_members) (void)hipFree(e->d_members);
   if (e->d_items) (void)hipFree(e->d_items);
   if (e->d_counters) (void)hipFree(e->d_counters);
+  if (e->d_nmembers) (void)hipFree(e->d_nmembers);
+  if (e->d_npre) (void)hipFree(e->d_npre);
   if (e->ev_lbm) (void)hipEventDestroy(e->ev_lbm);
   for (hipEvent_t ev : e->ev_part)
     if (ev) (void)hipEventDestroy(ev);
@@ -427,8 +512,6 @@ int rbn_ensemble_create(rbn_ctx *const *members, int n, rbn_ensemble **out) {
     const rbn_ctx *c = members[i];
     if (!c) return rbn::thread_fail(RBN_EINVAL, "null member");
     if (!c->initialised) return rbn::thread_fail(RBN_ESTATE, "member not initialised");
-    if (c->P.variant != RBN_VARIANT_TRAJECTORIES)
-      return rbn::thread_fail(RBN_EINVAL, "ensemble members must be truth-run (trajectories) contexts");
     if (c->dev != members[0]->dev || c->prec != members[0]->prec)
       return rbn::thread_fail(RBN_EINVAL, "ensemble members must share device and precision");
     for (int j = 0; j < i; j++)
@@ -438,6 +521,8 @@ int rbn_ensemble_create(rbn_ctx *const *members, int n, rbn_ensemble **out) {
   e->dev = members[0]->dev;
   e->prec = members[0]->prec;
   e->m.assign(members, members + n);
+  for (int i = 0; i < n; i++)
+    if (members[i]->P.variant == RBN_VARIANT_NUDGING) e->nudged.push_back(i);
   if (hipSetDevice(e->dev) != hipSuccess ||
       hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&e->pstream, hipStreamNonBlocking) != hipSuccess ||
@@ -488,6 +573,10 @@ int rbn_ensemble_step(rbn_ensemble *e, int nsteps) {
     int h = 0;
     EHIP(e, hipMemcpy(&h, c->d_err, sizeof(int), hipMemcpyDeviceToHost));
     if (h) return efail(e, RBN_ERANGE, "a particle of an ensemble member left its lattice");
+    if (c->P.variant == RBN_VARIANT_NUDGING && c->dep.overflow) {
+      EHIP(e, hipMemcpy(&h, c->dep.overflow, sizeof(int), hipMemcpyDeviceToHost));
+      if (h) return efail(e, RBN_ERANGE, "deposit list overflow in a nudged ensemble member");
+    }
   }
   return RBN_OK;
 }
@@ -501,12 +590,15 @@ int rbn_ensemble_autotune(rbn_ensemble *e, int trial_steps, double *best_ms_per_
   std::vector<Cand> cands;
   const int shapes[][2] = {{256, 1}, {256, 2}, {256, 4}, {128, 2}, {128, 4}, {64, 4}, {64, 8}};
   const int wpcs[] = {0, 2, 4, 8};
+  // with nudged members only the one-launch-per-step orders apply
+  const int norders = e->nudged.empty() ? 8 : 4;
   for (auto &sh : shapes)
     for (int w : wpcs)
-      for (int o = 0; o < 8; o++) cands.push_back({sh[0], sh[1], w, o});
+      for (int o = 0; o < norders; o++) cands.push_back({sh[0], sh[1], w, o});
   // two steps per launch: the tile is fixed (64 x 8 in FP32, 64 x 4 in FP64)
-  for (int w : wpcs)
-    for (int o = 8; o < 12; o++) cands.push_back({256, 1, w, o});
+  if (e->nudged.empty())
+    for (int w : wpcs)
+      for (int o = 8; o < 12; o++) cands.push_back({256, 1, w, o});
   int rc = sync_members(e);
   if (rc) return rc;
   Snapshot snap;
@@ -520,6 +612,23 @@ int rbn_ensemble_autotune(rbn_ensemble *e, int trial_steps, double *best_ms_per_
     if ((rc = rbn_ensemble_set_tuning(e, c.tw, c.th, c.wpc, c.order))) break;
     const int gs = graph_steps(e);
     const int reps = std::max(1, (trial_steps + gs - 1) / gs), trial = reps * gs;
+    if (!e->nudged.empty()) {
+      // eager steps (nudged members): time a call of `trial` steps after a warm-up call,
+      // then restore, so that every candidate replays the same steps (and records)
+      if ((rc = run_steps(e, gs))) break;
+      EHIP(e, hipEventRecord(a, e->stream));
+      if ((rc = run_steps(e, trial))) break;
+      EHIP(e, hipEventRecord(b, e->stream));
+      EHIP(e, hipEventSynchronize(b));
+      if ((rc = restore_snapshot(e, snap, true))) break;
+      float ms = 0.f;
+      EHIP(e, hipEventElapsedTime(&ms, a, b));
+      if (ms / trial < best) {
+        best = ms / trial;
+        bestc = c;
+      }
+      continue;
+    }
     if ((rc = run_steps(e, gs))) break;  // build, warm (a whole cycle: descriptors stay valid)
     if (!e->built && (rc = build(e))) break;
     if (!e->exec && (rc = capture(e))) break;

@@ -30,6 +30,12 @@ struct EnsMember {
   double *part;  // 10 arrays of np (x, y, u, v, xo, yo, uo, vo, 1/tau, beta)
   int np;
   int *err;
+  // nudged members (variant RBN_VARIANT_NUDGING): the deposit the step reads and the
+  // persistent nudging term; wq only for the bilinear deposit
+  int variant;
+  const int *nvol;
+  const double *Tbar, *wq;
+  R *nud;
 };
 
 // items: the n_lbm lattice items, then the n_part particle items.  phase 0: the lattice items

@@ -52,6 +52,22 @@ __device__ __forceinline__ void lbm_item(const EnsMember<R> &m, int tile, int k,
   p.rho = write_moments ? m.rho : nullptr;
   p.T = write_moments ? m.T : nullptr;
   const int col = threadIdx.x % tile_w;
+  if (m.variant == RBN_VARIANT_NUDGING) {
+    p.nvol = m.nvol;
+    p.Tbar = m.Tbar;
+    p.wq = m.wq;
+    p.nud = m.nud;
+    for (int r = threadIdx.x / tile_w; r < tile_h; r += kThreads / tile_w) {
+      const int i = tx * tile_w + col, j = ty * tile_h + r;
+      if (i < m.g.nx && j < m.g.ny) {
+        if constexpr (sizeof(R) == 8)
+          lbm::lbm_node64<RBN_VARIANT_NUDGING, 1>(m.g, p, m.c, i, j);
+        else
+          lbm::lbm_node32<RBN_VARIANT_NUDGING, 1>(m.g, p, m.c, i, j);
+      }
+    }
+    return;
+  }
   for (int r = threadIdx.x / tile_w; r < tile_h; r += kThreads / tile_w) {
     const int i = tx * tile_w + col, j = ty * tile_h + r;
     if (i < m.g.nx && j < m.g.ny) {

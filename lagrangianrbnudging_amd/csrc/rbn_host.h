@@ -81,8 +81,9 @@ ObsPtrs obs_view(int n, int *ints, double *dbls);
 ObsPtrs obs_ptrs(rbn_ctx *c);
 
 // Fetch record rstep from the observation provider, map observers to record entries by id
-// and upload it as the current record.
+// and upload it as the current record (on the context's stream, or on s).
 int load_observation(rbn_ctx *c, int rstep);
+int load_observation(rbn_ctx *c, int rstep, hipStream_t s);
 
 // Report (and clear) a particle / observer range error or a deposit overflow.
 int check_err_flag(rbn_ctx *c);

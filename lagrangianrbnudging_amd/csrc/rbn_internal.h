@@ -88,6 +88,18 @@ struct DepPtrs {
   int *overflow;
 };
 
+// One nudged member of a batched observer/deposit launch (ensembles): its lattice, the
+// deposit's right limit, record interval, the step count before the launching call, its
+// observers and deposit arrays.
+struct NudgeMember {
+  Geom g;
+  int right, ntempread, istep0;
+  ObsPtrs o;
+  DepPtrs d;
+};
+// Passes with per-member block counts (the prefix rows of a batched launch).
+enum { NB_OBS, NB_ZERO, NB_CAND, NB_SCAN, NB_FOLD, kNudgeBatches };
+
 // Launch shape of the fused step: block width (64/128/256/512 nodes), non-temporal stores of
 // the outputs / loads of the inputs, XCD-banded block order, alternating sweep direction.
 // Defaults: the fastest measured on MI355X at 4096x2048 in both precisions (256 wide,
@@ -160,6 +172,13 @@ hipError_t launch_obs_update(hipStream_t s, int branch, int nx, int ny, int ntem
                              const ObsPtrs &o);
 hipError_t launch_deposit(hipStream_t s, const Geom &g, int right_limit, const ObsPtrs &o,
                           const DepPtrs &d);
+// Blocks member d takes in each pass of a batched observer/deposit launch.
+void nudge_batch_blocks(const NudgeMember &d, int blocks[kNudgeBatches]);
+// Observer update and deposit of step t of a call (t = 0 is the first) for nm members at
+// once: members (device), pre (device, kNudgeBatches rows of nm + 1 block prefixes), totals
+// (host, the last entry of each row).
+hipError_t launch_nudge_batch(hipStream_t s, const NudgeMember *members, const int *pre, const int *totals, int nm,
+                              int t);
 hipError_t launch_energy(hipStream_t s, int prec, const Geom &g, const void *rho, const void *uv,
                          const void *T, double *partial, int nblocks, double *out2);
 int energy_blocks(const Geom &g);
@@ -237,6 +256,12 @@ struct rbn_ctx {
   double *d_obs_d = nullptr;  // dx, dy, dT, xint, yint, Tint, rx, ry, rT (9 * nobs)
   rbn_obs_fn obs_fn = nullptr;
   void *obs_user = nullptr;
+  // rbn_set_obs_table: caller-owned records served by the built-in provider
+  struct ObsTable {
+    int nrec = 0, n = 0;
+    const int *steps = nullptr, *id = nullptr;
+    const double *x = nullptr, *y = nullptr, *T = nullptr;
+  } obs_table;
   // Record staging: two pinned slots (x, y, T as doubles, then ids and the observer map as
   // ints) so that the upload of a record is queued on the stream without waiting for it;
   // a slot is refilled only after the event of its previous upload.  The observer -> record

@@ -71,9 +71,7 @@ __device__ __forceinline__ void advance_y_T(const ObsPtrs &o, int i, double nyd)
 
 // branch: 1 = istep%ntempread==1 (new record in rx/ry/rT, N:1489-1552), 0 = snap to the
 // record (N:1621-1628), 2 = advance by the slopes (N:1687-1720).
-__global__ void k_obs_update(ObsPtrs o, int branch, int nx, int ny, int ntempread) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= o.n) return;
+__device__ __forceinline__ void obs_update_one(const ObsPtrs &o, int i, int branch, int nx, int ny, int ntempread) {
   const double nxd = (double)nx, nyd = (double)ny, ntr = (double)ntempread;
   const int c = o.chk[i];
   if (branch == 1) {
@@ -116,6 +114,11 @@ __global__ void k_obs_update(ObsPtrs o, int branch, int nx, int ny, int ntemprea
     advance_x(o, i, nxd);
     advance_y_T(o, i, nyd);
   }
+}
+
+__global__ void k_obs_update(ObsPtrs o, int branch, int nx, int ny, int ntempread) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < o.n) obs_update_one(o, i, branch, nx, ny, ntempread);
 }
 
 // ---------------------------------------------------------------- deposit
@@ -168,13 +171,15 @@ __device__ __forceinline__ long long bil_cell(const Geom &g, double xx, double y
   return (long long)(J - 1) * g.pitch + (I - 1);
 }
 
-__global__ void k_dep_zero(size_t n, int *cnt, int *fill) {
-  for (size_t o = (size_t)blockIdx.x * blockDim.x + threadIdx.x; o < n;
-       o += (size_t)gridDim.x * blockDim.x) {
+// Zero the counts and fill cursors: block b of nb, grid-stride.
+__device__ __forceinline__ void zero_part(size_t n, int *cnt, int *fill, int b, int nb) {
+  for (size_t o = (size_t)b * kBlock + threadIdx.x; o < n; o += (size_t)nb * kBlock) {
     cnt[o] = 0;
     fill[o] = 0;
   }
 }
+
+__global__ void k_dep_zero(size_t n, int *cnt, int *fill) { zero_part(n, cnt, fill, blockIdx.x, gridDim.x); }
 
 // Node of candidate c of observer k for deposit MODE (-1: none); *key is the list key.
 template <int MODE>
@@ -189,21 +194,24 @@ __device__ __forceinline__ long long candidate(const Geom &g, int right, const O
   return cell_of(g, right, o.xidx[k], o.yidx[k], c);
 }
 
+// Candidate t (observer t / NC, cell t % NC) counts one contribution at its node.
 template <int MODE>
-__global__ void k_dep_count(Geom g, int right, ObsPtrs o, int *cnt) {
+__device__ __forceinline__ void count_one(const Geom &g, int right, const ObsPtrs &o, int *cnt, long long t) {
   constexpr int NC = MODE == RBN_DEPOSIT_BILINEAR ? kBilCells : kCells;
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long long)o.n * NC) return;
   int key;
   const long long off = candidate<MODE>(g, right, o, (int)(t / NC), (int)(t % NC), &key);
   if (off >= 0) atomicAdd(&cnt[off], 1);
 }
 
-// Exclusive scan, pass 1: tile-local scans and tile totals.
-__global__ __launch_bounds__(kBlock) void k_scan_tiles(size_t n, const int *in, int *out,
-                                                       int *tile_sum) {
-  __shared__ int s[kBlock];
-  const size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
+template <int MODE>
+__global__ void k_dep_count(Geom g, int right, ObsPtrs o, int *cnt) {
+  count_one<MODE>(g, right, o, cnt, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// Exclusive scan, pass 1: tile-local scans and tile totals (tile b; s: kBlock ints of LDS).
+__device__ __forceinline__ void scan_tile(size_t n, const int *in, int *out, int *tile_sum, int b, int *s) {
+  const size_t base = (size_t)b * kScanTile + (size_t)threadIdx.x * kScanItems;
   int v[kScanItems], t = 0;
 #pragma unroll
   for (int q = 0; q < kScanItems; q++) {
@@ -224,13 +232,18 @@ __global__ __launch_bounds__(kBlock) void k_scan_tiles(size_t n, const int *in, 
     if (base + q < n) out[base + q] = run;
     run += v[q];
   }
-  if (threadIdx.x == kBlock - 1) tile_sum[blockIdx.x] = s[kBlock - 1];
+  if (threadIdx.x == kBlock - 1) tile_sum[b] = s[kBlock - 1];
 }
 
-// Pass 2: exclusive scan of the tile totals by one block; total written to *total.
-__global__ __launch_bounds__(kBlock) void k_scan_sums(int ntiles, int *tile_sum, int *total) {
+__global__ __launch_bounds__(kBlock) void k_scan_tiles(size_t n, const int *in, int *out,
+                                                       int *tile_sum) {
   __shared__ int s[kBlock];
-  __shared__ int carry;
+  scan_tile(n, in, out, tile_sum, blockIdx.x, s);
+}
+
+// Pass 2: exclusive scan of the tile totals by one block; total written to *total (s:
+// kBlock ints of LDS, carry: one).
+__device__ __forceinline__ void scan_sums(int ntiles, int *tile_sum, int *total, int *s, int &carry) {
   if (threadIdx.x == 0) carry = 0;
   __syncthreads();
   for (int base = 0; base < ntiles; base += kBlock) {
@@ -252,16 +265,22 @@ __global__ __launch_bounds__(kBlock) void k_scan_sums(int ntiles, int *tile_sum,
   if (threadIdx.x == 0) *total = carry;
 }
 
+__global__ __launch_bounds__(kBlock) void k_scan_sums(int ntiles, int *tile_sum, int *total) {
+  __shared__ int s[kBlock];
+  __shared__ int carry;
+  scan_sums(ntiles, tile_sum, total, s, carry);
+}
+
 // Start of node c's slot range: tile-local exclusive prefix + the tile's exclusive prefix.
 __device__ __forceinline__ long long seg_start(const int *off, const int *tile_sum, size_t c) {
   return (long long)off[c] + tile_sum[c / kScanTile];
 }
 
 template <int MODE>
-__global__ void k_dep_fill(Geom g, int right, ObsPtrs o, const int *off, const int *tile_sum,
-                           int *fill, int *list, long long cap, int *overflow) {
+__device__ __forceinline__ void fill_one(const Geom &g, int right, const ObsPtrs &o, const int *off,
+                                         const int *tile_sum, int *fill, int *list, long long cap, int *overflow,
+                                         long long t) {
   constexpr int NC = MODE == RBN_DEPOSIT_BILINEAR ? kBilCells : kCells;
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long long)o.n * NC) return;
   int key;
   const long long node = candidate<MODE>(g, right, o, (int)(t / NC), (int)(t % NC), &key);
@@ -271,6 +290,13 @@ __global__ void k_dep_fill(Geom g, int right, ObsPtrs o, const int *off, const i
     list[slot] = key;
   else
     atomicOr(overflow, 1);
+}
+
+template <int MODE>
+__global__ void k_dep_fill(Geom g, int right, ObsPtrs o, const int *off, const int *tile_sum,
+                           int *fill, int *list, long long cap, int *overflow) {
+  fill_one<MODE>(g, right, o, off, tile_sum, fill, list, cap, overflow,
+                 (long long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // Per node: order the contributions by observer and fold them in that order, the
@@ -299,12 +325,9 @@ struct Fold {
 };
 
 template <int MODE>
-__global__ __launch_bounds__(kBlock) void k_dep_fold(Geom g, const int *cnt, const int *off,
-                                                     const int *tile_sum, int *list, ObsPtrs o,
-                                                     double *Tbar, double *wq, long long cap) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y;
-  if (i >= g.nx) return;
+__device__ __forceinline__ void fold_node(const Geom &g, const int *cnt, const int *off, const int *tile_sum,
+                                          int *list, const ObsPtrs &o, double *Tbar, double *wq, long long cap,
+                                          int i, int j) {
   const size_t c = (size_t)j * g.pitch + i;
   const int n = cnt[c];
   Fold<MODE> f{g, o};
@@ -348,6 +371,107 @@ __global__ __launch_bounds__(kBlock) void k_dep_fold(Geom g, const int *cnt, con
   }
 }
 
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_dep_fold(Geom g, const int *cnt, const int *off,
+                                                     const int *tile_sum, int *list, ObsPtrs o,
+                                                     double *Tbar, double *wq, long long cap) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < g.nx) fold_node<MODE>(g, cnt, off, tile_sum, list, o, Tbar, wq, cap, i, blockIdx.y);
+}
+
+// ---------------------------------------------------------------- batched (ensembles)
+//
+// One launch per pass over the nudged members of an ensemble: the blocks of pass p are the
+// members' blocks in member order, member m owning blocks pre[m] .. pre[m+1]-1 of the pass's
+// prefix row pre = pre_all + p * (nm + 1).  Each block runs the single-member body of its
+// member with its block index inside the member.
+
+__device__ __forceinline__ int member_of(const int *pre, int nm, int b) {
+  int lo = 0, hi = nm;  // pre[lo] <= b < pre[hi]; members without blocks are skipped
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (pre[mid] <= b)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ int nudge_blocks(const Geom &g) { return (g.nx + kBlock - 1) / kBlock; }
+
+__global__ __launch_bounds__(kBlock) void k_nb_obs(const NudgeMember *M, const int *pre_all, int nm, int t) {
+  const int *pre = pre_all + NB_OBS * (nm + 1);
+  const int m = member_of(pre, nm, blockIdx.x);
+  const NudgeMember &d = M[m];
+  const int i = (blockIdx.x - pre[m]) * kBlock + threadIdx.x;
+  // the step's place in the record interval, as the single-member loop derives it
+  const int istep = d.istep0 + t + 1, r = istep % d.ntempread;
+  const int branch = r == 1 ? 1 : (r == 0 ? 0 : 2);
+  if (i < d.o.n) obs_update_one(d.o, i, branch, d.g.nx, d.g.ny, d.ntempread);
+}
+
+__global__ __launch_bounds__(kBlock) void k_nb_zero(const NudgeMember *M, const int *pre_all, int nm) {
+  const int *pre = pre_all + NB_ZERO * (nm + 1);
+  const int m = member_of(pre, nm, blockIdx.x);
+  zero_part(M[m].g.plane, M[m].d.cnt, M[m].d.fill, blockIdx.x - pre[m], pre[m + 1] - pre[m]);
+}
+
+__global__ __launch_bounds__(kBlock) void k_nb_count(const NudgeMember *M, const int *pre_all, int nm) {
+  const int *pre = pre_all + NB_CAND * (nm + 1);
+  const int m = member_of(pre, nm, blockIdx.x);
+  const NudgeMember &d = M[m];
+  const long long t = (long long)(blockIdx.x - pre[m]) * kBlock + threadIdx.x;
+  if (d.d.mode == RBN_DEPOSIT_BILINEAR)
+    count_one<RBN_DEPOSIT_BILINEAR>(d.g, d.right, d.o, d.d.cnt, t);
+  else
+    count_one<RBN_DEPOSIT_BOX7>(d.g, d.right, d.o, d.d.cnt, t);
+}
+
+__global__ __launch_bounds__(kBlock) void k_nb_scan_tiles(const NudgeMember *M, const int *pre_all, int nm) {
+  __shared__ int s[kBlock];
+  const int *pre = pre_all + NB_SCAN * (nm + 1);
+  const int m = member_of(pre, nm, blockIdx.x);
+  const NudgeMember &d = M[m];
+  scan_tile(d.g.plane, d.d.cnt, d.d.off, d.d.blocksum, blockIdx.x - pre[m], s);
+}
+
+__global__ __launch_bounds__(kBlock) void k_nb_scan_sums(const NudgeMember *M) {
+  __shared__ int s[kBlock];
+  __shared__ int carry;
+  const NudgeMember &d = M[blockIdx.x];
+  const int ntiles = (int)((d.g.plane + kScanTile - 1) / kScanTile);
+  scan_sums(ntiles, d.d.blocksum, d.d.off + d.g.plane, s, carry);
+}
+
+__global__ __launch_bounds__(kBlock) void k_nb_fill(const NudgeMember *M, const int *pre_all, int nm) {
+  const int *pre = pre_all + NB_CAND * (nm + 1);
+  const int m = member_of(pre, nm, blockIdx.x);
+  const NudgeMember &d = M[m];
+  const long long t = (long long)(blockIdx.x - pre[m]) * kBlock + threadIdx.x;
+  if (d.d.mode == RBN_DEPOSIT_BILINEAR)
+    fill_one<RBN_DEPOSIT_BILINEAR>(d.g, d.right, d.o, d.d.off, d.d.blocksum, d.d.fill, d.d.list, d.d.list_cap,
+                                   d.d.overflow, t);
+  else
+    fill_one<RBN_DEPOSIT_BOX7>(d.g, d.right, d.o, d.d.off, d.d.blocksum, d.d.fill, d.d.list, d.d.list_cap,
+                               d.d.overflow, t);
+}
+
+__global__ __launch_bounds__(kBlock) void k_nb_fold(const NudgeMember *M, const int *pre_all, int nm) {
+  const int *pre = pre_all + NB_FOLD * (nm + 1);
+  const int m = member_of(pre, nm, blockIdx.x);
+  const NudgeMember &d = M[m];
+  const int lb = blockIdx.x - pre[m], bx = nudge_blocks(d.g);
+  const int j = lb / bx, i = (lb - j * bx) * kBlock + threadIdx.x;
+  if (i >= d.g.nx) return;
+  if (d.d.mode == RBN_DEPOSIT_BILINEAR)
+    fold_node<RBN_DEPOSIT_BILINEAR>(d.g, d.d.cnt, d.d.off, d.d.blocksum, d.d.list, d.o, d.d.Tbar, d.d.wq,
+                                    d.d.list_cap, i, j);
+  else
+    fold_node<RBN_DEPOSIT_BOX7>(d.g, d.d.cnt, d.d.off, d.d.blocksum, d.d.list, d.o, d.d.Tbar, d.d.wq,
+                                d.d.list_cap, i, j);
+}
+
 }  // namespace
 
 hipError_t launch_obs_update(hipStream_t s, int branch, int nx, int ny, int ntempread,
@@ -355,6 +479,29 @@ hipError_t launch_obs_update(hipStream_t s, int branch, int nx, int ny, int ntem
   if (o.n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_obs_update, dim3((o.n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, o,
                      branch, nx, ny, ntempread);
+  return hipGetLastError();
+}
+
+void nudge_batch_blocks(const NudgeMember &d, int blocks[kNudgeBatches]) {
+  const long long nc = (long long)d.o.n * (d.d.mode == RBN_DEPOSIT_BILINEAR ? kBilCells : kCells);
+  blocks[NB_OBS] = (d.o.n + kBlock - 1) / kBlock;
+  blocks[NB_ZERO] = (int)std::min<size_t>((d.g.plane + kBlock - 1) / kBlock, 1024);
+  blocks[NB_CAND] = (int)((nc + kBlock - 1) / kBlock);
+  blocks[NB_SCAN] = (int)((d.g.plane + kScanTile - 1) / kScanTile);
+  blocks[NB_FOLD] = (d.g.nx + kBlock - 1) / kBlock * d.g.ny;
+}
+
+hipError_t launch_nudge_batch(hipStream_t s, const NudgeMember *members, const int *pre, const int *totals, int nm,
+                              int t) {
+  if (nm <= 0) return hipSuccess;
+  const dim3 b(kBlock);
+  if (totals[NB_OBS] > 0) hipLaunchKernelGGL(k_nb_obs, dim3(totals[NB_OBS]), b, 0, s, members, pre, nm, t);
+  hipLaunchKernelGGL(k_nb_zero, dim3(totals[NB_ZERO]), b, 0, s, members, pre, nm);
+  if (totals[NB_CAND] > 0) hipLaunchKernelGGL(k_nb_count, dim3(totals[NB_CAND]), b, 0, s, members, pre, nm);
+  hipLaunchKernelGGL(k_nb_scan_tiles, dim3(totals[NB_SCAN]), b, 0, s, members, pre, nm);
+  hipLaunchKernelGGL(k_nb_scan_sums, dim3(nm), b, 0, s, members);
+  if (totals[NB_CAND] > 0) hipLaunchKernelGGL(k_nb_fill, dim3(totals[NB_CAND]), b, 0, s, members, pre, nm);
+  hipLaunchKernelGGL(k_nb_fold, dim3(totals[NB_FOLD]), b, 0, s, members, pre, nm);
   return hipGetLastError();
 }
 

@@ -114,6 +114,8 @@ _SIGS = {
     "rbn_set_record_callback": (C.c_int, [C.c_void_p, RECORD_FN, C.c_void_p]),
     "rbn_init_observers": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), _DP, _DP, _DP]),
     "rbn_set_obs_provider": (C.c_int, [C.c_void_p, OBS_FN, C.c_void_p]),
+    "rbn_set_obs_table": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                    C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "rbn_get_nudging": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), _DP, _DP]),
     "rbn_get_nudging_weights": (C.c_int, [C.c_void_p, _DP]),
     "rbn_set_diag_callback": (C.c_int, [C.c_void_p, DIAG_FN, C.c_void_p]),
@@ -381,6 +383,22 @@ class Solver:
         self._obs_cb = OBS_FN(cb)
         self._c(self.lib.rbn_set_obs_provider(self.ctx, self._obs_cb, None))
 
+    def set_observation_table(self, records: dict) -> None:
+        """As set_observation_source, served by the library from arrays kept alive here
+        (every record must hold the same number of entries)."""
+        steps = np.array(sorted(records), dtype=np.int32)
+        n = len(records[int(steps[0])]["id"]) if len(steps) else 0
+        cols = {}
+        for key, dt in (("id", np.int32), ("x", np.float64), ("y", np.float64), ("temp", np.float64)):
+            a = np.empty((len(steps), n), dtype=dt)
+            for r, st in enumerate(steps):
+                a[r] = records[int(st)][key]
+            cols[key] = np.ascontiguousarray(a)
+        self._obs_table = (steps, cols)
+        ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int))
+        self._c(self.lib.rbn_set_obs_table(self.ctx, len(steps), n, ip(steps), ip(cols["id"]), _dp(cols["x"]),
+                                           _dp(cols["y"]), _dp(cols["temp"])))
+
     def nudging(self) -> dict:
         nv = np.empty((self.nx, self.ny), dtype=np.int32)
         tb = np.empty((self.nx, self.ny))
@@ -447,7 +465,7 @@ class Solver:
 
 
 class Ensemble:
-    """Several truth-run Solvers advanced together (rbn_ensemble_*)."""
+    """Several Solvers (truth-run or nudged) advanced together (rbn_ensemble_*)."""
 
     def __init__(self, members: list):
         self.lib = load()

@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 from lagrangianrbnudging_amd import rbn
-from scenarios import assert_equal_arrays, conduction_T, random_particles, rb_params
+from scenarios import assert_equal_arrays, conduction_T, random_particles, rb_params, synthetic_observations
 
 pytestmark = pytest.mark.gpu
 
@@ -63,6 +63,101 @@ def test_ensemble_equals_individual_steps(prec, tuning):
     e.close()
 
 
+# Nudged members: (nx, ny, Ra, observers, ntempread, deposit, alpha)
+NUDGED = [(48, 24, 1e5, 40, 4, rbn.RBN_DEPOSIT_BOX7, 0.3), (64, 40, 3e5, 90, 5, rbn.RBN_DEPOSIT_BILINEAR, 1.0),
+          (40, 24, 5e4, 0, 5, rbn.RBN_DEPOSIT_BILINEAR, 0.1), (72, 32, 2e5, 25, 3, rbn.RBN_DEPOSIT_BOX7, 0.05)]
+
+
+def _mixed_members(prec, table):
+    """Two truth-run members with particles, then the nudged members (one without observers);
+    observations through the library's table (ensemble) or a Python provider (reference)."""
+    out = []
+    for k, (nx, ny, ra) in enumerate(SHAPES[:2]):
+        p = rb_params(nx, ny, ra, noutconfig=10 ** 6)
+        p.precision = prec
+        s = rbn.Solver(p)
+        s.init_fields(T=conduction_T(p, 1e-2, seed=60 + k))
+        ids, x, y, tau = random_particles(p, 30, 15, seed=70 + k)
+        s.init_particles(ids, x, y, tau_p=tau)
+        out.append(s)
+    for k, (nx, ny, ra, nobs, ntr, dep, alpha) in enumerate(NUDGED):
+        p = rb_params(nx, ny, ra, noutconfig=ntr)
+        p.precision = prec
+        p.variant = rbn.RBN_VARIANT_NUDGING
+        p.ntempread = ntr
+        p.alpha = alpha
+        p.deposit = dep
+        s = rbn.Solver(p)
+        s.init_fields(T=conduction_T(p, 1e-2, seed=80 + k))
+        if nobs > 0:
+            ids, x0, y0, T, recs = synthetic_observations(nx, ny, nobs, 240, ntr, seed=90 + k)
+            s.init_observers(ids, x0, y0, T)
+        else:  # no observers: records with no entries
+            none = {"id": np.zeros(0, np.int32), "x": np.zeros(0), "y": np.zeros(0), "temp": np.zeros(0)}
+            s.init_observers(none["id"], none["x"], none["y"], none["temp"])
+            recs = {st: none for st in range(ntr, 240 + ntr, ntr)}
+        if table:
+            s.set_observation_table(recs)
+        else:
+            s.set_observation_source(recs)
+        out.append(s)
+    return out
+
+
+def _compare_nudged(a, b):
+    for k, (sa, sb) in enumerate(zip(a, b)):
+        fa, ga = sa.populations()
+        fb, gb = sb.populations()
+        assert_equal_arrays(fa, fb, f"member {k} f")
+        assert_equal_arrays(ga, gb, f"member {k} g")
+        for key in ("rho", "T", "u", "v"):
+            assert_equal_arrays(sa.fields()[key], sb.fields()[key], f"member {k} {key}")
+        assert sa.istep == sb.istep
+        if k >= 2:
+            na, nb = sa.nudging(), sb.nudging()
+            for key in na:
+                assert_equal_arrays(na[key], nb[key], f"member {k} {key}")
+        else:
+            pa, pb = sa.particles(), sb.particles()
+            for key in ("x", "y", "u", "v", "xo", "yo", "uo", "vo"):
+                assert_equal_arrays(pa[key], pb[key], f"member {k} particle {key}")
+
+
+@pytest.mark.parametrize("prec", [rbn.RBN_FP64, rbn.RBN_FP32])
+@pytest.mark.parametrize("tuning", [(256, 1, 0, 0), (128, 4, 4, 3), (64, 8, 2, 1), (256, 1, 4, 10)])
+def test_ensemble_with_nudged_members_equals_individual_steps(prec, tuning):
+    # Nudged members (7x7 and bilinear deposits, different record intervals, one without
+    # observers) beside truth-run members; the observer updates and deposits of all nudged
+    # members run as batched launches, records are loaded between steps.  Two-step and
+    # separate-launch orders fall back to one launch per step.
+    ens_members = _mixed_members(prec, table=True)
+    ref_members = _mixed_members(prec, table=False)
+    e = rbn.Ensemble(ens_members)
+    e.set_tuning(*tuning)
+    calls = (9, 1, 2, 6, 13, 20)
+    for n in calls:
+        e.step(n)
+    for s in ref_members:
+        s.step(sum(calls))
+    _compare_nudged(ens_members, ref_members)
+    e.close()
+
+
+def test_ensemble_autotune_with_nudged_members_keeps_state():
+    ens_members = _mixed_members(rbn.RBN_FP32, table=True)
+    ref_members = _mixed_members(rbn.RBN_FP32, table=False)
+    e = rbn.Ensemble(ens_members)
+    assert e.autotune(4) > 0
+    assert e.tuning()["order"] < 4
+    for s in ens_members:
+        assert s.istep == 0
+    e.step(17)
+    for s in ref_members:
+        s.step(17)
+    _compare_nudged(ens_members, ref_members)
+    e.close()
+
+
 def test_ensemble_autotune_keeps_member_state():
     ens_members = _members(rbn.RBN_FP64)
     ref_members = _members(rbn.RBN_FP64)
@@ -89,12 +184,16 @@ def test_ensemble_rejects_unsupported_members():
     b.init_fields(T=conduction_T(q, 1e-2, 1))
     with pytest.raises(rbn.RbnError):
         rbn.Ensemble([a, b])
+    with pytest.raises(rbn.RbnError):
+        rbn.Ensemble([a, a])
     n = p.copy()
     n.variant = rbn.RBN_VARIANT_NUDGING
     n.ntempread = 4
     c = rbn.Solver(n)
     c.init_fields(T=conduction_T(n, 1e-2, 1))
+    ids, x0, y0, T, _ = synthetic_observations(32, 16, 10, 20, 4, seed=3)
+    c.init_observers(ids, x0, y0, T)
+    e = rbn.Ensemble([a, c])  # a nudged member without an observation source fails at its first record
     with pytest.raises(rbn.RbnError):
-        rbn.Ensemble([a, c])
-    with pytest.raises(rbn.RbnError):
-        rbn.Ensemble([a, a])
+        e.step(3)
+    e.close()
