@@ -16,8 +16,9 @@ dominant kernel (the two-step lattice launch; HIP events around launches in the 
 region; the HBM traffic per launch comes from the committed rocprofv3 counter summary under
 profiles/), the same workload at one step per launch and with the particle update as a
 launch of its own (the one-step lattice kernel's roofline alone),
-the FP64 path's MLUPS on the same workload, the batched ensemble (64 uneven members) against
-the same members stepped one by one, the nudging configuration (1024 x 512: the L2
+the FP64 path's MLUPS on the same workload, the batched ensemble (64 uneven members with
+1e3-1e4 nudged tracers each; also truth-run members advecting their tracers) against the
+same members stepped one by one, the nudging configuration (1024 x 512: the L2
 temperature error of nudged runs against the truth over 20,000 steps, and a sweep over the
 number of observed tracers and alpha), the adjoint window's cost against the plain nudged
 forward (512 x 256, 2000 steps, checkpoint every 100), and the CPU FP64 restatement of the
@@ -142,50 +143,95 @@ def run_gpu(rbn, precision, steps, warmup, timing=True, records_to_host=False, *
 
 ENS_SIZES = [(256, 128), (512, 256), (768, 384), (1024, 512)]
 ENS_MEMBERS = 64
+ENS_NTEMPREAD, ENS_ALPHA = 5, 0.1
 
 
-def run_ensemble(rbn, steps, warmup, tuning=None):
-    """64 FP32 truth-run members, sizes drawn from ENS_SIZES, Ra log-uniform in [1e6, 1e8],
-    1e3..1e4 tracers each, own seeds; batched ensemble (autotuned, or the given launch shape
-    (tile_w, tile_h, wg_per_cu, order); graph-captured) against the same members stepped one
-    after another with their own launches."""
-    rng = np.random.default_rng(7)
-    members, nodes = [], 0
+def ens_observations(rng, nx, ny, n, last_step, every):
+    """Synthetic observation records of n drifting tracers (straight lines, periodic in x,
+    folded into [1, ny] in y, temperatures drifting slowly), one every `every` steps up to
+    last_step: the initial state and a step -> record table."""
+    x0 = rng.uniform(0.0, nx, n)
+    y0 = rng.uniform(1.0, ny, n)
+    vx = rng.uniform(-0.05, 0.05, n)
+    vy = rng.uniform(-0.03, 0.03, n)
+    T0 = rng.uniform(0.0, 1.0, n)
+    ids = np.arange(n, dtype=np.int32)
+    recs = {}
+    for st in range(every, last_step + every, every):
+        recs[st] = {"id": ids, "x": np.mod(x0 + vx * st, nx), "y": 1.0 + np.mod(y0 + vy * st - 1.0, ny - 1.0),
+                    "temp": np.clip(T0 + 1e-4 * st, 0.0, 1.0)}
+    return ids, x0, y0, T0, recs
+
+
+def ens_members(rbn, nudged, last_step, seed=7):
+    """64 FP32 members, sizes drawn from ENS_SIZES, Ra log-uniform in [1e6, 1e8], own seeds,
+    1e3..1e4 tracers each: nudged members whose observers follow synthetic records (the
+    bilinear deposit, alpha 0.1, records every 5 steps), or truth runs advecting the tracers."""
+    rng = np.random.default_rng(seed)
+    members, nodes, nobs = [], 0, 0
     for m in range(ENS_MEMBERS):
         nx, ny = ENS_SIZES[int(rng.integers(len(ENS_SIZES)))]
         ra = 10.0 ** rng.uniform(6.0, 8.0)
+        n = int(rng.integers(1000, 10001))
         p = rbn.default_params(nx=nx, ny=ny, Tup=0.0, Tdown=1.0, scratch=1, nstart=0, nout=10**9,
-                               noutconfig=10**9, prtcl_writefull=0, particle=1,
+                               noutconfig=10**9, prtcl_writefull=0, particle=0 if nudged else 1,
                                precision=rbn.RBN_FP32, T0_shift=0.5, rho0_shift=1.0)
+        if nudged:
+            p.variant = rbn.RBN_VARIANT_NUDGING
+            p.ntempread = ENS_NTEMPREAD
+            p.alpha = ENS_ALPHA
+            p.deposit = rbn.RBN_DEPOSIT_BILINEAR
+            p.noutconfig = ENS_NTEMPREAD
         p = rbn.params_from_ra(p, ra, PR, U_FF)
         s = rbn.Solver(p, 0)
         s.init_conduction(1e-3, 100 + m)
-        n = int(rng.integers(1000, 10001))
-        x = rng.uniform(0.0, nx, n)
-        x[x == 0.0] = 0.5
-        y = rng.uniform(1.0, ny, n)
-        s.init_particles(np.arange(n, dtype=np.int32), x, y, tau_p=np.zeros(n))
+        if nudged:
+            ids, x0, y0, T0, recs = ens_observations(rng, nx, ny, n, last_step, ENS_NTEMPREAD)
+            s.init_observers(ids, x0, y0, T0)
+            s.set_observation_table(recs)
+        else:
+            x = rng.uniform(0.0, nx, n)
+            x[x == 0.0] = 0.5
+            y = rng.uniform(1.0, ny, n)
+            s.init_particles(np.arange(n, dtype=np.int32), x, y, tau_p=np.zeros(n))
         members.append(s)
         nodes += nx * ny
+        nobs += n
+    return members, nodes, nobs
+
+
+def run_ensemble(rbn, steps, warmup, tuning=None, nudged=True):
+    """The batched ensemble (autotuned, or the given launch shape (tile_w, tile_h, wg_per_cu,
+    order)) against the same members, built afresh, stepped one after another with their
+    own launches (rbn_step)."""
+    ens_warm = warmup - warmup % 12 + 12
+    last = max(warmup + steps, 2 * max(8, 6) + ens_warm + steps) + 2 * ENS_NTEMPREAD
+    members, nodes, nobs = ens_members(rbn, nudged, last)
     for s in members:
         s.step(warmup)
+    for s in members:
+        s.synchronize()
     t0 = time.perf_counter()
     for s in members:
         s.step(steps)
     for s in members:
         s.synchronize()
     t_seq = time.perf_counter() - t0
+    for s in members:
+        s.close()
+    members, nodes, nobs = ens_members(rbn, nudged, last)
     e = rbn.Ensemble(members)
     if tuning is None:
         best_trial = e.autotune(8)
     else:
         e.set_tuning(*tuning)
         best_trial = float("nan")
-    e.step(warmup - warmup % 4 + 4)
+    e.step(ens_warm)
     t0 = time.perf_counter()
     e.step(steps)
     t_ens = time.perf_counter() - t0
-    out = {"members": ENS_MEMBERS, "nodes": nodes, "steps": steps,
+    out = {"members": ENS_MEMBERS, "nudged": nudged, "nodes": nodes,
+           ("observers" if nudged else "tracers"): nobs, "steps": steps,
            "mlups": round(nodes * steps / t_ens / 1e6, 1),
            "sequential_mlups": round(nodes * steps / t_seq / 1e6, 1),
            "speedup_vs_sequential": round(t_seq / t_ens, 2),
@@ -193,16 +239,11 @@ def run_ensemble(rbn, steps, warmup, tuning=None):
            "roofline_frac": round(nodes * lbm_bytes_per_node(1) / (t_ens / steps) / 1e9 / HBM_PEAK_GBPS, 4),
            "tuning": e.tuning(), "autotune_best_ms": finite_or_null(round(best_trial, 4))}
     order = out["tuning"]["order"]
-    out["two_steps_per_launch"] = bool(order & 8) and not order & 4
+    out["two_steps_per_launch"] = bool(order & 8) and not order & 4 and not nudged
     e.close()
     for s in members:
         s.close()
     return out
-
-
-NUD_NX, NUD_NY, NUD_RA = 1024, 512, 1e7
-NUD_OBS, NUD_EVERY, NUD_ALPHA = 10_000, 5, 0.1
-NUD_SWEEP_OBS, NUD_SWEEP_ALPHA = (1_000, 10_000, 100_000), (0.01, 0.1, 1.0)
 
 
 def nudging_truth(rbn, spinup):
@@ -589,8 +630,10 @@ def main() -> int:
     ens = None
     if not args.no_ensemble:
         tun = tuple(int(v) for v in args.ensemble_tuning.split(",")) if args.ensemble_tuning else None
-        ens = run_ensemble(rbn, max(args.steps // 5, 8), 20, tun)
-        print(f"bench: ensemble done {ens}", file=sys.stderr, flush=True)
+        ens = run_ensemble(rbn, max(args.steps // 5, 8), 20, tun, nudged=True)
+        print(f"bench: ensemble (nudged members) done {ens}", file=sys.stderr, flush=True)
+        ens["truth_run_members"] = run_ensemble(rbn, max(args.steps // 5, 8), 20, None, nudged=False)
+        print(f"bench: ensemble (truth-run members) done {ens['truth_run_members']}", file=sys.stderr, flush=True)
     nud = None
     if not args.no_nudging:
         nud = run_nudging(rbn, args.nud_steps, args.nud_spinup)

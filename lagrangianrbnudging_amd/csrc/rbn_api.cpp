@@ -324,8 +324,51 @@ int run_energy(rbn_ctx *c, double out[2]) {
   return RBN_OK;
 }
 
-// Fetch record `rstep` through the provider, map observer ids to record slots (the first
-// match, as the reference's search N:1480-1487) and upload it on stream s.
+// Fetch record `rstep` through the provider into rx, ry, rT (ids into rid) and map observer
+// ids to record entries in chk (the first match, as the reference's search N:1480-1487).
+int fetch_observation(rbn_ctx *c, int rstep, double *rx, double *ry, double *rT, int *rid, int *chk,
+                      bool *same_map) {
+  const int n = c->nobs;
+  if (!c->obs_fn) return fail(c, RBN_ESTATE, "no observation provider set (record %d needed)", rstep);
+  if (c->obs_fn(c->obs_user, rstep, n, rid, rx, ry, rT) != 0)
+    return fail(c, RBN_ECALLBACK, "observation provider failed for record %d", rstep);
+  // chk[i] = the first record entry with observer i's id.  When the ids are those of the
+  // previous record, the map on the device is still current.
+  *same_map = false;
+  if (c->prev_rid.size() == (size_t)n && std::equal(rid, rid + n, c->prev_rid.begin())) {
+    *same_map = true;
+    return RBN_OK;
+  }
+  int lo = 0, hi = -1;
+  if (n > 0) {
+    lo = *std::min_element(rid, rid + n);
+    hi = *std::max_element(rid, rid + n);
+  }
+  const long long range = (long long)hi - (long long)lo + 1;
+  if (n > 0 && range <= 4LL * n + 1024) {  // dense ids: a direct table
+    std::vector<int> first((size_t)range, -1);
+    for (int j = n - 1; j >= 0; j--) first[(size_t)(rid[j] - lo)] = j;
+    for (int i = 0; i < n; i++) {
+      const long long id = c->oid[(size_t)i];
+      const int j = (id >= lo && id <= hi) ? first[(size_t)(id - lo)] : -1;
+      if (j < 0) return fail(c, RBN_ENOTFOUND, "observer id %d missing from record %d", c->oid[(size_t)i], rstep);
+      chk[i] = j;
+    }
+  } else {
+    std::unordered_map<int, int> first;
+    first.reserve((size_t)n * 2);
+    for (int j = 0; j < n; j++) first.emplace(rid[j], j);
+    for (int i = 0; i < n; i++) {
+      auto it = first.find(c->oid[(size_t)i]);
+      if (it == first.end()) return fail(c, RBN_ENOTFOUND, "observer id %d missing from record %d", c->oid[(size_t)i], rstep);
+      chk[i] = it->second;
+    }
+  }
+  c->prev_rid.assign(rid, rid + n);
+  return RBN_OK;
+}
+
+// Fetch record `rstep` and upload it on stream s_up.
 int load_observation(rbn_ctx *c, int rstep, hipStream_t s_up) {
   const int n = c->nobs;
   if (!c->obs_fn) return fail(c, RBN_ESTATE, "no observation provider set (record %d needed)", rstep);
@@ -347,44 +390,13 @@ int load_observation(rbn_ctx *c, int rstep, hipStream_t s_up) {
   }
   double *rx = (double *)s.h, *ry = rx + N, *rT = ry + N;
   int *rid = (int *)(rT + N), *chk = rid + N;
-  if (c->obs_fn(c->obs_user, rstep, n, rid, rx, ry, rT) != 0)
-    return fail(c, RBN_ECALLBACK, "observation provider failed for record %d", rstep);
-  // chk[i] = the first record entry with observer i's id (N:1480-1487).
-  if (c->prev_rid.size() == (size_t)n && std::equal(rid, rid + n, c->prev_rid.begin())) {
-    std::copy(c->prev_chk.begin(), c->prev_chk.end(), chk);
-  } else {
-    int lo = 0, hi = -1;
-    if (n > 0) {
-      lo = *std::min_element(rid, rid + n);
-      hi = *std::max_element(rid, rid + n);
-    }
-    const long long range = (long long)hi - (long long)lo + 1;
-    if (n > 0 && range <= 4LL * n + 1024) {  // dense ids: a direct table
-      std::vector<int> first((size_t)range, -1);
-      for (int j = n - 1; j >= 0; j--) first[(size_t)(rid[j] - lo)] = j;
-      for (int i = 0; i < n; i++) {
-        const long long id = c->oid[(size_t)i];
-        const int j = (id >= lo && id <= hi) ? first[(size_t)(id - lo)] : -1;
-        if (j < 0) return fail(c, RBN_ENOTFOUND, "observer id %d missing from record %d", c->oid[(size_t)i], rstep);
-        chk[i] = j;
-      }
-    } else {
-      std::unordered_map<int, int> first;
-      first.reserve((size_t)n * 2);
-      for (int j = 0; j < n; j++) first.emplace(rid[j], j);
-      for (int i = 0; i < n; i++) {
-        auto it = first.find(c->oid[(size_t)i]);
-        if (it == first.end()) return fail(c, RBN_ENOTFOUND, "observer id %d missing from record %d", c->oid[(size_t)i], rstep);
-        chk[i] = it->second;
-      }
-    }
-    c->prev_rid.assign(rid, rid + n);
-    c->prev_chk.assign(chk, chk + n);
-  }
+  bool same_map = false;
+  int rc = fetch_observation(c, rstep, rx, ry, rT, rid, chk, &same_map);
+  if (rc) return rc;
   rbn::ObsPtrs o = obs_ptrs(c);
   // rx, ry, rT are consecutive arrays on the device and in the slot
   HIPCHK(c, hipMemcpyAsync((void *)o.rx, rx, 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice, s_up));
-  HIPCHK(c, hipMemcpyAsync((void *)o.chk, chk, (size_t)n * sizeof(int), hipMemcpyHostToDevice, s_up));
+  if (!same_map) HIPCHK(c, hipMemcpyAsync((void *)o.chk, chk, (size_t)n * sizeof(int), hipMemcpyHostToDevice, s_up));
   HIPCHK(c, hipEventRecord(s.copied, s_up));
   return RBN_OK;
 }
@@ -463,6 +475,8 @@ void rbn_destroy(rbn_ctx *c) {
   dfree_t(c->dep.Tbar);
   dfree_t(c->dep.wq);
   dfree_t(c->dep.overflow);
+  dfree_t(c->dep.active);
+  dfree_t(c->dep.nactive);
   for (auto &p : c->pending) {
     (void)hipEventDestroy(p.a);
     (void)hipEventDestroy(p.b);
@@ -537,7 +551,7 @@ int rbn_create(const rbn_params *p, int device, rbn_ctx **out) {
   if (p->variant == RBN_VARIANT_NUDGING) {
     if ((rc = dalloc(c, &c->nud, fb))) goto fail_out;
     (void)hipMemsetAsync(c->nud, 0, fb, c->stream);
-    const size_t ntiles = (g.plane + 1023) / 1024;
+    const size_t ntiles = (g.plane + rbn::kDepScanTile - 1) / rbn::kDepScanTile;
     struct B {
       void **p;
       size_t b;
@@ -547,7 +561,8 @@ int rbn_create(const rbn_params *p, int device, rbn_ctx **out) {
                    {(void **)&c->dep.blocksum, ntiles * sizeof(int)},
                    {(void **)&c->dep.Tbar, g.plane * sizeof(double)},
                    {(void **)&c->dep.wq, g.plane * sizeof(double)},
-                   {(void **)&c->dep.overflow, sizeof(int)}};
+                   {(void **)&c->dep.overflow, sizeof(int)},
+                   {(void **)&c->dep.nactive, sizeof(int)}};
     for (auto &a : dallocs) {
       if ((rc = dalloc(c, a.p, a.b))) goto fail_out;
       (void)hipMemsetAsync(*a.p, 0, a.b, c->stream);
@@ -867,10 +882,10 @@ int rbn_init_observers(rbn_ctx *c, int n, const int *id, const double *x, const 
   dfree_t(c->d_obs_i);
   dfree_t(c->d_obs_d);
   dfree_t(c->dep.list);
+  dfree_t(c->dep.active);
   c->nobs = n;
   c->oid.assign(id, id + n);
   c->prev_rid.clear();
-  c->prev_chk.clear();
   const size_t N = (size_t)(n ? n : 1);
   void *tmp = nullptr;
   if ((rc = dalloc(c, &tmp, ONI * N * sizeof(int)))) return rc;
@@ -881,6 +896,10 @@ int rbn_init_observers(rbn_ctx *c, int n, const int *id, const double *x, const 
   c->dep.list_cap = 77LL * (long long)N;
   if ((rc = dalloc(c, &tmp, (size_t)c->dep.list_cap * sizeof(int)))) return rc;
   c->dep.list = (int *)tmp;
+  // at most one active node per contribution
+  c->dep.active_cap = std::min<long long>((long long)c->geom.plane, c->dep.list_cap);
+  if ((rc = dalloc(c, &tmp, (size_t)c->dep.active_cap * sizeof(int)))) return rc;
+  c->dep.active = (int *)tmp;
   HIPCHK(c, hipMemsetAsync(c->d_obs_i, 0, ONI * N * sizeof(int), c->stream));
   std::vector<double> h(OND * N, 0.0);
   for (int i = 0; i < n; i++) {
@@ -943,11 +962,18 @@ int rbn_get_nudging_weights(rbn_ctx *c, double *weight) {
   if (!weight) return fail(c, RBN_EINVAL, "null output");
   const Geom &g = c->geom;
   std::vector<double> h(g.plane, 0.0);
+  std::vector<int> n(g.plane, 0);
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  if (c->dep.mode == RBN_DEPOSIT_BILINEAR)
+  if (c->dep.mode == RBN_DEPOSIT_BILINEAR) {
     HIPCHK(c, hipMemcpy(h.data(), c->dep.wq, g.plane * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(n.data(), c->dep.cnt, g.plane * sizeof(int), hipMemcpyDeviceToHost));
+  }
+  // weights are written only where the deposit has contributions
   for (int i = 0; i < g.nx; i++)
-    for (int j = 0; j < g.ny; j++) weight[(size_t)i * g.ny + j] = h[(size_t)j * g.pitch + i];
+    for (int j = 0; j < g.ny; j++) {
+      const size_t o = (size_t)j * g.pitch + i;
+      weight[(size_t)i * g.ny + j] = n[o] > 0 ? h[o] : 0.0;
+    }
   return RBN_OK;
 }
 
@@ -957,17 +983,21 @@ int rbn_get_nudging(rbn_ctx *c, int *nudge_vol, double *tempreadint, double *nud
   if (c->P.variant != RBN_VARIANT_NUDGING) return fail(c, RBN_ESTATE, "not a nudged run");
   const Geom &g = c->geom;
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  if (nudge_vol) {
-    std::vector<int> h(g.plane);
-    HIPCHK(c, hipMemcpy(h.data(), c->dep.cnt, g.plane * sizeof(int), hipMemcpyDeviceToHost));
+  std::vector<int> cnt(g.plane);
+  HIPCHK(c, hipMemcpy(cnt.data(), c->dep.cnt, g.plane * sizeof(int), hipMemcpyDeviceToHost));
+  if (nudge_vol)
     for (int i = 0; i < g.nx; i++)
-      for (int j = 0; j < g.ny; j++) nudge_vol[(size_t)i * g.ny + j] = h[(size_t)j * g.pitch + i];
-  }
+      for (int j = 0; j < g.ny; j++) nudge_vol[(size_t)i * g.ny + j] = cnt[(size_t)j * g.pitch + i];
   if (tempreadint) {
+    // the means are written only where the deposit has contributions (tempreadint is zero
+    // elsewhere in the reference)
     std::vector<double> h(g.plane);
     HIPCHK(c, hipMemcpy(h.data(), c->dep.Tbar, g.plane * sizeof(double), hipMemcpyDeviceToHost));
     for (int i = 0; i < g.nx; i++)
-      for (int j = 0; j < g.ny; j++) tempreadint[(size_t)i * g.ny + j] = h[(size_t)j * g.pitch + i];
+      for (int j = 0; j < g.ny; j++) {
+        const size_t o = (size_t)j * g.pitch + i;
+        tempreadint[(size_t)i * g.ny + j] = cnt[o] > 0 ? h[o] : 0.0;
+      }
   }
   if (nud_term && (rc = download_field(c, c->nud, nud_term))) return rc;
   return RBN_OK;

@@ -28,6 +28,15 @@ struct rbn_ensemble {
   void *d_nmembers = nullptr;
   int *d_npre = nullptr;
   int ntot[rbn::kNudgeBatches] = {};
+  // their record uploads: two pinned staging buffers with a device twin each, alternating
+  // between the steps that load records
+  struct Stage {
+    unsigned char *h = nullptr, *d = nullptr;
+    size_t cap = 0;
+    hipEvent_t done = nullptr;
+  } stage[2];
+  int stage_next = 0;
+  std::vector<int> loading, rid;
   // tuning
   int tile_w = 256, tile_h = 1, wg_per_cu = 4, order = 0;
   bool built = false;
@@ -314,6 +323,58 @@ void advance_members(rbn_ensemble *e, int n, int launches, int pairs) {
   }
 }
 
+size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+// Fetch the records of the nudged members whose record interval starts at step t of the
+// call (their providers, on the host), stage them in one pinned buffer and send them to the
+// observers with one copy and one scatter launch on the ensemble's stream.
+int load_records(rbn_ensemble *e, int t) {
+  std::vector<int> &ld = e->loading;
+  ld.clear();
+  size_t data = 0;
+  for (int q : e->nudged) {
+    const rbn_ctx *c = e->m[(size_t)q];
+    if ((c->istep + t + 1) % c->P.ntempread == 1) {
+      ld.push_back(q);
+      data += align256((size_t)c->nobs * (3 * sizeof(double) + sizeof(int)));
+    }
+  }
+  if (ld.empty()) return RBN_OK;
+  const size_t jobs_bytes = align256(ld.size() * sizeof(rbn::ObsJob)), need = jobs_bytes + data;
+  rbn_ensemble::Stage &st = e->stage[e->stage_next];
+  e->stage_next ^= 1;
+  if (!st.done) EHIP(e, hipEventCreateWithFlags(&st.done, hipEventDisableTiming));
+  EHIP(e, hipEventSynchronize(st.done));  // this buffer's previous upload has been read
+  if (st.cap < need) {
+    if (st.h) (void)hipHostFree(st.h);
+    if (st.d) (void)hipFree(st.d);
+    st.h = st.d = nullptr;
+    st.cap = 0;
+    EHIP(e, hipHostMalloc((void **)&st.h, need, hipHostMallocDefault));
+    EHIP(e, hipMalloc((void **)&st.d, need));
+    st.cap = need;
+  }
+  auto *jobs = (rbn::ObsJob *)st.h;
+  size_t off = jobs_bytes;
+  for (size_t k = 0; k < ld.size(); k++) {
+    rbn_ctx *c = e->m[(size_t)ld[k]];
+    const int n = c->nobs, ntr = c->P.ntempread, istep = c->istep + t + 1;
+    double *rx = (double *)(st.h + off);
+    int *chk = (int *)(rx + 3 * (size_t)n);
+    e->rid.resize((size_t)std::max(n, 1));
+    bool same = false;
+    int rc = rbn::host::fetch_observation(c, istep + ntr - 1, rx, rx + n, rx + 2 * (size_t)n, e->rid.data(), chk, &same);
+    if (rc) return efail(e, rc, "member %d: %s", ld[k], rbn_last_error(c));
+    const rbn::ObsPtrs o = rbn::host::obs_ptrs(c);
+    jobs[k] = {(double *)o.rx, same ? nullptr : (int *)o.chk, (long long)off, n};
+    off += align256((size_t)n * (3 * sizeof(double) + sizeof(int)));
+  }
+  EHIP(e, hipMemcpyAsync(st.d, st.h, off, hipMemcpyHostToDevice, e->stream));
+  EHIP(e, rbn::launch_obs_scatter(e->stream, (const rbn::ObsJob *)st.d, (int)ld.size(), st.d));
+  EHIP(e, hipEventRecord(st.done, e->stream));
+  return RBN_OK;
+}
+
 int sync_members(rbn_ensemble *e) {
   for (rbn_ctx *c : e->m) {
     EHIP(e, hipStreamSynchronize(c->stream));
@@ -334,17 +395,12 @@ int run_steps(rbn_ensemble *e, int nsteps) {
   for (int q : e->nudged)
     if (!e->m[(size_t)q]->d_obs_i) return efail(e, RBN_ESTATE, "member %d: nudged run without observers", q);
   if (!e->nudged.empty()) {
-    // With nudged members every step runs eagerly: the record loads of the members whose
-    // record interval starts (host provider, upload on this stream), the batched observer
+    // With nudged members every step runs eagerly: the records of the members whose record
+    // interval starts (host providers, one upload on this stream), the batched observer
     // update and deposit, then the lattice launch carrying the previous step's particle
     // updates; the last step's updates run on their own.
     for (int t = 0; t < nsteps; t++) {
-      for (int q : e->nudged) {
-        rbn_ctx *c = e->m[(size_t)q];
-        const int istep = c->istep + t + 1, ntr = c->P.ntempread;
-        if (istep % ntr == 1 && (rc = rbn::host::load_observation(c, istep + ntr - 1, e->stream)))
-          return efail(e, rc, "member %d: %s", q, rbn_last_error(c));
-      }
+      if ((rc = load_records(e, t))) return rc;
       EHIP(e, rbn::launch_nudge_batch(e->stream, (const rbn::NudgeMember *)e->d_nmembers, e->d_npre, e->ntot,
                                       (int)e->nudged.size(), t));
       if ((rc = enqueue_fused(e, t, t == nsteps - 1))) return rc;
@@ -470,6 +526,8 @@ int restore_snapshot(rbn_ensemble *e, Snapshot &s, bool keep = false) {
       std::swap(c->gA, c->gB);
     }
     if (c->F != s.F[i]) std::swap(c->F, c->F2);
+    // the observers' record map on the device goes back too: forget which record it is for
+    c->prev_rid.clear();
     c->istep = s.istep[i];
     c->cur = s.cur[i];
   }
@@ -497,6 +555,11 @@ void rbn_ensemble_destroy(rbn_ensemble *e) {
   if (e->d_counters) (void)hipFree(e->d_counters);
   if (e->d_nmembers) (void)hipFree(e->d_nmembers);
   if (e->d_npre) (void)hipFree(e->d_npre);
+  for (auto &st : e->stage) {
+    if (st.h) (void)hipHostFree(st.h);
+    if (st.d) (void)hipFree(st.d);
+    if (st.done) (void)hipEventDestroy(st.done);
+  }
   if (e->ev_lbm) (void)hipEventDestroy(e->ev_lbm);
   for (hipEvent_t ev : e->ev_part)
     if (ev) (void)hipEventDestroy(ev);

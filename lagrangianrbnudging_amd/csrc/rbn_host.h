@@ -84,6 +84,10 @@ ObsPtrs obs_ptrs(rbn_ctx *c);
 // and upload it as the current record (on the context's stream, or on s).
 int load_observation(rbn_ctx *c, int rstep);
 int load_observation(rbn_ctx *c, int rstep, hipStream_t s);
+// The host half: record rstep into rx, ry, rT (n = nobs each; ids into rid) and the observer
+// map into chk; *same_map: the map is the one already on the device (chk not written).
+int fetch_observation(rbn_ctx *c, int rstep, double *rx, double *ry, double *rT, int *rid, int *chk,
+                      bool *same_map);
 
 // Report (and clear) a particle / observer range error or a deposit overflow.
 int check_err_flag(rbn_ctx *c);

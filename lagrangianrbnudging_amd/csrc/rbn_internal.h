@@ -81,11 +81,14 @@ struct DepPtrs {
   int *off;    // per node exclusive prefix of cnt (+1 total at the end)
   int *list;   // contributions, grouped per node
   int *blocksum;
-  double *Tbar;
-  double *wq;  // bilinear deposit: per-node weight sum (nullptr for the 7x7 deposit)
-  int mode;    // RBN_DEPOSIT_BOX7 or RBN_DEPOSIT_BILINEAR
+  double *Tbar;  // at nodes with contributions (elsewhere not written)
+  double *wq;    // bilinear deposit: per-node weight sum, likewise (nullptr for the 7x7 deposit)
+  int mode;      // RBN_DEPOSIT_BOX7 or RBN_DEPOSIT_BILINEAR
   long long list_cap;
   int *overflow;
+  int *active;   // the nodes with contributions, in no particular order (active_cap entries)
+  int *nactive;  // their number
+  long long active_cap;
 };
 
 // One nudged member of a batched observer/deposit launch (ensembles): its lattice, the
@@ -97,6 +100,18 @@ struct NudgeMember {
   ObsPtrs o;
   DepPtrs d;
 };
+// One record of a batched upload: 3n doubles (x, y, T) at byte offset src of the staging
+// buffer, then (when chk is set) the n-entry observer map, copied to the observers' current
+// record rx (rx, ry, rT consecutive) and map.
+struct ObsJob {
+  double *rx;
+  int *chk;
+  long long src;
+  int n;
+};
+
+// Nodes per tile of the deposit's count scan (DepPtrs::blocksum holds one total per tile).
+constexpr int kDepScanTile = 1024;
 // Passes with per-member block counts (the prefix rows of a batched launch).
 enum { NB_OBS, NB_ZERO, NB_CAND, NB_SCAN, NB_FOLD, kNudgeBatches };
 
@@ -179,6 +194,8 @@ void nudge_batch_blocks(const NudgeMember &d, int blocks[kNudgeBatches]);
 // (host, the last entry of each row).
 hipError_t launch_nudge_batch(hipStream_t s, const NudgeMember *members, const int *pre, const int *totals, int nm,
                               int t);
+// Scatter njobs staged records (jobs and data in the device staging buffer) to their observers.
+hipError_t launch_obs_scatter(hipStream_t s, const ObsJob *jobs, int njobs, const unsigned char *staging);
 hipError_t launch_energy(hipStream_t s, int prec, const Geom &g, const void *rho, const void *uv,
                          const void *T, double *partial, int nblocks, double *out2);
 int energy_blocks(const Geom &g);
@@ -265,14 +282,14 @@ struct rbn_ctx {
   // Record staging: two pinned slots (x, y, T as doubles, then ids and the observer map as
   // ints) so that the upload of a record is queued on the stream without waiting for it;
   // a slot is refilled only after the event of its previous upload.  The observer -> record
-  // map is rebuilt only when a record's ids differ from the previous record's.
+  // map is rebuilt and uploaded only when a record's ids differ from the previous record's.
   struct ObsSlot {
     unsigned char *h = nullptr;
     size_t cap = 0;
     hipEvent_t copied = nullptr;
   } obs_slot[2];
   int obs_next = 0;
-  std::vector<int> prev_rid, prev_chk;
+  std::vector<int> prev_rid;  // ids of the record whose observer map is on the device
   rbn::DepPtrs dep{};
 
   // diagnostics

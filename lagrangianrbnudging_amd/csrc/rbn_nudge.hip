@@ -13,8 +13,11 @@
 //   2. exclusive scan of the counts gives every node a slot range;
 //   3. write each contribution's observer index into its node's range (atomic slot
 //      allocation: order free);
-//   4. per node, sort its observer indices ascending and fold the running mean in that
-//      order — the reference's order, so the result is bit-identical.
+//   4. per node with contributions (the fill pass lists them), sort its observer indices
+//      ascending and fold the running mean in that order — the reference's order, so the
+//      result is bit-identical.  The means of the other nodes are not written: every reader
+//      (the nudged collision, the adjoint, the getters) looks at them only where the count
+//      is positive.
 // Each observer contributes at most once per node when nx >= 8, so observer order is the
 // complete per-node order.  Count and fill run one thread per (observer, candidate node), so
 // the atomics of one observer are independent; the scan's tile offsets are added where the
@@ -35,6 +38,7 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kScanItems = 4;
 constexpr int kScanTile = kBlock * kScanItems;
+static_assert(kScanTile == kDepScanTile, "tile totals are sized by kDepScanTile");
 
 // ---------------------------------------------------------------- observer update
 
@@ -171,15 +175,18 @@ __device__ __forceinline__ long long bil_cell(const Geom &g, double xx, double y
   return (long long)(J - 1) * g.pitch + (I - 1);
 }
 
-// Zero the counts and fill cursors: block b of nb, grid-stride.
-__device__ __forceinline__ void zero_part(size_t n, int *cnt, int *fill, int b, int nb) {
+// Zero the counts, fill cursors and active-node count: block b of nb, grid-stride.
+__device__ __forceinline__ void zero_part(size_t n, int *cnt, int *fill, int *nactive, int b, int nb) {
+  if (b == 0 && threadIdx.x == 0) *nactive = 0;
   for (size_t o = (size_t)b * kBlock + threadIdx.x; o < n; o += (size_t)nb * kBlock) {
     cnt[o] = 0;
     fill[o] = 0;
   }
 }
 
-__global__ void k_dep_zero(size_t n, int *cnt, int *fill) { zero_part(n, cnt, fill, blockIdx.x, gridDim.x); }
+__global__ void k_dep_zero(size_t n, int *cnt, int *fill, int *nactive) {
+  zero_part(n, cnt, fill, nactive, blockIdx.x, gridDim.x);
+}
 
 // Node of candidate c of observer k for deposit MODE (-1: none); *key is the list key.
 template <int MODE>
@@ -276,16 +283,35 @@ __device__ __forceinline__ long long seg_start(const int *off, const int *tile_s
   return (long long)off[c] + tile_sum[c / kScanTile];
 }
 
+// Append node to the active list for the lanes with first set: one atomic per wavefront
+// (the lanes of a wavefront take consecutive entries in lane order).
+__device__ __forceinline__ void append_active(bool first, int node, int *active, int *nactive, long long cap) {
+  const unsigned long long mask = __ballot(first);
+  if (!mask) return;
+  const int lane = __lane_id(), leader = __ffsll((long long)mask) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(nactive, __popcll(mask));
+  base = __shfl(base, leader);
+  if (first) {
+    const long long a = (long long)base + __popcll(mask & ((1ull << lane) - 1ull));
+    if (a < cap) active[a] = node;
+  }
+}
+
+// Contribution t takes the next slot of its node's range; the node's first contribution
+// also lists the node as active.
 template <int MODE>
 __device__ __forceinline__ void fill_one(const Geom &g, int right, const ObsPtrs &o, const int *off,
                                          const int *tile_sum, int *fill, int *list, long long cap, int *overflow,
-                                         long long t) {
+                                         int *active, int *nactive, long long active_cap, long long t) {
   constexpr int NC = MODE == RBN_DEPOSIT_BILINEAR ? kBilCells : kCells;
-  if (t >= (long long)o.n * NC) return;
-  int key;
-  const long long node = candidate<MODE>(g, right, o, (int)(t / NC), (int)(t % NC), &key);
+  int key = 0;
+  const long long node =
+      t < (long long)o.n * NC ? candidate<MODE>(g, right, o, (int)(t / NC), (int)(t % NC), &key) : -1;
+  const int rank = node >= 0 ? atomicAdd(&fill[node], 1) : -1;
+  append_active(rank == 0, (int)node, active, nactive, active_cap);
   if (node < 0) return;
-  const long long slot = seg_start(off, tile_sum, (size_t)node) + atomicAdd(&fill[node], 1);
+  const long long slot = seg_start(off, tile_sum, (size_t)node) + rank;
   if (slot < cap)
     list[slot] = key;
   else
@@ -293,16 +319,17 @@ __device__ __forceinline__ void fill_one(const Geom &g, int right, const ObsPtrs
 }
 
 template <int MODE>
-__global__ void k_dep_fill(Geom g, int right, ObsPtrs o, const int *off, const int *tile_sum,
-                           int *fill, int *list, long long cap, int *overflow) {
-  fill_one<MODE>(g, right, o, off, tile_sum, fill, list, cap, overflow,
-                 (long long)blockIdx.x * blockDim.x + threadIdx.x);
+__global__ void k_dep_fill(Geom g, int right, ObsPtrs o, const int *off, const int *tile_sum, DepPtrs d) {
+  fill_one<MODE>(g, right, o, off, tile_sum, d.fill, d.list, d.list_cap, d.overflow, d.active, d.nactive,
+                 d.active_cap, (long long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // Per node: order the contributions by observer and fold them in that order, the
 // reference's order.  7x7 deposit: the running mean (N:1578-1579).  Bilinear deposit:
-// W = sum w_k, S = sum w_k T_k, Tbar = S / W.  Lists of up to kSortRegs entries are sorted in
-// registers (odd-even transposition network, static indices); longer ones in place.
+// W = sum w_k, S = sum w_k T_k, Tbar = S / W.  Lists are sorted in registers (odd-even
+// transposition network, static indices) sized by the longest list of the wavefront (2, 4
+// or 16 keys), longer ones in place.  The keys of a list are distinct (an observer reaches
+// a node at most once), so every sort gives the same order.
 constexpr int kSortRegs = 16;
 
 template <int MODE>
@@ -324,32 +351,48 @@ struct Fold {
   }
 };
 
+// The n <= NS keys of a list sorted in registers and folded in that order.
+template <int MODE, int NS>
+__device__ __forceinline__ void fold_sorted(Fold<MODE> &f, const int *seg, int n) {
+  int key[NS];
+#pragma unroll
+  for (int a = 0; a < NS; a++) key[a] = a < n ? seg[a] : 0x7fffffff;
+#pragma unroll
+  for (int pass = 0; pass < NS; pass++) {
+#pragma unroll
+    for (int a = pass & 1; a + 1 < NS; a += 2) {
+      const int lo = min(key[a], key[a + 1]), hi = max(key[a], key[a + 1]);
+      key[a] = lo;
+      key[a + 1] = hi;
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < NS; a++)
+    if (a < n) f.take(a, key[a]);
+}
+
+// The longest list of the wavefront (every lane takes part).
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+  return v;
+}
+
+// Node c with n > 0 contributions; nmax: the wavefront's longest list.
 template <int MODE>
-__device__ __forceinline__ void fold_node(const Geom &g, const int *cnt, const int *off, const int *tile_sum,
-                                          int *list, const ObsPtrs &o, double *Tbar, double *wq, long long cap,
-                                          int i, int j) {
-  const size_t c = (size_t)j * g.pitch + i;
-  const int n = cnt[c];
+__device__ __forceinline__ void fold_node(const Geom &g, const int *off, const int *tile_sum, int *list,
+                                          const ObsPtrs &o, double *Tbar, double *wq, long long cap, size_t c,
+                                          int n, int nmax) {
   Fold<MODE> f{g, o};
   const long long base = n > 0 ? seg_start(off, tile_sum, c) : 0;
   if (n > 0 && base + n <= cap) {
     int *seg = list + base;
-    if (n <= kSortRegs) {
-      int key[kSortRegs];
-#pragma unroll
-      for (int a = 0; a < kSortRegs; a++) key[a] = a < n ? seg[a] : 0x7fffffff;
-#pragma unroll
-      for (int pass = 0; pass < kSortRegs; pass++) {
-#pragma unroll
-        for (int a = pass & 1; a + 1 < kSortRegs; a += 2) {
-          const int lo = min(key[a], key[a + 1]), hi = max(key[a], key[a + 1]);
-          key[a] = lo;
-          key[a + 1] = hi;
-        }
-      }
-#pragma unroll
-      for (int a = 0; a < kSortRegs; a++)
-        if (a < n) f.take(a, key[a]);
+    if (nmax <= 2) {
+      fold_sorted<MODE, 2>(f, seg, n);
+    } else if (nmax <= 4) {
+      fold_sorted<MODE, 4>(f, seg, n);
+    } else if (n <= kSortRegs) {
+      fold_sorted<MODE, kSortRegs>(f, seg, n);
     } else {
       for (int a = 1; a < n; a++) {  // insertion sort in place
         const int key = seg[a];
@@ -371,12 +414,23 @@ __device__ __forceinline__ void fold_node(const Geom &g, const int *cnt, const i
   }
 }
 
+// The active nodes of a deposit, block b of nb, grid-stride (every lane of a wavefront takes
+// part in the wavefront's maximum).
 template <int MODE>
-__global__ __launch_bounds__(kBlock) void k_dep_fold(Geom g, const int *cnt, const int *off,
-                                                     const int *tile_sum, int *list, ObsPtrs o,
-                                                     double *Tbar, double *wq, long long cap) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < g.nx) fold_node<MODE>(g, cnt, off, tile_sum, list, o, Tbar, wq, cap, i, blockIdx.y);
+__device__ __forceinline__ void fold_active(const Geom &g, const ObsPtrs &o, const DepPtrs &d, int b, int nb) {
+  const int na = (int)min((long long)*d.nactive, d.active_cap);
+  for (int k0 = b * kBlock; k0 < na; k0 += nb * kBlock) {
+    const int k = k0 + threadIdx.x;
+    const size_t c = k < na ? (size_t)d.active[k] : 0;
+    const int n = k < na ? d.cnt[c] : 0;
+    const int nmax = wave_max(n);
+    if (k < na) fold_node<MODE>(g, d.off, d.blocksum, d.list, o, d.Tbar, d.wq, d.list_cap, c, n, nmax);
+  }
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_dep_fold(Geom g, ObsPtrs o, DepPtrs d) {
+  fold_active<MODE>(g, o, d, blockIdx.x, gridDim.x);
 }
 
 // ---------------------------------------------------------------- batched (ensembles)
@@ -400,6 +454,15 @@ __device__ __forceinline__ int member_of(const int *pre, int nm, int b) {
 
 __device__ __forceinline__ int nudge_blocks(const Geom &g) { return (g.nx + kBlock - 1) / kBlock; }
 
+// Batched zero passes give each block more work than the single-member ones, so that the
+// member lookup and descriptor loads of a block start are spread over more nodes.
+constexpr int kZeroPerBlock = 16 * kBlock;  // counts zeroed per block
+
+// Blocks of a fold over a deposit's active nodes (at most kFoldBlocks, grid-stride).
+inline int fold_blocks(const DepPtrs &d, int max_blocks) {
+  return (int)std::max<long long>(1, std::min<long long>((d.active_cap + kBlock - 1) / kBlock, max_blocks));
+}
+
 __global__ __launch_bounds__(kBlock) void k_nb_obs(const NudgeMember *M, const int *pre_all, int nm, int t) {
   const int *pre = pre_all + NB_OBS * (nm + 1);
   const int m = member_of(pre, nm, blockIdx.x);
@@ -414,7 +477,7 @@ __global__ __launch_bounds__(kBlock) void k_nb_obs(const NudgeMember *M, const i
 __global__ __launch_bounds__(kBlock) void k_nb_zero(const NudgeMember *M, const int *pre_all, int nm) {
   const int *pre = pre_all + NB_ZERO * (nm + 1);
   const int m = member_of(pre, nm, blockIdx.x);
-  zero_part(M[m].g.plane, M[m].d.cnt, M[m].d.fill, blockIdx.x - pre[m], pre[m + 1] - pre[m]);
+  zero_part(M[m].g.plane, M[m].d.cnt, M[m].d.fill, M[m].d.nactive, blockIdx.x - pre[m], pre[m + 1] - pre[m]);
 }
 
 __global__ __launch_bounds__(kBlock) void k_nb_count(const NudgeMember *M, const int *pre_all, int nm) {
@@ -449,27 +512,23 @@ __global__ __launch_bounds__(kBlock) void k_nb_fill(const NudgeMember *M, const 
   const int m = member_of(pre, nm, blockIdx.x);
   const NudgeMember &d = M[m];
   const long long t = (long long)(blockIdx.x - pre[m]) * kBlock + threadIdx.x;
-  if (d.d.mode == RBN_DEPOSIT_BILINEAR)
-    fill_one<RBN_DEPOSIT_BILINEAR>(d.g, d.right, d.o, d.d.off, d.d.blocksum, d.d.fill, d.d.list, d.d.list_cap,
-                                   d.d.overflow, t);
+  const DepPtrs &p = d.d;
+  if (p.mode == RBN_DEPOSIT_BILINEAR)
+    fill_one<RBN_DEPOSIT_BILINEAR>(d.g, d.right, d.o, p.off, p.blocksum, p.fill, p.list, p.list_cap, p.overflow,
+                                   p.active, p.nactive, p.active_cap, t);
   else
-    fill_one<RBN_DEPOSIT_BOX7>(d.g, d.right, d.o, d.d.off, d.d.blocksum, d.d.fill, d.d.list, d.d.list_cap,
-                               d.d.overflow, t);
+    fill_one<RBN_DEPOSIT_BOX7>(d.g, d.right, d.o, p.off, p.blocksum, p.fill, p.list, p.list_cap, p.overflow,
+                               p.active, p.nactive, p.active_cap, t);
 }
 
 __global__ __launch_bounds__(kBlock) void k_nb_fold(const NudgeMember *M, const int *pre_all, int nm) {
   const int *pre = pre_all + NB_FOLD * (nm + 1);
   const int m = member_of(pre, nm, blockIdx.x);
   const NudgeMember &d = M[m];
-  const int lb = blockIdx.x - pre[m], bx = nudge_blocks(d.g);
-  const int j = lb / bx, i = (lb - j * bx) * kBlock + threadIdx.x;
-  if (i >= d.g.nx) return;
   if (d.d.mode == RBN_DEPOSIT_BILINEAR)
-    fold_node<RBN_DEPOSIT_BILINEAR>(d.g, d.d.cnt, d.d.off, d.d.blocksum, d.d.list, d.o, d.d.Tbar, d.d.wq,
-                                    d.d.list_cap, i, j);
+    fold_active<RBN_DEPOSIT_BILINEAR>(d.g, d.o, d.d, blockIdx.x - pre[m], pre[m + 1] - pre[m]);
   else
-    fold_node<RBN_DEPOSIT_BOX7>(d.g, d.d.cnt, d.d.off, d.d.blocksum, d.d.list, d.o, d.d.Tbar, d.d.wq,
-                                d.d.list_cap, i, j);
+    fold_active<RBN_DEPOSIT_BOX7>(d.g, d.o, d.d, blockIdx.x - pre[m], pre[m + 1] - pre[m]);
 }
 
 }  // namespace
@@ -482,13 +541,33 @@ hipError_t launch_obs_update(hipStream_t s, int branch, int nx, int ny, int ntem
   return hipGetLastError();
 }
 
+namespace {
+
+__global__ __launch_bounds__(kBlock) void k_obs_scatter(const ObsJob *jobs, const unsigned char *staging) {
+  const ObsJob j = jobs[blockIdx.x];
+  const double *src = (const double *)(staging + j.src);
+  for (int k = threadIdx.x; k < 3 * j.n; k += kBlock) j.rx[k] = src[k];
+  if (j.chk) {
+    const int *map = (const int *)(src + 3 * (size_t)j.n);
+    for (int k = threadIdx.x; k < j.n; k += kBlock) j.chk[k] = map[k];
+  }
+}
+
+}  // namespace
+
+hipError_t launch_obs_scatter(hipStream_t s, const ObsJob *jobs, int njobs, const unsigned char *staging) {
+  if (njobs <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_obs_scatter, dim3(njobs), dim3(kBlock), 0, s, jobs, staging);
+  return hipGetLastError();
+}
+
 void nudge_batch_blocks(const NudgeMember &d, int blocks[kNudgeBatches]) {
   const long long nc = (long long)d.o.n * (d.d.mode == RBN_DEPOSIT_BILINEAR ? kBilCells : kCells);
   blocks[NB_OBS] = (d.o.n + kBlock - 1) / kBlock;
-  blocks[NB_ZERO] = (int)std::min<size_t>((d.g.plane + kBlock - 1) / kBlock, 1024);
+  blocks[NB_ZERO] = (int)((d.g.plane + kZeroPerBlock - 1) / kZeroPerBlock);
   blocks[NB_CAND] = (int)((nc + kBlock - 1) / kBlock);
   blocks[NB_SCAN] = (int)((d.g.plane + kScanTile - 1) / kScanTile);
-  blocks[NB_FOLD] = (d.g.nx + kBlock - 1) / kBlock * d.g.ny;
+  blocks[NB_FOLD] = fold_blocks(d.d, 256);
 }
 
 hipError_t launch_nudge_batch(hipStream_t s, const NudgeMember *members, const int *pre, const int *totals, int nm,
@@ -510,11 +589,11 @@ hipError_t launch_deposit(hipStream_t s, const Geom &g, int right_limit, const O
   const size_t n = g.plane;
   const int ntiles = (int)((n + kScanTile - 1) / kScanTile);
   const int zb = (int)std::min<size_t>((n + kBlock - 1) / kBlock, 4096);
-  hipLaunchKernelGGL(k_dep_zero, dim3(zb), dim3(kBlock), 0, s, n, d.cnt, d.fill);
+  hipLaunchKernelGGL(k_dep_zero, dim3(zb), dim3(kBlock), 0, s, n, d.cnt, d.fill, d.nactive);
   const bool bil = d.mode == RBN_DEPOSIT_BILINEAR;
   const long long work = (long long)o.n * (bil ? kBilCells : kCells);
   const dim3 ob((unsigned)((work + kBlock - 1) / kBlock > 0 ? (work + kBlock - 1) / kBlock : 1));
-  const dim3 nb((g.nx + kBlock - 1) / kBlock, g.ny);
+  const dim3 fb((unsigned)fold_blocks(d, 1024));
   if (o.n > 0) {
     if (bil)
       hipLaunchKernelGGL(k_dep_count<RBN_DEPOSIT_BILINEAR>, ob, dim3(kBlock), 0, s, g, right_limit, o, d.cnt);
@@ -526,17 +605,15 @@ hipError_t launch_deposit(hipStream_t s, const Geom &g, int right_limit, const O
   if (o.n > 0) {
     if (bil)
       hipLaunchKernelGGL(k_dep_fill<RBN_DEPOSIT_BILINEAR>, ob, dim3(kBlock), 0, s, g, right_limit, o, d.off,
-                         d.blocksum, d.fill, d.list, d.list_cap, d.overflow);
+                         d.blocksum, d);
     else
       hipLaunchKernelGGL(k_dep_fill<RBN_DEPOSIT_BOX7>, ob, dim3(kBlock), 0, s, g, right_limit, o, d.off,
-                         d.blocksum, d.fill, d.list, d.list_cap, d.overflow);
+                         d.blocksum, d);
+    if (bil)
+      hipLaunchKernelGGL(k_dep_fold<RBN_DEPOSIT_BILINEAR>, fb, dim3(kBlock), 0, s, g, o, d);
+    else
+      hipLaunchKernelGGL(k_dep_fold<RBN_DEPOSIT_BOX7>, fb, dim3(kBlock), 0, s, g, o, d);
   }
-  if (bil)
-    hipLaunchKernelGGL(k_dep_fold<RBN_DEPOSIT_BILINEAR>, nb, dim3(kBlock), 0, s, g, d.cnt, d.off, d.blocksum,
-                       d.list, o, d.Tbar, d.wq, d.list_cap);
-  else
-    hipLaunchKernelGGL(k_dep_fold<RBN_DEPOSIT_BOX7>, nb, dim3(kBlock), 0, s, g, d.cnt, d.off, d.blocksum,
-                       d.list, o, d.Tbar, d.wq, d.list_cap);
   return hipGetLastError();
 }
 
