@@ -246,6 +246,11 @@ def run_ensemble(rbn, steps, warmup, tuning=None, nudged=True):
     return out
 
 
+NUD_NX, NUD_NY, NUD_RA = 1024, 512, 1e7
+NUD_OBS, NUD_EVERY, NUD_ALPHA = 10_000, 5, 0.1
+NUD_SWEEP_OBS, NUD_SWEEP_ALPHA = (1_000, 10_000, 100_000), (0.01, 0.1, 1.0)
+
+
 def nudging_truth(rbn, spinup):
     """The spun-up truth run of the nudging configuration and the nudged runs' first state."""
     pt = rbn.default_params(nx=NUD_NX, ny=NUD_NY, Tup=0.0, Tdown=1.0, scratch=1, nstart=0, nout=10**9,
