@@ -896,8 +896,12 @@ int rbn_init_observers(rbn_ctx *c, int n, const int *id, const double *x, const 
   c->dep.list_cap = 77LL * (long long)N;
   if ((rc = dalloc(c, &tmp, (size_t)c->dep.list_cap * sizeof(int)))) return rc;
   c->dep.list = (int *)tmp;
-  // at most one active node per contribution
+  // at most one active node per contribution; a deposit with at least one candidate
+  // contribution per four nodes folds every node in row order instead (an active list would
+  // serialise on its counter)
   c->dep.active_cap = std::min<long long>((long long)c->geom.plane, c->dep.list_cap);
+  const long long ncand = (long long)n * (c->dep.mode == RBN_DEPOSIT_BILINEAR ? 4 : 77);
+  c->dep.dense = 4 * ncand >= (long long)c->geom.plane;
   if ((rc = dalloc(c, &tmp, (size_t)c->dep.active_cap * sizeof(int)))) return rc;
   c->dep.active = (int *)tmp;
   HIPCHK(c, hipMemsetAsync(c->d_obs_i, 0, ONI * N * sizeof(int), c->stream));

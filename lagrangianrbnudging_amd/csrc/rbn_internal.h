@@ -89,6 +89,7 @@ struct DepPtrs {
   int *active;   // the nodes with contributions, in no particular order (active_cap entries)
   int *nactive;  // their number
   long long active_cap;
+  int dense;     // many contributions per node: fold every node in row order, no active list
 };
 
 // One nudged member of a batched observer/deposit launch (ensembles): its lattice, the

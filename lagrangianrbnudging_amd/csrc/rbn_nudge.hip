@@ -13,11 +13,13 @@
 //   2. exclusive scan of the counts gives every node a slot range;
 //   3. write each contribution's observer index into its node's range (atomic slot
 //      allocation: order free);
-//   4. per node with contributions (the fill pass lists them), sort its observer indices
-//      ascending and fold the running mean in that order — the reference's order, so the
-//      result is bit-identical.  The means of the other nodes are not written: every reader
-//      (the nudged collision, the adjoint, the getters) looks at them only where the count
-//      is positive.
+//   4. per node with contributions, sort its observer indices ascending and fold the
+//      running mean in that order — the reference's order, so the result is bit-identical.
+//      With fewer candidate contributions than a quarter of the nodes the fill pass lists
+//      the nodes that have any and the fold visits only those; otherwise it sweeps the
+//      rows (DepPtrs::dense).  The means of the other nodes are not written:
+//      every reader (the nudged collision, the adjoint, the getters) looks at them only where
+//      the count is positive.
 // Each observer contributes at most once per node when nx >= 8, so observer order is the
 // complete per-node order.  Count and fill run one thread per (observer, candidate node), so
 // the atomics of one observer are independent; the scan's tile offsets are added where the
@@ -309,7 +311,7 @@ __device__ __forceinline__ void fill_one(const Geom &g, int right, const ObsPtrs
   const long long node =
       t < (long long)o.n * NC ? candidate<MODE>(g, right, o, (int)(t / NC), (int)(t % NC), &key) : -1;
   const int rank = node >= 0 ? atomicAdd(&fill[node], 1) : -1;
-  append_active(rank == 0, (int)node, active, nactive, active_cap);
+  if (active) append_active(rank == 0, (int)node, active, nactive, active_cap);
   if (node < 0) return;
   const long long slot = seg_start(off, tile_sum, (size_t)node) + rank;
   if (slot < cap)
@@ -320,8 +322,8 @@ __device__ __forceinline__ void fill_one(const Geom &g, int right, const ObsPtrs
 
 template <int MODE>
 __global__ void k_dep_fill(Geom g, int right, ObsPtrs o, const int *off, const int *tile_sum, DepPtrs d) {
-  fill_one<MODE>(g, right, o, off, tile_sum, d.fill, d.list, d.list_cap, d.overflow, d.active, d.nactive,
-                 d.active_cap, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+  fill_one<MODE>(g, right, o, off, tile_sum, d.fill, d.list, d.list_cap, d.overflow, d.dense ? nullptr : d.active,
+                 d.nactive, d.active_cap, (long long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // Per node: order the contributions by observer and fold them in that order, the
@@ -428,9 +430,24 @@ __device__ __forceinline__ void fold_active(const Geom &g, const ObsPtrs &o, con
   }
 }
 
+// Row j of a deposit, nodes i0 .. i0 + kBlock - 1 (dense deposits).
+template <int MODE>
+__device__ __forceinline__ void fold_row(const Geom &g, const ObsPtrs &o, const DepPtrs &d, int i0, int j) {
+  const int i = i0 + threadIdx.x;
+  const size_t c = (size_t)j * g.pitch + i;
+  const int n = i < g.nx ? d.cnt[c] : 0;
+  const int nmax = wave_max(n);
+  if (n > 0) fold_node<MODE>(g, d.off, d.blocksum, d.list, o, d.Tbar, d.wq, d.list_cap, c, n, nmax);
+}
+
 template <int MODE>
 __global__ __launch_bounds__(kBlock) void k_dep_fold(Geom g, ObsPtrs o, DepPtrs d) {
   fold_active<MODE>(g, o, d, blockIdx.x, gridDim.x);
+}
+
+template <int MODE>
+__global__ __launch_bounds__(kBlock) void k_dep_fold_rows(Geom g, ObsPtrs o, DepPtrs d) {
+  fold_row<MODE>(g, o, d, blockIdx.x * kBlock, blockIdx.y);
 }
 
 // ---------------------------------------------------------------- batched (ensembles)
@@ -513,22 +530,32 @@ __global__ __launch_bounds__(kBlock) void k_nb_fill(const NudgeMember *M, const 
   const NudgeMember &d = M[m];
   const long long t = (long long)(blockIdx.x - pre[m]) * kBlock + threadIdx.x;
   const DepPtrs &p = d.d;
+  int *const act = p.dense ? nullptr : p.active;
   if (p.mode == RBN_DEPOSIT_BILINEAR)
     fill_one<RBN_DEPOSIT_BILINEAR>(d.g, d.right, d.o, p.off, p.blocksum, p.fill, p.list, p.list_cap, p.overflow,
-                                   p.active, p.nactive, p.active_cap, t);
+                                   act, p.nactive, p.active_cap, t);
   else
-    fill_one<RBN_DEPOSIT_BOX7>(d.g, d.right, d.o, p.off, p.blocksum, p.fill, p.list, p.list_cap, p.overflow,
-                               p.active, p.nactive, p.active_cap, t);
+    fill_one<RBN_DEPOSIT_BOX7>(d.g, d.right, d.o, p.off, p.blocksum, p.fill, p.list, p.list_cap, p.overflow, act,
+                               p.nactive, p.active_cap, t);
 }
 
 __global__ __launch_bounds__(kBlock) void k_nb_fold(const NudgeMember *M, const int *pre_all, int nm) {
   const int *pre = pre_all + NB_FOLD * (nm + 1);
   const int m = member_of(pre, nm, blockIdx.x);
   const NudgeMember &d = M[m];
-  if (d.d.mode == RBN_DEPOSIT_BILINEAR)
-    fold_active<RBN_DEPOSIT_BILINEAR>(d.g, d.o, d.d, blockIdx.x - pre[m], pre[m + 1] - pre[m]);
-  else
-    fold_active<RBN_DEPOSIT_BOX7>(d.g, d.o, d.d, blockIdx.x - pre[m], pre[m + 1] - pre[m]);
+  const int lb = blockIdx.x - pre[m];
+  const bool bil = d.d.mode == RBN_DEPOSIT_BILINEAR;
+  if (d.d.dense) {
+    const int bx = nudge_blocks(d.g), j = lb / bx, i0 = (lb - j * bx) * kBlock;
+    if (bil)
+      fold_row<RBN_DEPOSIT_BILINEAR>(d.g, d.o, d.d, i0, j);
+    else
+      fold_row<RBN_DEPOSIT_BOX7>(d.g, d.o, d.d, i0, j);
+  } else if (bil) {
+    fold_active<RBN_DEPOSIT_BILINEAR>(d.g, d.o, d.d, lb, pre[m + 1] - pre[m]);
+  } else {
+    fold_active<RBN_DEPOSIT_BOX7>(d.g, d.o, d.d, lb, pre[m + 1] - pre[m]);
+  }
 }
 
 }  // namespace
@@ -567,7 +594,7 @@ void nudge_batch_blocks(const NudgeMember &d, int blocks[kNudgeBatches]) {
   blocks[NB_ZERO] = (int)((d.g.plane + kZeroPerBlock - 1) / kZeroPerBlock);
   blocks[NB_CAND] = (int)((nc + kBlock - 1) / kBlock);
   blocks[NB_SCAN] = (int)((d.g.plane + kScanTile - 1) / kScanTile);
-  blocks[NB_FOLD] = fold_blocks(d.d, 256);
+  blocks[NB_FOLD] = d.d.dense ? (d.g.nx + kBlock - 1) / kBlock * d.g.ny : fold_blocks(d.d, 256);
 }
 
 hipError_t launch_nudge_batch(hipStream_t s, const NudgeMember *members, const int *pre, const int *totals, int nm,
@@ -593,7 +620,7 @@ hipError_t launch_deposit(hipStream_t s, const Geom &g, int right_limit, const O
   const bool bil = d.mode == RBN_DEPOSIT_BILINEAR;
   const long long work = (long long)o.n * (bil ? kBilCells : kCells);
   const dim3 ob((unsigned)((work + kBlock - 1) / kBlock > 0 ? (work + kBlock - 1) / kBlock : 1));
-  const dim3 fb((unsigned)fold_blocks(d, 1024));
+  const dim3 fb((unsigned)fold_blocks(d, 1024)), rows((g.nx + kBlock - 1) / kBlock, g.ny);
   if (o.n > 0) {
     if (bil)
       hipLaunchKernelGGL(k_dep_count<RBN_DEPOSIT_BILINEAR>, ob, dim3(kBlock), 0, s, g, right_limit, o, d.cnt);
@@ -609,7 +636,11 @@ hipError_t launch_deposit(hipStream_t s, const Geom &g, int right_limit, const O
     else
       hipLaunchKernelGGL(k_dep_fill<RBN_DEPOSIT_BOX7>, ob, dim3(kBlock), 0, s, g, right_limit, o, d.off,
                          d.blocksum, d);
-    if (bil)
+    if (d.dense && bil)
+      hipLaunchKernelGGL(k_dep_fold_rows<RBN_DEPOSIT_BILINEAR>, rows, dim3(kBlock), 0, s, g, o, d);
+    else if (d.dense)
+      hipLaunchKernelGGL(k_dep_fold_rows<RBN_DEPOSIT_BOX7>, rows, dim3(kBlock), 0, s, g, o, d);
+    else if (bil)
       hipLaunchKernelGGL(k_dep_fold<RBN_DEPOSIT_BILINEAR>, fb, dim3(kBlock), 0, s, g, o, d);
     else
       hipLaunchKernelGGL(k_dep_fold<RBN_DEPOSIT_BOX7>, fb, dim3(kBlock), 0, s, g, o, d);
