@@ -601,7 +601,9 @@ def main() -> int:
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 companion run")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble run")
     ap.add_argument("--ensemble-tuning", type=str, default="",
-                    help="tile_w,tile_h,wg_per_cu,order for the ensemble instead of autotuning")
+                    help="tile_w,tile_h,wg_per_cu,order for the nudged ensemble instead of autotuning")
+    ap.add_argument("--ensemble-kind", choices=("both", "nudged", "truth"), default="both",
+                    help="ensemble legs to run: nudged members (the ensemble configuration), truth-run members, or both")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-window run")
     ap.add_argument("--no-nudging", action="store_true", help="skip the nudging run")
     ap.add_argument("--nud-steps", type=int, default=20000)
@@ -635,10 +637,13 @@ def main() -> int:
     ens = None
     if not args.no_ensemble:
         tun = tuple(int(v) for v in args.ensemble_tuning.split(",")) if args.ensemble_tuning else None
-        ens = run_ensemble(rbn, max(args.steps // 5, 8), 20, tun, nudged=True)
-        print(f"bench: ensemble (nudged members) done {ens}", file=sys.stderr, flush=True)
-        ens["truth_run_members"] = run_ensemble(rbn, max(args.steps // 5, 8), 20, None, nudged=False)
-        print(f"bench: ensemble (truth-run members) done {ens['truth_run_members']}", file=sys.stderr, flush=True)
+        ens = {}
+        if args.ensemble_kind != "truth":
+            ens = run_ensemble(rbn, max(args.steps // 5, 8), 20, tun, nudged=True)
+            print(f"bench: ensemble (nudged members) done {ens}", file=sys.stderr, flush=True)
+        if args.ensemble_kind != "nudged":
+            ens["truth_run_members"] = run_ensemble(rbn, max(args.steps // 5, 8), 20, None, nudged=False)
+            print(f"bench: ensemble (truth-run members) done {ens['truth_run_members']}", file=sys.stderr, flush=True)
     nud = None
     if not args.no_nudging:
         nud = run_nudging(rbn, args.nud_steps, args.nud_spinup)
