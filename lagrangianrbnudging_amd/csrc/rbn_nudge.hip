@@ -285,17 +285,24 @@ __device__ __forceinline__ long long seg_start(const int *off, const int *tile_s
   return (long long)off[c] + tile_sum[c / kScanTile];
 }
 
-// Append node to the active list for the lanes with first set: one atomic per wavefront
-// (the lanes of a wavefront take consecutive entries in lane order).
+// Append node to the active list for the threads with first set: the wavefronts count
+// their entries in LDS and one thread per workgroup takes the workgroup's range with one
+// atomic, so the list's counter sees one atomic per workgroup.  Every thread of the
+// workgroup calls it.
 __device__ __forceinline__ void append_active(bool first, int node, int *active, int *nactive, long long cap) {
+  __shared__ int total, base;
+  if (threadIdx.x == 0) total = 0;
+  __syncthreads();
   const unsigned long long mask = __ballot(first);
-  if (!mask) return;
-  const int lane = __lane_id(), leader = __ffsll((long long)mask) - 1;
-  int base = 0;
-  if (lane == leader) base = atomicAdd(nactive, __popcll(mask));
-  base = __shfl(base, leader);
+  const int lane = __lane_id(), leader = mask ? __ffsll((long long)mask) - 1 : 0;
+  int woff = 0;
+  if (mask && lane == leader) woff = atomicAdd(&total, __popcll(mask));
+  woff = __shfl(woff, leader);
+  __syncthreads();
+  if (threadIdx.x == 0) base = total > 0 ? atomicAdd(nactive, total) : 0;
+  __syncthreads();
   if (first) {
-    const long long a = (long long)base + __popcll(mask & ((1ull << lane) - 1ull));
+    const long long a = (long long)base + woff + __popcll(mask & ((1ull << lane) - 1ull));
     if (a < cap) active[a] = node;
   }
 }
