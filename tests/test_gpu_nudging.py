@@ -185,3 +185,64 @@ def test_fp32_nudged_runs():
     assert_equal_arrays(a["nudge_vol"], b["nudge_vol"], "nudge_vol")  # positions are FP64 data
     assert np.max(np.abs(a["tempreadint"] - b["tempreadint"])) == 0.0
     assert np.max(np.abs(g.fields()["T"] - o.fields()["T"])) < 1e-5
+
+
+def test_observation_table_equals_provider_fp64():
+    # The same records served from the library's table and from a Python provider, with
+    # sparse ids whose order changes between records (the observer map rebuilt) and repeats
+    # (reused): bit-identical runs, and the table run matches the oracle.
+    pn = rb_params(40, 24, 1e5)
+    pn.variant = rbn.RBN_VARIANT_NUDGING
+    pn.ntempread = 3
+    pn.alpha = 0.2
+    pn.deposit = rbn.RBN_DEPOSIT_BILINEAR
+    ids, x0, y0, T, recs = synthetic_observations(pn.nx, pn.ny, 30, 60, 3, seed=17)
+    big = (ids.astype(np.int64) * 7919 + 5).astype(np.int32)
+    for k, (st, r) in enumerate(sorted(recs.items())):
+        r["id"] = big[r["id"]]
+        if k % 2:
+            order = np.argsort(r["id"])
+            for key in ("id", "x", "y", "temp"):
+                r[key] = r[key][order]
+    T0n = conduction_T(pn, 1e-2, seed=5)
+    g, o = _nudged_pair(pn, T0n, big, x0, y0, T, recs, 47)
+    t = rbn.Solver(pn)
+    t.init_fields(T=T0n)
+    t.init_observers(big, x0, y0, T)
+    t.set_observation_table(recs)
+    t.step(47)
+    _compare(t, o)
+    for k in ("rho", "T", "u", "v"):
+        assert_equal_arrays(t.fields()[k], g.fields()[k], k)
+
+
+def test_observation_table_errors():
+    pn = rb_params(32, 16, 1e5)
+    pn.variant = rbn.RBN_VARIANT_NUDGING
+    pn.ntempread = 4
+    g = rbn.Solver(pn)
+    g.init_fields(T=conduction_T(pn, 1e-2, 1))
+    ids = np.array([0, 1], np.int32)
+    g.init_observers(ids, np.array([3.0, 5.0]), np.array([3.0, 5.0]), np.array([0.2, 0.4]))
+    rec = {"id": ids, "x": np.array([3.0, 5.0]), "y": np.array([3.0, 5.0]), "temp": np.array([0.1, 0.2])}
+    lib = g.lib
+    steps = np.array([8, 4], np.int32)  # not ascending
+    cols = np.zeros((2, 2))
+    ip = lambda a: a.ctypes.data_as(rbn.C.POINTER(rbn.C.c_int))
+    dp = lambda a: a.ctypes.data_as(rbn.C.POINTER(rbn.C.c_double))
+    idt = np.zeros((2, 2), np.int32)
+    # RBN_EINVAL: steps not ascending, a missing array
+    assert lib.rbn_set_obs_table(g.ctx, 2, 2, ip(steps), ip(idt), dp(cols), dp(cols), dp(cols)) == -1
+    assert lib.rbn_set_obs_table(g.ctx, 2, 2, None, ip(idt), dp(cols), dp(cols), dp(cols)) == -1
+    # a record the table lacks, or one of the wrong length, is a provider failure at its step
+    g.set_observation_table({8: rec})
+    with pytest.raises(rbn.RbnError) as e:
+        g.step(1)  # step 1 needs record 4
+    assert e.value.code == -10  # RBN_ECALLBACK
+    h = rbn.Solver(pn)
+    h.init_fields(T=conduction_T(pn, 1e-2, 1))
+    h.init_observers(ids, np.array([3.0, 5.0]), np.array([3.0, 5.0]), np.array([0.2, 0.4]))
+    h.set_observation_table({4: {k: v[:1] for k, v in rec.items()}})
+    with pytest.raises(rbn.RbnError) as e:
+        h.step(1)
+    assert e.value.code == -10  # RBN_ECALLBACK
