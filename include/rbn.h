@@ -300,12 +300,14 @@ int rbn_adj_backward(rbn_ctx *ctx, double *dJ_dT0, double *dJ_dalpha);
  *   "lbm_block"  nodes per workgroup along x: 64, 128, 256 (default) or 512
  *   "lbm_npt"    nodes per thread along x: 1 (default), 2, 4 or 8 (row segments of
  *                lbm_block * lbm_npt nodes per workgroup)
- *   "lbm_nt"     1: non-temporal stores of the step's outputs (default 1)
+ *   "lbm_nt"     1: non-temporal stores of the step's outputs (default 1, 0 for lattices whose
+ *                working set fits in 160 MB: about 52 reals per node)
  *   "lbm_ntload" 1: non-temporal loads of the step's inputs (default 0)
  *   "lbm_uv_keep" 1: ordinary (cached) stores of the velocity under lbm_nt, for the particle
  *                update that gathers from it next (default 1)
  *   "lbm_xcd"    1: give each of the 8 XCDs a contiguous band of rows; 2: give each XCD one
- *                row in eight, its segments visited in order (default 0)
+ *                row in eight, its segments visited in order (default 0, 1 for lattices whose
+ *                working set fits in 160 MB)
  *   "lbm_reverse" 1: sweep every odd step last row first, so that it starts on the rows the
  *                previous step wrote last (default 1)
  *   "lbm_steps"  2: truth runs advance two steps per lattice launch, the first step's

@@ -527,6 +527,15 @@ int rbn_create(const rbn_params *p, int device, rbn_ctx **out) {
   g.ny = p->ny;
   g.pitch = (p->nx + 63) / 64 * 64;
   g.plane = (size_t)g.pitch * (size_t)g.ny;
+  // Lattices whose working set (two population plane sets, the force and velocity planes:
+  // about 52 reals per node) fits well inside the 256 MB MALL keep the step's outputs in the
+  // caches for the next step: cached stores and a band of rows per XCD (measured at 1024 x
+  // 512 FP32 nudged: the lattice step 0.018 ms against 0.021 ms).  Larger ones stream past
+  // the caches (non-temporal stores, hardware block order: the fastest at 4096 x 2048).
+  if ((double)g.plane * 52.0 * (double)c->rs <= 160e6) {
+    c->tune.nt = 0;
+    c->tune.xcd = 1;
+  }
   const size_t fb = g.plane * c->rs;
   struct A {
     void **p;
