@@ -248,9 +248,10 @@ int rbn_synchronize(rbn_ctx *ctx);
 /* Many independent contexts of uneven sizes and parameters advanced together: one ensemble
  * step is a persistent launch over a work list of every member's particle chunks (the update
  * of the previous step) and lattice tiles, and cycles of launches are captured in one
- * hipGraph that is replayed.  Nudged members add, per step, their record loads (from their
- * providers, between launches) and one launch per observer/deposit pass over all of them;
- * such ensembles step eagerly, one launch per step.  No reference counterpart (the
+ * hipGraph that is replayed.  Nudged members add, per step, one launch per observer/deposit
+ * pass over all of them and the upload of the records that start an interval (fetched from
+ * their providers before the graph replay that uploads them); such ensembles take one
+ * launch per step, in cycles of lcm(6, record intervals) steps.  No reference counterpart (the
  * reference runs one lattice per job).  Members must share device and precision, be
  * initialised, and stay owned by the caller; results equal stepping each member with
  * rbn_step, except that no trajectory records or diagnostics are produced during ensemble

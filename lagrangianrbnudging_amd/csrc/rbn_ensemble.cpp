@@ -37,6 +37,21 @@ struct rbn_ensemble {
   } stage[2];
   int stage_next = 0;
   std::vector<int> loading, rid;
+  // and their captured cycle: ncycle steps (the least common multiple of the velocity ring
+  // and every record interval; 0: no graph), the record uploads inside it (call step t of
+  // the cycle, the members loading, their staging region), two staging sets and two graph
+  // instances alternating between replays
+  int ncycle = 0;
+  struct CycleLoad {
+    int t;
+    std::vector<int> ld;
+    size_t off, bytes;
+  };
+  std::vector<CycleLoad> cyc;
+  size_t cyc_bytes = 0;
+  Stage cstage[2];
+  hipGraph_t ngraph[2] = {nullptr, nullptr};
+  hipGraphExec_t nexec[2] = {nullptr, nullptr};
   // tuning
   int tile_w = 256, tile_h = 1, wg_per_cu = 4, order = 0;
   bool built = false;
@@ -69,6 +84,12 @@ void drop_graph(rbn_ensemble *e) {
   if (e->graph) (void)hipGraphDestroy(e->graph);
   e->exec = nullptr;
   e->graph = nullptr;
+  for (int k = 0; k < 2; k++) {
+    if (e->nexec[k]) (void)hipGraphExecDestroy(e->nexec[k]);
+    if (e->ngraph[k]) (void)hipGraphDestroy(e->ngraph[k]);
+    e->nexec[k] = nullptr;
+    e->ngraph[k] = nullptr;
+  }
 }
 
 template <typename R>
@@ -103,8 +124,10 @@ void fill_member(const rbn_ctx *c, int tile_w, int tile_h, rbn::EnsMember<R> &d)
   }
 }
 
-// Observer/deposit descriptors of the nudged members as of their current step, and the block
-// prefixes of the batched passes.
+size_t loading_members(const rbn_ensemble *e, int t, std::vector<int> &ld);
+
+// Observer/deposit descriptors of the nudged members as of their current step, the block
+// prefixes of the batched passes, and the record uploads of the captured cycle.
 int build_nudged(rbn_ensemble *e) {
   const int nm = (int)e->nudged.size();
   if (nm == 0) return RBN_OK;
@@ -129,6 +152,34 @@ int build_nudged(rbn_ensemble *e) {
   if (!e->d_npre) EHIP(e, hipMalloc((void **)&e->d_npre, pre.size() * sizeof(int)));
   EHIP(e, hipMemcpy(e->d_nmembers, d.data(), d.size() * sizeof(rbn::NudgeMember), hipMemcpyHostToDevice));
   EHIP(e, hipMemcpy(e->d_npre, pre.data(), pre.size() * sizeof(int), hipMemcpyHostToDevice));
+  // A cycle of L steps replays the same kernel arguments when L is a multiple of the
+  // velocity ring and of every record interval; longer cycles than kMaxNudgedCycle run
+  // eagerly.
+  constexpr int kMaxNudgedCycle = 120;
+  long long L = rbn::kCycle;
+  for (int q : e->nudged) {
+    const long long a = e->m[(size_t)q]->P.ntempread;
+    long long x = L, y = a;
+    while (y) {
+      const long long r = x % y;
+      x = y;
+      y = r;
+    }
+    L = L / x * a;
+    if (L > kMaxNudgedCycle) break;
+  }
+  e->ncycle = L <= kMaxNudgedCycle ? (int)L : 0;
+  e->cyc.clear();
+  e->cyc_bytes = 0;
+  for (int t = 1; t <= e->ncycle; t++) {
+    rbn_ensemble::CycleLoad cl;
+    cl.t = t;
+    cl.bytes = loading_members(e, t, cl.ld);
+    if (!cl.bytes) continue;
+    cl.off = e->cyc_bytes;
+    e->cyc_bytes += cl.bytes;
+    e->cyc.push_back(std::move(cl));
+  }
   return RBN_OK;
 }
 
@@ -325,11 +376,10 @@ void advance_members(rbn_ensemble *e, int n, int launches, int pairs) {
 
 size_t align256(size_t b) { return (b + 255) / 256 * 256; }
 
-// Fetch the records of the nudged members whose record interval starts at step t of the
-// call (their providers, on the host), stage them in one pinned buffer and send them to the
-// observers with one copy and one scatter launch on the ensemble's stream.
-int load_records(rbn_ensemble *e, int t) {
-  std::vector<int> &ld = e->loading;
+// The nudged members whose record interval starts at step t of a call (t counted from the
+// call's first step, their step counts those at its start), and the staging bytes of their
+// records: the jobs, then each member's 3n doubles and n-entry map, 256-aligned.
+size_t loading_members(const rbn_ensemble *e, int t, std::vector<int> &ld) {
   ld.clear();
   size_t data = 0;
   for (int q : e->nudged) {
@@ -339,27 +389,19 @@ int load_records(rbn_ensemble *e, int t) {
       data += align256((size_t)c->nobs * (3 * sizeof(double) + sizeof(int)));
     }
   }
-  if (ld.empty()) return RBN_OK;
-  const size_t jobs_bytes = align256(ld.size() * sizeof(rbn::ObsJob)), need = jobs_bytes + data;
-  rbn_ensemble::Stage &st = e->stage[e->stage_next];
-  e->stage_next ^= 1;
-  if (!st.done) EHIP(e, hipEventCreateWithFlags(&st.done, hipEventDisableTiming));
-  EHIP(e, hipEventSynchronize(st.done));  // this buffer's previous upload has been read
-  if (st.cap < need) {
-    if (st.h) (void)hipHostFree(st.h);
-    if (st.d) (void)hipFree(st.d);
-    st.h = st.d = nullptr;
-    st.cap = 0;
-    EHIP(e, hipHostMalloc((void **)&st.h, need, hipHostMallocDefault));
-    EHIP(e, hipMalloc((void **)&st.d, need));
-    st.cap = need;
-  }
-  auto *jobs = (rbn::ObsJob *)st.h;
-  size_t off = jobs_bytes;
+  return ld.empty() ? 0 : align256(ld.size() * sizeof(rbn::ObsJob)) + data;
+}
+
+// Fetch the records of members ld for step t of the call (their providers, on the host)
+// into the staging region h laid out as loading_members says; job offsets are relative to
+// the region.
+int fetch_records(rbn_ensemble *e, const std::vector<int> &ld, int t, unsigned char *h) {
+  auto *jobs = (rbn::ObsJob *)h;
+  size_t off = align256(ld.size() * sizeof(rbn::ObsJob));
   for (size_t k = 0; k < ld.size(); k++) {
     rbn_ctx *c = e->m[(size_t)ld[k]];
     const int n = c->nobs, ntr = c->P.ntempread, istep = c->istep + t + 1;
-    double *rx = (double *)(st.h + off);
+    double *rx = (double *)(h + off);
     int *chk = (int *)(rx + 3 * (size_t)n);
     e->rid.resize((size_t)std::max(n, 1));
     bool same = false;
@@ -369,9 +411,87 @@ int load_records(rbn_ensemble *e, int t) {
     jobs[k] = {(double *)o.rx, same ? nullptr : (int *)o.chk, (long long)off, n};
     off += align256((size_t)n * (3 * sizeof(double) + sizeof(int)));
   }
-  EHIP(e, hipMemcpyAsync(st.d, st.h, off, hipMemcpyHostToDevice, e->stream));
-  EHIP(e, rbn::launch_obs_scatter(e->stream, (const rbn::ObsJob *)st.d, (int)ld.size(), st.d));
+  return RBN_OK;
+}
+
+// Pinned staging buffer of at least `need` bytes with a device twin.
+int ensure_stage(rbn_ensemble *e, rbn_ensemble::Stage &st, size_t need) {
+  if (!st.done) EHIP(e, hipEventCreateWithFlags(&st.done, hipEventDisableTiming));
+  EHIP(e, hipEventSynchronize(st.done));  // this buffer's previous upload has been read
+  if (st.cap >= need) return RBN_OK;
+  if (st.h) (void)hipHostFree(st.h);
+  if (st.d) (void)hipFree(st.d);
+  st.h = st.d = nullptr;
+  st.cap = 0;
+  EHIP(e, hipHostMalloc((void **)&st.h, need, hipHostMallocDefault));
+  EHIP(e, hipMalloc((void **)&st.d, need));
+  st.cap = need;
+  return RBN_OK;
+}
+
+// Eager step t: fetch the records that start an interval at t, stage them in one pinned
+// buffer and send them to the observers with one copy and one scatter launch.
+int load_records(rbn_ensemble *e, int t) {
+  const size_t need = loading_members(e, t, e->loading);
+  if (!need) return RBN_OK;
+  rbn_ensemble::Stage &st = e->stage[e->stage_next];
+  e->stage_next ^= 1;
+  int rc = ensure_stage(e, st, need);
+  if (rc || (rc = fetch_records(e, e->loading, t, st.h))) return rc;
+  EHIP(e, hipMemcpyAsync(st.d, st.h, need, hipMemcpyHostToDevice, e->stream));
+  EHIP(e, rbn::launch_obs_scatter(e->stream, (const rbn::ObsJob *)st.d, (int)e->loading.size(), st.d));
   EHIP(e, hipEventRecord(st.done, e->stream));
+  return RBN_OK;
+}
+
+// The records of replay r of the captured cycle (call steps r L + 1 .. r L + L) into staging
+// set k.
+int fill_cycle(rbn_ensemble *e, int k, int r) {
+  for (const auto &cl : e->cyc) {
+    int rc = fetch_records(e, cl.ld, cl.t + r * e->ncycle, e->cstage[k].h + cl.off);
+    if (rc) return rc;
+  }
+  return RBN_OK;
+}
+
+int enqueue_fused(rbn_ensemble *e, int t, bool last);
+
+// Capture the cycle of nudged steps 1 .. L of a call twice, reading the records from
+// staging set 0 and set 1: per step the record upload (a copy and a scatter, where members
+// start an interval), the batched observer/deposit passes and the lattice launch.
+int capture_nudged(rbn_ensemble *e) {
+  for (int k = 0; k < 2; k++) {
+    int rc = ensure_stage(e, e->cstage[k], std::max<size_t>(e->cyc_bytes, 256));
+    if (rc) return rc;
+  }
+  for (int k = 0; k < 2; k++) {
+    rbn_ensemble::Stage &st = e->cstage[k];
+    EHIP(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    int rc = RBN_OK;
+    size_t next = 0;
+    for (int t = 1; t <= e->ncycle && !rc; t++) {
+      if (next < e->cyc.size() && e->cyc[next].t == t) {
+        const auto &cl = e->cyc[next++];
+        if (hipMemcpyAsync(st.d + cl.off, st.h + cl.off, cl.bytes, hipMemcpyHostToDevice, e->stream) != hipSuccess ||
+            rbn::launch_obs_scatter(e->stream, (const rbn::ObsJob *)(st.d + cl.off), (int)cl.ld.size(),
+                                    st.d + cl.off) != hipSuccess)
+          rc = efail(e, RBN_EDEVICE, "record upload capture failed");
+      }
+      if (!rc && rbn::launch_nudge_batch(e->stream, (const rbn::NudgeMember *)e->d_nmembers, e->d_npre, e->ntot,
+                                         (int)e->nudged.size(), t) != hipSuccess)
+        rc = efail(e, RBN_EDEVICE, "observer/deposit capture failed");
+      if (!rc) rc = enqueue_fused(e, t, false);
+    }
+    hipGraph_t g = nullptr;
+    hipError_t ee = hipStreamEndCapture(e->stream, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (ee != hipSuccess) return efail(e, RBN_EDEVICE, "hipStreamEndCapture: %s", hipGetErrorString(ee));
+    e->ngraph[k] = g;
+    EHIP(e, hipGraphInstantiate(&e->nexec[k], g, nullptr, nullptr, 0));
+  }
   return RBN_OK;
 }
 
@@ -395,20 +515,42 @@ int run_steps(rbn_ensemble *e, int nsteps) {
   for (int q : e->nudged)
     if (!e->m[(size_t)q]->d_obs_i) return efail(e, RBN_ESTATE, "member %d: nudged run without observers", q);
   if (!e->nudged.empty()) {
-    // With nudged members every step runs eagerly: the records of the members whose record
-    // interval starts (host providers, one upload on this stream), the batched observer
-    // update and deposit, then the lattice launch carrying the previous step's particle
-    // updates; the last step's updates run on their own.
-    for (int t = 0; t < nsteps; t++) {
-      if ((rc = load_records(e, t))) return rc;
+    // With nudged members a step is: the records of the members whose record interval
+    // starts (host providers, one upload on this stream), the batched observer update and
+    // deposit, then the lattice launch carrying the previous step's particle updates.  Step
+    // 0 runs eagerly, whole cycles of steps 1 .. L from the graphs (their records fetched
+    // on the host before each replay, into the set the previous replay but one has read),
+    // the rest eagerly (the last writing the moments), then the last step's particle
+    // updates on their own.
+    auto eager = [&](int t) {
+      int r = load_records(e, t);
+      if (r) return r;
       EHIP(e, rbn::launch_nudge_batch(e->stream, (const rbn::NudgeMember *)e->d_nmembers, e->d_npre, e->ntot,
                                       (int)e->nudged.size(), t));
-      if ((rc = enqueue_fused(e, t, t == nsteps - 1))) return rc;
+      return enqueue_fused(e, t, t == nsteps - 1);
+    };
+    const int L = e->ncycle;
+    const int cycles = L > 0 && nsteps >= 2 ? (nsteps - 2) / L : 0;
+    if ((rc = eager(0))) return rc;
+    if (cycles > 0 && !e->nexec[0] && (rc = capture_nudged(e))) return rc;
+    for (int r = 0; r < cycles; r++) {
+      rbn_ensemble::Stage &st = e->cstage[r & 1];
+      EHIP(e, hipEventSynchronize(st.done));
+      if ((rc = fill_cycle(e, r & 1, r))) return rc;
+      EHIP(e, hipGraphLaunch(e->nexec[r & 1], e->stream));
+      EHIP(e, hipEventRecord(st.done, e->stream));
     }
+    for (int t = 1 + cycles * L; t < nsteps; t++)
+      if ((rc = eager(t))) return rc;
     if (e->n_part > 0 && (rc = launch(e, e->stream, 1, (nsteps - 1) % rbn::kCycle, false))) return rc;
     EHIP(e, hipStreamSynchronize(e->stream));
     advance_members(e, nsteps, nsteps, 0);
-    e->built = false;  // the observer descriptors hold the members' step counts
+    // The observer descriptors and the cycle's record plan hold the members' step counts
+    // modulo the cycle: they (and the graphs) stay valid after a whole number of cycles.
+    if (L == 0 || nsteps % L) {
+      e->built = false;
+      drop_graph(e);
+    }
     return RBN_OK;
   }
   int launches = nsteps, npairs = 0;
@@ -555,10 +697,10 @@ void rbn_ensemble_destroy(rbn_ensemble *e) {
   if (e->d_counters) (void)hipFree(e->d_counters);
   if (e->d_nmembers) (void)hipFree(e->d_nmembers);
   if (e->d_npre) (void)hipFree(e->d_npre);
-  for (auto &st : e->stage) {
-    if (st.h) (void)hipHostFree(st.h);
-    if (st.d) (void)hipFree(st.d);
-    if (st.done) (void)hipEventDestroy(st.done);
+  for (auto *set : {&e->stage[0], &e->stage[1], &e->cstage[0], &e->cstage[1]}) {
+    if (set->h) (void)hipHostFree(set->h);
+    if (set->d) (void)hipFree(set->d);
+    if (set->done) (void)hipEventDestroy(set->done);
   }
   if (e->ev_lbm) (void)hipEventDestroy(e->ev_lbm);
   for (hipEvent_t ev : e->ev_part)

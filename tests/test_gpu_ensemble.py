@@ -68,9 +68,10 @@ NUDGED = [(48, 24, 1e5, 40, 4, rbn.RBN_DEPOSIT_BOX7, 0.3), (64, 40, 3e5, 90, 5, 
           (40, 24, 5e4, 0, 5, rbn.RBN_DEPOSIT_BILINEAR, 0.1), (72, 32, 2e5, 25, 3, rbn.RBN_DEPOSIT_BOX7, 0.05)]
 
 
-def _mixed_members(prec, table):
+def _mixed_members(prec, table, intervals=None):
     """Two truth-run members with particles, then the nudged members (one without observers);
-    observations through the library's table (ensemble) or a Python provider (reference)."""
+    observations through the library's table (ensemble) or a Python provider (reference).
+    intervals: the nudged members' record intervals instead of NUDGED's."""
     out = []
     for k, (nx, ny, ra) in enumerate(SHAPES[:2]):
         p = rb_params(nx, ny, ra, noutconfig=10 ** 6)
@@ -81,6 +82,8 @@ def _mixed_members(prec, table):
         s.init_particles(ids, x, y, tau_p=tau)
         out.append(s)
     for k, (nx, ny, ra, nobs, ntr, dep, alpha) in enumerate(NUDGED):
+        if intervals is not None:
+            ntr = intervals[k]
         p = rb_params(nx, ny, ra, noutconfig=ntr)
         p.precision = prec
         p.variant = rbn.RBN_VARIANT_NUDGING
@@ -90,12 +93,12 @@ def _mixed_members(prec, table):
         s = rbn.Solver(p)
         s.init_fields(T=conduction_T(p, 1e-2, seed=80 + k))
         if nobs > 0:
-            ids, x0, y0, T, recs = synthetic_observations(nx, ny, nobs, 240, ntr, seed=90 + k)
+            ids, x0, y0, T, recs = synthetic_observations(nx, ny, nobs, 420, ntr, seed=90 + k)
             s.init_observers(ids, x0, y0, T)
         else:  # no observers: records with no entries
             none = {"id": np.zeros(0, np.int32), "x": np.zeros(0), "y": np.zeros(0), "temp": np.zeros(0)}
             s.init_observers(none["id"], none["x"], none["y"], none["temp"])
-            recs = {st: none for st in range(ntr, 240 + ntr, ntr)}
+            recs = {st: none for st in range(ntr, 420 + ntr, ntr)}
         if table:
             s.set_observation_table(recs)
         else:
@@ -124,17 +127,23 @@ def _compare_nudged(a, b):
 
 
 @pytest.mark.parametrize("prec", [rbn.RBN_FP64, rbn.RBN_FP32])
-@pytest.mark.parametrize("tuning", [(256, 1, 0, 0), (128, 4, 4, 3), (64, 8, 2, 1), (256, 1, 4, 10)])
-def test_ensemble_with_nudged_members_equals_individual_steps(prec, tuning):
+@pytest.mark.parametrize("tuning,intervals", [((256, 1, 0, 0), None), ((128, 4, 4, 3), None), ((64, 8, 2, 1), None),
+                                              ((256, 1, 4, 10), None), ((256, 1, 0, 2), (3, 3, 2, 6)),
+                                              ((128, 4, 4, 1), (3, 3, 2, 6))])
+def test_ensemble_with_nudged_members_equals_individual_steps(prec, tuning, intervals):
     # Nudged members (7x7 and bilinear deposits, different record intervals, one without
     # observers) beside truth-run members; the observer updates and deposits of all nudged
-    # members run as batched launches, records are loaded between steps.  Two-step and
-    # separate-launch orders fall back to one launch per step.
-    ens_members = _mixed_members(prec, table=True)
-    ref_members = _mixed_members(prec, table=False)
+    # members run as batched launches, records are uploaded between steps or, inside the
+    # captured cycles (60 steps for intervals 4, 5, 5, 3; 6 steps for 3, 3, 2, 6), by the
+    # graph from a staging set the host fills before each replay.  Calls of 122 steps (two
+    # cycles of 60 and eager steps), 120 (whole cycles: the graphs stay for the next call)
+    # and 62 (a replay of the kept graph).  Two-step and separate-launch orders fall back to
+    # one launch per step.
+    ens_members = _mixed_members(prec, table=True, intervals=intervals)
+    ref_members = _mixed_members(prec, table=False, intervals=intervals)
     e = rbn.Ensemble(ens_members)
     e.set_tuning(*tuning)
-    calls = (9, 1, 2, 6, 13, 20)
+    calls = (9, 1, 2, 6, 13, 20, 122, 120, 62)
     for n in calls:
         e.step(n)
     for s in ref_members:
