@@ -269,7 +269,8 @@ int rbn_ensemble_step(rbn_ensemble *e, int nsteps);
  * items last first; bit 2: a lattice launch and a particle launch per step, the particle
  * launch on a second stream beside the next lattice launch, instead of one launch; bit 3
  * without bit 2: two steps per launch on 64-node-wide tiles of 8 rows in FP32, 4 in FP64,
- * whatever tile_w and tile_h say, except for a single step at the end of a call). */
+ * whatever tile_w and tile_h say, except for a single step at the end of a call; bit 4: no
+ * graph capture, every launch eager, e.g. for tracing tools). */
 int rbn_ensemble_set_tuning(rbn_ensemble *e, int tile_w, int tile_h, int wg_per_cu, int order);
 int rbn_ensemble_get_tuning(const rbn_ensemble *e, int *tile_w, int *tile_h, int *wg_per_cu,
                             int *order);

@@ -254,6 +254,9 @@ bool separate(const rbn_ensemble *e) { return (e->order & 4) != 0 && e->nudged.e
 // particle updates carried.  Not with nudged members (a nudged step has no two-step form).
 bool pairs(const rbn_ensemble *e) { return (e->order & 8) != 0 && !separate(e) && e->nudged.empty(); }
 
+// order bit 4: no graphs, every launch eager (for debugging and tracing tools).
+bool graphs(const rbn_ensemble *e) { return (e->order & 16) == 0; }
+
 // Steps of one replay of the captured graph: the velocity ring's period, doubled in the
 // two-step mode so that the A/B and F planes (flipped once per launch) also return.
 int graph_steps(const rbn_ensemble *e) { return pairs(e) ? 2 * rbn::kCycle : rbn::kCycle; }
@@ -530,7 +533,7 @@ int run_steps(rbn_ensemble *e, int nsteps) {
       return enqueue_fused(e, t, t == nsteps - 1);
     };
     const int L = e->ncycle;
-    const int cycles = L > 0 && nsteps >= 2 ? (nsteps - 2) / L : 0;
+    const int cycles = graphs(e) && L > 0 && nsteps >= 2 ? (nsteps - 2) / L : 0;
     if ((rc = eager(0))) return rc;
     if (cycles > 0 && !e->nexec[0] && (rc = capture_nudged(e))) return rc;
     for (int r = 0; r < cycles; r++) {
@@ -561,7 +564,7 @@ int run_steps(rbn_ensemble *e, int nsteps) {
     PairState ps;
     if ((rc = enqueue_pair(e, ps, nsteps, nsteps <= 2))) return rc;
     const int span = 2 * rbn::kCycle;
-    const int cycles = nsteps - ps.t >= 1 ? (nsteps - ps.t - 1) / span : 0;
+    const int cycles = graphs(e) && nsteps - ps.t >= 1 ? (nsteps - ps.t - 1) / span : 0;
     if (cycles > 0) {
       if (!e->exec && (rc = capture(e))) return rc;
       for (int c = 0; c < cycles; c++) EHIP(e, hipGraphLaunch(e->exec, e->stream));
@@ -584,7 +587,7 @@ int run_steps(rbn_ensemble *e, int nsteps) {
     launches = ps.L;
     npairs = ps.P;
   } else if (separate(e)) {
-    const int cycles = nsteps / rbn::kCycle, rem = nsteps % rbn::kCycle;
+    const int cycles = graphs(e) ? nsteps / rbn::kCycle : 0, rem = nsteps - cycles * rbn::kCycle;
     if (cycles > 0) {
       if (!e->exec && (rc = capture(e))) return rc;
       for (int c = 0; c < cycles; c++) EHIP(e, hipGraphLaunch(e->exec, e->stream));
@@ -597,7 +600,7 @@ int run_steps(rbn_ensemble *e, int nsteps) {
     // step when nsteps >= 2, the last writing the moments) eagerly, then the particle
     // update of the last step on its own.
     if ((rc = enqueue_fused(e, 0, nsteps == 1))) return rc;
-    const int cycles = nsteps >= 2 ? (nsteps - 2) / rbn::kCycle : 0;
+    const int cycles = graphs(e) && nsteps >= 2 ? (nsteps - 2) / rbn::kCycle : 0;
     if (cycles > 0) {
       if (!e->exec && (rc = capture(e))) return rc;
       for (int c = 0; c < cycles; c++) EHIP(e, hipGraphLaunch(e->exec, e->stream));
@@ -749,8 +752,8 @@ const char *rbn_ensemble_last_error(const rbn_ensemble *e) { return e ? e->err.c
 int rbn_ensemble_set_tuning(rbn_ensemble *e, int tile_w, int tile_h, int wg_per_cu, int order) {
   if (!e) return rbn::thread_fail(RBN_EINVAL, "null ensemble");
   if (!(tile_w == 64 || tile_w == 128 || tile_w == 256) || tile_h < 1 || tile_h > 64 ||
-      (tile_w * tile_h) % 256 != 0 || wg_per_cu < 0 || wg_per_cu > 32 || order < 0 || order > 15)
-    return efail(e, RBN_EINVAL, "tile must be 64/128/256 wide with a multiple of 256 nodes; wg_per_cu 0..32; order 0..15");
+      (tile_w * tile_h) % 256 != 0 || wg_per_cu < 0 || wg_per_cu > 32 || order < 0 || order > 31)
+    return efail(e, RBN_EINVAL, "tile must be 64/128/256 wide with a multiple of 256 nodes; wg_per_cu 0..32; order 0..31");
   e->tile_w = tile_w;
   e->tile_h = tile_h;
   e->wg_per_cu = wg_per_cu;

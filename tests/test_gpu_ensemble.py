@@ -44,7 +44,7 @@ def _compare(a, b):
 @pytest.mark.parametrize("prec", [rbn.RBN_FP64, rbn.RBN_FP32])
 @pytest.mark.parametrize("tuning", [(256, 1, 0, 0), (256, 2, 4, 1), (64, 8, 2, 2), (128, 4, 8, 3), (256, 1, 0, 2),
                                     (256, 1, 4, 4), (128, 4, 2, 7), (256, 1, 0, 8), (128, 4, 4, 11), (64, 8, 2, 9),
-                                    (256, 2, 8, 10), (128, 4, 2, 15)])
+                                    (256, 2, 8, 10), (128, 4, 2, 15), (256, 1, 0, 16), (128, 4, 4, 20), (256, 1, 4, 26)])
 def test_ensemble_equals_individual_steps(prec, tuning):
     ens_members = _members(prec)
     ref_members = _members(prec)
@@ -129,7 +129,7 @@ def _compare_nudged(a, b):
 @pytest.mark.parametrize("prec", [rbn.RBN_FP64, rbn.RBN_FP32])
 @pytest.mark.parametrize("tuning,intervals", [((256, 1, 0, 0), None), ((128, 4, 4, 3), None), ((64, 8, 2, 1), None),
                                               ((256, 1, 4, 10), None), ((256, 1, 0, 2), (3, 3, 2, 6)),
-                                              ((128, 4, 4, 1), (3, 3, 2, 6))])
+                                              ((128, 4, 4, 1), (3, 3, 2, 6)), ((256, 1, 0, 18), (3, 3, 2, 6))])
 def test_ensemble_with_nudged_members_equals_individual_steps(prec, tuning, intervals):
     # Nudged members (7x7 and bilinear deposits, different record intervals, one without
     # observers) beside truth-run members; the observer updates and deposits of all nudged
