@@ -261,7 +261,9 @@ int rbn_ensemble_create(rbn_ctx *const *members, int n, rbn_ensemble **out);
 void rbn_ensemble_destroy(rbn_ensemble *e);
 const char *rbn_ensemble_last_error(const rbn_ensemble *e);
 /* Advance every member by nsteps (graph replays for whole cycles of six launches; the first
- * and last launches of a call, and the last launch's particle updates, run eagerly). */
+ * and last launches of a call, and the last launch's particle updates, run eagerly).  After
+ * an error (a provider failure, a particle leaving its lattice, a device error) the
+ * members' states are unspecified: initialise them again before further steps. */
 int rbn_ensemble_step(rbn_ensemble *e, int nsteps);
 /* Tile shape (tile_w in {64,128,256} nodes, tile_w*tile_h a multiple of 256), persistent
  * workgroups per CU (0: one workgroup per work item) and work order (bit 0 clear: member by
