@@ -346,8 +346,9 @@ int rbn_get_tuning(const rbn_ctx *ctx, const char *key, int *value);
  * them by a few microseconds; sampling keeps that out of a timed run). */
 int rbn_set_kernel_timing(rbn_ctx *ctx, int every);
 /* Launch count and summed duration (ms) of the timed launches of kernel `name`
- * ("lbm", "particles", "obs", "deposit", "adjoint", "lbm_fused": a one-step lattice launch
- * carrying earlier steps' particle updates, "lbm2": a two-step lattice launch); resets the
+ * ("lbm", "particles", "deposit": a nudged step's observer update and deposit, "adjoint",
+ * "lbm_fused": a one-step lattice launch carrying earlier steps' particle updates, "lbm2": a
+ * two-step lattice launch); resets the
  * counters when reset != 0. */
 int rbn_kernel_stats(rbn_ctx *ctx, const char *name, int reset, long long *launches,
                      double *total_ms);

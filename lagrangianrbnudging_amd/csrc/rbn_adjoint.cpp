@@ -128,10 +128,8 @@ int launch_step_kernels(rbn_ctx *c, int istep, const rbn::ObsPtrs &o, const rbn:
   const int r = istep % P.ntempread;
   const int branch = r == 1 ? 1 : (r == 0 ? 0 : 2);
   const int right = P.fix_q3 ? P.nx - 3 : P.nx - 2;
-  int rc;
-  if ((rc = timed(c, KT_OBS, c->stream, [&] { return rbn::launch_obs_update(c->stream, branch, P.nx, P.ny, P.ntempread, o); })))
-    return rc;
-  return timed(c, KT_DEPOSIT, c->stream, [&] { return rbn::launch_deposit(c->stream, c->geom, right, o, d); });
+  return timed(c, KT_DEPOSIT, c->stream,
+               [&] { return rbn::launch_obs_deposit(c->stream, branch, P.ntempread, c->geom, right, o, d); });
 }
 
 int ensure_forward_storage(rbn_ctx *c, int W, int K) {

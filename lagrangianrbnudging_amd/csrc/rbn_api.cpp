@@ -83,7 +83,7 @@ void fill_consts(const rbn_params &P, rbn::StepConst<R> &c) {
   }
 }
 
-const char *const kKernelNames[KT_COUNT] = {"lbm", "particles", "obs", "deposit", "adjoint", "lbm_fused", "lbm2"};
+const char *const kKernelNames[KT_COUNT] = {"lbm", "particles", "deposit", "adjoint", "lbm_fused", "lbm2"};
 
 int kernel_index(const char *name) {
   for (int k = 0; k < KT_COUNT; k++)
@@ -1138,9 +1138,9 @@ int step_loop(rbn_ctx *c, int nsteps) {
       if (r == 1 && (rc = load_observation(c, istep + P.ntempread - 1))) return rc;
       const int branch = r == 1 ? 1 : (r == 0 ? 0 : 2);
       rbn::ObsPtrs o = obs_ptrs(c);
-      if ((rc = timed(c, KT_OBS, c->stream, [&] { return rbn::launch_obs_update(c->stream, branch, P.nx, P.ny, P.ntempread, o); })))
-        return rc;
-      if ((rc = timed(c, KT_DEPOSIT, c->stream, [&] { return rbn::launch_deposit(c->stream, c->geom, right, o, c->dep); })))
+      if ((rc = timed(c, KT_DEPOSIT, c->stream, [&] {
+             return rbn::launch_obs_deposit(c->stream, branch, P.ntempread, c->geom, right, o, c->dep);
+           })))
         return rc;
     }
     // A launch writes the velocity slots after cur: K1 of step s overwrites the slot of step

@@ -184,10 +184,10 @@ hipError_t launch_sample(hipStream_t s, int prec, const Geom &g, const void *fie
                          const double *x, const double *y, double *out, int *err);
 hipError_t launch_particle_init(hipStream_t s, int prec, const Geom &g, const void *uv,
                                 const PartPtrs &pp, double heavy_factor);
-hipError_t launch_obs_update(hipStream_t s, int branch, int nx, int ny, int ntempread,
-                             const ObsPtrs &o);
-hipError_t launch_deposit(hipStream_t s, const Geom &g, int right_limit, const ObsPtrs &o,
-                          const DepPtrs &d);
+// One nudged step's observer update (branch: 1 first step of a record interval, 0 its last,
+// 2 the others) and its deposit (counts, scan, contribution lists, fold).
+hipError_t launch_obs_deposit(hipStream_t s, int branch, int ntempread, const Geom &g, int right_limit,
+                              const ObsPtrs &o, const DepPtrs &d);
 // Blocks member d takes in each pass of a batched observer/deposit launch.
 void nudge_batch_blocks(const NudgeMember &d, int blocks[kNudgeBatches]);
 // Observer update and deposit of step t of a call (t = 0 is the first) for nm members at
@@ -299,7 +299,7 @@ struct rbn_ctx {
   double *d_energy = nullptr;  // partials + 2 results
 
   // timing: every timing_every-th launch of each kernel class is bracketed with events
-  static constexpr int kKernelClasses = 7;
+  static constexpr int kKernelClasses = 6;
   bool timing = false;
   int timing_every = 1;
   long long k_seen[kKernelClasses] = {};

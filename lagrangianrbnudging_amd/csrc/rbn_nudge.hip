@@ -122,11 +122,6 @@ __device__ __forceinline__ void obs_update_one(const ObsPtrs &o, int i, int bran
   }
 }
 
-__global__ void k_obs_update(ObsPtrs o, int branch, int nx, int ny, int ntempread) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < o.n) obs_update_one(o, i, branch, nx, ny, ntempread);
-}
-
 // ---------------------------------------------------------------- deposit
 
 // The nodes observer k deposits into (N:1568-1616, global form): the 7x7 box at its rounded
@@ -186,8 +181,18 @@ __device__ __forceinline__ void zero_part(size_t n, int *cnt, int *fill, int *na
   }
 }
 
-__global__ void k_dep_zero(size_t n, int *cnt, int *fill, int *nactive) {
-  zero_part(n, cnt, fill, nactive, blockIdx.x, gridDim.x);
+// The step's observer update and the deposit's zero pass in one launch: they touch disjoint
+// arrays (observer state; counts, fill cursors and the active-node count), so blocks
+// [0, obs_blocks) update observers and the rest zero the counts.
+__global__ __launch_bounds__(kBlock) void k_obs_zero(ObsPtrs o, int branch, int nx, int ny, int ntempread,
+                                                     int obs_blocks, size_t n, int *cnt, int *fill, int *nactive) {
+  const int b = blockIdx.x;
+  if (b < obs_blocks) {
+    const int i = b * kBlock + threadIdx.x;
+    if (i < o.n) obs_update_one(o, i, branch, nx, ny, ntempread);
+  } else {
+    zero_part(n, cnt, fill, nactive, b - obs_blocks, gridDim.x - obs_blocks);
+  }
 }
 
 // Node of candidate c of observer k for deposit MODE (-1: none); *key is the list key.
@@ -487,21 +492,24 @@ inline int fold_blocks(const DepPtrs &d, int max_blocks) {
   return (int)std::max<long long>(1, std::min<long long>((d.active_cap + kBlock - 1) / kBlock, max_blocks));
 }
 
-__global__ __launch_bounds__(kBlock) void k_nb_obs(const NudgeMember *M, const int *pre_all, int nm, int t) {
-  const int *pre = pre_all + NB_OBS * (nm + 1);
-  const int m = member_of(pre, nm, blockIdx.x);
-  const NudgeMember &d = M[m];
-  const int i = (blockIdx.x - pre[m]) * kBlock + threadIdx.x;
-  // the step's place in the record interval, as the single-member loop derives it
-  const int istep = d.istep0 + t + 1, r = istep % d.ntempread;
-  const int branch = r == 1 ? 1 : (r == 0 ? 0 : 2);
-  if (i < d.o.n) obs_update_one(d.o, i, branch, d.g.nx, d.g.ny, d.ntempread);
-}
-
-__global__ __launch_bounds__(kBlock) void k_nb_zero(const NudgeMember *M, const int *pre_all, int nm) {
-  const int *pre = pre_all + NB_ZERO * (nm + 1);
-  const int m = member_of(pre, nm, blockIdx.x);
-  zero_part(M[m].g.plane, M[m].d.cnt, M[m].d.fill, M[m].d.nactive, blockIdx.x - pre[m], pre[m + 1] - pre[m]);
+// Observer updates of every member (the first NB_OBS total blocks), then the zero passes of
+// every member's deposit (disjoint arrays, as in k_obs_zero).
+__global__ __launch_bounds__(kBlock) void k_nb_obs_zero(const NudgeMember *M, const int *pre_all, int nm, int t) {
+  const int *pre_obs = pre_all + NB_OBS * (nm + 1);
+  const int nobs = pre_obs[nm];
+  if ((int)blockIdx.x < nobs) {
+    const int m = member_of(pre_obs, nm, blockIdx.x);
+    const NudgeMember &d = M[m];
+    const int i = (blockIdx.x - pre_obs[m]) * kBlock + threadIdx.x;
+    // the step's place in the record interval, as the single-member loop derives it
+    const int istep = d.istep0 + t + 1, r = istep % d.ntempread;
+    const int branch = r == 1 ? 1 : (r == 0 ? 0 : 2);
+    if (i < d.o.n) obs_update_one(d.o, i, branch, d.g.nx, d.g.ny, d.ntempread);
+  } else {
+    const int *pre = pre_all + NB_ZERO * (nm + 1);
+    const int zb = blockIdx.x - nobs, m = member_of(pre, nm, zb);
+    zero_part(M[m].g.plane, M[m].d.cnt, M[m].d.fill, M[m].d.nactive, zb - pre[m], pre[m + 1] - pre[m]);
+  }
 }
 
 __global__ __launch_bounds__(kBlock) void k_nb_count(const NudgeMember *M, const int *pre_all, int nm) {
@@ -567,14 +575,6 @@ __global__ __launch_bounds__(kBlock) void k_nb_fold(const NudgeMember *M, const 
 
 }  // namespace
 
-hipError_t launch_obs_update(hipStream_t s, int branch, int nx, int ny, int ntempread,
-                             const ObsPtrs &o) {
-  if (o.n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_obs_update, dim3((o.n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, o,
-                     branch, nx, ny, ntempread);
-  return hipGetLastError();
-}
-
 namespace {
 
 __global__ __launch_bounds__(kBlock) void k_obs_scatter(const ObsJob *jobs, const unsigned char *staging) {
@@ -608,8 +608,7 @@ hipError_t launch_nudge_batch(hipStream_t s, const NudgeMember *members, const i
                               int t) {
   if (nm <= 0) return hipSuccess;
   const dim3 b(kBlock);
-  if (totals[NB_OBS] > 0) hipLaunchKernelGGL(k_nb_obs, dim3(totals[NB_OBS]), b, 0, s, members, pre, nm, t);
-  hipLaunchKernelGGL(k_nb_zero, dim3(totals[NB_ZERO]), b, 0, s, members, pre, nm);
+  hipLaunchKernelGGL(k_nb_obs_zero, dim3(totals[NB_OBS] + totals[NB_ZERO]), b, 0, s, members, pre, nm, t);
   if (totals[NB_CAND] > 0) hipLaunchKernelGGL(k_nb_count, dim3(totals[NB_CAND]), b, 0, s, members, pre, nm);
   hipLaunchKernelGGL(k_nb_scan_tiles, dim3(totals[NB_SCAN]), b, 0, s, members, pre, nm);
   hipLaunchKernelGGL(k_nb_scan_sums, dim3(nm), b, 0, s, members);
@@ -618,12 +617,14 @@ hipError_t launch_nudge_batch(hipStream_t s, const NudgeMember *members, const i
   return hipGetLastError();
 }
 
-hipError_t launch_deposit(hipStream_t s, const Geom &g, int right_limit, const ObsPtrs &o,
-                          const DepPtrs &d) {
+hipError_t launch_obs_deposit(hipStream_t s, int branch, int ntempread, const Geom &g, int right_limit,
+                              const ObsPtrs &o, const DepPtrs &d) {
   const size_t n = g.plane;
   const int ntiles = (int)((n + kScanTile - 1) / kScanTile);
   const int zb = (int)std::min<size_t>((n + kBlock - 1) / kBlock, 4096);
-  hipLaunchKernelGGL(k_dep_zero, dim3(zb), dim3(kBlock), 0, s, n, d.cnt, d.fill, d.nactive);
+  const int obs_blocks = (o.n + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(k_obs_zero, dim3(obs_blocks + zb), dim3(kBlock), 0, s, o, branch, g.nx, g.ny, ntempread,
+                     obs_blocks, n, d.cnt, d.fill, d.nactive);
   const bool bil = d.mode == RBN_DEPOSIT_BILINEAR;
   const long long work = (long long)o.n * (bil ? kBilCells : kCells);
   const dim3 ob((unsigned)((work + kBlock - 1) / kBlock > 0 ? (work + kBlock - 1) / kBlock : 1));
