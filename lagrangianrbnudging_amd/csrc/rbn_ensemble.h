@@ -30,11 +30,13 @@ struct EnsMember {
   double *part;  // 10 arrays of np (x, y, u, v, xo, yo, uo, vo, 1/tau, beta)
   int np;
   int *err;
-  // nudged members (variant RBN_VARIANT_NUDGING): the deposit the step reads and the
-  // persistent nudging term; wq only for the bilinear deposit
+  // nudged members (variant RBN_VARIANT_NUDGING): the deposits the steps read, [k & 1] for
+  // step k (1: the member's own arrays, 0: the ensemble's second set, so that the deposit of
+  // step k + 1 can be built while step k runs), and the persistent nudging term; wq only for
+  // the bilinear deposit
   int variant;
-  const int *nvol;
-  const double *Tbar, *wq;
+  const int *nvol[2];
+  const double *Tbar[2], *wq[2];
   R *nud;
 };
 

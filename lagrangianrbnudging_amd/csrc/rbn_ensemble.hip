@@ -53,9 +53,9 @@ __device__ __forceinline__ void lbm_item(const EnsMember<R> &m, int tile, int k,
   p.T = write_moments ? m.T : nullptr;
   const int col = threadIdx.x % tile_w;
   if (m.variant == RBN_VARIANT_NUDGING) {
-    p.nvol = m.nvol;
-    p.Tbar = m.Tbar;
-    p.wq = m.wq;
+    p.nvol = m.nvol[par];
+    p.Tbar = m.Tbar[par];
+    p.wq = m.wq[par];
     p.nud = m.nud;
     for (int r = threadIdx.x / tile_w; r < tile_h; r += kThreads / tile_w) {
       const int i = tx * tile_w + col, j = ty * tile_h + r;
