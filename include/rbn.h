@@ -251,7 +251,10 @@ int rbn_synchronize(rbn_ctx *ctx);
  * hipGraph that is replayed.  Nudged members add, per step, one launch per observer/deposit
  * pass over all of them and the upload of the records that start an interval (fetched from
  * their providers before the graph replay that uploads them); such ensembles take one
- * launch per step, in cycles of lcm(6, record intervals) steps.  No reference counterpart (the
+ * launch per step, in cycles of lcm(6, record intervals) steps.  Their eager steps build the
+ * next step's deposit (a second set of deposit arrays per member, owned by the ensemble) on
+ * a second stream beside the current lattice launch; graph replays run the passes in order
+ * on one stream.  No reference counterpart (the
  * reference runs one lattice per job).  Members must share device and precision, be
  * initialised, and stay owned by the caller; results equal stepping each member with
  * rbn_step, except that no trajectory records or diagnostics are produced during ensemble
@@ -279,7 +282,10 @@ int rbn_ensemble_get_tuning(const rbn_ensemble *e, int *tile_w, int *tile_h, int
 /* Time every candidate launch shape on trial_steps real steps, keep the fastest, and restore
  * every member's state afterwards; *best_ms_per_step receives the winner's time.  With
  * nudged members every candidate steps from the starting state (their records must cover
- * 2 * max(trial_steps, 6) steps from it). */
+ * 2 * max(trial_steps, 6) and 2 L + 2 steps from it, L = lcm(6, record intervals) <= 120),
+ * and the fastest shape is timed once more over steps 1 .. 2L of a call in both forms,
+ * graph replays and eager steps with the deposits overlapped (order bit 4), keeping the
+ * faster. */
 int rbn_ensemble_autotune(rbn_ensemble *e, int trial_steps, double *best_ms_per_step);
 
 /* ---------------------------------------------------------------- adjoint window */
