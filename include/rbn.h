@@ -333,6 +333,12 @@ int rbn_adj_backward(rbn_ctx *ctx, double *dJ_dT0, double *dJ_dalpha);
  *   "lbm2_variant" 1: the row sweep issues the loads of a row one row ahead (default 0)
  *   "lbm2_tile_h" rows of a two-step tile of 64 columns (lbm2_seg 0): 4, 8 or 16 (FP32
  *                only), 0 for the default (8 in FP32, 4 in FP64)
+ *   "dep_overlap" nudged run, 1: the observer update and deposit of step s+1 run on a second
+ *                stream beside the lattice update of step s (a second set of deposit arrays,
+ *                allocated on first use); 0: in turn on one stream (default 0: with one
+ *                lattice the hand-over between the streams costs more than it hides, e.g.
+ *                1024x512 FP32 with 1e4 observers 54 against 50 us per step; ensembles,
+ *                whose lattice launch is much longer than their deposits, gain from it)
  *   "particle_overlap"  1: run the particle update of step s on its own stream concurrently
  *                with the lattice update of step s+1 (default 0)
  *   "particle_fuse" 1: run the particle update of step s as extra workgroups of the lattice

@@ -138,6 +138,7 @@ struct LbmTuning {
   int seg = 0;      // two steps by row sweeps over segments of seg rows (8..64); 0: by tiles
   int xcd2 = 1;     // row sweeps: consecutive strips and segments on one XCD
   int variant = 0;  // row sweeps: 1 pulls one row ahead
+  int dep_overlap = 0;  // nudged run: observer update and deposit of step s+1 beside the lattice step s
 };
 
 // The particle updates of one or two consecutive steps, in step order, carried by the next
@@ -233,6 +234,10 @@ struct rbn_ctx {
   hipStream_t pstream = nullptr;       // particle stream (default priority)
   hipEvent_t ev_lbm = nullptr;         // K1 of the latest step finished
   hipEvent_t ev_part[2] = {};          // K2 of step s finished, by s % 2
+  // nudged run with dep_overlap: the second deposit set (allocated on first use) and the
+  // events between the deposits (pstream) and the lattice launches (stream), by step parity
+  rbn::DepPtrs dep2{};
+  hipEvent_t ev_dep[2] = {}, ev_lat[2] = {};
   bool part_record_pending = false;    // K2 of the latest step reads T (record step)
   void *nud = nullptr;
   int istep = 0;

@@ -101,3 +101,34 @@ def test_ensemble_config_sizes_equal_individual_steps():
             for key in ("x", "y", "u", "v"):
                 assert_equal_arrays(pa[key], pb[key], f"member {k} particle {key}")
     e.close()
+
+
+@pytest.mark.parametrize("prec", [rbn.RBN_FP64, rbn.RBN_FP32])
+def test_deposit_overlap_equals_in_turn(prec):
+    # dep_overlap 1 (deposits on a second stream, two deposit sets by step parity) against 0,
+    # over calls of odd and even lengths, both ending on either set; the getters too.
+    p = rb_params(256, 128, 1e7, noutconfig=5)
+    p.precision = prec
+    p.variant = rbn.RBN_VARIANT_NUDGING
+    p.ntempread = 5
+    p.alpha = 0.1
+    p.deposit = rbn.RBN_DEPOSIT_BILINEAR
+    ids, x0, y0, T, recs = synthetic_observations(p.nx, p.ny, 2000, 40, 5, seed=31)
+    T0 = conduction_T(p, 1e-2, seed=32)
+    sims = []
+    for ovl in (0, 1):
+        s = rbn.Solver(p)
+        s.set_tuning(dep_overlap=ovl)
+        s.init_fields(T=T0)
+        s.init_observers(ids, x0, y0, T)
+        s.set_observation_table(recs)
+        for n in (1, 2, 7, 12, 5, 3):
+            s.step(n)
+            sims.append((ovl, n, s.nudging()))
+        fa, ga = s.populations()
+        sims.append((ovl, "final", {"f": fa, "g": ga}))
+        s.close()
+    half = len(sims) // 2
+    for (_, n, a), (_, _, b) in zip(sims[:half], sims[half:]):
+        for k in a:
+            assert_equal_arrays(a[k], b[k], f"after call {n}: {k}")
