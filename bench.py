@@ -122,7 +122,7 @@ def run_gpu(rbn, precision, steps, warmup, timing=True, records_to_host=False, *
     s.step(steps)
     s.synchronize()
     wall = time.perf_counter() - t0
-    out = {"wall_s": wall, "mlups": NX * NY * steps / wall / 1e6, "ms_per_step": 1e3 * wall / steps}
+    out = {"wall_s": wall, "steps": steps, "mlups": NX * NY * steps / wall / 1e6, "ms_per_step": 1e3 * wall / steps}
     if timing:
         # sampled launch counts and summed durations per kernel class
         n_lbm, ms_lbm = s.kernel_stats("lbm")
@@ -200,12 +200,33 @@ def ens_members(rbn, nudged, last_step, seed=7):
     return members, nodes, nobs
 
 
+ENS_TRIAL_STEPS = 8
+
+
+def ens_autotune_steps(trial, ntempread):
+    """Steps from its starting state over which rbn_ensemble_autotune advances nudged members
+    whose records come every `ntempread` steps (include/rbn.h): each candidate's warm-up call
+    of 6 steps and timed call of 6 * max(1, ceil(trial / 6)) steps, and the comparison of the
+    two forms over 2 L + 2 steps, L = lcm(6, ntempread); the state is restored after each."""
+    lcm = 6 * ntempread // int(np.gcd(6, ntempread))
+    return max(6 + 6 * max(1, -(-trial // 6)), 2 * lcm + 2)
+
+
+def ens_record_horizon(trial, warmup, ens_warm, steps, ntempread):
+    """The last step whose observation record a nudged ensemble leg reads: the sequential
+    reference (warmup + steps), the autotuner (restores the state afterwards) and the ensemble
+    run (ens_warm + steps), each reading at step s with s % ntempread == 1 the record of step
+    s + ntempread - 1."""
+    advanced = max(warmup + steps, ens_autotune_steps(trial, ntempread), ens_warm + steps)
+    return advanced + ntempread - 1
+
+
 def run_ensemble(rbn, steps, warmup, tuning=None, nudged=True):
     """The batched ensemble (autotuned, or the given launch shape (tile_w, tile_h, wg_per_cu,
     order)) against the same members, built afresh, stepped one after another with their
     own launches (rbn_step)."""
     ens_warm = warmup - warmup % 12 + 12
-    last = max(warmup + steps, 2 * max(8, 6) + ens_warm + steps) + 2 * ENS_NTEMPREAD
+    last = ens_record_horizon(ENS_TRIAL_STEPS, warmup, ens_warm, steps, ENS_NTEMPREAD)
     members, nodes, nobs = ens_members(rbn, nudged, last)
     for s in members:
         s.step(warmup)
@@ -222,7 +243,7 @@ def run_ensemble(rbn, steps, warmup, tuning=None, nudged=True):
     members, nodes, nobs = ens_members(rbn, nudged, last)
     e = rbn.Ensemble(members)
     if tuning is None:
-        best_trial = e.autotune(8)
+        best_trial = e.autotune(ENS_TRIAL_STEPS)
     else:
         e.set_tuning(*tuning)
         best_trial = float("nan")
@@ -590,12 +611,65 @@ def finite_or_null(v):
     return v
 
 
+# The headline is timed over at least this many steps after at least MIN_WARMUP warm-up
+# steps, whatever --steps / --warmup ask for (a steady window of the benchmark workload).
+MIN_TIMED_STEPS, MIN_WARMUP = 1000, 50
+# Ensemble legs: at least this many timed ensemble steps (ten record cycles of 30 steps).
+MIN_ENS_STEPS = 300
+
+
+def launch_min_bytes(precision, record_fraction, particles=True):
+    """The fewest HBM bytes a two-step lattice launch can move: every population and the
+    force read once (19 reals per node) and written once (19), the velocity of both steps
+    written (4), rho and T written on the launches whose second step writes the moments
+    (2 reals, on 2 * record_fraction of the launches), plus the two carried particle
+    updates' algorithmic bytes."""
+    s = 8 if precision == 0 else 4
+    b = NX * NY * s * (42 + 2 * 2 * record_fraction)
+    return b + (2 * particle_bytes_per_step(precision) if particles else 0)
+
+
+def run_leg(name, fn, *args, **kw):
+    """Run one benchmark leg; a failure is reported as {"error": ...} inside the JSON line
+    (with the traceback on stderr) instead of ending the run."""
+    import traceback
+
+    t0 = time.perf_counter()
+    try:
+        r = fn(*args, **kw)
+    except Exception as exc:  # noqa: BLE001 - every leg failure is reported, none is fatal
+        traceback.print_exc(file=sys.stderr)
+        print(f"bench: {name} failed after {time.perf_counter() - t0:.1f} s: {exc!r}", file=sys.stderr, flush=True)
+        return {"error": f"{type(exc).__name__}: {exc}"}
+    print(f"bench: {name} done in {time.perf_counter() - t0:.1f} s {r}", file=sys.stderr, flush=True)
+    return r
+
+
+def ok(r):
+    return isinstance(r, dict) and "error" not in r
+
+
+def lattice_summary(r, kernel_key, bytes_launch, extra=None):
+    """MLUPS, ms per step and the dominant launch's roofline fraction of one lattice leg."""
+    if not ok(r):
+        return r
+    out = {"mlups": round(r["mlups"], 1), "ms_per_step": round(r["ms_per_step"], 5), "steps": r["steps"]}
+    if kernel_key:
+        avg = r[kernel_key]
+        out["avg_launch_ms"] = round(avg, 5)
+        out["bytes_per_launch"] = round(bytes_launch)
+        out["roofline_frac"] = round(bytes_launch / (avg * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4) if avg > 0 else None
+    out.update(extra or {})
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=2000,
+                    help=f"timed steps of the headline (at least {MIN_TIMED_STEPS} are timed)")
+    ap.add_argument("--warmup", type=int, default=100, help=f"warm-up steps (at least {MIN_WARMUP})")
+    ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--cpu-threads", type=int, default=0, help="CPU baseline threads (default: host cores, at most 16)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 companion run")
@@ -621,60 +695,45 @@ def main() -> int:
     if rbn.device_count() == 0:
         print("bench.py: no GPU visible", file=sys.stderr)
         return 1
-    print(f"bench: fp32 {NX}x{NY} steps={args.steps} warmup={args.warmup}", file=sys.stderr, flush=True)
-    r32 = run_gpu(rbn, rbn.RBN_FP32, args.steps, args.warmup)
-    print(f"bench: fp32 done {r32}", file=sys.stderr, flush=True)
-    one = run_gpu(rbn, rbn.RBN_FP32, max(args.steps // 2, 1), max(args.warmup // 2, 1), lbm_steps=1)
-    print(f"bench: fp32 one step per launch done {one}", file=sys.stderr, flush=True)
-    sep = run_gpu(rbn, rbn.RBN_FP32, max(args.steps // 2, 1), max(args.warmup // 2, 1), particle_fuse=0)
-    print(f"bench: fp32 separate particle launches done {sep}", file=sys.stderr, flush=True)
-    host = run_gpu(rbn, rbn.RBN_FP32, args.steps, args.warmup, timing=False, records_to_host=True)
-    print(f"bench: fp32 records to host done {host}", file=sys.stderr, flush=True)
-    r64 = None
-    if not args.no_fp64:
-        r64 = run_gpu(rbn, rbn.RBN_FP64, max(args.steps // 2, 1), max(args.warmup // 2, 1))
-        print(f"bench: fp64 done {r64}", file=sys.stderr, flush=True)
+    steps = max(args.steps, MIN_TIMED_STEPS)
+    warmup = max(args.warmup, MIN_WARMUP)
+    half, half_w = max(steps // 2, MIN_TIMED_STEPS // 2), max(warmup // 2, MIN_WARMUP)
+    print(f"bench: fp32 {NX}x{NY} steps={steps} warmup={warmup}", file=sys.stderr, flush=True)
+    r32 = run_leg("fp32", run_gpu, rbn, rbn.RBN_FP32, steps, warmup)
+    r64 = None if args.no_fp64 else run_leg("fp64", run_gpu, rbn, rbn.RBN_FP64, half, half_w)
+    one = run_leg("fp32 one step per launch", run_gpu, rbn, rbn.RBN_FP32, half, half_w, lbm_steps=1)
+    sep = run_leg("fp32 separate particle launches", run_gpu, rbn, rbn.RBN_FP32, half, half_w, particle_fuse=0)
+    host = run_leg("fp32 records to host", run_gpu, rbn, rbn.RBN_FP32, steps, warmup, timing=False,
+                   records_to_host=True)
     ens = None
     if not args.no_ensemble:
         tun = tuple(int(v) for v in args.ensemble_tuning.split(",")) if args.ensemble_tuning else None
+        ens_steps = max(args.steps // 5, MIN_ENS_STEPS)
         ens = {}
         if args.ensemble_kind != "truth":
-            ens = run_ensemble(rbn, max(args.steps // 5, 8), 20, tun, nudged=True)
-            print(f"bench: ensemble (nudged members) done {ens}", file=sys.stderr, flush=True)
+            ens = run_leg("ensemble (nudged members)", run_ensemble, rbn, ens_steps, 20, tun, nudged=True)
         if args.ensemble_kind != "nudged":
-            ens["truth_run_members"] = run_ensemble(rbn, max(args.steps // 5, 8), 20, None, nudged=False)
-            print(f"bench: ensemble (truth-run members) done {ens['truth_run_members']}", file=sys.stderr, flush=True)
+            ens["truth_run_members"] = run_leg("ensemble (truth-run members)", run_ensemble, rbn, ens_steps, 20,
+                                               None, nudged=False)
     nud = None
     if not args.no_nudging:
-        nud = run_nudging(rbn, args.nud_steps, args.nud_spinup)
-        print(f"bench: nudging done {nud}", file=sys.stderr, flush=True)
+        nud = run_leg("nudging", run_nudging, rbn, args.nud_steps, args.nud_spinup)
         if args.nud_sweep_steps > 0:
-            nud["sweep"] = run_nudging_sweep(rbn, args.nud_sweep_steps, args.nud_spinup)
-            print(f"bench: nudging sweep done {nud['sweep']}", file=sys.stderr, flush=True)
-    adj = None
-    if not args.no_adjoint:
-        adj = run_adjoint(rbn, args.adj_window, args.adj_ckpt)
-        print(f"bench: adjoint done {adj}", file=sys.stderr, flush=True)
+            nud["sweep"] = run_leg("nudging sweep", run_nudging_sweep, rbn, args.nud_sweep_steps, args.nud_spinup)
+    adj = None if args.no_adjoint else run_leg("adjoint", run_adjoint, rbn, args.adj_window, args.adj_ckpt)
     cpu = None
     if not args.no_cpu:
-        cpu = run_cpu_baseline(rbn, args.cpu_steps, args.cpu_threads or cpu_threads())
-        print(f"bench: cpu done {cpu}", file=sys.stderr, flush=True)
+        cpu = run_leg("cpu", run_cpu_baseline, rbn, args.cpu_steps, args.cpu_threads or cpu_threads())
 
-    bytes_launch = launch_bytes(1, r32["record_fraction"], True, steps=2)
-    achieved = bytes_launch / (r32["lbm2_avg_ms"] * 1e-3) / 1e9
-    traffic = measured_traffic()
-    one_bytes = launch_bytes(1, one["record_fraction"], True)
-    one_achieved = one_bytes / (one["fused_avg_ms"] * 1e-3) / 1e9
-    lat_bytes = launch_bytes(1, sep["record_fraction"], False)
-    sep_achieved = lat_bytes / (sep["lbm_avg_ms"] * 1e-3) / 1e9
     line = {
         "metric": METRIC,
-        "value": round(r32["mlups"], 1),
+        "value": round(r32["mlups"], 1) if ok(r32) else None,
         "unit": "MLUPS",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(r32["ms_per_step"], 5),
+        "steps": r32["steps"] if ok(r32) else steps,
+        "warmup": warmup,
+        "steps_requested": args.steps,
+        "ms_per_step": round(r32["ms_per_step"], 5) if ok(r32) else None,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -686,58 +745,62 @@ def main() -> int:
             "nx": NX, "ny": NY, "Ra": RA, "Pr": PR, "particles": N_TRACER + N_INERTIAL,
             "record_every": RECORD_EVERY,
         },
-        "roofline": {
+        "mlups_fp64": round(r64["mlups"], 1) if ok(r64) else None,
+        "ms_per_step_fp64": round(r64["ms_per_step"], 5) if ok(r64) else None,
+    }
+    if ok(r32):
+        bytes_launch = launch_bytes(1, r32["record_fraction"], True, steps=2)
+        min_bytes = launch_min_bytes(1, r32["record_fraction"])
+        avg = r32["lbm2_avg_ms"]
+        achieved = bytes_launch / (avg * 1e-3) / 1e9
+        traffic = measured_traffic()
+        line["roofline"] = {
             "bound": "hbm",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 4),
             "traffic": traffic,
-            "traffic_frac": traffic and round(traffic / (r32["lbm2_avg_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
+            "traffic_frac": traffic and round(traffic / (avg * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
+            "min_traffic_bytes": round(min_bytes),
+            "min_traffic_frac": round(min_bytes / (avg * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
             "kernel": "k_lbm2<float, 8> (two fused f/g collide+stream steps, the first step's populations in "
                       "LDS, + the two previous steps' particle updates)",
             "steps_per_launch": 2,
             "bytes_per_launch": round(bytes_launch),
-            "avg_launch_ms": round(r32["lbm2_avg_ms"], 5),
+            "avg_launch_ms": round(avg, 5),
             "launches_timed": r32["lbm2_timed"],
             "timed_every": TIMING_EVERY,
-        },
-        "cpu_baseline": cpu and {k: (round(v, 3) if isinstance(v, float) else v) for k, v in cpu.items()},
-        "records_to_host": {
-            "mlups": round(host["mlups"], 1),
-            "ms_per_step": round(host["ms_per_step"], 5),
-            "note": "PCIe-inclusive: every record also copied to pinned host memory and handed to a callback",
-        },
-        "one_step_per_launch": {
-            "mlups": round(one["mlups"], 1),
-            "ms_per_step": round(one["ms_per_step"], 5),
-            "lattice_kernel": "k_lbm_fused32 (one step + the previous step's particle update)",
-            "avg_launch_ms": round(one["fused_avg_ms"], 5),
-            "bytes_per_launch": round(one_bytes),
-            "roofline_frac": round(one_achieved / HBM_PEAK_GBPS, 4),
-        },
-        "separate_particle_launches": {
-            "mlups": round(sep["mlups"], 1),
-            "ms_per_step": round(sep["ms_per_step"], 5),
-            "lattice_kernel": "k_lbm32",
-            "lattice_avg_ms": round(sep["lbm_avg_ms"], 5),
-            "lattice_bytes_per_launch": round(lat_bytes),
-            "lattice_roofline_frac": round(sep_achieved / HBM_PEAK_GBPS, 4),
-            "particles_avg_ms": round(sep["particles_avg_ms"], 5),
-        },
-        "fp64": r64 and {
-            "mlups": round(r64["mlups"], 1),
-            "ms_per_step": round(r64["ms_per_step"], 5),
-            "lbm2_avg_ms": round(r64["lbm2_avg_ms"], 5),
-            "roofline_frac": round(launch_bytes(0, r64["record_fraction"], True, steps=2) / (r64["lbm2_avg_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
-        },
-        "ensemble": ens,
-        "nudging": nud,
-        "adjoint": adj,
-        "finite": r32["finite"] and (r64 is None or r64["finite"]) and (adj is None or adj["finite"]),
-    }
-    print(json.dumps(finite_or_null(line), allow_nan=False))
-    return 0
+            "note": "achieved counts both steps' algorithmic bytes (160 B/LUP each), so frac can exceed 1 when "
+                    "the first step stays on chip; min_traffic_frac divides the fewest bytes such a launch can "
+                    "move (42 reals per node + particles) by the same duration and cannot exceed 1",
+        }
+    else:
+        line["roofline"] = None
+        line["error"] = r32.get("error")
+    line["cpu_baseline"] = cpu and ({k: (round(v, 3) if isinstance(v, float) else v) for k, v in cpu.items()}
+                                    if ok(cpu) else cpu)
+    line["fp64"] = None if r64 is None else lattice_summary(
+        r64, "lbm2_avg_ms", launch_bytes(0, r64.get("record_fraction", 0.0), True, steps=2),
+        {"min_traffic_frac": ok(r64) and r64["lbm2_avg_ms"] > 0 and round(
+            launch_min_bytes(0, r64["record_fraction"]) / (r64["lbm2_avg_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)})
+    line["records_to_host"] = lattice_summary(host, None, 0, {
+        "note": "PCIe-inclusive: every record also copied to pinned host memory and handed to a callback"})
+    line["one_step_per_launch"] = lattice_summary(
+        one, "fused_avg_ms", launch_bytes(1, one.get("record_fraction", 0.0), True),
+        {"lattice_kernel": "k_lbm_fused32 (one step + the previous step's particle update)"})
+    if ok(sep):
+        sep_bytes = launch_bytes(1, sep["record_fraction"], False)
+        line["separate_particle_launches"] = lattice_summary(
+            sep, "lbm_avg_ms", sep_bytes, {"lattice_kernel": "k_lbm32", "particles_avg_ms": round(sep["particles_avg_ms"], 5)})
+    else:
+        line["separate_particle_launches"] = sep
+    line["ensemble"] = ens
+    line["nudging"] = nud
+    line["adjoint"] = adj
+    line["finite"] = all(r["finite"] for r in (r32, r64, adj) if ok(r))
+    print(json.dumps(finite_or_null(line), allow_nan=False), flush=True)
+    return 0 if ok(r32) else 1
 
 
 if __name__ == "__main__":

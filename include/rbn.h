@@ -237,7 +237,11 @@ int rbn_get_nudging_weights(rbn_ctx *ctx, double *weight);
 /* Diagnostics callback (evolution.dat line), every ndiag steps. */
 int rbn_set_diag_callback(rbn_ctx *ctx, rbn_diag_fn fn, void *user);
 /* Advance nsteps time steps in the reference's order (one iteration of the main loops,
- * Trajectories/code/temppart_dump.c:1674-1759 or Nudging/code/tempvel_nudge.c:2346-2411). */
+ * Trajectories/code/temppart_dump.c:1674-1759 or Nudging/code/tempvel_nudge.c:2346-2411).
+ * Returns RBN_ENUMERIC when a step of the call met a node whose rho or T is not finite (the
+ * reference stops on a non-finite state, Trajectories/code/temppart_dump.c:414-425); the
+ * state is then that of the last step, and the call's records are not delivered.  Every call
+ * with nsteps > 0 waits for its steps to finish to read the check. */
 int rbn_step(rbn_ctx *ctx, int nsteps);
 /* Index of the last completed step (istep). */
 int rbn_get_step(const rbn_ctx *ctx, int *istep);
@@ -258,7 +262,7 @@ int rbn_synchronize(rbn_ctx *ctx);
  * reference runs one lattice per job).  Members must share device and precision, be
  * initialised, and stay owned by the caller; results equal stepping each member with
  * rbn_step, except that no trajectory records or diagnostics are produced during ensemble
- * steps. */
+ * steps.  A member meeting non-finite moments makes rbn_ensemble_step return RBN_ENUMERIC. */
 typedef struct rbn_ensemble rbn_ensemble;
 int rbn_ensemble_create(rbn_ctx *const *members, int n, rbn_ensemble **out);
 void rbn_ensemble_destroy(rbn_ensemble *e);
@@ -280,12 +284,14 @@ int rbn_ensemble_set_tuning(rbn_ensemble *e, int tile_w, int tile_h, int wg_per_
 int rbn_ensemble_get_tuning(const rbn_ensemble *e, int *tile_w, int *tile_h, int *wg_per_cu,
                             int *order);
 /* Time every candidate launch shape on trial_steps real steps, keep the fastest, and restore
- * every member's state afterwards; *best_ms_per_step receives the winner's time.  With
- * nudged members every candidate steps from the starting state (their records must cover
- * 2 * max(trial_steps, 6) and 2 L + 2 steps from it, L = lcm(6, record intervals) <= 120),
- * and the fastest shape is timed once more over steps 1 .. 2L of a call in both forms,
- * graph replays and eager steps with the deposits overlapped (order bit 4), keeping the
- * faster. */
+ * every member's state afterwards, also when a step fails (the error is returned and every
+ * member is back at its starting state); *best_ms_per_step receives the winner's time.  With
+ * nudged members every candidate steps from the starting state (a warm-up call of 6 steps
+ * and a timed call of 6 * max(1, ceil(trial_steps / 6)) steps), and the fastest shape is
+ * timed once more over steps 1 .. 2L of a call of 2 L + 2 steps in both forms, graph replays
+ * and eager steps with the deposits overlapped (order bit 4), keeping the faster
+ * (L = lcm(6, record intervals) <= 120): the members' records must cover the longer of the
+ * two spans from the starting state. */
 int rbn_ensemble_autotune(rbn_ensemble *e, int trial_steps, double *best_ms_per_step);
 
 /* ---------------------------------------------------------------- adjoint window */

@@ -88,14 +88,6 @@ def build_driver() -> Path:
     return DRIVER
 
 
-def build_probe() -> Path:
-    """copy_probe: a plain copy of known size (achievable bandwidth, counter calibration)."""
-    src = PKG / "measure" / "copy_probe.hip"
-    out = PKG / "copy_probe"
-    _run_parallel([[_hipcc(), *HIP_FLAGS, "-x", "hip", str(src), "-o", str(out)]])
-    return out
-
-
 def build_oracle() -> Path:
     """The CPU restatement used by the tests as the checker (never linked into librbn)."""
     subprocess.run(["make", "-s", "-C", str(ROOT / "oracle")], check=True)
@@ -105,7 +97,6 @@ def build_oracle() -> Path:
 def build_all(verbose: bool = False) -> None:
     build_library(verbose)
     build_driver()
-    build_probe()
     build_oracle()
 
 

@@ -286,6 +286,7 @@ int rbn_adj_forward(rbn_ctx *c, int nsteps, int ckpt_every, double *J) {
     lp.Tbar = d.Tbar;
     lp.nud = c->nud;
     lp.wq = c->dep.mode == RBN_DEPOSIT_BILINEAR ? d.wq : nullptr;
+    lp.bad = c->d_err + 1;
     if ((rc = timed(c, KT_LBM, s, [&] { return rbn::launch_lbm(s, c->prec, P.variant, g, lp, c->consts.data(), c->tune); })))
       return rc;
     std::swap(c->fA, c->fB);

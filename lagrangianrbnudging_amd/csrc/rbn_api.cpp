@@ -276,13 +276,16 @@ rbn::ObsPtrs obs_view(int nobs, int *ints, double *d) {
 rbn::ObsPtrs obs_ptrs(rbn_ctx *c) { return obs_view(c->nobs, c->d_obs_i, c->d_obs_d); }
 
 int check_err_flag(rbn_ctx *c) {
-  int h = 0;
-  HIPCHK(c, hipMemcpyAsync(&h, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  int e[2] = {0, 0}, h = 0;
+  HIPCHK(c, hipMemcpyAsync(e, c->d_err, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  if (h) {
-    HIPCHK(c, hipMemsetAsync(c->d_err, 0, sizeof(int), c->stream));
+  if (e[0] || e[1]) HIPCHK(c, hipMemsetAsync(c->d_err, 0, 2 * sizeof(int), c->stream));
+  // The reference stops on a non-finite state (Trajectories/code/temppart_dump.c:414-425);
+  // here the step call reports it.
+  if (e[1])
+    return fail(c, RBN_ENUMERIC, "non-finite moments (rho or T) in a lattice step by step %d", c->istep);
+  if (e[0])
     return fail(c, RBN_ERANGE, "a particle left the lattice (interpolation corner out of range) by step %d", c->istep);
-  }
   if (c->dep.overflow) {
     HIPCHK(c, hipMemcpyAsync(&h, c->dep.overflow, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -598,9 +601,10 @@ int rbn_create(const rbn_params *p, int device, rbn_ctx **out) {
     (void)hipMemsetAsync(*a.p, 0, a.b, c->stream);
   }
   void *tmp = nullptr;
-  if ((rc = dalloc(c, &tmp, sizeof(int)))) goto fail_out;
+  // d_err[0]: a particle left the lattice; d_err[1]: a step met non-finite moments
+  if ((rc = dalloc(c, &tmp, 2 * sizeof(int)))) goto fail_out;
   c->d_err = (int *)tmp;
-  (void)hipMemsetAsync(c->d_err, 0, sizeof(int), c->stream);
+  (void)hipMemsetAsync(c->d_err, 0, 2 * sizeof(int), c->stream);
   if ((rc = dalloc(c, &tmp, (2 * 1024 + 2) * sizeof(double)))) goto fail_out;
   c->d_energy = (double *)tmp;
   if (p->variant == RBN_VARIANT_NUDGING) {
@@ -1242,6 +1246,7 @@ int step_loop(rbn_ctx *c, int nsteps) {
       lp.wq = dset.mode == RBN_DEPOSIT_BILINEAR ? dset.wq : nullptr;
       lp.nud = c->nud;
     }
+    lp.bad = c->d_err + 1;
     const int kclass = pair ? KT_LBM2 : (carry ? KT_LBM_FUSED : KT_LBM);
     if ((rc = timed(c, kclass, c->stream, [&] {
            rbn::LbmTuning t = c->tune;
@@ -1334,11 +1339,9 @@ int rbn_step(rbn_ctx *c, int nsteps) {
   if (parts) HIPCHK(c, hipStreamSynchronize(c->pstream));
   c->part_record_pending = false;
   if (c->timing && (rc = drain_timing(c))) return rc;
-  if (parts || nudged) {
-    // A particle that left the lattice invalidates the records of this call: report it
-    // before delivering them.
-    if ((rc = check_err_flag(c))) return rc;
-  }
+  // Non-finite moments, or a particle that left the lattice, invalidate the records of this
+  // call: report them before delivering the records.
+  if (nsteps > 0 && (rc = check_err_flag(c))) return rc;
   return deliver_all(c);
 }
 

@@ -175,6 +175,7 @@ void fill_member(const rbn_ctx *c, const rbn::DepPtrs *spare, int tile_w, int ti
   d.part = c->d_part;
   d.np = (c->P.particle && c->d_part && c->P.variant != RBN_VARIANT_NUDGING) ? c->np : 0;
   d.err = c->d_err;
+  d.bad = c->d_err + 1;
   d.variant = c->P.variant;
   if (c->P.variant == RBN_VARIANT_NUDGING) {
     const bool bil = c->dep.mode == RBN_DEPOSIT_BILINEAR;
@@ -754,13 +755,21 @@ struct Snapshot {
   std::vector<void *> bufs;
   std::vector<std::pair<void *, size_t>> src;
   std::vector<int> istep, cur;
+  std::vector<bool> moments_valid;
   std::vector<void *> fA, gA, F;
 };
+
+void free_snapshot(Snapshot &s) {
+  for (void *b : s.bufs) (void)hipFree(b);
+  s.bufs.clear();
+  s.src.clear();
+}
 
 int take_snapshot(rbn_ensemble *e, Snapshot &s) {
   for (rbn_ctx *c : e->m) {
     const size_t fb = c->geom.plane * c->rs;
-    std::vector<std::pair<void *, size_t>> parts = {{c->fA, 9 * fb}, {c->gA, 9 * fb}, {c->F, fb}, {c->F2, fb}};
+    std::vector<std::pair<void *, size_t>> parts = {{c->fA, 9 * fb}, {c->gA, 9 * fb}, {c->F, fb},
+                                                    {c->F2, fb},     {c->rho, fb},    {c->T, fb}};
     for (int k = 0; k < rbn_ctx::kVel; k++) parts.push_back({c->uv[k], 2 * fb});
     if (c->d_part && c->np > 0) parts.push_back({c->d_part, 10 * (size_t)c->np * sizeof(double)});
     if (c->P.variant == RBN_VARIANT_NUDGING) {
@@ -777,13 +786,20 @@ int take_snapshot(rbn_ensemble *e, Snapshot &s) {
     }
     for (auto &p : parts) {
       void *b = nullptr;
-      EHIP(e, hipMalloc(&b, p.second));
-      EHIP(e, hipMemcpy(b, p.first, p.second, hipMemcpyDeviceToDevice));
-      s.bufs.push_back(b);
-      s.src.push_back(p);
+      hipError_t he = hipMalloc(&b, p.second);
+      if (he == hipSuccess) {
+        s.bufs.push_back(b);
+        s.src.push_back(p);
+        he = hipMemcpy(b, p.first, p.second, hipMemcpyDeviceToDevice);
+      }
+      if (he != hipSuccess) {
+        free_snapshot(s);
+        return efail(e, RBN_EDEVICE, "autotune snapshot failed: %s", hipGetErrorString(he));
+      }
     }
     s.istep.push_back(c->istep);
     s.cur.push_back(c->cur);
+    s.moments_valid.push_back(c->moments_valid);
     s.fA.push_back(c->fA);
     s.gA.push_back(c->gA);
     s.F.push_back(c->F);
@@ -791,9 +807,13 @@ int take_snapshot(rbn_ensemble *e, Snapshot &s) {
   return RBN_OK;
 }
 
-// Put the saved state back; keep: the snapshot stays for another restore.
+// Put the saved state back; keep: the snapshot stays for another restore.  Waits for the
+// ensemble's streams first (a failed step may leave launches in flight that would overwrite
+// the restored planes); every copy is attempted, and the first error is returned.
 int restore_snapshot(rbn_ensemble *e, Snapshot &s, bool keep = false) {
-  for (size_t i = 0; i < e->m.size(); i++) {
+  (void)hipStreamSynchronize(e->stream);
+  (void)hipStreamSynchronize(e->pstream);
+  for (size_t i = 0; i < s.istep.size() && i < e->m.size(); i++) {
     rbn_ctx *c = e->m[i];
     if (c->fA != s.fA[i]) {
       std::swap(c->fA, c->fB);
@@ -804,13 +824,17 @@ int restore_snapshot(rbn_ensemble *e, Snapshot &s, bool keep = false) {
     c->prev_rid.clear();
     c->istep = s.istep[i];
     c->cur = s.cur[i];
+    c->moments_valid = s.moments_valid[i];
   }
+  hipError_t first = hipSuccess;
   for (size_t k = 0; k < s.bufs.size(); k++) {
-    EHIP(e, hipMemcpy(s.src[k].first, s.bufs[k], s.src[k].second, hipMemcpyDeviceToDevice));
-    if (!keep) (void)hipFree(s.bufs[k]);
+    const hipError_t he = hipMemcpy(s.src[k].first, s.bufs[k], s.src[k].second, hipMemcpyDeviceToDevice);
+    if (he != hipSuccess && first == hipSuccess) first = he;
   }
-  if (!keep) s.bufs.clear();
+  if (!keep) free_snapshot(s);
   e->built = false;
+  drop_graph(e);
+  if (first != hipSuccess) return efail(e, RBN_EDEVICE, "autotune restore failed: %s", hipGetErrorString(first));
   return RBN_OK;
 }
 
@@ -907,10 +931,13 @@ int rbn_ensemble_step(rbn_ensemble *e, int nsteps) {
   if (hipSetDevice(e->dev) != hipSuccess) return efail(e, RBN_EDEVICE, "hipSetDevice failed");
   int rc = run_steps(e, nsteps);
   if (rc) return rc;
-  for (rbn_ctx *c : e->m) {
-    int h = 0;
-    EHIP(e, hipMemcpy(&h, c->d_err, sizeof(int), hipMemcpyDeviceToHost));
-    if (h) return efail(e, RBN_ERANGE, "a particle of an ensemble member left its lattice");
+  for (size_t q = 0; q < e->m.size(); q++) {
+    rbn_ctx *c = e->m[q];
+    int h = 0, he[2] = {0, 0};
+    EHIP(e, hipMemcpy(he, c->d_err, 2 * sizeof(int), hipMemcpyDeviceToHost));
+    if (he[0] || he[1]) EHIP(e, hipMemset(c->d_err, 0, 2 * sizeof(int)));
+    if (he[1]) return efail(e, RBN_ENUMERIC, "ensemble member %zu: non-finite moments (rho or T) by step %d", q, c->istep);
+    if (he[0]) return efail(e, RBN_ERANGE, "a particle of ensemble member %zu left its lattice", q);
     if (c->P.variant == RBN_VARIANT_NUDGING && c->dep.overflow) {
       EHIP(e, hipMemcpy(&h, c->dep.overflow, sizeof(int), hipMemcpyDeviceToHost));
       if (h) return efail(e, RBN_ERANGE, "deposit list overflow in a nudged ensemble member");
@@ -937,87 +964,98 @@ int rbn_ensemble_autotune(rbn_ensemble *e, int trial_steps, double *best_ms_per_
   if (e->nudged.empty())
     for (int w : wpcs)
       for (int o = 8; o < 12; o++) cands.push_back({256, 1, w, o});
+  const int tile_w0 = e->tile_w, tile_h0 = e->tile_h, wpc0 = e->wg_per_cu, order0 = e->order;
   int rc = sync_members(e);
   if (rc) return rc;
   Snapshot snap;
   if ((rc = take_snapshot(e, snap))) return rc;
-  hipEvent_t a, b;
-  EHIP(e, hipEventCreate(&a));
-  EHIP(e, hipEventCreate(&b));
+  hipEvent_t a = nullptr, b = nullptr;
   double best = 1e300;
   Cand bestc = cands[0];
-  for (const Cand &c : cands) {
-    if ((rc = rbn_ensemble_set_tuning(e, c.tw, c.th, c.wpc, c.order))) break;
+  // Time one candidate: a warm-up call, then the timed steps, then back to the snapshot.
+  auto trial_of = [&](const Cand &c, float *ms_per_step) -> int {
+    int r = rbn_ensemble_set_tuning(e, c.tw, c.th, c.wpc, c.order);
+    if (r) return r;
     const int gs = graph_steps(e);
     const int reps = std::max(1, (trial_steps + gs - 1) / gs), trial = reps * gs;
+    float ms = 0.f;
     if (!e->nudged.empty()) {
       // eager steps (nudged members): time a call of `trial` steps after a warm-up call,
       // then restore, so that every candidate replays the same steps (and records)
-      if ((rc = run_steps(e, gs))) break;
+      if ((r = run_steps(e, gs))) return r;
       EHIP(e, hipEventRecord(a, e->stream));
-      if ((rc = run_steps(e, trial))) break;
+      if ((r = run_steps(e, trial))) return r;
       EHIP(e, hipEventRecord(b, e->stream));
       EHIP(e, hipEventSynchronize(b));
-      if ((rc = restore_snapshot(e, snap, true))) break;
-      float ms = 0.f;
-      EHIP(e, hipEventElapsedTime(&ms, a, b));
-      if (ms / trial < best) {
-        best = ms / trial;
-        bestc = c;
-      }
-      continue;
+      if ((r = restore_snapshot(e, snap, true))) return r;
+    } else {
+      if ((r = run_steps(e, gs))) return r;  // build, warm (a whole cycle: descriptors stay valid)
+      if (!e->built && (r = build(e))) return r;
+      if (!e->exec && (r = capture(e))) return r;
+      // The captured cycle starts past the first launch of a call (counter set 1), which the
+      // eager launches above leave used: clear the counters before replaying.
+      EHIP(e, hipMemsetAsync(e->d_counters, 0, 4 * sizeof(int), e->stream));
+      EHIP(e, hipEventRecord(a, e->stream));
+      for (int s = 0; s < reps; s++) EHIP(e, hipGraphLaunch(e->exec, e->stream));
+      EHIP(e, hipEventRecord(b, e->stream));
+      EHIP(e, hipEventSynchronize(b));
+      EHIP(e, hipStreamSynchronize(e->pstream));
+      if (pairs(e))
+        advance_members(e, trial, trial / 2, trial / 2);
+      else
+        advance_members(e, trial, trial, 0);
     }
-    if ((rc = run_steps(e, gs))) break;  // build, warm (a whole cycle: descriptors stay valid)
-    if (!e->built && (rc = build(e))) break;
-    if (!e->exec && (rc = capture(e))) break;
-    // The captured cycle starts past the first launch of a call (counter set 1), which the
-    // eager launches above leave used: clear the counters before replaying.
-    EHIP(e, hipMemsetAsync(e->d_counters, 0, 4 * sizeof(int), e->stream));
-    EHIP(e, hipEventRecord(a, e->stream));
-    for (int s = 0; s < reps; s++) EHIP(e, hipGraphLaunch(e->exec, e->stream));
-    EHIP(e, hipEventRecord(b, e->stream));
-    EHIP(e, hipEventSynchronize(b));
-    EHIP(e, hipStreamSynchronize(e->pstream));
-    if (pairs(e))
-      advance_members(e, trial, trial / 2, trial / 2);
-    else
-      advance_members(e, trial, trial, 0);
-    float ms = 0.f;
     EHIP(e, hipEventElapsedTime(&ms, a, b));
-    if (ms / trial < best) {
-      best = ms / trial;
-      bestc = c;
+    *ms_per_step = ms / trial;
+    return RBN_OK;
+  };
+  // Nudged members: eager steps overlap each step's deposit with the previous lattice launch,
+  // captured cycles replay the steps in order.  Time both with the best launch shape over
+  // steps 1 .. 2L of a call of 2L + 2 steps (the graph form: two replays).
+  auto forms = [&]() -> int {
+    int r = rbn_ensemble_set_tuning(e, bestc.tw, bestc.th, bestc.wpc, bestc.order & ~16);
+    if (r || (!e->built && (r = build(e)))) return r;
+    if (e->ncycle <= 0) return RBN_OK;
+    const int T = 2 * e->ncycle + 2;
+    double ms_form[2] = {1e300, 1e300};
+    e->seg_a = a;
+    e->seg_b = b;
+    for (int form = 0; form < 2 && !r; form++) {
+      if ((r = rbn_ensemble_set_tuning(e, bestc.tw, bestc.th, bestc.wpc, (bestc.order & ~16) | (form ? 16 : 0))))
+        break;
+      e->seg_steps = 0;
+      if ((r = run_steps(e, T))) break;
+      float ms = 0.f;
+      if (e->seg_steps > 0 && hipEventElapsedTime(&ms, a, b) == hipSuccess) ms_form[form] = ms / e->seg_steps;
+      r = restore_snapshot(e, snap, true);
+    }
+    e->seg_a = e->seg_b = nullptr;
+    if (r) return r;
+    if (ms_form[1] < ms_form[0]) bestc.order |= 16;
+    if (std::min(ms_form[0], ms_form[1]) < 1e300) best = std::min(ms_form[0], ms_form[1]);
+    return RBN_OK;
+  };
+  if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess)
+    rc = efail(e, RBN_EDEVICE, "autotune: hipEventCreate failed");
+  for (size_t k = 0; k < cands.size() && !rc; k++) {
+    float ms = 0.f;
+    if ((rc = trial_of(cands[k], &ms))) break;
+    if (ms < best) {
+      best = ms;
+      bestc = cands[k];
     }
   }
-  if (!rc && !e->nudged.empty() && best < 1e300) {
-    // Nudged members: eager steps overlap each step's deposit with the previous lattice
-    // launch, captured cycles replay the steps in order.  Time both with the best launch
-    // shape over steps 1 .. 2L of a call of 2L + 2 steps (the graph form: two replays).
-    if ((rc = rbn_ensemble_set_tuning(e, bestc.tw, bestc.th, bestc.wpc, bestc.order & ~16)) == RBN_OK &&
-        (e->built || (rc = build(e)) == RBN_OK) && e->ncycle > 0) {
-      const int T = 2 * e->ncycle + 2;
-      double ms_form[2] = {1e300, 1e300};
-      e->seg_a = a;
-      e->seg_b = b;
-      for (int form = 0; form < 2 && !rc; form++) {
-        if ((rc = rbn_ensemble_set_tuning(e, bestc.tw, bestc.th, bestc.wpc, (bestc.order & ~16) | (form ? 16 : 0))))
-          break;
-        e->seg_steps = 0;
-        if ((rc = run_steps(e, T))) break;
-        float ms = 0.f;
-        if (e->seg_steps > 0 && hipEventElapsedTime(&ms, a, b) == hipSuccess) ms_form[form] = ms / e->seg_steps;
-        if ((rc = restore_snapshot(e, snap, true))) break;
-      }
-      e->seg_a = e->seg_b = nullptr;
-      if (!rc && ms_form[1] < ms_form[0]) bestc.order |= 16;
-      if (!rc && std::min(ms_form[0], ms_form[1]) < 1e300) best = std::min(ms_form[0], ms_form[1]);
-    }
+  if (!rc && !e->nudged.empty() && best < 1e300) rc = forms();
+  // Every exit path: the members go back to the snapshot (their state before the call).
+  const int rc2 = restore_snapshot(e, snap);
+  if (a) (void)hipEventDestroy(a);
+  if (b) (void)hipEventDestroy(b);
+  if (rc || rc2) {
+    const std::string msg = e->err;
+    (void)rbn_ensemble_set_tuning(e, tile_w0, tile_h0, wpc0, order0);
+    e->err = msg;
+    return rc ? rc : rc2;
   }
-  (void)hipEventDestroy(a);
-  (void)hipEventDestroy(b);
-  int rc2 = restore_snapshot(e, snap);
-  if (rc) return rc;
-  if (rc2) return rc2;
   if ((rc = rbn_ensemble_set_tuning(e, bestc.tw, bestc.th, bestc.wpc, bestc.order))) return rc;
   if (best_ms_per_step) *best_ms_per_step = best;
   return RBN_OK;

@@ -30,6 +30,7 @@ struct EnsMember {
   double *part;  // 10 arrays of np (x, y, u, v, xo, yo, uo, vo, 1/tau, beta)
   int np;
   int *err;
+  int *bad;  // non-finite moments flag (the member's d_err[1])
   // nudged members (variant RBN_VARIANT_NUDGING): the deposits the steps read, [k & 1] for
   // step k (1: the member's own arrays, 0: the ensemble's second set, so that the deposit of
   // step k + 1 can be built while step k runs), and the persistent nudging term; wq only for

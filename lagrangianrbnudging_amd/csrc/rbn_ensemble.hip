@@ -51,6 +51,7 @@ __device__ __forceinline__ void lbm_item(const EnsMember<R> &m, int tile, int k,
   p.uv = m.uv[out];
   p.rho = write_moments ? m.rho : nullptr;
   p.T = write_moments ? m.T : nullptr;
+  p.bad = m.bad;
   const int col = threadIdx.x % tile_w;
   if (m.variant == RBN_VARIANT_NUDGING) {
     p.nvol = m.nvol[par];
@@ -201,6 +202,7 @@ __device__ __forceinline__ void pair_item(const EnsMember<R> &m, int tile, const
   p.uv2 = m.uv[(m.cur0 + 2 + st.k) % kCycle];
   p.rho2 = st.moments ? m.rho : nullptr;
   p.T2 = st.moments ? m.T : nullptr;
+  p.bad = m.bad;
   lbm2::two_step_tile<R, TH>(m.g, p, m.c, tx, ty, L);
 }
 

@@ -54,6 +54,7 @@ struct LbmPtrs {
   const double *Tbar;// nudged run: tempreadint
   void *nud;         // nudged run: persistent nud_term
   const double *wq;  // nudged run, bilinear deposit: per-node weight W (Q = alpha*(W*(T-Tbar)))
+  int *bad;          // set to 1 when a node's rho or T is not finite (or null: no check)
 };
 
 // Particle kernel arguments (device pointers).
@@ -157,6 +158,14 @@ struct PartJob {
 // Strength of the bilinear deposit's nudging at a node whose observer weights sum to W:
 // min(W, 1), so that alpha <= 1 never over-relaxes a node that several observers reach.
 __host__ __device__ inline double nudge_weight(double W) { return W < 1.0 ? W : 1.0; }
+
+// Non-finite state: a node whose moments are not finite sets *bad (a NaN or infinity in any
+// population it pulls makes its rho or T NaN or infinite).  Only the store is conditional, so
+// the check costs two compares per node and no traffic.
+template <typename R>
+__device__ __forceinline__ void flag_nonfinite(int *bad, R rho, R T) {
+  if (bad && !(__builtin_isfinite(rho) && __builtin_isfinite(T))) *bad = 1;
+}
 
 struct AdjState;  // rbn_adjoint.cpp
 void adj_free(rbn_ctx *c);

@@ -123,6 +123,7 @@ hipError_t launch_lbm2(hipStream_t s, int prec, const Geom &g, const LbmPtrs &p,
     w.p.uv2 = uv2;
     w.p.rho2 = rho2;
     w.p.T2 = T2;
+    w.p.bad = p.bad;
     w.seg = min(seg, kMaxSeg);
     w.strips = (g.nx + kStripW - 1) / kStripW;
     w.nwg = w.strips * ((g.ny + w.seg - 1) / w.seg);
@@ -146,6 +147,7 @@ hipError_t launch_lbm2(hipStream_t s, int prec, const Geom &g, const LbmPtrs &p,
   a.p.uv2 = uv2;
   a.p.rho2 = rho2;
   a.p.T2 = T2;
+  a.p.bad = p.bad;
   a.tiles_x = (g.nx + kTW - 1) / kTW;
   a.ntiles = a.tiles_x * ((g.ny + th - 1) / th);
   a.flip = flip;

@@ -193,6 +193,7 @@ struct TwoStepPtrs {
   void *Fout;           // F(s+2)
   void *uv1, *uv2;      // velocity slots of steps s and s+1
   void *rho2, *T2;      // moments of step s+1, or null
+  int *bad;             // non-finite flag of the second step's moments, or null
 };
 
 // LDS of a two-step tile: the ring's post-collision f and g, and the tile's next force.
@@ -257,6 +258,7 @@ __device__ __forceinline__ void two_step_tile(const Geom &g, const TwoStepPtrs &
     R f[9], q[9], fo[9], go[9], rho, T, u, v, Fn;
     gather_truth<R>(acc, bot, top, c, f, q);
     collide(c, f, q, lF[qn], fo, go, rho, T, u, v, Fn);
+    flag_nonfinite(a.bad, rho, T);
     const size_t o = (size_t)y * g.pitch + x;
     R *fB = (R *)a.fB, *gB = (R *)a.gB;
 #pragma unroll

@@ -184,6 +184,7 @@ __device__ __forceinline__ void two_step_strip(const Geom &g, const TwoStepPtrs 
       R f[9], q[9], fo[9], go[9], rho, T, u, v, Fn;
       gather_truth<R>(acc, j == 0, j == g.ny - 1, c, f, q);
       collide(c, f, q, PF, fo, go, rho, T, u, v, Fn);
+      flag_nonfinite(a.bad, rho, T);
       const size_t o = (size_t)j * g.pitch + x;
 #pragma unroll
       for (int k = 0; k < 9; k++) st<1>(&fB[(size_t)k * g.plane + o], fo[k]);

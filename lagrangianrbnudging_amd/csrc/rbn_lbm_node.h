@@ -214,6 +214,7 @@ __device__ __forceinline__ void lbm_node64(const Geom &g, const LbmPtrs &p, cons
     nv = p.nvol[r0 + i];
     if (nv > 0) T = (T + 0.5 * ((const double *)p.nud)[r0 + i]);
   }
+  flag_nonfinite(p.bad, rho, T);
   const double rhoi = 1. / rho;
   double fl[9];
 #pragma unroll
@@ -369,6 +370,7 @@ __device__ __forceinline__ void lbm_node32(const Geom &g, const LbmPtrs &p, cons
   }
   const float rho = c.rho0 + dr;
   const float T = c.T0 + dT;
+  flag_nonfinite(p.bad, rho, T);
   const float rhoi = 1.f / rho;
   float *F = (float *)p.F;
   const float Fy = ld<NT>(&F[r0 + i]);

@@ -27,6 +27,7 @@ STATUS = {
     -5: "RBN_ESTATE", -6: "RBN_ERANGE", -7: "RBN_ENOMEM", -8: "RBN_ENUMERIC",
     -9: "RBN_ENOTFOUND", -10: "RBN_ECALLBACK",
 }
+STATUS_CODE = {name: code for code, name in STATUS.items()}
 
 
 class RbnError(RuntimeError):

@@ -45,3 +45,52 @@ def test_slab_cpu_baseline_places_every_particle(monkeypatch):
     r = bench.run_cpu_baseline(rbn, 2, 4)
     assert r["cores"] == 4 and r["kind"] == "port" and r["particles"] == 500
     assert r["value"] > 0 and "4 threads" in r["sample"]
+
+
+def _autotune_records(trial, ntempread):
+    """The records rbn_ensemble_autotune reads from the starting state, restated from its
+    loop: every candidate runs a warm-up call of 6 steps and a timed call of reps * 6 steps
+    (reps = max(1, ceil(trial / 6))) and is restored; then a call of 2 L + 2 steps per form.
+    A call reads at step s (s % ntempread == 1) the record of step s + ntempread - 1."""
+    import math
+
+    reps = max(1, (trial + 5) // 6)
+    L = 6 * ntempread // math.gcd(6, ntempread)
+    spans = [6 + reps * 6, 2 * L + 2]
+    return {s + ntempread - 1 for span in spans for s in range(1, span + 1) if s % ntempread == 1}
+
+
+@pytest.mark.parametrize("ntempread", [2, 3, 5, 7, 10, 12])
+@pytest.mark.parametrize("trial", [1, 6, 8, 13, 40])
+def test_ensemble_record_table_covers_the_autotuner(trial, ntempread):
+    # the driver's short run (--steps 20 --warmup 5) once asked for record 65 of a table
+    # that ended at step 58
+    need = max(_autotune_records(trial, ntempread))
+    assert bench.ens_autotune_steps(trial, ntempread) + ntempread - 1 >= need
+    horizon = bench.ens_record_horizon(trial, 20, 24, 8, ntempread)
+    assert horizon >= need
+    import numpy as np
+
+    _, _, _, _, recs = bench.ens_observations(np.random.default_rng(0), 64, 32, 4, horizon, ntempread)
+    assert all(r in recs for r in _autotune_records(trial, ntempread))
+    # the timed run itself: ens_warm + steps and the sequential warmup + steps
+    assert max(recs) >= 24 + 8 + ntempread - 1 and max(recs) >= 20 + 8 + ntempread - 1
+
+
+def test_a_failing_leg_is_reported_inside_the_line(capsys):
+    def boom():
+        raise RuntimeError("observation provider failed for record 65")
+
+    r = bench.run_leg("demo", boom)
+    assert r == {"error": "RuntimeError: observation provider failed for record 65"}
+    assert not bench.ok(r) and bench.ok({"mlups": 1.0})
+    assert "demo failed" in capsys.readouterr().err
+    assert bench.lattice_summary(r, "lbm2_avg_ms", 1.0) is r
+
+
+def test_launch_minimum_bytes_never_exceed_the_algorithmic_bytes():
+    # two one-step launches move 2 * 40 reals per node; a two-step launch at least 42
+    for prec, s in ((0, 8), (1, 4)):
+        nodes = bench.NX * bench.NY
+        assert bench.launch_min_bytes(prec, 0.0, False) == nodes * 42 * s
+        assert bench.launch_min_bytes(prec, 0.05, True) < bench.launch_bytes(prec, 0.05, True, steps=2)

@@ -157,7 +157,9 @@ def test_ensemble_autotune_with_nudged_members_keeps_state():
     ref_members = _mixed_members(rbn.RBN_FP32, table=False)
     e = rbn.Ensemble(ens_members)
     assert e.autotune(4) > 0
-    assert e.tuning()["order"] < 4
+    # one launch per step (bits 2, 3 clear); bit 4 (eager steps with the deposits overlapped)
+    # or graph replays, whichever measured faster
+    assert (e.tuning()["order"] & 15) < 4
     for s in ens_members:
         assert s.istep == 0
     e.step(17)
