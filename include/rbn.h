@@ -192,6 +192,15 @@ int rbn_profiles(rbn_ctx *ctx, double *out);
  * initial particle temperatures init_prtcl writes (:196-201).  field: 0 rho, 1 T, 2 u, 3 v.
  * Positions must lie in (-1, nx+1) x (-1, ny+1) (RBN_ERANGE otherwise). */
 int rbn_sample(rbn_ctx *ctx, int field, int n, const double *x, const double *y, double *out);
+/* rbn_sample without any device allocation: the positions and samples are staged in a fixed
+ * device array of RBN_SAMPLE_NOALLOC_MAX positions (6 MiB, held by every GPU the library runs
+ * on until the process ends) instead of a buffer allocated per call; n above the maximum is
+ * RBN_ERANGE.  No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops).  All contexts of one GPU share
+ * the array, so calls that overlap on one GPU race; sequential calls do not (each returns
+ * after its samples are back).  Results equal rbn_sample's bit for bit. */
+#define RBN_SAMPLE_NOALLOC_MAX (1 << 18)
+int rbn_sample_noalloc(rbn_ctx *ctx, int field, int n, const double *x, const double *y, double *out);
 
 /* ---------------------------------------------------------------- particles (truth run) */
 /* init_prtcl (Trajectories/code/mympi2.h:116-290) with caller-chosen positions.

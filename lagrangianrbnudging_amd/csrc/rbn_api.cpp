@@ -449,6 +449,32 @@ int load_observation(rbn_ctx *c, int rstep, hipStream_t s_up) {
 
 int load_observation(rbn_ctx *c, int rstep) { return load_observation(c, rstep, c->stream); }
 
+int check_sample_args(rbn_ctx *c, const char *who, int field, int n, const double *x, const double *y,
+                      const double *out) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (field < 0 || field > 3 || n < 0 || (n > 0 && (!x || !y || !out))) return fail(c, RBN_EINVAL, "%s: bad arguments", who);
+  if (!c->initialised) return fail(c, RBN_ESTATE, "not initialised");
+  if ((field == 0 || field == 1) && !c->moments_valid) return fail(c, RBN_ESTATE, "rho/T of the latest step not available");
+  return RBN_OK;
+}
+
+int sample_staged(rbn_ctx *c, int field, int n, const double *x, const double *y, double *out, double *buf) {
+  const void *src = field == 0   ? c->rho
+                    : field == 1 ? c->T
+                    : field == 2 ? c->uv[c->cur]
+                                 : (const void *)((const char *)c->uv[c->cur] + c->rs);
+  const int cs = field >= 2 ? 2 : 1;
+  double *dx = buf, *dy = dx + n, *dout = dy + n;
+  if (hipMemcpyAsync(dx, x, (size_t)n * sizeof(double), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+      hipMemcpyAsync(dy, y, (size_t)n * sizeof(double), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+      rbn::launch_sample(c->stream, c->prec, c->geom, src, cs, n, dx, dy, dout, c->d_err) != hipSuccess ||
+      hipMemcpyAsync(out, dout, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess)
+    return fail(c, RBN_EDEVICE, "bilinear sampling: device operation failed");
+  return check_err_flag(c);
+}
+
 int init_common_after_fields(rbn_ctx *c) {
   c->istep = c->P.nstart;
   c->at_init = false;
@@ -808,31 +834,13 @@ int rbn_profiles(rbn_ctx *c, double *out) {
 }
 
 int rbn_sample(rbn_ctx *c, int field, int n, const double *x, const double *y, double *out) {
-  int rc = check_ctx(c);
-  if (rc) return rc;
-  if (field < 0 || field > 3 || n < 0 || (n > 0 && (!x || !y || !out)))
-    return fail(c, RBN_EINVAL, "rbn_sample: bad arguments");
-  if (!c->initialised) return fail(c, RBN_ESTATE, "not initialised");
-  if ((field == 0 || field == 1) && !c->moments_valid) return fail(c, RBN_ESTATE, "rho/T of the latest step not available");
-  if (n == 0) return RBN_OK;
-  const void *src = field == 0   ? c->rho
-                    : field == 1 ? c->T
-                    : field == 2 ? c->uv[c->cur]
-                                 : (const void *)((const char *)c->uv[c->cur] + c->rs);
-  const int cs = field >= 2 ? 2 : 1;
+  int rc = check_sample_args(c, "rbn_sample", field, n, x, y, out);
+  if (rc || n == 0) return rc;
   void *buf = nullptr;
   if ((rc = dalloc(c, &buf, 3 * (size_t)n * sizeof(double)))) return rc;
-  double *dx = (double *)buf, *dy = dx + n, *dout = dy + n;
-  rc = RBN_OK;
-  if (hipMemcpyAsync(dx, x, (size_t)n * sizeof(double), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
-      hipMemcpyAsync(dy, y, (size_t)n * sizeof(double), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
-      rbn::launch_sample(c->stream, c->prec, c->geom, src, cs, n, dx, dy, dout, c->d_err) != hipSuccess ||
-      hipMemcpyAsync(out, dout, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
-      hipStreamSynchronize(c->stream) != hipSuccess)
-    rc = fail(c, RBN_EDEVICE, "rbn_sample: device operation failed");
+  rc = sample_staged(c, field, n, x, y, out, (double *)buf);
   (void)hipFree(buf);
-  if (rc) return rc;
-  return check_err_flag(c);
+  return rc;
 }
 
 int rbn_init_particles(rbn_ctx *c, int n, const int *id, const double *x, const double *y,

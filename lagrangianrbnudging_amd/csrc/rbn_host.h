@@ -92,5 +92,11 @@ int fetch_observation(rbn_ctx *c, int rstep, double *rx, double *ry, double *rT,
 // Report (and clear) a particle / observer range error or a deposit overflow.
 int check_err_flag(rbn_ctx *c);
 
+// rbn_sample's argument and state checks, and its work given a device buffer of 3n doubles
+// (positions in, samples out): copies, the sampling launch, the copy back, the range check.
+int check_sample_args(rbn_ctx *c, const char *who, int field, int n, const double *x, const double *y,
+                      const double *out);
+int sample_staged(rbn_ctx *c, int field, int n, const double *x, const double *y, double *out, double *buf);
+
 }  // namespace host
 }  // namespace rbn

@@ -109,6 +109,7 @@ _SIGS = {
     "rbn_energy": (C.c_int, [C.c_void_p, _DP, _DP]),
     "rbn_profiles": (C.c_int, [C.c_void_p, _DP]),
     "rbn_sample": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _DP, _DP, _DP]),
+    "rbn_sample_noalloc": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _DP, _DP, _DP]),
     "rbn_init_particles": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), _DP, _DP, _DP, _DP, _DP, _DP]),
     "rbn_get_particles": (C.c_int, [C.c_void_p, C.POINTER(Particles)]),
     "rbn_num_particles": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
