@@ -574,3 +574,177 @@ def test_nudging_recurrence_on_a_uniform_state():
         S = (1 - w) * S + w * T - (1 - 0.5 * w) * Q
         Qp = Q
     assert abs(T - T_obs) < abs(T_init - T_obs)
+
+
+# ------------------------------------------------------------------ nudged step, independently
+
+
+def _numpy_nudged_step(f, g, rho_p, T_p, nud, nvol, Tbar, W_dep, p):
+    """One nudged-run step written independently with array shifts (out-of-place pull):
+    the f step of the truth run; g with zero-gradient walls (the wall rows take the
+    pre-stream populations of their inner neighbour row, unstreamed, and the rows next to
+    them pull from those copies); hydrovar with half the persisted nudging term at nodes
+    that receive a deposit; the nudged collision, whose -Q persists where no deposit lands.
+    W_dep: the bilinear deposit's weights (None for the 7x7 running mean)."""
+    nx, ny = p.nx, p.ny
+    gpre = g.copy()
+    gpre[:, 0, :] = g[:, 1, :]
+    gpre[:, ny - 1, :] = g[:, ny - 2, :]
+    fs, gs = np.empty_like(f), np.empty_like(g)
+    for k in range(9):
+        fs[:, :, k] = np.roll(np.roll(f[:, :, k], CX[k], axis=0), CY[k], axis=1)
+        gs[:, :, k] = np.roll(np.roll(gpre[:, :, k], CX[k], axis=0), CY[k], axis=1)
+    gs[:, 0, :] = gpre[:, 0, :]
+    gs[:, ny - 1, :] = gpre[:, ny - 1, :]
+    for a, b in ((2, 4), (5, 7), (6, 8)):
+        fs[:, 0, a] = f[:, 0, b]
+        fs[:, ny - 1, b] = f[:, ny - 1, a]
+    Fy = -p.gravity * rho_p * T_p
+    rho = fs.sum(-1)
+    on = nvol > 0
+    T = gs.sum(-1) + np.where(on, 0.5 * nud, 0.0)
+    u = (fs * CX).sum(-1) / rho
+    v = (fs * CY).sum(-1) / rho + 0.5 * Fy / rho
+    om, omg = 1 / p.relax, 1 / p.relaxg
+    usq = u * u + v * v
+    cu = CX * u[..., None] + CY * v[..., None]
+    bracket = 1 + cu / CS2 + cu ** 2 / (2 * CS2 ** 2) - usq[..., None] / (2 * CS2)
+    feq = W * rho[..., None] * bracket
+    geq = W * T[..., None] * bracket
+    fac = ((CY - v[..., None]) * Fy[..., None]) / CS2 + cu * (CY * Fy[..., None]) / CS2 ** 2
+    f2 = fs * (1 - om) + om * feq + (1 - 0.5 * om) * W * fac
+    strength = 1.0 if W_dep is None else np.minimum(W_dep, 1.0)
+    Q = np.where(on, p.alpha * strength * (T - Tbar), 0.0)
+    g2 = gs * (1 - omg) + omg * geq - (1 - 0.5 * omg) * W * Q[..., None]
+    nud2 = np.where(on, -Q, nud)
+    return f2, g2, rho, T, u, v, nud2
+
+
+@pytest.mark.parametrize("deposit", [rbn.RBN_DEPOSIT_BOX7, rbn.RBN_DEPOSIT_BILINEAR])
+def test_oracle_nudged_step_matches_an_independent_array_restatement(deposit):
+    # From a random non-equilibrium state with a temperature that differs row by row (so
+    # that a wall row copying the wrong row, or being streamed, shows), observers on and
+    # next to both wall rows and across x = nx, and the deposit as the oracle builds it
+    # (pinned on its own by the deposit tests); the persisted nudging term starts at zero
+    # and must carry over at nodes an observer leaves.
+    p = params(20, 12, variant=1, ntempread=3, alpha=0.4, Tup=0.1, Tdown=0.9, gravity=-3e-4, relax=0.61,
+               relaxg=0.57, deposit=deposit)
+    nx, ny = p.nx, p.ny
+    rng = np.random.default_rng(14)
+    f = W * (1 + 0.1 * rng.uniform(-1, 1, (nx, ny, 9)))
+    rows = np.linspace(0.2, 0.8, ny)[None, :, None]
+    g = W * (rows + 0.2 * rng.uniform(-1, 1, (nx, ny, 9)))
+    rho_p = 1 + 0.01 * rng.uniform(-1, 1, (nx, ny))
+    T_p = 0.5 + 0.1 * rng.uniform(-1, 1, (nx, ny))
+    o = Oracle(p)
+    o.set_populations(f, g, rho_p, T_p)
+    ids = np.arange(5, dtype=np.int32)
+    x0 = np.array([3.2, 10.6, 19.7, 7.5, 15.1])
+    y0 = np.array([1.2, ny - 0.3, 6.4, 2.1, ny - 1.4])
+    vx = np.array([0.9, -0.7, 0.5, 1.3, 0.0])
+    vy = np.array([0.2, -0.4, 0.3, 0.0, 0.25])
+    T_obs = np.array([0.05, 0.95, 0.4, 0.7, 0.6])
+    o.init_observers(ids, x0, y0, T_obs)
+    recs = {}
+    for s in range(3, 40, 3):
+        recs[s] = {"id": ids, "x": np.mod(x0 + vx * s, nx), "y": 1 + np.mod(y0 + vy * s - 1, ny - 1),
+                   "temp": T_obs + 0.01 * s}
+    o.set_observation_source(recs)
+    nud = np.zeros((nx, ny))
+    walls_nudged = [False, False]
+    for n in range(7):
+        o.step(1)
+        nd = o.nudging()
+        W_dep = nd["weight"] if deposit == rbn.RBN_DEPOSIT_BILINEAR else None
+        f, g, rho, T, u, v, nud = _numpy_nudged_step(f, g, rho_p, T_p, nud, nd["nudge_vol"], nd["tempreadint"],
+                                                     W_dep, p)
+        rho_p, T_p = rho, T
+        fo, go = o.populations()
+        fl = o.fields()
+        assert np.allclose(fo, f, rtol=0, atol=2e-15), n
+        assert np.allclose(go, g, rtol=0, atol=2e-15), n
+        for k, a in (("rho", rho), ("T", T), ("u", u), ("v", v)):
+            assert np.allclose(fl[k], a, rtol=0, atol=2e-15), (n, k)
+        assert np.allclose(nd["nud_term"], nud, rtol=0, atol=2e-15), n
+        walls_nudged[0] |= bool((nd["nudge_vol"][:, 0] > 0).any())
+        walls_nudged[1] |= bool((nd["nudge_vol"][:, ny - 1] > 0).any())
+    assert all(walls_nudged)
+    # nodes that were nudged before and are not now keep their term
+    assert np.any((nd["nudge_vol"] == 0) & (nud != 0))
+
+
+# ------------------------------------------------------------------ observers crossing the walls
+
+
+def _crossing_oracle(y0, y_rec, T0=0.2, T_rec=0.9, ny=20, ntr=5):
+    p = params(40, ny, variant=1, ntempread=ntr, alpha=0.0)
+    o = Oracle(p)
+    o.init_fields(T=np.full((p.nx, ny), 0.5))
+    ids = np.array([7], np.int32)
+    o.init_observers(ids, np.array([12.0]), np.array([y0]), np.array([T0]))
+    o.set_observation_source({ntr: {"id": ids, "x": np.array([12.0]), "y": np.array([y_rec]),
+                                    "temp": np.array([T_rec])},
+                              2 * ntr: {"id": ids, "x": np.array([12.0]), "y": np.array([y_rec]),
+                                        "temp": np.array([T_rec])}})
+    return p, o
+
+
+@pytest.mark.parametrize("y0,y_rec,unwrapped,cross_step", [
+    # up through y = ny: 19.5 -> 1.5 is +2 with period ny - 1 = 19 (record unwrapped to 20.5)
+    (19.5, 1.5, 20.5, 3),
+    # down through y = 1: 1.4 -> 19.6 is -0.8 (record unwrapped to 0.6)
+    (1.4, 19.6, 0.6, 3),
+])
+def test_observer_crossing_a_wall_unwraps_with_period_ny_minus_1_and_freezes_T(y0, y_rec, unwrapped, cross_step):
+    # read_prtcl (Nudging/code/tempvel_nudge.c:1514-1550, 1700-1718): a record more than
+    # ny/2 away in y is unwrapped by ny - 1; until the interpolated position leaves [1, ny]
+    # the temperature stays frozen at the previous value, on the crossing step it is wrapped
+    # by ny - 1 and takes the record temperature, which then stays until the snap.
+    ny, ntr, T0, T_rec = 20, 5, 0.2, 0.9
+    p, o = _crossing_oracle(y0, y_rec, T0, T_rec, ny, ntr)
+    if y_rec < y0:
+        y_unw = (y_rec + float(ny)) - 1.0
+    else:
+        y_unw = (y_rec - float(ny)) + 1.0
+    assert abs(y_unw - unwrapped) < 1e-12
+    dy = (y_unw - y0) / float(ntr)
+    y = y0
+    for s in range(1, ntr):
+        o.step(1)
+        st = o.observer_state()
+        y = y + dy
+        crossed = False
+        if y > ny:
+            y = (y - float(ny)) + 1.0
+            crossed = True
+        if y < 1.0:
+            y = (y + float(ny)) - 1.0
+            crossed = True
+        assert crossed == (s == cross_step)
+        assert st["yint"][0] == y, s  # the same IEEE operations in the same order
+        assert st["yidx"][0] == int(np.round(y))
+        assert st["Tint"][0] == (T_rec if s >= cross_step else T0), s
+        assert 1.0 <= st["yint"][0] <= ny
+    o.step(1)  # istep % ntempread == 0: the record exactly
+    st = o.observer_state()
+    assert st["yint"][0] == y_rec and st["Tint"][0] == T_rec
+
+
+def test_observer_y_unwrap_uses_the_truncated_distance():
+    # The distance test is abs() of a double, i.e. truncated to an int (Q5): 10.7 > ny/2 = 10
+    # is read as 10 > 10, so this observer is not unwrapped and moves straight up through
+    # the middle, its temperature interpolated linearly.
+    ny, ntr = 20, 5
+    p, o = _crossing_oracle(1.2, 11.9, 0.2, 0.7, ny, ntr)
+    for s in range(1, ntr):
+        o.step(1)
+        st = o.observer_state()
+        assert abs(st["yint"][0] - (1.2 + s * (11.9 - 1.2) / ntr)) < 1e-13
+        assert abs(st["Tint"][0] - (0.2 + s * (0.7 - 0.2) / ntr)) < 1e-14
+    # 11.0 away is unwrapped: down through y = 1, which the first step already crosses
+    p, o = _crossing_oracle(1.2, 12.2, 0.2, 0.7, ny, ntr)
+    o.step(1)
+    st = o.observer_state()
+    dy = (((12.2 - 20.0) + 1.0) - 1.2) / ntr
+    assert st["yint"][0] == ((1.2 + dy) + 20.0) - 1.0
+    assert st["Tint"][0] == 0.7
