@@ -638,6 +638,13 @@ static const or_obsrec *find_obs(const or_sim *s, int step) {
   return NULL;
 }
 
+/* Order of (id, record index) pairs: by id, then by index. */
+static int cmp_id_then_index(const void *a, const void *b) {
+  const int *x = (const int *)a, *y = (const int *)b;
+  if (x[0] != y[0]) return x[0] < y[0] ? -1 : 1;
+  return x[1] < y[1] ? -1 : (x[1] > y[1]);
+}
+
 /* abs() of the reference applied to a double: the argument converts to int (N:1496,1515). */
 static inline int iabs_trunc(double d) { return abs((int)d); }
 
@@ -684,13 +691,28 @@ static int or_read_obs(or_sim *s) {
     const or_obsrec *rec = find_obs(s, s->istep + ntr - 1);
     if (!rec) return fail(s, RBN_ENOTFOUND, "observation record %d missing", s->istep + ntr - 1);
     if (rec->n != n) return fail(s, RBN_EINVAL, "record %d has %d entries, expected %d", rec->step, rec->n, n);
-    for (int i = 0; i < n; i++) {
-      int found = -1;
-      for (int j = 0; j < n; j++)
-        if (rec->id[j] == s->oid[i]) { found = j; break; }
-      if (found < 0) return fail(s, RBN_ENOTFOUND, "id %d missing from record %d", s->oid[i], rec->step);
-      s->chk[i] = found;
+    /* chk[i] = the first record entry j whose id is observer i's (the reference's search,
+     * N:1480-1487); found by binary search over the entries sorted by (id, j), the same j. */
+    int *order = (int *)malloc((size_t)(n > 0 ? n : 1) * 2 * sizeof(int));
+    if (!order) return fail(s, RBN_ENOMEM, "observer map allocation failed");
+    for (int j = 0; j < n; j++) {
+      order[2 * j] = rec->id[j];
+      order[2 * j + 1] = j;
     }
+    qsort(order, (size_t)n, 2 * sizeof(int), cmp_id_then_index);
+    for (int i = 0; i < n; i++) {
+      int lo = 0, hi = n;
+      while (lo < hi) {
+        const int mid = lo + (hi - lo) / 2;
+        if (order[2 * mid] < s->oid[i]) lo = mid + 1; else hi = mid;
+      }
+      if (lo == n || order[2 * lo] != s->oid[i]) {
+        free(order);
+        return fail(s, RBN_ENOTFOUND, "id %d missing from record %d", s->oid[i], rec->step);
+      }
+      s->chk[i] = order[2 * lo + 1];
+    }
+    free(order);
     for (int i = 0; i < n; i++) {
       s->bcy[i] = 0;
       double xread = rec->x[s->chk[i]], yread = rec->y[s->chk[i]];

@@ -748,3 +748,22 @@ def test_observer_y_unwrap_uses_the_truncated_distance():
     dy = (((12.2 - 20.0) + 1.0) - 1.2) / ntr
     assert st["yint"][0] == ((1.2 + dy) + 20.0) - 1.0
     assert st["Tint"][0] == 0.7
+
+
+def test_observer_map_takes_the_first_entry_of_a_repeated_id():
+    # The reference's search stops at the first record entry with the observer's id
+    # (Nudging/code/tempvel_nudge.c:1480-1487): two observers with id 4 and a record listing
+    # id 4 at entries 1 and 2 both follow entry 1; the record's order is otherwise free.
+    nx, ny = 40, 20
+    p = params(nx, ny, variant=1, ntempread=4, alpha=0.0)
+    o = Oracle(p)
+    o.init_fields(T=np.full((nx, ny), 0.5))
+    ids = np.array([4, 4, 2], np.int32)
+    o.init_observers(ids, np.array([5.0, 6.0, 7.0]), np.array([5.0, 6.0, 7.0]), np.array([0.1, 0.2, 0.3]))
+    rec = {"id": np.array([2, 4, 4], np.int32), "x": np.array([17.0, 18.0, 30.0]),
+           "y": np.array([7.5, 6.5, 3.5]), "temp": np.array([0.7, 0.8, 0.9])}
+    o.set_observation_source({4: rec, 8: rec})
+    o.step(4)  # the snap at istep % ntempread == 0
+    st = o.observer_state()
+    assert np.array_equal(st["xint"], np.array([18.0, 18.0, 17.0]))
+    assert np.array_equal(st["Tint"], np.array([0.8, 0.8, 0.7]))
