@@ -105,9 +105,19 @@ void fill_consts(const rbn_params &P, rbn::StepConst<R> &c) {
   // expressed for the shifted g.
   const double om = 1. / P.relax, omg = 1. / P.relaxg;
   c.om = (R)om;
-  c.one_m_om = (R)(1. - om);
   c.omg = (R)omg;
-  c.one_m_omg = (R)(1. - omg);
+  if (sizeof(R) == 8) {
+    c.one_m_om = (R)(1. - om);
+    c.one_m_omg = (R)(1. - omg);
+  } else {
+    // FP32 relaxes the deviations from the reference level: the level's share,
+    // w_k*level*((1-om) + om - 1), is dropped, which is exact only if the two rounded factors
+    // sum to 1.  Rounding 1-om and om separately leaves a relative defect of up to an ulp,
+    // which multiplies the deviation sum (T - T0) every step and drifts T linearly in time
+    // (about 4e-8 per step at T - T0 = 0.5); 1 - (float)om is exact for om in [0.5, 2].
+    c.one_m_om = (R)1 - c.om;
+    c.one_m_omg = (R)1 - c.omg;
+  }
   const double B = 1.0 - 0.5 * omg;
   for (int k = 0; k < 9; k++) {
     c.src[k] = (R)((1.0 - 0.5 * om) * rbn::kW[k]);

@@ -61,11 +61,13 @@ __global__ void k_init32(Geom g, const float *rho, const float *T, const float *
   float A[9];
   eq_dev32(uv[2 * o], uv[2 * o + 1], A);
   const float r = rho[o], t = T[o];
+  float feq[9], geq[9];
+  eq_pops32(r - rho0, r, A, feq);
+  eq_pops32(t - T0, t, A, geq);
 #pragma unroll
   for (int k = 0; k < 9; k++) {
-    const float w = (float)kW[k];
-    f[(size_t)k * g.plane + o] = w * ((r - rho0) + r * A[k]);
-    gp[(size_t)k * g.plane + o] = w * ((t - T0) + t * A[k]);
+    f[(size_t)k * g.plane + o] = feq[k];
+    gp[(size_t)k * g.plane + o] = geq[k];
   }
   F[o] = g0 * r * t;
 }

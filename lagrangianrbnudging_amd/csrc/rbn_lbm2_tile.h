@@ -159,19 +159,14 @@ __device__ __forceinline__ void collide32(const StepConst<float> &c, const float
   const float tz = -3.f * v * Fy, tp = 3.f * (1.f - v) * Fy, tm = 3.f * (-1.f - v) * Fy;
   const float d2 = 9.f * v * Fy, d5 = 9.f * (u + v) * Fy, d6 = 9.f * (v - u) * Fy;
   const float fac[9] = {tz, tz, tp + d2, tz, tm + d2, tp + d5, tp + d6, tm + d5, tm + d6};
+  float feq[9], geq[9];
+  eq_pops32(dr, rho, A, feq);
 #pragma unroll
-  for (int k = 0; k < 9; k++) {
-    const float w = (float)kW[k];
-    const float feq = w * (dr + rho * A[k]);
-    fo[k] = f[k] * c.one_m_om + c.om * feq + c.src[k] * fac[k];
-  }
+  for (int k = 0; k < 9; k++) fo[k] = f[k] * c.one_m_om + c.om * feq[k] + c.src[k] * fac[k];
   const float Q = 0.f;
+  eq_pops32(dT, T, A, geq);
 #pragma unroll
-  for (int k = 0; k < 9; k++) {
-    const float w = (float)kW[k];
-    const float geq = w * (dT + T * A[k]);
-    go[k] = q[k] * c.one_m_omg + c.omg * geq - c.bw[k] * Q;
-  }
+  for (int k = 0; k < 9; k++) go[k] = q[k] * c.one_m_omg + c.omg * geq[k] - c.bw[k] * Q;
 }
 
 __device__ __forceinline__ void collide(const StepConst<double> &c, const double f[9], const double q[9],

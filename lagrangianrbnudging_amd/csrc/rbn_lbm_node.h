@@ -279,6 +279,23 @@ __device__ __forceinline__ void eq_dev32(float u, float v, float A[9]) {
   A[8] = sumsq2 + ui - vi - uv;
 }
 
+// FP32 equilibria of the deviations from the reference levels, for f (mass deviation dr,
+// density rho) and g (temperature deviation dT, temperature T): eq_k = w_k (d + level A_k)
+// for the moving populations and the rest population as the remainder, eq_0 = d - sum of the
+// others.  The float weights sum to 1 + 7.5e-9, so eq_0 = w_0 (d + level A_0) would scale the
+// deviation sums by that factor through every collision and drift T linearly in time (about
+// 7e-9 per step at T - T0 = 0.5); with the remainder the equilibria carry d exactly, up to the
+// rounding of the sums.
+__device__ __forceinline__ void eq_pops32(float d, float level, const float A[9], float eq[9]) {
+  float s = 0.f;
+#pragma unroll
+  for (int k = 1; k < 9; k++) {
+    eq[k] = (float)kW[k] * (d + level * A[k]);
+    s += eq[k];
+  }
+  eq[0] = d - s;
+}
+
 // The fused step at interior node (i, j) (0-based), FP32 with shifted populations.
 template <int VARIANT, int NT>
 __device__ __forceinline__ void lbm_node32(const Geom &g, const LbmPtrs &p, const StepConst<float> &c,
@@ -390,24 +407,21 @@ __device__ __forceinline__ void lbm_node32(const Geom &g, const LbmPtrs &p, cons
   const float d2 = 9.f * v * Fy, d5 = 9.f * (u + v) * Fy, d6 = 9.f * (v - u) * Fy;
   const float fac[9] = {tz, tz, tp + d2, tz, tm + d2, tp + d5, tp + d6, tm + d5, tm + d6};
   float *fB = (float *)p.fB, *gB = (float *)p.gB;
+  float feq[9], geq[9];
+  eq_pops32(dr, rho, A, feq);
 #pragma unroll
-  for (int k = 0; k < 9; k++) {
-    const float w = (float)kW[k];
-    const float feq = w * (dr + rho * A[k]);
-    st<NT>(&fB[(size_t)k * g.plane + r0 + i], f[k] * c.one_m_om + c.om * feq + c.src[k] * fac[k]);
-  }
+  for (int k = 0; k < 9; k++)
+    st<NT>(&fB[(size_t)k * g.plane + r0 + i], f[k] * c.one_m_om + c.om * feq[k] + c.src[k] * fac[k]);
   float Q = 0.f;
   if (VARIANT == RBN_VARIANT_NUDGING && nv > 0) {
     Q = p.wq ? c.alpha * ((float)nudge_weight(p.wq[r0 + i]) * (T - (float)p.Tbar[r0 + i]))
              : c.alpha * (T - (float)p.Tbar[r0 + i]);
     ((float *)p.nud)[r0 + i] = -Q;
   }
+  eq_pops32(dT, T, A, geq);
 #pragma unroll
-  for (int k = 0; k < 9; k++) {
-    const float w = (float)kW[k];
-    const float geq = w * (dT + T * A[k]);
-    st<NT>(&gB[(size_t)k * g.plane + r0 + i], q[k] * c.one_m_omg + c.omg * geq - c.bw[k] * Q);
-  }
+  for (int k = 0; k < 9; k++)
+    st<NT>(&gB[(size_t)k * g.plane + r0 + i], q[k] * c.one_m_omg + c.omg * geq[k] - c.bw[k] * Q);
 }
 
 // First node of this thread in lattice block L of N.  The lattice blocks enumerate row
