@@ -157,3 +157,56 @@ def test_driver_rejects_bad_input(tmp_path):
     (tmp_path / "bad.h").write_text("#define nx 4\n#define ny 32\n#define scratch 1\n")
     out = subprocess.run([str(DRIVER), "bad.h"], cwd=tmp_path, capture_output=True, text=True, timeout=60)
     assert out.returncode != 0 and "lattice" in out.stderr
+
+
+@pytest.mark.parametrize("kind,writefull", [("tracer", 0), ("heavy", 1)])
+def test_driver_hdf5_records_hold_the_oracle_records(tmp_path, kind, writefull):
+    # Prtcl_Veldata1/Particles<step>.h5 read back independently (tests/h5read.py): exactly the
+    # datasets prtcl_write names (Trajectories/code/temppart_dump.c:1467-1571; the Writefull
+    # block, mympi2.h:441-556), /id as 4-byte signed little-endian integers, the others as
+    # 8-byte little-endian floats, each of length Nprtcl, in ascending id order, with the FP64
+    # oracle's record values bit for bit at every record step.
+    import h5read
+    from oracle.oracle import Oracle
+
+    if h5read.library() is None:
+        pytest.skip("HDF5 runtime library not available on this host")
+    n, steps = 50, 30
+    text = _params_text() + (f"#define Nprtcl {n}\n#define nsteps {steps}\n#define Prtcl_Writefull {writefull}\n"
+                             + ("#define tracer 1\n#define heavyprtcl 0\n" if kind == "tracer"
+                                else "#define tracer 0\n#define heavyprtcl 1\n#define taulag 40.0\n"))
+    (tmp_path / "truth.h").write_text(text)
+    rng = np.random.default_rng(17)
+    xy = np.concatenate([rng.uniform(0.5, 63.5, n), rng.uniform(1.0, 32.0, n)])
+    xy.tofile(tmp_path / "parts.bin")
+    _run(["--out", "truth", "--init", "conduction:0.01:5", "--particles", "parts.bin", "truth.h"], tmp_path)
+    t = tmp_path / "truth"
+    p = rbn.parse_params(str(tmp_path / "truth.h"))
+    s = rbn.Solver(p)
+    s.init_conduction(0.01, 5)
+    T0 = s.fields()["T"]
+    s.close()
+    o = Oracle(p)
+    o.init_fields(T=T0)
+    ids = np.fromfile(t / "init_prtcl_id0.in", dtype=np.int32)
+    assert np.array_equal(ids, np.arange(n)) and np.array_equal(np.fromfile(t / "init_prtclx0.in"), xy[:n])
+    o.init_particles(ids, np.fromfile(t / "init_prtclx0.in"), np.fromfile(t / "init_prtcly0.in"))
+    o.step(steps)
+    names = ["id", "x", "y", "u", "v", "xo", "yo", "uo", "vo", "temp"]
+    if writefull:
+        names += ["dtu", "dtv", "dxu", "dxv", "dyu", "dyv", "fu", "fv"]
+    oracle_recs = {r["step"]: r for r in o.records}
+    files = sorted((t / "Prtcl_Veldata1").glob("Particles*.h5"), key=lambda f: int(f.stem[9:]))
+    assert [int(f.stem[9:]) for f in files] == sorted(oracle_recs) == list(range(5, steps + 1, 5))
+    for f in files:
+        got = h5read.read_root(f)
+        assert sorted(got) == sorted(names), f.name
+        ref = oracle_recs[int(f.stem[9:])]
+        for name in names:
+            arr, (cls, size, le, signed) = got[name]
+            if name == "id":
+                assert (cls, size, le, signed) == (h5read.H5T_INTEGER, 4, True, True)
+            else:
+                assert (cls, size, le) == (h5read.H5T_FLOAT, 8, True), name
+            assert arr.shape == (n,), name
+            assert np.array_equal(arr, ref[name]), f"{f.name} {name}"
