@@ -17,9 +17,21 @@ struct Lbm2Args {
   int tiles_x, ntiles, flip;
 };
 
+// FP32 tiles of 64 x 8 run at most three workgroups per CU: with four (40.4 KB of LDS each)
+// the launch measured 3.6 % slower at 4096 x 2048 (0.361 against 0.349 ms with the
+// benchmark's particles), so the pad takes a workgroup's LDS above a quarter of the CU's
+// 160 KB.  FP64 tiles of 64 x 4 (43.6 KB) fit three per CU without it and gain from the
+// third (0.556 against 0.594 ms with two).
+template <typename R, int TH>
+struct KernelLds {
+  TileLds<R, TH> L;
+  char cap[(sizeof(R) == 4 && TH == 8) ? 8192 : 1];
+};
+
 template <typename R, int TH>
 __global__ __launch_bounds__(kTileThreads) void k_lbm2(Geom g, Lbm2Args a, StepConst<R> c, part::PartSteps<R> ps) {
-  __shared__ TileLds<R, TH> L;
+  __shared__ KernelLds<R, TH> S;
+  TileLds<R, TH> &L = S.L;
   if ((int)blockIdx.x < ps.nblocks) {
     part::run_part_steps(ps, blockIdx.x * kTileThreads + threadIdx.x);
     return;

@@ -3,9 +3,11 @@
 //
 // A workgroup of kTileThreads threads owns a tile of kTW x TH nodes.  Phase 1 computes step s
 // on the tile and a one-node ring around it (the ring is computed again by the neighbouring
-// tiles), pulling from the step's input planes in HBM, and keeps the post-collision f and g
-// of every ring node in LDS; it writes the step's velocity for the tile and keeps the tile's
-// next force F(s+1) in LDS.  Phase 2 computes step s+1 on the tile, pulling from LDS, and
+// tiles), pulling from the step's input planes in HBM, and keeps in LDS the post-collision f
+// and g that step s+1 pulls (all directions of the tile, the inward ones of the ring); it
+// writes the step's velocity for the tile and keeps each tile node's next force F(s+1) in the
+// registers of the thread that updates the node again.  Phase 2 computes step s+1 on the
+// tile, pulling from LDS, and
 // writes the output planes, the force F(s+2), the velocity of step s+1 and, when asked, its
 // moments.  Per node and two steps the HBM traffic is one read and one write of the
 // populations (plus the ring reads) instead of two of each.
@@ -91,16 +93,6 @@ struct GlobalAcc {
   }
   __device__ __forceinline__ R f(int k, int di, int dj) const { return fa[(size_t)k * plane + off(di, dj)]; }
   __device__ __forceinline__ R g(int k, int di, int dj) const { return ga[(size_t)k * plane + off(di, dj)]; }
-};
-
-// The step-s populations in LDS at ring-local node (li, lj), rows of RW nodes, planes of RS.
-template <typename R, int RW, int RS>
-struct LdsAcc {
-  const R *lf;
-  const R *lg;
-  int o;  // lj * RW + li
-  __device__ __forceinline__ R f(int k, int di, int dj) const { return lf[k * RS + o + dj * RW + di]; }
-  __device__ __forceinline__ R g(int k, int di, int dj) const { return lg[k * RS + o + dj * RW + di]; }
 };
 
 // Moments and collision of a truth-run node: lbm_node64's arithmetic.
@@ -191,68 +183,158 @@ struct TwoStepPtrs {
   int *bad;             // non-finite flag of the second step's moments, or null
 };
 
-// LDS of a two-step tile: the ring's post-collision f and g, and the tile's next force.
+// LDS of a two-step tile: the post-collision f and g of step s that step s+1 pulls.  Local
+// coordinates: rows 0 .. TH+1 and columns 0 .. kTW+1, the tile at rows 1 .. TH and columns
+// 1 .. kTW, the one-node ring around it.  Tile nodes keep all nine directions; a ring node
+// only the three that point into the tile: the row below (row 0) directions 2, 5, 6, the row
+// above (row TH+1) 4, 7, 8, the left column (column 0) 1, 5, 8, the right column (column
+// kTW+1) 3, 6, 7.  A tile cut short by the lattice edge (nrows < TH or ncols < kTW) has its
+// upper or right ring inside the tile rows or columns, where all nine directions are kept.
+// FP32 at TH = 8: 40.4 KB, so that four workgroups share a CU (the full ring took 49.6 KB,
+// three per CU); FP64 at TH = 4: 43.6 KB, three per CU (two before).
 template <typename R, int TH>
 struct TileLds {
-  static constexpr int RW = kTW + 2, RH = TH + 2, RS = RW * RH;
-  R f[9 * RS];
-  R g[9 * RS];
-  R F[kTW * TH];
+  static constexpr int RW = kTW + 2;
+  R t[2][9][TH][kTW];  // [f/g][k][row-1][col-1]
+  R rb[2][3][RW];      // row 0: directions 2, 5, 6
+  R ra[2][3][RW];      // row TH+1: directions 4, 7, 8
+  R cl[2][3][TH];      // column 0, rows 1 .. TH: directions 1, 5, 8
+  R cr[2][3][TH];      // column kTW+1, rows 1 .. TH: directions 3, 6, 7
 };
 
-// Both steps on tile (tx, ty) of 64 x TH nodes; every thread of the workgroup calls it.
+// Slot of direction k in each ring store (directions that never reach the tile from there
+// map to 0: no access to them is ever made).
+__device__ __forceinline__ constexpr int slot_below(int k) { return k == 5 ? 1 : (k == 6 ? 2 : 0); }
+__device__ __forceinline__ constexpr int slot_above(int k) { return k == 7 ? 1 : (k == 8 ? 2 : 0); }
+__device__ __forceinline__ constexpr int slot_left(int k) { return k == 5 ? 1 : (k == 8 ? 2 : 0); }
+__device__ __forceinline__ constexpr int slot_right(int k) { return k == 6 ? 1 : (k == 7 ? 2 : 0); }
+
+// The step-s populations in LDS at local node (r, cc) of the tile: a.f(k, di, dj) is
+// direction k of the node (r + dj, cc + di).
+template <typename R, int TH>
+struct LdsAcc {
+  const TileLds<R, TH> &L;
+  int r, cc;
+  __device__ __forceinline__ R at(int w, int k, int di, int dj) const {
+    const int rs = r + dj, cs = cc + di;
+    if (rs == 0) return L.rb[w][slot_below(k)][cs];
+    if (rs == TH + 1) return L.ra[w][slot_above(k)][cs];
+    if (cs == 0) return L.cl[w][slot_left(k)][rs - 1];
+    if (cs == kTW + 1) return L.cr[w][slot_right(k)][rs - 1];
+    return L.t[w][k][rs - 1][cs - 1];
+  }
+  __device__ __forceinline__ R f(int k, int di, int dj) const { return at(0, k, di, dj); }
+  __device__ __forceinline__ R g(int k, int di, int dj) const { return at(1, k, di, dj); }
+};
+
+// Store the post-collision populations of local node (r, cc) where step s+1 reads them.
+template <typename R, int TH>
+__device__ __forceinline__ void keep_node(TileLds<R, TH> &L, int r, int cc, const R fo[9], const R go[9]) {
+  if (r == 0) {
+    L.rb[0][0][cc] = fo[2], L.rb[0][1][cc] = fo[5], L.rb[0][2][cc] = fo[6];
+    L.rb[1][0][cc] = go[2], L.rb[1][1][cc] = go[5], L.rb[1][2][cc] = go[6];
+  } else if (r == TH + 1) {
+    L.ra[0][0][cc] = fo[4], L.ra[0][1][cc] = fo[7], L.ra[0][2][cc] = fo[8];
+    L.ra[1][0][cc] = go[4], L.ra[1][1][cc] = go[7], L.ra[1][2][cc] = go[8];
+  } else if (cc == 0) {
+    L.cl[0][0][r - 1] = fo[1], L.cl[0][1][r - 1] = fo[5], L.cl[0][2][r - 1] = fo[8];
+    L.cl[1][0][r - 1] = go[1], L.cl[1][1][r - 1] = go[5], L.cl[1][2][r - 1] = go[8];
+  } else if (cc == kTW + 1) {
+    L.cr[0][0][r - 1] = fo[3], L.cr[0][1][r - 1] = fo[6], L.cr[0][2][r - 1] = fo[7];
+    L.cr[1][0][r - 1] = go[3], L.cr[1][1][r - 1] = go[6], L.cr[1][2][r - 1] = go[7];
+  } else {
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+      L.t[0][k][r - 1][cc - 1] = fo[k];
+      L.t[1][k][r - 1][cc - 1] = go[k];
+    }
+  }
+}
+
+// Step s at local node (r, cc) of tile (i0, j0), pulled from the input planes in HBM; its
+// populations go to LDS; returns F(s+1) in Fn, and writes the velocity of a tile node.
+template <typename R, int TH>
+__device__ __forceinline__ void first_step(const Geom &g, const TwoStepPtrs &a, const StepConst<R> &c, int i0,
+                                           int j0, int r, int cc, bool tile, TileLds<R, TH> &L, R &Fn) {
+  const int y = j0 - 1 + r;
+  int x = i0 - 1 + cc;
+  x = x < 0 ? x + g.nx : (x >= g.nx ? x - g.nx : x);
+  const bool bot = (y == 0), top = (y == g.ny - 1);
+  GlobalAcc<R> acc;
+  acc.fa = (const R *)a.fA;
+  acc.ga = (const R *)a.gA;
+  acc.plane = g.plane;
+  acc.r0 = (size_t)y * g.pitch;
+  acc.rm = bot ? acc.r0 : acc.r0 - g.pitch;
+  acc.rp = top ? acc.r0 : acc.r0 + g.pitch;
+  acc.i = x;
+  acc.im = x == 0 ? g.nx - 1 : x - 1;
+  acc.ip = x == g.nx - 1 ? 0 : x + 1;
+  R f[9], q[9], fo[9], go[9], rho, T, u, v;
+  gather_truth<R>(acc, bot, top, c, f, q);
+  const R Fy = ((const R *)a.Fin)[acc.r0 + x];
+  collide(c, f, q, Fy, fo, go, rho, T, u, v, Fn);
+  keep_node<R, TH>(L, r, cc, fo, go);
+  if (tile) st_uv<0, R>(a.uv1, acc.r0 + x, u, v);
+}
+
+// Both steps on tile (tx, ty) of kTW x TH nodes; every thread of the workgroup calls it.
+// Phase 1 computes step s on the tile and its ring: first the tile nodes, thread t taking
+// nodes t, t + kTileThreads, ... of the tile in row order (the same nodes it updates in phase 2,
+// so their next force stays in its registers), then the ring nodes, one per thread.  Phase 2
+// computes step s+1 on the tile from LDS and writes the output planes.
 template <typename R, int TH>
 __device__ __forceinline__ void two_step_tile(const Geom &g, const TwoStepPtrs &a, const StepConst<R> &c, int tx,
                                               int ty, TileLds<R, TH> &L) {
-  constexpr int RW = TileLds<R, TH>::RW, RS = TileLds<R, TH>::RS;
-  R *lf = L.f, *lg = L.g, *lF = L.F;
+  constexpr int NQ = kTW * TH / kTileThreads;  // tile nodes per thread
+  constexpr int NRING = 2 * (kTW + 2) + 2 * TH;
+  static_assert(kTW * TH % kTileThreads == 0 && NRING <= kTileThreads, "tile shape");
   const int i0 = tx * kTW, j0 = ty * TH;
   const int ncols = min(kTW, g.nx - i0), nrows = min(TH, g.ny - j0);
+  R Fk[NQ];
 
-  // Phase 1: step s on the tile and its ring.
-  for (int qn = threadIdx.x; qn < RS; qn += kTileThreads) {
-    const int r = qn / RW, cc = qn - r * RW;
-    const int y = j0 - 1 + r;
-    if (cc > ncols + 1 || r > nrows + 1 || y < 0 || y >= g.ny) continue;
-    int x = i0 - 1 + cc;
-    x = x < 0 ? x + g.nx : (x >= g.nx ? x - g.nx : x);
-    const bool bot = (y == 0), top = (y == g.ny - 1);
-    GlobalAcc<R> acc;
-    acc.fa = (const R *)a.fA;
-    acc.ga = (const R *)a.gA;
-    acc.plane = g.plane;
-    acc.r0 = (size_t)y * g.pitch;
-    acc.rm = bot ? acc.r0 : acc.r0 - g.pitch;
-    acc.rp = top ? acc.r0 : acc.r0 + g.pitch;
-    acc.i = x;
-    acc.im = x == 0 ? g.nx - 1 : x - 1;
-    acc.ip = x == g.nx - 1 ? 0 : x + 1;
-    R f[9], q[9], fo[9], go[9], rho, T, u, v, Fn;
-    gather_truth<R>(acc, bot, top, c, f, q);
-    const R Fy = ((const R *)a.Fin)[acc.r0 + x];
-    collide(c, f, q, Fy, fo, go, rho, T, u, v, Fn);
+  // Phase 1: step s.  Tile rows / columns up to nrows+1 / ncols+1 (the ring of a cut-short
+  // tile lies there).
 #pragma unroll
-    for (int k = 0; k < 9; k++) {
-      lf[k * RS + qn] = fo[k];
-      lg[k * RS + qn] = go[k];
+  for (int q = 0; q < NQ; q++) {
+    const int n = threadIdx.x + q * kTileThreads;
+    const int r = n / kTW + 1, cc = n % kTW + 1;
+    Fk[q] = R(0);
+    if (r > nrows + 1 || cc > ncols + 1 || j0 - 1 + r >= g.ny) continue;
+    first_step<R, TH>(g, a, c, i0, j0, r, cc, r <= nrows && cc <= ncols, L, Fk[q]);
+  }
+  if (threadIdx.x < NRING) {
+    const int t = threadIdx.x;
+    int r, cc;
+    if (t < kTW + 2) {
+      r = 0, cc = t;
+    } else if (t < 2 * (kTW + 2)) {
+      r = TH + 1, cc = t - (kTW + 2);
+    } else if (t < 2 * (kTW + 2) + TH) {
+      r = t - 2 * (kTW + 2) + 1, cc = 0;
+    } else {
+      r = t - 2 * (kTW + 2) - TH + 1, cc = kTW + 1;
     }
-    if (r >= 1 && r <= nrows && cc >= 1 && cc <= ncols) {
-      st_uv<0, R>(a.uv1, acc.r0 + x, u, v);
-      lF[(r - 1) * kTW + (cc - 1)] = Fn;
+    const int y = j0 - 1 + r;
+    if (r <= nrows + 1 && cc <= ncols + 1 && y >= 0 && y < g.ny) {
+      R Fn;
+      first_step<R, TH>(g, a, c, i0, j0, r, cc, false, L, Fn);
     }
   }
   __syncthreads();
 
   // Phase 2: step s+1 on the tile, from LDS.
-  for (int qn = threadIdx.x; qn < kTW * TH; qn += kTileThreads) {
-    const int r = qn / kTW + 1, cc = qn - (r - 1) * kTW + 1;
+#pragma unroll
+  for (int q = 0; q < NQ; q++) {
+    const int n = threadIdx.x + q * kTileThreads;
+    const int r = n / kTW + 1, cc = n % kTW + 1;
     if (cc > ncols || r > nrows) continue;
     const int x = i0 - 1 + cc, y = j0 - 1 + r;
     const bool bot = (y == 0), top = (y == g.ny - 1);
-    LdsAcc<R, RW, RS> acc{lf, lg, r * RW + cc};
-    R f[9], q[9], fo[9], go[9], rho, T, u, v, Fn;
-    gather_truth<R>(acc, bot, top, c, f, q);
-    collide(c, f, q, lF[qn], fo, go, rho, T, u, v, Fn);
+    LdsAcc<R, TH> acc{L, r, cc};
+    R f[9], q9[9], fo[9], go[9], rho, T, u, v, Fn;
+    gather_truth<R>(acc, bot, top, c, f, q9);
+    collide(c, f, q9, Fk[q], fo, go, rho, T, u, v, Fn);
     flag_nonfinite(a.bad, rho, T);
     const size_t o = (size_t)y * g.pitch + x;
     R *fB = (R *)a.fB, *gB = (R *)a.gB;
