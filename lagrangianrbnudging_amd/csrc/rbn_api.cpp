@@ -60,7 +60,7 @@ void dfree(void *&p) {
 void free_dep2(rbn_ctx *c) {
   rbn::DepPtrs &s = c->dep2;
   for (void *p : {(void *)s.cnt, (void *)s.fill, (void *)s.off, (void *)s.list, (void *)s.blocksum,
-                  (void *)s.active, (void *)s.nactive, (void *)s.Tbar, (void *)s.wq})
+                  (void *)s.active, (void *)s.nactive, (void *)s.Tbar, (void *)s.wq, (void *)s.S, (void *)s.Wi})
     if (p) (void)hipFree(p);
   s = rbn::DepPtrs{};
 }
@@ -73,11 +73,14 @@ int ensure_dep2(rbn_ctx *c) {
   s = c->dep;
   s.cnt = s.fill = s.off = s.list = s.blocksum = s.active = s.nactive = nullptr;
   s.Tbar = s.wq = nullptr;
+  s.S = s.Wi = nullptr;
   const size_t plane = c->geom.plane, ntiles = (plane + rbn::kDepScanTile - 1) / rbn::kDepScanTile;
+  const size_t sums = s.mode == RBN_DEPOSIT_BILINEAR ? plane * sizeof(long long) : 0;
   struct B {
     void **p;
     size_t b;
-  } a[] = {{(void **)&s.off, (plane + 1) * sizeof(int)},
+  } a[] = {{(void **)&s.S, sums}, {(void **)&s.Wi, sums},
+           {(void **)&s.off, (plane + 1) * sizeof(int)},
            {(void **)&s.fill, plane * sizeof(int)},
            {(void **)&s.blocksum, ntiles * sizeof(int)},
            {(void **)&s.list, (size_t)c->dep.list_cap * sizeof(int)},
@@ -87,6 +90,7 @@ int ensure_dep2(rbn_ctx *c) {
            {(void **)&s.wq, plane * sizeof(double)},
            {(void **)&s.cnt, plane * sizeof(int)}};  // last: a set without cnt is incomplete
   for (auto &x : a) {
+    if (x.b == 0) continue;
     int rc = dalloc(c, x.p, x.b);
     if (rc) {
       free_dep2(c);
@@ -559,6 +563,8 @@ void rbn_destroy(rbn_ctx *c) {
   dfree_t(c->dep.overflow);
   dfree_t(c->dep.active);
   dfree_t(c->dep.nactive);
+  dfree_t(c->dep.S);
+  dfree_t(c->dep.Wi);
   free_dep2(c);
   for (hipEvent_t ev : {c->ev_dep[0], c->ev_dep[1], c->ev_lat[0], c->ev_lat[1]})
     if (ev) (void)hipEventDestroy(ev);
@@ -657,8 +663,11 @@ int rbn_create(const rbn_params *p, int device, rbn_ctx **out) {
                    {(void **)&c->dep.Tbar, g.plane * sizeof(double)},
                    {(void **)&c->dep.wq, g.plane * sizeof(double)},
                    {(void **)&c->dep.overflow, sizeof(int)},
-                   {(void **)&c->dep.nactive, sizeof(int)}};
+                   {(void **)&c->dep.nactive, sizeof(int)},
+                   {(void **)&c->dep.S, p->deposit == RBN_DEPOSIT_BILINEAR ? g.plane * sizeof(long long) : 0},
+                   {(void **)&c->dep.Wi, p->deposit == RBN_DEPOSIT_BILINEAR ? g.plane * sizeof(long long) : 0}};
     for (auto &a : dallocs) {
+      if (a.b == 0) continue;
       if ((rc = dalloc(c, a.p, a.b))) goto fail_out;
       (void)hipMemsetAsync(*a.p, 0, a.b, c->stream);
     }

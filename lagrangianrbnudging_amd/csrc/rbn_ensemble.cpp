@@ -103,7 +103,7 @@ void drop_graph(rbn_ensemble *e) {
 // The second deposit set of a nudged member: the sizes of its own, the overflow flag shared.
 void free_spare(rbn::DepPtrs &s) {
   for (void *p : {(void *)s.cnt, (void *)s.fill, (void *)s.off, (void *)s.list, (void *)s.blocksum,
-                  (void *)s.active, (void *)s.nactive, (void *)s.Tbar, (void *)s.wq})
+                  (void *)s.active, (void *)s.nactive, (void *)s.Tbar, (void *)s.wq, (void *)s.S, (void *)s.Wi})
     if (p) (void)hipFree(p);
   s = rbn::DepPtrs{};
 }
@@ -113,12 +113,14 @@ int alloc_spare(rbn_ensemble *e, const rbn_ctx *c, rbn::DepPtrs &s) {
   rbn::DepPtrs x = c->dep;
   x.cnt = x.fill = x.off = x.list = x.blocksum = x.active = x.nactive = nullptr;
   x.Tbar = x.wq = nullptr;
+  x.S = x.Wi = nullptr;
   s = x;
   const size_t plane = c->geom.plane, ntiles = (plane + rbn::kDepScanTile - 1) / rbn::kDepScanTile;
+  const size_t sums = x.mode == RBN_DEPOSIT_BILINEAR ? plane * sizeof(long long) : 0;
   struct A {
     void **p;
     size_t b;
-  } a[] = {{(void **)&s.cnt, plane * sizeof(int)},
+  } a[] = {{(void **)&s.S, sums}, {(void **)&s.Wi, sums}, {(void **)&s.cnt, plane * sizeof(int)},
            {(void **)&s.fill, plane * sizeof(int)},
            {(void **)&s.off, (plane + 1) * sizeof(int)},
            {(void **)&s.blocksum, ntiles * sizeof(int)},
@@ -128,6 +130,9 @@ int alloc_spare(rbn_ensemble *e, const rbn_ctx *c, rbn::DepPtrs &s) {
            {(void **)&s.Tbar, plane * sizeof(double)},
            {(void **)&s.wq, plane * sizeof(double)}};
   for (auto &y : a) {
+    if (y.p == (void **)&s.S || y.p == (void **)&s.Wi) {
+      if (y.b == 0) continue;
+    }
     const size_t b = std::max<size_t>(y.b, sizeof(int));
     EHIP(e, hipMalloc(y.p, b));
     EHIP(e, hipMemset(*y.p, 0, b));

@@ -91,6 +91,9 @@ struct DepPtrs {
   int *nactive;  // their number
   long long active_cap;
   int dense;     // many contributions per node: fold every node in row order, no active list
+  // bilinear deposit: per-node fixed-point sums of w and w*T (2^-40 units), zero between
+  // deposits (the pass that turns them into Tbar and wq clears them); null for the 7x7 deposit
+  long long *S, *Wi;
 };
 
 // One nudged member of a batched observer/deposit launch (ensembles): its lattice, the
@@ -114,8 +117,10 @@ struct ObsJob {
 
 // Nodes per tile of the deposit's count scan (DepPtrs::blocksum holds one total per tile).
 constexpr int kDepScanTile = 1024;
-// Passes with per-member block counts (the prefix rows of a batched launch).
-enum { NB_OBS, NB_ZERO, NB_CAND, NB_SCAN, NB_FOLD, kNudgeBatches };
+// Passes with per-member block counts (the prefix rows of a batched launch): observer
+// update, zero pass, candidates (count or accumulate), count scan and list fill (7x7
+// deposit only), fold or finalize.
+enum { NB_OBS, NB_ZERO, NB_CAND, NB_SCAN, NB_FILL, NB_FOLD, kNudgeBatches };
 
 // Launch shape of the fused step: block width (64/128/256/512 nodes), non-temporal stores of
 // the outputs / loads of the inputs, XCD-banded block order, alternating sweep direction.

@@ -28,8 +28,12 @@
 // Bilinear deposit (RBN_DEPOSIT_BILINEAR; no reference counterpart, an option beside the
 // reference's running mean): each observer gives the four corners of its interpolation cell
 // the weights the record interpolation uses, so the deposit is the transpose of that
-// operator; per node W = sum w_k and S = sum w_k T_k in observer order (same four passes),
-// Tbar = S / W, and the nudged collision uses Q = alpha * (W * (T - Tbar)).
+// operator.  Per node W = sum w_k and S = sum w_k T_k are taken in fixed point (every term
+// rounded to a multiple of 2^-40, 64-bit integer atomics), so the sums are exact and do not
+// depend on the order in which the contributions arrive: one pass accumulates them (and
+// lists the nodes reached), a second turns them into Tbar = S / W and the weight W and clears
+// them for the next deposit; the nudged collision uses Q = alpha * (min(W, 1) * (T - Tbar)).
+// The sums must stay below 2^23 in magnitude (|T| W < 8.4e6 at a node).
 #include <algorithm>
 
 #include "rbn_internal.h"
@@ -155,8 +159,7 @@ __device__ __forceinline__ long long cell_of(const Geom &g, int right, int xi, i
 // Bilinear deposit (RBN_DEPOSIT_BILINEAR, no reference counterpart): observer k gives each
 // corner c of its interpolation cell the record-interpolation weight (M:292-311, same
 // operand order), ghost corners folded onto the interior node they copy; zero weights are
-// not contributions.  Returns the node offset and the weight, or -1.  The list key of a
-// contribution is 4k + c, so sorting keys sorts by observer.
+// not contributions.  Returns the node offset and the weight, or -1.
 constexpr int kBilCells = 4;
 
 __device__ __forceinline__ long long bil_cell(const Geom &g, double xx, double yy, int c, double &w) {
@@ -172,12 +175,17 @@ __device__ __forceinline__ long long bil_cell(const Geom &g, double xx, double y
   return (long long)(J - 1) * g.pitch + (I - 1);
 }
 
-// Zero the counts, fill cursors and active-node count: block b of nb, grid-stride.
+// Fixed point of the bilinear deposit's sums: v * 2^40 rounded to the nearest integer, ties
+// to even (the scaling is exact).
+__device__ __forceinline__ long long dep_fix(double v) { return (long long)rint(v * 0x1p40); }
+
+// Zero the counts, the fill cursors (7x7 deposit; null for the bilinear one) and the
+// active-node count: block b of nb, grid-stride.
 __device__ __forceinline__ void zero_part(size_t n, int *cnt, int *fill, int *nactive, int b, int nb) {
   if (b == 0 && threadIdx.x == 0) *nactive = 0;
   for (size_t o = (size_t)b * kBlock + threadIdx.x; o < n; o += (size_t)nb * kBlock) {
     cnt[o] = 0;
-    fill[o] = 0;
+    if (fill) fill[o] = 0;
   }
 }
 
@@ -195,32 +203,16 @@ __global__ __launch_bounds__(kBlock) void k_obs_zero(ObsPtrs o, int branch, int 
   }
 }
 
-// Node of candidate c of observer k for deposit MODE (-1: none); *key is the list key.
-template <int MODE>
-__device__ __forceinline__ long long candidate(const Geom &g, int right, const ObsPtrs &o, int k, int c,
-                                               int *key) {
-  if (MODE == RBN_DEPOSIT_BILINEAR) {
-    double w;
-    *key = 4 * k + c;
-    return bil_cell(g, o.xint[k], o.yint[k], c, w);
-  }
-  *key = k;
-  return cell_of(g, right, o.xidx[k], o.yidx[k], c);
-}
-
-// Candidate t (observer t / NC, cell t % NC) counts one contribution at its node.
-template <int MODE>
+// 7x7 deposit: candidate t (observer t / kCells, cell t % kCells) counts one contribution
+// at its node.
 __device__ __forceinline__ void count_one(const Geom &g, int right, const ObsPtrs &o, int *cnt, long long t) {
-  constexpr int NC = MODE == RBN_DEPOSIT_BILINEAR ? kBilCells : kCells;
-  if (t >= (long long)o.n * NC) return;
-  int key;
-  const long long off = candidate<MODE>(g, right, o, (int)(t / NC), (int)(t % NC), &key);
+  if (t >= (long long)o.n * kCells) return;
+  const long long off = cell_of(g, right, o.xidx[(int)(t / kCells)], o.yidx[(int)(t / kCells)], (int)(t % kCells));
   if (off >= 0) atomicAdd(&cnt[off], 1);
 }
 
-template <int MODE>
 __global__ void k_dep_count(Geom g, int right, ObsPtrs o, int *cnt) {
-  count_one<MODE>(g, right, o, cnt, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+  count_one(g, right, o, cnt, (long long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // Exclusive scan, pass 1: tile-local scans and tile totals (tile b; s: kBlock ints of LDS).
@@ -312,16 +304,15 @@ __device__ __forceinline__ void append_active(bool first, int node, int *active,
   }
 }
 
-// Contribution t takes the next slot of its node's range; the node's first contribution
-// also lists the node as active.
-template <int MODE>
+// 7x7 deposit: contribution t takes the next slot of its node's range (its key is the
+// observer); the node's first contribution also lists the node as active.
 __device__ __forceinline__ void fill_one(const Geom &g, int right, const ObsPtrs &o, const int *off,
                                          const int *tile_sum, int *fill, int *list, long long cap, int *overflow,
                                          int *active, int *nactive, long long active_cap, long long t) {
-  constexpr int NC = MODE == RBN_DEPOSIT_BILINEAR ? kBilCells : kCells;
-  int key = 0;
-  const long long node =
-      t < (long long)o.n * NC ? candidate<MODE>(g, right, o, (int)(t / NC), (int)(t % NC), &key) : -1;
+  const int key = (int)(t / kCells);
+  const long long node = t < (long long)o.n * kCells
+                             ? cell_of(g, right, o.xidx[key], o.yidx[key], (int)(t % kCells))
+                             : -1;
   const int rank = node >= 0 ? atomicAdd(&fill[node], 1) : -1;
   if (active) append_active(rank == 0, (int)node, active, nactive, active_cap);
   if (node < 0) return;
@@ -332,10 +323,43 @@ __device__ __forceinline__ void fill_one(const Geom &g, int right, const ObsPtrs
     atomicOr(overflow, 1);
 }
 
-template <int MODE>
 __global__ void k_dep_fill(Geom g, int right, ObsPtrs o, const int *off, const int *tile_sum, DepPtrs d) {
-  fill_one<MODE>(g, right, o, off, tile_sum, d.fill, d.list, d.list_cap, d.overflow, d.dense ? nullptr : d.active,
-                 d.nactive, d.active_cap, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+  fill_one(g, right, o, off, tile_sum, d.fill, d.list, d.list_cap, d.overflow, d.dense ? nullptr : d.active,
+           d.nactive, d.active_cap, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// Bilinear deposit: candidate t (observer t / 4, corner t % 4) adds its weight and weighted
+// temperature to the node's fixed-point sums and counts one contribution; the node's first
+// contribution lists it as active (unless dense).  Every thread of the workgroup calls it.
+__device__ __forceinline__ void acc_one(const Geom &g, const ObsPtrs &o, const DepPtrs &d, long long t) {
+  long long node = -1;
+  double w = 0.0, wT = 0.0;
+  if (t < (long long)o.n * kBilCells) {
+    const int k = (int)(t / kBilCells);
+    node = bil_cell(g, o.xint[k], o.yint[k], (int)(t % kBilCells), w);
+    wT = w * o.Tint[k];
+  }
+  int first = 0;
+  if (node >= 0) {
+    first = atomicAdd(&d.cnt[node], 1) == 0;
+    atomicAdd((unsigned long long *)&d.S[node], (unsigned long long)dep_fix(wT));
+    atomicAdd((unsigned long long *)&d.Wi[node], (unsigned long long)dep_fix(w));
+  }
+  if (!d.dense) append_active(first, (int)node, d.active, d.nactive, d.active_cap);
+}
+
+__global__ __launch_bounds__(kBlock) void k_dep_acc(Geom g, ObsPtrs o, DepPtrs d) {
+  acc_one(g, o, d, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// Bilinear deposit at node c with contributions: Tbar = S / W and W from the fixed-point
+// sums (each converted to double), which are cleared for the next deposit.
+__device__ __forceinline__ void fin_node(const DepPtrs &d, size_t c) {
+  const long long s = d.S[c], wi = d.Wi[c];
+  d.Tbar[c] = wi != 0 ? (double)s / (double)wi : 0.0;
+  d.wq[c] = (double)wi * 0x1p-40;
+  d.S[c] = 0;
+  d.Wi[c] = 0;
 }
 
 // Per node: order the contributions by observer and fold them in that order, the
@@ -346,28 +370,17 @@ __global__ void k_dep_fill(Geom g, int right, ObsPtrs o, const int *off, const i
 // a node at most once), so every sort gives the same order.
 constexpr int kSortRegs = 16;
 
-template <int MODE>
 struct Fold {
-  const Geom &g;
   const ObsPtrs &o;
-  double t = 0.0, S = 0.0, W = 0.0;
+  double t = 0.0;
   __device__ __forceinline__ void take(int a, int key) {
-    if (MODE == RBN_DEPOSIT_BILINEAR) {
-      const int k = key >> 2;
-      double w;
-      if (k < o.n && bil_cell(g, o.xint[k], o.yint[k], key & 3, w) >= 0) {
-        W = W + w;
-        S = S + w * o.Tint[k];
-      }
-    } else if (key >= 0 && key < o.n) {
-      t = (t * (double)a + o.Tint[key]) / ((double)a + 1.0);
-    }
+    if (key >= 0 && key < o.n) t = (t * (double)a + o.Tint[key]) / ((double)a + 1.0);
   }
 };
 
 // The n <= NS keys of a list sorted in registers and folded in that order.
-template <int MODE, int NS>
-__device__ __forceinline__ void fold_sorted(Fold<MODE> &f, const int *seg, int n) {
+template <int NS>
+__device__ __forceinline__ void fold_sorted(Fold &f, const int *seg, int n) {
   int key[NS];
 #pragma unroll
   for (int a = 0; a < NS; a++) key[a] = a < n ? seg[a] : 0x7fffffff;
@@ -392,21 +405,19 @@ __device__ __forceinline__ int wave_max(int v) {
   return v;
 }
 
-// Node c with n > 0 contributions; nmax: the wavefront's longest list.
-template <int MODE>
-__device__ __forceinline__ void fold_node(const Geom &g, const int *off, const int *tile_sum, int *list,
-                                          const ObsPtrs &o, double *Tbar, double *wq, long long cap, size_t c,
-                                          int n, int nmax) {
-  Fold<MODE> f{g, o};
+// 7x7 deposit at node c with n > 0 contributions; nmax: the wavefront's longest list.
+__device__ __forceinline__ void fold_node(const int *off, const int *tile_sum, int *list, const ObsPtrs &o,
+                                          double *Tbar, long long cap, size_t c, int n, int nmax) {
+  Fold f{o};
   const long long base = n > 0 ? seg_start(off, tile_sum, c) : 0;
   if (n > 0 && base + n <= cap) {
     int *seg = list + base;
     if (nmax <= 2) {
-      fold_sorted<MODE, 2>(f, seg, n);
+      fold_sorted<2>(f, seg, n);
     } else if (nmax <= 4) {
-      fold_sorted<MODE, 4>(f, seg, n);
+      fold_sorted<4>(f, seg, n);
     } else if (n <= kSortRegs) {
-      fold_sorted<MODE, kSortRegs>(f, seg, n);
+      fold_sorted<kSortRegs>(f, seg, n);
     } else {
       for (int a = 1; a < n; a++) {  // insertion sort in place
         const int key = seg[a];
@@ -420,46 +431,44 @@ __device__ __forceinline__ void fold_node(const Geom &g, const int *off, const i
       for (int a = 0; a < n; a++) f.take(a, seg[a]);
     }
   }
-  if (MODE == RBN_DEPOSIT_BILINEAR) {
-    Tbar[c] = f.W != 0.0 ? f.S / f.W : 0.0;
-    wq[c] = f.W;
-  } else {
-    Tbar[c] = f.t;
-  }
+  Tbar[c] = f.t;
 }
 
-// The active nodes of a deposit, block b of nb, grid-stride (every lane of a wavefront takes
-// part in the wavefront's maximum).
-template <int MODE>
-__device__ __forceinline__ void fold_active(const Geom &g, const ObsPtrs &o, const DepPtrs &d, int b, int nb) {
+// The active nodes of a deposit, block b of nb, grid-stride: the 7x7 fold (every lane of a
+// wavefront takes part in the wavefront's maximum) or the bilinear finalisation.
+__device__ __forceinline__ void fold_active(const ObsPtrs &o, const DepPtrs &d, int b, int nb) {
   const int na = (int)min((long long)*d.nactive, d.active_cap);
   for (int k0 = b * kBlock; k0 < na; k0 += nb * kBlock) {
     const int k = k0 + threadIdx.x;
     const size_t c = k < na ? (size_t)d.active[k] : 0;
+    if (d.mode == RBN_DEPOSIT_BILINEAR) {
+      if (k < na) fin_node(d, c);
+      continue;
+    }
     const int n = k < na ? d.cnt[c] : 0;
     const int nmax = wave_max(n);
-    if (k < na) fold_node<MODE>(g, d.off, d.blocksum, d.list, o, d.Tbar, d.wq, d.list_cap, c, n, nmax);
+    if (k < na) fold_node(d.off, d.blocksum, d.list, o, d.Tbar, d.list_cap, c, n, nmax);
   }
 }
 
-// Row j of a deposit, nodes i0 .. i0 + kBlock - 1 (dense deposits).
-template <int MODE>
+// Row j of a deposit, nodes i0 .. i0 + kBlock - 1 (dense deposits): the nodes with
+// contributions are folded (7x7) or finalised (bilinear).
 __device__ __forceinline__ void fold_row(const Geom &g, const ObsPtrs &o, const DepPtrs &d, int i0, int j) {
   const int i = i0 + threadIdx.x;
   const size_t c = (size_t)j * g.pitch + i;
   const int n = i < g.nx ? d.cnt[c] : 0;
+  if (d.mode == RBN_DEPOSIT_BILINEAR) {
+    if (n > 0) fin_node(d, c);
+    return;
+  }
   const int nmax = wave_max(n);
-  if (n > 0) fold_node<MODE>(g, d.off, d.blocksum, d.list, o, d.Tbar, d.wq, d.list_cap, c, n, nmax);
+  if (n > 0) fold_node(d.off, d.blocksum, d.list, o, d.Tbar, d.list_cap, c, n, nmax);
 }
 
-template <int MODE>
-__global__ __launch_bounds__(kBlock) void k_dep_fold(Geom g, ObsPtrs o, DepPtrs d) {
-  fold_active<MODE>(g, o, d, blockIdx.x, gridDim.x);
-}
+__global__ __launch_bounds__(kBlock) void k_dep_fold(ObsPtrs o, DepPtrs d) { fold_active(o, d, blockIdx.x, gridDim.x); }
 
-template <int MODE>
 __global__ __launch_bounds__(kBlock) void k_dep_fold_rows(Geom g, ObsPtrs o, DepPtrs d) {
-  fold_row<MODE>(g, o, d, blockIdx.x * kBlock, blockIdx.y);
+  fold_row(g, o, d, blockIdx.x * kBlock, blockIdx.y);
 }
 
 // ---------------------------------------------------------------- batched (ensembles)
@@ -508,19 +517,22 @@ __global__ __launch_bounds__(kBlock) void k_nb_obs_zero(const NudgeMember *M, co
   } else {
     const int *pre = pre_all + NB_ZERO * (nm + 1);
     const int zb = blockIdx.x - nobs, m = member_of(pre, nm, zb);
-    zero_part(M[m].g.plane, M[m].d.cnt, M[m].d.fill, M[m].d.nactive, zb - pre[m], pre[m + 1] - pre[m]);
+    const DepPtrs &d = M[m].d;
+    zero_part(M[m].g.plane, d.cnt, d.mode == RBN_DEPOSIT_BILINEAR ? nullptr : d.fill, d.nactive, zb - pre[m],
+              pre[m + 1] - pre[m]);
   }
 }
 
+// Counts (7x7 deposit) or fixed-point sums (bilinear deposit) of every member's candidates.
 __global__ __launch_bounds__(kBlock) void k_nb_count(const NudgeMember *M, const int *pre_all, int nm) {
   const int *pre = pre_all + NB_CAND * (nm + 1);
   const int m = member_of(pre, nm, blockIdx.x);
   const NudgeMember &d = M[m];
   const long long t = (long long)(blockIdx.x - pre[m]) * kBlock + threadIdx.x;
   if (d.d.mode == RBN_DEPOSIT_BILINEAR)
-    count_one<RBN_DEPOSIT_BILINEAR>(d.g, d.right, d.o, d.d.cnt, t);
+    acc_one(d.g, d.o, d.d, t);
   else
-    count_one<RBN_DEPOSIT_BOX7>(d.g, d.right, d.o, d.d.cnt, t);
+    count_one(d.g, d.right, d.o, d.d.cnt, t);
 }
 
 __global__ __launch_bounds__(kBlock) void k_nb_scan_tiles(const NudgeMember *M, const int *pre_all, int nm) {
@@ -535,41 +547,33 @@ __global__ __launch_bounds__(kBlock) void k_nb_scan_sums(const NudgeMember *M) {
   __shared__ int s[kBlock];
   __shared__ int carry;
   const NudgeMember &d = M[blockIdx.x];
+  if (d.d.mode == RBN_DEPOSIT_BILINEAR) return;  // uniform per block: no count scan
   const int ntiles = (int)((d.g.plane + kScanTile - 1) / kScanTile);
   scan_sums(ntiles, d.d.blocksum, d.d.off + d.g.plane, s, carry);
 }
 
+// The list fill of the 7x7 members (bilinear members have no blocks in this pass).
 __global__ __launch_bounds__(kBlock) void k_nb_fill(const NudgeMember *M, const int *pre_all, int nm) {
-  const int *pre = pre_all + NB_CAND * (nm + 1);
+  const int *pre = pre_all + NB_FILL * (nm + 1);
   const int m = member_of(pre, nm, blockIdx.x);
   const NudgeMember &d = M[m];
   const long long t = (long long)(blockIdx.x - pre[m]) * kBlock + threadIdx.x;
   const DepPtrs &p = d.d;
-  int *const act = p.dense ? nullptr : p.active;
-  if (p.mode == RBN_DEPOSIT_BILINEAR)
-    fill_one<RBN_DEPOSIT_BILINEAR>(d.g, d.right, d.o, p.off, p.blocksum, p.fill, p.list, p.list_cap, p.overflow,
-                                   act, p.nactive, p.active_cap, t);
-  else
-    fill_one<RBN_DEPOSIT_BOX7>(d.g, d.right, d.o, p.off, p.blocksum, p.fill, p.list, p.list_cap, p.overflow, act,
-                               p.nactive, p.active_cap, t);
+  fill_one(d.g, d.right, d.o, p.off, p.blocksum, p.fill, p.list, p.list_cap, p.overflow, p.dense ? nullptr : p.active,
+           p.nactive, p.active_cap, t);
 }
 
+// The 7x7 fold or the bilinear finalisation of every member.
 __global__ __launch_bounds__(kBlock) void k_nb_fold(const NudgeMember *M, const int *pre_all, int nm) {
   const int *pre = pre_all + NB_FOLD * (nm + 1);
   const int m = member_of(pre, nm, blockIdx.x);
   const NudgeMember &d = M[m];
   const int lb = blockIdx.x - pre[m];
-  const bool bil = d.d.mode == RBN_DEPOSIT_BILINEAR;
   if (d.d.dense) {
     const int bx = nudge_blocks(d.g), j = lb / bx, i0 = (lb - j * bx) * kBlock;
-    if (bil)
-      fold_row<RBN_DEPOSIT_BILINEAR>(d.g, d.o, d.d, i0, j);
-    else
-      fold_row<RBN_DEPOSIT_BOX7>(d.g, d.o, d.d, i0, j);
-  } else if (bil) {
-    fold_active<RBN_DEPOSIT_BILINEAR>(d.g, d.o, d.d, lb, pre[m + 1] - pre[m]);
+    fold_row(d.g, d.o, d.d, i0, j);
   } else {
-    fold_active<RBN_DEPOSIT_BOX7>(d.g, d.o, d.d, lb, pre[m + 1] - pre[m]);
+    fold_active(d.o, d.d, lb, pre[m + 1] - pre[m]);
   }
 }
 
@@ -596,11 +600,13 @@ hipError_t launch_obs_scatter(hipStream_t s, const ObsJob *jobs, int njobs, cons
 }
 
 void nudge_batch_blocks(const NudgeMember &d, int blocks[kNudgeBatches]) {
-  const long long nc = (long long)d.o.n * (d.d.mode == RBN_DEPOSIT_BILINEAR ? kBilCells : kCells);
+  const bool bil = d.d.mode == RBN_DEPOSIT_BILINEAR;
+  const long long nc = (long long)d.o.n * (bil ? kBilCells : kCells);
   blocks[NB_OBS] = (d.o.n + kBlock - 1) / kBlock;
   blocks[NB_ZERO] = (int)((d.g.plane + kZeroPerBlock - 1) / kZeroPerBlock);
   blocks[NB_CAND] = (int)((nc + kBlock - 1) / kBlock);
-  blocks[NB_SCAN] = (int)((d.g.plane + kScanTile - 1) / kScanTile);
+  blocks[NB_SCAN] = bil ? 0 : (int)((d.g.plane + kScanTile - 1) / kScanTile);
+  blocks[NB_FILL] = bil ? 0 : blocks[NB_CAND];
   blocks[NB_FOLD] = d.d.dense ? (d.g.nx + kBlock - 1) / kBlock * d.g.ny : fold_blocks(d.d, 256);
 }
 
@@ -610,9 +616,11 @@ hipError_t launch_nudge_batch(hipStream_t s, const NudgeMember *members, const i
   const dim3 b(kBlock);
   hipLaunchKernelGGL(k_nb_obs_zero, dim3(totals[NB_OBS] + totals[NB_ZERO]), b, 0, s, members, pre, nm, t);
   if (totals[NB_CAND] > 0) hipLaunchKernelGGL(k_nb_count, dim3(totals[NB_CAND]), b, 0, s, members, pre, nm);
-  hipLaunchKernelGGL(k_nb_scan_tiles, dim3(totals[NB_SCAN]), b, 0, s, members, pre, nm);
-  hipLaunchKernelGGL(k_nb_scan_sums, dim3(nm), b, 0, s, members);
-  if (totals[NB_CAND] > 0) hipLaunchKernelGGL(k_nb_fill, dim3(totals[NB_CAND]), b, 0, s, members, pre, nm);
+  if (totals[NB_SCAN] > 0) {  // the 7x7 members' count scan and list fill
+    hipLaunchKernelGGL(k_nb_scan_tiles, dim3(totals[NB_SCAN]), b, 0, s, members, pre, nm);
+    hipLaunchKernelGGL(k_nb_scan_sums, dim3(nm), b, 0, s, members);
+  }
+  if (totals[NB_FILL] > 0) hipLaunchKernelGGL(k_nb_fill, dim3(totals[NB_FILL]), b, 0, s, members, pre, nm);
   hipLaunchKernelGGL(k_nb_fold, dim3(totals[NB_FOLD]), b, 0, s, members, pre, nm);
   return hipGetLastError();
 }
@@ -620,39 +628,29 @@ hipError_t launch_nudge_batch(hipStream_t s, const NudgeMember *members, const i
 hipError_t launch_obs_deposit(hipStream_t s, int branch, int ntempread, const Geom &g, int right_limit,
                               const ObsPtrs &o, const DepPtrs &d) {
   const size_t n = g.plane;
-  const int ntiles = (int)((n + kScanTile - 1) / kScanTile);
+  const bool bil = d.mode == RBN_DEPOSIT_BILINEAR;
   const int zb = (int)std::min<size_t>((n + kBlock - 1) / kBlock, 4096);
   const int obs_blocks = (o.n + kBlock - 1) / kBlock;
   hipLaunchKernelGGL(k_obs_zero, dim3(obs_blocks + zb), dim3(kBlock), 0, s, o, branch, g.nx, g.ny, ntempread,
-                     obs_blocks, n, d.cnt, d.fill, d.nactive);
-  const bool bil = d.mode == RBN_DEPOSIT_BILINEAR;
+                     obs_blocks, n, d.cnt, bil ? nullptr : d.fill, d.nactive);
+  if (o.n <= 0) return hipGetLastError();
   const long long work = (long long)o.n * (bil ? kBilCells : kCells);
-  const dim3 ob((unsigned)((work + kBlock - 1) / kBlock > 0 ? (work + kBlock - 1) / kBlock : 1));
+  const dim3 ob((unsigned)((work + kBlock - 1) / kBlock));
   const dim3 fb((unsigned)fold_blocks(d, 1024)), rows((g.nx + kBlock - 1) / kBlock, g.ny);
-  if (o.n > 0) {
-    if (bil)
-      hipLaunchKernelGGL(k_dep_count<RBN_DEPOSIT_BILINEAR>, ob, dim3(kBlock), 0, s, g, right_limit, o, d.cnt);
-    else
-      hipLaunchKernelGGL(k_dep_count<RBN_DEPOSIT_BOX7>, ob, dim3(kBlock), 0, s, g, right_limit, o, d.cnt);
+  if (bil) {
+    // fixed-point sums, then Tbar and W at the nodes reached (and the sums cleared)
+    hipLaunchKernelGGL(k_dep_acc, ob, dim3(kBlock), 0, s, g, o, d);
+  } else {
+    const int ntiles = (int)((n + kScanTile - 1) / kScanTile);
+    hipLaunchKernelGGL(k_dep_count, ob, dim3(kBlock), 0, s, g, right_limit, o, d.cnt);
+    hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(kBlock), 0, s, n, d.cnt, d.off, d.blocksum);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kBlock), 0, s, ntiles, d.blocksum, d.off + n);
+    hipLaunchKernelGGL(k_dep_fill, ob, dim3(kBlock), 0, s, g, right_limit, o, d.off, d.blocksum, d);
   }
-  hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(kBlock), 0, s, n, d.cnt, d.off, d.blocksum);
-  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kBlock), 0, s, ntiles, d.blocksum, d.off + n);
-  if (o.n > 0) {
-    if (bil)
-      hipLaunchKernelGGL(k_dep_fill<RBN_DEPOSIT_BILINEAR>, ob, dim3(kBlock), 0, s, g, right_limit, o, d.off,
-                         d.blocksum, d);
-    else
-      hipLaunchKernelGGL(k_dep_fill<RBN_DEPOSIT_BOX7>, ob, dim3(kBlock), 0, s, g, right_limit, o, d.off,
-                         d.blocksum, d);
-    if (d.dense && bil)
-      hipLaunchKernelGGL(k_dep_fold_rows<RBN_DEPOSIT_BILINEAR>, rows, dim3(kBlock), 0, s, g, o, d);
-    else if (d.dense)
-      hipLaunchKernelGGL(k_dep_fold_rows<RBN_DEPOSIT_BOX7>, rows, dim3(kBlock), 0, s, g, o, d);
-    else if (bil)
-      hipLaunchKernelGGL(k_dep_fold<RBN_DEPOSIT_BILINEAR>, fb, dim3(kBlock), 0, s, g, o, d);
-    else
-      hipLaunchKernelGGL(k_dep_fold<RBN_DEPOSIT_BOX7>, fb, dim3(kBlock), 0, s, g, o, d);
-  }
+  if (d.dense)
+    hipLaunchKernelGGL(k_dep_fold_rows, rows, dim3(kBlock), 0, s, g, o, d);
+  else
+    hipLaunchKernelGGL(k_dep_fold, fb, dim3(kBlock), 0, s, o, d);
   return hipGetLastError();
 }
 

@@ -582,22 +582,36 @@ int or_deposit_only(or_sim *s, int n, const int *xidx, const int *yidx, const do
   return 0;
 }
 
+/* Fixed point of the bilinear deposit's sums: v * 2^40 rounded to the nearest integer, ties
+ * to even (the scaling is exact, rint rounds in the default mode). */
+static long long or_dep_fix(double v) { return (long long)rint(v * 0x1p40); }
+
 /* Bilinear deposit (RBN_DEPOSIT_BILINEAR; no reference counterpart): observer k at the
  * continuous position (x_k, y_k) gives each of the four corners of its interpolation cell the
  * bilinear weight that the record interpolation (M:292-311) gives that corner, with ghost
  * corners folded onto the interior node they copy (periodic in x and y, as communication).
- * It is the transpose of the interpolation operator.  Per node, in observer order:
- *   W = sum w_k,  S = sum w_k T_k,  tempreadint = S / W,  nudge_vol = number of observers
- * with w_k != 0.  The nudged collision then uses Q = alpha * (min(W, 1) * (T - tempreadint)):
+ * It is the transpose of the interpolation operator.  Per node the sums are taken in fixed
+ * point, each term rounded to a multiple of 2^-40 and added exactly, so they do not depend
+ * on the order of the observers:
+ *   Wi = sum fix(w_k),  Si = sum fix(w_k T_k),  W = Wi 2^-40,  tempreadint = Si / Wi
+ * (each converted to double; 0 where Wi = 0), nudge_vol = number of observers with w_k != 0.
+ * The nudged collision then uses Q = alpha * (min(W, 1) * (T - tempreadint)):
  * sum_k alpha w_k (T - T_k) where the observers' weights at the node sum to at most 1, and a
  * full relaxation toward the weighted mean where several observers reach the node (W > 1),
- * so that alpha <= 1 never over-relaxes.  Corners are visited in the order (x1,y1), (x2,y1), (x1,y2),
- * (x2,y2); an observer reaches a node at most once, so the per-node order is observer order. */
+ * so that alpha <= 1 never over-relaxes. */
 static void or_deposit_bilinear(or_sim *s, int n, const double *x, const double *y, const double *Tk) {
   const int NX = s->NX, NY = s->NY;
+  const size_t nn = (size_t)NX * (size_t)NY;
+  long long *Si = (long long *)calloc(nn ? nn : 1, sizeof(long long));
+  long long *Wi = (long long *)calloc(nn ? nn : 1, sizeof(long long));
   memset(s->nudge_vol, 0, s->nn * sizeof(int));
-  memset(s->tempreadint, 0, (size_t)NX * (size_t)NY * sizeof(double));
-  memset(s->nudge_w, 0, (size_t)NX * (size_t)NY * sizeof(double));
+  memset(s->tempreadint, 0, nn * sizeof(double));
+  memset(s->nudge_w, 0, nn * sizeof(double));
+  if (!Si || !Wi) {
+    free(Si);
+    free(Wi);
+    return;
+  }
   for (int k = 0; k < n; k++) {
     const double xx = x[k], yy = y[k];
     if (!(xx > -1.0 && xx < (double)NX + 1.0 && yy > -1.0 && yy < (double)NY + 1.0)) continue;
@@ -610,13 +624,17 @@ static void or_deposit_bilinear(or_sim *s, int n, const double *x, const double 
       const int i = ci[c] == 0 ? NX : (ci[c] == NX + 1 ? 1 : ci[c]);
       const int j = cj[c] == 0 ? NY : (cj[c] == NY + 1 ? 1 : cj[c]);
       const size_t id = IX(s, i, j), ir = (size_t)(j - 1) + (size_t)NY * (size_t)(i - 1);
-      s->nudge_w[ir] = s->nudge_w[ir] + w[c];
-      s->tempreadint[ir] = s->tempreadint[ir] + w[c] * Tk[k];
+      Wi[ir] += or_dep_fix(w[c]);
+      Si[ir] += or_dep_fix(w[c] * Tk[k]);
       s->nudge_vol[id] += 1;
     }
   }
-  for (size_t ir = 0; ir < (size_t)NX * (size_t)NY; ir++)
-    if (s->nudge_w[ir] != 0.0) s->tempreadint[ir] = s->tempreadint[ir] / s->nudge_w[ir];
+  for (size_t ir = 0; ir < nn; ir++) {
+    s->nudge_w[ir] = (double)Wi[ir] * 0x1p-40;
+    s->tempreadint[ir] = Wi[ir] != 0 ? (double)Si[ir] / (double)Wi[ir] : 0.0;
+  }
+  free(Si);
+  free(Wi);
 }
 
 int or_deposit_bilinear_only(or_sim *s, int n, const double *x, const double *y, const double *T) {

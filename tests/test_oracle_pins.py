@@ -438,6 +438,11 @@ def _bil_oracle(nx=32, ny=16, **kw):
     return p, o
 
 
+def _fix(v):
+    """The deposit's fixed point: v * 2^40 rounded to the nearest integer, ties to even."""
+    return int(np.rint(np.float64(v) * 2.0 ** 40))
+
+
 def test_bilinear_deposit_single_observer_gives_the_interpolation_weights():
     p, o = _bil_oracle()
     x, y, T = 10.3, 8.6, 0.7
@@ -447,14 +452,19 @@ def test_bilinear_deposit_single_observer_gives_the_interpolation_weights():
     # 1-based corners (10,8), (11,8), (10,9), (11,9) with the record-interpolation weights
     w[9, 7], w[10, 7] = (11 - x) * (9 - y), (x - 10) * (9 - y)
     w[9, 8], w[10, 8] = (11 - x) * (y - 8), (x - 10) * (y - 8)
-    assert np.array_equal(nd["weight"], w)
+    # the weights in fixed point: within half a unit of 2^-40 of the interpolation weights
+    expect = np.vectorize(lambda v: _fix(v) * 2.0 ** -40)(w)
+    assert np.array_equal(nd["weight"], expect) and np.max(np.abs(expect - w)) <= 2.0 ** -41
     assert np.array_equal(nd["nudge_vol"], (w > 0).astype(np.int32))
-    assert np.max(np.abs(nd["tempreadint"][w > 0] - T)) <= 1e-15
+    for i, j in ((9, 7), (10, 7), (9, 8), (10, 8)):
+        assert nd["tempreadint"][i, j] == float(_fix(w[i, j] * T)) / float(_fix(w[i, j]))
+    assert np.max(np.abs(nd["tempreadint"][w > 0] - T)) <= 1e-11
     assert np.all(nd["tempreadint"][w == 0] == 0.0)
     # a node on a lattice line: zero weights are not contributions
     o.deposit_bilinear_only([12.0], [5.0], [0.4])
     nd = o.nudging()
-    assert nd["nudge_vol"].sum() == 1 and nd["weight"][11, 4] == 1.0 and nd["tempreadint"][11, 4] == 0.4
+    assert nd["nudge_vol"].sum() == 1 and nd["weight"][11, 4] == 1.0
+    assert nd["tempreadint"][11, 4] == float(_fix(0.4)) / float(_fix(1.0))  # 0.4 to within 2^-41
 
 
 def test_bilinear_deposit_folds_ghost_corners_periodically():
@@ -478,19 +488,24 @@ def test_bilinear_deposit_conserves_weight_and_heat():
     assert abs(np.sum(nd["weight"] * nd["tempreadint"]) - T.sum()) < 1e-11
 
 
-def test_bilinear_deposit_sums_in_observer_order():
-    p, o = _bil_oracle()
-    x = np.array([10.2, 10.9, 10.55])
-    y = np.array([8.3, 8.8, 8.05])
-    T = np.array([0.1, 0.7, 0.3])
-    o.deposit_bilinear_only(x, y, T)
-    nd = o.nudging()
-    S = W_ = 0.0
-    for k in range(3):  # node (11, 9) is corner (x2, y2) of every observer
-        w = (x[k] - 10) * (y[k] - 8)
-        W_ = W_ + w
-        S = S + w * T[k]
-    assert nd["weight"][10, 8] == W_ and nd["tempreadint"][10, 8] == S / W_
+def test_bilinear_deposit_sums_are_exact_and_order_free():
+    # Node (11, 9) is corner (x2, y2) of every observer: its sums are the exact integer sums
+    # of the terms' fixed points, whatever the observers' order.
+    x = np.array([10.2, 10.9, 10.55, 10.05])
+    y = np.array([8.3, 8.8, 8.05, 8.95])
+    T = np.array([0.1, 0.7, 0.3, -0.45])
+    Wi = sum(_fix((x[k] - 10) * (y[k] - 8)) for k in range(4))
+    Si = sum(_fix((x[k] - 10) * (y[k] - 8) * T[k]) for k in range(4))
+    results = []
+    for order in ([0, 1, 2, 3], [3, 2, 1, 0], [2, 0, 3, 1]):
+        p, o = _bil_oracle()
+        o.deposit_bilinear_only(x[order], y[order], T[order])
+        nd = o.nudging()
+        assert nd["weight"][10, 8] == Wi * 2.0 ** -40 and nd["tempreadint"][10, 8] == float(Si) / float(Wi)
+        assert nd["nudge_vol"][10, 8] == 4
+        results.append((nd["weight"], nd["tempreadint"]))
+    for w, t in results[1:]:
+        assert np.array_equal(w, results[0][0]) and np.array_equal(t, results[0][1])
 
 
 @pytest.mark.parametrize("copies", [1, 8])
