@@ -335,12 +335,14 @@ int rbn_adj_backward(rbn_ctx *ctx, double *dJ_dT0, double *dJ_dalpha);
  *                working set fits in 160 MB)
  *   "lbm_reverse" 1: sweep every odd step last row first, so that it starts on the rows the
  *                previous step wrote last (default 1)
- *   "lbm_steps"  2: truth runs advance two steps per lattice launch, the first step's
- *                populations kept in LDS (temporal blocking), for every step that writes no
- *                moments and is followed by another step of the same rbn_step call; with
- *                particles only under particle_fuse (default 2); 1: one step per launch
- *   "lbm2_seg"   8..64: two steps per launch by row sweeps, every workgroup sweeping a strip
- *                of 256 columns over a segment of lbm2_seg rows, the first step's
+ *   "lbm_steps"  2: advance two steps per lattice launch, the first step's populations kept
+ *                in LDS (temporal blocking), for every step that writes no moments and is
+ *                followed by another step of the same rbn_step call; truth runs with
+ *                particles only under particle_fuse; nudged runs without dep_overlap, the
+ *                observer updates and deposits of both steps run before the launch (the
+ *                deposits depend on the records only) (default 2); 1: one step per launch
+ *   "lbm2_seg"   (truth runs) 8..64: two steps per launch by row sweeps, every workgroup
+ *                sweeping a strip of 256 columns over a segment of lbm2_seg rows, the first step's
  *                sideways-moving populations passed between its threads through LDS; 0: by
  *                tiles of 64 columns and lbm2_tile_h rows (default 0)
  *   "lbm2_xcd"   1: the row-sweep workgroups one XCD receives take consecutive strips and

@@ -529,7 +529,7 @@ void rbn_destroy(rbn_ctx *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->pstream) (void)hipStreamSynchronize(c->pstream);
   rbn::adj_free(c);
-  void *ptrs[] = {c->fA, c->fB, c->gA, c->gB, c->F, c->F2, c->rho, c->T, c->nud};
+  void *ptrs[] = {c->fA, c->fB, c->gA, c->gB, c->F, c->F2, c->rho, c->T, c->nud, c->nud2};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   for (int k = 0; k < rbn_ctx::kVel; k++)
@@ -650,8 +650,9 @@ int rbn_create(const rbn_params *p, int device, rbn_ctx **out) {
   if ((rc = dalloc(c, &tmp, (2 * 1024 + 2) * sizeof(double)))) goto fail_out;
   c->d_energy = (double *)tmp;
   if (p->variant == RBN_VARIANT_NUDGING) {
-    if ((rc = dalloc(c, &c->nud, fb))) goto fail_out;
+    if ((rc = dalloc(c, &c->nud, fb)) || (rc = dalloc(c, &c->nud2, fb))) goto fail_out;
     (void)hipMemsetAsync(c->nud, 0, fb, c->stream);
+    (void)hipMemsetAsync(c->nud2, 0, fb, c->stream);
     const size_t ntiles = (g.plane + rbn::kDepScanTile - 1) / rbn::kDepScanTile;
     struct B {
       void **p;
@@ -1206,7 +1207,7 @@ int step_loop(rbn_ctx *c, int nsteps) {
   const bool fuse = parts && !overlap && c->tune.fuse != 0;
   // lbm_steps 2: two truth-run steps per lattice launch, the first of which writes no
   // moments; with particles only when their updates ride in lattice launches.
-  const bool pairs = c->tune.steps2 && !nudged && (!parts || fuse);
+  const bool pairs = c->tune.steps2 && (!parts || fuse);
   hipStream_t ks = overlap ? c->pstream : c->stream;
   // dep_overlap: the observer update and deposit of step s+1 run on pstream (unused by nudged
   // runs, which have no particles) beside the lattice step s.  Off by default: measured at
@@ -1215,13 +1216,18 @@ int step_loop(rbn_ctx *c, int nsteps) {
   // 16 us lattice step it hides).
   const bool dovl = nudged && c->tune.dep_overlap && nsteps > 1;
   if (dovl && (rc = ensure_dep2(c))) return rc;
+  // Nudged pairs (lbm_steps 2, without dep_overlap): the deposit of the pair's first step goes
+  // to dep2 and that of its second step to dep, the set the getters and a following one-step
+  // launch read.
+  const bool npairs = nudged && pairs && !dovl && nsteps > 1;
+  if (npairs && (rc = ensure_dep2(c))) return rc;
   if (nsteps > 0) c->at_init = false;
   for (int n = 0; n < nsteps; n++) {
     const int istep = ++c->istep;
     const bool record = parts && (istep % P.noutconfig) == 0;
     const bool diag = nd > 0 && (istep % nd) == 0;
     const bool moments = record || diag || n == nsteps - 1;
-    const bool pair = pairs && !moments && n + 1 < nsteps;
+    const bool pair = pairs && (!nudged || npairs) && !moments && n + 1 < nsteps;
     int istep2 = 0;
     bool record2 = false, diag2 = false, moments2 = false;
     if (pair) {
@@ -1232,7 +1238,10 @@ int step_loop(rbn_ctx *c, int nsteps) {
       moments2 = record2 || diag2 || n == nsteps - 1;
     }
     const bool writes_moments = pair ? moments2 : moments;
-    if (nudged && !dovl) {
+    if (nudged && pair) {
+      if ((rc = nudged_deposit(c, istep, c->stream, c->dep2)) || (rc = nudged_deposit(c, istep2, c->stream, c->dep)))
+        return rc;
+    } else if (nudged && !dovl) {
       if ((rc = nudged_deposit(c, istep, c->stream, c->dep))) return rc;
     } else if (nudged) {
       // step n's deposit is in set n & 1 (dep, dep2), built on pstream: the first here, the
@@ -1241,7 +1250,7 @@ int step_loop(rbn_ctx *c, int nsteps) {
       if (n == 0) HIPCHK(c, hipEventRecord(c->ev_dep[0], c->pstream));
       HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_dep[n & 1], 0));
     }
-    const rbn::DepPtrs &dset = dovl && (n & 1) ? c->dep2 : c->dep;
+    const rbn::DepPtrs &dset = (dovl && (n & 1)) || (nudged && pair) ? c->dep2 : c->dep;
     // A launch writes the velocity slots after cur: K1 of step s overwrites the slot of step
     // s-6, a two-step launch also that of s-5; pending updates read slots s-3 .. s-1.
     bool carry = false;
@@ -1279,9 +1288,17 @@ int step_loop(rbn_ctx *c, int nsteps) {
            rbn::LbmTuning t = c->tune;
            if (pair) {
              const rbn::PartJob job = carry ? pending_job(c) : rbn::PartJob{};
+             rbn::NudgePair np{};
+             if (nudged) {
+               np.nvol2 = c->dep.cnt;
+               np.Tbar2 = c->dep.Tbar;
+               np.wq2 = c->dep.mode == RBN_DEPOSIT_BILINEAR ? c->dep.wq : nullptr;
+               np.nud_out = c->nud2;
+             }
              return rbn::launch_lbm2(c->stream, c->prec, c->geom, lp, c->F2, c->uv[s2], moments2 ? c->rho : nullptr,
                                      moments2 ? c->T : nullptr, c->consts.data(), t.tile_h, t.seg, t.xcd2, t.variant,
-                                     t.reverse && ((istep >> 1) & 1), carry ? &job : nullptr);
+                                     t.reverse && ((istep >> 1) & 1), carry ? &job : nullptr,
+                                     nudged ? &np : nullptr);
            }
            t.flip = t.reverse && (istep & 1);
            if (carry)
@@ -1311,6 +1328,7 @@ int step_loop(rbn_ctx *c, int nsteps) {
     std::swap(c->fA, c->fB);
     std::swap(c->gA, c->gB);
     if (pair) std::swap(c->F, c->F2);
+    if (pair && nudged) std::swap(c->nud, c->nud2);
     const int prv = c->cur;
     c->cur = pair ? s2 : s1;
     c->moments_valid = writes_moments;

@@ -189,9 +189,9 @@ void launch_prec(hipStream_t s, int phase, dim3 g, const void *members, const in
 // Two-step tile of steps k, k+1 (64 x TH nodes, rbn_lbm2_tile.h).
 template <typename R, int TH>
 __device__ __forceinline__ void pair_item(const EnsMember<R> &m, int tile, const EnsStep &st,
-                                          lbm2::TileLds<R, TH> &L) {
+                                          lbm2::TileLds<R, TH, 3> &L) {
   const int tx = tile % m.t2x, ty = tile / m.t2x;
-  lbm2::TwoStepPtrs p;
+  lbm2::TwoStepPtrs p{};
   p.fA = m.pf[st.lpar];
   p.gA = m.pg[st.lpar];
   p.fB = m.pf[1 - st.lpar];
@@ -203,14 +203,14 @@ __device__ __forceinline__ void pair_item(const EnsMember<R> &m, int tile, const
   p.rho2 = st.moments ? m.rho : nullptr;
   p.T2 = st.moments ? m.T : nullptr;
   p.bad = m.bad;
-  lbm2::two_step_tile<R, TH>(m.g, p, m.c, tx, ty, L);
+  lbm2::two_step_tile<R, TH, RBN_VARIANT_TRAJECTORIES, 3>(m.g, p, m.c, tx, ty, L);
 }
 
 // Item `it` of a two-step-mode launch: carried particle items first, then lattice items.
 template <typename R, int TH>
 __device__ __forceinline__ void run_item2(const EnsMember<R> *__restrict__ M, const int2 *__restrict__ items, int it,
                                           int n_lbm, int n_part, int n_first, int n_lat, const EnsStep &st,
-                                          int tile_w, int tile_h, lbm2::TileLds<R, TH> &L) {
+                                          int tile_w, int tile_h, lbm2::TileLds<R, TH, 3> &L) {
   if (it < n_first) {
     const int2 w = items[n_lbm + it];
     for (int u = 0; u < st.ncarry; u++) particle_item(M[w.x], w.y, st.k - st.ncarry + u);
@@ -231,7 +231,7 @@ __global__ __launch_bounds__(kThreads) void k_ensemble2(const EnsMember<R> *__re
                                                         const int2 *__restrict__ items, int n_lbm, int n_part,
                                                         int n_lbm2, int *counters, EnsStep st, int tile_w,
                                                         int tile_h, int persistent) {
-  __shared__ lbm2::TileLds<R, TH> L;
+  __shared__ lbm2::TileLds<R, TH, 3> L;
   const int slot = st.lat ? 0 : 1;
   const int n_first = st.ncarry > 0 ? n_part : 0;
   const int n_lat = st.lat == 2 ? n_lbm2 : (st.lat == 1 ? n_lbm : 0);

@@ -178,13 +178,22 @@ void adj_free(rbn_ctx *c);
 // Launchers (defined in the .hip files).
 hipError_t launch_lbm(hipStream_t s, int prec, int variant, const Geom &g, const LbmPtrs &p,
                       const void *step_const, const LbmTuning &t);
-// Two truth-run steps of K1 in one launch (rbn_lbm2.hip): p holds the first step's inputs and
-// the second step's outputs (fB, gB), p.F = F(s), p.uv = the first step's velocity slot;
-// Fout receives F(s+2), uv2 the second step's velocity, rho2 / T2 (or null) its moments.
-// job (or null): particle updates carried as extra workgroups ahead of the tiles.
+// The second step's deposit of a two-step nudged launch, and the plane that receives the
+// persisted nudging term after it (the first step's deposit and the term entering it are the
+// LbmPtrs' nvol, Tbar, wq and nud).
+struct NudgePair {
+  const int *nvol2;
+  const double *Tbar2, *wq2;
+  void *nud_out;
+};
+// Two steps of K1 in one launch (rbn_lbm2.hip): p holds the first step's inputs and the second
+// step's outputs (fB, gB), p.F = F(s), p.uv = the first step's velocity slot; Fout receives
+// F(s+2), uv2 the second step's velocity, rho2 / T2 (or null) its moments.  job (or null):
+// particle updates carried as extra workgroups ahead of the tiles.  np (or null): the nudged
+// run's pair (tiles only).
 hipError_t launch_lbm2(hipStream_t s, int prec, const Geom &g, const LbmPtrs &p, void *Fout, void *uv2,
                        void *rho2, void *T2, const void *step_const, int tile_h, int seg, int xcd, int variant,
-                       int flip, const PartJob *job);
+                       int flip, const PartJob *job, const NudgePair *np = nullptr);
 // K1 of a truth-run step with the particle update of the previous step in the same grid
 // (its own workgroups, ahead of the lattice ones).
 hipError_t launch_lbm_fused(hipStream_t s, int prec, const Geom &g, const LbmPtrs &p,
@@ -254,6 +263,7 @@ struct rbn_ctx {
   hipEvent_t ev_dep[2] = {}, ev_lat[2] = {};
   bool part_record_pending = false;    // K2 of the latest step reads T (record step)
   void *nud = nullptr;
+  void *nud2 = nullptr;  // nudged run: a two-step launch reads nud and writes nud2, then they swap
   int istep = 0;
   bool initialised = false;
   bool moments_valid = false;  // rho/T hold the latest hydrovar

@@ -1,8 +1,9 @@
-// rbn_lbm2.hip — two truth-run steps of K1 in one launch: by row sweeps over column strips
-// (rbn_lbm2_wave.h) or by tiles (rbn_lbm2_tile.h), the first step's populations passed
-// through LDS either way.  A step that must write its moments is never the first of a pair
-// (the host launches it alone).  Particle updates of earlier steps ride in the launch as
-// workgroups ahead of the lattice ones (rbn_lbm_fused.hip explains why that is safe).
+// rbn_lbm2.hip — two steps of K1 in one launch: truth-run steps by row sweeps over column
+// strips (rbn_lbm2_wave.h) or by tiles (rbn_lbm2_tile.h), nudged-run steps by tiles, the first
+// step's populations passed through LDS either way.  A step that must write its moments is
+// never the first of a pair (the host launches it alone).  Particle updates of earlier steps
+// ride in the launch as workgroups ahead of the lattice ones (rbn_lbm_fused.hip explains why
+// that is safe).
 #include "rbn_lbm2_tile.h"
 #include "rbn_lbm2_wave.h"
 #include "rbn_particle_node.h"
@@ -22,16 +23,16 @@ struct Lbm2Args {
 // benchmark's particles), so the pad takes a workgroup's LDS above a quarter of the CU's
 // 160 KB.  FP64 tiles of 64 x 4 (43.6 KB) fit three per CU without it and gain from the
 // third (0.556 against 0.594 ms with two).
-template <typename R, int TH>
+template <typename R, int TH, int V>
 struct KernelLds {
-  TileLds<R, TH> L;
+  TileLds<R, TH, ring_g<V>()> L;
   char cap[(sizeof(R) == 4 && TH == 8) ? 8192 : 1];
 };
 
-template <typename R, int TH>
+template <typename R, int TH, int V>
 __global__ __launch_bounds__(kTileThreads) void k_lbm2(Geom g, Lbm2Args a, StepConst<R> c, part::PartSteps<R> ps) {
-  __shared__ KernelLds<R, TH> S;
-  TileLds<R, TH> &L = S.L;
+  __shared__ KernelLds<R, TH, V> S;
+  auto &L = S.L;
   if ((int)blockIdx.x < ps.nblocks) {
     part::run_part_steps(ps, blockIdx.x * kTileThreads + threadIdx.x);
     return;
@@ -39,7 +40,7 @@ __global__ __launch_bounds__(kTileThreads) void k_lbm2(Geom g, Lbm2Args a, StepC
   const int b = (int)blockIdx.x - ps.nblocks;
   const int t = a.flip ? a.ntiles - 1 - b : b;
   const int ty = t / a.tiles_x, tx = t - ty * a.tiles_x;
-  two_step_tile<R, TH>(g, a.p, c, tx, ty, L);
+  two_step_tile<R, TH, V, ring_g<V>()>(g, a.p, c, tx, ty, L);
 }
 
 struct WaveArgs {
@@ -84,9 +85,9 @@ void dispatch_wave(hipStream_t s, const Geom &g, const WaveArgs &a, int variant,
     hipLaunchKernelGGL((k_lbm2w<R, 0, kStripW>), grid, block, 0, s, g, a, c, ps);
 }
 
-// Tile heights: 4, 8 or 16 rows (LDS 29.5 / 49.6 / 89.6 KB in FP32); FP64 at most 8 rows
-// (99 KB; 16 rows would exceed the 160 KB of a CU).  0 picks the measured best: 8 rows in
-// FP32 (3 workgroups per CU), 4 in FP64 (2 workgroups per CU).
+// Tile heights: 4, 8 or 16 rows in FP32; FP64 at most 8 rows (16 rows would exceed the 160 KB
+// of a CU).  0 picks the measured best: 8 rows in FP32, 4 in FP64 (three workgroups per CU
+// each).
 int tile_rows(int prec, int tile_h) {
   if (tile_h == 0) return prec == RBN_FP64 ? 4 : 8;
   if (tile_h == 4) return 4;
@@ -94,37 +95,39 @@ int tile_rows(int prec, int tile_h) {
   return 8;
 }
 
+template <int V>
 void dispatch32(hipStream_t s, int th, const Geom &g, const Lbm2Args &a, const void *step_const,
                 const PartJob *job) {
   const StepConst<float> &c = *(const StepConst<float> *)step_const;
   const part::PartSteps<float> ps = part::part_steps<float>(g, job, kTileThreads);
   const dim3 grid((unsigned)(a.ntiles + ps.nblocks)), block(kTileThreads);
   if (th == 4)
-    hipLaunchKernelGGL((k_lbm2<float, 4>), grid, block, 0, s, g, a, c, ps);
+    hipLaunchKernelGGL((k_lbm2<float, 4, V>), grid, block, 0, s, g, a, c, ps);
   else if (th == 16)
-    hipLaunchKernelGGL((k_lbm2<float, 16>), grid, block, 0, s, g, a, c, ps);
+    hipLaunchKernelGGL((k_lbm2<float, 16, V>), grid, block, 0, s, g, a, c, ps);
   else
-    hipLaunchKernelGGL((k_lbm2<float, 8>), grid, block, 0, s, g, a, c, ps);
+    hipLaunchKernelGGL((k_lbm2<float, 8, V>), grid, block, 0, s, g, a, c, ps);
 }
 
+template <int V>
 void dispatch64(hipStream_t s, int th, const Geom &g, const Lbm2Args &a, const void *step_const,
                 const PartJob *job) {
   const StepConst<double> &c = *(const StepConst<double> *)step_const;
   const part::PartSteps<double> ps = part::part_steps<double>(g, job, kTileThreads);
   const dim3 grid((unsigned)(a.ntiles + ps.nblocks)), block(kTileThreads);
   if (th == 4)
-    hipLaunchKernelGGL((k_lbm2<double, 4>), grid, block, 0, s, g, a, c, ps);
+    hipLaunchKernelGGL((k_lbm2<double, 4, V>), grid, block, 0, s, g, a, c, ps);
   else
-    hipLaunchKernelGGL((k_lbm2<double, 8>), grid, block, 0, s, g, a, c, ps);
+    hipLaunchKernelGGL((k_lbm2<double, 8, V>), grid, block, 0, s, g, a, c, ps);
 }
 
 }  // namespace
 
 hipError_t launch_lbm2(hipStream_t s, int prec, const Geom &g, const LbmPtrs &p, void *Fout, void *uv2,
                        void *rho2, void *T2, const void *step_const, int tile_h, int seg, int xcd, int variant,
-                       int flip, const PartJob *job) {
-  if (seg > 0) {
-    WaveArgs w;
+                       int flip, const PartJob *job, const NudgePair *np) {
+  if (seg > 0 && !np) {
+    WaveArgs w{};
     w.p.fA = p.fA;
     w.p.gA = p.gA;
     w.p.fB = p.fB;
@@ -148,7 +151,7 @@ hipError_t launch_lbm2(hipStream_t s, int prec, const Geom &g, const LbmPtrs &p,
     return hipGetLastError();
   }
   const int th = tile_rows(prec, tile_h);
-  Lbm2Args a;
+  Lbm2Args a{};
   a.p.fA = p.fA;
   a.p.gA = p.gA;
   a.p.fB = p.fB;
@@ -160,13 +163,34 @@ hipError_t launch_lbm2(hipStream_t s, int prec, const Geom &g, const LbmPtrs &p,
   a.p.rho2 = rho2;
   a.p.T2 = T2;
   a.p.bad = p.bad;
+  a.p.nv[0] = a.p.nv[1] = nullptr;
+  a.p.tb[0] = a.p.tb[1] = a.p.wq[0] = a.p.wq[1] = nullptr;
+  a.p.nud_in = nullptr;
+  a.p.nud_out = nullptr;
+  if (np) {
+    a.p.nv[0] = p.nvol;
+    a.p.tb[0] = p.Tbar;
+    a.p.wq[0] = p.wq;
+    a.p.nud_in = p.nud;
+    a.p.nv[1] = np->nvol2;
+    a.p.tb[1] = np->Tbar2;
+    a.p.wq[1] = np->wq2;
+    a.p.nud_out = np->nud_out;
+  }
   a.tiles_x = (g.nx + kTW - 1) / kTW;
   a.ntiles = a.tiles_x * ((g.ny + th - 1) / th);
   a.flip = flip;
-  if (prec == RBN_FP64)
-    dispatch64(s, th, g, a, step_const, job);
-  else
-    dispatch32(s, th, g, a, step_const, job);
+  if (prec == RBN_FP64) {
+    if (np)
+      dispatch64<RBN_VARIANT_NUDGING>(s, th, g, a, step_const, job);
+    else
+      dispatch64<RBN_VARIANT_TRAJECTORIES>(s, th, g, a, step_const, job);
+  } else {
+    if (np)
+      dispatch32<RBN_VARIANT_NUDGING>(s, th, g, a, step_const, job);
+    else
+      dispatch32<RBN_VARIANT_TRAJECTORIES>(s, th, g, a, step_const, job);
+  }
   return hipGetLastError();
 }
 

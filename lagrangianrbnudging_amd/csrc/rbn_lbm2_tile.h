@@ -28,12 +28,10 @@ using namespace lbm;
 constexpr int kTileThreads = 256;
 constexpr int kTW = 64;  // tile width: one wavefront per tile row
 
-// Pulled populations of a truth-run node (walls: half-way bounce-back for f from the node's
-// own pre-stream values, the plate rule for g), through an accessor a.f(k, di, dj) /
-// a.g(k, di, dj) of the input at the node offset (di, dj).
+// Pulled f of a node (walls: half-way bounce-back from the node's own pre-stream values),
+// through an accessor a.f(k, di, dj) of the input at the node offset (di, dj).
 template <typename R, class Acc>
-__device__ __forceinline__ void gather_truth(const Acc &a, bool bot, bool top, const StepConst<R> &c, R f[9],
-                                             R q[9]) {
+__device__ __forceinline__ void gather_f(const Acc &a, bool bot, bool top, R f[9]) {
   f[0] = a.f(0, 0, 0);
   f[1] = a.f(1, -1, 0);
   f[3] = a.f(3, 1, 0);
@@ -55,6 +53,13 @@ __device__ __forceinline__ void gather_truth(const Acc &a, bool bot, bool top, c
     f[7] = a.f(5, 0, 0);
     f[8] = a.f(6, 0, 0);
   }
+}
+
+// Pulled populations of a truth-run node: f as gather_f, g with the plate rule at the walls.
+template <typename R, class Acc>
+__device__ __forceinline__ void gather_truth(const Acc &a, bool bot, bool top, const StepConst<R> &c, R f[9],
+                                             R q[9]) {
+  gather_f<R>(a, bot, top, f);
   q[0] = a.g(0, 0, 0);
   q[1] = a.g(1, -1, 0);
   q[3] = a.g(3, 1, 0);
@@ -78,6 +83,32 @@ __device__ __forceinline__ void gather_truth(const Acc &a, bool bot, bool top, c
     q[5] = -q[7] + c.wbot[1];
     q[6] = -q[8] + c.wbot[2];
   }
+}
+
+// Pulled populations of a nudged-run node: f as gather_f; g with the zero-gradient walls of
+// lbm_node64's nudged branch (Nudging/code/tempvel_nudge.c:684-702, 783-962): a wall row takes
+// the pre-stream populations of its inner neighbour row, unshifted, and the rows next to the
+// walls (b1: row 1, t1: row ny-2) pull the directions that cross toward the wall from their own
+// row, where the wall row's copy came from.
+template <typename R, class Acc>
+__device__ __forceinline__ void gather_nudged(const Acc &a, bool bot, bool top, bool b1, bool t1, R f[9], R q[9]) {
+  gather_f<R>(a, bot, top, f);
+  if (bot || top) {
+    const int dj = bot ? 1 : -1;
+#pragma unroll
+    for (int k = 0; k < 9; k++) q[k] = a.g(k, 0, dj);
+    return;
+  }
+  const int dm = b1 ? 0 : -1, dp = t1 ? 0 : 1;
+  q[0] = a.g(0, 0, 0);
+  q[1] = a.g(1, -1, 0);
+  q[3] = a.g(3, 1, 0);
+  q[2] = a.g(2, 0, dm);
+  q[5] = a.g(5, -1, dm);
+  q[6] = a.g(6, 1, dm);
+  q[4] = a.g(4, 0, dp);
+  q[7] = a.g(7, 1, dp);
+  q[8] = a.g(8, -1, dp);
 }
 
 // The input planes in HBM at node (i, j): periodic x, rows j-1 / j+1 (never read across a
@@ -161,10 +192,114 @@ __device__ __forceinline__ void collide32(const StepConst<float> &c, const float
   for (int k = 0; k < 9; k++) go[k] = q[k] * c.one_m_omg + c.omg * geq[k] - c.bw[k] * Q;
 }
 
+// The nudging of one node in a step of the nudged run: its deposit count, mean and (bilinear
+// deposit) weight, and the persisted term entering the step.
+struct NodeNudge {
+  int nv;
+  double Tbar, W;
+  bool bil;
+};
+
+// Moments and collision of a nudged-run node: lbm_node64's nudged arithmetic (half the
+// persisted term in T where the node receives a deposit; the nudged collisg), operand for
+// operand.  nud: the term entering the step; nud_out: after it (unchanged without a deposit).
+__device__ __forceinline__ void collide_n64(const StepConst<double> &c, const double f[9], const double q[9],
+                                            double Fy, const NodeNudge &d, double nud, double fo[9], double go[9],
+                                            double &rho, double &T, double &u, double &v, double &Fn,
+                                            double &nud_out) {
+  rho = 0.0;
+  T = 0.0;
+#pragma unroll
+  for (int k = 0; k < 9; k++) rho = rho + f[k];
+#pragma unroll
+  for (int k = 0; k < 9; k++) T = T + q[k];
+  if (d.nv > 0) T = (T + 0.5 * nud);
+  const double rhoi = 1. / rho;
+  double fl[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) fl[k] = f[k] * rhoi;
+  const double Fx = 0.0;
+  u = fl[1] - fl[3] + fl[5] - fl[6] - fl[7] + fl[8] + 0.5 * rhoi * Fx;
+  v = fl[5] + fl[2] + fl[6] - fl[7] - fl[4] - fl[8] + 0.5 * rhoi * Fy;
+  Fn = c.g0 * rho * (T - 0.0);
+  double b[9], fac[9];
+  eq_brackets64(u, v, b);
+  guo_factor64(u, v, Fy, fac);
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    const double feq = rho * kW[k] * b[k];
+    fo[k] = f[k] * c.one_m_om + c.om * feq + c.src[k] * fac[k];
+  }
+  if (d.nv > 0) {
+    const double Q = d.bil ? c.alpha * (nudge_weight(d.W) * (T - d.Tbar)) : c.alpha * (T - d.Tbar);
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+      const double geq = T * kW[k] * b[k];
+      go[k] = q[k] * c.one_m_omg + c.omg * geq - c.bw[k] * Q;
+    }
+    nud_out = -1. * Q;
+  } else {
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+      const double geq = T * kW[k] * b[k];
+      go[k] = q[k] * c.one_m_omg + c.omg * geq;
+    }
+    nud_out = nud;
+  }
+}
+
+// lbm_node32's nudged arithmetic with shifted populations.
+__device__ __forceinline__ void collide_n32(const StepConst<float> &c, const float f[9], const float q[9], float Fy,
+                                            const NodeNudge &d, float nud, float fo[9], float go[9], float &rho,
+                                            float &T, float &u, float &v, float &Fn, float &nud_out) {
+  float dr = 0.f, dT = 0.f;
+#pragma unroll
+  for (int k = 0; k < 9; k++) dr += f[k];
+#pragma unroll
+  for (int k = 0; k < 9; k++) dT += q[k];
+  if (d.nv > 0) dT += 0.5f * nud;
+  rho = c.rho0 + dr;
+  T = c.T0 + dT;
+  const float rhoi = 1.f / rho;
+  const float jx = f[1] - f[3] + f[5] - f[6] - f[7] + f[8];
+  const float jy = f[2] - f[4] + f[5] + f[6] - f[7] - f[8];
+  u = jx * rhoi;
+  v = jy * rhoi + 0.5f * rhoi * Fy;
+  Fn = c.g0 * rho * T;
+  float A[9];
+  eq_dev32(u, v, A);
+  const float tz = -3.f * v * Fy, tp = 3.f * (1.f - v) * Fy, tm = 3.f * (-1.f - v) * Fy;
+  const float d2 = 9.f * v * Fy, d5 = 9.f * (u + v) * Fy, d6 = 9.f * (v - u) * Fy;
+  const float fac[9] = {tz, tz, tp + d2, tz, tm + d2, tp + d5, tp + d6, tm + d5, tm + d6};
+  float feq[9], geq[9];
+  eq_pops32(dr, rho, A, feq);
+#pragma unroll
+  for (int k = 0; k < 9; k++) fo[k] = f[k] * c.one_m_om + c.om * feq[k] + c.src[k] * fac[k];
+  float Q = 0.f;
+  nud_out = nud;
+  if (d.nv > 0) {
+    Q = d.bil ? c.alpha * ((float)nudge_weight(d.W) * (T - (float)d.Tbar)) : c.alpha * (T - (float)d.Tbar);
+    nud_out = -Q;
+  }
+  eq_pops32(dT, T, A, geq);
+#pragma unroll
+  for (int k = 0; k < 9; k++) go[k] = q[k] * c.one_m_omg + c.omg * geq[k] - c.bw[k] * Q;
+}
+
 __device__ __forceinline__ void collide(const StepConst<double> &c, const double f[9], const double q[9],
                                         double Fy, double fo[9], double go[9], double &rho, double &T,
                                         double &u, double &v, double &Fn) {
   collide64(c, f, q, Fy, fo, go, rho, T, u, v, Fn);
+}
+__device__ __forceinline__ void collide_n(const StepConst<double> &c, const double f[9], const double q[9],
+                                          double Fy, const NodeNudge &d, double nud, double fo[9], double go[9],
+                                          double &rho, double &T, double &u, double &v, double &Fn, double &nud_out) {
+  collide_n64(c, f, q, Fy, d, nud, fo, go, rho, T, u, v, Fn, nud_out);
+}
+__device__ __forceinline__ void collide_n(const StepConst<float> &c, const float f[9], const float q[9], float Fy,
+                                          const NodeNudge &d, float nud, float fo[9], float go[9], float &rho,
+                                          float &T, float &u, float &v, float &Fn, float &nud_out) {
+  collide_n32(c, f, q, Fy, d, nud, fo, go, rho, T, u, v, Fn, nud_out);
 }
 __device__ __forceinline__ void collide(const StepConst<float> &c, const float f[9], const float q[9],
                                         float Fy, float fo[9], float go[9], float &rho, float &T, float &u,
@@ -181,6 +316,13 @@ struct TwoStepPtrs {
   void *uv1, *uv2;      // velocity slots of steps s and s+1
   void *rho2, *T2;      // moments of step s+1, or null
   int *bad;             // non-finite flag of the second step's moments, or null
+  // nudged run: the deposits of steps s and s+1 (counts, means, bilinear weights or null) and
+  // the persisted nudging term entering step s and after step s+1 (double-buffered: other
+  // tiles still read the input at the nodes of this tile's ring)
+  const int *nv[2];
+  const double *tb[2], *wq[2];
+  const void *nud_in;
+  void *nud_out;
 };
 
 // LDS of a two-step tile: the post-collision f and g of step s that step s+1 pulls.  Local
@@ -188,16 +330,25 @@ struct TwoStepPtrs {
 // 1 .. kTW, the one-node ring around it.  Tile nodes keep all nine directions; a ring node
 // only the three that point into the tile: the row below (row 0) directions 2, 5, 6, the row
 // above (row TH+1) 4, 7, 8, the left column (column 0) 1, 5, 8, the right column (column
-// kTW+1) 3, 6, 7.  A tile cut short by the lattice edge (nrows < TH or ncols < kTW) has its
-// upper or right ring inside the tile rows or columns, where all nine directions are kept.
-// FP32 at TH = 8: 40.4 KB, so that four workgroups share a CU (the full ring took 49.6 KB,
-// three per CU); FP64 at TH = 4: 43.6 KB, three per CU (two before).
-template <typename R, int TH>
+// kTW+1) 3, 6, 7.  In the nudged run the ring rows keep all nine directions of g: a wall row
+// copies its inner neighbour row, which for a one-row tile at the top wall is the row below.
+// A tile cut short by the lattice edge (nrows < TH or ncols < kTW) has its upper or right
+// ring inside the tile rows or columns, where all nine directions are kept.
+// FP32 at TH = 8: 40.4 KB for the truth run; FP64 at TH = 4: 43.6 KB.
+// GR: g directions kept in the ring rows, 3 (truth run) or 9 (nudged run, or an ensemble
+// launch that may hold both).
+template <int V>
+constexpr int ring_g() { return V == RBN_VARIANT_NUDGING ? 9 : 3; }
+
+template <typename R, int TH, int GR_>
 struct TileLds {
   static constexpr int RW = kTW + 2;
+  static constexpr int GR = GR_;
   R t[2][9][TH][kTW];  // [f/g][k][row-1][col-1]
-  R rb[2][3][RW];      // row 0: directions 2, 5, 6
-  R ra[2][3][RW];      // row TH+1: directions 4, 7, 8
+  R rbf[3][RW];        // row 0, f: directions 2, 5, 6
+  R raf[3][RW];        // row TH+1, f: directions 4, 7, 8
+  R rbg[GR][RW];       // row 0, g: 2, 5, 6 (or all nine)
+  R rag[GR][RW];       // row TH+1, g: 4, 7, 8 (or all nine)
   R cl[2][3][TH];      // column 0, rows 1 .. TH: directions 1, 5, 8
   R cr[2][3][TH];      // column kTW+1, rows 1 .. TH: directions 3, 6, 7
 };
@@ -211,14 +362,15 @@ __device__ __forceinline__ constexpr int slot_right(int k) { return k == 6 ? 1 :
 
 // The step-s populations in LDS at local node (r, cc) of the tile: a.f(k, di, dj) is
 // direction k of the node (r + dj, cc + di).
-template <typename R, int TH>
+template <typename R, int TH, int GR>
 struct LdsAcc {
-  const TileLds<R, TH> &L;
+  using L_t = TileLds<R, TH, GR>;
+  const L_t &L;
   int r, cc;
   __device__ __forceinline__ R at(int w, int k, int di, int dj) const {
     const int rs = r + dj, cs = cc + di;
-    if (rs == 0) return L.rb[w][slot_below(k)][cs];
-    if (rs == TH + 1) return L.ra[w][slot_above(k)][cs];
+    if (rs == 0) return w ? L.rbg[L_t::GR == 9 ? k : slot_below(k)][cs] : L.rbf[slot_below(k)][cs];
+    if (rs == TH + 1) return w ? L.rag[L_t::GR == 9 ? k : slot_above(k)][cs] : L.raf[slot_above(k)][cs];
     if (cs == 0) return L.cl[w][slot_left(k)][rs - 1];
     if (cs == kTW + 1) return L.cr[w][slot_right(k)][rs - 1];
     return L.t[w][k][rs - 1][cs - 1];
@@ -228,14 +380,25 @@ struct LdsAcc {
 };
 
 // Store the post-collision populations of local node (r, cc) where step s+1 reads them.
-template <typename R, int TH>
-__device__ __forceinline__ void keep_node(TileLds<R, TH> &L, int r, int cc, const R fo[9], const R go[9]) {
+template <typename R, int TH, int GR>
+__device__ __forceinline__ void keep_node(TileLds<R, TH, GR> &L, int r, int cc, const R fo[9], const R go[9]) {
+  constexpr bool all_g = GR == 9;
   if (r == 0) {
-    L.rb[0][0][cc] = fo[2], L.rb[0][1][cc] = fo[5], L.rb[0][2][cc] = fo[6];
-    L.rb[1][0][cc] = go[2], L.rb[1][1][cc] = go[5], L.rb[1][2][cc] = go[6];
+    L.rbf[0][cc] = fo[2], L.rbf[1][cc] = fo[5], L.rbf[2][cc] = fo[6];
+    if (all_g) {
+#pragma unroll
+      for (int k = 0; k < 9; k++) L.rbg[k][cc] = go[k];
+    } else {
+      L.rbg[0][cc] = go[2], L.rbg[1][cc] = go[5], L.rbg[2][cc] = go[6];
+    }
   } else if (r == TH + 1) {
-    L.ra[0][0][cc] = fo[4], L.ra[0][1][cc] = fo[7], L.ra[0][2][cc] = fo[8];
-    L.ra[1][0][cc] = go[4], L.ra[1][1][cc] = go[7], L.ra[1][2][cc] = go[8];
+    L.raf[0][cc] = fo[4], L.raf[1][cc] = fo[7], L.raf[2][cc] = fo[8];
+    if (all_g) {
+#pragma unroll
+      for (int k = 0; k < 9; k++) L.rag[k][cc] = go[k];
+    } else {
+      L.rag[0][cc] = go[4], L.rag[1][cc] = go[7], L.rag[2][cc] = go[8];
+    }
   } else if (cc == 0) {
     L.cl[0][0][r - 1] = fo[1], L.cl[0][1][r - 1] = fo[5], L.cl[0][2][r - 1] = fo[8];
     L.cl[1][0][r - 1] = go[1], L.cl[1][1][r - 1] = go[5], L.cl[1][2][r - 1] = go[8];
@@ -251,11 +414,23 @@ __device__ __forceinline__ void keep_node(TileLds<R, TH> &L, int r, int cc, cons
   }
 }
 
+// The nudging of node o in step u of the launch (0: s, 1: s+1).
+__device__ __forceinline__ NodeNudge node_nudge(const TwoStepPtrs &a, int u, size_t o) {
+  NodeNudge d;
+  d.nv = a.nv[u][o];
+  d.bil = a.wq[u] != nullptr;
+  d.Tbar = d.nv > 0 ? a.tb[u][o] : 0.0;
+  d.W = d.nv > 0 && d.bil ? a.wq[u][o] : 0.0;
+  return d;
+}
+
 // Step s at local node (r, cc) of tile (i0, j0), pulled from the input planes in HBM; its
-// populations go to LDS; returns F(s+1) in Fn, and writes the velocity of a tile node.
-template <typename R, int TH>
+// populations go to LDS; returns F(s+1) in Fn and (nudged run) the nudging term after the
+// step in nud, and writes the velocity of a tile node.
+template <typename R, int TH, int V, int GR>
 __device__ __forceinline__ void first_step(const Geom &g, const TwoStepPtrs &a, const StepConst<R> &c, int i0,
-                                           int j0, int r, int cc, bool tile, TileLds<R, TH> &L, R &Fn) {
+                                           int j0, int r, int cc, bool tile, TileLds<R, TH, GR> &L, R &Fn, R &nud) {
+  static_assert(V != RBN_VARIANT_NUDGING || GR == 9, "the nudged run keeps all g directions of the ring rows");
   const int y = j0 - 1 + r;
   int x = i0 - 1 + cc;
   x = x < 0 ? x + g.nx : (x >= g.nx ? x - g.nx : x);
@@ -270,28 +445,37 @@ __device__ __forceinline__ void first_step(const Geom &g, const TwoStepPtrs &a, 
   acc.i = x;
   acc.im = x == 0 ? g.nx - 1 : x - 1;
   acc.ip = x == g.nx - 1 ? 0 : x + 1;
+  const size_t o = acc.r0 + x;
   R f[9], q[9], fo[9], go[9], rho, T, u, v;
-  gather_truth<R>(acc, bot, top, c, f, q);
-  const R Fy = ((const R *)a.Fin)[acc.r0 + x];
-  collide(c, f, q, Fy, fo, go, rho, T, u, v, Fn);
-  keep_node<R, TH>(L, r, cc, fo, go);
-  if (tile) st_uv<0, R>(a.uv1, acc.r0 + x, u, v);
+  const R Fy = ((const R *)a.Fin)[o];
+  if (V == RBN_VARIANT_NUDGING) {
+    gather_nudged<R>(acc, bot, top, y == 1, y == g.ny - 2, f, q);
+    const NodeNudge d = node_nudge(a, 0, o);
+    const R nud_in = d.nv > 0 ? ((const R *)a.nud_in)[o] : R(0);
+    collide_n(c, f, q, Fy, d, nud_in, fo, go, rho, T, u, v, Fn, nud);
+    if (tile && d.nv <= 0) nud = ((const R *)a.nud_in)[o];  // persists where no deposit lands
+  } else {
+    gather_truth<R>(acc, bot, top, c, f, q);
+    collide(c, f, q, Fy, fo, go, rho, T, u, v, Fn);
+  }
+  keep_node<R, TH, GR>(L, r, cc, fo, go);
+  if (tile) st_uv<0, R>(a.uv1, o, u, v);
 }
 
 // Both steps on tile (tx, ty) of kTW x TH nodes; every thread of the workgroup calls it.
 // Phase 1 computes step s on the tile and its ring: first the tile nodes, thread t taking
 // nodes t, t + kTileThreads, ... of the tile in row order (the same nodes it updates in phase 2,
-// so their next force stays in its registers), then the ring nodes, one per thread.  Phase 2
-// computes step s+1 on the tile from LDS and writes the output planes.
-template <typename R, int TH>
+// so their next force and nudging term stay in its registers), then the ring nodes, one per
+// thread.  Phase 2 computes step s+1 on the tile from LDS and writes the output planes.
+template <typename R, int TH, int V, int GR>
 __device__ __forceinline__ void two_step_tile(const Geom &g, const TwoStepPtrs &a, const StepConst<R> &c, int tx,
-                                              int ty, TileLds<R, TH> &L) {
+                                              int ty, TileLds<R, TH, GR> &L) {
   constexpr int NQ = kTW * TH / kTileThreads;  // tile nodes per thread
   constexpr int NRING = 2 * (kTW + 2) + 2 * TH;
   static_assert(kTW * TH % kTileThreads == 0 && NRING <= kTileThreads, "tile shape");
   const int i0 = tx * kTW, j0 = ty * TH;
   const int ncols = min(kTW, g.nx - i0), nrows = min(TH, g.ny - j0);
-  R Fk[NQ];
+  R Fk[NQ], Nk[NQ];
 
   // Phase 1: step s.  Tile rows / columns up to nrows+1 / ncols+1 (the ring of a cut-short
   // tile lies there).
@@ -300,8 +484,9 @@ __device__ __forceinline__ void two_step_tile(const Geom &g, const TwoStepPtrs &
     const int n = threadIdx.x + q * kTileThreads;
     const int r = n / kTW + 1, cc = n % kTW + 1;
     Fk[q] = R(0);
+    Nk[q] = R(0);
     if (r > nrows + 1 || cc > ncols + 1 || j0 - 1 + r >= g.ny) continue;
-    first_step<R, TH>(g, a, c, i0, j0, r, cc, r <= nrows && cc <= ncols, L, Fk[q]);
+    first_step<R, TH, V, GR>(g, a, c, i0, j0, r, cc, r <= nrows && cc <= ncols, L, Fk[q], Nk[q]);
   }
   if (threadIdx.x < NRING) {
     const int t = threadIdx.x;
@@ -317,8 +502,8 @@ __device__ __forceinline__ void two_step_tile(const Geom &g, const TwoStepPtrs &
     }
     const int y = j0 - 1 + r;
     if (r <= nrows + 1 && cc <= ncols + 1 && y >= 0 && y < g.ny) {
-      R Fn;
-      first_step<R, TH>(g, a, c, i0, j0, r, cc, false, L, Fn);
+      R Fn, nud;
+      first_step<R, TH, V, GR>(g, a, c, i0, j0, r, cc, false, L, Fn, nud);
     }
   }
   __syncthreads();
@@ -331,12 +516,19 @@ __device__ __forceinline__ void two_step_tile(const Geom &g, const TwoStepPtrs &
     if (cc > ncols || r > nrows) continue;
     const int x = i0 - 1 + cc, y = j0 - 1 + r;
     const bool bot = (y == 0), top = (y == g.ny - 1);
-    LdsAcc<R, TH> acc{L, r, cc};
-    R f[9], q9[9], fo[9], go[9], rho, T, u, v, Fn;
-    gather_truth<R>(acc, bot, top, c, f, q9);
-    collide(c, f, q9, Fk[q], fo, go, rho, T, u, v, Fn);
-    flag_nonfinite(a.bad, rho, T);
     const size_t o = (size_t)y * g.pitch + x;
+    LdsAcc<R, TH, GR> acc{L, r, cc};
+    R f[9], q9[9], fo[9], go[9], rho, T, u, v, Fn;
+    if (V == RBN_VARIANT_NUDGING) {
+      gather_nudged<R>(acc, bot, top, y == 1, y == g.ny - 2, f, q9);
+      R nud;
+      collide_n(c, f, q9, Fk[q], node_nudge(a, 1, o), Nk[q], fo, go, rho, T, u, v, Fn, nud);
+      ((R *)a.nud_out)[o] = nud;
+    } else {
+      gather_truth<R>(acc, bot, top, c, f, q9);
+      collide(c, f, q9, Fk[q], fo, go, rho, T, u, v, Fn);
+    }
+    flag_nonfinite(a.bad, rho, T);
     R *fB = (R *)a.fB, *gB = (R *)a.gB;
 #pragma unroll
     for (int k = 0; k < 9; k++) st<1>(&fB[(size_t)k * g.plane + o], fo[k]);

@@ -246,3 +246,57 @@ def test_observation_table_errors():
     with pytest.raises(rbn.RbnError) as e:
         h.step(1)
     assert e.value.code == -10  # RBN_ECALLBACK
+
+
+@pytest.mark.parametrize("prec", [rbn.RBN_FP64, rbn.RBN_FP32])
+@pytest.mark.parametrize("deposit", [rbn.RBN_DEPOSIT_BOX7, rbn.RBN_DEPOSIT_BILINEAR])
+def test_two_step_nudged_launches_equal_one_step_launches(prec, deposit):
+    # 70 x 25: tiles cut short in x, and a top tile of one row (25 = 3 x 8 + 1 = 6 x 4 + 1)
+    # whose wall row copies its inner neighbour row from the ring row below; observers on
+    # and next to both walls.  Calls of odd and even lengths: pairs, single steps at the end
+    # of a call, the deposit the getters see after either.  FP64 also against the oracle.
+    from oracle.oracle import Oracle
+
+    p = rb_params(70, 25, 2e5, noutconfig=4)
+    p.variant = rbn.RBN_VARIANT_NUDGING
+    p.ntempread = 4
+    p.alpha = 0.3
+    p.deposit = deposit
+    p.precision = prec
+    ids, x0, y0, T, recs = synthetic_observations(p.nx, p.ny, 60, 200, 4, seed=33)
+    T0 = conduction_T(p, 1e-2, seed=34)
+    sims = []
+    for steps in (1, 2):
+        s = rbn.Solver(p)
+        s.set_tuning(lbm_steps=steps)
+        s.init_fields(T=T0)
+        s.init_observers(ids, x0, y0, T)
+        s.set_observation_table(recs)
+        sims.append(s)
+    o = None
+    if prec == rbn.RBN_FP64:
+        o = Oracle(p)
+        o.init_fields(T=T0)
+        o.init_observers(ids, x0, y0, T)
+        o.set_observation_source(recs)
+    for n in (1, 2, 7, 12, 5, 3, 30):
+        for s in sims:
+            s.step(n)
+        refs = sims[:1]
+        if o is not None:
+            o.step(n)
+            refs.append(o)
+        a = sims[1]
+        for b in refs:
+            fa, ga = a.populations()
+            fb, gb = b.populations()
+            assert_equal_arrays(fa, fb, f"after a call of {n}: f")
+            assert_equal_arrays(ga, gb, f"after a call of {n}: g")
+            x, y = a.fields(), b.fields()
+            for k in ("rho", "T", "u", "v"):
+                assert_equal_arrays(x[k], y[k], f"after a call of {n}: {k}")
+            x, y = a.nudging(), b.nudging()
+            for k in ("nudge_vol", "tempreadint", "nud_term", "weight"):
+                assert_equal_arrays(x[k], y[k], f"after a call of {n}: {k}")
+    for s in sims:
+        s.close()
