@@ -206,10 +206,15 @@ ENS_TRIAL_STEPS = 8
 def ens_autotune_steps(trial, ntempread):
     """Steps from its starting state over which rbn_ensemble_autotune advances nudged members
     whose records come every `ntempread` steps (include/rbn.h): each candidate's warm-up call
-    of 6 steps and timed call of 6 * max(1, ceil(trial / 6)) steps, and the comparison of the
-    two forms over 2 L + 2 steps, L = lcm(6, ntempread); the state is restored after each."""
-    lcm = 6 * ntempread // int(np.gcd(6, ntempread))
-    return max(6 + 6 * max(1, -(-trial // 6)), 2 * lcm + 2)
+    of G steps and timed call of G * max(1, ceil(trial / G)) steps (G = 6 for one launch per
+    step, 12 for two steps per launch), and the comparison of the two forms over 2 L + 2
+    steps, L = lcm(6, ntempread), or 2 L + 3 steps, L = lcm(12, ntempread); the state is
+    restored after each."""
+    spans = []
+    for g, t0 in ((6, 1), (12, 2)):
+        lcm = g * ntempread // int(np.gcd(g, ntempread))
+        spans += [g + g * max(1, -(-trial // g)), 2 * lcm + t0 + 1]
+    return max(spans)
 
 
 def ens_record_horizon(trial, warmup, ens_warm, steps, ntempread):
@@ -225,7 +230,10 @@ def run_ensemble(rbn, steps, warmup, tuning=None, nudged=True):
     """The batched ensemble (autotuned, or the given launch shape (tile_w, tile_h, wg_per_cu,
     order)) against the same members, built afresh, stepped one after another with their
     own launches (rbn_step)."""
-    ens_warm = warmup - warmup % 12 + 12
+    # a whole number of the captured cycles (L = lcm(12, ntempread) steps covers both launch
+    # modes), so that the timed call replays graphs kept from the warm-up call
+    cyc = 12 * ENS_NTEMPREAD // int(np.gcd(12, ENS_NTEMPREAD))
+    ens_warm = (warmup // cyc + 1) * cyc
     last = ens_record_horizon(ENS_TRIAL_STEPS, warmup, ens_warm, steps, ENS_NTEMPREAD)
     members, nodes, nobs = ens_members(rbn, nudged, last)
     for s in members:
@@ -260,7 +268,8 @@ def run_ensemble(rbn, steps, warmup, tuning=None, nudged=True):
            "roofline_frac": round(nodes * lbm_bytes_per_node(1) / (t_ens / steps) / 1e9 / HBM_PEAK_GBPS, 4),
            "tuning": e.tuning(), "autotune_best_ms": finite_or_null(round(best_trial, 4))}
     order = out["tuning"]["order"]
-    out["two_steps_per_launch"] = bool(order & 8) and not order & 4 and not nudged
+    out["two_steps_per_launch"] = bool(order & 8) and not order & 4
+    out["graph_replays"] = not order & 16
     e.close()
     for s in members:
         s.close()

@@ -1297,7 +1297,7 @@ int step_loop(rbn_ctx *c, int nsteps) {
              }
              return rbn::launch_lbm2(c->stream, c->prec, c->geom, lp, c->F2, c->uv[s2], moments2 ? c->rho : nullptr,
                                      moments2 ? c->T : nullptr, c->consts.data(), t.tile_h, t.seg, t.xcd2, t.variant,
-                                     t.reverse && ((istep >> 1) & 1), carry ? &job : nullptr,
+                                     t.reverse && ((istep >> 1) & 1), t.nt, carry ? &job : nullptr,
                                      nudged ? &np : nullptr);
            }
            t.flip = t.reverse && (istep & 1);

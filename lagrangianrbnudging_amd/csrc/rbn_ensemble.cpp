@@ -30,7 +30,8 @@ struct rbn_ensemble {
   std::vector<rbn::DepPtrs> spare;
   hipEvent_t ev_dep[2] = {nullptr, nullptr}, ev_lat[2] = {nullptr, nullptr}, ev_seq = nullptr;
   // autotuner: when seg_a is set, a nudged call records seg_a / seg_b on stream around its
-  // graph replays, or around its eager steps 1 .. 2L without replays (seg_steps steps)
+  // graph replays, or around its eager steps cyc_t0 .. cyc_t0 + 2L - 1 without replays
+  // (seg_steps steps)
   hipEvent_t seg_a = nullptr, seg_b = nullptr;
   int seg_steps = 0;
   void *d_nmembers = nullptr;
@@ -45,11 +46,12 @@ struct rbn_ensemble {
   } stage[2];
   int stage_next = 0;
   std::vector<int> loading, rid;
-  // and their captured cycle: ncycle steps (the least common multiple of the velocity ring
-  // and every record interval; 0: no graph), the record uploads inside it (call step t of
-  // the cycle, the members loading, their staging region), two staging sets and two graph
+  // and their captured cycle: ncycle steps from call step cyc_t0 (the least common multiple of
+  // the velocity ring and every record interval, and of 4 in the two-step mode; 0: no graph),
+  // the record scatters inside it (call step t of the cycle, the members loading, their
+  // staging region), two staging sets (uploaded whole before a replay) and two graph
   // instances alternating between replays
-  int ncycle = 0;
+  int ncycle = 0, cyc_t0 = 1;
   struct CycleLoad {
     int t;
     std::vector<int> ld;
@@ -190,11 +192,13 @@ void fill_member(const rbn_ctx *c, const rbn::DepPtrs *spare, int tile_w, int ti
       d.Tbar[p] = s.Tbar;
       d.wq[p] = bil ? s.wq : nullptr;
     }
-    d.nud = (R *)c->nud;
+    d.nud[0] = (R *)c->nud;
+    d.nud[1] = (R *)c->nud2;
   }
 }
 
 size_t loading_members(const rbn_ensemble *e, int t, std::vector<int> &ld);
+bool pairs(const rbn_ensemble *e);
 
 // Observer/deposit descriptors of the nudged members as of their current step, the block
 // prefixes of the batched passes, and the record uploads of the captured cycle.
@@ -225,10 +229,11 @@ int build_nudged(rbn_ensemble *e) {
   EHIP(e, hipMemcpy(e->d_nmembers, d.data(), d.size() * sizeof(rbn::NudgeMember), hipMemcpyHostToDevice));
   EHIP(e, hipMemcpy(e->d_npre, pre.data(), pre.size() * sizeof(int), hipMemcpyHostToDevice));
   // A cycle of L steps replays the same kernel arguments when L is a multiple of the
-  // velocity ring and of every record interval; longer cycles than kMaxNudgedCycle run
-  // eagerly.
+  // velocity ring and of every record interval, in the two-step mode also of 4 (an even
+  // number of launches, each flipping the A/B planes, an even number of them flipping the F
+  // planes); longer cycles than kMaxNudgedCycle run eagerly.
   constexpr int kMaxNudgedCycle = 120;
-  long long L = rbn::kCycle;
+  long long L = pairs(e) ? 2 * rbn::kCycle : rbn::kCycle;
   for (int q : e->nudged) {
     const long long a = e->m[(size_t)q]->P.ntempread;
     long long x = L, y = a;
@@ -241,9 +246,10 @@ int build_nudged(rbn_ensemble *e) {
     if (L > kMaxNudgedCycle) break;
   }
   e->ncycle = L <= kMaxNudgedCycle ? (int)L : 0;
+  e->cyc_t0 = pairs(e) ? 2 : 1;
   e->cyc.clear();
   e->cyc_bytes = 0;
-  for (int t = 1; t <= e->ncycle; t++) {
+  for (int t = e->cyc_t0; t < e->cyc_t0 + e->ncycle; t++) {
     rbn_ensemble::CycleLoad cl;
     cl.t = t;
     cl.bytes = loading_members(e, t, cl.ld);
@@ -324,8 +330,9 @@ int grid_size(const rbn_ensemble *e) { return e->wg_per_cu > 0 ? e->wg_per_cu * 
 bool separate(const rbn_ensemble *e) { return (e->order & 4) != 0 && e->nudged.empty(); }
 
 // order bit 3 (without bit 2): two steps per launch on two-step tiles, the previous launch's
-// particle updates carried.  Not with nudged members (a nudged step has no two-step form).
-bool pairs(const rbn_ensemble *e) { return (e->order & 8) != 0 && !separate(e) && e->nudged.empty(); }
+// particle updates carried; with nudged members, both steps' deposits are built before the
+// launch.
+bool pairs(const rbn_ensemble *e) { return (e->order & 8) != 0 && !separate(e); }
 
 // order bit 4: no graphs, every launch eager (for debugging and tracing tools).
 bool graphs(const rbn_ensemble *e) { return (e->order & 16) == 0; }
@@ -340,9 +347,14 @@ struct PairState {
   int t = 0, L = 0, P = 0, pending = 0;
 };
 
+// 0: truth runs only, 1: nudged runs only, 2: both (the two-step kernels compile each apart).
+int composition(const rbn_ensemble *e) {
+  return e->nudged.empty() ? 0 : (e->nudged.size() == e->m.size() ? 1 : 2);
+}
+
 int launch_pairs(rbn_ensemble *e, const rbn::EnsStep &st) {
   EHIP(e, rbn::launch_ensemble_pairs(e->stream, e->prec, e->d_members, e->d_items, e->n_lbm, e->n_part, e->n_lbm2,
-                                     e->d_counters, st, e->tile_w, e->tile_h, grid_size(e)));
+                                     e->d_counters, st, e->tile_w, e->tile_h, grid_size(e), composition(e)));
   return RBN_OK;
 }
 
@@ -444,7 +456,10 @@ void advance_members(rbn_ensemble *e, int n, int launches, int pairs) {
       std::swap(c->fA, c->fB);
       std::swap(c->gA, c->gB);
     }
-    if (pairs & 1) std::swap(c->F, c->F2);
+    if (pairs & 1) {
+      std::swap(c->F, c->F2);
+      std::swap(c->nud, c->nud2);
+    }
     c->cur = (c->cur + n) % rbn_ctx::kVel;
     c->moments_valid = true;
   }
@@ -544,8 +559,8 @@ int enqueue_nudged_lattice(rbn_ensemble *e, int t, bool last) {
   return RBN_OK;
 }
 
-// The records of replay r of the captured cycle (call steps r L + 1 .. r L + L) into staging
-// set k.
+// The records of replay r of the captured cycle (call steps t0 + r L .. t0 + r L + L - 1)
+// into staging set k.
 int fill_cycle(rbn_ensemble *e, int k, int r) {
   for (const auto &cl : e->cyc) {
     int rc = fetch_records(e, cl.ld, cl.t + r * e->ncycle, e->cstage[k].h + cl.off);
@@ -554,13 +569,27 @@ int fill_cycle(rbn_ensemble *e, int k, int r) {
   return RBN_OK;
 }
 
-int enqueue_fused(rbn_ensemble *e, int t, bool last);
+// Two-step mode with nudged members: the records and deposits of a launch's steps (step t
+// into set t & 1, t + 1 into set (t + 1) & 1) run on stream before it, after the previous
+// launch, which read both sets.
+int enqueue_nudged_pair(rbn_ensemble *e, PairState &ps, int nsteps, bool moments) {
+  const int nm = (int)e->nudged.size();
+  for (int t = ps.t; t < std::min(ps.t + 2, nsteps); t++) {
+    int rc = load_records(e, t, e->stream);
+    if (rc) return rc;
+    EHIP(e, rbn::launch_nudge_batch(e->stream, (const rbn::NudgeMember *)e->d_nmembers + (size_t)(t & 1) * nm,
+                                    e->d_npre, e->ntot, nm, t));
+  }
+  return enqueue_pair(e, ps, nsteps, moments);
+}
 
-// Capture the cycle of nudged steps 1 .. L of a call twice, reading the records from
-// staging set 0 and set 1: per step the record upload (a copy and a scatter, where members
-// start an interval), the batched observer/deposit passes and the lattice launch, all on
-// stream (graph replays of the two-stream form of eager steps measured slower than those
-// eager steps: 1.01 against 0.67 ms per step for 64 members).
+// Capture the cycle of nudged steps t0 .. t0 + L - 1 of a call twice, the scatters reading
+// the records from staging set 0 and set 1 (each uploaded whole, outside the graph, before
+// its replay): per step the record scatter (where members start an interval) and the batched
+// observer/deposit passes, and the lattice launches — one per step, or in the two-step mode
+// one per two steps after both steps' deposits — all on stream (graph replays of the
+// two-stream form of eager steps measured slower than those eager steps: 1.01 against
+// 0.67 ms per step for 64 members).
 int capture_nudged(rbn_ensemble *e) {
   for (int k = 0; k < 2; k++) {
     int rc = ensure_stage(e, e->cstage[k], std::max<size_t>(e->cyc_bytes, 256));
@@ -572,18 +601,31 @@ int capture_nudged(rbn_ensemble *e) {
     EHIP(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
     int rc = RBN_OK;
     size_t next = 0;
-    for (int t = 1; t <= e->ncycle && !rc; t++) {
+    auto deposit = [&](int t) -> int {
       if (next < e->cyc.size() && e->cyc[next].t == t) {
         const auto &cl = e->cyc[next++];
-        if (hipMemcpyAsync(st.d + cl.off, st.h + cl.off, cl.bytes, hipMemcpyHostToDevice, e->stream) != hipSuccess ||
-            rbn::launch_obs_scatter(e->stream, (const rbn::ObsJob *)(st.d + cl.off), (int)cl.ld.size(),
+        if (rbn::launch_obs_scatter(e->stream, (const rbn::ObsJob *)(st.d + cl.off), (int)cl.ld.size(),
                                     st.d + cl.off) != hipSuccess)
-          rc = efail(e, RBN_EDEVICE, "record upload capture failed");
+          return efail(e, RBN_EDEVICE, "record scatter capture failed");
       }
-      if (!rc && rbn::launch_nudge_batch(e->stream, (const rbn::NudgeMember *)e->d_nmembers + (size_t)(t & 1) * nm,
-                                         e->d_npre, e->ntot, nm, t) != hipSuccess)
-        rc = efail(e, RBN_EDEVICE, "observer/deposit capture failed");
-      if (!rc) rc = enqueue_fused(e, t, false);
+      if (rbn::launch_nudge_batch(e->stream, (const rbn::NudgeMember *)e->d_nmembers + (size_t)(t & 1) * nm,
+                                  e->d_npre, e->ntot, nm, t) != hipSuccess)
+        return efail(e, RBN_EDEVICE, "observer/deposit capture failed");
+      return RBN_OK;
+    };
+    if (pairs(e)) {
+      // launches 1 .. L / 2 of a call (steps 2 .. L + 1), two steps each, carrying two updates
+      PairState ps;
+      ps.t = 2;
+      ps.L = 1;
+      ps.P = 1;
+      ps.pending = 2;
+      for (int l = 0; l < e->ncycle / 2 && !rc; l++) {
+        if (!(rc = deposit(ps.t)) && !(rc = deposit(ps.t + 1))) rc = enqueue_pair(e, ps, 1 << 30, false);
+      }
+    } else {
+      for (int t = 1; t <= e->ncycle && !rc; t++)
+        if (!(rc = deposit(t))) rc = enqueue_fused(e, t, false);
     }
     hipGraph_t g = nullptr;
     hipError_t ee = hipStreamEndCapture(e->stream, &g);
@@ -606,6 +648,116 @@ int sync_members(rbn_ensemble *e) {
   return RBN_OK;
 }
 
+// Steps of an ensemble with nudged members.  A step is: the records of the members whose
+// record interval starts (host providers, one upload), the batched observer update and
+// deposit, then the lattice.  One launch per step: the lattice launch of step t carries the
+// particle updates of step t - 1, and the first three run on pstream beside the previous
+// step's lattice launch (enqueue_deposit).  Two-step mode: everything on stream, both
+// steps' deposits before their launch (enqueue_nudged_pair).  Either way the first launch
+// runs eagerly, whole cycles of L steps from the graphs (their records fetched on the host
+// before each replay, into the set the previous replay but one has read), the rest eagerly
+// (the last writing the moments), then the last launch's particle updates on their own.
+int run_nudged(rbn_ensemble *e, int nsteps) {
+  int rc;
+  const bool two = pairs(e);
+  auto eager_deposit = [&](int t, int t0) {
+    int r = load_records(e, t, e->pstream);
+    if (r) return r;
+    return enqueue_deposit(e, t, t0);
+  };
+  PairState ps;
+  if (two) {
+    if ((rc = enqueue_nudged_pair(e, ps, nsteps, nsteps <= 2))) return rc;
+  } else {
+    if ((rc = eager_deposit(0, 0)) || (rc = enqueue_nudged_lattice(e, 0, nsteps == 1))) return rc;
+    ps.t = 1;
+  }
+  const int L = e->ncycle, t0 = ps.t;
+  const int cycles = graphs(e) && L > 0 && nsteps > t0 ? (nsteps - t0 - 1) / L : 0;
+  if (cycles > 0) {
+    if (!e->nexec[0] && (rc = capture_nudged(e))) return rc;
+    if (e->seg_a) {
+      EHIP(e, hipEventRecord(e->seg_a, e->stream));
+      e->seg_steps = cycles * L;
+    }
+    for (int r = 0; r < cycles; r++) {
+      rbn_ensemble::Stage &st = e->cstage[r & 1];
+      EHIP(e, hipEventSynchronize(st.done));
+      if ((rc = fill_cycle(e, r & 1, r))) return rc;
+      if (e->cyc_bytes) EHIP(e, hipMemcpyAsync(st.d, st.h, e->cyc_bytes, hipMemcpyHostToDevice, e->stream));
+      EHIP(e, hipEventRecord(st.done, e->stream));
+      EHIP(e, hipGraphLaunch(e->nexec[r & 1], e->stream));
+    }
+    if (e->seg_a) EHIP(e, hipEventRecord(e->seg_b, e->stream));
+    ps.t += cycles * L;
+    if (two) {
+      ps.L += cycles * L / 2;
+      ps.P += cycles * L / 2;
+    } else {
+      // the eager steps after the replays: their deposits follow the replays' work, and the
+      // lattice events they wait for are recorded again outside the captured graphs
+      EHIP(e, hipEventRecord(e->ev_lat[0], e->stream));
+      EHIP(e, hipEventRecord(e->ev_lat[1], e->stream));
+      EHIP(e, hipEventRecord(e->ev_seq, e->stream));
+      EHIP(e, hipStreamWaitEvent(e->pstream, e->ev_seq, 0));
+    }
+  }
+  // the autotuner's eager form: steps t0 .. t0 + 2L - 1, when there were no replays
+  const int seg_len = 2 * L;
+  const bool seg = e->seg_a && L > 0 && cycles == 0 && nsteps > t0 + seg_len;
+  if (seg) {
+    EHIP(e, hipEventRecord(e->seg_a, e->stream));
+    e->seg_steps = seg_len;
+  }
+  const int tail = ps.t;
+  if (two) {
+    while (ps.t < nsteps) {
+      if ((rc = enqueue_nudged_pair(e, ps, nsteps, ps.t + 2 >= nsteps))) return rc;
+      if (seg && ps.t == t0 + seg_len) EHIP(e, hipEventRecord(e->seg_b, e->stream));
+    }
+    if (e->n_part > 0) {
+      rbn::EnsStep st{};
+      st.k = ps.t % rbn::kCycle;
+      st.ncarry = ps.pending;
+      st.lat = 0;
+      st.cset = ps.L & 1;
+      if ((rc = launch_pairs(e, st))) return rc;
+    }
+  } else {
+    for (int t = tail; t < nsteps; t++) {
+      if ((rc = eager_deposit(t, t == tail ? tail : 0))) return rc;
+      if ((rc = enqueue_nudged_lattice(e, t, t == nsteps - 1))) return rc;
+      if (seg && t + 1 == t0 + seg_len) EHIP(e, hipEventRecord(e->seg_b, e->stream));
+    }
+    if (e->n_part > 0 && (rc = launch(e, e->stream, 1, (nsteps - 1) % rbn::kCycle, false))) return rc;
+    ps.L = ps.t = nsteps;
+  }
+  // The last step's deposit is in set (nsteps - 1) & 1: set 0 (the second set) after an
+  // odd number of steps, copied into the members' own arrays for their readers.
+  if ((nsteps & 1) == 1) {
+    for (int q : e->nudged) {
+      rbn_ctx *c = e->m[(size_t)q];
+      const rbn::DepPtrs &s = e->spare[(size_t)q];
+      const size_t plane = c->geom.plane;
+      EHIP(e, hipMemcpyAsync(c->dep.cnt, s.cnt, plane * sizeof(int), hipMemcpyDeviceToDevice, e->stream));
+      EHIP(e, hipMemcpyAsync(c->dep.Tbar, s.Tbar, plane * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
+      if (c->dep.mode == RBN_DEPOSIT_BILINEAR)
+        EHIP(e, hipMemcpyAsync(c->dep.wq, s.wq, plane * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
+    }
+  }
+  EHIP(e, hipStreamSynchronize(e->stream));
+  EHIP(e, hipStreamSynchronize(e->pstream));
+  advance_members(e, nsteps, ps.L, ps.P);
+  // The observer descriptors and the cycle's record plan hold the members' step counts
+  // modulo the cycle (and, in the two-step mode, an even number of launches and of two-step
+  // launches): they (and the graphs) stay valid after a whole number of cycles.
+  if (L == 0 || nsteps % L) {
+    e->built = false;
+    drop_graph(e);
+  }
+  return RBN_OK;
+}
+
 int run_steps(rbn_ensemble *e, int nsteps) {
   int rc = sync_members(e);
   if (rc) return rc;
@@ -617,80 +769,7 @@ int run_steps(rbn_ensemble *e, int nsteps) {
   EHIP(e, hipMemsetAsync(e->d_counters, 0, 4 * sizeof(int), e->stream));
   for (int q : e->nudged)
     if (!e->m[(size_t)q]->d_obs_i) return efail(e, RBN_ESTATE, "member %d: nudged run without observers", q);
-  if (!e->nudged.empty()) {
-    // With nudged members a step is: the records of the members whose record interval
-    // starts (host providers, one upload), the batched observer update and deposit, then the
-    // lattice launch carrying the previous step's particle updates; the first three run on
-    // pstream beside the previous step's lattice launch (enqueue_deposit).  Step 0 runs
-    // eagerly, whole cycles of steps 1 .. L from the graphs (their records fetched on the
-    // host before each replay, into the set the previous replay but one has read), the rest
-    // eagerly (the last writing the moments), then the last step's particle updates on their
-    // own.
-    auto eager_deposit = [&](int t, int t0) {
-      int r = load_records(e, t, e->pstream);
-      if (r) return r;
-      return enqueue_deposit(e, t, t0);
-    };
-    const int L = e->ncycle;
-    const int cycles = graphs(e) && L > 0 && nsteps >= 2 ? (nsteps - 2) / L : 0;
-    const int tail = 1 + cycles * L;
-    if ((rc = eager_deposit(0, 0)) || (rc = enqueue_nudged_lattice(e, 0, nsteps == 1))) return rc;
-    if (cycles > 0) {
-      if (!e->nexec[0] && (rc = capture_nudged(e))) return rc;
-      if (e->seg_a) {
-        EHIP(e, hipEventRecord(e->seg_a, e->stream));
-        e->seg_steps = cycles * L;
-      }
-      for (int r = 0; r < cycles; r++) {
-        rbn_ensemble::Stage &st = e->cstage[r & 1];
-        EHIP(e, hipEventSynchronize(st.done));
-        if ((rc = fill_cycle(e, r & 1, r))) return rc;
-        EHIP(e, hipGraphLaunch(e->nexec[r & 1], e->stream));
-        EHIP(e, hipEventRecord(st.done, e->stream));
-      }
-      if (e->seg_a) EHIP(e, hipEventRecord(e->seg_b, e->stream));
-      // the eager steps after the replays: their deposits follow the replays' work, and the
-      // lattice events they wait for are recorded again outside the captured graphs
-      EHIP(e, hipEventRecord(e->ev_lat[0], e->stream));
-      EHIP(e, hipEventRecord(e->ev_lat[1], e->stream));
-      EHIP(e, hipEventRecord(e->ev_seq, e->stream));
-      EHIP(e, hipStreamWaitEvent(e->pstream, e->ev_seq, 0));
-    }
-    const int seg_end = e->seg_a && cycles == 0 ? std::min(nsteps - 1, 2 * std::max(L, 1)) : 0;
-    if (seg_end > 0) {
-      EHIP(e, hipEventRecord(e->seg_a, e->stream));
-      e->seg_steps = seg_end;
-    }
-    for (int t = tail; t < nsteps; t++) {
-      if ((rc = eager_deposit(t, t == tail ? tail : 0))) return rc;
-      if ((rc = enqueue_nudged_lattice(e, t, t == nsteps - 1))) return rc;
-      if (t == seg_end) EHIP(e, hipEventRecord(e->seg_b, e->stream));
-    }
-    if (e->n_part > 0 && (rc = launch(e, e->stream, 1, (nsteps - 1) % rbn::kCycle, false))) return rc;
-    // The last step's deposit is in set (nsteps - 1) & 1: set 0 (the second set) after an
-    // odd number of steps, copied into the members' own arrays for their readers.
-    if ((nsteps & 1) == 1) {
-      for (int q : e->nudged) {
-        rbn_ctx *c = e->m[(size_t)q];
-        const rbn::DepPtrs &s = e->spare[(size_t)q];
-        const size_t plane = c->geom.plane;
-        EHIP(e, hipMemcpyAsync(c->dep.cnt, s.cnt, plane * sizeof(int), hipMemcpyDeviceToDevice, e->stream));
-        EHIP(e, hipMemcpyAsync(c->dep.Tbar, s.Tbar, plane * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
-        if (c->dep.mode == RBN_DEPOSIT_BILINEAR)
-          EHIP(e, hipMemcpyAsync(c->dep.wq, s.wq, plane * sizeof(double), hipMemcpyDeviceToDevice, e->stream));
-      }
-    }
-    EHIP(e, hipStreamSynchronize(e->stream));
-    EHIP(e, hipStreamSynchronize(e->pstream));
-    advance_members(e, nsteps, nsteps, 0);
-    // The observer descriptors and the cycle's record plan hold the members' step counts
-    // modulo the cycle: they (and the graphs) stay valid after a whole number of cycles.
-    if (L == 0 || nsteps % L) {
-      e->built = false;
-      drop_graph(e);
-    }
-    return RBN_OK;
-  }
+  if (!e->nudged.empty()) return run_nudged(e, nsteps);
   int launches = nsteps, npairs = 0;
   if (pairs(e)) {
     // launch 0 eagerly, whole cycles of kCycle two-step launches from the graph, the rest
@@ -761,7 +840,7 @@ struct Snapshot {
   std::vector<std::pair<void *, size_t>> src;
   std::vector<int> istep, cur;
   std::vector<bool> moments_valid;
-  std::vector<void *> fA, gA, F;
+  std::vector<void *> fA, gA, F, nud;
 };
 
 void free_snapshot(Snapshot &s) {
@@ -808,6 +887,7 @@ int take_snapshot(rbn_ensemble *e, Snapshot &s) {
     s.fA.push_back(c->fA);
     s.gA.push_back(c->gA);
     s.F.push_back(c->F);
+    s.nud.push_back(c->nud);
   }
   return RBN_OK;
 }
@@ -825,6 +905,7 @@ int restore_snapshot(rbn_ensemble *e, Snapshot &s, bool keep = false) {
       std::swap(c->gA, c->gB);
     }
     if (c->F != s.F[i]) std::swap(c->F, c->F2);
+    if (c->nud != s.nud[i]) std::swap(c->nud, c->nud2);
     // the observers' record map on the device goes back too: forget which record it is for
     c->prev_rid.clear();
     c->istep = s.istep[i];
@@ -960,15 +1041,14 @@ int rbn_ensemble_autotune(rbn_ensemble *e, int trial_steps, double *best_ms_per_
   std::vector<Cand> cands;
   const int shapes[][2] = {{256, 1}, {256, 2}, {256, 4}, {128, 2}, {128, 4}, {64, 4}, {64, 8}};
   const int wpcs[] = {0, 2, 4, 8};
-  // with nudged members only the one-launch-per-step orders apply
+  // with nudged members the separate-launch orders (bit 2) do not apply
   const int norders = e->nudged.empty() ? 8 : 4;
   for (auto &sh : shapes)
     for (int w : wpcs)
       for (int o = 0; o < norders; o++) cands.push_back({sh[0], sh[1], w, o});
   // two steps per launch: the tile is fixed (64 x 8 in FP32, 64 x 4 in FP64)
-  if (e->nudged.empty())
-    for (int w : wpcs)
-      for (int o = 8; o < 12; o++) cands.push_back({256, 1, w, o});
+  for (int w : wpcs)
+    for (int o = 8; o < 12; o++) cands.push_back({256, 1, w, o});
   const int tile_w0 = e->tile_w, tile_h0 = e->tile_h, wpc0 = e->wg_per_cu, order0 = e->order;
   int rc = sync_members(e);
   if (rc) return rc;
@@ -1014,14 +1094,15 @@ int rbn_ensemble_autotune(rbn_ensemble *e, int trial_steps, double *best_ms_per_
     *ms_per_step = ms / trial;
     return RBN_OK;
   };
-  // Nudged members: eager steps overlap each step's deposit with the previous lattice launch,
-  // captured cycles replay the steps in order.  Time both with the best launch shape over
-  // steps 1 .. 2L of a call of 2L + 2 steps (the graph form: two replays).
+  // Nudged members: eager steps overlap each step's deposit with the previous lattice launch
+  // (one launch per step), captured cycles replay the steps in order.  Time both with the
+  // best launch shape over steps t0 .. t0 + 2L - 1 of a call of 2L + t0 + 1 steps (t0: the
+  // cycle's first step; the graph form: two replays).
   auto forms = [&]() -> int {
     int r = rbn_ensemble_set_tuning(e, bestc.tw, bestc.th, bestc.wpc, bestc.order & ~16);
     if (r || (!e->built && (r = build(e)))) return r;
     if (e->ncycle <= 0) return RBN_OK;
-    const int T = 2 * e->ncycle + 2;
+    const int T = 2 * e->ncycle + e->cyc_t0 + 1;
     double ms_form[2] = {1e300, 1e300};
     e->seg_a = a;
     e->seg_b = b;

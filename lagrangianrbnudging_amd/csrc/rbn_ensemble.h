@@ -33,12 +33,14 @@ struct EnsMember {
   int *bad;  // non-finite moments flag (the member's d_err[1])
   // nudged members (variant RBN_VARIANT_NUDGING): the deposits the steps read, [k & 1] for
   // step k (1: the member's own arrays, 0: the ensemble's second set, so that the deposit of
-  // step k + 1 can be built while step k runs), and the persistent nudging term; wq only for
-  // the bilinear deposit
+  // step k + 1 can be built while step k runs, or both deposits of a two-step launch exist),
+  // and the persistent nudging term, read and written in place by one-step launches and
+  // swapped between nud[0] and nud[1] by two-step launches (with the force planes); wq only
+  // for the bilinear deposit
   int variant;
   const int *nvol[2];
   const double *Tbar[2], *wq[2];
-  R *nud;
+  R *nud[2];
 };
 
 // items: the n_lbm lattice items, then the n_part particle items.  phase 0: the lattice items
@@ -51,12 +53,14 @@ hipError_t launch_ensemble_phase(hipStream_t s, int prec, int phase, const void 
 // One launch of the two-step mode: the particle updates of steps k-ncarry .. k-1 (items
 // n_lbm .. n_lbm+n_part), then the lattice of step k as one-step tiles (lat 1, items 0 ..
 // n_lbm) or of steps k, k+1 as two-step tiles (lat 2, items n_lbm+n_part ..).  lpar: which
-// of pf/pg is the input; fpar: which of F is the input force; cset: the counter set.
+// of pf/pg is the input; fpar: which of F (and of a nudged member's nud) is the input;
+// cset: the counter set.  nudged (of the launch): 0 the ensemble holds truth runs only, 1
+// nudged runs only, 2 both.
 struct EnsStep {
   int k, lpar, fpar, ncarry, lat, moments, reverse, cset;
 };
 hipError_t launch_ensemble_pairs(hipStream_t s, int prec, const void *members, const int2 *items, int n_lbm,
                                  int n_part, int n_lbm2, int *counters, const EnsStep &st, int tile_w,
-                                 int tile_h, int grid);
+                                 int tile_h, int grid, int nudged);
 
 }  // namespace rbn

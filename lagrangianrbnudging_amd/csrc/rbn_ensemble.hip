@@ -38,8 +38,8 @@ static_assert(kThreads == lbm2::kTileThreads, "two-step tiles run on whole workg
 // One-step tile of step k: input planes pf[par] / pg[par], force F (read and rewritten in
 // place).
 template <typename R>
-__device__ __forceinline__ void lbm_item(const EnsMember<R> &m, int tile, int k, int par, R *F, int tile_w,
-                                         int tile_h, int write_moments) {
+__device__ __forceinline__ void lbm_item(const EnsMember<R> &m, int tile, int k, int par, R *F, R *nud, int dset,
+                                         int tile_w, int tile_h, int write_moments) {
   const int tx = tile % m.tiles_x, ty = tile / m.tiles_x;
   LbmPtrs p{};
   p.fA = par ? m.pf[1] : m.pf[0];
@@ -54,10 +54,10 @@ __device__ __forceinline__ void lbm_item(const EnsMember<R> &m, int tile, int k,
   p.bad = m.bad;
   const int col = threadIdx.x % tile_w;
   if (m.variant == RBN_VARIANT_NUDGING) {
-    p.nvol = m.nvol[par];
-    p.Tbar = m.Tbar[par];
-    p.wq = m.wq[par];
-    p.nud = m.nud;
+    p.nvol = m.nvol[dset];
+    p.Tbar = m.Tbar[dset];
+    p.wq = m.wq[dset];
+    p.nud = nud;
     for (int r = threadIdx.x / tile_w; r < tile_h; r += kThreads / tile_w) {
       const int i = tx * tile_w + col, j = ty * tile_h + r;
       if (i < m.g.nx && j < m.g.ny) {
@@ -131,7 +131,7 @@ __device__ __forceinline__ void run_item(const EnsMember<R> *__restrict__ M, con
   }
   const int l = it - n_first;
   const int2 w = items[reverse ? n_lbm - 1 - l : l];
-  lbm_item(M[w.x], w.y, k, k & 1, M[w.x].F[0], tile_w, tile_h, write_moments);
+  lbm_item(M[w.x], w.y, k, k & 1, M[w.x].F[0], M[w.x].nud[0], k & 1, tile_w, tile_h, write_moments);
 }
 
 template <typename R, int PHASE>
@@ -154,7 +154,7 @@ __global__ __launch_bounds__(kThreads) void k_ensemble(const EnsMember<R> *__res
   for (;;) {
     if (threadIdx.x == 0) item = atomicAdd(ctr, 1);
     __syncthreads();
-    const int it = item;
+    const int it = __builtin_amdgcn_readfirstlane(item);  // uniform: descriptors in scalar registers
     __syncthreads();
     if (it >= n_items) break;
     run_item<R, PHASE>(M, items, it, n_lbm, n_first, k, tile_w, tile_h, write_moments, reverse);
@@ -186,10 +186,20 @@ void launch_prec(hipStream_t s, int phase, dim3 g, const void *members, const in
     launch_phase<R, 2>(s, g, members, items, n_lbm, n_part, counters, k, tile_w, tile_h, write_moments,
                        persistent, reverse);
 }
-// Two-step tile of steps k, k+1 (64 x TH nodes, rbn_lbm2_tile.h).
-template <typename R, int TH>
+// Kinds of two-step-mode launches: one step or particle updates only (no tile LDS), two-step
+// tiles of an ensemble of truth runs, of nudged runs, of both.  Each kind compiles only the
+// bodies it runs: both bodies in one kernel take 173 VGPRs in FP32 (two waves per SIMD), the
+// nudged one alone 163 (three, as many as the LDS allows).
+enum { kOneStep = 0, kPairsTruth = 1, kPairsNudged = 2, kPairsMixed = 3 };
+constexpr int pair_ring_g(int kind) { return kind == kPairsTruth ? 3 : 9; }
+
+// Two-step tile of steps k, k+1 (64 x TH nodes, rbn_lbm2_tile.h): the truth-run body, or the
+// nudged one (its LDS ring keeps all g directions) reading the deposits of steps k and k+1
+// (sets k & 1 and (k + 1) & 1) and moving the nudging term from nud[fpar] to nud[1 - fpar].
+template <typename R, int TH, int KIND>
 __device__ __forceinline__ void pair_item(const EnsMember<R> &m, int tile, const EnsStep &st,
-                                          lbm2::TileLds<R, TH, 3> &L) {
+                                          lbm2::TileLds<R, TH, pair_ring_g(KIND)> &L) {
+  constexpr int GR = pair_ring_g(KIND);
   const int tx = tile % m.t2x, ty = tile / m.t2x;
   lbm2::TwoStepPtrs p{};
   p.fA = m.pf[st.lpar];
@@ -203,35 +213,61 @@ __device__ __forceinline__ void pair_item(const EnsMember<R> &m, int tile, const
   p.rho2 = st.moments ? m.rho : nullptr;
   p.T2 = st.moments ? m.T : nullptr;
   p.bad = m.bad;
-  lbm2::two_step_tile<R, TH, RBN_VARIANT_TRAJECTORIES, 3>(m.g, p, m.c, tx, ty, L);
+  if constexpr (KIND != kPairsTruth) {
+    if (KIND == kPairsNudged || m.variant == RBN_VARIANT_NUDGING) {
+      for (int u = 0; u < 2; u++) {
+        const int ds = (st.k + u) & 1;
+        p.nv[u] = m.nvol[ds];
+        p.tb[u] = m.Tbar[ds];
+        p.wq[u] = m.wq[ds];
+      }
+      p.nud_in = m.nud[st.fpar];
+      p.nud_out = m.nud[1 - st.fpar];
+      lbm2::two_step_tile<R, TH, RBN_VARIANT_NUDGING, GR>(m.g, p, m.c, tx, ty, L);
+      return;
+    }
+  }
+  if constexpr (KIND != kPairsNudged) lbm2::two_step_tile<R, TH, RBN_VARIANT_TRAJECTORIES, GR>(m.g, p, m.c, tx, ty, L);
 }
 
-// Item `it` of a two-step-mode launch: carried particle items first, then lattice items.
+// LDS of a two-step-mode launch: the two-step tile's, none for one-step launches.
+template <typename R, int TH, int KIND>
+struct PairLds {
+  lbm2::TileLds<R, TH, pair_ring_g(KIND)> L;
+};
 template <typename R, int TH>
+struct PairLds<R, TH, kOneStep> {};
+
+// Item `it` of a two-step-mode launch: carried particle items first, then lattice items
+// (two-step tiles, or one-step tiles for kOneStep).
+template <typename R, int TH, int KIND>
 __device__ __forceinline__ void run_item2(const EnsMember<R> *__restrict__ M, const int2 *__restrict__ items, int it,
                                           int n_lbm, int n_part, int n_first, int n_lat, const EnsStep &st,
-                                          int tile_w, int tile_h, lbm2::TileLds<R, TH, 3> &L) {
+                                          int tile_w, int tile_h, PairLds<R, TH, KIND> &S) {
   if (it < n_first) {
     const int2 w = items[n_lbm + it];
     for (int u = 0; u < st.ncarry; u++) particle_item(M[w.x], w.y, st.k - st.ncarry + u);
     return;
   }
   const int l = it - n_first, lr = st.reverse ? n_lat - 1 - l : l;
-  if (st.lat == 2) {
+  if constexpr (KIND != kOneStep) {
     const int2 w = items[n_lbm + n_part + lr];
-    pair_item<R, TH>(M[w.x], w.y, st, L);
+    pair_item<R, TH, KIND>(M[w.x], w.y, st, S.L);
   } else {
     const int2 w = items[lr];
-    lbm_item(M[w.x], w.y, st.k, st.lpar, M[w.x].F[st.fpar], tile_w, tile_h, st.moments);
+    lbm_item(M[w.x], w.y, st.k, st.lpar, M[w.x].F[st.fpar], M[w.x].nud[st.fpar], st.k & 1, tile_w, tile_h,
+             st.moments);
   }
 }
 
-template <typename R, int TH>
+// KIND: kOneStep for the launches of one step or particle updates only (lat 1, 0), a pairs
+// kind for those of two-step tiles (lat 2).
+template <typename R, int TH, int KIND>
 __global__ __launch_bounds__(kThreads) void k_ensemble2(const EnsMember<R> *__restrict__ M,
                                                         const int2 *__restrict__ items, int n_lbm, int n_part,
                                                         int n_lbm2, int *counters, EnsStep st, int tile_w,
                                                         int tile_h, int persistent) {
-  __shared__ lbm2::TileLds<R, TH, 3> L;
+  __shared__ PairLds<R, TH, KIND> L;
   const int slot = st.lat ? 0 : 1;
   const int n_first = st.ncarry > 0 ? n_part : 0;
   const int n_lat = st.lat == 2 ? n_lbm2 : (st.lat == 1 ? n_lbm : 0);
@@ -239,7 +275,7 @@ __global__ __launch_bounds__(kThreads) void k_ensemble2(const EnsMember<R> *__re
   clear_next(counters, st.cset, slot);
   if (!persistent) {
     const int it = blockIdx.x;
-    if (it < n_items) run_item2<R, TH>(M, items, it, n_lbm, n_part, n_first, n_lat, st, tile_w, tile_h, L);
+    if (it < n_items) run_item2<R, TH, KIND>(M, items, it, n_lbm, n_part, n_first, n_lat, st, tile_w, tile_h, L);
     return;
   }
   __shared__ int item;
@@ -247,10 +283,10 @@ __global__ __launch_bounds__(kThreads) void k_ensemble2(const EnsMember<R> *__re
   for (;;) {
     if (threadIdx.x == 0) item = atomicAdd(ctr, 1);
     __syncthreads();
-    const int it = item;
+    const int it = __builtin_amdgcn_readfirstlane(item);  // uniform: descriptors in scalar registers
     __syncthreads();
     if (it >= n_items) break;
-    run_item2<R, TH>(M, items, it, n_lbm, n_part, n_first, n_lat, st, tile_w, tile_h, L);
+    run_item2<R, TH, KIND>(M, items, it, n_lbm, n_part, n_first, n_lat, st, tile_w, tile_h, L);
   }
 }
 
@@ -274,17 +310,38 @@ hipError_t launch_ensemble_phase(hipStream_t s, int prec, int phase, const void 
 
 hipError_t launch_ensemble_pairs(hipStream_t s, int prec, const void *members, const int2 *items, int n_lbm,
                                  int n_part, int n_lbm2, int *counters, const EnsStep &st, int tile_w,
-                                 int tile_h, int grid) {
+                                 int tile_h, int grid, int nudged) {
   const int n_items = (st.ncarry > 0 ? n_part : 0) + (st.lat == 2 ? n_lbm2 : (st.lat == 1 ? n_lbm : 0));
   if (n_items <= 0) return hipSuccess;
   const int persistent = grid > 0;
   const dim3 g(persistent ? (unsigned)grid : (unsigned)n_items), b(kThreads);
-  if (prec == RBN_FP64)
-    hipLaunchKernelGGL((k_ensemble2<double, kEnsTH64>), g, b, 0, s, (const EnsMember<double> *)members, items,
-                       n_lbm, n_part, n_lbm2, counters, st, tile_w, tile_h, persistent);
-  else
-    hipLaunchKernelGGL((k_ensemble2<float, kEnsTH32>), g, b, 0, s, (const EnsMember<float> *)members, items,
-                       n_lbm, n_part, n_lbm2, counters, st, tile_w, tile_h, persistent);
+  const auto *M64 = (const EnsMember<double> *)members;
+  const auto *M32 = (const EnsMember<float> *)members;
+  const int kind = st.lat != 2 ? kOneStep : (nudged == 0 ? kPairsTruth : (nudged == 1 ? kPairsNudged : kPairsMixed));
+#define RBN_ENS2(R, M, TH, KIND)                                                                                     \
+  hipLaunchKernelGGL((k_ensemble2<R, TH, KIND>), g, b, 0, s, M, items, n_lbm, n_part, n_lbm2, counters, st, tile_w, \
+                     tile_h, persistent)
+#define RBN_ENS2_KINDS(R, M, TH)           \
+  switch (kind) {                          \
+    case kPairsTruth:                      \
+      RBN_ENS2(R, M, TH, kPairsTruth);     \
+      break;                               \
+    case kPairsNudged:                     \
+      RBN_ENS2(R, M, TH, kPairsNudged);    \
+      break;                               \
+    case kPairsMixed:                      \
+      RBN_ENS2(R, M, TH, kPairsMixed);     \
+      break;                               \
+    default:                               \
+      RBN_ENS2(R, M, TH, kOneStep);        \
+  }
+  if (prec == RBN_FP64) {
+    RBN_ENS2_KINDS(double, M64, kEnsTH64)
+  } else {
+    RBN_ENS2_KINDS(float, M32, kEnsTH32)
+  }
+#undef RBN_ENS2_KINDS
+#undef RBN_ENS2
   return hipGetLastError();
 }
 

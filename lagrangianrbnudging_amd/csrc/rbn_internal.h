@@ -190,10 +190,10 @@ struct NudgePair {
 // step's outputs (fB, gB), p.F = F(s), p.uv = the first step's velocity slot; Fout receives
 // F(s+2), uv2 the second step's velocity, rho2 / T2 (or null) its moments.  job (or null):
 // particle updates carried as extra workgroups ahead of the tiles.  np (or null): the nudged
-// run's pair (tiles only).
+// run's pair (tiles only).  nt: non-temporal stores of the tiles' outputs.
 hipError_t launch_lbm2(hipStream_t s, int prec, const Geom &g, const LbmPtrs &p, void *Fout, void *uv2,
                        void *rho2, void *T2, const void *step_const, int tile_h, int seg, int xcd, int variant,
-                       int flip, const PartJob *job, const NudgePair *np = nullptr);
+                       int flip, int nt, const PartJob *job, const NudgePair *np = nullptr);
 // K1 of a truth-run step with the particle update of the previous step in the same grid
 // (its own workgroups, ahead of the lattice ones).
 hipError_t launch_lbm_fused(hipStream_t s, int prec, const Geom &g, const LbmPtrs &p,

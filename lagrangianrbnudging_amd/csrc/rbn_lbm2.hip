@@ -29,7 +29,7 @@ struct KernelLds {
   char cap[(sizeof(R) == 4 && TH == 8) ? 8192 : 1];
 };
 
-template <typename R, int TH, int V>
+template <typename R, int TH, int V, int NT>
 __global__ __launch_bounds__(kTileThreads) void k_lbm2(Geom g, Lbm2Args a, StepConst<R> c, part::PartSteps<R> ps) {
   __shared__ KernelLds<R, TH, V> S;
   auto &L = S.L;
@@ -40,7 +40,7 @@ __global__ __launch_bounds__(kTileThreads) void k_lbm2(Geom g, Lbm2Args a, StepC
   const int b = (int)blockIdx.x - ps.nblocks;
   const int t = a.flip ? a.ntiles - 1 - b : b;
   const int ty = t / a.tiles_x, tx = t - ty * a.tiles_x;
-  two_step_tile<R, TH, V, ring_g<V>()>(g, a.p, c, tx, ty, L);
+  two_step_tile<R, TH, V, ring_g<V>(), NT>(g, a.p, c, tx, ty, L);
 }
 
 struct WaveArgs {
@@ -95,37 +95,53 @@ int tile_rows(int prec, int tile_h) {
   return 8;
 }
 
-template <int V>
+template <int V, int NT>
 void dispatch32(hipStream_t s, int th, const Geom &g, const Lbm2Args &a, const void *step_const,
                 const PartJob *job) {
   const StepConst<float> &c = *(const StepConst<float> *)step_const;
   const part::PartSteps<float> ps = part::part_steps<float>(g, job, kTileThreads);
   const dim3 grid((unsigned)(a.ntiles + ps.nblocks)), block(kTileThreads);
   if (th == 4)
-    hipLaunchKernelGGL((k_lbm2<float, 4, V>), grid, block, 0, s, g, a, c, ps);
+    hipLaunchKernelGGL((k_lbm2<float, 4, V, NT>), grid, block, 0, s, g, a, c, ps);
   else if (th == 16)
-    hipLaunchKernelGGL((k_lbm2<float, 16, V>), grid, block, 0, s, g, a, c, ps);
+    hipLaunchKernelGGL((k_lbm2<float, 16, V, NT>), grid, block, 0, s, g, a, c, ps);
   else
-    hipLaunchKernelGGL((k_lbm2<float, 8, V>), grid, block, 0, s, g, a, c, ps);
+    hipLaunchKernelGGL((k_lbm2<float, 8, V, NT>), grid, block, 0, s, g, a, c, ps);
 }
 
-template <int V>
+template <int V, int NT>
 void dispatch64(hipStream_t s, int th, const Geom &g, const Lbm2Args &a, const void *step_const,
                 const PartJob *job) {
   const StepConst<double> &c = *(const StepConst<double> *)step_const;
   const part::PartSteps<double> ps = part::part_steps<double>(g, job, kTileThreads);
   const dim3 grid((unsigned)(a.ntiles + ps.nblocks)), block(kTileThreads);
   if (th == 4)
-    hipLaunchKernelGGL((k_lbm2<double, 4, V>), grid, block, 0, s, g, a, c, ps);
+    hipLaunchKernelGGL((k_lbm2<double, 4, V, NT>), grid, block, 0, s, g, a, c, ps);
   else
-    hipLaunchKernelGGL((k_lbm2<double, 8, V>), grid, block, 0, s, g, a, c, ps);
+    hipLaunchKernelGGL((k_lbm2<double, 8, V, NT>), grid, block, 0, s, g, a, c, ps);
+}
+
+template <int V>
+void dispatch(hipStream_t s, int prec, int nt, int th, const Geom &g, const Lbm2Args &a, const void *step_const,
+              const PartJob *job) {
+  if (prec == RBN_FP64) {
+    if (nt)
+      dispatch64<V, 1>(s, th, g, a, step_const, job);
+    else
+      dispatch64<V, 0>(s, th, g, a, step_const, job);
+  } else {
+    if (nt)
+      dispatch32<V, 1>(s, th, g, a, step_const, job);
+    else
+      dispatch32<V, 0>(s, th, g, a, step_const, job);
+  }
 }
 
 }  // namespace
 
 hipError_t launch_lbm2(hipStream_t s, int prec, const Geom &g, const LbmPtrs &p, void *Fout, void *uv2,
                        void *rho2, void *T2, const void *step_const, int tile_h, int seg, int xcd, int variant,
-                       int flip, const PartJob *job, const NudgePair *np) {
+                       int flip, int nt, const PartJob *job, const NudgePair *np) {
   if (seg > 0 && !np) {
     WaveArgs w{};
     w.p.fA = p.fA;
@@ -180,17 +196,10 @@ hipError_t launch_lbm2(hipStream_t s, int prec, const Geom &g, const LbmPtrs &p,
   a.tiles_x = (g.nx + kTW - 1) / kTW;
   a.ntiles = a.tiles_x * ((g.ny + th - 1) / th);
   a.flip = flip;
-  if (prec == RBN_FP64) {
-    if (np)
-      dispatch64<RBN_VARIANT_NUDGING>(s, th, g, a, step_const, job);
-    else
-      dispatch64<RBN_VARIANT_TRAJECTORIES>(s, th, g, a, step_const, job);
-  } else {
-    if (np)
-      dispatch32<RBN_VARIANT_NUDGING>(s, th, g, a, step_const, job);
-    else
-      dispatch32<RBN_VARIANT_TRAJECTORIES>(s, th, g, a, step_const, job);
-  }
+  if (np)
+    dispatch<RBN_VARIANT_NUDGING>(s, prec, nt, th, g, a, step_const, job);
+  else
+    dispatch<RBN_VARIANT_TRAJECTORIES>(s, prec, nt, th, g, a, step_const, job);
   return hipGetLastError();
 }
 

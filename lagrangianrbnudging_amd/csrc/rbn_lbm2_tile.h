@@ -466,8 +466,10 @@ __device__ __forceinline__ void first_step(const Geom &g, const TwoStepPtrs &a, 
 // Phase 1 computes step s on the tile and its ring: first the tile nodes, thread t taking
 // nodes t, t + kTileThreads, ... of the tile in row order (the same nodes it updates in phase 2,
 // so their next force and nudging term stay in its registers), then the ring nodes, one per
-// thread.  Phase 2 computes step s+1 on the tile from LDS and writes the output planes.
-template <typename R, int TH, int V, int GR>
+// thread.  Phase 2 computes step s+1 on the tile from LDS and writes the output planes, with
+// non-temporal stores (NT = 1: lattices larger than the caches) or ordinary ones (NT = 0:
+// the next launch finds them in the caches).
+template <typename R, int TH, int V, int GR, int NT = 1>
 __device__ __forceinline__ void two_step_tile(const Geom &g, const TwoStepPtrs &a, const StepConst<R> &c, int tx,
                                               int ty, TileLds<R, TH, GR> &L) {
   constexpr int NQ = kTW * TH / kTileThreads;  // tile nodes per thread
@@ -531,10 +533,10 @@ __device__ __forceinline__ void two_step_tile(const Geom &g, const TwoStepPtrs &
     flag_nonfinite(a.bad, rho, T);
     R *fB = (R *)a.fB, *gB = (R *)a.gB;
 #pragma unroll
-    for (int k = 0; k < 9; k++) st<1>(&fB[(size_t)k * g.plane + o], fo[k]);
+    for (int k = 0; k < 9; k++) st<NT>(&fB[(size_t)k * g.plane + o], fo[k]);
 #pragma unroll
-    for (int k = 0; k < 9; k++) st<1>(&gB[(size_t)k * g.plane + o], go[k]);
-    st<1>(&((R *)a.Fout)[o], Fn);
+    for (int k = 0; k < 9; k++) st<NT>(&gB[(size_t)k * g.plane + o], go[k]);
+    st<NT>(&((R *)a.Fout)[o], Fn);
     st_uv<0, R>(a.uv2, o, u, v);
     if (a.rho2) ((R *)a.rho2)[o] = rho;
     if (a.T2) ((R *)a.T2)[o] = T;

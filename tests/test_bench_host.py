@@ -49,14 +49,18 @@ def test_slab_cpu_baseline_places_every_particle(monkeypatch):
 
 def _autotune_records(trial, ntempread):
     """The records rbn_ensemble_autotune reads from the starting state, restated from its
-    loop: every candidate runs a warm-up call of 6 steps and a timed call of reps * 6 steps
-    (reps = max(1, ceil(trial / 6))) and is restored; then a call of 2 L + 2 steps per form.
-    A call reads at step s (s % ntempread == 1) the record of step s + ntempread - 1."""
+    loop: every candidate runs a warm-up call of G steps and a timed call of reps * G steps
+    (reps = max(1, ceil(trial / G)); G = 6 one launch per step, 12 two steps per launch) and
+    is restored; then per form a call of 2 L + 2 steps (L = lcm(6, ntempread)) or 2 L + 3
+    (L = lcm(12, ntempread)), whichever launch mode won.  A call reads at step s
+    (s % ntempread == 1) the record of step s + ntempread - 1."""
     import math
 
-    reps = max(1, (trial + 5) // 6)
-    L = 6 * ntempread // math.gcd(6, ntempread)
-    spans = [6 + reps * 6, 2 * L + 2]
+    spans = []
+    for g, t0 in ((6, 1), (12, 2)):
+        reps = max(1, (trial + g - 1) // g)
+        L = g * ntempread // math.gcd(g, ntempread)
+        spans += [g + reps * g, 2 * L + t0 + 1]
     return {s + ntempread - 1 for span in spans for s in range(1, span + 1) if s % ntempread == 1}
 
 

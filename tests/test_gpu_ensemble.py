@@ -107,7 +107,7 @@ def _mixed_members(prec, table, intervals=None):
     return out
 
 
-def _compare_nudged(a, b):
+def _compare_nudged(a, b, first_nudged=2):
     for k, (sa, sb) in enumerate(zip(a, b)):
         fa, ga = sa.populations()
         fb, gb = sb.populations()
@@ -116,7 +116,7 @@ def _compare_nudged(a, b):
         for key in ("rho", "T", "u", "v"):
             assert_equal_arrays(sa.fields()[key], sb.fields()[key], f"member {k} {key}")
         assert sa.istep == sb.istep
-        if k >= 2:
+        if k >= first_nudged:  # the deposit of the last step and the nudging term
             na, nb = sa.nudging(), sb.nudging()
             for key in na:
                 assert_equal_arrays(na[key], nb[key], f"member {k} {key}")
@@ -129,21 +129,24 @@ def _compare_nudged(a, b):
 @pytest.mark.parametrize("prec", [rbn.RBN_FP64, rbn.RBN_FP32])
 @pytest.mark.parametrize("tuning,intervals", [((256, 1, 0, 0), None), ((128, 4, 4, 3), None), ((64, 8, 2, 1), None),
                                               ((256, 1, 4, 10), None), ((256, 1, 0, 2), (3, 3, 2, 6)),
-                                              ((128, 4, 4, 1), (3, 3, 2, 6)), ((256, 1, 0, 18), (3, 3, 2, 6))])
+                                              ((128, 4, 4, 1), (3, 3, 2, 6)), ((256, 1, 0, 18), (3, 3, 2, 6)),
+                                              ((256, 1, 0, 8), (3, 3, 2, 6)), ((256, 1, 2, 9), None),
+                                              ((256, 1, 4, 27), (3, 3, 2, 6)), ((256, 1, 0, 4), None)])
 def test_ensemble_with_nudged_members_equals_individual_steps(prec, tuning, intervals):
     # Nudged members (7x7 and bilinear deposits, different record intervals, one without
     # observers) beside truth-run members; the observer updates and deposits of all nudged
     # members run as batched launches, records are uploaded between steps or, inside the
-    # captured cycles (60 steps for intervals 4, 5, 5, 3; 6 steps for 3, 3, 2, 6), by the
-    # graph from a staging set the host fills before each replay.  Calls of 122 steps (two
-    # cycles of 60 and eager steps), 120 (whole cycles: the graphs stay for the next call)
-    # and 62 (a replay of the kept graph).  Two-step and separate-launch orders fall back to
-    # one launch per step.
+    # captured cycles, scattered by the graph from a staging set uploaded before each replay.
+    # One launch per step: cycles of 60 steps for intervals 4, 5, 5, 3, of 6 for 3, 3, 2, 6,
+    # from call step 1.  Two steps per launch (order bit 3), both steps' deposits before it:
+    # cycles of 60 and 12 from call step 2.  Calls of 122 steps (cycles and eager steps), 120
+    # (whole cycles: the graphs stay for the next call) and 63 (a replay of the kept graph).
+    # Separate-launch orders (bit 2) fall back to one launch per step.
     ens_members = _mixed_members(prec, table=True, intervals=intervals)
     ref_members = _mixed_members(prec, table=False, intervals=intervals)
     e = rbn.Ensemble(ens_members)
     e.set_tuning(*tuning)
-    calls = (9, 1, 2, 6, 13, 20, 122, 120, 62)
+    calls = (9, 1, 2, 6, 13, 20, 122, 120, 63)
     for n in calls:
         e.step(n)
     for s in ref_members:
@@ -152,14 +155,33 @@ def test_ensemble_with_nudged_members_equals_individual_steps(prec, tuning, inte
     e.close()
 
 
+@pytest.mark.parametrize("prec", [rbn.RBN_FP64, rbn.RBN_FP32])
+@pytest.mark.parametrize("order", [0, 8, 10, 24])
+def test_ensemble_of_nudged_members_only_equals_individual_steps(prec, order):
+    # An ensemble of nudged members only runs the two-step kernel compiled for nudged tiles
+    # alone (order bit 3); odd calls end on a one-step launch and leave the last deposit in
+    # the second set.
+    ens_members = _mixed_members(prec, table=True, intervals=(3, 3, 2, 6))[2:]
+    ref_members = _mixed_members(prec, table=False, intervals=(3, 3, 2, 6))[2:]
+    e = rbn.Ensemble(ens_members)
+    e.set_tuning(256, 1, 4, order)
+    calls = (1, 2, 15, 12, 40, 27)
+    for n in calls:
+        e.step(n)
+    for s in ref_members:
+        s.step(sum(calls))
+    _compare_nudged(ens_members, ref_members, first_nudged=0)
+    e.close()
+
+
 def test_ensemble_autotune_with_nudged_members_keeps_state():
     ens_members = _mixed_members(rbn.RBN_FP32, table=True)
     ref_members = _mixed_members(rbn.RBN_FP32, table=False)
     e = rbn.Ensemble(ens_members)
     assert e.autotune(4) > 0
-    # one launch per step (bits 2, 3 clear); bit 4 (eager steps with the deposits overlapped)
-    # or graph replays, whichever measured faster
-    assert (e.tuning()["order"] & 15) < 4
+    # one launch per step or two (bit 3); not the separate-launch orders (bit 2); bit 4 (eager
+    # steps) or graph replays, whichever measured faster
+    assert (e.tuning()["order"] & 4) == 0
     for s in ens_members:
         assert s.istep == 0
     e.step(17)
