@@ -263,11 +263,12 @@ int rbn_synchronize(rbn_ctx *ctx);
  * of the previous step) and lattice tiles, and cycles of launches are captured in one
  * hipGraph that is replayed.  Nudged members add, per step, one launch per observer/deposit
  * pass over all of them and the upload of the records that start an interval (fetched from
- * their providers before the graph replay that uploads them); such ensembles take one
- * launch per step, in cycles of lcm(6, record intervals) steps.  Their eager steps build the
- * next step's deposit (a second set of deposit arrays per member, owned by the ensemble) on
- * a second stream beside the current lattice launch; graph replays run the passes in order
- * on one stream.  No reference counterpart (the
+ * their providers before the graph replay that uploads them), in cycles of lcm(6, record
+ * intervals) steps with one launch per step, lcm(12, record intervals) with two.  With one
+ * launch per step, eager steps build the next step's deposit (a second set of deposit arrays
+ * per member, owned by the ensemble) on a second stream beside the current lattice launch;
+ * with two, both steps' deposits precede their launch.  Graph replays run the passes in
+ * order on one stream.  No reference counterpart (the
  * reference runs one lattice per job).  Members must share device and precision, be
  * initialised, and stay owned by the caller; results equal stepping each member with
  * rbn_step, except that no trajectory records or diagnostics are produced during ensemble
@@ -295,12 +296,13 @@ int rbn_ensemble_get_tuning(const rbn_ensemble *e, int *tile_w, int *tile_h, int
 /* Time every candidate launch shape on trial_steps real steps, keep the fastest, and restore
  * every member's state afterwards, also when a step fails (the error is returned and every
  * member is back at its starting state); *best_ms_per_step receives the winner's time.  With
- * nudged members every candidate steps from the starting state (a warm-up call of 6 steps
- * and a timed call of 6 * max(1, ceil(trial_steps / 6)) steps), and the fastest shape is
- * timed once more over steps 1 .. 2L of a call of 2 L + 2 steps in both forms, graph replays
- * and eager steps with the deposits overlapped (order bit 4), keeping the faster
- * (L = lcm(6, record intervals) <= 120): the members' records must cover the longer of the
- * two spans from the starting state. */
+ * nudged members every candidate steps from the starting state (a warm-up call of G steps
+ * and a timed call of G * max(1, ceil(trial_steps / G)) steps; G = 6 for one launch per
+ * step, 12 for two), and the fastest shape is timed once more in both forms, graph replays
+ * and eager steps (order bit 4), keeping the faster: over steps 1 .. 2L of a call of 2 L + 2
+ * steps with one launch per step (L = lcm(6, record intervals)), steps 2 .. 2L + 1 of a call
+ * of 2 L + 3 steps with two (L = lcm(12, record intervals)), L <= 120.  The members' records
+ * must cover the longest of these spans from the starting state. */
 int rbn_ensemble_autotune(rbn_ensemble *e, int trial_steps, double *best_ms_per_step);
 
 /* ---------------------------------------------------------------- adjoint window */

@@ -60,7 +60,8 @@ void dfree(void *&p) {
 void free_dep2(rbn_ctx *c) {
   rbn::DepPtrs &s = c->dep2;
   for (void *p : {(void *)s.cnt, (void *)s.fill, (void *)s.off, (void *)s.list, (void *)s.blocksum,
-                  (void *)s.active, (void *)s.nactive, (void *)s.Tbar, (void *)s.wq, (void *)s.S, (void *)s.Wi})
+                  (void *)s.active, (void *)s.nactive, (void *)s.Tbar, (void *)s.wq, (void *)s.S, (void *)s.Wi,
+                  s.tw, (void *)s.tcnt, (void *)s.bin, (void *)s.ocnt})
     if (p) (void)hipFree(p);
   s = rbn::DepPtrs{};
 }
@@ -74,12 +75,21 @@ int ensure_dep2(rbn_ctx *c) {
   s.cnt = s.fill = s.off = s.list = s.blocksum = s.active = s.nactive = nullptr;
   s.Tbar = s.wq = nullptr;
   s.S = s.Wi = nullptr;
+  s.tw = nullptr;
+  s.tcnt = s.ocnt = nullptr;
+  s.bin = nullptr;
   const size_t plane = c->geom.plane, ntiles = (plane + rbn::kDepScanTile - 1) / rbn::kDepScanTile;
-  const size_t sums = s.mode == RBN_DEPOSIT_BILINEAR ? plane * sizeof(long long) : 0;
+  const bool bil = s.mode == RBN_DEPOSIT_BILINEAR;
+  const size_t sums = bil ? plane * sizeof(long long) : 0;
+  const size_t dtiles = (size_t)rbn::dep_tiles(c->geom);
   struct B {
     void **p;
     size_t b;
   } a[] = {{(void **)&s.S, sums}, {(void **)&s.Wi, sums},
+           {&s.tw, plane * 2 * c->rs},
+           {(void **)&s.tcnt, bil ? dtiles * rbn::kTcntStride * sizeof(int) : 0},
+           {(void **)&s.ocnt, bil ? plane * sizeof(int) : 0},
+           {(void **)&s.bin, bil ? dtiles * (size_t)s.bin_cap * 2 * sizeof(long long) : 0},
            {(void **)&s.off, (plane + 1) * sizeof(int)},
            {(void **)&s.fill, plane * sizeof(int)},
            {(void **)&s.blocksum, ntiles * sizeof(int)},
@@ -496,6 +506,7 @@ int init_common_after_fields(rbn_ctx *c) {
   c->moments_valid = true;
   if (c->nud) HIPCHK(c, hipMemsetAsync(c->nud, 0, c->geom.plane * c->rs, c->stream));
   if (c->dep.cnt) HIPCHK(c, hipMemsetAsync(c->dep.cnt, 0, c->geom.plane * sizeof(int), c->stream));
+  if (c->dep.tw) HIPCHK(c, hipMemsetAsync(c->dep.tw, 0, c->geom.plane * 2 * c->rs, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return RBN_OK;
 }
@@ -565,6 +576,10 @@ void rbn_destroy(rbn_ctx *c) {
   dfree_t(c->dep.nactive);
   dfree_t(c->dep.S);
   dfree_t(c->dep.Wi);
+  dfree_t(c->dep.tw);
+  dfree_t(c->dep.tcnt);
+  dfree_t(c->dep.bin);
+  dfree_t(c->dep.ocnt);
   free_dep2(c);
   for (hipEvent_t ev : {c->ev_dep[0], c->ev_dep[1], c->ev_lat[0], c->ev_lat[1]})
     if (ev) (void)hipEventDestroy(ev);
@@ -654,6 +669,7 @@ int rbn_create(const rbn_params *p, int device, rbn_ctx **out) {
     (void)hipMemsetAsync(c->nud, 0, fb, c->stream);
     (void)hipMemsetAsync(c->nud2, 0, fb, c->stream);
     const size_t ntiles = (g.plane + rbn::kDepScanTile - 1) / rbn::kDepScanTile;
+    const bool bil = p->deposit == RBN_DEPOSIT_BILINEAR;
     struct B {
       void **p;
       size_t b;
@@ -665,14 +681,18 @@ int rbn_create(const rbn_params *p, int device, rbn_ctx **out) {
                    {(void **)&c->dep.wq, g.plane * sizeof(double)},
                    {(void **)&c->dep.overflow, sizeof(int)},
                    {(void **)&c->dep.nactive, sizeof(int)},
-                   {(void **)&c->dep.S, p->deposit == RBN_DEPOSIT_BILINEAR ? g.plane * sizeof(long long) : 0},
-                   {(void **)&c->dep.Wi, p->deposit == RBN_DEPOSIT_BILINEAR ? g.plane * sizeof(long long) : 0}};
+                   {(void **)&c->dep.S, bil ? g.plane * sizeof(long long) : 0},
+                   {(void **)&c->dep.Wi, bil ? g.plane * sizeof(long long) : 0},
+                   {&c->dep.tw, g.plane * 2 * c->rs},
+                   {(void **)&c->dep.tcnt, bil ? (size_t)rbn::dep_tiles(g) * rbn::kTcntStride * sizeof(int) : 0},
+                   {(void **)&c->dep.ocnt, bil ? g.plane * sizeof(int) : 0}};
     for (auto &a : dallocs) {
       if (a.b == 0) continue;
       if ((rc = dalloc(c, a.p, a.b))) goto fail_out;
       (void)hipMemsetAsync(*a.p, 0, a.b, c->stream);
     }
     c->dep.mode = p->deposit;
+    c->dep.tw64 = c->prec == RBN_FP64;
   }
   c->consts.resize(std::max(sizeof(rbn::StepConst<double>), sizeof(rbn::StepConst<float>)));
   if (c->prec == RBN_FP64)
@@ -970,6 +990,7 @@ int rbn_init_observers(rbn_ctx *c, int n, const int *id, const double *x, const 
   dfree_t(c->d_obs_d);
   dfree_t(c->dep.list);
   dfree_t(c->dep.active);
+  dfree_t(c->dep.bin);
   free_dep2(c);  // sized by the observers: allocated again on first use
   c->nobs = n;
   c->oid.assign(id, id + n);
@@ -992,6 +1013,13 @@ int rbn_init_observers(rbn_ctx *c, int n, const int *id, const double *x, const 
   c->dep.dense = 4 * ncand >= (long long)c->geom.plane;
   if ((rc = dalloc(c, &tmp, (size_t)c->dep.active_cap * sizeof(int)))) return rc;
   c->dep.active = (int *)tmp;
+  if (c->dep.mode == RBN_DEPOSIT_BILINEAR) {
+    // the bilinear deposit's tile slots, sized by the observers' mean density
+    c->dep.bin_cap = rbn::dep_bin_cap(c->geom, n);
+    const size_t bb = (size_t)rbn::dep_tiles(c->geom) * (size_t)c->dep.bin_cap * 2 * sizeof(long long);
+    if ((rc = dalloc(c, &tmp, bb))) return rc;
+    c->dep.bin = (long long *)tmp;
+  }
   HIPCHK(c, hipMemsetAsync(c->d_obs_i, 0, ONI * N * sizeof(int), c->stream));
   std::vector<double> h(OND * N, 0.0);
   for (int i = 0; i < n; i++) {
@@ -1003,6 +1031,7 @@ int rbn_init_observers(rbn_ctx *c, int n, const int *id, const double *x, const 
   // set_nudge_vol zeroes nud_term and nudge_vol (N:2051-2056).
   if (c->nud) HIPCHK(c, hipMemsetAsync(c->nud, 0, c->geom.plane * c->rs, c->stream));
   HIPCHK(c, hipMemsetAsync(c->dep.cnt, 0, c->geom.plane * sizeof(int), c->stream));
+  HIPCHK(c, hipMemsetAsync(c->dep.tw, 0, c->geom.plane * 2 * c->rs, c->stream));  // empty deposit
   HIPCHK(c, hipMemsetAsync(c->dep.overflow, 0, sizeof(int), c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return RBN_OK;
@@ -1177,7 +1206,9 @@ int report_energy(rbn_ctx *c, int istep) {
 
 // Observer update and deposit of step istep into d on stream s (with the record upload of an
 // interval's first step).
-int nudged_deposit(rbn_ctx *c, int istep, hipStream_t s, const rbn::DepPtrs &d) {
+// tail, defer: as launch_obs_deposit's.
+int nudged_deposit(rbn_ctx *c, int istep, hipStream_t s, const rbn::DepPtrs &d, const rbn::DepPtrs *tail = nullptr,
+                   bool defer = false) {
   const rbn_params &P = c->P;
   const int right = P.fix_q3 ? P.nx - 3 : P.nx - 2;
   const int r = istep % P.ntempread;
@@ -1186,7 +1217,7 @@ int nudged_deposit(rbn_ctx *c, int istep, hipStream_t s, const rbn::DepPtrs &d) 
   const int branch = r == 1 ? 1 : (r == 0 ? 0 : 2);
   const rbn::ObsPtrs o = obs_ptrs(c);
   return timed(c, KT_DEPOSIT, s,
-               [&] { return rbn::launch_obs_deposit(s, branch, P.ntempread, c->geom, right, o, d); });
+               [&] { return rbn::launch_obs_deposit(s, branch, P.ntempread, c->geom, right, o, d, tail, defer); });
 }
 
 int step_loop(rbn_ctx *c, int nsteps) {
@@ -1239,7 +1270,10 @@ int step_loop(rbn_ctx *c, int nsteps) {
     }
     const bool writes_moments = pair ? moments2 : moments;
     if (nudged && pair) {
-      if ((rc = nudged_deposit(c, istep, c->stream, c->dep2)) || (rc = nudged_deposit(c, istep2, c->stream, c->dep)))
+      // the first step's bilinear tile pass rides in the second step's first launch
+      const bool bil = c->dep.mode == RBN_DEPOSIT_BILINEAR;
+      if ((rc = nudged_deposit(c, istep, c->stream, c->dep2, nullptr, bil)) ||
+          (rc = nudged_deposit(c, istep2, c->stream, c->dep, bil ? &c->dep2 : nullptr)))
         return rc;
     } else if (nudged && !dovl) {
       if ((rc = nudged_deposit(c, istep, c->stream, c->dep))) return rc;
@@ -1277,9 +1311,7 @@ int step_loop(rbn_ctx *c, int nsteps) {
     lp.rho = !pair && moments ? c->rho : nullptr;
     lp.T = !pair && moments ? c->T : nullptr;
     if (nudged) {
-      lp.nvol = dset.cnt;
-      lp.Tbar = dset.Tbar;
-      lp.wq = dset.mode == RBN_DEPOSIT_BILINEAR ? dset.wq : nullptr;
+      lp.tw = dset.tw;
       lp.nud = c->nud;
     }
     lp.bad = c->d_err + 1;
@@ -1290,9 +1322,7 @@ int step_loop(rbn_ctx *c, int nsteps) {
              const rbn::PartJob job = carry ? pending_job(c) : rbn::PartJob{};
              rbn::NudgePair np{};
              if (nudged) {
-               np.nvol2 = c->dep.cnt;
-               np.Tbar2 = c->dep.Tbar;
-               np.wq2 = c->dep.mode == RBN_DEPOSIT_BILINEAR ? c->dep.wq : nullptr;
+               np.tw2 = c->dep.tw;
                np.nud_out = c->nud2;
              }
              return rbn::launch_lbm2(c->stream, c->prec, c->geom, lp, c->F2, c->uv[s2], moments2 ? c->rho : nullptr,

@@ -35,8 +35,9 @@ struct rbn_ensemble {
   hipEvent_t seg_a = nullptr, seg_b = nullptr;
   int seg_steps = 0;
   void *d_nmembers = nullptr;
-  int *d_npre = nullptr;
-  int ntot[rbn::kNudgeBatches] = {};
+  int4 *d_nmap = nullptr;
+  size_t nmap_cap = 0;
+  rbn::NudgePlan plan{};
   // their record uploads: two pinned staging buffers with a device twin each, alternating
   // between the steps that load records
   struct Stage {
@@ -105,7 +106,8 @@ void drop_graph(rbn_ensemble *e) {
 // The second deposit set of a nudged member: the sizes of its own, the overflow flag shared.
 void free_spare(rbn::DepPtrs &s) {
   for (void *p : {(void *)s.cnt, (void *)s.fill, (void *)s.off, (void *)s.list, (void *)s.blocksum,
-                  (void *)s.active, (void *)s.nactive, (void *)s.Tbar, (void *)s.wq, (void *)s.S, (void *)s.Wi})
+                  (void *)s.active, (void *)s.nactive, (void *)s.Tbar, (void *)s.wq, (void *)s.S, (void *)s.Wi,
+                  s.tw, (void *)s.tcnt, (void *)s.bin, (void *)s.ocnt})
     if (p) (void)hipFree(p);
   s = rbn::DepPtrs{};
 }
@@ -116,13 +118,22 @@ int alloc_spare(rbn_ensemble *e, const rbn_ctx *c, rbn::DepPtrs &s) {
   x.cnt = x.fill = x.off = x.list = x.blocksum = x.active = x.nactive = nullptr;
   x.Tbar = x.wq = nullptr;
   x.S = x.Wi = nullptr;
+  x.tw = nullptr;
+  x.tcnt = x.ocnt = nullptr;
+  x.bin = nullptr;
   s = x;
   const size_t plane = c->geom.plane, ntiles = (plane + rbn::kDepScanTile - 1) / rbn::kDepScanTile;
-  const size_t sums = x.mode == RBN_DEPOSIT_BILINEAR ? plane * sizeof(long long) : 0;
+  const bool bil = x.mode == RBN_DEPOSIT_BILINEAR;
+  const size_t sums = bil ? plane * sizeof(long long) : 0;
+  const size_t dtiles = (size_t)rbn::dep_tiles(c->geom);
   struct A {
     void **p;
     size_t b;
   } a[] = {{(void **)&s.S, sums}, {(void **)&s.Wi, sums}, {(void **)&s.cnt, plane * sizeof(int)},
+           {&s.tw, plane * 2 * c->rs},
+           {(void **)&s.tcnt, bil ? dtiles * rbn::kTcntStride * sizeof(int) : 0},
+           {(void **)&s.ocnt, bil ? plane * sizeof(int) : 0},
+           {(void **)&s.bin, bil ? dtiles * (size_t)x.bin_cap * 2 * sizeof(long long) : 0},
            {(void **)&s.fill, plane * sizeof(int)},
            {(void **)&s.off, (plane + 1) * sizeof(int)},
            {(void **)&s.blocksum, ntiles * sizeof(int)},
@@ -132,9 +143,7 @@ int alloc_spare(rbn_ensemble *e, const rbn_ctx *c, rbn::DepPtrs &s) {
            {(void **)&s.Tbar, plane * sizeof(double)},
            {(void **)&s.wq, plane * sizeof(double)}};
   for (auto &y : a) {
-    if (y.p == (void **)&s.S || y.p == (void **)&s.Wi) {
-      if (y.b == 0) continue;
-    }
+    if (y.b == 0 && y.p != (void **)&s.cnt) continue;  // parts of the other deposit mode
     const size_t b = std::max<size_t>(y.b, sizeof(int));
     EHIP(e, hipMalloc(y.p, b));
     EHIP(e, hipMemset(*y.p, 0, b));
@@ -149,7 +158,7 @@ int ensure_spares(rbn_ensemble *e) {
     const rbn::DepPtrs &d = e->m[(size_t)q]->dep;
     rbn::DepPtrs &s = e->spare[(size_t)q];
     if (s.cnt && s.list_cap == d.list_cap && s.active_cap == d.active_cap && s.mode == d.mode &&
-        s.dense == d.dense && s.overflow == d.overflow)
+        s.dense == d.dense && s.overflow == d.overflow && s.bin_cap == d.bin_cap)
       continue;
     int rc = alloc_spare(e, e->m[(size_t)q], s);
     if (rc) return rc;
@@ -185,12 +194,9 @@ void fill_member(const rbn_ctx *c, const rbn::DepPtrs *spare, int tile_w, int ti
   d.bad = c->d_err + 1;
   d.variant = c->P.variant;
   if (c->P.variant == RBN_VARIANT_NUDGING) {
-    const bool bil = c->dep.mode == RBN_DEPOSIT_BILINEAR;
     for (int p = 0; p < 2; p++) {
       const rbn::DepPtrs &s = p ? c->dep : *spare;
-      d.nvol[p] = s.cnt;
-      d.Tbar[p] = s.Tbar;
-      d.wq[p] = bil ? s.wq : nullptr;
+      d.tw[p] = s.tw;
     }
     d.nud[0] = (R *)c->nud;
     d.nud[1] = (R *)c->nud2;
@@ -201,12 +207,12 @@ size_t loading_members(const rbn_ensemble *e, int t, std::vector<int> &ld);
 bool pairs(const rbn_ensemble *e);
 
 // Observer/deposit descriptors of the nudged members as of their current step, the block
-// prefixes of the batched passes, and the record uploads of the captured cycle.
+// map of the batched passes, and the record uploads of the captured cycle.
 int build_nudged(rbn_ensemble *e) {
   const int nm = (int)e->nudged.size();
   if (nm == 0) return RBN_OK;
   std::vector<rbn::NudgeMember> d(2 * (size_t)nm);
-  std::vector<int> pre((size_t)rbn::kNudgeBatches * (nm + 1), 0);
+  std::vector<int> blocks((size_t)rbn::kNudgeBatches * nm, 0);
   for (int q = 0; q < nm; q++) {
     rbn_ctx *c = e->m[(size_t)e->nudged[(size_t)q]];
     rbn::NudgeMember &x = d[(size_t)q];
@@ -218,16 +224,29 @@ int build_nudged(rbn_ensemble *e) {
     x.d = c->dep;
     d[(size_t)nm + q] = x;  // set 1: the member's own deposit arrays
     x.d = e->spare[(size_t)e->nudged[(size_t)q]];
-    int b[rbn::kNudgeBatches];
-    rbn::nudge_batch_blocks(x, b);
-    for (int p = 0; p < rbn::kNudgeBatches; p++)
-      pre[(size_t)p * (nm + 1) + q + 1] = pre[(size_t)p * (nm + 1) + q] + b[p];
+    rbn::nudge_batch_blocks(x, &blocks[(size_t)q * rbn::kNudgeBatches]);
   }
-  for (int p = 0; p < rbn::kNudgeBatches; p++) e->ntot[p] = pre[(size_t)p * (nm + 1) + nm];
+  std::vector<int4> map;
+  for (int p = 0; p < rbn::kNudgeBatches; p++) {
+    e->plan.off[p] = (int)map.size();
+    for (int q = 0; q < nm; q++) {
+      const int nb = blocks[(size_t)q * rbn::kNudgeBatches + p];
+      for (int b = 0; b < nb; b++) map.push_back(make_int4(q, b, nb, p));
+    }
+    e->plan.tot[p] = (int)map.size() - e->plan.off[p];
+  }
+  e->plan.nm = nm;
   if (!e->d_nmembers) EHIP(e, hipMalloc(&e->d_nmembers, d.size() * sizeof(rbn::NudgeMember)));
-  if (!e->d_npre) EHIP(e, hipMalloc((void **)&e->d_npre, pre.size() * sizeof(int)));
+  if (e->nmap_cap < map.size()) {
+    if (e->d_nmap) (void)hipFree(e->d_nmap);
+    e->d_nmap = nullptr;
+    e->nmap_cap = 0;
+    EHIP(e, hipMalloc((void **)&e->d_nmap, std::max<size_t>(map.size(), 1) * sizeof(int4)));
+    e->nmap_cap = map.size();
+  }
+  e->plan.map = e->d_nmap;
   EHIP(e, hipMemcpy(e->d_nmembers, d.data(), d.size() * sizeof(rbn::NudgeMember), hipMemcpyHostToDevice));
-  EHIP(e, hipMemcpy(e->d_npre, pre.data(), pre.size() * sizeof(int), hipMemcpyHostToDevice));
+  if (!map.empty()) EHIP(e, hipMemcpy(e->d_nmap, map.data(), map.size() * sizeof(int4), hipMemcpyHostToDevice));
   // A cycle of L steps replays the same kernel arguments when L is a multiple of the
   // velocity ring and of every record interval, in the two-step mode also of 4 (an even
   // number of launches, each flipping the A/B planes, an even number of them flipping the F
@@ -467,6 +486,11 @@ void advance_members(rbn_ensemble *e, int n, int launches, int pairs) {
 
 size_t align256(size_t b) { return (b + 255) / 256 * 256; }
 
+// The nudged members' descriptors of the deposit set of call step t.
+const rbn::NudgeMember *nudge_row(const rbn_ensemble *e, int t) {
+  return (const rbn::NudgeMember *)e->d_nmembers + (size_t)(t & 1) * e->nudged.size();
+}
+
 // The nudged members whose record interval starts at step t of a call (t counted from the
 // call's first step, their step counts those at its start), and the staging bytes of their
 // records: the jobs, then each member's 3n doubles and n-entry map, 256-aligned.
@@ -541,10 +565,8 @@ int load_records(rbn_ensemble *e, int t, hipStream_t s) {
 // after the lattice launch of step t - 1 (the last reader of its set) when that one is a
 // step of this run of eager steps (from t0; earlier work is ordered before it).
 int enqueue_deposit(rbn_ensemble *e, int t, int t0) {
-  const int nm = (int)e->nudged.size();
   if (t - 2 >= t0) EHIP(e, hipStreamWaitEvent(e->pstream, e->ev_lat[t & 1], 0));
-  EHIP(e, rbn::launch_nudge_batch(e->pstream, (const rbn::NudgeMember *)e->d_nmembers + (size_t)(t & 1) * nm,
-                                  e->d_npre, e->ntot, nm, t));
+  EHIP(e, rbn::launch_nudge_batch(e->pstream, nudge_row(e, t), e->plan, t));
   EHIP(e, hipEventRecord(e->ev_dep[t & 1], e->pstream));
   return RBN_OK;
 }
@@ -571,14 +593,16 @@ int fill_cycle(rbn_ensemble *e, int k, int r) {
 
 // Two-step mode with nudged members: the records and deposits of a launch's steps (step t
 // into set t & 1, t + 1 into set (t + 1) & 1) run on stream before it, after the previous
-// launch, which read both sets.
+// launch, which read both sets.  The first step's bilinear tile passes ride in the second
+// step's first launch.
 int enqueue_nudged_pair(rbn_ensemble *e, PairState &ps, int nsteps, bool moments) {
-  const int nm = (int)e->nudged.size();
-  for (int t = ps.t; t < std::min(ps.t + 2, nsteps); t++) {
+  const bool two = ps.t + 1 < nsteps;
+  for (int t = ps.t; t < ps.t + (two ? 2 : 1); t++) {
     int rc = load_records(e, t, e->stream);
     if (rc) return rc;
-    EHIP(e, rbn::launch_nudge_batch(e->stream, (const rbn::NudgeMember *)e->d_nmembers + (size_t)(t & 1) * nm,
-                                    e->d_npre, e->ntot, nm, t));
+    const bool second = t > ps.t;
+    EHIP(e, rbn::launch_nudge_batch(e->stream, nudge_row(e, t), e->plan, t, second ? nudge_row(e, t - 1) : nullptr,
+                                    two && !second));
   }
   return enqueue_pair(e, ps, nsteps, moments);
 }
@@ -595,21 +619,22 @@ int capture_nudged(rbn_ensemble *e) {
     int rc = ensure_stage(e, e->cstage[k], std::max<size_t>(e->cyc_bytes, 256));
     if (rc) return rc;
   }
-  const int nm = (int)e->nudged.size();
   for (int k = 0; k < 2; k++) {
     rbn_ensemble::Stage &st = e->cstage[k];
     EHIP(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
     int rc = RBN_OK;
     size_t next = 0;
-    auto deposit = [&](int t) -> int {
+    // tail: the previous step's tile passes ride along (two-step mode, a pair's second step);
+    // defer: this step's wait for the next (a pair's first step)
+    auto deposit = [&](int t, bool tail, bool defer) -> int {
       if (next < e->cyc.size() && e->cyc[next].t == t) {
         const auto &cl = e->cyc[next++];
         if (rbn::launch_obs_scatter(e->stream, (const rbn::ObsJob *)(st.d + cl.off), (int)cl.ld.size(),
                                     st.d + cl.off) != hipSuccess)
           return efail(e, RBN_EDEVICE, "record scatter capture failed");
       }
-      if (rbn::launch_nudge_batch(e->stream, (const rbn::NudgeMember *)e->d_nmembers + (size_t)(t & 1) * nm,
-                                  e->d_npre, e->ntot, nm, t) != hipSuccess)
+      if (rbn::launch_nudge_batch(e->stream, nudge_row(e, t), e->plan, t, tail ? nudge_row(e, t - 1) : nullptr,
+                                  defer) != hipSuccess)
         return efail(e, RBN_EDEVICE, "observer/deposit capture failed");
       return RBN_OK;
     };
@@ -621,11 +646,12 @@ int capture_nudged(rbn_ensemble *e) {
       ps.P = 1;
       ps.pending = 2;
       for (int l = 0; l < e->ncycle / 2 && !rc; l++) {
-        if (!(rc = deposit(ps.t)) && !(rc = deposit(ps.t + 1))) rc = enqueue_pair(e, ps, 1 << 30, false);
+        if (!(rc = deposit(ps.t, false, true)) && !(rc = deposit(ps.t + 1, true, false)))
+          rc = enqueue_pair(e, ps, 1 << 30, false);
       }
     } else {
       for (int t = 1; t <= e->ncycle && !rc; t++)
-        if (!(rc = deposit(t))) rc = enqueue_fused(e, t, false);
+        if (!(rc = deposit(t, false, false))) rc = enqueue_fused(e, t, false);
     }
     hipGraph_t g = nullptr;
     hipError_t ee = hipStreamEndCapture(e->stream, &g);
@@ -938,7 +964,7 @@ void rbn_ensemble_destroy(rbn_ensemble *e) {
   if (e->d_items) (void)hipFree(e->d_items);
   if (e->d_counters) (void)hipFree(e->d_counters);
   if (e->d_nmembers) (void)hipFree(e->d_nmembers);
-  if (e->d_npre) (void)hipFree(e->d_npre);
+  if (e->d_nmap) (void)hipFree(e->d_nmap);
   for (auto *set : {&e->stage[0], &e->stage[1], &e->cstage[0], &e->cstage[1]}) {
     if (set->h) (void)hipHostFree(set->h);
     if (set->d) (void)hipFree(set->d);

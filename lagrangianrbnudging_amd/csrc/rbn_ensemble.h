@@ -31,15 +31,14 @@ struct EnsMember {
   int np;
   int *err;
   int *bad;  // non-finite moments flag (the member's d_err[1])
-  // nudged members (variant RBN_VARIANT_NUDGING): the deposits the steps read, [k & 1] for
-  // step k (1: the member's own arrays, 0: the ensemble's second set, so that the deposit of
-  // step k + 1 can be built while step k runs, or both deposits of a two-step launch exist),
-  // and the persistent nudging term, read and written in place by one-step launches and
-  // swapped between nud[0] and nud[1] by two-step launches (with the force planes); wq only
-  // for the bilinear deposit
+  // nudged members (variant RBN_VARIANT_NUDGING): the deposits the steps read (packed,
+  // DepPtrs::tw), [k & 1] for step k (1: the member's own set, 0: the ensemble's second set,
+  // so that the deposit of step k + 1 can be built while step k runs, or both deposits of a
+  // two-step launch exist), and the persistent nudging term, read and written in place by
+  // one-step launches and swapped between nud[0] and nud[1] by two-step launches (with the
+  // force planes)
   int variant;
-  const int *nvol[2];
-  const double *Tbar[2], *wq[2];
+  const void *tw[2];
   R *nud[2];
 };
 

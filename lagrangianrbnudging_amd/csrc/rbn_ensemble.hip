@@ -54,9 +54,7 @@ __device__ __forceinline__ void lbm_item(const EnsMember<R> &m, int tile, int k,
   p.bad = m.bad;
   const int col = threadIdx.x % tile_w;
   if (m.variant == RBN_VARIANT_NUDGING) {
-    p.nvol = m.nvol[dset];
-    p.Tbar = m.Tbar[dset];
-    p.wq = m.wq[dset];
+    p.tw = m.tw[dset];
     p.nud = nud;
     for (int r = threadIdx.x / tile_w; r < tile_h; r += kThreads / tile_w) {
       const int i = tx * tile_w + col, j = ty * tile_h + r;
@@ -215,12 +213,7 @@ __device__ __forceinline__ void pair_item(const EnsMember<R> &m, int tile, const
   p.bad = m.bad;
   if constexpr (KIND != kPairsTruth) {
     if (KIND == kPairsNudged || m.variant == RBN_VARIANT_NUDGING) {
-      for (int u = 0; u < 2; u++) {
-        const int ds = (st.k + u) & 1;
-        p.nv[u] = m.nvol[ds];
-        p.tb[u] = m.Tbar[ds];
-        p.wq[u] = m.wq[ds];
-      }
+      for (int u = 0; u < 2; u++) p.tw[u] = m.tw[(st.k + u) & 1];
       p.nud_in = m.nud[st.fpar];
       p.nud_out = m.nud[1 - st.fpar];
       lbm2::two_step_tile<R, TH, RBN_VARIANT_NUDGING, GR>(m.g, p, m.c, tx, ty, L);

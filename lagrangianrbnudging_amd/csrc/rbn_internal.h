@@ -6,6 +6,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/rbn.h"
@@ -50,12 +51,39 @@ struct LbmPtrs {
   void *F;           // lagged F_y: read F(s), write F(s+1) (one value per node)
   void *uv;          // hydrovar velocity of this step, (u, v) interleaved per node
   void *rho, *T;     // written when non-null (record/diagnostic/last steps)
-  const int *nvol;   // nudged run: nudge_vol counts (nullptr: not nudged)
+  const int *nvol;   // nudged run: nudge_vol counts (nullptr: not nudged, unless tw is set)
   const double *Tbar;// nudged run: tempreadint
   void *nud;         // nudged run: persistent nud_term
   const double *wq;  // nudged run, bilinear deposit: per-node weight W (Q = alpha*(W*(T-Tbar)))
+  const void *tw;    // nudged run: the deposit packed per node (DepPtrs::tw); read instead of
+                     // nvol, Tbar and wq when set
   int *bad;          // set to 1 when a node's rho or T is not finite (or null: no check)
 };
+
+// Strength of the bilinear deposit's nudging at a node whose observer weights sum to W:
+// min(W, 1), so that alpha <= 1 never over-relaxes a node that several observers reach.
+__host__ __device__ inline double nudge_weight(double W) { return W < 1.0 ? W : 1.0; }
+
+// Two reals of the lattice's precision.
+template <typename R>
+using Real2 = typename std::conditional<sizeof(R) == 4, float2, double2>::type;
+
+// The nudging input of node o: from the packed deposit p.tw when set, else from the counts,
+// means and (bilinear deposit) weights; false where the node has no contributions.  Tb and w
+// are the mean and weight rounded to R, w = 1 for the 7x7 deposit.
+template <typename R>
+__device__ __forceinline__ bool nudge_input(const LbmPtrs &p, size_t o, R &Tb, R &w) {
+  if (p.tw) {
+    const Real2<R> v = ((const Real2<R> *)p.tw)[o];
+    Tb = v.x;
+    w = -v.y;
+    return __builtin_signbit(v.y);
+  }
+  if (p.nvol[o] <= 0) return false;
+  Tb = (R)p.Tbar[o];
+  w = p.wq ? (R)nudge_weight(p.wq[o]) : R(1);
+  return true;
+}
 
 // Particle kernel arguments (device pointers).
 struct PartPtrs {
@@ -91,10 +119,44 @@ struct DepPtrs {
   int *nactive;  // their number
   long long active_cap;
   int dense;     // many contributions per node: fold every node in row order, no active list
-  // bilinear deposit: per-node fixed-point sums of w and w*T (2^-40 units), zero between
-  // deposits (the pass that turns them into Tbar and wq clears them); null for the 7x7 deposit
+  // The deposit as the nudged collision reads it, one Real2 per node in the lattice's
+  // precision (tw64: double2, else float2): x = Tbar, y = -(the weight w, min(W, 1) for the
+  // bilinear deposit, 1 for the 7x7 one) where the node has contributions, so that the sign
+  // bit of y marks them (w = 0 gives -0.0); both +0.0 where it has none (zeroed memory is an
+  // empty deposit).  Q = alpha * (w * (T - Tbar)) then equals the unpacked expressions.
+  void *tw;
+  int tw64;
+  // bilinear deposit: contributions binned by deposit tile (kDepTileW x kDepTileH nodes):
+  // per tile the number binned (tcnt[tile * kTcntStride], zero between deposits) and bin_cap
+  // slots of two long longs
+  // (w T and, shifted left by kDepTileBits, w in fixed point, with the node's index in the
+  // tile in the low bits).  Contributions past a full tile's slots go to the fixed-point sums
+  // S, Wi (2^-40 units) and the count ocnt at their node, zero between deposits.  Null for the
+  // 7x7 deposit.
+  int *tcnt;
+  long long *bin;
+  int bin_cap;
   long long *S, *Wi;
+  int *ocnt;
 };
+
+// Deposit tiles of the bilinear deposit: 64 x 8 nodes, one workgroup each.  Their counters
+// lie kTcntStride ints apart (256 bytes), so that the atomics on different tiles' counters do
+// not queue on a few lines.
+constexpr int kDepTileW = 64, kDepTileH = 8, kDepTileBits = 9, kTcntStride = 64;
+static_assert(kDepTileW * kDepTileH == 1 << kDepTileBits, "a tile index fits kDepTileBits");
+__host__ __device__ inline int dep_tiles_x(const Geom &g) { return (g.nx + kDepTileW - 1) / kDepTileW; }
+__host__ __device__ inline int dep_tiles(const Geom &g) {
+  return dep_tiles_x(g) * ((g.ny + kDepTileH - 1) / kDepTileH);
+}
+// Slots per tile for n observers (four contributions each) spread over the lattice: twice the
+// mean and 64 more, a multiple of 64, at most 4096; a tile that receives more takes its
+// overflow through the fixed-point sums.
+inline int dep_bin_cap(const Geom &g, long long n) {
+  const long long mean = 4 * n / dep_tiles(g) + 1;
+  const long long cap = (2 * mean + 64 + 63) / 64 * 64;
+  return (int)(cap < 4096 ? cap : 4096);
+}
 
 // One nudged member of a batched observer/deposit launch (ensembles): its lattice, the
 // deposit's right limit, record interval, the step count before the launching call, its
@@ -120,7 +182,7 @@ constexpr int kDepScanTile = 1024;
 // Passes with per-member block counts (the prefix rows of a batched launch): observer
 // update, zero pass, candidates (count or accumulate), count scan and list fill (7x7
 // deposit only), fold or finalize.
-enum { NB_OBS, NB_ZERO, NB_CAND, NB_SCAN, NB_FILL, NB_FOLD, kNudgeBatches };
+enum { NB_OBS, NB_ZERO, NB_CAND, NB_SCAN, NB_FILL, NB_FOLD, NB_TILES, kNudgeBatches };
 
 // Launch shape of the fused step: block width (64/128/256/512 nodes), non-temporal stores of
 // the outputs / loads of the inputs, XCD-banded block order, alternating sweep direction.
@@ -160,10 +222,6 @@ struct PartJob {
   PartPtrs pp{};
 };
 
-// Strength of the bilinear deposit's nudging at a node whose observer weights sum to W:
-// min(W, 1), so that alpha <= 1 never over-relaxes a node that several observers reach.
-__host__ __device__ inline double nudge_weight(double W) { return W < 1.0 ? W : 1.0; }
-
 // Non-finite state: a node whose moments are not finite sets *bad (a NaN or infinity in any
 // population it pulls makes its rho or T NaN or infinite).  Only the store is conditional, so
 // the check costs two compares per node and no traffic.
@@ -178,12 +236,11 @@ void adj_free(rbn_ctx *c);
 // Launchers (defined in the .hip files).
 hipError_t launch_lbm(hipStream_t s, int prec, int variant, const Geom &g, const LbmPtrs &p,
                       const void *step_const, const LbmTuning &t);
-// The second step's deposit of a two-step nudged launch, and the plane that receives the
-// persisted nudging term after it (the first step's deposit and the term entering it are the
-// LbmPtrs' nvol, Tbar, wq and nud).
+// The second step's deposit (packed, DepPtrs::tw) of a two-step nudged launch, and the plane
+// that receives the persisted nudging term after it (the first step's deposit and the term
+// entering it are the LbmPtrs' tw and nud).
 struct NudgePair {
-  const int *nvol2;
-  const double *Tbar2, *wq2;
+  const void *tw2;
   void *nud_out;
 };
 // Two steps of K1 in one launch (rbn_lbm2.hip): p holds the first step's inputs and the second
@@ -210,15 +267,29 @@ hipError_t launch_particle_init(hipStream_t s, int prec, const Geom &g, const vo
                                 const PartPtrs &pp, double heavy_factor);
 // One nudged step's observer update (branch: 1 first step of a record interval, 0 its last,
 // 2 the others) and its deposit (counts, scan, contribution lists, fold).
+// tail (or null): a bilinear deposit of the same lattice whose tile pass was deferred, run in
+// the first launch (a 7x7 set is never one: it has no bins, and is ignored); defer: leave
+// this deposit's tile pass (bilinear; a 7x7 deposit always completes) to a later call's
+// tail, which must come before anything reads the deposit.
 hipError_t launch_obs_deposit(hipStream_t s, int branch, int ntempread, const Geom &g, int right_limit,
-                              const ObsPtrs &o, const DepPtrs &d);
+                              const ObsPtrs &o, const DepPtrs &d, const DepPtrs *tail = nullptr,
+                              bool defer = false);
 // Blocks member d takes in each pass of a batched observer/deposit launch.
 void nudge_batch_blocks(const NudgeMember &d, int blocks[kNudgeBatches]);
-// Observer update and deposit of step t of a call (t = 0 is the first) for nm members at
-// once: members (device), pre (device, kNudgeBatches rows of nm + 1 block prefixes), totals
-// (host, the last entry of each row).
-hipError_t launch_nudge_batch(hipStream_t s, const NudgeMember *members, const int *pre, const int *totals, int nm,
-                              int t);
+// The block map of the batched passes: for every block of pass p (entries off[p] ..
+// off[p] + tot[p] - 1 of map, device memory, passes in order) its member, its index among
+// that member's blocks of the pass, their number, and p.
+struct NudgePlan {
+  const int4 *map;
+  int off[kNudgeBatches], tot[kNudgeBatches];
+  int nm;
+};
+// Observer update and deposit of step t of a call (t = 0 is the first) for plan.nm members
+// at once (members: device descriptors of the deposit set of step t).  tail (or null): the
+// descriptors of the previous step's set, whose bilinear tile passes were deferred; defer:
+// leave this step's bilinear tile passes to the next call's tail.
+hipError_t launch_nudge_batch(hipStream_t s, const NudgeMember *members, const NudgePlan &plan, int t,
+                              const NudgeMember *tail = nullptr, bool defer = false);
 // Scatter njobs staged records (jobs and data in the device staging buffer) to their observers.
 hipError_t launch_obs_scatter(hipStream_t s, const ObsJob *jobs, int njobs, const unsigned char *staging);
 hipError_t launch_energy(hipStream_t s, int prec, const Geom &g, const void *rho, const void *uv,

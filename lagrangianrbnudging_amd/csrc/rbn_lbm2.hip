@@ -179,18 +179,10 @@ hipError_t launch_lbm2(hipStream_t s, int prec, const Geom &g, const LbmPtrs &p,
   a.p.rho2 = rho2;
   a.p.T2 = T2;
   a.p.bad = p.bad;
-  a.p.nv[0] = a.p.nv[1] = nullptr;
-  a.p.tb[0] = a.p.tb[1] = a.p.wq[0] = a.p.wq[1] = nullptr;
-  a.p.nud_in = nullptr;
-  a.p.nud_out = nullptr;
   if (np) {
-    a.p.nv[0] = p.nvol;
-    a.p.tb[0] = p.Tbar;
-    a.p.wq[0] = p.wq;
+    a.p.tw[0] = p.tw;
     a.p.nud_in = p.nud;
-    a.p.nv[1] = np->nvol2;
-    a.p.tb[1] = np->Tbar2;
-    a.p.wq[1] = np->wq2;
+    a.p.tw[1] = np->tw2;
     a.p.nud_out = np->nud_out;
   }
   a.tiles_x = (g.nx + kTW - 1) / kTW;

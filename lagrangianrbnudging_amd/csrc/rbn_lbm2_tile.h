@@ -194,17 +194,18 @@ __device__ __forceinline__ void collide32(const StepConst<float> &c, const float
 
 // The nudging of one node in a step of the nudged run: its deposit count, mean and (bilinear
 // deposit) weight, and the persisted term entering the step.
+// A node's nudging input (nudge_input): on where it has contributions, the mean and weight.
+template <typename R>
 struct NodeNudge {
-  int nv;
-  double Tbar, W;
-  bool bil;
+  bool on;
+  R Tb, w;
 };
 
 // Moments and collision of a nudged-run node: lbm_node64's nudged arithmetic (half the
 // persisted term in T where the node receives a deposit; the nudged collisg), operand for
 // operand.  nud: the term entering the step; nud_out: after it (unchanged without a deposit).
 __device__ __forceinline__ void collide_n64(const StepConst<double> &c, const double f[9], const double q[9],
-                                            double Fy, const NodeNudge &d, double nud, double fo[9], double go[9],
+                                            double Fy, const NodeNudge<double> &d, double nud, double fo[9], double go[9],
                                             double &rho, double &T, double &u, double &v, double &Fn,
                                             double &nud_out) {
   rho = 0.0;
@@ -213,7 +214,7 @@ __device__ __forceinline__ void collide_n64(const StepConst<double> &c, const do
   for (int k = 0; k < 9; k++) rho = rho + f[k];
 #pragma unroll
   for (int k = 0; k < 9; k++) T = T + q[k];
-  if (d.nv > 0) T = (T + 0.5 * nud);
+  if (d.on) T = (T + 0.5 * nud);
   const double rhoi = 1. / rho;
   double fl[9];
 #pragma unroll
@@ -230,8 +231,8 @@ __device__ __forceinline__ void collide_n64(const StepConst<double> &c, const do
     const double feq = rho * kW[k] * b[k];
     fo[k] = f[k] * c.one_m_om + c.om * feq + c.src[k] * fac[k];
   }
-  if (d.nv > 0) {
-    const double Q = d.bil ? c.alpha * (nudge_weight(d.W) * (T - d.Tbar)) : c.alpha * (T - d.Tbar);
+  if (d.on) {
+    const double Q = c.alpha * (d.w * (T - d.Tb));
 #pragma unroll
     for (int k = 0; k < 9; k++) {
       const double geq = T * kW[k] * b[k];
@@ -250,14 +251,14 @@ __device__ __forceinline__ void collide_n64(const StepConst<double> &c, const do
 
 // lbm_node32's nudged arithmetic with shifted populations.
 __device__ __forceinline__ void collide_n32(const StepConst<float> &c, const float f[9], const float q[9], float Fy,
-                                            const NodeNudge &d, float nud, float fo[9], float go[9], float &rho,
+                                            const NodeNudge<float> &d, float nud, float fo[9], float go[9], float &rho,
                                             float &T, float &u, float &v, float &Fn, float &nud_out) {
   float dr = 0.f, dT = 0.f;
 #pragma unroll
   for (int k = 0; k < 9; k++) dr += f[k];
 #pragma unroll
   for (int k = 0; k < 9; k++) dT += q[k];
-  if (d.nv > 0) dT += 0.5f * nud;
+  if (d.on) dT += 0.5f * nud;
   rho = c.rho0 + dr;
   T = c.T0 + dT;
   const float rhoi = 1.f / rho;
@@ -277,8 +278,8 @@ __device__ __forceinline__ void collide_n32(const StepConst<float> &c, const flo
   for (int k = 0; k < 9; k++) fo[k] = f[k] * c.one_m_om + c.om * feq[k] + c.src[k] * fac[k];
   float Q = 0.f;
   nud_out = nud;
-  if (d.nv > 0) {
-    Q = d.bil ? c.alpha * ((float)nudge_weight(d.W) * (T - (float)d.Tbar)) : c.alpha * (T - (float)d.Tbar);
+  if (d.on) {
+    Q = c.alpha * (d.w * (T - d.Tb));
     nud_out = -Q;
   }
   eq_pops32(dT, T, A, geq);
@@ -292,12 +293,12 @@ __device__ __forceinline__ void collide(const StepConst<double> &c, const double
   collide64(c, f, q, Fy, fo, go, rho, T, u, v, Fn);
 }
 __device__ __forceinline__ void collide_n(const StepConst<double> &c, const double f[9], const double q[9],
-                                          double Fy, const NodeNudge &d, double nud, double fo[9], double go[9],
+                                          double Fy, const NodeNudge<double> &d, double nud, double fo[9], double go[9],
                                           double &rho, double &T, double &u, double &v, double &Fn, double &nud_out) {
   collide_n64(c, f, q, Fy, d, nud, fo, go, rho, T, u, v, Fn, nud_out);
 }
 __device__ __forceinline__ void collide_n(const StepConst<float> &c, const float f[9], const float q[9], float Fy,
-                                          const NodeNudge &d, float nud, float fo[9], float go[9], float &rho,
+                                          const NodeNudge<float> &d, float nud, float fo[9], float go[9], float &rho,
                                           float &T, float &u, float &v, float &Fn, float &nud_out) {
   collide_n32(c, f, q, Fy, d, nud, fo, go, rho, T, u, v, Fn, nud_out);
 }
@@ -316,11 +317,10 @@ struct TwoStepPtrs {
   void *uv1, *uv2;      // velocity slots of steps s and s+1
   void *rho2, *T2;      // moments of step s+1, or null
   int *bad;             // non-finite flag of the second step's moments, or null
-  // nudged run: the deposits of steps s and s+1 (counts, means, bilinear weights or null) and
-  // the persisted nudging term entering step s and after step s+1 (double-buffered: other
-  // tiles still read the input at the nodes of this tile's ring)
-  const int *nv[2];
-  const double *tb[2], *wq[2];
+  // nudged run: the deposits of steps s and s+1 (packed, DepPtrs::tw) and the persisted
+  // nudging term entering step s and after step s+1 (double-buffered: other tiles still read
+  // the input at the nodes of this tile's ring)
+  const void *tw[2];
   const void *nud_in;
   void *nud_out;
 };
@@ -415,13 +415,10 @@ __device__ __forceinline__ void keep_node(TileLds<R, TH, GR> &L, int r, int cc, 
 }
 
 // The nudging of node o in step u of the launch (0: s, 1: s+1).
-__device__ __forceinline__ NodeNudge node_nudge(const TwoStepPtrs &a, int u, size_t o) {
-  NodeNudge d;
-  d.nv = a.nv[u][o];
-  d.bil = a.wq[u] != nullptr;
-  d.Tbar = d.nv > 0 ? a.tb[u][o] : 0.0;
-  d.W = d.nv > 0 && d.bil ? a.wq[u][o] : 0.0;
-  return d;
+template <typename R>
+__device__ __forceinline__ NodeNudge<R> node_nudge(const TwoStepPtrs &a, int u, size_t o) {
+  const Real2<R> v = ((const Real2<R> *)a.tw[u])[o];
+  return {__builtin_signbit(v.y) != 0, v.x, -v.y};
 }
 
 // Step s at local node (r, cc) of tile (i0, j0), pulled from the input planes in HBM; its
@@ -450,10 +447,10 @@ __device__ __forceinline__ void first_step(const Geom &g, const TwoStepPtrs &a, 
   const R Fy = ((const R *)a.Fin)[o];
   if (V == RBN_VARIANT_NUDGING) {
     gather_nudged<R>(acc, bot, top, y == 1, y == g.ny - 2, f, q);
-    const NodeNudge d = node_nudge(a, 0, o);
-    const R nud_in = d.nv > 0 ? ((const R *)a.nud_in)[o] : R(0);
+    const NodeNudge<R> d = node_nudge<R>(a, 0, o);
+    const R nud_in = d.on ? ((const R *)a.nud_in)[o] : R(0);
     collide_n(c, f, q, Fy, d, nud_in, fo, go, rho, T, u, v, Fn, nud);
-    if (tile && d.nv <= 0) nud = ((const R *)a.nud_in)[o];  // persists where no deposit lands
+    if (tile && !d.on) nud = ((const R *)a.nud_in)[o];  // persists where no deposit lands
   } else {
     gather_truth<R>(acc, bot, top, c, f, q);
     collide(c, f, q, Fy, fo, go, rho, T, u, v, Fn);
@@ -524,7 +521,7 @@ __device__ __forceinline__ void two_step_tile(const Geom &g, const TwoStepPtrs &
     if (V == RBN_VARIANT_NUDGING) {
       gather_nudged<R>(acc, bot, top, y == 1, y == g.ny - 2, f, q9);
       R nud;
-      collide_n(c, f, q9, Fk[q], node_nudge(a, 1, o), Nk[q], fo, go, rho, T, u, v, Fn, nud);
+      collide_n(c, f, q9, Fk[q], node_nudge<R>(a, 1, o), Nk[q], fo, go, rho, T, u, v, Fn, nud);
       ((R *)a.nud_out)[o] = nud;
     } else {
       gather_truth<R>(acc, bot, top, c, f, q9);

@@ -209,10 +209,11 @@ __device__ __forceinline__ void lbm_node64(const Geom &g, const LbmPtrs &p, cons
   for (int k = 0; k < 9; k++) rho = rho + f[k];
 #pragma unroll
   for (int k = 0; k < 9; k++) T = T + q[k];
-  int nv = 0;
-  if (VARIANT == RBN_VARIANT_NUDGING && p.nvol) {
-    nv = p.nvol[r0 + i];
-    if (nv > 0) T = (T + 0.5 * ((const double *)p.nud)[r0 + i]);
+  bool nv = false;
+  double Tb = 0.0, wn = 0.0;
+  if (VARIANT == RBN_VARIANT_NUDGING && (p.tw || p.nvol)) {
+    nv = nudge_input<double>(p, r0 + i, Tb, wn);
+    if (nv) T = (T + 0.5 * ((const double *)p.nud)[r0 + i]);
   }
   flag_nonfinite(p.bad, rho, T);
   const double rhoi = 1. / rho;
@@ -239,9 +240,9 @@ __device__ __forceinline__ void lbm_node64(const Geom &g, const LbmPtrs &p, cons
     const double feq = rho * kW[k] * b[k];
     st<NT>(&fB[(size_t)k * g.plane + r0 + i], f[k] * c.one_m_om + c.om * feq + c.src[k] * fac[k]);
   }
-  if (VARIANT == RBN_VARIANT_NUDGING && nv > 0) {
-    const double Q = p.wq ? c.alpha * (nudge_weight(p.wq[r0 + i]) * (T - p.Tbar[r0 + i]))
-                          : c.alpha * (T - p.Tbar[r0 + i]);
+  if (VARIANT == RBN_VARIANT_NUDGING && nv) {
+    // w = 1 (7x7 deposit) leaves alpha * (T - Tbar) unchanged
+    const double Q = c.alpha * (wn * (T - Tb));
 #pragma unroll
     for (int k = 0; k < 9; k++) {
       const double geq = T * kW[k] * b[k];
@@ -380,10 +381,11 @@ __device__ __forceinline__ void lbm_node32(const Geom &g, const LbmPtrs &p, cons
   for (int k = 0; k < 9; k++) dr += f[k];
 #pragma unroll
   for (int k = 0; k < 9; k++) dT += q[k];
-  int nv = 0;
-  if (VARIANT == RBN_VARIANT_NUDGING && p.nvol) {
-    nv = p.nvol[r0 + i];
-    if (nv > 0) dT += 0.5f * ((const float *)p.nud)[r0 + i];
+  bool nv = false;
+  float Tb = 0.f, wn = 0.f;
+  if (VARIANT == RBN_VARIANT_NUDGING && (p.tw || p.nvol)) {
+    nv = nudge_input<float>(p, r0 + i, Tb, wn);
+    if (nv) dT += 0.5f * ((const float *)p.nud)[r0 + i];
   }
   const float rho = c.rho0 + dr;
   const float T = c.T0 + dT;
@@ -413,9 +415,8 @@ __device__ __forceinline__ void lbm_node32(const Geom &g, const LbmPtrs &p, cons
   for (int k = 0; k < 9; k++)
     st<NT>(&fB[(size_t)k * g.plane + r0 + i], f[k] * c.one_m_om + c.om * feq[k] + c.src[k] * fac[k]);
   float Q = 0.f;
-  if (VARIANT == RBN_VARIANT_NUDGING && nv > 0) {
-    Q = p.wq ? c.alpha * ((float)nudge_weight(p.wq[r0 + i]) * (T - (float)p.Tbar[r0 + i]))
-             : c.alpha * (T - (float)p.Tbar[r0 + i]);
+  if (VARIANT == RBN_VARIANT_NUDGING && nv) {
+    Q = c.alpha * (wn * (T - Tb));
     ((float *)p.nud)[r0 + i] = -Q;
   }
   eq_pops32(dT, T, A, geq);

@@ -29,11 +29,17 @@
 // reference's running mean): each observer gives the four corners of its interpolation cell
 // the weights the record interpolation uses, so the deposit is the transpose of that
 // operator.  Per node W = sum w_k and S = sum w_k T_k are taken in fixed point (every term
-// rounded to a multiple of 2^-40, 64-bit integer atomics), so the sums are exact and do not
-// depend on the order in which the contributions arrive: one pass accumulates them (and
-// lists the nodes reached), a second turns them into Tbar = S / W and the weight W and clears
-// them for the next deposit; the nudged collision uses Q = alpha * (min(W, 1) * (T - Tbar)).
-// The sums must stay below 2^23 in magnitude (|T| W < 8.4e6 at a node).
+// rounded to a multiple of 2^-40, 64-bit integer sums), so the sums are exact and do not
+// depend on the order in which the contributions arrive.  The observer update's thread bins
+// each contribution (w T, w and the node) into a slot of its node's deposit tile of 64 x 8
+// nodes (one atomic on the tile's counter); one workgroup per tile then sums its slots in LDS
+// and writes every node of the tile: the count, the packed deposit, and Tbar = S / W and W
+// where contributions arrived.  Contributions past a tile's slots take 64-bit atomics on the
+// node's sums in HBM instead, which the tile adds and clears.  The nudged collision uses
+// Q = alpha * (min(W, 1) * (T - Tbar)).  The sums must stay below 2^23 in magnitude
+// (|T| W < 8.4e6 at a node).  Random contributions thus cost one atomic on an L2-resident
+// counter and a 16-byte store each, where per-node atomics on the sums would each move a
+// line of HBM.
 #include <algorithm>
 
 #include "rbn_internal.h"
@@ -179,29 +185,35 @@ __device__ __forceinline__ long long bil_cell(const Geom &g, double xx, double y
 // to even (the scaling is exact).
 __device__ __forceinline__ long long dep_fix(double v) { return (long long)rint(v * 0x1p40); }
 
-// Zero the counts, the fill cursors (7x7 deposit; null for the bilinear one) and the
-// active-node count: block b of nb, grid-stride.
-__device__ __forceinline__ void zero_part(size_t n, int *cnt, int *fill, int *nactive, int b, int nb) {
-  if (b == 0 && threadIdx.x == 0) *nactive = 0;
+// The packed deposit (DepPtrs::tw) at node c: the mean and weight w (stored negated), or
+// the empty value.
+__device__ __forceinline__ void tw_put(const DepPtrs &d, size_t c, double Tbar, double w) {
+  if (d.tw64)
+    ((double2 *)d.tw)[c] = make_double2(Tbar, -w);
+  else
+    ((float2 *)d.tw)[c] = make_float2((float)Tbar, -(float)w);
+}
+__device__ __forceinline__ void tw_zero(const DepPtrs &d, size_t c) {
+  if (d.tw64)
+    ((double2 *)d.tw)[c] = make_double2(0.0, 0.0);
+  else
+    ((float2 *)d.tw)[c] = make_float2(0.f, 0.f);
+}
+
+// 7x7 deposit: zero the counts, the fill cursors, the packed deposit and the active-node
+// count: block b of nb, grid-stride.
+__device__ __forceinline__ void zero_part(size_t n, const DepPtrs &d, int b, int nb) {
+  if (b == 0 && threadIdx.x == 0) *d.nactive = 0;
   for (size_t o = (size_t)b * kBlock + threadIdx.x; o < n; o += (size_t)nb * kBlock) {
-    cnt[o] = 0;
-    if (fill) fill[o] = 0;
+    d.cnt[o] = 0;
+    d.fill[o] = 0;
+    tw_zero(d, o);
   }
 }
 
-// The step's observer update and the deposit's zero pass in one launch: they touch disjoint
-// arrays (observer state; counts, fill cursors and the active-node count), so blocks
-// [0, obs_blocks) update observers and the rest zero the counts.
-__global__ __launch_bounds__(kBlock) void k_obs_zero(ObsPtrs o, int branch, int nx, int ny, int ntempread,
-                                                     int obs_blocks, size_t n, int *cnt, int *fill, int *nactive) {
-  const int b = blockIdx.x;
-  if (b < obs_blocks) {
-    const int i = b * kBlock + threadIdx.x;
-    if (i < o.n) obs_update_one(o, i, branch, nx, ny, ntempread);
-  } else {
-    zero_part(n, cnt, fill, nactive, b - obs_blocks, gridDim.x - obs_blocks);
-  }
-}
+// Bilinear deposit: observer k's contributions to its cell's corners, each binned into a slot
+// of its node's deposit tile, or past the tile's slots added to the node's sums.
+__device__ __forceinline__ void bin_one(const Geom &g, const DepPtrs &d, const ObsPtrs &o, int k);
 
 // 7x7 deposit: candidate t (observer t / kCells, cell t % kCells) counts one contribution
 // at its node.
@@ -328,43 +340,140 @@ __global__ void k_dep_fill(Geom g, int right, ObsPtrs o, const int *off, const i
            d.nactive, d.active_cap, (long long)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
-// Bilinear deposit: candidate t (observer t / 4, corner t % 4) adds its weight and weighted
-// temperature to the node's fixed-point sums and counts one contribution; the node's first
-// contribution lists it as active (unless dense).  Every thread of the workgroup calls it.
-__device__ __forceinline__ void acc_one(const Geom &g, const ObsPtrs &o, const DepPtrs &d, long long t) {
-  long long node = -1;
-  double w = 0.0, wT = 0.0;
-  if (t < (long long)o.n * kBilCells) {
-    const int k = (int)(t / kBilCells);
-    node = bil_cell(g, o.xint[k], o.yint[k], (int)(t % kBilCells), w);
-    wT = w * o.Tint[k];
+// The corners that share a tile (all four but near a tile's edge) take their slots with one
+// atomic.
+__device__ __forceinline__ void bin_one(const Geom &g, const DepPtrs &d, const ObsPtrs &o, int k) {
+  const double xx = o.xint[k], yy = o.yint[k], Tk = o.Tint[k];
+  const int tiles_x = dep_tiles_x(g);
+  long long node[kBilCells], wf[kBilCells], wTf[kBilCells];
+  int tile[kBilCells], local[kBilCells];
+#pragma unroll
+  for (int c = 0; c < kBilCells; c++) {
+    double w = 0.0;
+    node[c] = bil_cell(g, xx, yy, c, w);
+    wf[c] = dep_fix(w);
+    wTf[c] = dep_fix(w * Tk);
+    const int j = node[c] >= 0 ? (int)(node[c] / g.pitch) : 0, i = node[c] >= 0 ? (int)(node[c] - (long long)j * g.pitch) : 0;
+    tile[c] = node[c] >= 0 ? (j / kDepTileH) * tiles_x + i / kDepTileW : -1;  // -1: no contribution
+    local[c] = (j % kDepTileH) * kDepTileW + i % kDepTileW;
   }
-  int first = 0;
-  if (node >= 0) {
-    first = atomicAdd(&d.cnt[node], 1) == 0;
-    atomicAdd((unsigned long long *)&d.S[node], (unsigned long long)dep_fix(wT));
-    atomicAdd((unsigned long long *)&d.Wi[node], (unsigned long long)dep_fix(w));
+#pragma unroll
+  for (int c = 0; c < kBilCells; c++) {
+    if (tile[c] < 0) continue;
+    const int t = tile[c];
+    int n = 0;
+#pragma unroll
+    for (int c2 = c; c2 < kBilCells; c2++) n += tile[c2] == t;
+    int slot = atomicAdd(&d.tcnt[(size_t)t * kTcntStride], n);
+#pragma unroll
+    for (int c2 = c; c2 < kBilCells; c2++) {
+      if (tile[c2] != t) continue;
+      tile[c2] = -1;  // taken
+      if (slot < d.bin_cap) {
+        // w in the high bits (times 2^kDepTileBits: a weight of either sign survives the
+        // arithmetic shift that takes it out), the node's index in the tile in the low ones
+        longlong2 e;
+        e.x = wTf[c2];
+        e.y = wf[c2] * (1LL << kDepTileBits) + local[c2];
+        ((longlong2 *)d.bin)[(size_t)t * d.bin_cap + slot] = e;
+      } else {
+        atomicAdd((unsigned long long *)&d.S[node[c2]], (unsigned long long)wTf[c2]);
+        atomicAdd((unsigned long long *)&d.Wi[node[c2]], (unsigned long long)wf[c2]);
+        atomicAdd(&d.ocnt[node[c2]], 1);
+      }
+      slot++;
+    }
   }
-  if (!d.dense) append_active(first, (int)node, d.active, d.nactive, d.active_cap);
 }
 
-__global__ __launch_bounds__(kBlock) void k_dep_acc(Geom g, ObsPtrs o, DepPtrs d) {
-  acc_one(g, o, d, (long long)blockIdx.x * blockDim.x + threadIdx.x);
+// Bilinear deposit, deposit tile t: the sums of its binned contributions in LDS (exact, in
+// any order), plus the sums in HBM when the tile overflowed (cleared here), then every node
+// of the tile: the count, the packed deposit, and Tbar = S / W and W (each sum converted to
+// double) where the count is positive.  Leaves the tile's counter at zero.  Every thread of
+// the workgroup calls it.
+struct DepTileLds {
+  unsigned long long S[kDepTileW * kDepTileH], W[kDepTileW * kDepTileH];
+  int n[kDepTileW * kDepTileH];
+};
+
+__device__ __forceinline__ void dep_tile(const Geom &g, const DepPtrs &d, int t, DepTileLds &L) {
+  constexpr int kNodes = kDepTileW * kDepTileH;
+  for (int k = threadIdx.x; k < kNodes; k += kBlock) {
+    L.S[k] = 0;
+    L.W[k] = 0;
+    L.n[k] = 0;
+  }
+  const int n = d.tcnt[(size_t)t * kTcntStride];
+  __syncthreads();
+  const int m = min(n, d.bin_cap);
+  const longlong2 *e = (const longlong2 *)d.bin + (size_t)t * d.bin_cap;
+  for (int k = threadIdx.x; k < m; k += kBlock) {
+    const longlong2 v = e[k];
+    const int local = (int)(v.y & (kNodes - 1));
+    atomicAdd(&L.S[local], (unsigned long long)v.x);
+    atomicAdd(&L.W[local], (unsigned long long)(v.y >> kDepTileBits));
+    atomicAdd(&L.n[local], 1);
+  }
+  __syncthreads();
+  const int tiles_x = dep_tiles_x(g), ty = t / tiles_x, tx = t - ty * tiles_x;
+  for (int k = threadIdx.x; k < kNodes; k += kBlock) {
+    const int i = tx * kDepTileW + k % kDepTileW, j = ty * kDepTileH + k / kDepTileW;
+    if (i >= g.nx || j >= g.ny) continue;
+    const size_t c = (size_t)j * g.pitch + i;
+    long long s = (long long)L.S[k], wi = (long long)L.W[k];
+    int cnt = L.n[k];
+    if (n > d.bin_cap && d.ocnt[c] != 0) {
+      cnt += d.ocnt[c];
+      s += d.S[c];
+      wi += d.Wi[c];
+      d.ocnt[c] = 0;
+      d.S[c] = 0;
+      d.Wi[c] = 0;
+    }
+    d.cnt[c] = cnt;
+    if (cnt > 0) {
+      const double Tbar = wi != 0 ? (double)s / (double)wi : 0.0, W = (double)wi * 0x1p-40;
+      d.Tbar[c] = Tbar;
+      d.wq[c] = W;
+      tw_put(d, c, Tbar, nudge_weight(W));
+    } else {
+      tw_zero(d, c);
+    }
+  }
+  if (threadIdx.x == 0) d.tcnt[(size_t)t * kTcntStride] = 0;  // every thread read it before the first barrier
 }
 
-// Bilinear deposit at node c with contributions: Tbar = S / W and W from the fixed-point
-// sums (each converted to double), which are cleared for the next deposit.
-__device__ __forceinline__ void fin_node(const DepPtrs &d, size_t c) {
-  const long long s = d.S[c], wi = d.Wi[c];
-  d.Tbar[c] = wi != 0 ? (double)s / (double)wi : 0.0;
-  d.wq[c] = (double)wi * 0x1p-40;
-  d.S[c] = 0;
-  d.Wi[c] = 0;
+// The step's observer update, and per deposit: 7x7, the zero pass (blocks past the
+// observers'; the two touch disjoint arrays); bilinear, each observer's contributions binned
+// by the thread that updated it (no zero pass: the tile pass leaves every array it reads
+// clear).  The first tail_blocks blocks run the tile pass of `tail`, an earlier bilinear
+// deposit whose bins are complete (it reads only its own set's arrays).
+__global__ __launch_bounds__(kBlock) void k_obs_zero(Geom g, ObsPtrs o, int branch, int ntempread, int tail_blocks,
+                                                     int obs_blocks, DepPtrs d, DepPtrs tail) {
+  __shared__ DepTileLds L;
+  const int b = blockIdx.x;
+  if (b < tail_blocks) {
+    dep_tile(g, tail, b, L);
+  } else if (b < tail_blocks + obs_blocks) {
+    const int i = (b - tail_blocks) * kBlock + threadIdx.x;
+    if (i < o.n) {
+      obs_update_one(o, i, branch, g.nx, g.ny, ntempread);
+      if (d.mode == RBN_DEPOSIT_BILINEAR) bin_one(g, d, o, i);
+    }
+  } else {
+    const int zb = tail_blocks + obs_blocks;
+    zero_part(g.plane, d, b - zb, gridDim.x - zb);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_dep_tiles(Geom g, DepPtrs d) {
+  __shared__ DepTileLds L;
+  dep_tile(g, d, blockIdx.x, L);
 }
 
 // Per node: order the contributions by observer and fold them in that order, the
-// reference's order.  7x7 deposit: the running mean (N:1578-1579).  Bilinear deposit:
-// W = sum w_k, S = sum w_k T_k, Tbar = S / W.  Lists are sorted in registers (odd-even
+// reference's order: the running mean of the 7x7 deposit (N:1578-1579).  Lists are sorted in
+// registers (odd-even
 // transposition network, static indices) sized by the longest list of the wavefront (2, 4
 // or 16 keys), longer ones in place.  The keys of a list are distinct (an observer reaches
 // a node at most once), so every sort gives the same order.
@@ -405,9 +514,10 @@ __device__ __forceinline__ int wave_max(int v) {
   return v;
 }
 
-// 7x7 deposit at node c with n > 0 contributions; nmax: the wavefront's longest list.
+// 7x7 deposit at node c with n > 0 contributions; nmax: the wavefront's longest list.  Writes
+// the mean and the packed deposit (weight 1).
 __device__ __forceinline__ void fold_node(const int *off, const int *tile_sum, int *list, const ObsPtrs &o,
-                                          double *Tbar, long long cap, size_t c, int n, int nmax) {
+                                          const DepPtrs &d, long long cap, size_t c, int n, int nmax) {
   Fold f{o};
   const long long base = n > 0 ? seg_start(off, tile_sum, c) : 0;
   if (n > 0 && base + n <= cap) {
@@ -431,38 +541,31 @@ __device__ __forceinline__ void fold_node(const int *off, const int *tile_sum, i
       for (int a = 0; a < n; a++) f.take(a, seg[a]);
     }
   }
-  Tbar[c] = f.t;
+  d.Tbar[c] = f.t;
+  tw_put(d, c, f.t, 1.0);
 }
 
-// The active nodes of a deposit, block b of nb, grid-stride: the 7x7 fold (every lane of a
-// wavefront takes part in the wavefront's maximum) or the bilinear finalisation.
+// The active nodes of a 7x7 deposit, block b of nb, grid-stride (every lane of a wavefront
+// takes part in the wavefront's maximum).
 __device__ __forceinline__ void fold_active(const ObsPtrs &o, const DepPtrs &d, int b, int nb) {
   const int na = (int)min((long long)*d.nactive, d.active_cap);
   for (int k0 = b * kBlock; k0 < na; k0 += nb * kBlock) {
     const int k = k0 + threadIdx.x;
     const size_t c = k < na ? (size_t)d.active[k] : 0;
-    if (d.mode == RBN_DEPOSIT_BILINEAR) {
-      if (k < na) fin_node(d, c);
-      continue;
-    }
     const int n = k < na ? d.cnt[c] : 0;
     const int nmax = wave_max(n);
-    if (k < na) fold_node(d.off, d.blocksum, d.list, o, d.Tbar, d.list_cap, c, n, nmax);
+    if (k < na) fold_node(d.off, d.blocksum, d.list, o, d, d.list_cap, c, n, nmax);
   }
 }
 
-// Row j of a deposit, nodes i0 .. i0 + kBlock - 1 (dense deposits): the nodes with
-// contributions are folded (7x7) or finalised (bilinear).
+// Row j of a dense 7x7 deposit, nodes i0 .. i0 + kBlock - 1: the nodes with contributions
+// are folded.
 __device__ __forceinline__ void fold_row(const Geom &g, const ObsPtrs &o, const DepPtrs &d, int i0, int j) {
   const int i = i0 + threadIdx.x;
   const size_t c = (size_t)j * g.pitch + i;
   const int n = i < g.nx ? d.cnt[c] : 0;
-  if (d.mode == RBN_DEPOSIT_BILINEAR) {
-    if (n > 0) fin_node(d, c);
-    return;
-  }
   const int nmax = wave_max(n);
-  if (n > 0) fold_node(d.off, d.blocksum, d.list, o, d.Tbar, d.list_cap, c, n, nmax);
+  if (n > 0) fold_node(d.off, d.blocksum, d.list, o, d, d.list_cap, c, n, nmax);
 }
 
 __global__ __launch_bounds__(kBlock) void k_dep_fold(ObsPtrs o, DepPtrs d) { fold_active(o, d, blockIdx.x, gridDim.x); }
@@ -474,26 +577,14 @@ __global__ __launch_bounds__(kBlock) void k_dep_fold_rows(Geom g, ObsPtrs o, Dep
 // ---------------------------------------------------------------- batched (ensembles)
 //
 // One launch per pass over the nudged members of an ensemble: the blocks of pass p are the
-// members' blocks in member order, member m owning blocks pre[m] .. pre[m+1]-1 of the pass's
-// prefix row pre = pre_all + p * (nm + 1).  Each block runs the single-member body of its
-// member with its block index inside the member.
-
-__device__ __forceinline__ int member_of(const int *pre, int nm, int b) {
-  int lo = 0, hi = nm;  // pre[lo] <= b < pre[hi]; members without blocks are skipped
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (pre[mid] <= b)
-      lo = mid;
-    else
-      hi = mid;
-  }
-  return lo;
-}
+// members' blocks in member order.  A block map (NudgePlan::map, built on the host) gives
+// each block its member, its index among the member's blocks of the pass and their number,
+// in one load; each block runs the single-member body of its member with that index.
 
 __device__ __forceinline__ int nudge_blocks(const Geom &g) { return (g.nx + kBlock - 1) / kBlock; }
 
 // Batched zero passes give each block more work than the single-member ones, so that the
-// member lookup and descriptor loads of a block start are spread over more nodes.
+// descriptor loads of a block start are spread over more nodes.
 constexpr int kZeroPerBlock = 16 * kBlock;  // counts zeroed per block
 
 // Blocks of a fold over a deposit's active nodes (at most kFoldBlocks, grid-stride).
@@ -501,46 +592,47 @@ inline int fold_blocks(const DepPtrs &d, int max_blocks) {
   return (int)std::max<long long>(1, std::min<long long>((d.active_cap + kBlock - 1) / kBlock, max_blocks));
 }
 
-// Observer updates of every member (the first NB_OBS total blocks), then the zero passes of
-// every member's deposit (disjoint arrays, as in k_obs_zero).
-__global__ __launch_bounds__(kBlock) void k_nb_obs_zero(const NudgeMember *M, const int *pre_all, int nm, int t) {
-  const int *pre_obs = pre_all + NB_OBS * (nm + 1);
-  const int nobs = pre_obs[nm];
-  if ((int)blockIdx.x < nobs) {
-    const int m = member_of(pre_obs, nm, blockIdx.x);
-    const NudgeMember &d = M[m];
-    const int i = (blockIdx.x - pre_obs[m]) * kBlock + threadIdx.x;
+// The head of a step's batched deposit: the tile passes of the bilinear members' previous
+// deposit when it was deferred (the first ntail blocks, members `tail`, map entries `tmap`;
+// those read only their own set's bins and write their own set), then every member's
+// observer update (bilinear members bin their contributions) and the 7x7 members' zero
+// passes (map entries `hmap`, pass NB_OBS or NB_ZERO).
+__global__ __launch_bounds__(kBlock) void k_nb_head(const NudgeMember *M, const NudgeMember *tail, const int4 *tmap,
+                                                    int ntail, const int4 *hmap, int t) {
+  __shared__ DepTileLds L;
+  if ((int)blockIdx.x < ntail) {
+    const int4 w = tmap[blockIdx.x];
+    dep_tile(tail[w.x].g, tail[w.x].d, w.y, L);
+    return;
+  }
+  const int4 w = hmap[blockIdx.x - ntail];
+  const NudgeMember &d = M[w.x];
+  if (w.w == NB_OBS) {
+    const int i = w.y * kBlock + threadIdx.x;
     // the step's place in the record interval, as the single-member loop derives it
     const int istep = d.istep0 + t + 1, r = istep % d.ntempread;
     const int branch = r == 1 ? 1 : (r == 0 ? 0 : 2);
-    if (i < d.o.n) obs_update_one(d.o, i, branch, d.g.nx, d.g.ny, d.ntempread);
+    if (i < d.o.n) {
+      obs_update_one(d.o, i, branch, d.g.nx, d.g.ny, d.ntempread);
+      if (d.d.mode == RBN_DEPOSIT_BILINEAR) bin_one(d.g, d.d, d.o, i);
+    }
   } else {
-    const int *pre = pre_all + NB_ZERO * (nm + 1);
-    const int zb = blockIdx.x - nobs, m = member_of(pre, nm, zb);
-    const DepPtrs &d = M[m].d;
-    zero_part(M[m].g.plane, d.cnt, d.mode == RBN_DEPOSIT_BILINEAR ? nullptr : d.fill, d.nactive, zb - pre[m],
-              pre[m + 1] - pre[m]);
+    zero_part(d.g.plane, d.d, w.y, w.z);
   }
 }
 
-// Counts (7x7 deposit) or fixed-point sums (bilinear deposit) of every member's candidates.
-__global__ __launch_bounds__(kBlock) void k_nb_count(const NudgeMember *M, const int *pre_all, int nm) {
-  const int *pre = pre_all + NB_CAND * (nm + 1);
-  const int m = member_of(pre, nm, blockIdx.x);
-  const NudgeMember &d = M[m];
-  const long long t = (long long)(blockIdx.x - pre[m]) * kBlock + threadIdx.x;
-  if (d.d.mode == RBN_DEPOSIT_BILINEAR)
-    acc_one(d.g, d.o, d.d, t);
-  else
-    count_one(d.g, d.right, d.o, d.d.cnt, t);
+// Counts of every 7x7 member's candidates.
+__global__ __launch_bounds__(kBlock) void k_nb_count(const NudgeMember *M, const int4 *map) {
+  const int4 w = map[blockIdx.x];
+  const NudgeMember &d = M[w.x];
+  count_one(d.g, d.right, d.o, d.d.cnt, (long long)w.y * kBlock + threadIdx.x);
 }
 
-__global__ __launch_bounds__(kBlock) void k_nb_scan_tiles(const NudgeMember *M, const int *pre_all, int nm) {
+__global__ __launch_bounds__(kBlock) void k_nb_scan_tiles(const NudgeMember *M, const int4 *map) {
   __shared__ int s[kBlock];
-  const int *pre = pre_all + NB_SCAN * (nm + 1);
-  const int m = member_of(pre, nm, blockIdx.x);
-  const NudgeMember &d = M[m];
-  scan_tile(d.g.plane, d.d.cnt, d.d.off, d.d.blocksum, blockIdx.x - pre[m], s);
+  const int4 w = map[blockIdx.x];
+  const NudgeMember &d = M[w.x];
+  scan_tile(d.g.plane, d.d.cnt, d.d.off, d.d.blocksum, w.y, s);
 }
 
 __global__ __launch_bounds__(kBlock) void k_nb_scan_sums(const NudgeMember *M) {
@@ -552,29 +644,32 @@ __global__ __launch_bounds__(kBlock) void k_nb_scan_sums(const NudgeMember *M) {
   scan_sums(ntiles, d.d.blocksum, d.d.off + d.g.plane, s, carry);
 }
 
-// The list fill of the 7x7 members (bilinear members have no blocks in this pass).
-__global__ __launch_bounds__(kBlock) void k_nb_fill(const NudgeMember *M, const int *pre_all, int nm) {
-  const int *pre = pre_all + NB_FILL * (nm + 1);
-  const int m = member_of(pre, nm, blockIdx.x);
-  const NudgeMember &d = M[m];
-  const long long t = (long long)(blockIdx.x - pre[m]) * kBlock + threadIdx.x;
+// The list fill of the 7x7 members.
+__global__ __launch_bounds__(kBlock) void k_nb_fill(const NudgeMember *M, const int4 *map) {
+  const int4 w = map[blockIdx.x];
+  const NudgeMember &d = M[w.x];
   const DepPtrs &p = d.d;
   fill_one(d.g, d.right, d.o, p.off, p.blocksum, p.fill, p.list, p.list_cap, p.overflow, p.dense ? nullptr : p.active,
-           p.nactive, p.active_cap, t);
+           p.nactive, p.active_cap, (long long)w.y * kBlock + threadIdx.x);
 }
 
-// The 7x7 fold or the bilinear finalisation of every member.
-__global__ __launch_bounds__(kBlock) void k_nb_fold(const NudgeMember *M, const int *pre_all, int nm) {
-  const int *pre = pre_all + NB_FOLD * (nm + 1);
-  const int m = member_of(pre, nm, blockIdx.x);
-  const NudgeMember &d = M[m];
-  const int lb = blockIdx.x - pre[m];
+// The 7x7 members' folds.
+__global__ __launch_bounds__(kBlock) void k_nb_fold(const NudgeMember *M, const int4 *map) {
+  const int4 w = map[blockIdx.x];
+  const NudgeMember &d = M[w.x];
   if (d.d.dense) {
-    const int bx = nudge_blocks(d.g), j = lb / bx, i0 = (lb - j * bx) * kBlock;
+    const int bx = nudge_blocks(d.g), j = w.y / bx, i0 = (w.y - j * bx) * kBlock;
     fold_row(d.g, d.o, d.d, i0, j);
   } else {
-    fold_active(d.o, d.d, lb, pre[m + 1] - pre[m]);
+    fold_active(d.o, d.d, w.y, w.z);
   }
+}
+
+// The bilinear members' tile passes.
+__global__ __launch_bounds__(kBlock) void k_nb_tiles(const NudgeMember *M, const int4 *map) {
+  __shared__ DepTileLds L;
+  const int4 w = map[blockIdx.x];
+  dep_tile(M[w.x].g, M[w.x].d, w.y, L);
 }
 
 }  // namespace
@@ -600,53 +695,68 @@ hipError_t launch_obs_scatter(hipStream_t s, const ObsJob *jobs, int njobs, cons
 }
 
 void nudge_batch_blocks(const NudgeMember &d, int blocks[kNudgeBatches]) {
-  const bool bil = d.d.mode == RBN_DEPOSIT_BILINEAR;
-  const long long nc = (long long)d.o.n * (bil ? kBilCells : kCells);
+  for (int p = 0; p < kNudgeBatches; p++) blocks[p] = 0;
   blocks[NB_OBS] = (d.o.n + kBlock - 1) / kBlock;
+  if (d.d.mode == RBN_DEPOSIT_BILINEAR) {  // binned in the observer pass, then the tile pass
+    blocks[NB_TILES] = dep_tiles(d.g);
+    return;
+  }
+  const long long nc = (long long)d.o.n * kCells;
   blocks[NB_ZERO] = (int)((d.g.plane + kZeroPerBlock - 1) / kZeroPerBlock);
   blocks[NB_CAND] = (int)((nc + kBlock - 1) / kBlock);
-  blocks[NB_SCAN] = bil ? 0 : (int)((d.g.plane + kScanTile - 1) / kScanTile);
-  blocks[NB_FILL] = bil ? 0 : blocks[NB_CAND];
+  blocks[NB_SCAN] = (int)((d.g.plane + kScanTile - 1) / kScanTile);
+  blocks[NB_FILL] = blocks[NB_CAND];
   blocks[NB_FOLD] = d.d.dense ? (d.g.nx + kBlock - 1) / kBlock * d.g.ny : fold_blocks(d.d, 256);
 }
 
-hipError_t launch_nudge_batch(hipStream_t s, const NudgeMember *members, const int *pre, const int *totals, int nm,
-                              int t) {
-  if (nm <= 0) return hipSuccess;
+hipError_t launch_nudge_batch(hipStream_t s, const NudgeMember *members, const NudgePlan &plan, int t,
+                              const NudgeMember *tail, bool defer) {
+  if (plan.nm <= 0) return hipSuccess;
   const dim3 b(kBlock);
-  hipLaunchKernelGGL(k_nb_obs_zero, dim3(totals[NB_OBS] + totals[NB_ZERO]), b, 0, s, members, pre, nm, t);
-  if (totals[NB_CAND] > 0) hipLaunchKernelGGL(k_nb_count, dim3(totals[NB_CAND]), b, 0, s, members, pre, nm);
-  if (totals[NB_SCAN] > 0) {  // the 7x7 members' count scan and list fill
-    hipLaunchKernelGGL(k_nb_scan_tiles, dim3(totals[NB_SCAN]), b, 0, s, members, pre, nm);
-    hipLaunchKernelGGL(k_nb_scan_sums, dim3(nm), b, 0, s, members);
+  const int *tot = plan.tot;
+  auto at = [&](int p) { return plan.map + plan.off[p]; };
+  const int ntail = tail ? tot[NB_TILES] : 0, nhead = tot[NB_OBS] + tot[NB_ZERO];
+  if (ntail + nhead > 0)
+    hipLaunchKernelGGL(k_nb_head, dim3(ntail + nhead), b, 0, s, members, tail ? tail : members, at(NB_TILES), ntail,
+                       at(NB_OBS), t);
+  if (tot[NB_CAND] > 0) hipLaunchKernelGGL(k_nb_count, dim3(tot[NB_CAND]), b, 0, s, members, at(NB_CAND));
+  if (tot[NB_SCAN] > 0) {  // the 7x7 members' count scan and list fill
+    hipLaunchKernelGGL(k_nb_scan_tiles, dim3(tot[NB_SCAN]), b, 0, s, members, at(NB_SCAN));
+    hipLaunchKernelGGL(k_nb_scan_sums, dim3(plan.nm), b, 0, s, members);
   }
-  if (totals[NB_FILL] > 0) hipLaunchKernelGGL(k_nb_fill, dim3(totals[NB_FILL]), b, 0, s, members, pre, nm);
-  hipLaunchKernelGGL(k_nb_fold, dim3(totals[NB_FOLD]), b, 0, s, members, pre, nm);
+  if (tot[NB_FILL] > 0) hipLaunchKernelGGL(k_nb_fill, dim3(tot[NB_FILL]), b, 0, s, members, at(NB_FILL));
+  if (tot[NB_FOLD] > 0) hipLaunchKernelGGL(k_nb_fold, dim3(tot[NB_FOLD]), b, 0, s, members, at(NB_FOLD));
+  if (!defer && tot[NB_TILES] > 0)
+    hipLaunchKernelGGL(k_nb_tiles, dim3(tot[NB_TILES]), b, 0, s, members, at(NB_TILES));
   return hipGetLastError();
 }
 
 hipError_t launch_obs_deposit(hipStream_t s, int branch, int ntempread, const Geom &g, int right_limit,
-                              const ObsPtrs &o, const DepPtrs &d) {
+                              const ObsPtrs &o, const DepPtrs &d, const DepPtrs *tail, bool defer) {
   const size_t n = g.plane;
   const bool bil = d.mode == RBN_DEPOSIT_BILINEAR;
-  const int zb = (int)std::min<size_t>((n + kBlock - 1) / kBlock, 4096);
+  const int zb = bil ? 0 : (int)std::min<size_t>((n + kBlock - 1) / kBlock, 4096);
   const int obs_blocks = (o.n + kBlock - 1) / kBlock;
-  hipLaunchKernelGGL(k_obs_zero, dim3(obs_blocks + zb), dim3(kBlock), 0, s, o, branch, g.nx, g.ny, ntempread,
-                     obs_blocks, n, d.cnt, bil ? nullptr : d.fill, d.nactive);
+  // only a bilinear deposit has a tile pass to carry (a 7x7 set has no bins)
+  const int tb = tail && tail->mode == RBN_DEPOSIT_BILINEAR && tail->tcnt && tail->bin ? dep_tiles(g) : 0;
+  if (tb + obs_blocks + zb > 0)
+    hipLaunchKernelGGL(k_obs_zero, dim3(tb + obs_blocks + zb), dim3(kBlock), 0, s, g, o, branch, ntempread, tb,
+                       obs_blocks, d, tail ? *tail : d);
+  if (bil) {
+    // the observers binned their contributions; the tile pass writes every node
+    if (!defer) hipLaunchKernelGGL(k_dep_tiles, dim3(dep_tiles(g)), dim3(kBlock), 0, s, g, d);
+    return hipGetLastError();
+  }
+  // a 7x7 deposit completes here whatever defer says (it has no pass to defer)
   if (o.n <= 0) return hipGetLastError();
-  const long long work = (long long)o.n * (bil ? kBilCells : kCells);
+  const long long work = (long long)o.n * kCells;
   const dim3 ob((unsigned)((work + kBlock - 1) / kBlock));
   const dim3 fb((unsigned)fold_blocks(d, 1024)), rows((g.nx + kBlock - 1) / kBlock, g.ny);
-  if (bil) {
-    // fixed-point sums, then Tbar and W at the nodes reached (and the sums cleared)
-    hipLaunchKernelGGL(k_dep_acc, ob, dim3(kBlock), 0, s, g, o, d);
-  } else {
-    const int ntiles = (int)((n + kScanTile - 1) / kScanTile);
-    hipLaunchKernelGGL(k_dep_count, ob, dim3(kBlock), 0, s, g, right_limit, o, d.cnt);
-    hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(kBlock), 0, s, n, d.cnt, d.off, d.blocksum);
-    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kBlock), 0, s, ntiles, d.blocksum, d.off + n);
-    hipLaunchKernelGGL(k_dep_fill, ob, dim3(kBlock), 0, s, g, right_limit, o, d.off, d.blocksum, d);
-  }
+  const int ntiles = (int)((n + kScanTile - 1) / kScanTile);
+  hipLaunchKernelGGL(k_dep_count, ob, dim3(kBlock), 0, s, g, right_limit, o, d.cnt);
+  hipLaunchKernelGGL(k_scan_tiles, dim3(ntiles), dim3(kBlock), 0, s, n, d.cnt, d.off, d.blocksum);
+  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kBlock), 0, s, ntiles, d.blocksum, d.off + n);
+  hipLaunchKernelGGL(k_dep_fill, ob, dim3(kBlock), 0, s, g, right_limit, o, d.off, d.blocksum, d);
   if (d.dense)
     hipLaunchKernelGGL(k_dep_fold_rows, rows, dim3(kBlock), 0, s, g, o, d);
   else

@@ -65,7 +65,9 @@ def _config_members(table):
         s = rbn.Solver(p)
         s.init_fields(T=conduction_T(p, 1e-3, seed=100 + m))
         if nudged:
-            ids, x0, y0, T, recs = synthetic_observations(nx, ny, n, 120, 5, seed=200 + m)
+            # records up to step 130: the autotuner's longest span (2 * 60 + 3 steps with two
+            # steps per launch) reads the record of step 125
+            ids, x0, y0, T, recs = synthetic_observations(nx, ny, n, 130, 5, seed=200 + m)
             s.init_observers(ids, x0, y0, T)
             if table:
                 s.set_observation_table(recs)
@@ -83,10 +85,12 @@ def test_ensemble_config_sizes_equal_individual_steps():
     ref = _config_members(table=False)
     e = rbn.Ensemble(ens)
     e.autotune(4)
-    for n in (1, 31, 62):  # 62 = a replay of the 30-step cycle and eager steps
+    # 63: replays of the captured cycle (30 steps with one launch per step, 60 with two) and
+    # eager steps
+    for n in (1, 31, 63):
         e.step(n)
     for s in ref:
-        s.step(94)
+        s.step(95)
     for k, (a, b) in enumerate(zip(ens, ref)):
         fa, ga = a.populations()
         fb, gb = b.populations()
