@@ -38,14 +38,18 @@ struct rbn_ensemble {
   int4 *d_nmap = nullptr;
   size_t nmap_cap = 0;
   rbn::NudgePlan plan{};
-  // their record uploads: two pinned staging buffers with a device twin each, alternating
-  // between the steps that load records
+  // their record uploads: pinned staging buffers with a device twin each, taken in turn by
+  // the steps that load records, copied on cstream beside the running kernels (copied: the
+  // copy ended; done: the device twin's reader ended)
   struct Stage {
     unsigned char *h = nullptr, *d = nullptr;
     size_t cap = 0;
-    hipEvent_t done = nullptr;
-  } stage[2];
+    hipEvent_t done = nullptr, copied = nullptr;
+  };
+  static constexpr int kStages = 4;
+  Stage stage[kStages];
   int stage_next = 0;
+  hipStream_t cstream = nullptr;
   std::vector<int> loading, rid;
   // and their captured cycle: ncycle steps from call step cyc_t0 (the least common multiple of
   // the velocity ring and every record interval, and of 4 in the two-step mode; 0: no graph),
@@ -532,6 +536,7 @@ int fetch_records(rbn_ensemble *e, const std::vector<int> &ld, int t, unsigned c
 // Pinned staging buffer of at least `need` bytes with a device twin.
 int ensure_stage(rbn_ensemble *e, rbn_ensemble::Stage &st, size_t need) {
   if (!st.done) EHIP(e, hipEventCreateWithFlags(&st.done, hipEventDisableTiming));
+  if (!st.copied) EHIP(e, hipEventCreateWithFlags(&st.copied, hipEventDisableTiming));
   EHIP(e, hipEventSynchronize(st.done));  // this buffer's previous upload has been read
   if (st.cap >= need) return RBN_OK;
   if (st.h) (void)hipHostFree(st.h);
@@ -545,15 +550,18 @@ int ensure_stage(rbn_ensemble *e, rbn_ensemble::Stage &st, size_t need) {
 }
 
 // Eager step t: fetch the records that start an interval at t, stage them in one pinned
-// buffer and send them to the observers with one copy and one scatter launch.
+// buffer and send them to the observers with one copy (on cstream, beside the kernels queued
+// before it) and one scatter launch on s after it.
 int load_records(rbn_ensemble *e, int t, hipStream_t s) {
   const size_t need = loading_members(e, t, e->loading);
   if (!need) return RBN_OK;
   rbn_ensemble::Stage &st = e->stage[e->stage_next];
-  e->stage_next ^= 1;
+  e->stage_next = (e->stage_next + 1) % rbn_ensemble::kStages;
   int rc = ensure_stage(e, st, need);
   if (rc || (rc = fetch_records(e, e->loading, t, st.h))) return rc;
-  EHIP(e, hipMemcpyAsync(st.d, st.h, need, hipMemcpyHostToDevice, s));
+  EHIP(e, hipMemcpyAsync(st.d, st.h, need, hipMemcpyHostToDevice, e->cstream));
+  EHIP(e, hipEventRecord(st.copied, e->cstream));
+  EHIP(e, hipStreamWaitEvent(s, st.copied, 0));
   EHIP(e, rbn::launch_obs_scatter(s, (const rbn::ObsJob *)st.d, (int)e->loading.size(), st.d));
   EHIP(e, hipEventRecord(st.done, s));
   return RBN_OK;
@@ -563,10 +571,11 @@ int load_records(rbn_ensemble *e, int t, hipStream_t s) {
 // the lattice launches on stream.  The deposit of step t goes to set t & 1 and the lattice
 // launch of step t waits for it; the deposit of step t + 1 then runs beside that launch,
 // after the lattice launch of step t - 1 (the last reader of its set) when that one is a
-// step of this run of eager steps (from t0; earlier work is ordered before it).
-int enqueue_deposit(rbn_ensemble *e, int t, int t0) {
+// step of this run of eager steps (from t0; earlier work is ordered before it).  last: the
+// call's last step (its deposit also fills the arrays the getters read).
+int enqueue_deposit(rbn_ensemble *e, int t, int t0, bool last) {
   if (t - 2 >= t0) EHIP(e, hipStreamWaitEvent(e->pstream, e->ev_lat[t & 1], 0));
-  EHIP(e, rbn::launch_nudge_batch(e->pstream, nudge_row(e, t), e->plan, t));
+  EHIP(e, rbn::launch_nudge_batch(e->pstream, nudge_row(e, t), e->plan, t, nullptr, false, last));
   EHIP(e, hipEventRecord(e->ev_dep[t & 1], e->pstream));
   return RBN_OK;
 }
@@ -602,7 +611,7 @@ int enqueue_nudged_pair(rbn_ensemble *e, PairState &ps, int nsteps, bool moments
     if (rc) return rc;
     const bool second = t > ps.t;
     EHIP(e, rbn::launch_nudge_batch(e->stream, nudge_row(e, t), e->plan, t, second ? nudge_row(e, t - 1) : nullptr,
-                                    two && !second));
+                                    two && !second, t == nsteps - 1));
   }
   return enqueue_pair(e, ps, nsteps, moments);
 }
@@ -634,7 +643,7 @@ int capture_nudged(rbn_ensemble *e) {
           return efail(e, RBN_EDEVICE, "record scatter capture failed");
       }
       if (rbn::launch_nudge_batch(e->stream, nudge_row(e, t), e->plan, t, tail ? nudge_row(e, t - 1) : nullptr,
-                                  defer) != hipSuccess)
+                                  defer, false) != hipSuccess)
         return efail(e, RBN_EDEVICE, "observer/deposit capture failed");
       return RBN_OK;
     };
@@ -689,7 +698,7 @@ int run_nudged(rbn_ensemble *e, int nsteps) {
   auto eager_deposit = [&](int t, int t0) {
     int r = load_records(e, t, e->pstream);
     if (r) return r;
-    return enqueue_deposit(e, t, t0);
+    return enqueue_deposit(e, t, t0, t == nsteps - 1);
   };
   PairState ps;
   if (two) {
@@ -707,12 +716,20 @@ int run_nudged(rbn_ensemble *e, int nsteps) {
       e->seg_steps = cycles * L;
     }
     for (int r = 0; r < cycles; r++) {
+      // replay r's records: filled on the host once the copy from this set two replays ago
+      // has read it, copied on cstream once that replay has read the device twin, beside the
+      // replay before
       rbn_ensemble::Stage &st = e->cstage[r & 1];
-      EHIP(e, hipEventSynchronize(st.done));
+      EHIP(e, hipEventSynchronize(st.copied));
       if ((rc = fill_cycle(e, r & 1, r))) return rc;
-      if (e->cyc_bytes) EHIP(e, hipMemcpyAsync(st.d, st.h, e->cyc_bytes, hipMemcpyHostToDevice, e->stream));
-      EHIP(e, hipEventRecord(st.done, e->stream));
+      if (e->cyc_bytes) {
+        EHIP(e, hipStreamWaitEvent(e->cstream, st.done, 0));
+        EHIP(e, hipMemcpyAsync(st.d, st.h, e->cyc_bytes, hipMemcpyHostToDevice, e->cstream));
+        EHIP(e, hipEventRecord(st.copied, e->cstream));
+        EHIP(e, hipStreamWaitEvent(e->stream, st.copied, 0));
+      }
       EHIP(e, hipGraphLaunch(e->nexec[r & 1], e->stream));
+      EHIP(e, hipEventRecord(st.done, e->stream));
     }
     if (e->seg_a) EHIP(e, hipEventRecord(e->seg_b, e->stream));
     ps.t += cycles * L;
@@ -924,6 +941,7 @@ int take_snapshot(rbn_ensemble *e, Snapshot &s) {
 int restore_snapshot(rbn_ensemble *e, Snapshot &s, bool keep = false) {
   (void)hipStreamSynchronize(e->stream);
   (void)hipStreamSynchronize(e->pstream);
+  (void)hipStreamSynchronize(e->cstream);
   for (size_t i = 0; i < s.istep.size() && i < e->m.size(); i++) {
     rbn_ctx *c = e->m[i];
     if (c->fA != s.fA[i]) {
@@ -959,21 +977,24 @@ void rbn_ensemble_destroy(rbn_ensemble *e) {
   (void)hipSetDevice(e->dev);
   if (e->stream) (void)hipStreamSynchronize(e->stream);
   if (e->pstream) (void)hipStreamSynchronize(e->pstream);
+  if (e->cstream) (void)hipStreamSynchronize(e->cstream);
   drop_graph(e);
   if (e->d_members) (void)hipFree(e->d_members);
   if (e->d_items) (void)hipFree(e->d_items);
   if (e->d_counters) (void)hipFree(e->d_counters);
   if (e->d_nmembers) (void)hipFree(e->d_nmembers);
   if (e->d_nmap) (void)hipFree(e->d_nmap);
-  for (auto *set : {&e->stage[0], &e->stage[1], &e->cstage[0], &e->cstage[1]}) {
+  for (auto *set : {&e->stage[0], &e->stage[1], &e->stage[2], &e->stage[3], &e->cstage[0], &e->cstage[1]}) {
     if (set->h) (void)hipHostFree(set->h);
     if (set->d) (void)hipFree(set->d);
     if (set->done) (void)hipEventDestroy(set->done);
+    if (set->copied) (void)hipEventDestroy(set->copied);
   }
   if (e->ev_lbm) (void)hipEventDestroy(e->ev_lbm);
   for (hipEvent_t ev : {e->ev_part[0], e->ev_part[1], e->ev_dep[0], e->ev_dep[1], e->ev_lat[0], e->ev_lat[1], e->ev_seq})
     if (ev) (void)hipEventDestroy(ev);
   for (rbn::DepPtrs &s : e->spare) free_spare(s);
+  if (e->cstream) (void)hipStreamDestroy(e->cstream);
   if (e->pstream) (void)hipStreamDestroy(e->pstream);
   if (e->stream) (void)hipStreamDestroy(e->stream);
   delete e;
@@ -1000,6 +1021,7 @@ int rbn_ensemble_create(rbn_ctx *const *members, int n, rbn_ensemble **out) {
   if (hipSetDevice(e->dev) != hipSuccess ||
       hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
       hipStreamCreateWithFlags(&e->pstream, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&e->cstream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&e->ev_lbm, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&e->ev_part[0], hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&e->ev_part[1], hipEventDisableTiming) != hipSuccess ||

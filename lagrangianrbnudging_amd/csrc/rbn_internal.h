@@ -287,9 +287,12 @@ struct NudgePlan {
 // Observer update and deposit of step t of a call (t = 0 is the first) for plan.nm members
 // at once (members: device descriptors of the deposit set of step t).  tail (or null): the
 // descriptors of the previous step's set, whose bilinear tile passes were deferred; defer:
-// leave this step's bilinear tile passes to the next call's tail.
+// leave this step's bilinear tile passes to the next call's tail.  api: the bilinear tile
+// passes also write the counts, means and weights (the last deposit of a call, which the
+// getters and the members' own arrays receive); without it, and in a deferred pass, only the
+// packed deposit the collisions read.
 hipError_t launch_nudge_batch(hipStream_t s, const NudgeMember *members, const NudgePlan &plan, int t,
-                              const NudgeMember *tail = nullptr, bool defer = false);
+                              const NudgeMember *tail = nullptr, bool defer = false, bool api = true);
 // Scatter njobs staged records (jobs and data in the device staging buffer) to their observers.
 hipError_t launch_obs_scatter(hipStream_t s, const ObsJob *jobs, int njobs, const unsigned char *staging);
 hipError_t launch_energy(hipStream_t s, int prec, const Geom &g, const void *rho, const void *uv,

@@ -48,6 +48,7 @@ namespace rbn {
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kScatterSplit = 16;  // blocks per record of a batched upload
 constexpr int kScanItems = 4;
 constexpr int kScanTile = kBlock * kScanItems;
 static_assert(kScanTile == kDepScanTile, "tile totals are sized by kDepScanTile");
@@ -388,32 +389,41 @@ __device__ __forceinline__ void bin_one(const Geom &g, const DepPtrs &d, const O
 
 // Bilinear deposit, deposit tile t: the sums of its binned contributions in LDS (exact, in
 // any order), plus the sums in HBM when the tile overflowed (cleared here), then every node
-// of the tile: the count, the packed deposit, and Tbar = S / W and W (each sum converted to
-// double) where the count is positive.  Leaves the tile's counter at zero.  Every thread of
-// the workgroup calls it.
+// of the tile: the packed deposit, and with api also the count, and Tbar = S / W and W (each
+// sum converted to double) where the count is positive (the arrays the getters, the adjoint
+// and the ensemble's copies read; the nudged collision reads only the packed deposit).
+// Leaves the tile's counter at zero.  Every thread of the workgroup calls it.
 struct DepTileLds {
   unsigned long long S[kDepTileW * kDepTileH], W[kDepTileW * kDepTileH];
   int n[kDepTileW * kDepTileH];
 };
 
-__device__ __forceinline__ void dep_tile(const Geom &g, const DepPtrs &d, int t, DepTileLds &L) {
+__device__ __forceinline__ void dep_tile(const Geom &g, const DepPtrs &d, int t, DepTileLds &L, bool api) {
   constexpr int kNodes = kDepTileW * kDepTileH;
+  const longlong2 *e = (const longlong2 *)d.bin + (size_t)t * d.bin_cap;
+  // the counter and each thread's first two slots in one round trip (slots past the count
+  // hold stale entries, read but not used)
+  const int n = d.tcnt[(size_t)t * kTcntStride];
+  const int k0 = threadIdx.x, k1 = threadIdx.x + kBlock;
+  longlong2 v0 = make_longlong2(0, 0), v1 = make_longlong2(0, 0);
+  if (k0 < d.bin_cap) v0 = e[k0];
+  if (k1 < d.bin_cap) v1 = e[k1];
   for (int k = threadIdx.x; k < kNodes; k += kBlock) {
     L.S[k] = 0;
     L.W[k] = 0;
     L.n[k] = 0;
   }
-  const int n = d.tcnt[(size_t)t * kTcntStride];
   __syncthreads();
   const int m = min(n, d.bin_cap);
-  const longlong2 *e = (const longlong2 *)d.bin + (size_t)t * d.bin_cap;
-  for (int k = threadIdx.x; k < m; k += kBlock) {
-    const longlong2 v = e[k];
+  auto add = [&](const longlong2 &v) {
     const int local = (int)(v.y & (kNodes - 1));
     atomicAdd(&L.S[local], (unsigned long long)v.x);
     atomicAdd(&L.W[local], (unsigned long long)(v.y >> kDepTileBits));
     atomicAdd(&L.n[local], 1);
-  }
+  };
+  if (k0 < m) add(v0);
+  if (k1 < m) add(v1);
+  for (int k = k1 + kBlock; k < m; k += kBlock) add(e[k]);
   __syncthreads();
   const int tiles_x = dep_tiles_x(g), ty = t / tiles_x, tx = t - ty * tiles_x;
   for (int k = threadIdx.x; k < kNodes; k += kBlock) {
@@ -430,11 +440,13 @@ __device__ __forceinline__ void dep_tile(const Geom &g, const DepPtrs &d, int t,
       d.S[c] = 0;
       d.Wi[c] = 0;
     }
-    d.cnt[c] = cnt;
+    if (api) d.cnt[c] = cnt;
     if (cnt > 0) {
       const double Tbar = wi != 0 ? (double)s / (double)wi : 0.0, W = (double)wi * 0x1p-40;
-      d.Tbar[c] = Tbar;
-      d.wq[c] = W;
+      if (api) {
+        d.Tbar[c] = Tbar;
+        d.wq[c] = W;
+      }
       tw_put(d, c, Tbar, nudge_weight(W));
     } else {
       tw_zero(d, c);
@@ -453,7 +465,7 @@ __global__ __launch_bounds__(kBlock) void k_obs_zero(Geom g, ObsPtrs o, int bran
   __shared__ DepTileLds L;
   const int b = blockIdx.x;
   if (b < tail_blocks) {
-    dep_tile(g, tail, b, L);
+    dep_tile(g, tail, b, L, true);
   } else if (b < tail_blocks + obs_blocks) {
     const int i = (b - tail_blocks) * kBlock + threadIdx.x;
     if (i < o.n) {
@@ -468,7 +480,7 @@ __global__ __launch_bounds__(kBlock) void k_obs_zero(Geom g, ObsPtrs o, int bran
 
 __global__ __launch_bounds__(kBlock) void k_dep_tiles(Geom g, DepPtrs d) {
   __shared__ DepTileLds L;
-  dep_tile(g, d, blockIdx.x, L);
+  dep_tile(g, d, blockIdx.x, L, true);
 }
 
 // Per node: order the contributions by observer and fold them in that order, the
@@ -602,7 +614,7 @@ __global__ __launch_bounds__(kBlock) void k_nb_head(const NudgeMember *M, const 
   __shared__ DepTileLds L;
   if ((int)blockIdx.x < ntail) {
     const int4 w = tmap[blockIdx.x];
-    dep_tile(tail[w.x].g, tail[w.x].d, w.y, L);
+    dep_tile(tail[w.x].g, tail[w.x].d, w.y, L, false);
     return;
   }
   const int4 w = hmap[blockIdx.x - ntail];
@@ -665,24 +677,26 @@ __global__ __launch_bounds__(kBlock) void k_nb_fold(const NudgeMember *M, const 
   }
 }
 
-// The bilinear members' tile passes.
-__global__ __launch_bounds__(kBlock) void k_nb_tiles(const NudgeMember *M, const int4 *map) {
+// The bilinear members' tile passes (api: as dep_tile's).
+__global__ __launch_bounds__(kBlock) void k_nb_tiles(const NudgeMember *M, const int4 *map, int api) {
   __shared__ DepTileLds L;
   const int4 w = map[blockIdx.x];
-  dep_tile(M[w.x].g, M[w.x].d, w.y, L);
+  dep_tile(M[w.x].g, M[w.x].d, w.y, L, api != 0);
 }
 
 }  // namespace
 
 namespace {
 
+// Job blockIdx.y, split over gridDim.x blocks.
 __global__ __launch_bounds__(kBlock) void k_obs_scatter(const ObsJob *jobs, const unsigned char *staging) {
-  const ObsJob j = jobs[blockIdx.x];
+  const ObsJob j = jobs[blockIdx.y];
   const double *src = (const double *)(staging + j.src);
-  for (int k = threadIdx.x; k < 3 * j.n; k += kBlock) j.rx[k] = src[k];
+  const int k0 = blockIdx.x * kBlock + threadIdx.x, stride = gridDim.x * kBlock;
+  for (int k = k0; k < 3 * j.n; k += stride) j.rx[k] = src[k];
   if (j.chk) {
     const int *map = (const int *)(src + 3 * (size_t)j.n);
-    for (int k = threadIdx.x; k < j.n; k += kBlock) j.chk[k] = map[k];
+    for (int k = k0; k < j.n; k += stride) j.chk[k] = map[k];
   }
 }
 
@@ -690,7 +704,7 @@ __global__ __launch_bounds__(kBlock) void k_obs_scatter(const ObsJob *jobs, cons
 
 hipError_t launch_obs_scatter(hipStream_t s, const ObsJob *jobs, int njobs, const unsigned char *staging) {
   if (njobs <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_obs_scatter, dim3(njobs), dim3(kBlock), 0, s, jobs, staging);
+  hipLaunchKernelGGL(k_obs_scatter, dim3(kScatterSplit, njobs), dim3(kBlock), 0, s, jobs, staging);
   return hipGetLastError();
 }
 
@@ -710,7 +724,7 @@ void nudge_batch_blocks(const NudgeMember &d, int blocks[kNudgeBatches]) {
 }
 
 hipError_t launch_nudge_batch(hipStream_t s, const NudgeMember *members, const NudgePlan &plan, int t,
-                              const NudgeMember *tail, bool defer) {
+                              const NudgeMember *tail, bool defer, bool api) {
   if (plan.nm <= 0) return hipSuccess;
   const dim3 b(kBlock);
   const int *tot = plan.tot;
@@ -727,7 +741,7 @@ hipError_t launch_nudge_batch(hipStream_t s, const NudgeMember *members, const N
   if (tot[NB_FILL] > 0) hipLaunchKernelGGL(k_nb_fill, dim3(tot[NB_FILL]), b, 0, s, members, at(NB_FILL));
   if (tot[NB_FOLD] > 0) hipLaunchKernelGGL(k_nb_fold, dim3(tot[NB_FOLD]), b, 0, s, members, at(NB_FOLD));
   if (!defer && tot[NB_TILES] > 0)
-    hipLaunchKernelGGL(k_nb_tiles, dim3(tot[NB_TILES]), b, 0, s, members, at(NB_TILES));
+    hipLaunchKernelGGL(k_nb_tiles, dim3(tot[NB_TILES]), b, 0, s, members, at(NB_TILES), api ? 1 : 0);
   return hipGetLastError();
 }
 
