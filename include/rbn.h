@@ -358,6 +358,10 @@ int rbn_adj_backward(rbn_ctx *ctx, double *dJ_dT0, double *dJ_dalpha);
  *                lattice the hand-over between the streams costs more than it hides, e.g.
  *                1024x512 FP32 with 1e4 observers 54 against 50 us per step; ensembles,
  *                whose lattice launch is much longer than their deposits, gain from it)
+ *   "dep_in_tile" nudged run with the bilinear deposit, two-step launches of 64 x 8 tiles: 1:
+ *                the second step's deposit is summed inside the launch, each tile from its
+ *                own bins, instead of by a separate pass (not in a call's last launch, whose
+ *                deposit the getters read) (default 1)
  *   "particle_overlap"  1: run the particle update of step s on its own stream concurrently
  *                with the lattice update of step s+1 (default 0)
  *   "particle_fuse" 1: run the particle update of step s as extra workgroups of the lattice

@@ -1269,11 +1269,15 @@ int step_loop(rbn_ctx *c, int nsteps) {
       moments2 = record2 || diag2 || n == nsteps - 1;
     }
     const bool writes_moments = pair ? moments2 : moments;
+    // bilinear deposit: the second step's deposit summed inside the two-step launch from its
+    // bins, except in the call's last launch (its deposit fills the arrays the getters read)
+    const bool in_tile = nudged && pair && c->dep.mode == RBN_DEPOSIT_BILINEAR && c->tune.dep_in_tile &&
+                         rbn::lbm2_tile_rows(c->prec, c->tune.tile_h) == rbn::kDepTileH && n < nsteps - 1;
     if (nudged && pair) {
       // the first step's bilinear tile pass rides in the second step's first launch
       const bool bil = c->dep.mode == RBN_DEPOSIT_BILINEAR;
       if ((rc = nudged_deposit(c, istep, c->stream, c->dep2, nullptr, bil)) ||
-          (rc = nudged_deposit(c, istep2, c->stream, c->dep, bil ? &c->dep2 : nullptr)))
+          (rc = nudged_deposit(c, istep2, c->stream, c->dep, bil ? &c->dep2 : nullptr, in_tile)))
         return rc;
     } else if (nudged && !dovl) {
       if ((rc = nudged_deposit(c, istep, c->stream, c->dep))) return rc;
@@ -1324,6 +1328,7 @@ int step_loop(rbn_ctx *c, int nsteps) {
              if (nudged) {
                np.tw2 = c->dep.tw;
                np.nud_out = c->nud2;
+               np.bins2 = in_tile ? &c->dep : nullptr;
              }
              return rbn::launch_lbm2(c->stream, c->prec, c->geom, lp, c->F2, c->uv[s2], moments2 ? c->rho : nullptr,
                                      moments2 ? c->T : nullptr, c->consts.data(), t.tile_h, t.seg, t.xcd2, t.variant,
@@ -1471,6 +1476,8 @@ int rbn_set_tuning(rbn_ctx *c, const char *key, int value) {
     c->tune.variant = value;
   } else if (!strcmp(key, "dep_overlap")) {
     c->tune.dep_overlap = value != 0;
+  } else if (!strcmp(key, "dep_in_tile")) {
+    c->tune.dep_in_tile = value != 0;
   } else if (!strcmp(key, "particle_overlap")) {
     c->tune.overlap = value != 0;
   } else if (!strcmp(key, "particle_fuse")) {
@@ -1498,6 +1505,7 @@ int rbn_get_tuning(const rbn_ctx *c, const char *key, int *value) {
   else if (!strcmp(key, "lbm2_xcd")) *value = c->tune.xcd2;
   else if (!strcmp(key, "lbm2_variant")) *value = c->tune.variant;
   else if (!strcmp(key, "dep_overlap")) *value = c->tune.dep_overlap;
+  else if (!strcmp(key, "dep_in_tile")) *value = c->tune.dep_in_tile;
   else if (!strcmp(key, "particle_overlap")) *value = c->tune.overlap;
   else if (!strcmp(key, "particle_fuse")) *value = c->tune.fuse;
   else if (!strcmp(key, "adj_store_deposit")) *value = c->adj_store_deposit;

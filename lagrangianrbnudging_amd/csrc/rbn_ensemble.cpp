@@ -198,10 +198,17 @@ void fill_member(const rbn_ctx *c, const rbn::DepPtrs *spare, int tile_w, int ti
   d.bad = c->d_err + 1;
   d.variant = c->P.variant;
   if (c->P.variant == RBN_VARIANT_NUDGING) {
+    const bool bil = c->dep.mode == RBN_DEPOSIT_BILINEAR;
     for (int p = 0; p < 2; p++) {
       const rbn::DepPtrs &s = p ? c->dep : *spare;
       d.tw[p] = s.tw;
+      d.bin[p] = bil ? s.bin : nullptr;
+      d.tcnt[p] = bil ? s.tcnt : nullptr;
+      d.S[p] = bil ? s.S : nullptr;
+      d.Wi[p] = bil ? s.Wi : nullptr;
+      d.ocnt[p] = bil ? s.ocnt : nullptr;
     }
+    d.bin_cap = c->dep.bin_cap;
     d.nud[0] = (R *)c->nud;
     d.nud[1] = (R *)c->nud2;
   }
@@ -382,9 +389,10 @@ int launch_pairs(rbn_ensemble *e, const rbn::EnsStep &st) {
 }
 
 // The next launch of the two-step mode: two steps when two remain, carrying what is pending.
-int enqueue_pair(rbn_ensemble *e, PairState &ps, int nsteps, bool moments) {
+// bins2: the bilinear members' second-step deposits are summed in the tiles.
+int enqueue_pair(rbn_ensemble *e, PairState &ps, int nsteps, bool moments, bool bins2 = false) {
   const bool two = ps.t + 1 < nsteps;
-  rbn::EnsStep st;
+  rbn::EnsStep st{};
   st.k = ps.t % rbn::kCycle;
   st.lpar = ps.L & 1;
   st.fpar = ps.P & 1;
@@ -393,6 +401,7 @@ int enqueue_pair(rbn_ensemble *e, PairState &ps, int nsteps, bool moments) {
   st.moments = moments ? 1 : 0;
   st.reverse = (e->order & 2) && (ps.L & 1);
   st.cset = ps.L & 1;
+  st.bins2 = two && bins2 ? 1 : 0;
   int rc = launch_pairs(e, st);
   if (rc) return rc;
   ps.pending = two ? 2 : 1;
@@ -600,20 +609,27 @@ int fill_cycle(rbn_ensemble *e, int k, int r) {
   return RBN_OK;
 }
 
+// Two-step tiles of the ensemble (FP32) are deposit tiles: a launch can sum its bilinear
+// members' second-step deposits from their bins.
+bool tile_bins(const rbn_ensemble *e) { return e->prec == RBN_FP32; }
+static_assert(rbn::kEnsTH32 == rbn::kDepTileH, "FP32 ensemble tiles are deposit tiles");
+
 // Two-step mode with nudged members: the records and deposits of a launch's steps (step t
 // into set t & 1, t + 1 into set (t + 1) & 1) run on stream before it, after the previous
 // launch, which read both sets.  The first step's bilinear tile passes ride in the second
-// step's first launch.
+// step's first launch; the second step's are summed in the lattice launch's tiles, except in
+// the call's last launch (its deposit fills the arrays the getters read).
 int enqueue_nudged_pair(rbn_ensemble *e, PairState &ps, int nsteps, bool moments) {
   const bool two = ps.t + 1 < nsteps;
+  const bool in_tile = two && tile_bins(e) && ps.t + 2 < nsteps;
   for (int t = ps.t; t < ps.t + (two ? 2 : 1); t++) {
     int rc = load_records(e, t, e->stream);
     if (rc) return rc;
     const bool second = t > ps.t;
     EHIP(e, rbn::launch_nudge_batch(e->stream, nudge_row(e, t), e->plan, t, second ? nudge_row(e, t - 1) : nullptr,
-                                    two && !second, t == nsteps - 1));
+                                    second ? in_tile : two, t == nsteps - 1));
   }
-  return enqueue_pair(e, ps, nsteps, moments);
+  return enqueue_pair(e, ps, nsteps, moments, in_tile);
 }
 
 // Capture the cycle of nudged steps t0 .. t0 + L - 1 of a call twice, the scatters reading
@@ -655,8 +671,8 @@ int capture_nudged(rbn_ensemble *e) {
       ps.P = 1;
       ps.pending = 2;
       for (int l = 0; l < e->ncycle / 2 && !rc; l++) {
-        if (!(rc = deposit(ps.t, false, true)) && !(rc = deposit(ps.t + 1, true, false)))
-          rc = enqueue_pair(e, ps, 1 << 30, false);
+        if (!(rc = deposit(ps.t, false, true)) && !(rc = deposit(ps.t + 1, true, tile_bins(e))))
+          rc = enqueue_pair(e, ps, 1 << 30, false, tile_bins(e));
       }
     } else {
       for (int t = 1; t <= e->ncycle && !rc; t++)

@@ -40,6 +40,13 @@ struct EnsMember {
   int variant;
   const void *tw[2];
   R *nud[2];
+  // bilinear deposit: each set's bins (DepPtrs), from which a two-step launch may sum its
+  // second step's deposit in the tiles (EnsStep::bins2); null for the 7x7 deposit
+  const long long *bin[2];
+  int *tcnt[2];
+  long long *S[2], *Wi[2];
+  int *ocnt[2];
+  int bin_cap;
 };
 
 // items: the n_lbm lattice items, then the n_part particle items.  phase 0: the lattice items
@@ -53,10 +60,11 @@ hipError_t launch_ensemble_phase(hipStream_t s, int prec, int phase, const void 
 // n_lbm .. n_lbm+n_part), then the lattice of step k as one-step tiles (lat 1, items 0 ..
 // n_lbm) or of steps k, k+1 as two-step tiles (lat 2, items n_lbm+n_part ..).  lpar: which
 // of pf/pg is the input; fpar: which of F (and of a nudged member's nud) is the input;
-// cset: the counter set.  nudged (of the launch): 0 the ensemble holds truth runs only, 1
-// nudged runs only, 2 both.
+// cset: the counter set.  bins2: the bilinear members' second-step deposits are summed in the
+// two-step tiles from their bins (their tile passes were left out).  nudged (of the launch):
+// 0 the ensemble holds truth runs only, 1 nudged runs only, 2 both.
 struct EnsStep {
-  int k, lpar, fpar, ncarry, lat, moments, reverse, cset;
+  int k, lpar, fpar, ncarry, lat, moments, reverse, cset, bins2;
 };
 hipError_t launch_ensemble_pairs(hipStream_t s, int prec, const void *members, const int2 *items, int n_lbm,
                                  int n_part, int n_lbm2, int *counters, const EnsStep &st, int tile_w,

@@ -214,6 +214,15 @@ __device__ __forceinline__ void pair_item(const EnsMember<R> &m, int tile, const
   if constexpr (KIND != kPairsTruth) {
     if (KIND == kPairsNudged || m.variant == RBN_VARIANT_NUDGING) {
       for (int u = 0; u < 2; u++) p.tw[u] = m.tw[(st.k + u) & 1];
+      const int s2 = (st.k + 1) & 1;
+      if (st.bins2 && m.bin[s2]) {
+        p.bin2 = m.bin[s2];
+        p.tcnt2 = m.tcnt[s2];
+        p.bin_cap2 = m.bin_cap;
+        p.S2 = m.S[s2];
+        p.Wi2 = m.Wi[s2];
+        p.ocnt2 = m.ocnt[s2];
+      }
       p.nud_in = m.nud[st.fpar];
       p.nud_out = m.nud[1 - st.fpar];
       lbm2::two_step_tile<R, TH, RBN_VARIANT_NUDGING, GR>(m.g, p, m.c, tx, ty, L);
@@ -254,9 +263,12 @@ __device__ __forceinline__ void run_item2(const EnsMember<R> *__restrict__ M, co
 }
 
 // KIND: kOneStep for the launches of one step or particle updates only (lat 1, 0), a pairs
-// kind for those of two-step tiles (lat 2).
+// kind for those of two-step tiles (lat 2).  The nudged pairs kernel is held to three waves
+// per SIMD (as many as its LDS allows): unconstrained, the FP32 build takes 171 VGPRs (two
+// waves), constrained 168 without spills.
 template <typename R, int TH, int KIND>
-__global__ __launch_bounds__(kThreads) void k_ensemble2(const EnsMember<R> *__restrict__ M,
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(KIND == kPairsNudged ? 3 : 1)))
+void k_ensemble2(const EnsMember<R> *__restrict__ M,
                                                         const int2 *__restrict__ items, int n_lbm, int n_part,
                                                         int n_lbm2, int *counters, EnsStep st, int tile_w,
                                                         int tile_h, int persistent) {

@@ -207,6 +207,7 @@ struct LbmTuning {
   int xcd2 = 1;     // row sweeps: consecutive strips and segments on one XCD
   int variant = 0;  // row sweeps: 1 pulls one row ahead
   int dep_overlap = 0;  // nudged run: observer update and deposit of step s+1 beside the lattice step s
+  int dep_in_tile = 1;  // nudged two-step launches: the second step's bilinear deposit summed in the tiles
 };
 
 // The particle updates of one or two consecutive steps, in step order, carried by the next
@@ -242,7 +243,12 @@ hipError_t launch_lbm(hipStream_t s, int prec, int variant, const Geom &g, const
 struct NudgePair {
   const void *tw2;
   void *nud_out;
+  // or (non-null, bilinear deposit, tiles of kDepTileH rows) the second step's set, whose tile
+  // pass was deferred: its deposit is summed from the bins inside the launch
+  const DepPtrs *bins2 = nullptr;
 };
+// Rows of the two-step tiles launch_lbm2 uses for precision prec and tuning tile_h.
+int lbm2_tile_rows(int prec, int tile_h);
 // Two steps of K1 in one launch (rbn_lbm2.hip): p holds the first step's inputs and the second
 // step's outputs (fB, gB), p.F = F(s), p.uv = the first step's velocity slot; Fout receives
 // F(s+2), uv2 the second step's velocity, rho2 / T2 (or null) its moments.  job (or null):

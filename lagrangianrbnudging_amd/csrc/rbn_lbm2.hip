@@ -18,15 +18,15 @@ struct Lbm2Args {
   int tiles_x, ntiles, flip;
 };
 
-// FP32 tiles of 64 x 8 run at most three workgroups per CU: with four (40.4 KB of LDS each)
-// the launch measured 3.6 % slower at 4096 x 2048 (0.361 against 0.349 ms with the
-// benchmark's particles), so the pad takes a workgroup's LDS above a quarter of the CU's
-// 160 KB.  FP64 tiles of 64 x 4 (43.6 KB) fit three per CU without it and gain from the
-// third (0.556 against 0.594 ms with two).
+// FP32 truth-run tiles of 64 x 8 run at most three workgroups per CU: with four (40.4 KB of
+// LDS each) the launch measured 3.6 % slower at 4096 x 2048 (0.361 against 0.349 ms with
+// the benchmark's particles), so the pad takes a workgroup's LDS above a quarter of the CU's
+// 160 KB.  Nudged-run tiles (47.6 KB) and FP64 tiles of 64 x 4 (43.6 KB) fit three per CU
+// without it; FP64 gains from the third (0.556 against 0.594 ms with two).
 template <typename R, int TH, int V>
 struct KernelLds {
   TileLds<R, TH, ring_g<V>()> L;
-  char cap[(sizeof(R) == 4 && TH == 8) ? 8192 : 1];
+  char cap[(sizeof(R) == 4 && TH == 8 && V != RBN_VARIANT_NUDGING) ? 8192 : 1];
 };
 
 template <typename R, int TH, int V, int NT>
@@ -88,7 +88,7 @@ void dispatch_wave(hipStream_t s, const Geom &g, const WaveArgs &a, int variant,
 // Tile heights: 4, 8 or 16 rows in FP32; FP64 at most 8 rows (16 rows would exceed the 160 KB
 // of a CU).  0 picks the measured best: 8 rows in FP32, 4 in FP64 (three workgroups per CU
 // each).
-int tile_rows(int prec, int tile_h) {
+int tile_rows_of(int prec, int tile_h) {
   if (tile_h == 0) return prec == RBN_FP64 ? 4 : 8;
   if (tile_h == 4) return 4;
   if (tile_h == 16 && prec != RBN_FP64) return 16;
@@ -139,6 +139,8 @@ void dispatch(hipStream_t s, int prec, int nt, int th, const Geom &g, const Lbm2
 
 }  // namespace
 
+int lbm2_tile_rows(int prec, int tile_h) { return tile_rows_of(prec, tile_h); }
+
 hipError_t launch_lbm2(hipStream_t s, int prec, const Geom &g, const LbmPtrs &p, void *Fout, void *uv2,
                        void *rho2, void *T2, const void *step_const, int tile_h, int seg, int xcd, int variant,
                        int flip, int nt, const PartJob *job, const NudgePair *np) {
@@ -166,7 +168,7 @@ hipError_t launch_lbm2(hipStream_t s, int prec, const Geom &g, const LbmPtrs &p,
       dispatch_wave<float>(s, g, w, variant, step_const, job);
     return hipGetLastError();
   }
-  const int th = tile_rows(prec, tile_h);
+  const int th = tile_rows_of(prec, tile_h);
   Lbm2Args a{};
   a.p.fA = p.fA;
   a.p.gA = p.gA;
@@ -184,6 +186,15 @@ hipError_t launch_lbm2(hipStream_t s, int prec, const Geom &g, const LbmPtrs &p,
     a.p.nud_in = p.nud;
     a.p.tw[1] = np->tw2;
     a.p.nud_out = np->nud_out;
+    if (np->bins2) {
+      if (th != kDepTileH || np->bins2->mode != RBN_DEPOSIT_BILINEAR) return hipErrorInvalidValue;
+      a.p.bin2 = np->bins2->bin;
+      a.p.tcnt2 = np->bins2->tcnt;
+      a.p.bin_cap2 = np->bins2->bin_cap;
+      a.p.S2 = np->bins2->S;
+      a.p.Wi2 = np->bins2->Wi;
+      a.p.ocnt2 = np->bins2->ocnt;
+    }
   }
   a.tiles_x = (g.nx + kTW - 1) / kTW;
   a.ntiles = a.tiles_x * ((g.ny + th - 1) / th);

@@ -321,6 +321,14 @@ struct TwoStepPtrs {
   // nudging term entering step s and after step s+1 (double-buffered: other tiles still read
   // the input at the nodes of this tile's ring)
   const void *tw[2];
+  // or, when bin2 is set (bilinear deposit, tiles of kDepTileH rows: every tile is one
+  // deposit tile), step s+1's deposit summed from its bins inside the launch (DepPtrs' tcnt,
+  // bin, bin_cap, S, Wi, ocnt of its set), tw[1] unused
+  const long long *bin2;
+  int *tcnt2;
+  int bin_cap2;
+  long long *S2, *Wi2;
+  int *ocnt2;
   const void *nud_in;
   void *nud_out;
 };
@@ -351,6 +359,10 @@ struct TileLds {
   R rag[GR][RW];       // row TH+1, g: 4, 7, 8 (or all nine)
   R cl[2][3][TH];      // column 0, rows 1 .. TH: directions 1, 5, 8
   R cr[2][3][TH];      // column kTW+1, rows 1 .. TH: directions 3, 6, 7
+  // nudged run, tiles of kDepTileH rows: the second step's deposit per tile node (mean, weight
+  // or -1) when the launch sums it from the bins (kept here, not in registers, across the
+  // first step)
+  Real2<R> d2[(GR == 9 && TH == kDepTileH) ? kTW * TH : 1];
 };
 
 // Slot of direction k in each ring store (directions that never reach the tile from there
@@ -459,6 +471,76 @@ __device__ __forceinline__ void first_step(const Geom &g, const TwoStepPtrs &a, 
   if (tile) st_uv<0, R>(a.uv1, o, u, v);
 }
 
+// Step s+1's bilinear deposit of tile (tx, ty) (kDepTileH rows) from its bins, as the tile
+// pass sums it (rbn_nudge.hip: fixed-point sums in any order, the overflow's sums in HBM
+// added and cleared, the counter left at zero), in the LDS the first step has not yet
+// filled; the mean and weight of each tile node (w = -1 where it has none) go to L.d2.
+template <typename R, int TH, int GR>
+__device__ __forceinline__ void bin_deposit(const Geom &g, const TwoStepPtrs &a, int tx, int ty,
+                                            TileLds<R, TH, GR> &L) {
+  constexpr int kNodes = kTW * TH, NQ = kNodes / kTileThreads;
+  static_assert(kNodes == kDepTileW * kDepTileH && NQ * kTileThreads == kNodes, "one deposit tile per tile");
+  static_assert(sizeof(L.d2) == kNodes * sizeof(Real2<R>), "tile deposit store");
+  static_assert(kNodes * (2 * sizeof(long long) + sizeof(int)) <= sizeof(L.t), "LDS scratch");
+  auto *S = reinterpret_cast<unsigned long long *>(&L.t[0][0][0][0]);
+  auto *W = S + kNodes;
+  auto *n = reinterpret_cast<int *>(W + kNodes);
+  const int t = ty * dep_tiles_x(g) + tx;
+  const longlong2 *e = (const longlong2 *)a.bin2 + (size_t)t * a.bin_cap2;
+  const int cnt = a.tcnt2[(size_t)t * kTcntStride];
+  const int k0 = threadIdx.x, k1 = threadIdx.x + kTileThreads;
+  longlong2 v0 = make_longlong2(0, 0), v1 = make_longlong2(0, 0);
+  if (k0 < a.bin_cap2) v0 = e[k0];  // slots past the count: stale, read but not used
+  if (k1 < a.bin_cap2) v1 = e[k1];
+  for (int k = threadIdx.x; k < kNodes; k += kTileThreads) {
+    S[k] = 0;
+    W[k] = 0;
+    n[k] = 0;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) a.tcnt2[(size_t)t * kTcntStride] = 0;  // every thread has read it
+  const int m = min(cnt, a.bin_cap2);
+  auto add = [&](const longlong2 &v) {
+    const int local = (int)(v.y & (kNodes - 1));
+    atomicAdd(&S[local], (unsigned long long)v.x);
+    atomicAdd(&W[local], (unsigned long long)(v.y >> kDepTileBits));
+    atomicAdd(&n[local], 1);
+  };
+  if (k0 < m) add(v0);
+  if (k1 < m) add(v1);
+  for (int k = k1 + kTileThreads; k < m; k += kTileThreads) add(e[k]);
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NQ; q++) {
+    const int k = threadIdx.x + q * kTileThreads;
+    const int i = tx * kTW + k % kTW, j = ty * TH + k / kTW;
+    long long s = (long long)S[k], wi = (long long)W[k];
+    int c = n[k];
+    if (cnt > a.bin_cap2 && i < g.nx && j < g.ny) {
+      const size_t o = (size_t)j * g.pitch + i;
+      if (a.ocnt2[o] != 0) {
+        c += a.ocnt2[o];
+        s += a.S2[o];
+        wi += a.Wi2[o];
+        a.ocnt2[o] = 0;
+        a.S2[o] = 0;
+        a.Wi2[o] = 0;
+      }
+    }
+    Real2<R> d;
+    if (c > 0) {
+      const double Tbar = wi != 0 ? (double)s / (double)wi : 0.0, Wd = (double)wi * 0x1p-40;
+      d.x = (R)Tbar;
+      d.y = (R)nudge_weight(Wd);
+    } else {
+      d.x = R(0);
+      d.y = R(-1);
+    }
+    L.d2[k] = d;
+  }
+  __syncthreads();  // the first step overwrites the sums
+}
+
 // Both steps on tile (tx, ty) of kTW x TH nodes; every thread of the workgroup calls it.
 // Phase 1 computes step s on the tile and its ring: first the tile nodes, thread t taking
 // nodes t, t + kTileThreads, ... of the tile in row order (the same nodes it updates in phase 2,
@@ -475,6 +557,14 @@ __device__ __forceinline__ void two_step_tile(const Geom &g, const TwoStepPtrs &
   const int i0 = tx * kTW, j0 = ty * TH;
   const int ncols = min(kTW, g.nx - i0), nrows = min(TH, g.ny - j0);
   R Fk[NQ], Nk[NQ];
+  // step s+1's deposit, when summed here from its bins
+  bool binned = false;
+  if constexpr (V == RBN_VARIANT_NUDGING && TH == kDepTileH) {
+    if (a.bin2) {
+      bin_deposit<R, TH, GR>(g, a, tx, ty, L);
+      binned = true;
+    }
+  }
 
   // Phase 1: step s.  Tile rows / columns up to nrows+1 / ncols+1 (the ring of a cut-short
   // tile lies there).
@@ -521,7 +611,18 @@ __device__ __forceinline__ void two_step_tile(const Geom &g, const TwoStepPtrs &
     if (V == RBN_VARIANT_NUDGING) {
       gather_nudged<R>(acc, bot, top, y == 1, y == g.ny - 2, f, q9);
       R nud;
-      collide_n(c, f, q9, Fk[q], node_nudge<R>(a, 1, o), Nk[q], fo, go, rho, T, u, v, Fn, nud);
+      NodeNudge<R> d2;
+      if constexpr (V == RBN_VARIANT_NUDGING && TH == kDepTileH) {
+        if (binned) {
+          const Real2<R> v = L.d2[n];
+          d2 = {v.y >= R(0), v.x, v.y};
+        } else {
+          d2 = node_nudge<R>(a, 1, o);
+        }
+      } else {
+        d2 = node_nudge<R>(a, 1, o);
+      }
+      collide_n(c, f, q9, Fk[q], d2, Nk[q], fo, go, rho, T, u, v, Fn, nud);
       ((R *)a.nud_out)[o] = nud;
     } else {
       gather_truth<R>(acc, bot, top, c, f, q9);
