@@ -66,6 +66,17 @@ void free_dep2(rbn_ctx *c) {
   s = rbn::DepPtrs{};
 }
 
+hipError_t clear_dep_bins(const rbn::DepPtrs &d, const rbn::Geom &g, hipStream_t s) {
+  if (d.mode != RBN_DEPOSIT_BILINEAR) return hipSuccess;
+  const size_t plane = g.plane, tiles = (size_t)rbn::dep_tiles(g);
+  hipError_t e = hipSuccess;
+  if (d.tcnt && e == hipSuccess) e = hipMemsetAsync(d.tcnt, 0, tiles * rbn::kTcntStride * sizeof(int), s);
+  if (d.ocnt && e == hipSuccess) e = hipMemsetAsync(d.ocnt, 0, plane * sizeof(int), s);
+  if (d.S && e == hipSuccess) e = hipMemsetAsync(d.S, 0, plane * sizeof(long long), s);
+  if (d.Wi && e == hipSuccess) e = hipMemsetAsync(d.Wi, 0, plane * sizeof(long long), s);
+  return e;
+}
+
 int ensure_dep2(rbn_ctx *c) {
   for (hipEvent_t *ev : {&c->ev_dep[0], &c->ev_dep[1], &c->ev_lat[0], &c->ev_lat[1]})
     if (!*ev) HIPCHK(c, hipEventCreateWithFlags(ev, hipEventDisableTiming));
@@ -1414,7 +1425,18 @@ int rbn_step(rbn_ctx *c, int nsteps) {
   rc = step_loop(c, nsteps);
   // The last step's particle update (particle_fuse) has no lattice launch to ride in.
   const int rf = flush_particles(c);
-  if (rc) return rc;
+  if (rc) {
+    // a failure between an observer update and its tile pass (a provider error) leaves
+    // bilinear tile counters set: clear them, so that later deposits start clean
+    if (nudged) {
+      (void)hipStreamSynchronize(c->stream);
+      (void)hipStreamSynchronize(c->pstream);
+      (void)clear_dep_bins(c->dep, c->geom, c->stream);
+      if (c->dep2.cnt) (void)clear_dep_bins(c->dep2, c->geom, c->stream);
+      (void)hipStreamSynchronize(c->stream);
+    }
+    return rc;
+  }
   if (rf) return rf;
   if (parts) HIPCHK(c, hipStreamSynchronize(c->pstream));
   c->part_record_pending = false;

@@ -57,6 +57,9 @@ struct rbn_ensemble {
   // staging region), two staging sets (uploaded whole before a replay) and two graph
   // instances alternating between replays
   int ncycle = 0, cyc_t0 = 1;
+  // first call step of a replay's record window: cyc_t0, or cyc_t0 + 1 when two-step
+  // launches carry the next pair's first observer update (its records load one launch early)
+  int plan_t0 = 1;
   struct CycleLoad {
     int t;
     std::vector<int> ld;
@@ -216,6 +219,7 @@ void fill_member(const rbn_ctx *c, const rbn::DepPtrs *spare, int tile_w, int ti
 
 size_t loading_members(const rbn_ensemble *e, int t, std::vector<int> &ld);
 bool pairs(const rbn_ensemble *e);
+bool rides(const rbn_ensemble *e);
 
 // Observer/deposit descriptors of the nudged members as of their current step, the block
 // map of the batched passes, and the record uploads of the captured cycle.
@@ -277,9 +281,10 @@ int build_nudged(rbn_ensemble *e) {
   }
   e->ncycle = L <= kMaxNudgedCycle ? (int)L : 0;
   e->cyc_t0 = pairs(e) ? 2 : 1;
+  e->plan_t0 = e->cyc_t0 + (rides(e) ? 1 : 0);
   e->cyc.clear();
   e->cyc_bytes = 0;
-  for (int t = e->cyc_t0; t < e->cyc_t0 + e->ncycle; t++) {
+  for (int t = e->plan_t0; t < e->plan_t0 + e->ncycle; t++) {
     rbn_ensemble::CycleLoad cl;
     cl.t = t;
     cl.bytes = loading_members(e, t, cl.ld);
@@ -364,6 +369,14 @@ bool separate(const rbn_ensemble *e) { return (e->order & 4) != 0 && e->nudged.e
 // launch.
 bool pairs(const rbn_ensemble *e) { return (e->order & 8) != 0 && !separate(e); }
 
+// Two steps per launch with every nudged member on the bilinear deposit: a two-step launch
+// carries the observer update of the next pair's first step as extra work items (it writes
+// that step's bins and the observers, which the launch does not read; a 7x7 member's update
+// shares its launch with the zero pass of the deposit set the launch reads).
+bool rides(const rbn_ensemble *e) {
+  return pairs(e) && !e->nudged.empty() && e->plan.tot[rbn::NB_ZERO] == 0 && e->plan.tot[rbn::NB_CAND] == 0;
+}
+
 // order bit 4: no graphs, every launch eager (for debugging and tracing tools).
 bool graphs(const rbn_ensemble *e) { return (e->order & 16) == 0; }
 
@@ -375,6 +388,7 @@ int graph_steps(const rbn_ensemble *e) { return pairs(e) ? 2 * rbn::kCycle : rbn
 // updates waiting).
 struct PairState {
   int t = 0, L = 0, P = 0, pending = 0;
+  int head_done = -1;  // the step whose observer update an earlier launch carried
 };
 
 // 0: truth runs only, 1: nudged runs only, 2: both (the two-step kernels compile each apart).
@@ -382,15 +396,16 @@ int composition(const rbn_ensemble *e) {
   return e->nudged.empty() ? 0 : (e->nudged.size() == e->m.size() ? 1 : 2);
 }
 
-int launch_pairs(rbn_ensemble *e, const rbn::EnsStep &st) {
+int launch_pairs(rbn_ensemble *e, const rbn::EnsStep &st, const rbn::EnsHead &head = rbn::EnsHead{}) {
   EHIP(e, rbn::launch_ensemble_pairs(e->stream, e->prec, e->d_members, e->d_items, e->n_lbm, e->n_part, e->n_lbm2,
-                                     e->d_counters, st, e->tile_w, e->tile_h, grid_size(e), composition(e)));
+                                     e->d_counters, st, e->tile_w, e->tile_h, grid_size(e), composition(e), head));
   return RBN_OK;
 }
 
 // The next launch of the two-step mode: two steps when two remain, carrying what is pending.
 // bins2: the bilinear members' second-step deposits are summed in the tiles.
-int enqueue_pair(rbn_ensemble *e, PairState &ps, int nsteps, bool moments, bool bins2 = false) {
+int enqueue_pair(rbn_ensemble *e, PairState &ps, int nsteps, bool moments, bool bins2 = false,
+                 const rbn::EnsHead &head = rbn::EnsHead{}) {
   const bool two = ps.t + 1 < nsteps;
   rbn::EnsStep st{};
   st.k = ps.t % rbn::kCycle;
@@ -402,7 +417,7 @@ int enqueue_pair(rbn_ensemble *e, PairState &ps, int nsteps, bool moments, bool 
   st.reverse = (e->order & 2) && (ps.L & 1);
   st.cset = ps.L & 1;
   st.bins2 = two && bins2 ? 1 : 0;
-  int rc = launch_pairs(e, st);
+  int rc = launch_pairs(e, st, head);
   if (rc) return rc;
   ps.pending = two ? 2 : 1;
   ps.t += two ? 2 : 1;
@@ -622,17 +637,34 @@ static_assert(rbn::kEnsTH32 == rbn::kDepTileH, "FP32 ensemble tiles are deposit 
 // launch, which read both sets.  The first step's bilinear tile passes ride in the second
 // step's first launch; the second step's are summed in the lattice launch's tiles, except in
 // the call's last launch (its deposit fills the arrays the getters read).
+// When the launch is followed by another two-step launch and rides(e), it carries that
+// launch's first observer update (its records loaded before it); a step whose update an
+// earlier launch carried runs only its tile pass.
 int enqueue_nudged_pair(rbn_ensemble *e, PairState &ps, int nsteps, bool moments) {
-  const bool two = ps.t + 1 < nsteps;
-  const bool in_tile = two && tile_bins(e) && ps.t + 2 < nsteps;
-  for (int t = ps.t; t < ps.t + (two ? 2 : 1); t++) {
-    int rc = load_records(e, t, e->stream);
-    if (rc) return rc;
-    const bool second = t > ps.t;
+  const int t0 = ps.t;
+  const bool two = t0 + 1 < nsteps;
+  const bool in_tile = two && tile_bins(e) && t0 + 2 < nsteps;
+  const bool carry = rides(e) && two && t0 + 3 < nsteps;
+  int rc;
+  for (int t = t0; t < t0 + (two ? 2 : 1); t++) {
+    const bool second = t > t0;
+    if (t == ps.head_done) {
+      // observers updated and binned by the previous launch: the tile pass, unless deferred
+      if (!two) EHIP(e, rbn::launch_nudge_tiles(e->stream, nudge_row(e, t), e->plan, t == nsteps - 1));
+      continue;
+    }
+    if ((rc = load_records(e, t, e->stream))) return rc;
     EHIP(e, rbn::launch_nudge_batch(e->stream, nudge_row(e, t), e->plan, t, second ? nudge_row(e, t - 1) : nullptr,
                                     second ? in_tile : two, t == nsteps - 1));
   }
-  return enqueue_pair(e, ps, nsteps, moments, in_tile);
+  rbn::EnsHead head{};
+  if (carry) {
+    if ((rc = load_records(e, t0 + 2, e->stream))) return rc;
+    head = {nudge_row(e, t0 + 2), e->plan.map + e->plan.off[rbn::NB_OBS], e->plan.tot[rbn::NB_OBS], t0 + 2};
+  }
+  if ((rc = enqueue_pair(e, ps, nsteps, moments, in_tile, head))) return rc;
+  ps.head_done = carry ? t0 + 2 : -1;
+  return RBN_OK;
 }
 
 // Capture the cycle of nudged steps t0 .. t0 + L - 1 of a call twice, the scatters reading
@@ -652,30 +684,46 @@ int capture_nudged(rbn_ensemble *e) {
     EHIP(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
     int rc = RBN_OK;
     size_t next = 0;
-    // tail: the previous step's tile passes ride along (two-step mode, a pair's second step);
-    // defer: this step's wait for the next (a pair's first step)
-    auto deposit = [&](int t, bool tail, bool defer) -> int {
+    // the record scatter of step t, where members start an interval
+    auto scatter = [&](int t) -> int {
       if (next < e->cyc.size() && e->cyc[next].t == t) {
         const auto &cl = e->cyc[next++];
         if (rbn::launch_obs_scatter(e->stream, (const rbn::ObsJob *)(st.d + cl.off), (int)cl.ld.size(),
                                     st.d + cl.off) != hipSuccess)
           return efail(e, RBN_EDEVICE, "record scatter capture failed");
       }
+      return RBN_OK;
+    };
+    // tail: the previous step's tile passes ride along (two-step mode, a pair's second step);
+    // defer: this step's wait for the next (a pair's first step)
+    auto deposit = [&](int t, bool tail, bool defer) -> int {
+      int r = scatter(t);
+      if (r) return r;
       if (rbn::launch_nudge_batch(e->stream, nudge_row(e, t), e->plan, t, tail ? nudge_row(e, t - 1) : nullptr,
                                   defer, false) != hipSuccess)
         return efail(e, RBN_EDEVICE, "observer/deposit capture failed");
       return RBN_OK;
     };
     if (pairs(e)) {
-      // launches 1 .. L / 2 of a call (steps 2 .. L + 1), two steps each, carrying two updates
+      // launches 1 .. L / 2 of a call (steps 2 .. L + 1), two steps each, carrying two updates;
+      // with rides(e) each also carries the next launch's first observer update (the first
+      // launch's was carried by the launch before the replay)
       PairState ps;
       ps.t = 2;
       ps.L = 1;
       ps.P = 1;
       ps.pending = 2;
+      const bool ride = rides(e);
       for (int l = 0; l < e->ncycle / 2 && !rc; l++) {
-        if (!(rc = deposit(ps.t, false, true)) && !(rc = deposit(ps.t + 1, true, tile_bins(e))))
-          rc = enqueue_pair(e, ps, 1 << 30, false, tile_bins(e));
+        const int t0 = ps.t;
+        if (!ride) rc = deposit(t0, false, true);
+        if (!rc) rc = deposit(t0 + 1, true, tile_bins(e));
+        rbn::EnsHead head{};
+        if (!rc && ride) {
+          rc = scatter(t0 + 2);
+          head = {nudge_row(e, t0 + 2), e->plan.map + e->plan.off[rbn::NB_OBS], e->plan.tot[rbn::NB_OBS], t0 + 2};
+        }
+        if (!rc) rc = enqueue_pair(e, ps, 1 << 30, false, tile_bins(e), head);
       }
     } else {
       for (int t = 1; t <= e->ncycle && !rc; t++)
@@ -755,6 +803,7 @@ int run_nudged(rbn_ensemble *e, int nsteps) {
     if (two) {
       ps.L += cycles * L / 2;
       ps.P += cycles * L / 2;
+      ps.head_done = rides(e) ? ps.t : -1;  // the last replayed launch carried it
     } else {
       // the eager steps after the replays: their deposits follow the replays' work, and the
       // lattice events they wait for are recorded again outside the captured graphs
@@ -980,6 +1029,16 @@ int restore_snapshot(rbn_ensemble *e, Snapshot &s, bool keep = false) {
     const hipError_t he = hipMemcpy(s.src[k].first, s.bufs[k], s.src[k].second, hipMemcpyDeviceToDevice);
     if (he != hipSuccess && first == hipSuccess) first = he;
   }
+  // a failed trial can stop between an observer update and its tile pass: the bilinear tile
+  // counters and overflow sums (zero between deposits) of both sets go back to zero
+  for (int q : e->nudged) {
+    const rbn_ctx *c = e->m[(size_t)q];
+    hipError_t he = rbn::host::clear_dep_bins(c->dep, c->geom, e->stream);
+    if (he == hipSuccess && (size_t)q < e->spare.size() && e->spare[(size_t)q].cnt)
+      he = rbn::host::clear_dep_bins(e->spare[(size_t)q], c->geom, e->stream);
+    if (he != hipSuccess && first == hipSuccess) first = he;
+  }
+  if (hipStreamSynchronize(e->stream) != hipSuccess && first == hipSuccess) first = hipErrorUnknown;
   if (!keep) free_snapshot(s);
   e->built = false;
   drop_graph(e);

@@ -66,8 +66,17 @@ hipError_t launch_ensemble_phase(hipStream_t s, int prec, int phase, const void 
 struct EnsStep {
   int k, lpar, fpar, ncarry, lat, moments, reverse, cset, bins2;
 };
+// The observer update (with the bilinear deposit's binning) of a later step carried by a
+// two-step launch as extra work items, one per block of the batched observer pass: the nudged
+// members' descriptors of that step's deposit set, the pass's block map, its n blocks and the
+// step t in the call.
+struct EnsHead {
+  const NudgeMember *M;
+  const int4 *map;
+  int n, t;
+};
 hipError_t launch_ensemble_pairs(hipStream_t s, int prec, const void *members, const int2 *items, int n_lbm,
                                  int n_part, int n_lbm2, int *counters, const EnsStep &st, int tile_w,
-                                 int tile_h, int grid, int nudged);
+                                 int tile_h, int grid, int nudged, const EnsHead &head);
 
 }  // namespace rbn

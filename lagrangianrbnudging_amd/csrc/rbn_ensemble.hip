@@ -28,6 +28,7 @@
 #include "rbn_lbm_node.h"
 #include "rbn_particle_node.h"
 #include "rbn_ensemble.h"
+#include "rbn_nudge_node.h"
 
 namespace rbn {
 namespace {
@@ -241,14 +242,20 @@ template <typename R, int TH>
 struct PairLds<R, TH, kOneStep> {};
 
 // Item `it` of a two-step-mode launch: carried particle items first, then lattice items
-// (two-step tiles, or one-step tiles for kOneStep).
+// (two-step tiles, or one-step tiles for kOneStep), then the carried observer-update items
+// (last: they fill the launch's drain, when lattice items run out).
 template <typename R, int TH, int KIND>
 __device__ __forceinline__ void run_item2(const EnsMember<R> *__restrict__ M, const int2 *__restrict__ items, int it,
                                           int n_lbm, int n_part, int n_first, int n_lat, const EnsStep &st,
-                                          int tile_w, int tile_h, PairLds<R, TH, KIND> &S) {
+                                          const EnsHead &head, int tile_w, int tile_h, PairLds<R, TH, KIND> &S) {
   if (it < n_first) {
     const int2 w = items[n_lbm + it];
     for (int u = 0; u < st.ncarry; u++) particle_item(M[w.x], w.y, st.k - st.ncarry + u);
+    return;
+  }
+  if (it >= n_first + n_lat) {
+    const int4 w = head.map[it - n_first - n_lat];
+    obs::head_one(head.M[w.x], w.y * kNudgeBlock + threadIdx.x, head.t);
     return;
   }
   const int l = it - n_first, lr = st.reverse ? n_lat - 1 - l : l;
@@ -268,19 +275,18 @@ __device__ __forceinline__ void run_item2(const EnsMember<R> *__restrict__ M, co
 // waves), constrained 168 without spills.
 template <typename R, int TH, int KIND>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(KIND == kPairsNudged ? 3 : 1)))
-void k_ensemble2(const EnsMember<R> *__restrict__ M,
-                                                        const int2 *__restrict__ items, int n_lbm, int n_part,
-                                                        int n_lbm2, int *counters, EnsStep st, int tile_w,
-                                                        int tile_h, int persistent) {
+void k_ensemble2(const EnsMember<R> *__restrict__ M, const int2 *__restrict__ items, int n_lbm, int n_part,
+                 int n_lbm2, int *counters, EnsStep st, EnsHead head, int tile_w, int tile_h, int persistent) {
   __shared__ PairLds<R, TH, KIND> L;
   const int slot = st.lat ? 0 : 1;
   const int n_first = st.ncarry > 0 ? n_part : 0;
   const int n_lat = st.lat == 2 ? n_lbm2 : (st.lat == 1 ? n_lbm : 0);
-  const int n_items = n_first + n_lat;
+  const int n_items = n_first + n_lat + head.n;
   clear_next(counters, st.cset, slot);
   if (!persistent) {
     const int it = blockIdx.x;
-    if (it < n_items) run_item2<R, TH, KIND>(M, items, it, n_lbm, n_part, n_first, n_lat, st, tile_w, tile_h, L);
+    if (it < n_items)
+      run_item2<R, TH, KIND>(M, items, it, n_lbm, n_part, n_first, n_lat, st, head, tile_w, tile_h, L);
     return;
   }
   __shared__ int item;
@@ -291,7 +297,7 @@ void k_ensemble2(const EnsMember<R> *__restrict__ M,
     const int it = __builtin_amdgcn_readfirstlane(item);  // uniform: descriptors in scalar registers
     __syncthreads();
     if (it >= n_items) break;
-    run_item2<R, TH, KIND>(M, items, it, n_lbm, n_part, n_first, n_lat, st, tile_w, tile_h, L);
+    run_item2<R, TH, KIND>(M, items, it, n_lbm, n_part, n_first, n_lat, st, head, tile_w, tile_h, L);
   }
 }
 
@@ -315,8 +321,11 @@ hipError_t launch_ensemble_phase(hipStream_t s, int prec, int phase, const void 
 
 hipError_t launch_ensemble_pairs(hipStream_t s, int prec, const void *members, const int2 *items, int n_lbm,
                                  int n_part, int n_lbm2, int *counters, const EnsStep &st, int tile_w,
-                                 int tile_h, int grid, int nudged) {
-  const int n_items = (st.ncarry > 0 ? n_part : 0) + (st.lat == 2 ? n_lbm2 : (st.lat == 1 ? n_lbm : 0));
+                                 int tile_h, int grid, int nudged, const EnsHead &head_in) {
+  EnsHead head = head_in;
+  if (st.lat != 2 || !head.M) head.n = 0;  // only two-step launches carry observer updates
+  const int n_items =
+      (st.ncarry > 0 ? n_part : 0) + (st.lat == 2 ? n_lbm2 : (st.lat == 1 ? n_lbm : 0)) + head.n;
   if (n_items <= 0) return hipSuccess;
   const int persistent = grid > 0;
   const dim3 g(persistent ? (unsigned)grid : (unsigned)n_items), b(kThreads);
@@ -324,8 +333,8 @@ hipError_t launch_ensemble_pairs(hipStream_t s, int prec, const void *members, c
   const auto *M32 = (const EnsMember<float> *)members;
   const int kind = st.lat != 2 ? kOneStep : (nudged == 0 ? kPairsTruth : (nudged == 1 ? kPairsNudged : kPairsMixed));
 #define RBN_ENS2(R, M, TH, KIND)                                                                                     \
-  hipLaunchKernelGGL((k_ensemble2<R, TH, KIND>), g, b, 0, s, M, items, n_lbm, n_part, n_lbm2, counters, st, tile_w, \
-                     tile_h, persistent)
+  hipLaunchKernelGGL((k_ensemble2<R, TH, KIND>), g, b, 0, s, M, items, n_lbm, n_part, n_lbm2, counters, st, head, \
+                     tile_w, tile_h, persistent)
 #define RBN_ENS2_KINDS(R, M, TH)           \
   switch (kind) {                          \
     case kPairsTruth:                      \

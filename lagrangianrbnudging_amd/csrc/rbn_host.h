@@ -97,6 +97,9 @@ int check_err_flag(rbn_ctx *c);
 int check_sample_args(rbn_ctx *c, const char *who, int field, int n, const double *x, const double *y,
                       const double *out);
 int sample_staged(rbn_ctx *c, int field, int n, const double *x, const double *y, double *out, double *buf);
+// Clear a bilinear deposit set's tile counters and overflow sums (zero between deposits) on
+// stream s: a call that failed between an observer update and its tile pass leaves them set.
+hipError_t clear_dep_bins(const rbn::DepPtrs &d, const rbn::Geom &g, hipStream_t s);
 
 }  // namespace host
 }  // namespace rbn

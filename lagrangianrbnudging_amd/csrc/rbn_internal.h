@@ -280,7 +280,9 @@ hipError_t launch_particle_init(hipStream_t s, int prec, const Geom &g, const vo
 hipError_t launch_obs_deposit(hipStream_t s, int branch, int ntempread, const Geom &g, int right_limit,
                               const ObsPtrs &o, const DepPtrs &d, const DepPtrs *tail = nullptr,
                               bool defer = false);
-// Blocks member d takes in each pass of a batched observer/deposit launch.
+// Blocks member d takes in each pass of a batched observer/deposit launch (kNudgeBlock
+// threads each; an observer pass block updates observers lb * kNudgeBlock onward).
+constexpr int kNudgeBlock = 256;
 void nudge_batch_blocks(const NudgeMember &d, int blocks[kNudgeBatches]);
 // The block map of the batched passes: for every block of pass p (entries off[p] ..
 // off[p] + tot[p] - 1 of map, device memory, passes in order) its member, its index among
@@ -299,6 +301,8 @@ struct NudgePlan {
 // packed deposit the collisions read.
 hipError_t launch_nudge_batch(hipStream_t s, const NudgeMember *members, const NudgePlan &plan, int t,
                               const NudgeMember *tail = nullptr, bool defer = false, bool api = true);
+// The bilinear members' tile passes alone (their observer updates ran earlier).
+hipError_t launch_nudge_tiles(hipStream_t s, const NudgeMember *members, const NudgePlan &plan, bool api);
 // Scatter njobs staged records (jobs and data in the device staging buffer) to their observers.
 hipError_t launch_obs_scatter(hipStream_t s, const ObsJob *jobs, int njobs, const unsigned char *staging);
 hipError_t launch_energy(hipStream_t s, int prec, const Geom &g, const void *rho, const void *uv,

@@ -43,95 +43,18 @@
 #include <algorithm>
 
 #include "rbn_internal.h"
+#include "rbn_nudge_node.h"
 
 namespace rbn {
 namespace {
 
-constexpr int kBlock = 256;
+using namespace obs;
+
+constexpr int kBlock = kNudgeBlock;
 constexpr int kScatterSplit = 16;  // blocks per record of a batched upload
 constexpr int kScanItems = 4;
 constexpr int kScanTile = kBlock * kScanItems;
 static_assert(kScanTile == kDepScanTile, "tile totals are sized by kDepScanTile");
-
-// ---------------------------------------------------------------- observer update
-
-__device__ __forceinline__ void advance_x(const ObsPtrs &o, int i, double nxd) {
-  double xposnow = o.xint[i] + o.dx[i];
-  if (xposnow > nxd) xposnow = xposnow - nxd;
-  if (xposnow < 0.0) xposnow = xposnow + nxd;
-  o.xint[i] = xposnow;
-  o.xidx[i] = (int)round(xposnow);
-}
-
-__device__ __forceinline__ void advance_y_T(const ObsPtrs &o, int i, double nyd) {
-  double yposnow = o.yint[i] + o.dy[i];
-  int bc = o.bcy[i];
-  if (yposnow > nyd) {
-    yposnow = yposnow - nyd + 1.0;
-    bc = 2;
-  }
-  if (yposnow < 1.0) {
-    yposnow = yposnow + nyd - 1.0;
-    bc = 2;
-  }
-  o.yint[i] = yposnow;
-  o.yidx[i] = (int)round(yposnow);
-  double Tnow = o.Tint[i] + o.dT[i];
-  if (bc == 1) Tnow = o.Tint[i];
-  if (bc == 2) {
-    Tnow = o.rT[o.chk[i]];
-    bc = 1;
-  }
-  o.bcy[i] = bc;
-  o.Tint[i] = Tnow;
-}
-
-// branch: 1 = istep%ntempread==1 (new record in rx/ry/rT, N:1489-1552), 0 = snap to the
-// record (N:1621-1628), 2 = advance by the slopes (N:1687-1720).
-__device__ __forceinline__ void obs_update_one(const ObsPtrs &o, int i, int branch, int nx, int ny, int ntempread) {
-  const double nxd = (double)nx, nyd = (double)ny, ntr = (double)ntempread;
-  const int c = o.chk[i];
-  if (branch == 1) {
-    o.bcy[i] = 0;
-    double xread = o.rx[c], yread = o.ry[c];
-    const double xint = o.xint[i], yint = o.yint[i];
-    double dx = (xread - xint) / (ntr);
-    if (abs((int)(xread - xint)) > (nxd / 2.0)) {  // abs() of a double: int truncation
-      if (xread > xint) {
-        xread = xread - nxd;
-        dx = (xread - xint) / (ntr);
-      } else {
-        xread = xread + nxd;
-        dx = (xread - xint) / (ntr);
-      }
-    }
-    o.dx[i] = dx;
-    advance_x(o, i, nxd);
-    double dy = (yread - yint) / (ntr);
-    if (abs((int)(yread - yint)) > (nyd / 2.0)) {
-      o.bcy[i] = 1;
-      if (yread > yint) {
-        yread = yread - nyd + 1.0;
-        dy = (yread - yint) / (ntr);
-      } else {
-        yread = yread + nyd - 1.0;
-        dy = (yread - yint) / (ntr);
-      }
-    }
-    o.dy[i] = dy;
-    o.dT[i] = (o.rT[c] - o.Tint[i]) / (ntr);
-    advance_y_T(o, i, nyd);
-  } else if (branch == 0) {
-    o.xint[i] = o.rx[c];
-    o.yint[i] = o.ry[c];
-    o.Tint[i] = o.rT[c];
-    o.xidx[i] = (int)round(o.rx[c]);
-    o.yidx[i] = (int)round(o.ry[c]);
-  } else {
-    advance_x(o, i, nxd);
-    advance_y_T(o, i, nyd);
-  }
-}
 
 // ---------------------------------------------------------------- deposit
 
@@ -163,29 +86,6 @@ __device__ __forceinline__ long long cell_of(const Geom &g, int right, int xi, i
   return -1;
 }
 
-// Bilinear deposit (RBN_DEPOSIT_BILINEAR, no reference counterpart): observer k gives each
-// corner c of its interpolation cell the record-interpolation weight (M:292-311, same
-// operand order), ghost corners folded onto the interior node they copy; zero weights are
-// not contributions.  Returns the node offset and the weight, or -1.
-constexpr int kBilCells = 4;
-
-__device__ __forceinline__ long long bil_cell(const Geom &g, double xx, double yy, int c, double &w) {
-  if (!(xx > -1.0 && xx < (double)g.nx + 1.0 && yy > -1.0 && yy < (double)g.ny + 1.0)) return -1;
-  const int x1 = (int)xx, y1 = (int)yy, x2 = x1 + 1, y2 = y1 + 1;
-  const int ic = (c & 1) ? x2 : x1, jc = (c & 2) ? y2 : y1;
-  const double wx = (c & 1) ? (xx - (double)x1) : ((double)x2 - xx);
-  const double wy = (c & 2) ? (yy - (double)y1) : ((double)y2 - yy);
-  w = wx * wy;
-  if (w == 0.0) return -1;
-  const int I = ic == 0 ? g.nx : (ic == g.nx + 1 ? 1 : ic);
-  const int J = jc == 0 ? g.ny : (jc == g.ny + 1 ? 1 : jc);
-  return (long long)(J - 1) * g.pitch + (I - 1);
-}
-
-// Fixed point of the bilinear deposit's sums: v * 2^40 rounded to the nearest integer, ties
-// to even (the scaling is exact).
-__device__ __forceinline__ long long dep_fix(double v) { return (long long)rint(v * 0x1p40); }
-
 // The packed deposit (DepPtrs::tw) at node c: the mean and weight w (stored negated), or
 // the empty value.
 __device__ __forceinline__ void tw_put(const DepPtrs &d, size_t c, double Tbar, double w) {
@@ -211,10 +111,6 @@ __device__ __forceinline__ void zero_part(size_t n, const DepPtrs &d, int b, int
     tw_zero(d, o);
   }
 }
-
-// Bilinear deposit: observer k's contributions to its cell's corners, each binned into a slot
-// of its node's deposit tile, or past the tile's slots added to the node's sums.
-__device__ __forceinline__ void bin_one(const Geom &g, const DepPtrs &d, const ObsPtrs &o, int k);
 
 // 7x7 deposit: candidate t (observer t / kCells, cell t % kCells) counts one contribution
 // at its node.
@@ -339,52 +235,6 @@ __device__ __forceinline__ void fill_one(const Geom &g, int right, const ObsPtrs
 __global__ void k_dep_fill(Geom g, int right, ObsPtrs o, const int *off, const int *tile_sum, DepPtrs d) {
   fill_one(g, right, o, off, tile_sum, d.fill, d.list, d.list_cap, d.overflow, d.dense ? nullptr : d.active,
            d.nactive, d.active_cap, (long long)blockIdx.x * blockDim.x + threadIdx.x);
-}
-
-// The corners that share a tile (all four but near a tile's edge) take their slots with one
-// atomic.
-__device__ __forceinline__ void bin_one(const Geom &g, const DepPtrs &d, const ObsPtrs &o, int k) {
-  const double xx = o.xint[k], yy = o.yint[k], Tk = o.Tint[k];
-  const int tiles_x = dep_tiles_x(g);
-  long long node[kBilCells], wf[kBilCells], wTf[kBilCells];
-  int tile[kBilCells], local[kBilCells];
-#pragma unroll
-  for (int c = 0; c < kBilCells; c++) {
-    double w = 0.0;
-    node[c] = bil_cell(g, xx, yy, c, w);
-    wf[c] = dep_fix(w);
-    wTf[c] = dep_fix(w * Tk);
-    const int j = node[c] >= 0 ? (int)(node[c] / g.pitch) : 0, i = node[c] >= 0 ? (int)(node[c] - (long long)j * g.pitch) : 0;
-    tile[c] = node[c] >= 0 ? (j / kDepTileH) * tiles_x + i / kDepTileW : -1;  // -1: no contribution
-    local[c] = (j % kDepTileH) * kDepTileW + i % kDepTileW;
-  }
-#pragma unroll
-  for (int c = 0; c < kBilCells; c++) {
-    if (tile[c] < 0) continue;
-    const int t = tile[c];
-    int n = 0;
-#pragma unroll
-    for (int c2 = c; c2 < kBilCells; c2++) n += tile[c2] == t;
-    int slot = atomicAdd(&d.tcnt[(size_t)t * kTcntStride], n);
-#pragma unroll
-    for (int c2 = c; c2 < kBilCells; c2++) {
-      if (tile[c2] != t) continue;
-      tile[c2] = -1;  // taken
-      if (slot < d.bin_cap) {
-        // w in the high bits (times 2^kDepTileBits: a weight of either sign survives the
-        // arithmetic shift that takes it out), the node's index in the tile in the low ones
-        longlong2 e;
-        e.x = wTf[c2];
-        e.y = wf[c2] * (1LL << kDepTileBits) + local[c2];
-        ((longlong2 *)d.bin)[(size_t)t * d.bin_cap + slot] = e;
-      } else {
-        atomicAdd((unsigned long long *)&d.S[node[c2]], (unsigned long long)wTf[c2]);
-        atomicAdd((unsigned long long *)&d.Wi[node[c2]], (unsigned long long)wf[c2]);
-        atomicAdd(&d.ocnt[node[c2]], 1);
-      }
-      slot++;
-    }
-  }
 }
 
 // Bilinear deposit, deposit tile t: the sums of its binned contributions in LDS (exact, in
@@ -620,14 +470,8 @@ __global__ __launch_bounds__(kBlock) void k_nb_head(const NudgeMember *M, const 
   const int4 w = hmap[blockIdx.x - ntail];
   const NudgeMember &d = M[w.x];
   if (w.w == NB_OBS) {
-    const int i = w.y * kBlock + threadIdx.x;
-    // the step's place in the record interval, as the single-member loop derives it
-    const int istep = d.istep0 + t + 1, r = istep % d.ntempread;
-    const int branch = r == 1 ? 1 : (r == 0 ? 0 : 2);
-    if (i < d.o.n) {
-      obs_update_one(d.o, i, branch, d.g.nx, d.g.ny, d.ntempread);
-      if (d.d.mode == RBN_DEPOSIT_BILINEAR) bin_one(d.g, d.d, d.o, i);
-    }
+    // the step's place in the record interval as the single-member loop derives it
+    head_one(d, w.y * kBlock + threadIdx.x, t);
   } else {
     zero_part(d.g.plane, d.d, w.y, w.z);
   }
@@ -742,6 +586,13 @@ hipError_t launch_nudge_batch(hipStream_t s, const NudgeMember *members, const N
   if (tot[NB_FOLD] > 0) hipLaunchKernelGGL(k_nb_fold, dim3(tot[NB_FOLD]), b, 0, s, members, at(NB_FOLD));
   if (!defer && tot[NB_TILES] > 0)
     hipLaunchKernelGGL(k_nb_tiles, dim3(tot[NB_TILES]), b, 0, s, members, at(NB_TILES), api ? 1 : 0);
+  return hipGetLastError();
+}
+
+hipError_t launch_nudge_tiles(hipStream_t s, const NudgeMember *members, const NudgePlan &plan, bool api) {
+  if (plan.nm <= 0 || plan.tot[NB_TILES] <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_nb_tiles, dim3(plan.tot[NB_TILES]), dim3(kBlock), 0, s, members,
+                     plan.map + plan.off[NB_TILES], api ? 1 : 0);
   return hipGetLastError();
 }
 
