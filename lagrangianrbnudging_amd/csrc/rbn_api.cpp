@@ -1264,6 +1264,9 @@ int step_loop(rbn_ctx *c, int nsteps) {
   const bool npairs = nudged && pairs && !dovl && nsteps > 1;
   if (npairs && (rc = ensure_dep2(c))) return rc;
   if (nsteps > 0) c->at_init = false;
+  // nudged pairs, bilinear deposit: the step whose observer update (and binning into dep2) the
+  // previous two-step launch carried, when the next launch is a two-step one too
+  int head_done = -1;
   for (int n = 0; n < nsteps; n++) {
     const int istep = ++c->istep;
     const bool record = parts && (istep % P.noutconfig) == 0;
@@ -1284,13 +1287,27 @@ int step_loop(rbn_ctx *c, int nsteps) {
     // bins, except in the call's last launch (its deposit fills the arrays the getters read)
     const bool in_tile = nudged && pair && c->dep.mode == RBN_DEPOSIT_BILINEAR && c->tune.dep_in_tile &&
                          rbn::lbm2_tile_rows(c->prec, c->tune.tile_h) == rbn::kDepTileH && n < nsteps - 1;
+    // bilinear deposit: the two-step launch also carries the next two-step launch's first
+    // observer update (into dep2, whose bins the tile pass of this pair's first step has
+    // consumed; the launch reads dep2's packed deposit and dep's bins only)
+    bool carry_head = false;
+    int istep3 = 0;
+    if (nudged && pair && c->dep.mode == RBN_DEPOSIT_BILINEAR) {
+      istep3 = istep2 + 1;
+      const bool moments3 = (nd > 0 && (istep3 % nd) == 0) || n + 1 == nsteps - 1;
+      carry_head = !moments3 && n + 2 < nsteps;
+    }
+    const rbn::ObsPtrs o3 = nudged ? obs_ptrs(c) : rbn::ObsPtrs{};
     if (nudged && pair) {
       // the first step's bilinear tile pass rides in the second step's first launch
       const bool bil = c->dep.mode == RBN_DEPOSIT_BILINEAR;
-      if ((rc = nudged_deposit(c, istep, c->stream, c->dep2, nullptr, bil)) ||
-          (rc = nudged_deposit(c, istep2, c->stream, c->dep, bil ? &c->dep2 : nullptr, in_tile)))
+      if (head_done != istep && (rc = nudged_deposit(c, istep, c->stream, c->dep2, nullptr, bil))) return rc;
+      if ((rc = nudged_deposit(c, istep2, c->stream, c->dep, bil ? &c->dep2 : nullptr, in_tile))) return rc;
+      if (carry_head && istep3 % P.ntempread == 1 &&
+          (rc = load_observation(c, istep3 + P.ntempread - 1, c->stream)))
         return rc;
     } else if (nudged && !dovl) {
+      if (head_done == istep) return fail(c, RBN_ESTATE, "internal: a carried observer update before a single step");
       if ((rc = nudged_deposit(c, istep, c->stream, c->dep))) return rc;
     } else if (nudged) {
       // step n's deposit is in set n & 1 (dep, dep2), built on pstream: the first here, the
@@ -1340,6 +1357,13 @@ int step_loop(rbn_ctx *c, int nsteps) {
                np.tw2 = c->dep.tw;
                np.nud_out = c->nud2;
                np.bins2 = in_tile ? &c->dep : nullptr;
+               if (carry_head) {
+                 np.head_obs = &o3;
+                 np.head_dep = &c->dep2;
+                 const int r3 = istep3 % P.ntempread;
+                 np.head_branch = r3 == 1 ? 1 : (r3 == 0 ? 0 : 2);
+                 np.head_ntempread = P.ntempread;
+               }
              }
              return rbn::launch_lbm2(c->stream, c->prec, c->geom, lp, c->F2, c->uv[s2], moments2 ? c->rho : nullptr,
                                      moments2 ? c->T : nullptr, c->consts.data(), t.tile_h, t.seg, t.xcd2, t.variant,
@@ -1354,6 +1378,7 @@ int step_loop(rbn_ctx *c, int nsteps) {
       c->npend = 0;
       return rc;
     }
+    head_done = carry_head ? istep3 : -1;
     if (carry && (rc = pending_launched(c))) return rc;
     if (nudged && dovl) {
       HIPCHK(c, hipEventRecord(c->ev_lat[n & 1], c->stream));

@@ -246,6 +246,11 @@ struct NudgePair {
   // or (non-null, bilinear deposit, tiles of kDepTileH rows) the second step's set, whose tile
   // pass was deferred: its deposit is summed from the bins inside the launch
   const DepPtrs *bins2 = nullptr;
+  // (non-null, bilinear deposit) a later step's observer update and binning into head_dep,
+  // carried as extra workgroups after the tiles (branch of the state machine at that step)
+  const ObsPtrs *head_obs = nullptr;
+  const DepPtrs *head_dep = nullptr;
+  int head_branch = 0, head_ntempread = 1;
 };
 // Rows of the two-step tiles launch_lbm2 uses for precision prec and tuning tile_h.
 int lbm2_tile_rows(int prec, int tile_h);

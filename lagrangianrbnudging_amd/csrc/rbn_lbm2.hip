@@ -6,6 +6,7 @@
 // that is safe).
 #include "rbn_lbm2_tile.h"
 #include "rbn_lbm2_wave.h"
+#include "rbn_nudge_node.h"
 #include "rbn_particle_node.h"
 
 namespace rbn {
@@ -13,9 +14,14 @@ namespace {
 
 using namespace lbm2;
 
+// head_blocks > 0 (nudged run): workgroup k after the tiles updates observers
+// k kTileThreads onward to a later step and bins their contributions (rbn_nudge_node.h).
 struct Lbm2Args {
   TwoStepPtrs p;
   int tiles_x, ntiles, flip;
+  int head_blocks, head_branch, head_ntempread;
+  ObsPtrs head_obs;
+  DepPtrs head_dep;
 };
 
 // FP32 truth-run tiles of 64 x 8 run at most three workgroups per CU: with four (40.4 KB of
@@ -38,6 +44,16 @@ __global__ __launch_bounds__(kTileThreads) void k_lbm2(Geom g, Lbm2Args a, StepC
     return;
   }
   const int b = (int)blockIdx.x - ps.nblocks;
+  if constexpr (V == RBN_VARIANT_NUDGING) {
+    if (b >= a.ntiles) {
+      const int i = (b - a.ntiles) * kTileThreads + threadIdx.x;
+      if (i < a.head_obs.n) {
+        obs::obs_update_one(a.head_obs, i, a.head_branch, g.nx, g.ny, a.head_ntempread);
+        obs::bin_one(g, a.head_dep, a.head_obs, i);
+      }
+      return;
+    }
+  }
   const int t = a.flip ? a.ntiles - 1 - b : b;
   const int ty = t / a.tiles_x, tx = t - ty * a.tiles_x;
   two_step_tile<R, TH, V, ring_g<V>(), NT>(g, a.p, c, tx, ty, L);
@@ -100,7 +116,7 @@ void dispatch32(hipStream_t s, int th, const Geom &g, const Lbm2Args &a, const v
                 const PartJob *job) {
   const StepConst<float> &c = *(const StepConst<float> *)step_const;
   const part::PartSteps<float> ps = part::part_steps<float>(g, job, kTileThreads);
-  const dim3 grid((unsigned)(a.ntiles + ps.nblocks)), block(kTileThreads);
+  const dim3 grid((unsigned)(a.ntiles + a.head_blocks + ps.nblocks)), block(kTileThreads);
   if (th == 4)
     hipLaunchKernelGGL((k_lbm2<float, 4, V, NT>), grid, block, 0, s, g, a, c, ps);
   else if (th == 16)
@@ -114,7 +130,7 @@ void dispatch64(hipStream_t s, int th, const Geom &g, const Lbm2Args &a, const v
                 const PartJob *job) {
   const StepConst<double> &c = *(const StepConst<double> *)step_const;
   const part::PartSteps<double> ps = part::part_steps<double>(g, job, kTileThreads);
-  const dim3 grid((unsigned)(a.ntiles + ps.nblocks)), block(kTileThreads);
+  const dim3 grid((unsigned)(a.ntiles + a.head_blocks + ps.nblocks)), block(kTileThreads);
   if (th == 4)
     hipLaunchKernelGGL((k_lbm2<double, 4, V, NT>), grid, block, 0, s, g, a, c, ps);
   else
@@ -194,6 +210,14 @@ hipError_t launch_lbm2(hipStream_t s, int prec, const Geom &g, const LbmPtrs &p,
       a.p.S2 = np->bins2->S;
       a.p.Wi2 = np->bins2->Wi;
       a.p.ocnt2 = np->bins2->ocnt;
+    }
+    if (np->head_obs && np->head_dep) {
+      if (np->head_dep->mode != RBN_DEPOSIT_BILINEAR) return hipErrorInvalidValue;
+      a.head_obs = *np->head_obs;
+      a.head_dep = *np->head_dep;
+      a.head_branch = np->head_branch;
+      a.head_ntempread = np->head_ntempread;
+      a.head_blocks = (np->head_obs->n + kTileThreads - 1) / kTileThreads;
     }
   }
   a.tiles_x = (g.nx + kTW - 1) / kTW;
