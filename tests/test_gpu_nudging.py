@@ -101,6 +101,49 @@ def test_bilinear_deposit_run_fp64_bit_exact(alpha):
     assert np.count_nonzero(a["weight"]) > 0 and np.all(a["weight"][a["nudge_vol"] == 0] == 0)
 
 
+@pytest.mark.parametrize("lbm_steps", [1, 2])
+def test_bilinear_deposit_overflowing_a_tile_fp64_bit_exact(lbm_steps):
+    # 300 observers drifting inside one 64 x 8 deposit tile of a 40 x 24 lattice: 1200
+    # contributions against the tile's 896 slots, so the rest take the fixed-point sums in HBM
+    # that the tile pass (or the two-step launch's in-tile sum) adds and clears.  Against the
+    # oracle's sequential loop, bit for bit, one and two steps per launch.
+    pn = rb_params(40, 24, 1e5)
+    pn.variant = rbn.RBN_VARIANT_NUDGING
+    pn.ntempread = 4
+    pn.alpha = 0.5
+    pn.deposit = rbn.RBN_DEPOSIT_BILINEAR
+    n, every, steps = 300, 4, 49
+    rng = np.random.default_rng(17)
+    ids = np.arange(n, dtype=np.int32)
+    x0 = rng.uniform(6.0, 30.0, n)
+    y0 = rng.uniform(9.5, 14.5, n)  # rows 9 .. 15: the second deposit tile
+    vx = rng.uniform(-0.05, 0.05, n)
+    vy = rng.uniform(-0.01, 0.01, n)
+    T = rng.uniform(0.0, 1.0, n)
+    recs = {}
+    for s in range(every, steps + 2 * every, every):
+        perm = rng.permutation(n)
+        recs[s] = {"id": ids[perm], "x": (x0 + vx * s)[perm], "y": (y0 + vy * s)[perm],
+                   "temp": (T + 0.005 * s)[perm]}
+    T0n = conduction_T(pn, 1e-2, seed=4)
+    g = rbn.Solver(pn)
+    g.set_tuning(lbm_steps=lbm_steps)
+    g.init_fields(T=T0n)
+    g.init_observers(ids, x0, y0, T)
+    g.set_observation_source(recs)
+    o = Oracle(pn)
+    o.init_fields(T=T0n)
+    o.init_observers(ids, x0, y0, T)
+    o.set_observation_source(recs)
+    for k in (1, 20, 28):  # calls ending on either deposit set
+        g.step(k)
+        o.step(k)
+    _compare(g, o)
+    a, b = g.nudging(), o.nudging()
+    assert_equal_arrays(a["weight"], b["weight"], "weight")
+    assert int(a["nudge_vol"].sum()) > 896  # more contributions than the tile's slots
+
+
 def test_bilinear_deposit_fp32_against_fp64():
     pn = rb_params(40, 24, 1e5)
     pn.variant = rbn.RBN_VARIANT_NUDGING
