@@ -242,29 +242,40 @@ __global__ void k_dep_fill(Geom g, int right, ObsPtrs o, const int *off, const i
 // of the tile: the packed deposit, and with api also the count, and Tbar = S / W and W (each
 // sum converted to double) where the count is positive (the arrays the getters, the adjoint
 // and the ensemble's copies read; the nudged collision reads only the packed deposit).
-// Leaves the tile's counter at zero.  Every thread of the workgroup calls it.
+// Leaves the tile's counter at zero.  A workgroup runs two tiles, one per half (kTileHalf
+// threads): with 10 KB of LDS per tile, twice as many tiles are in flight per CU as with one
+// per workgroup, which hides the pass's latency (72 us for the ensemble's 26,880 tiles with
+// one per workgroup).  Every thread of the workgroup calls it, with its half's tile (t < 0:
+// none; the half still takes part in the barriers).
 struct DepTileLds {
   unsigned long long S[kDepTileW * kDepTileH], W[kDepTileW * kDepTileH];
   int n[kDepTileW * kDepTileH];
 };
+constexpr int kTileHalf = kBlock / 2;
+struct DepTilePairLds {
+  DepTileLds h[2];
+};
 
-__device__ __forceinline__ void dep_tile(const Geom &g, const DepPtrs &d, int t, DepTileLds &L, bool api) {
+__device__ __forceinline__ void dep_tile(const Geom &g, const DepPtrs &d, int t, DepTilePairLds &P, bool api) {
   constexpr int kNodes = kDepTileW * kDepTileH;
-  const longlong2 *e = (const longlong2 *)d.bin + (size_t)t * d.bin_cap;
+  const int tid = threadIdx.x % kTileHalf;
+  DepTileLds &L = P.h[threadIdx.x / kTileHalf];
+  const bool on = t >= 0;
+  const longlong2 *e = on ? (const longlong2 *)d.bin + (size_t)t * d.bin_cap : nullptr;
   // the counter and each thread's first two slots in one round trip (slots past the count
   // hold stale entries, read but not used)
-  const int n = d.tcnt[(size_t)t * kTcntStride];
-  const int k0 = threadIdx.x, k1 = threadIdx.x + kBlock;
+  const int n = on ? d.tcnt[(size_t)t * kTcntStride] : 0;
+  const int k0 = tid, k1 = tid + kTileHalf;
   longlong2 v0 = make_longlong2(0, 0), v1 = make_longlong2(0, 0);
-  if (k0 < d.bin_cap) v0 = e[k0];
-  if (k1 < d.bin_cap) v1 = e[k1];
-  for (int k = threadIdx.x; k < kNodes; k += kBlock) {
+  if (on && k0 < d.bin_cap) v0 = e[k0];
+  if (on && k1 < d.bin_cap) v1 = e[k1];
+  for (int k = tid; k < kNodes; k += kTileHalf) {
     L.S[k] = 0;
     L.W[k] = 0;
     L.n[k] = 0;
   }
   __syncthreads();
-  const int m = min(n, d.bin_cap);
+  const int m = on ? min(n, d.bin_cap) : 0;
   auto add = [&](const longlong2 &v) {
     const int local = (int)(v.y & (kNodes - 1));
     atomicAdd(&L.S[local], (unsigned long long)v.x);
@@ -273,10 +284,11 @@ __device__ __forceinline__ void dep_tile(const Geom &g, const DepPtrs &d, int t,
   };
   if (k0 < m) add(v0);
   if (k1 < m) add(v1);
-  for (int k = k1 + kBlock; k < m; k += kBlock) add(e[k]);
+  for (int k = k1 + kTileHalf; k < m; k += kTileHalf) add(e[k]);
   __syncthreads();
+  if (!on) return;
   const int tiles_x = dep_tiles_x(g), ty = t / tiles_x, tx = t - ty * tiles_x;
-  for (int k = threadIdx.x; k < kNodes; k += kBlock) {
+  for (int k = tid; k < kNodes; k += kTileHalf) {
     const int i = tx * kDepTileW + k % kDepTileW, j = ty * kDepTileH + k / kDepTileW;
     if (i >= g.nx || j >= g.ny) continue;
     const size_t c = (size_t)j * g.pitch + i;
@@ -302,7 +314,13 @@ __device__ __forceinline__ void dep_tile(const Geom &g, const DepPtrs &d, int t,
       tw_zero(d, c);
     }
   }
-  if (threadIdx.x == 0) d.tcnt[(size_t)t * kTcntStride] = 0;  // every thread read it before the first barrier
+  if (tid == 0) d.tcnt[(size_t)t * kTcntStride] = 0;  // every thread read it before the first barrier
+}
+
+// Tile index of this thread's half in a tile pass of ntiles tiles, block b (two per block).
+__device__ __forceinline__ int half_tile(int b, int ntiles) {
+  const int t = 2 * b + (int)threadIdx.x / kTileHalf;
+  return t < ntiles ? t : -1;
 }
 
 // The step's observer update, and per deposit: 7x7, the zero pass (blocks past the
@@ -312,10 +330,10 @@ __device__ __forceinline__ void dep_tile(const Geom &g, const DepPtrs &d, int t,
 // deposit whose bins are complete (it reads only its own set's arrays).
 __global__ __launch_bounds__(kBlock) void k_obs_zero(Geom g, ObsPtrs o, int branch, int ntempread, int tail_blocks,
                                                      int obs_blocks, DepPtrs d, DepPtrs tail) {
-  __shared__ DepTileLds L;
+  __shared__ DepTilePairLds L;
   const int b = blockIdx.x;
   if (b < tail_blocks) {
-    dep_tile(g, tail, b, L, true);
+    dep_tile(g, tail, half_tile(b, dep_tiles(g)), L, true);
   } else if (b < tail_blocks + obs_blocks) {
     const int i = (b - tail_blocks) * kBlock + threadIdx.x;
     if (i < o.n) {
@@ -329,8 +347,8 @@ __global__ __launch_bounds__(kBlock) void k_obs_zero(Geom g, ObsPtrs o, int bran
 }
 
 __global__ __launch_bounds__(kBlock) void k_dep_tiles(Geom g, DepPtrs d) {
-  __shared__ DepTileLds L;
-  dep_tile(g, d, blockIdx.x, L, true);
+  __shared__ DepTilePairLds L;
+  dep_tile(g, d, half_tile(blockIdx.x, dep_tiles(g)), L, true);
 }
 
 // Per node: order the contributions by observer and fold them in that order, the
@@ -455,16 +473,19 @@ inline int fold_blocks(const DepPtrs &d, int max_blocks) {
 }
 
 // The head of a step's batched deposit: the tile passes of the bilinear members' previous
-// deposit when it was deferred (the first ntail blocks, members `tail`, map entries `tmap`;
-// those read only their own set's bins and write their own set), then every member's
+// deposit when it was deferred (ntail_tiles map entries `tmap` of members `tail`, two per
+// block in the first blocks; they read only their own set's bins and write their own set),
+// then every member's
 // observer update (bilinear members bin their contributions) and the 7x7 members' zero
 // passes (map entries `hmap`, pass NB_OBS or NB_ZERO).
 __global__ __launch_bounds__(kBlock) void k_nb_head(const NudgeMember *M, const NudgeMember *tail, const int4 *tmap,
-                                                    int ntail, const int4 *hmap, int t) {
-  __shared__ DepTileLds L;
+                                                    int ntail_tiles, const int4 *hmap, int t) {
+  __shared__ DepTilePairLds L;
+  const int ntail = (ntail_tiles + 1) / 2;  // blocks of the tile passes, two tiles each
   if ((int)blockIdx.x < ntail) {
-    const int4 w = tmap[blockIdx.x];
-    dep_tile(tail[w.x].g, tail[w.x].d, w.y, L, false);
+    const int k = half_tile(blockIdx.x, ntail_tiles);
+    const int4 w = tmap[k < 0 ? 0 : k];
+    dep_tile(tail[w.x].g, tail[w.x].d, k < 0 ? -1 : w.y, L, false);
     return;
   }
   const int4 w = hmap[blockIdx.x - ntail];
@@ -522,10 +543,11 @@ __global__ __launch_bounds__(kBlock) void k_nb_fold(const NudgeMember *M, const 
 }
 
 // The bilinear members' tile passes (api: as dep_tile's).
-__global__ __launch_bounds__(kBlock) void k_nb_tiles(const NudgeMember *M, const int4 *map, int api) {
-  __shared__ DepTileLds L;
-  const int4 w = map[blockIdx.x];
-  dep_tile(M[w.x].g, M[w.x].d, w.y, L, api != 0);
+__global__ __launch_bounds__(kBlock) void k_nb_tiles(const NudgeMember *M, const int4 *map, int ntiles, int api) {
+  __shared__ DepTilePairLds L;
+  const int k = half_tile(blockIdx.x, ntiles);
+  const int4 w = map[k < 0 ? 0 : k];
+  dep_tile(M[w.x].g, M[w.x].d, k < 0 ? -1 : w.y, L, api != 0);
 }
 
 }  // namespace
@@ -575,7 +597,8 @@ hipError_t launch_nudge_batch(hipStream_t s, const NudgeMember *members, const N
   auto at = [&](int p) { return plan.map + plan.off[p]; };
   const int ntail = tail ? tot[NB_TILES] : 0, nhead = tot[NB_OBS] + tot[NB_ZERO];
   if (ntail + nhead > 0)
-    hipLaunchKernelGGL(k_nb_head, dim3(ntail + nhead), b, 0, s, members, tail ? tail : members, at(NB_TILES), ntail,
+    hipLaunchKernelGGL(k_nb_head, dim3((ntail + 1) / 2 + nhead), b, 0, s, members, tail ? tail : members, at(NB_TILES),
+                       ntail,
                        at(NB_OBS), t);
   if (tot[NB_CAND] > 0) hipLaunchKernelGGL(k_nb_count, dim3(tot[NB_CAND]), b, 0, s, members, at(NB_CAND));
   if (tot[NB_SCAN] > 0) {  // the 7x7 members' count scan and list fill
@@ -585,14 +608,15 @@ hipError_t launch_nudge_batch(hipStream_t s, const NudgeMember *members, const N
   if (tot[NB_FILL] > 0) hipLaunchKernelGGL(k_nb_fill, dim3(tot[NB_FILL]), b, 0, s, members, at(NB_FILL));
   if (tot[NB_FOLD] > 0) hipLaunchKernelGGL(k_nb_fold, dim3(tot[NB_FOLD]), b, 0, s, members, at(NB_FOLD));
   if (!defer && tot[NB_TILES] > 0)
-    hipLaunchKernelGGL(k_nb_tiles, dim3(tot[NB_TILES]), b, 0, s, members, at(NB_TILES), api ? 1 : 0);
+    hipLaunchKernelGGL(k_nb_tiles, dim3((tot[NB_TILES] + 1) / 2), b, 0, s, members, at(NB_TILES), tot[NB_TILES],
+                       api ? 1 : 0);
   return hipGetLastError();
 }
 
 hipError_t launch_nudge_tiles(hipStream_t s, const NudgeMember *members, const NudgePlan &plan, bool api) {
   if (plan.nm <= 0 || plan.tot[NB_TILES] <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_nb_tiles, dim3(plan.tot[NB_TILES]), dim3(kBlock), 0, s, members,
-                     plan.map + plan.off[NB_TILES], api ? 1 : 0);
+  hipLaunchKernelGGL(k_nb_tiles, dim3((plan.tot[NB_TILES] + 1) / 2), dim3(kBlock), 0, s, members,
+                     plan.map + plan.off[NB_TILES], plan.tot[NB_TILES], api ? 1 : 0);
   return hipGetLastError();
 }
 
@@ -603,13 +627,13 @@ hipError_t launch_obs_deposit(hipStream_t s, int branch, int ntempread, const Ge
   const int zb = bil ? 0 : (int)std::min<size_t>((n + kBlock - 1) / kBlock, 4096);
   const int obs_blocks = (o.n + kBlock - 1) / kBlock;
   // only a bilinear deposit has a tile pass to carry (a 7x7 set has no bins)
-  const int tb = tail && tail->mode == RBN_DEPOSIT_BILINEAR && tail->tcnt && tail->bin ? dep_tiles(g) : 0;
+  const int tb = tail && tail->mode == RBN_DEPOSIT_BILINEAR && tail->tcnt && tail->bin ? (dep_tiles(g) + 1) / 2 : 0;
   if (tb + obs_blocks + zb > 0)
     hipLaunchKernelGGL(k_obs_zero, dim3(tb + obs_blocks + zb), dim3(kBlock), 0, s, g, o, branch, ntempread, tb,
                        obs_blocks, d, tail ? *tail : d);
   if (bil) {
     // the observers binned their contributions; the tile pass writes every node
-    if (!defer) hipLaunchKernelGGL(k_dep_tiles, dim3(dep_tiles(g)), dim3(kBlock), 0, s, g, d);
+    if (!defer) hipLaunchKernelGGL(k_dep_tiles, dim3((dep_tiles(g) + 1) / 2), dim3(kBlock), 0, s, g, d);
     return hipGetLastError();
   }
   // a 7x7 deposit completes here whatever defer says (it has no pass to defer)
