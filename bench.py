@@ -201,17 +201,20 @@ def ens_members(rbn, nudged, last_step, seed=7):
 
 
 ENS_TRIAL_STEPS = 8
+# steps of a call before its first graph replay with nudged members (one launch per step;
+# one more with two): eager steps during which the first replay's records are fetched
+ENS_REPLAY_LEAD = 31
 
 
 def ens_autotune_steps(trial, ntempread):
     """Steps from its starting state over which rbn_ensemble_autotune advances nudged members
     whose records come every `ntempread` steps (include/rbn.h): each candidate's warm-up call
     of G steps and timed call of G * max(1, ceil(trial / G)) steps (G = 6 for one launch per
-    step, 12 for two steps per launch), and the comparison of the two forms over 2 L + 2
-    steps, L = lcm(6, ntempread), or 2 L + 3 steps, L = lcm(12, ntempread); the state is
+    step, 12 for two steps per launch), and the comparison of the two forms over 2 L + 32
+    steps, L = lcm(6, ntempread), or 2 L + 33 steps, L = lcm(12, ntempread); the state is
     restored after each."""
     spans = []
-    for g, t0 in ((6, 1), (12, 2)):
+    for g, t0 in ((6, ENS_REPLAY_LEAD), (12, ENS_REPLAY_LEAD + 1)):
         lcm = g * ntempread // int(np.gcd(g, ntempread))
         spans += [g + g * max(1, -(-trial // g)), 2 * lcm + t0 + 1]
     return max(spans)
@@ -231,9 +234,10 @@ def run_ensemble(rbn, steps, warmup, tuning=None, nudged=True):
     order)) against the same members, built afresh, stepped one after another with their
     own launches (rbn_step)."""
     # a whole number of the captured cycles (L = lcm(12, ntempread) steps covers both launch
-    # modes), so that the timed call replays graphs kept from the warm-up call
+    # modes), long enough to capture them (the lead and a cycle), so that the timed call
+    # replays graphs kept from the warm-up call
     cyc = 12 * ENS_NTEMPREAD // int(np.gcd(12, ENS_NTEMPREAD))
-    ens_warm = (warmup // cyc + 1) * cyc
+    ens_warm = -(-max(warmup, ENS_REPLAY_LEAD + 2 + cyc) // cyc) * cyc
     last = ens_record_horizon(ENS_TRIAL_STEPS, warmup, ens_warm, steps, ENS_NTEMPREAD)
     members, nodes, nobs = ens_members(rbn, nudged, last)
     for s in members:

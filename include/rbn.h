@@ -278,7 +278,9 @@ int rbn_ensemble_create(rbn_ctx *const *members, int n, rbn_ensemble **out);
 void rbn_ensemble_destroy(rbn_ensemble *e);
 const char *rbn_ensemble_last_error(const rbn_ensemble *e);
 /* Advance every member by nsteps (graph replays for whole cycles of six launches; the first
- * and last launches of a call, and the last launch's particle updates, run eagerly).  After
+ * and last launches of a call, and the last launch's particle updates, run eagerly; with
+ * nudged members the first 31 steps with one launch per step, 32 with two, run eagerly while
+ * the first replay's records are fetched and uploaded).  After
  * an error (a provider failure, a particle leaving its lattice, a device error) the
  * members' states are unspecified: initialise them again before further steps. */
 int rbn_ensemble_step(rbn_ensemble *e, int nsteps);
@@ -299,9 +301,9 @@ int rbn_ensemble_get_tuning(const rbn_ensemble *e, int *tile_w, int *tile_h, int
  * nudged members every candidate steps from the starting state (a warm-up call of G steps
  * and a timed call of G * max(1, ceil(trial_steps / G)) steps; G = 6 for one launch per
  * step, 12 for two), and the fastest shape is timed once more in both forms, graph replays
- * and eager steps (order bit 4), keeping the faster: over steps 1 .. 2L of a call of 2 L + 2
- * steps with one launch per step (L = lcm(6, record intervals)), steps 2 .. 2L + 1 of a call
- * of 2 L + 3 steps with two (L = lcm(12, record intervals)), L <= 120.  The members' records
+ * and eager steps (order bit 4), keeping the faster: over steps 31 .. 2L + 30 of a call of
+ * 2 L + 32 steps with one launch per step (L = lcm(6, record intervals)), steps 32 .. 2L + 31
+ * of a call of 2 L + 33 steps with two (L = lcm(12, record intervals)), L <= 120.  The members' records
  * must cover the longest of these spans from the starting state. */
 int rbn_ensemble_autotune(rbn_ensemble *e, int trial_steps, double *best_ms_per_step);
 

@@ -221,6 +221,14 @@ size_t loading_members(const rbn_ensemble *e, int t, std::vector<int> &ld);
 bool pairs(const rbn_ensemble *e);
 bool rides(const rbn_ensemble *e);
 
+// Nudged members' graph replays start kReplayLead steps after the first launch of a call:
+// those steps run eagerly while the host fetches the first replay's records (a cycle's
+// worth, about 120 MB for 64 members of 1e3..1e4 observers every 5 steps: 5-6 ms of host
+// copying) and uploads them, which would otherwise leave the GPU idle before that replay.
+// Even, and a multiple of the velocity ring.
+constexpr int kReplayLead = 30;
+static_assert(kReplayLead % 2 == 0 && kReplayLead % rbn::kCycle == 0, "the lead keeps the launch parities");
+
 // Observer/deposit descriptors of the nudged members as of their current step, the block
 // map of the batched passes, and the record uploads of the captured cycle.
 int build_nudged(rbn_ensemble *e) {
@@ -280,7 +288,7 @@ int build_nudged(rbn_ensemble *e) {
     if (L > kMaxNudgedCycle) break;
   }
   e->ncycle = L <= kMaxNudgedCycle ? (int)L : 0;
-  e->cyc_t0 = pairs(e) ? 2 : 1;
+  e->cyc_t0 = (pairs(e) ? 2 : 1) + kReplayLead;
   e->plan_t0 = e->cyc_t0 + (rides(e) ? 1 : 0);
   e->cyc.clear();
   e->cyc_bytes = 0;
@@ -705,13 +713,14 @@ int capture_nudged(rbn_ensemble *e) {
       return RBN_OK;
     };
     if (pairs(e)) {
-      // launches 1 .. L / 2 of a call (steps 2 .. L + 1), two steps each, carrying two updates;
-      // with rides(e) each also carries the next launch's first observer update (the first
-      // launch's was carried by the launch before the replay)
+      // launches t0 / 2 + 1 .. t0 / 2 + L / 2 of a call (steps t0 .. t0 + L - 1, t0 = cyc_t0),
+      // two steps each, carrying two updates; with rides(e) each also carries the next
+      // launch's first observer update (the first launch's was carried by the launch before
+      // the replay)
       PairState ps;
-      ps.t = 2;
-      ps.L = 1;
-      ps.P = 1;
+      ps.t = e->cyc_t0;
+      ps.L = e->cyc_t0 / 2;
+      ps.P = e->cyc_t0 / 2;
       ps.pending = 2;
       const bool ride = rides(e);
       for (int l = 0; l < e->ncycle / 2 && !rc; l++) {
@@ -726,7 +735,7 @@ int capture_nudged(rbn_ensemble *e) {
         if (!rc) rc = enqueue_pair(e, ps, 1 << 30, false, tile_bins(e), head);
       }
     } else {
-      for (int t = 1; t <= e->ncycle && !rc; t++)
+      for (int t = e->cyc_t0; t < e->cyc_t0 + e->ncycle && !rc; t++)
         if (!(rc = deposit(t, false, false))) rc = enqueue_fused(e, t, false);
     }
     hipGraph_t g = nullptr;
@@ -756,9 +765,10 @@ int sync_members(rbn_ensemble *e) {
 // particle updates of step t - 1, and the first three run on pstream beside the previous
 // step's lattice launch (enqueue_deposit).  Two-step mode: everything on stream, both
 // steps' deposits before their launch (enqueue_nudged_pair).  Either way the first launch
-// runs eagerly, whole cycles of L steps from the graphs (their records fetched on the host
-// before each replay, into the set the previous replay but one has read), the rest eagerly
-// (the last writing the moments), then the last launch's particle updates on their own.
+// and the kReplayLead steps after it run eagerly, whole cycles of L steps from the graphs
+// (their records fetched on the host before each replay, into the set the previous replay
+// but one has read; the first replay's while the lead runs), the rest eagerly (the last
+// writing the moments), then the last launch's particle updates on their own.
 int run_nudged(rbn_ensemble *e, int nsteps) {
   int rc;
   const bool two = pairs(e);
@@ -774,10 +784,19 @@ int run_nudged(rbn_ensemble *e, int nsteps) {
     if ((rc = eager_deposit(0, 0)) || (rc = enqueue_nudged_lattice(e, 0, nsteps == 1))) return rc;
     ps.t = 1;
   }
-  const int L = e->ncycle, t0 = ps.t;
+  const int L = e->ncycle, t0 = e->cyc_t0;
   const int cycles = graphs(e) && L > 0 && nsteps > t0 ? (nsteps - t0 - 1) / L : 0;
   if (cycles > 0) {
     if (!e->nexec[0] && (rc = capture_nudged(e))) return rc;
+    // the lead: the steps before the cycle's first, eagerly (the first replay's records are
+    // fetched on the host meanwhile)
+    if (two) {
+      while (ps.t < t0)
+        if ((rc = enqueue_nudged_pair(e, ps, nsteps, false))) return rc;
+    } else {
+      for (; ps.t < t0; ps.t++)
+        if ((rc = eager_deposit(ps.t, 0)) || (rc = enqueue_nudged_lattice(e, ps.t, false))) return rc;
+    }
     if (e->seg_a) {
       EHIP(e, hipEventRecord(e->seg_a, e->stream));
       e->seg_steps = cycles * L;
@@ -816,13 +835,11 @@ int run_nudged(rbn_ensemble *e, int nsteps) {
   // the autotuner's eager form: steps t0 .. t0 + 2L - 1, when there were no replays
   const int seg_len = 2 * L;
   const bool seg = e->seg_a && L > 0 && cycles == 0 && nsteps > t0 + seg_len;
-  if (seg) {
-    EHIP(e, hipEventRecord(e->seg_a, e->stream));
-    e->seg_steps = seg_len;
-  }
+  if (seg) e->seg_steps = seg_len;
   const int tail = ps.t;
   if (two) {
     while (ps.t < nsteps) {
+      if (seg && ps.t == t0) EHIP(e, hipEventRecord(e->seg_a, e->stream));
       if ((rc = enqueue_nudged_pair(e, ps, nsteps, ps.t + 2 >= nsteps))) return rc;
       if (seg && ps.t == t0 + seg_len) EHIP(e, hipEventRecord(e->seg_b, e->stream));
     }
@@ -836,6 +853,7 @@ int run_nudged(rbn_ensemble *e, int nsteps) {
     }
   } else {
     for (int t = tail; t < nsteps; t++) {
+      if (seg && t == t0) EHIP(e, hipEventRecord(e->seg_a, e->stream));
       if ((rc = eager_deposit(t, t == tail ? tail : 0))) return rc;
       if ((rc = enqueue_nudged_lattice(e, t, t == nsteps - 1))) return rc;
       if (seg && t + 1 == t0 + seg_len) EHIP(e, hipEventRecord(e->seg_b, e->stream));

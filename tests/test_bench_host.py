@@ -51,13 +51,13 @@ def _autotune_records(trial, ntempread):
     """The records rbn_ensemble_autotune reads from the starting state, restated from its
     loop: every candidate runs a warm-up call of G steps and a timed call of reps * G steps
     (reps = max(1, ceil(trial / G)); G = 6 one launch per step, 12 two steps per launch) and
-    is restored; then per form a call of 2 L + 2 steps (L = lcm(6, ntempread)) or 2 L + 3
+    is restored; then per form a call of 2 L + 32 steps (L = lcm(6, ntempread)) or 2 L + 33
     (L = lcm(12, ntempread)), whichever launch mode won.  A call reads at step s
     (s % ntempread == 1) the record of step s + ntempread - 1."""
     import math
 
     spans = []
-    for g, t0 in ((6, 1), (12, 2)):
+    for g, t0 in ((6, 31), (12, 32)):
         reps = max(1, (trial + g - 1) // g)
         L = g * ntempread // math.gcd(g, ntempread)
         spans += [g + reps * g, 2 * L + t0 + 1]

@@ -175,10 +175,11 @@ def test_c3_nudging_grid_fp64_bit_exact(nobs, deposit):
 
 def test_c4_benchmark_ensemble_equals_individual_steps():
     # The 64 members bench.py times (the same seeds), advanced by one ensemble: calls of 1
-    # and 62 steps with graph replays of the 30-step cycle (one launch per step, the passes
-    # in turn), then calls of 9 and 31 eager steps with each step's deposit built beside the
-    # previous lattice launch (order bit 4); against rbn_step of every member.
-    calls_graph, calls_eager = (1, 62), (9, 31)
+    # and 92 steps with two graph replays of the 30-step cycle (one launch per step, the
+    # passes in turn, from call step 31), then calls of 9 and 31 eager steps with each step's
+    # deposit built beside the previous lattice launch (order bit 4); against rbn_step of
+    # every member.
+    calls_graph, calls_eager = (1, 92), (9, 31)
     total = sum(calls_graph) + sum(calls_eager)
     last = total + 2 * bench.ENS_NTEMPREAD
     ens, nodes, nobs = bench.ens_members(rbn, True, last)
@@ -233,10 +234,10 @@ def test_c4_fp64_ensemble_members_equal_the_oracle(order):
     orc = _fp64_members(Oracle)
     e = rbn.Ensemble(ens)
     e.set_tuning(256, 1, 4, order)
-    for n in (1, 62, 5):  # cycle of lcm(6, 3, 5) = 30: two replays when graphs are on
+    for n in (1, 92, 5):  # cycle of lcm(6, 3, 5) = 30 from call step 31: two replays when graphs are on
         e.step(n)
     for o in orc:
-        o.step(68)
+        o.step(98)
     for k, (a, b) in enumerate(zip(ens, orc)):
         _compare_populations(a, b, f"member {k} ")
         _compare_fields(a, b, f"member {k} ")

@@ -65,9 +65,9 @@ def _config_members(table):
         s = rbn.Solver(p)
         s.init_fields(T=conduction_T(p, 1e-3, seed=100 + m))
         if nudged:
-            # records up to step 130: the autotuner's longest span (2 * 60 + 3 steps with two
-            # steps per launch) reads the record of step 125
-            ids, x0, y0, T, recs = synthetic_observations(nx, ny, n, 130, 5, seed=200 + m)
+            # records up to step 160: the autotuner's longest span (2 * 60 + 33 steps with two
+            # steps per launch) reads the record of step 155
+            ids, x0, y0, T, recs = synthetic_observations(nx, ny, n, 160, 5, seed=200 + m)
             s.init_observers(ids, x0, y0, T)
             if table:
                 s.set_observation_table(recs)
@@ -85,12 +85,12 @@ def test_ensemble_config_sizes_equal_individual_steps():
     ref = _config_members(table=False)
     e = rbn.Ensemble(ens)
     e.autotune(4)
-    # 63: replays of the captured cycle (30 steps with one launch per step, 60 with two) and
-    # eager steps
-    for n in (1, 31, 63):
+    # 93: eager steps, a replay of the captured cycle (30 steps from call step 31 with one
+    # launch per step, 60 from call step 32 with two) and eager steps
+    for n in (1, 31, 93):
         e.step(n)
     for s in ref:
-        s.step(95)
+        s.step(125)
     for k, (a, b) in enumerate(zip(ens, ref)):
         fa, ga = a.populations()
         fb, gb = b.populations()

@@ -138,15 +138,15 @@ def test_ensemble_with_nudged_members_equals_individual_steps(prec, tuning, inte
     # members run as batched launches, records are uploaded between steps or, inside the
     # captured cycles, scattered by the graph from a staging set uploaded before each replay.
     # One launch per step: cycles of 60 steps for intervals 4, 5, 5, 3, of 6 for 3, 3, 2, 6,
-    # from call step 1.  Two steps per launch (order bit 3), both steps' deposits before it:
-    # cycles of 60 and 12 from call step 2.  Calls of 122 steps (cycles and eager steps), 120
-    # (whole cycles: the graphs stay for the next call) and 63 (a replay of the kept graph).
+    # from call step 31.  Two steps per launch (order bit 3), both steps' deposits before it:
+    # cycles of 60 and 12 from call step 32.  Calls of 122 steps (cycles and eager steps), 120
+    # (whole cycles: the graphs stay for the next call) and 93 (a replay of the kept graph).
     # Separate-launch orders (bit 2) fall back to one launch per step.
     ens_members = _mixed_members(prec, table=True, intervals=intervals)
     ref_members = _mixed_members(prec, table=False, intervals=intervals)
     e = rbn.Ensemble(ens_members)
     e.set_tuning(*tuning)
-    calls = (9, 1, 2, 6, 13, 20, 122, 120, 63)
+    calls = (9, 1, 2, 6, 13, 20, 122, 120, 93)
     for n in calls:
         e.step(n)
     for s in ref_members:
@@ -160,12 +160,13 @@ def test_ensemble_with_nudged_members_equals_individual_steps(prec, tuning, inte
 def test_ensemble_of_nudged_members_only_equals_individual_steps(prec, order):
     # An ensemble of nudged members only runs the two-step kernel compiled for nudged tiles
     # alone (order bit 3); odd calls end on a one-step launch and leave the last deposit in
-    # the second set.
+    # the second set.  The call of 46 steps replays the captured cycle (6 or 12 steps from
+    # call step 31 or 32) where graphs are on.
     ens_members = _mixed_members(prec, table=True, intervals=(3, 3, 2, 6))[2:]
     ref_members = _mixed_members(prec, table=False, intervals=(3, 3, 2, 6))[2:]
     e = rbn.Ensemble(ens_members)
     e.set_tuning(256, 1, 4, order)
-    calls = (1, 2, 15, 12, 40, 27)
+    calls = (1, 2, 15, 12, 46, 27)
     for n in calls:
         e.step(n)
     for s in ref_members:

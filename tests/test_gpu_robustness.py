@@ -113,7 +113,7 @@ def _state(s):
 
 def test_failed_autotune_restores_every_member():
     # Records reach step 40 only: the autotuner's comparison of the two forms needs at least
-    # 2 lcm(6, 5, 3) + 2 = 62 steps and fails on its first missing record.
+    # 2 lcm(6, 5, 3) + 32 = 92 steps and fails on its first missing record.
     members = _nudged_members(40)
     for s in members:
         s.step(2)
