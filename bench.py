@@ -7,15 +7,17 @@ inertial particles (Stokes number log-uniform in [0.01, 1], tau_p = St * t_ff wi
 t_ff = (ny-1)/u_ff, zero initial slip), uniform random positions; trajectory records every
 20 steps, written to device memory (the rate with every record also copied to the host over
 PCIe is reported beside it).  One "step" = the fused f/g LBM update + the particle update
-(+ the record on record steps); by default one lattice launch advances two steps (the first
-step's populations kept in LDS) and carries the previous two steps' particle updates as
-extra workgroups.  Inputs are resident on the GPU before timing starts.
+(+ the record on record steps); by default one lattice launch advances three steps (a row
+sweep passing the first two steps' populations through LDS; two steps where a record step
+ends a launch) and carries the previous launch's particle updates as extra workgroups.
+Inputs are resident on the GPU before timing starts.
 
 Prints one JSON line: MLUPS of the FP32 (shifted-population) path, the HBM roofline of the
-dominant kernel (the two-step lattice launch; HIP events around launches in the timed
+dominant kernel (the three-step lattice launch; HIP events around launches in the timed
 region; the HBM traffic per launch comes from the committed rocprofv3 counter summary under
-profiles/), the same workload at one step per launch and with the particle update as a
-launch of its own (the one-step lattice kernel's roofline alone),
+profiles/), the same workload at two steps per launch (64 x 8 tiles), at one step per
+launch and with the particle update as a launch of its own (the one-step lattice kernel's
+roofline alone),
 the FP64 path's MLUPS on the same workload, the batched ensemble (64 uneven members with
 1e3-1e4 nudged tracers each; also truth-run members advecting their tracers) against the
 same members stepped one by one, the nudging configuration (1024 x 512: the L2
@@ -116,7 +118,7 @@ def run_gpu(rbn, precision, steps, warmup, timing=True, records_to_host=False, *
     s.synchronize()
     if timing:
         s.kernel_timing(True, every=TIMING_EVERY)
-        for k in ("lbm", "lbm_fused", "lbm2", "particles"):
+        for k in ("lbm", "lbm_fused", "lbm2", "lbm3", "particles"):
             s.kernel_stats(k, reset=True)
     t0 = time.perf_counter()
     s.step(steps)
@@ -128,11 +130,13 @@ def run_gpu(rbn, precision, steps, warmup, timing=True, records_to_host=False, *
         n_lbm, ms_lbm = s.kernel_stats("lbm")
         n_f, ms_f = s.kernel_stats("lbm_fused")
         n_2, ms_2 = s.kernel_stats("lbm2")
+        n_3, ms_3 = s.kernel_stats("lbm3")
         n_p, ms_p = s.kernel_stats("particles")
         s.kernel_timing(False)
         out.update(lbm_timed=n_lbm, lbm_avg_ms=ms_lbm / max(n_lbm, 1),
                    fused_timed=n_f, fused_avg_ms=ms_f / max(n_f, 1),
                    lbm2_timed=n_2, lbm2_avg_ms=ms_2 / max(n_2, 1),
+                   lbm3_timed=n_3, lbm3_avg_ms=ms_3 / max(n_3, 1),
                    particles_timed=n_p, particles_avg_ms=ms_p / max(n_p, 1),
                    record_fraction=record_steps(warmup, steps) / steps)
     fl = s.fields()
@@ -532,14 +536,16 @@ def launch_bytes(precision, record_fraction, particles, steps=1):
     return steps * (b + (particle_bytes_per_step(precision) if particles else 0))
 
 
-TRAFFIC_PROFILE = ROOT / "profiles" / "k_lbm2_32_4096x2048_pmc.txt"
+TRAFFIC_PROFILES = {2: ROOT / "profiles" / "k_lbm2_32_4096x2048_pmc.txt",
+                    3: ROOT / "profiles" / "k_lbm3w_32_4096x2048_pmc.txt"}
 
 
-def measured_traffic():
-    """HBM bytes per lattice launch carrying a particle update, from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary (counters scaled by a copy kernel of known size), or None."""
+def measured_traffic(steps_per_launch):
+    """HBM bytes per lattice launch of that many steps carrying particle updates, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (counters scaled by a copy kernel of
+    known size), or None."""
     try:
-        for line in TRAFFIC_PROFILE.read_text().splitlines():
+        for line in TRAFFIC_PROFILES[steps_per_launch].read_text().splitlines():
             if line.startswith("traffic_bytes_per_launch:"):
                 return int(line.split(":")[1])
     except OSError:
@@ -631,15 +637,15 @@ MIN_TIMED_STEPS, MIN_WARMUP = 1000, 50
 MIN_ENS_STEPS = 300
 
 
-def launch_min_bytes(precision, record_fraction, particles=True):
-    """The fewest HBM bytes a two-step lattice launch can move: every population and the
-    force read once (19 reals per node) and written once (19), the velocity of both steps
-    written (4), rho and T written on the launches whose second step writes the moments
-    (2 reals, on 2 * record_fraction of the launches), plus the two carried particle
-    updates' algorithmic bytes."""
+def launch_min_bytes(precision, record_fraction, particles=True, steps=2):
+    """The fewest HBM bytes a lattice launch of `steps` steps can move: every population and
+    the force read once (19 reals per node) and written once (19), the velocity of every step
+    written (2 reals each), rho and T written on the launches whose last step writes the
+    moments (2 reals, on steps * record_fraction of the launches), plus the carried particle
+    updates' algorithmic bytes (one per step)."""
     s = 8 if precision == 0 else 4
-    b = NX * NY * s * (42 + 2 * 2 * record_fraction)
-    return b + (2 * particle_bytes_per_step(precision) if particles else 0)
+    b = NX * NY * s * (38 + 2 * steps + 2 * steps * record_fraction)
+    return b + (steps * particle_bytes_per_step(precision) if particles else 0)
 
 
 def run_leg(name, fn, *args, **kw):
@@ -714,6 +720,7 @@ def main() -> int:
     print(f"bench: fp32 {NX}x{NY} steps={steps} warmup={warmup}", file=sys.stderr, flush=True)
     r32 = run_leg("fp32", run_gpu, rbn, rbn.RBN_FP32, steps, warmup)
     r64 = None if args.no_fp64 else run_leg("fp64", run_gpu, rbn, rbn.RBN_FP64, half, half_w)
+    two = run_leg("fp32 two steps per launch", run_gpu, rbn, rbn.RBN_FP32, half, half_w, lbm_steps=2)
     one = run_leg("fp32 one step per launch", run_gpu, rbn, rbn.RBN_FP32, half, half_w, lbm_steps=1)
     sep = run_leg("fp32 separate particle launches", run_gpu, rbn, rbn.RBN_FP32, half, half_w, particle_fuse=0)
     host = run_leg("fp32 records to host", run_gpu, rbn, rbn.RBN_FP32, steps, warmup, timing=False,
@@ -762,11 +769,15 @@ def main() -> int:
         "ms_per_step_fp64": round(r64["ms_per_step"], 5) if ok(r64) else None,
     }
     if ok(r32):
-        bytes_launch = launch_bytes(1, r32["record_fraction"], True, steps=2)
-        min_bytes = launch_min_bytes(1, r32["record_fraction"])
-        avg = r32["lbm2_avg_ms"]
+        # the dominant launch: three steps (where record steps allow) or two; record steps end
+        # launches, so three-step launches never write the moments at 20-step records
+        spl = 3 if r32["lbm3_timed"] > 0 else 2
+        rf = 0.0 if spl == 3 else r32["record_fraction"]
+        bytes_launch = launch_bytes(1, rf, True, steps=spl)
+        min_bytes = launch_min_bytes(1, rf, steps=spl)
+        avg = r32[f"lbm{spl}_avg_ms"]
         achieved = bytes_launch / (avg * 1e-3) / 1e9
-        traffic = measured_traffic()
+        traffic = measured_traffic(spl)
         line["roofline"] = {
             "bound": "hbm",
             "achieved": round(achieved, 1),
@@ -777,16 +788,19 @@ def main() -> int:
             "traffic_frac": traffic and round(traffic / (avg * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
             "min_traffic_bytes": round(min_bytes),
             "min_traffic_frac": round(min_bytes / (avg * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
-            "kernel": "k_lbm2<float, 8> (two fused f/g collide+stream steps, the first step's populations in "
-                      "LDS, + the two previous steps' particle updates)",
-            "steps_per_launch": 2,
+            "kernel": ("k_lbm3w<float, 256> (three fused f/g collide+stream steps by row sweeps, the first two "
+                       "steps' populations passed through LDS, + the previous launch's particle updates)") if spl == 3
+            else ("k_lbm2<float, 8> (two fused f/g collide+stream steps, the first step's populations in "
+                  "LDS, + the two previous steps' particle updates)"),
+            "steps_per_launch": spl,
             "bytes_per_launch": round(bytes_launch),
             "avg_launch_ms": round(avg, 5),
-            "launches_timed": r32["lbm2_timed"],
+            "launches_timed": r32[f"lbm{spl}_timed"],
             "timed_every": TIMING_EVERY,
-            "note": "achieved counts both steps' algorithmic bytes (160 B/LUP each), so frac can exceed 1 when "
-                    "the first step stays on chip; min_traffic_frac divides the fewest bytes such a launch can "
-                    "move (42 reals per node + particles) by the same duration and cannot exceed 1",
+            "note": "achieved counts every step's algorithmic bytes (160 B/LUP each), so frac can exceed 1 when "
+                    "the earlier steps of a launch stay on chip; min_traffic_frac divides the fewest bytes such a "
+                    "launch can move (38 + 2 x steps reals per node + particles) by the same duration and "
+                    "cannot exceed 1",
         }
     else:
         line["roofline"] = None
@@ -799,6 +813,14 @@ def main() -> int:
             launch_min_bytes(0, r64["record_fraction"]) / (r64["lbm2_avg_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)})
     line["records_to_host"] = lattice_summary(host, None, 0, {
         "note": "PCIe-inclusive: every record also copied to pinned host memory and handed to a callback"})
+    if ok(two):
+        line["two_steps_per_launch"] = lattice_summary(
+            two, "lbm2_avg_ms", launch_bytes(1, two["record_fraction"], True, steps=2),
+            {"lattice_kernel": "k_lbm2<float, 8> (64 x 8 tiles, two steps + the previous two steps' particle updates)",
+             "min_traffic_frac": two["lbm2_avg_ms"] > 0 and round(
+                 launch_min_bytes(1, two["record_fraction"]) / (two["lbm2_avg_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4)})
+    else:
+        line["two_steps_per_launch"] = two
     line["one_step_per_launch"] = lattice_summary(
         one, "fused_avg_ms", launch_bytes(1, one.get("record_fraction", 0.0), True),
         {"lattice_kernel": "k_lbm_fused32 (one step + the previous step's particle update)"})

@@ -344,8 +344,14 @@ int rbn_adj_backward(rbn_ctx *ctx, double *dJ_dT0, double *dJ_dalpha);
  *                followed by another step of the same rbn_step call; truth runs with
  *                particles only under particle_fuse; nudged runs without dep_overlap, the
  *                observer updates and deposits of both steps run before the launch (the
- *                deposits depend on the records only) (default 2); 1: one step per launch
- *   "lbm2_seg"   (truth runs) 8..64: two steps per launch by row sweeps, every workgroup
+ *                deposits depend on the records only) (default 2); 1: one step per launch;
+ *                3 (truth runs): three steps per launch by row sweeps wherever the first two
+ *                write no moments (the first two steps' sideways-moving populations passed
+ *                through LDS, strips of 256 threads overlapping by four columns), else two
+ *                or one as with 2 (default 3 for FP32 truth runs of at least 2^21 nodes)
+ *   "lbm3_seg"   rows per segment of a three-step sweep, 4..4096; 0 (default): the fewest
+ *                rows that put every workgroup of the launch on the device at once, at least 16
+ *   "lbm2_seg"  (truth runs) 8..64: two steps per launch by row sweeps, every workgroup
  *                sweeping a strip of 256 columns over a segment of lbm2_seg rows, the first step's
  *                sideways-moving populations passed between its threads through LDS; 0: by
  *                tiles of 64 columns and lbm2_tile_h rows (default 0)
@@ -385,7 +391,7 @@ int rbn_set_kernel_timing(rbn_ctx *ctx, int every);
 /* Launch count and summed duration (ms) of the timed launches of kernel `name`
  * ("lbm", "particles", "deposit": a nudged step's observer update and deposit, "adjoint",
  * "lbm_fused": a one-step lattice launch carrying earlier steps' particle updates, "lbm2": a
- * two-step lattice launch); resets the
+ * two-step lattice launch, "lbm3": a three-step lattice launch); resets the
  * counters when reset != 0. */
 int rbn_kernel_stats(rbn_ctx *ctx, const char *name, int reset, long long *launches,
                      double *total_ms);

@@ -161,7 +161,7 @@ void fill_consts(const rbn_params &P, rbn::StepConst<R> &c) {
   }
 }
 
-const char *const kKernelNames[KT_COUNT] = {"lbm", "particles", "deposit", "adjoint", "lbm_fused", "lbm2"};
+const char *const kKernelNames[KT_COUNT] = {"lbm", "particles", "deposit", "adjoint", "lbm_fused", "lbm2", "lbm3"};
 
 int kernel_index(const char *name) {
   for (int k = 0; k < KT_COUNT; k++)
@@ -653,13 +653,21 @@ int rbn_create(const rbn_params *p, int device, rbn_ctx **out) {
     c->tune.nt = 0;
     c->tune.xcd = 1;
   }
+  // FP32 truth runs of 2^21 nodes and more advance three steps per launch by row sweeps
+  // (measured with particles: 2048 x 1024 0.0465 against 0.0489 ms per step, 4096 x 2048
+  // 0.156 against 0.175, 8192 x 4096 0.663 against 0.700); smaller lattices give the sweep
+  // too few workgroups (1024 x 512: 0.0212 against 0.0138), and FP64 sweeps hold one wave
+  // per SIMD (4096 x 2048: 0.363 against 0.299).
+  c->tune.steps3 = c->prec == RBN_FP32 && p->variant == RBN_VARIANT_TRAJECTORIES &&
+                   (long long)p->nx * p->ny >= (1LL << 21);
   const size_t fb = g.plane * c->rs;
   struct A {
     void **p;
     size_t b;
-  } allocs[] = {{&c->fA, 9 * fb}, {&c->fB, 9 * fb}, {&c->gA, 9 * fb}, {&c->gB, 9 * fb},
-                {&c->F, fb},      {&c->F2, fb},        {&c->uv[0], 2 * fb}, {&c->uv[1], 2 * fb}, {&c->uv[2], 2 * fb},
-                {&c->uv[3], 2 * fb}, {&c->uv[4], 2 * fb}, {&c->uv[5], 2 * fb}, {&c->rho, fb}, {&c->T, fb}};
+  };
+  std::vector<A> allocs = {{&c->fA, 9 * fb}, {&c->fB, 9 * fb}, {&c->gA, 9 * fb}, {&c->gB, 9 * fb},
+                           {&c->F, fb},      {&c->F2, fb},      {&c->rho, fb},     {&c->T, fb}};
+  for (int k = 0; k < rbn_ctx::kVel; k++) allocs.push_back({&c->uv[k], 2 * fb});
   for (auto &a : allocs) {
     if ((rc = dalloc(c, a.p, a.b))) {
       rbn::thread_fail(rc, c->err.c_str());
@@ -1139,8 +1147,8 @@ namespace {
 
 // ---------------------------------------------------------------- pending particle updates
 // With particle_fuse the particle update of a step waits for the next lattice launch and
-// runs inside it; after a two-step launch the updates of both steps wait.  None is pending
-// when rbn_step returns.
+// runs inside it; after a two- or three-step launch the updates of all its steps wait.  None
+// is pending when rbn_step returns.
 
 void add_pending(rbn_ctx *c, int istep, int cur, int prv, bool record, bool full, rbn_ctx::RecSlot *slot) {
   rbn_ctx::PendingParticles &j = c->pend[c->npend++];
@@ -1250,6 +1258,9 @@ int step_loop(rbn_ctx *c, int nsteps) {
   // lbm_steps 2: two truth-run steps per lattice launch, the first of which writes no
   // moments; with particles only when their updates ride in lattice launches.
   const bool pairs = c->tune.steps2 && (!parts || fuse);
+  // lbm_steps 3: three truth-run steps per launch (row sweeps) where the first two write no
+  // moments; a launch of two or one step elsewhere.
+  const bool triples = pairs && c->tune.steps3 && !nudged;
   hipStream_t ks = overlap ? c->pstream : c->stream;
   // dep_overlap: the observer update and deposit of step s+1 run on pstream (unused by nudged
   // runs, which have no particles) beside the lattice step s.  Off by default: measured at
@@ -1282,7 +1293,18 @@ int step_loop(rbn_ctx *c, int nsteps) {
       diag2 = nd > 0 && (istep2 % nd) == 0;
       moments2 = record2 || diag2 || n == nsteps - 1;
     }
-    const bool writes_moments = pair ? moments2 : moments;
+    // a three-step launch: a pair whose second step writes no moments, and a third step
+    const bool triple = triples && pair && !moments2;
+    int istep3 = 0;
+    bool record3 = false, diag3 = false, moments3 = false;
+    if (triple) {
+      istep3 = ++c->istep;
+      n++;
+      record3 = parts && (istep3 % P.noutconfig) == 0;
+      diag3 = nd > 0 && (istep3 % nd) == 0;
+      moments3 = record3 || diag3 || n == nsteps - 1;
+    }
+    const bool writes_moments = triple ? moments3 : (pair ? moments2 : moments);
     // bilinear deposit: the second step's deposit summed inside the two-step launch from its
     // bins, except in the call's last launch (its deposit fills the arrays the getters read)
     const bool in_tile = nudged && pair && c->dep.mode == RBN_DEPOSIT_BILINEAR && c->tune.dep_in_tile &&
@@ -1291,11 +1313,11 @@ int step_loop(rbn_ctx *c, int nsteps) {
     // observer update (into dep2, whose bins the tile pass of this pair's first step has
     // consumed; the launch reads dep2's packed deposit and dep's bins only)
     bool carry_head = false;
-    int istep3 = 0;
+    int hstep = 0;
     if (nudged && pair && c->dep.mode == RBN_DEPOSIT_BILINEAR) {
-      istep3 = istep2 + 1;
-      const bool moments3 = (nd > 0 && (istep3 % nd) == 0) || n + 1 == nsteps - 1;
-      carry_head = !moments3 && n + 2 < nsteps;
+      hstep = istep2 + 1;
+      const bool hmoments = (nd > 0 && (hstep % nd) == 0) || n + 1 == nsteps - 1;
+      carry_head = !hmoments && n + 2 < nsteps;
     }
     const rbn::ObsPtrs o3 = nudged ? obs_ptrs(c) : rbn::ObsPtrs{};
     if (nudged && pair) {
@@ -1303,8 +1325,8 @@ int step_loop(rbn_ctx *c, int nsteps) {
       const bool bil = c->dep.mode == RBN_DEPOSIT_BILINEAR;
       if (head_done != istep && (rc = nudged_deposit(c, istep, c->stream, c->dep2, nullptr, bil))) return rc;
       if ((rc = nudged_deposit(c, istep2, c->stream, c->dep, bil ? &c->dep2 : nullptr, in_tile))) return rc;
-      if (carry_head && istep3 % P.ntempread == 1 &&
-          (rc = load_observation(c, istep3 + P.ntempread - 1, c->stream)))
+      if (carry_head && hstep % P.ntempread == 1 &&
+          (rc = load_observation(c, hstep + P.ntempread - 1, c->stream)))
         return rc;
     } else if (nudged && !dovl) {
       if (head_done == istep) return fail(c, RBN_ESTATE, "internal: a carried observer update before a single step");
@@ -1318,7 +1340,8 @@ int step_loop(rbn_ctx *c, int nsteps) {
     }
     const rbn::DepPtrs &dset = (dovl && (n & 1)) || (nudged && pair) ? c->dep2 : c->dep;
     // A launch writes the velocity slots after cur: K1 of step s overwrites the slot of step
-    // s-6, a two-step launch also that of s-5; pending updates read slots s-3 .. s-1.
+    // s-8, a two-step launch also that of s-7, a three-step one those of s-7 and s-6; pending
+    // updates read slots s-4 .. s-1.
     bool carry = false;
     if (c->npend > 0) {
       if (pending_reads_T(c) && writes_moments) {
@@ -1332,7 +1355,8 @@ int step_loop(rbn_ctx *c, int nsteps) {
       if (moments && c->part_record_pending)
         HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_part[(istep + 1) % 2], 0));
     }
-    const int s1 = (c->cur + 1) % rbn_ctx::kVel, s2 = (c->cur + 2) % rbn_ctx::kVel;
+    const int s1 = (c->cur + 1) % rbn_ctx::kVel, s2 = (c->cur + 2) % rbn_ctx::kVel,
+              s3 = (c->cur + 3) % rbn_ctx::kVel;
     rbn::LbmPtrs lp{};
     lp.fA = c->fA;
     lp.gA = c->gA;
@@ -1347,9 +1371,16 @@ int step_loop(rbn_ctx *c, int nsteps) {
       lp.nud = c->nud;
     }
     lp.bad = c->d_err + 1;
-    const int kclass = pair ? KT_LBM2 : (carry ? KT_LBM_FUSED : KT_LBM);
+    const int kclass = triple ? KT_LBM3 : (pair ? KT_LBM2 : (carry ? KT_LBM_FUSED : KT_LBM));
     if ((rc = timed(c, kclass, c->stream, [&] {
            rbn::LbmTuning t = c->tune;
+           if (triple) {
+             const rbn::PartJob job = carry ? pending_job(c) : rbn::PartJob{};
+             return rbn::launch_lbm3(c->stream, c->prec, c->geom, lp, c->F2, c->uv[s2], c->uv[s3],
+                                     moments3 ? c->rho : nullptr, moments3 ? c->T : nullptr, c->consts.data(),
+                                     t.seg3, t.xcd2, t.reverse && ((istep / 3) & 1),
+                                     carry ? &job : nullptr);
+           }
            if (pair) {
              const rbn::PartJob job = carry ? pending_job(c) : rbn::PartJob{};
              rbn::NudgePair np{};
@@ -1360,7 +1391,7 @@ int step_loop(rbn_ctx *c, int nsteps) {
                if (carry_head) {
                  np.head_obs = &o3;
                  np.head_dep = &c->dep2;
-                 const int r3 = istep3 % P.ntempread;
+                 const int r3 = hstep % P.ntempread;
                  np.head_branch = r3 == 1 ? 1 : (r3 == 0 ? 0 : 2);
                  np.head_ntempread = P.ntempread;
                }
@@ -1378,7 +1409,7 @@ int step_loop(rbn_ctx *c, int nsteps) {
       c->npend = 0;
       return rc;
     }
-    head_done = carry_head ? istep3 : -1;
+    head_done = carry_head ? hstep : -1;
     if (carry && (rc = pending_launched(c))) return rc;
     if (nudged && dovl) {
       HIPCHK(c, hipEventRecord(c->ev_lat[n & 1], c->stream));
@@ -1401,16 +1432,18 @@ int step_loop(rbn_ctx *c, int nsteps) {
     if (pair) std::swap(c->F, c->F2);
     if (pair && nudged) std::swap(c->nud, c->nud2);
     const int prv = c->cur;
-    c->cur = pair ? s2 : s1;
+    c->cur = triple ? s3 : (pair ? s2 : s1);
     c->moments_valid = writes_moments;
     if (fuse) {
-      // this launch's particle updates wait for the next launch
-      const int rstep = pair ? istep2 : istep;
-      const bool rec = pair ? record2 : record;
+      // this launch's particle updates wait for the next launch (only its last step can be a
+      // record step)
+      const int rstep = triple ? istep3 : (pair ? istep2 : istep);
+      const bool rec = triple ? record3 : (pair ? record2 : record);
       rbn_ctx::RecSlot *slot = nullptr;
       if (rec && (rc = claim_record_slot(c, &slot))) return rc;
       if (pair) add_pending(c, istep, s1, prv, false, false, nullptr);
-      add_pending(c, rstep, c->cur, pair ? s1 : prv, rec, rec && P.prtcl_writefull, slot);
+      if (triple) add_pending(c, istep2, s2, s1, false, false, nullptr);
+      add_pending(c, rstep, c->cur, triple ? s2 : (pair ? s1 : prv), rec, rec && P.prtcl_writefull, slot);
     } else if (parts) {
       // K2 of step s; with overlap, on the particle stream alongside K1 of step s+1.
       if (overlap) {
@@ -1430,8 +1463,8 @@ int step_loop(rbn_ctx *c, int nsteps) {
       c->part_record_pending = overlap && record;
       if (slot && c->rec_fn && (rc = queue_record_copy(c, *slot, istep, ks))) return rc;
     }
-    const bool dstep = pair ? diag2 : diag;
-    if (dstep && c->diag_fn && (rc = report_energy(c, pair ? istep2 : istep))) return rc;
+    const bool dstep = triple ? diag3 : (pair ? diag2 : diag);
+    if (dstep && c->diag_fn && (rc = report_energy(c, triple ? istep3 : (pair ? istep2 : istep)))) return rc;
   }
   return RBN_OK;
 }
@@ -1507,8 +1540,12 @@ int rbn_set_tuning(rbn_ctx *c, const char *key, int value) {
   } else if (!strcmp(key, "lbm_reverse")) {
     c->tune.reverse = value != 0;
   } else if (!strcmp(key, "lbm_steps")) {
-    if (value != 1 && value != 2) return fail(c, RBN_EINVAL, "lbm_steps must be 1 or 2");
-    c->tune.steps2 = value == 2;
+    if (value < 1 || value > 3) return fail(c, RBN_EINVAL, "lbm_steps must be 1, 2 or 3");
+    c->tune.steps2 = value >= 2;
+    c->tune.steps3 = value == 3;
+  } else if (!strcmp(key, "lbm3_seg")) {
+    if (value != 0 && (value < 4 || value > 4096)) return fail(c, RBN_EINVAL, "lbm3_seg must be 0 or 4..4096");
+    c->tune.seg3 = value;
   } else if (!strcmp(key, "lbm2_tile_h")) {
     if (value != 0 && value != 4 && value != 8 && value != 16)
       return fail(c, RBN_EINVAL, "lbm2_tile_h must be 0, 4, 8 or 16");
@@ -1546,7 +1583,8 @@ int rbn_get_tuning(const rbn_ctx *c, const char *key, int *value) {
   else if (!strcmp(key, "lbm_uv_keep")) *value = c->tune.uv_keep;
   else if (!strcmp(key, "lbm_xcd")) *value = c->tune.xcd;
   else if (!strcmp(key, "lbm_reverse")) *value = c->tune.reverse;
-  else if (!strcmp(key, "lbm_steps")) *value = c->tune.steps2 ? 2 : 1;
+  else if (!strcmp(key, "lbm_steps")) *value = c->tune.steps3 ? 3 : (c->tune.steps2 ? 2 : 1);
+  else if (!strcmp(key, "lbm3_seg")) *value = c->tune.seg3;
   else if (!strcmp(key, "lbm2_tile_h")) *value = c->tune.tile_h;
   else if (!strcmp(key, "lbm2_seg")) *value = c->tune.seg;
   else if (!strcmp(key, "lbm2_xcd")) *value = c->tune.xcd2;

@@ -73,6 +73,13 @@ struct rbn_ensemble {
   // tuning
   int tile_w = 256, tile_h = 1, wg_per_cu = 4, order = 0;
   bool built = false;
+  // each member's state as the last successful call left it (step, ring position, planes):
+  // a member stepped on its own since then makes the next call rebuild descriptors and graphs
+  struct Sig {
+    int istep, cur;
+    const void *fA, *F, *nud;
+  };
+  std::vector<Sig> sig;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   std::string err;
@@ -515,7 +522,7 @@ void advance_members(rbn_ensemble *e, int n, int launches, int pairs) {
       std::swap(c->F, c->F2);
       std::swap(c->nud, c->nud2);
     }
-    c->cur = (c->cur + n) % rbn_ctx::kVel;
+    c->cur = (c->cur + n) % rbn::kCycle;
     c->moments_valid = true;
   }
 }
@@ -751,10 +758,39 @@ int capture_nudged(rbn_ensemble *e) {
   return RBN_OK;
 }
 
+rbn_ensemble::Sig member_sig(const rbn_ctx *c) { return {c->istep, c->cur, c->fA, c->F, c->nud}; }
+
+void record_sigs(rbn_ensemble *e) {
+  e->sig.clear();
+  for (const rbn_ctx *c : e->m) e->sig.push_back(member_sig(c));
+}
+
+// Wait for the members' own streams.  A member whose latest velocity sits past the
+// ensemble's ring (rbn_step's three-step launches use slots kCycle and up) has that slot's
+// buffer swapped into the ring (pointers only: no step is pending between calls, and only
+// the latest velocity is read again).  Either that or a member stepped on its own since the
+// last call invalidates the descriptors and graphs.
 int sync_members(rbn_ensemble *e) {
-  for (rbn_ctx *c : e->m) {
+  bool same = e->sig.size() == e->m.size();
+  for (size_t i = 0; i < e->m.size(); i++) {
+    rbn_ctx *c = e->m[i];
     EHIP(e, hipStreamSynchronize(c->stream));
     EHIP(e, hipStreamSynchronize(c->pstream));
+    if (c->cur >= rbn::kCycle) {
+      const int k = c->cur % rbn::kCycle;
+      std::swap(c->uv[c->cur], c->uv[k]);
+      c->cur = k;
+      same = false;
+    }
+    if (same) {
+      const rbn_ensemble::Sig &a = e->sig[i];
+      const rbn_ensemble::Sig b = member_sig(c);
+      same = a.istep == b.istep && a.cur == b.cur && a.fA == b.fA && a.F == b.F && a.nud == b.nud;
+    }
+  }
+  if (!same) {
+    e->built = false;
+    drop_graph(e);
   }
   return RBN_OK;
 }
@@ -877,6 +913,7 @@ int run_nudged(rbn_ensemble *e, int nsteps) {
   EHIP(e, hipStreamSynchronize(e->stream));
   EHIP(e, hipStreamSynchronize(e->pstream));
   advance_members(e, nsteps, ps.L, ps.P);
+  record_sigs(e);
   // The observer descriptors and the cycle's record plan hold the members' step counts
   // modulo the cycle (and, in the two-step mode, an even number of launches and of two-step
   // launches): they (and the graphs) stay valid after a whole number of cycles.
@@ -955,6 +992,7 @@ int run_steps(rbn_ensemble *e, int nsteps) {
   EHIP(e, hipStreamSynchronize(e->stream));
   EHIP(e, hipStreamSynchronize(e->pstream));
   advance_members(e, nsteps, launches, npairs);
+  record_sigs(e);
   if (nsteps % rbn::kCycle || (launches & 1) || (npairs & 1)) {
     // member parity or ring position changed: rebuild descriptors and graph
     e->built = false;

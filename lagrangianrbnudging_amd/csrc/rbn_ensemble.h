@@ -6,8 +6,10 @@
 namespace rbn {
 
 // Steps of a replayed cycle: after kCycle steps the A/B population planes and the velocity
-// ring of every member are back where they started.
-constexpr int kCycle = rbn_ctx::kVel;
+// ring of every member are back where they started.  An ensemble keeps each member's ring in
+// its first kCycle velocity slots (rbn_step uses all rbn_ctx::kVel).
+constexpr int kCycle = 6;
+static_assert(kCycle <= rbn_ctx::kVel, "the ensemble's ring lies inside the member's");
 static_assert(kCycle % 2 == 0, "a cycle must return the A/B planes");
 
 // Rows of a two-step tile (64 columns) in the ensemble: the single-lattice defaults.

@@ -36,9 +36,9 @@ void dfree_t(T *&p) {
 int check_ctx(rbn_ctx *c);
 
 // Kernel classes of the per-launch timing (rbn_set_kernel_timing / rbn_kernel_stats).
-// KT_LBM_FUSED: a lattice launch carrying earlier steps' particle updates; KT_LBM2: a
-// two-step lattice launch (with or without carried particle updates).
-enum { KT_LBM, KT_PARTICLES, KT_DEPOSIT, KT_ADJOINT, KT_LBM_FUSED, KT_LBM2, KT_COUNT };
+// KT_LBM_FUSED: a lattice launch carrying earlier steps' particle updates; KT_LBM2 / KT_LBM3:
+// a two-step / three-step lattice launch (with or without carried particle updates).
+enum { KT_LBM, KT_PARTICLES, KT_DEPOSIT, KT_ADJOINT, KT_LBM_FUSED, KT_LBM2, KT_LBM3, KT_COUNT };
 static_assert(KT_COUNT == rbn_ctx::kKernelClasses, "kernel classes");
 extern const char *const kKernelNames[KT_COUNT];
 hipEvent_t take_event(rbn_ctx *c);

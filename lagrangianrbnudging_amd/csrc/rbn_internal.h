@@ -202,6 +202,8 @@ struct LbmTuning {
   int overlap = 0;  // particle update of step s concurrent with the lattice step s+1
   int fuse = 1;     // particle update of step s in the grid of the lattice step s+1
   int steps2 = 1;   // truth run: two steps per lattice launch (temporal blocking in LDS)
+  int steps3 = 0;   // truth run: three steps per lattice launch by row sweeps (with steps2)
+  int seg3 = 0;     // three-step sweeps: rows per segment (0: one wave of workgroups)
   int tile_h = 0;   // rows of a two-step tile (4, 8, 16; 0: 8 in FP32, 4 in FP64)
   int seg = 0;      // two steps by row sweeps over segments of seg rows (8..64); 0: by tiles
   int xcd2 = 1;     // row sweeps: consecutive strips and segments on one XCD
@@ -210,15 +212,16 @@ struct LbmTuning {
   int dep_in_tile = 1;  // nudged two-step launches: the second step's bilinear deposit summed in the tiles
 };
 
-// The particle updates of one or two consecutive steps, in step order, carried by the next
+// The particle updates of one to three consecutive steps, in step order, carried by the next
 // lattice launch: uv[u] / uvp[u] are the velocity slots of update u's step and the one
 // before, rec[u] its record slot (or null), T the temperature of the one record step among
 // them (a record update is never carried past a launch that rewrites the moments).
+constexpr int kMaxCarried = 3;
 struct PartJob {
   int n = 0;
-  const void *uv[2] = {}, *uvp[2] = {};
-  double *rec[2] = {};
-  int record[2] = {}, full[2] = {};
+  const void *uv[kMaxCarried] = {}, *uvp[kMaxCarried] = {};
+  double *rec[kMaxCarried] = {};
+  int record[kMaxCarried] = {}, full[kMaxCarried] = {};
   const void *T = nullptr;
   PartPtrs pp{};
 };
@@ -254,6 +257,16 @@ struct NudgePair {
 };
 // Rows of the two-step tiles launch_lbm2 uses for precision prec and tuning tile_h.
 int lbm2_tile_rows(int prec, int tile_h);
+// Three steps of K1 in one launch by row sweeps (rbn_lbm2.hip, rbn_lbm3_wave.h): p holds the
+// first step's inputs and the third step's outputs (fB, gB), p.F = F(s), p.uv = the first
+// step's velocity slot; Fout receives F(s+3), uv2 / uv3 the velocities of steps s+1 / s+2,
+// rho3 / T3 (or null) the moments of step s+2.  seg: rows per segment (0: the fewest that
+// put every workgroup on the device at once); xcd: consecutive strips and segments on one
+// XCD.  job (or null): particle updates carried as extra workgroups ahead of the strips.
+hipError_t launch_lbm3(hipStream_t s, int prec, const Geom &g, const LbmPtrs &p, void *Fout, void *uv2, void *uv3,
+                       void *rho3, void *T3, const void *step_const, int seg, int xcd, int flip, const PartJob *job);
+// Rows per segment launch_lbm3 uses for tuning seg.
+int lbm3_segment_rows(int prec, const Geom &g, int seg);
 // Two steps of K1 in one launch (rbn_lbm2.hip): p holds the first step's inputs and the second
 // step's outputs (fB, gB), p.F = F(s), p.uv = the first step's velocity slot; Fout receives
 // F(s+2), uv2 the second step's velocity, rho2 / T2 (or null) its moments.  job (or null):
@@ -334,12 +347,13 @@ struct rbn_ctx {
   void *fA = nullptr, *fB = nullptr, *gA = nullptr, *gB = nullptr;
   void *F = nullptr;
   void *F2 = nullptr;  // second force plane: a two-step launch reads F and writes F2, then swaps
-  // Velocity ring: uv[cur] is the latest hydrovar velocity, uv[(cur+5)%6] the step before.
-  // Each slot holds (u, v) interleaved per node (2 * plane reals), so that the particle
-  // update fetches one cache line per patch row for both components.  Six slots let a
-  // two-step lattice launch (steps s, s+1) carry the particle updates of steps s-2 and s-1,
-  // which read the slots of steps s-3 .. s-1.
-  static constexpr int kVel = 6;
+  // Velocity ring: uv[cur] is the latest hydrovar velocity; the step before it is in the slot
+  // that was cur then (uv[(cur+7)%8] while rbn_step advances the ring; an ensemble keeps the
+  // ring in the first six slots).  Each slot holds (u, v) interleaved per node (2 * plane
+  // reals), so that the particle update fetches one cache line per patch row for both
+  // components.  Eight slots let a three-step lattice launch (steps s .. s+2) carry the
+  // particle updates of steps s-3 .. s-1, which read the slots of steps s-4 .. s-1.
+  static constexpr int kVel = 8;
   void *uv[kVel] = {};
   int cur = 0;
   void *rho = nullptr, *T = nullptr;
@@ -373,12 +387,12 @@ struct rbn_ctx {
   } rec[kRecSlots];
   int rec_next = 0;
   // The particle updates waiting to ride in the next lattice launch (particle_fuse): the
-  // latest step's, or the latest two steps' after a two-step launch, in step order; never
-  // pending when rbn_step returns.
+  // latest step's, or the latest two or three steps' after a launch of that many steps, in
+  // step order; never pending when rbn_step returns.
   struct PendingParticles {
     int istep = 0, cur = 0, prv = 0, record = 0, full = 0;
     RecSlot *slot = nullptr;
-  } pend[2];
+  } pend[rbn::kMaxCarried];
   int npend = 0;
   hipStream_t copy_stream = nullptr;
   int *d_err = nullptr;
@@ -417,7 +431,7 @@ struct rbn_ctx {
   double *d_energy = nullptr;  // partials + 2 results
 
   // timing: every timing_every-th launch of each kernel class is bracketed with events
-  static constexpr int kKernelClasses = 6;
+  static constexpr int kKernelClasses = 7;
   bool timing = false;
   int timing_every = 1;
   long long k_seen[kKernelClasses] = {};

@@ -6,6 +6,7 @@
 // that is safe).
 #include "rbn_lbm2_tile.h"
 #include "rbn_lbm2_wave.h"
+#include "rbn_lbm3_wave.h"
 #include "rbn_nudge_node.h"
 #include "rbn_particle_node.h"
 
@@ -89,6 +90,61 @@ __global__ __launch_bounds__(SW) void k_lbm2w(Geom g, WaveArgs a, StepConst<R> c
 
 constexpr int kStripW = 256;
 
+struct Sweep3Args {
+  lbm3::ThreeStepPtrs p;
+  int strips, seg, nwg, flip, xcd;
+};
+
+// Three steps per launch (rbn_lbm3_wave.h): strip sx outputs columns [sx nx / strips,
+// (sx + 1) nx / strips), so that every strip has nearly the same number of live threads.
+template <typename R, int SW>
+__global__ __launch_bounds__(SW) __attribute__((amdgpu_waves_per_eu(3))) void k_lbm3w(Geom g, Sweep3Args a,
+                                                                                    StepConst<R> c,
+                                                                                    part::PartSteps<R> ps) {
+  __shared__ lbm3::Lds3<R, SW> L;
+  if ((int)blockIdx.x < ps.nblocks) {
+    part::run_part_steps(ps, blockIdx.x * SW + threadIdx.x);
+    return;
+  }
+  int b = (int)blockIdx.x - ps.nblocks;
+  if (a.xcd) b = xcd_order(b, a.nwg);
+  const int w = a.flip ? a.nwg - 1 - b : b;
+  const int sy = w / a.strips, sx = w - sy * a.strips;
+  const int ox0 = (int)((long long)sx * g.nx / a.strips), ox1 = (int)((long long)(sx + 1) * g.nx / a.strips);
+  const int ja = sy * a.seg;
+  lbm3::three_step_strip<R, SW>(g, a.p, c, ox0, ox1, ja, min(ja + a.seg, g.ny), L);
+}
+
+template <typename R>
+const void *lbm3_kernel() {
+  return reinterpret_cast<const void *>(&k_lbm3w<R, kStripW>);
+}
+
+// Workgroups of k_lbm3w that fit on the device at once (resident on every CU), per device
+// and kernel, measured once.
+int lbm3_slots(int prec) {
+  static int cache[64][2] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
+  int &slot = cache[dev][prec == RBN_FP64];
+  if (slot > 0) return slot;
+  const void *k = prec == RBN_FP64 ? lbm3_kernel<double>() : lbm3_kernel<float>();
+  int per_cu = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, kStripW, 0) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  slot = per_cu * cus;
+  return slot;
+}
+
+template <typename R>
+void dispatch_sweep3(hipStream_t s, const Geom &g, const Sweep3Args &a, const void *step_const,
+                     const PartJob *job) {
+  const StepConst<R> &c = *(const StepConst<R> *)step_const;
+  const part::PartSteps<R> ps = part::part_steps<R>(g, job, kStripW);
+  const dim3 grid((unsigned)(a.nwg + ps.nblocks)), block(kStripW);
+  hipLaunchKernelGGL((k_lbm3w<R, kStripW>), grid, block, 0, s, g, a, c, ps);
+}
+
 template <typename R>
 void dispatch_wave(hipStream_t s, const Geom &g, const WaveArgs &a, int variant, const void *step_const,
                    const PartJob *job) {
@@ -156,6 +212,45 @@ void dispatch(hipStream_t s, int prec, int nt, int th, const Geom &g, const Lbm2
 }  // namespace
 
 int lbm2_tile_rows(int prec, int tile_h) { return tile_rows_of(prec, tile_h); }
+
+int lbm3_strips(int nx) { return (nx + kStripW - 5) / (kStripW - 4); }
+
+int lbm3_segment_rows(int prec, const Geom &g, int seg) {
+  if (seg > 0) return min(seg, g.ny);
+  // 0: the fewest rows per segment that still put every workgroup on the device at once
+  // (one wave of workgroups, no tail), at least 16
+  const int slots = lbm3_slots(prec);
+  const int strips = lbm3_strips(g.nx);
+  const int nseg = slots > 0 ? max(1, slots / strips) : 1;
+  return min(g.ny, max(16, (g.ny + nseg - 1) / nseg));
+}
+
+hipError_t launch_lbm3(hipStream_t s, int prec, const Geom &g, const LbmPtrs &p, void *Fout, void *uv2, void *uv3,
+                       void *rho3, void *T3, const void *step_const, int seg, int xcd, int flip, const PartJob *job) {
+  Sweep3Args a{};
+  a.p.fA = p.fA;
+  a.p.gA = p.gA;
+  a.p.fB = p.fB;
+  a.p.gB = p.gB;
+  a.p.Fin = p.F;
+  a.p.Fout = Fout;
+  a.p.uv[0] = p.uv;
+  a.p.uv[1] = uv2;
+  a.p.uv[2] = uv3;
+  a.p.rho = rho3;
+  a.p.T = T3;
+  a.p.bad = p.bad;
+  a.seg = lbm3_segment_rows(prec, g, seg);
+  a.strips = lbm3_strips(g.nx);
+  a.nwg = a.strips * ((g.ny + a.seg - 1) / a.seg);
+  a.flip = flip;
+  a.xcd = xcd;
+  if (prec == RBN_FP64)
+    dispatch_sweep3<double>(s, g, a, step_const, job);
+  else
+    dispatch_sweep3<float>(s, g, a, step_const, job);
+  return hipGetLastError();
+}
 
 hipError_t launch_lbm2(hipStream_t s, int prec, const Geom &g, const LbmPtrs &p, void *Fout, void *uv2,
                        void *rho2, void *T2, const void *step_const, int tile_h, int seg, int xcd, int variant,

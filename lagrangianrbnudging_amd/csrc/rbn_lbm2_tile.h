@@ -186,10 +186,9 @@ __device__ __forceinline__ void collide32(const StepConst<float> &c, const float
   eq_pops32(dr, rho, A, feq);
 #pragma unroll
   for (int k = 0; k < 9; k++) fo[k] = f[k] * c.one_m_om + c.om * feq[k] + c.src[k] * fac[k];
-  const float Q = 0.f;
   eq_pops32(dT, T, A, geq);
 #pragma unroll
-  for (int k = 0; k < 9; k++) go[k] = q[k] * c.one_m_omg + c.omg * geq[k] - c.bw[k] * Q;
+  for (int k = 0; k < 9; k++) go[k] = q[k] * c.one_m_omg + c.omg * geq[k];
 }
 
 // The nudging of one node in a step of the nudged run: its deposit count, mean and (bilinear

@@ -414,15 +414,21 @@ __device__ __forceinline__ void lbm_node32(const Geom &g, const LbmPtrs &p, cons
 #pragma unroll
   for (int k = 0; k < 9; k++)
     st<NT>(&fB[(size_t)k * g.plane + r0 + i], f[k] * c.one_m_om + c.om * feq[k] + c.src[k] * fac[k]);
-  float Q = 0.f;
-  if (VARIANT == RBN_VARIANT_NUDGING && nv) {
-    Q = c.alpha * (wn * (T - Tb));
-    ((float *)p.nud)[r0 + i] = -Q;
-  }
   eq_pops32(dT, T, A, geq);
+  if (VARIANT == RBN_VARIANT_NUDGING) {
+    float Q = 0.f;
+    if (nv) {
+      Q = c.alpha * (wn * (T - Tb));
+      ((float *)p.nud)[r0 + i] = -Q;
+    }
 #pragma unroll
-  for (int k = 0; k < 9; k++)
-    st<NT>(&gB[(size_t)k * g.plane + r0 + i], q[k] * c.one_m_omg + c.omg * geq[k] - c.bw[k] * Q);
+    for (int k = 0; k < 9; k++)
+      st<NT>(&gB[(size_t)k * g.plane + r0 + i], q[k] * c.one_m_omg + c.omg * geq[k] - c.bw[k] * Q);
+  } else {
+    // truth run: no nudging source (the shared form's - bw_k * 0 term is left out)
+#pragma unroll
+    for (int k = 0; k < 9; k++) st<NT>(&gB[(size_t)k * g.plane + r0 + i], q[k] * c.one_m_omg + c.omg * geq[k]);
+  }
 }
 
 // First node of this thread in lattice block L of N.  The lattice blocks enumerate row

@@ -318,11 +318,11 @@ __device__ __forceinline__ void particle_update(const VelField<R> &UV, const Vel
 // workgroups of the launch, one thread per particle, run them in step order.
 template <typename R>
 struct PartSteps {
-  VelField<R> uv[2], uvp[2];
+  VelField<R> uv[kMaxCarried], uvp[kMaxCarried];
   Field<R> T;
   PartPtrs pp;
-  double *rec[2];
-  int record[2], full[2];
+  double *rec[kMaxCarried];
+  int record[kMaxCarried], full[kMaxCarried];
   int n, nblocks;
 };
 

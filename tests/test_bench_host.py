@@ -93,8 +93,12 @@ def test_a_failing_leg_is_reported_inside_the_line(capsys):
 
 
 def test_launch_minimum_bytes_never_exceed_the_algorithmic_bytes():
-    # two one-step launches move 2 * 40 reals per node; a two-step launch at least 42
+    # two one-step launches move 2 * 40 reals per node; a two-step launch at least 42, a
+    # three-step one 44 (every population and the force read and written once, the velocity
+    # of each step written)
     for prec, s in ((0, 8), (1, 4)):
         nodes = bench.NX * bench.NY
         assert bench.launch_min_bytes(prec, 0.0, False) == nodes * 42 * s
+        assert bench.launch_min_bytes(prec, 0.0, False, steps=3) == nodes * 44 * s
         assert bench.launch_min_bytes(prec, 0.05, True) < bench.launch_bytes(prec, 0.05, True, steps=2)
+        assert bench.launch_min_bytes(prec, 0.0, True, steps=3) < bench.launch_bytes(prec, 0.0, True, steps=3)

@@ -63,6 +63,35 @@ def test_ensemble_equals_individual_steps(prec, tuning):
     e.close()
 
 
+@pytest.mark.parametrize("prec", [rbn.RBN_FP64, rbn.RBN_FP32])
+@pytest.mark.parametrize("order", [0, 8])
+def test_ensemble_after_members_step_on_their_own(prec, order):
+    # Members stepped on their own between ensemble calls (three steps per launch, which
+    # leaves a member's latest velocity past the ensemble's six-slot ring), all of them or one
+    # of them: the next ensemble call sees the changed members (rebuilds its descriptors and
+    # graph) and continues exactly as stepping every member on its own does.
+    ens_members = _members(prec)
+    ref_members = _members(prec)
+    for s in ens_members + ref_members:
+        s.set_tuning(lbm_steps=3)
+    e = rbn.Ensemble(ens_members)
+    e.set_tuning(256, 1, 0, order)
+    for n, own in ((12, None), (7, "all"), (12, None), (4, 3), (24, None), (5, "all"), (13, None)):
+        if own is None:
+            e.step(n)
+            for s in ref_members:
+                s.step(n)
+        elif own == "all":
+            for a, b in zip(ens_members, ref_members):
+                a.step(n)
+                b.step(n)
+        else:
+            ens_members[own].step(n)
+            ref_members[own].step(n)
+    _compare(ens_members, ref_members)
+    e.close()
+
+
 # Nudged members: (nx, ny, Ra, observers, ntempread, deposit, alpha)
 NUDGED = [(48, 24, 1e5, 40, 4, rbn.RBN_DEPOSIT_BOX7, 0.3), (64, 40, 3e5, 90, 5, rbn.RBN_DEPOSIT_BILINEAR, 1.0),
           (40, 24, 5e4, 0, 5, rbn.RBN_DEPOSIT_BILINEAR, 0.1), (72, 32, 2e5, 25, 3, rbn.RBN_DEPOSIT_BOX7, 0.05)]

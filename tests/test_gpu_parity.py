@@ -55,7 +55,7 @@ def _compare_records(g, o, full):
             assert_equal_arrays(x[k], y[k], f"record {x['step']} {k}")
 
 
-@pytest.mark.parametrize("fuse,steps", [(1, 1), (0, 1), (1, 2)])
+@pytest.mark.parametrize("fuse,steps", [(1, 1), (0, 1), (1, 2), (1, 3)])
 @pytest.mark.parametrize("writefull", [0, 1])
 def test_truth_run_fp64_bit_exact_tiny(writefull, fuse, steps):
     p = rb_params(32, 16, 1e5, noutconfig=7, prtcl_writefull=writefull, nout=50)
@@ -74,7 +74,7 @@ def test_truth_run_fp64_bit_exact_tiny(writefull, fuse, steps):
         assert abs(a[2] - b[2]) <= 1e-12 * abs(b[2])
 
 
-@pytest.mark.parametrize("steps", [1, 2])
+@pytest.mark.parametrize("steps", [1, 2, 3])
 @pytest.mark.parametrize("writefull", [0, 1])
 def test_particle_fuse_fp64_bit_exact(writefull, steps):
     # The particle update riding in the next lattice launch, including the cases that fall
@@ -110,7 +110,8 @@ def test_particle_fuse_fp32_matches_separate_launches(writefull):
     for tune in (dict(particle_fuse=0, lbm_steps=1), dict(particle_fuse=1, lbm_steps=1),
                  dict(particle_fuse=1, lbm_steps=1, lbm_xcd=2), dict(particle_fuse=1, lbm_steps=2),
                  dict(particle_fuse=1, lbm_steps=2, lbm2_seg=0, lbm2_tile_h=4),
-                 dict(particle_fuse=1, lbm_steps=2, lbm2_seg=16, lbm2_xcd=0), dict(particle_fuse=0, lbm_steps=2)):
+                 dict(particle_fuse=1, lbm_steps=2, lbm2_seg=16, lbm2_xcd=0), dict(particle_fuse=0, lbm_steps=2),
+                 dict(particle_fuse=1, lbm_steps=3), dict(particle_fuse=1, lbm_steps=3, lbm3_seg=7, lbm2_xcd=0), dict(particle_fuse=0, lbm_steps=3)):
         g = rbn.Solver(p)
         g.set_tuning(**tune)
         g.collect_records()
@@ -165,6 +166,64 @@ def test_two_step_launch_is_two_steps(prec, shape, tile_h, seg, variant):
         _compare_state(runs[1], o)
     for s in runs:
         s.close()
+
+
+@pytest.mark.parametrize("prec", [rbn.RBN_FP64, rbn.RBN_FP32])
+@pytest.mark.parametrize("shape,seg,xcd", [((200, 44), 0, 1), ((200, 44), 8, 0), ((40, 20), 4, 1), ((8, 12), 0, 1),
+                                           ((252, 30), 11, 1), ((253, 17), 5, 0), ((520, 16), 64, 1),
+                                           ((300, 37), 13, 0), ((64, 70), 4096, 1)])
+def test_three_step_launch_is_three_steps(prec, shape, seg, xcd):
+    # Row sweeps three steps deep: a lattice of 8 columns (the strip's threads wrap around it
+    # many times), exactly one strip (252 columns), two uneven
+    # strips (253), several strips, segments of 4 rows up to one segment for the whole
+    # lattice, segments that do not divide the rows, both workgroup orders.  Energy diagnostics
+    # every 7 steps (a diagnostics step is never among the first two of a launch), calls of
+    # lengths that leave launches of three, two and one step.
+    nx, ny = shape
+    p = rb_params(nx, ny, 1e5, particle=0, nout=7)
+    p.precision = prec
+    T0 = conduction_T(p, 1e-2, seed=29)
+    runs = []
+    for steps in (1, 3):
+        s = rbn.Solver(p)
+        s.set_tuning(lbm_steps=steps, lbm3_seg=seg, lbm2_xcd=xcd)
+        s.collect_diagnostics()
+        s.init_fields(T=T0)
+        for n in (23, 8, 1, 2, 9, 4):
+            s.step(n)
+        runs.append(s)
+    _compare_state_pair(*runs)
+    assert runs[0].diagnostics == runs[1].diagnostics
+    if prec == rbn.RBN_FP64:
+        o = Oracle(p)
+        o.init_fields(T=T0)
+        o.step(47)
+        _compare_state(runs[1], o)
+    for s in runs:
+        s.close()
+
+
+def test_three_step_launch_counts():
+    # 31 steps, records every 1000: ten three-step launches, the last step alone (it writes the
+    # moments); records every 5: launches end on each record step (steps 1-3, 4-5, 6-8, 9-10, ...).
+    p = rb_params(64, 32, 1e5, noutconfig=1000)
+    g = rbn.Solver(p)
+    g.set_tuning(lbm_steps=3)
+    assert g.tuning()["lbm_steps"] == 3
+    setup(g, p, conduction_T(p, 1e-2, seed=25), random_particles(p, 50, 50, seed=26))
+    g.kernel_timing(True, every=1)
+    g.step(31)
+    assert [g.kernel_stats(k)[0] for k in ("lbm", "lbm2", "lbm3", "lbm_fused", "particles")] == [0, 0, 10, 1, 1]
+    g.close()
+    p = rb_params(64, 32, 1e5, noutconfig=5)
+    g = rbn.Solver(p)
+    g.set_tuning(lbm_steps=3)
+    setup(g, p, conduction_T(p, 1e-2, seed=25), random_particles(p, 50, 50, seed=26))
+    g.kernel_timing(True, every=1)
+    g.step(30)
+    # per 5 steps: a three-step and a two-step launch (the record step ends a launch)
+    assert [g.kernel_stats(k)[0] for k in ("lbm3", "lbm2", "lbm", "lbm_fused")] == [6, 6, 0, 0]
+    g.close()
 
 
 def test_sampled_kernel_timing_counts():
