@@ -160,35 +160,11 @@ __device__ __forceinline__ void collide64(const StepConst<double> &c, const doub
 }
 
 // Moments and collision of a truth-run node with shifted FP32 populations: lbm_node32's
-// arithmetic.
+// (collide_t32).
 __device__ __forceinline__ void collide32(const StepConst<float> &c, const float f[9], const float q[9],
                                           float Fy, float fo[9], float go[9], float &rho, float &T, float &u,
                                           float &v, float &Fn) {
-  float dr = 0.f, dT = 0.f;
-#pragma unroll
-  for (int k = 0; k < 9; k++) dr += f[k];
-#pragma unroll
-  for (int k = 0; k < 9; k++) dT += q[k];
-  rho = c.rho0 + dr;
-  T = c.T0 + dT;
-  const float rhoi = 1.f / rho;
-  const float jx = f[1] - f[3] + f[5] - f[6] - f[7] + f[8];
-  const float jy = f[2] - f[4] + f[5] + f[6] - f[7] - f[8];
-  u = jx * rhoi;
-  v = jy * rhoi + 0.5f * rhoi * Fy;
-  Fn = c.g0 * rho * T;
-  float A[9];
-  eq_dev32(u, v, A);
-  const float tz = -3.f * v * Fy, tp = 3.f * (1.f - v) * Fy, tm = 3.f * (-1.f - v) * Fy;
-  const float d2 = 9.f * v * Fy, d5 = 9.f * (u + v) * Fy, d6 = 9.f * (v - u) * Fy;
-  const float fac[9] = {tz, tz, tp + d2, tz, tm + d2, tp + d5, tp + d6, tm + d5, tm + d6};
-  float feq[9], geq[9];
-  eq_pops32(dr, rho, A, feq);
-#pragma unroll
-  for (int k = 0; k < 9; k++) fo[k] = f[k] * c.one_m_om + c.om * feq[k] + c.src[k] * fac[k];
-  eq_pops32(dT, T, A, geq);
-#pragma unroll
-  for (int k = 0; k < 9; k++) go[k] = q[k] * c.one_m_omg + c.omg * geq[k];
+  collide_t32(c, f, q, Fy, fo, go, rho, T, u, v, Fn);
 }
 
 // The nudging of one node in a step of the nudged run: its deposit count, mean and (bilinear

@@ -297,6 +297,59 @@ __device__ __forceinline__ void eq_pops32(float d, float level, const float A[9]
   eq[0] = d - s;
 }
 
+// Moments and collision of a truth-run node with shifted FP32 populations, written for short
+// dependent chains: pairwise sums for the mass and temperature deviations, the hardware
+// reciprocal for 1/rho (at most 1 ulp from the rounded quotient), fused multiply-adds in the
+// equilibria and the relaxation.  The same operations as the FP64 path otherwise (the
+// deviations' equilibria of eq_pops32, the Guo forcing, the BGK relaxation with 1 - omega
+// formed from the rounded omega), so the error against FP64 stays within the stated bound.
+// Every FP32 truth-run launch form (one, two or three steps per launch, ensembles) calls
+// this function, so they agree bit for bit.
+__device__ __forceinline__ void collide_t32(const StepConst<float> &c, const float f[9], const float q[9],
+                                            float Fy, float fo[9], float go[9], float &rho, float &T,
+                                            float &u, float &v, float &Fn) {
+  const float dr = ((f[0] + f[1]) + (f[2] + f[3])) + ((f[4] + f[5]) + (f[6] + f[7])) + f[8];
+  const float dT = ((q[0] + q[1]) + (q[2] + q[3])) + ((q[4] + q[5]) + (q[6] + q[7])) + q[8];
+  rho = c.rho0 + dr;
+  T = c.T0 + dT;
+  const float rhoi = __builtin_amdgcn_rcpf(rho);
+  const float jx = ((f[1] - f[3]) + (f[5] - f[6])) + (f[8] - f[7]);
+  const float jy = ((f[2] - f[4]) + (f[5] + f[6])) - (f[7] + f[8]);
+  u = jx * rhoi;
+  v = __builtin_fmaf(0.5f, Fy, jy) * rhoi;
+  Fn = c.g0 * rho * T;
+  // A_k = bracket_k - 1 of the equilibrium: 1/cs2 = 3, 1/(2 cs2) = 1.5, 1/(2 cs2^2) = 4.5
+  const float ui = 3.f * u, vi = 3.f * v;
+  const float usq = u * u, vsq = v * v;
+  const float sumsq = 1.5f * (usq + vsq);
+  const float a = __builtin_fmaf(4.5f, usq, -sumsq), b = __builtin_fmaf(4.5f, vsq, -sumsq);
+  const float uv = ui * vi, s2 = 2.f * sumsq;
+  const float sp = s2 + uv, sm = s2 - uv, pp = ui + vi, pm = ui - vi;
+  const float A[9] = {-sumsq, a + ui, b + vi, a - ui, b - vi, sp + pp, sm - pm, sp - pp, sm + pm};
+  // Guo factor with F_x = 0: ((c_y - v) F_y)/cs2 + ((c.u)(c_y F_y))/cs2^2
+  const float f3 = 3.f * Fy, f9 = 9.f * Fy;
+  const float tz = -v * f3, tp = (1.f - v) * f3, tm = (-1.f - v) * f3;
+  const float d2 = v * f9, d5 = (u + v) * f9, d6 = (v - u) * f9;
+  const float fac[9] = {tz, tz, tp + d2, tz, tm + d2, tp + d5, tp + d6, tm + d5, tm + d6};
+  // equilibria of the deviations (eq_pops32's form): the moving populations
+  // w_k (d + level A_k), the rest population the remainder
+  float feq[9], geq[9];
+#pragma unroll
+  for (int k = 1; k < 9; k++) {
+    feq[k] = (float)kW[k] * __builtin_fmaf(rho, A[k], dr);
+    geq[k] = (float)kW[k] * __builtin_fmaf(T, A[k], dT);
+  }
+  feq[0] = dr - (((feq[1] + feq[2]) + (feq[3] + feq[4])) + ((feq[5] + feq[6]) + (feq[7] + feq[8])));
+  geq[0] = dT - (((geq[1] + geq[2]) + (geq[3] + geq[4])) + ((geq[5] + geq[6]) + (geq[7] + geq[8])));
+  // the forcing weights (1 - om/2) w_k take three values (w_k does)
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    const float src = c.src[k == 0 ? 0 : (k < 5 ? 1 : 5)];
+    fo[k] = __builtin_fmaf(f[k], c.one_m_om, __builtin_fmaf(c.om, feq[k], src * fac[k]));
+    go[k] = __builtin_fmaf(q[k], c.one_m_omg, c.omg * geq[k]);
+  }
+}
+
 // The fused step at interior node (i, j) (0-based), FP32 with shifted populations.
 template <int VARIANT, int NT>
 __device__ __forceinline__ void lbm_node32(const Geom &g, const LbmPtrs &p, const StepConst<float> &c,
@@ -376,6 +429,22 @@ __device__ __forceinline__ void lbm_node32(const Geom &g, const LbmPtrs &p, cons
     }
   }
 
+  float *fB = (float *)p.fB, *gB = (float *)p.gB;
+  float *F = (float *)p.F;
+  if (VARIANT == RBN_VARIANT_TRAJECTORIES) {
+    float fo[9], go[9], rho, T, u, v, Fn;
+    collide_t32(c, f, q, ld<NT>(&F[r0 + i]), fo, go, rho, T, u, v, Fn);
+    flag_nonfinite(p.bad, rho, T);
+    st<NT>(&F[r0 + i], Fn);
+    st_uv<NT, float>(p.uv, r0 + i, u, v);
+    if (p.rho) ((float *)p.rho)[r0 + i] = rho;
+    if (p.T) ((float *)p.T)[r0 + i] = T;
+#pragma unroll
+    for (int k = 0; k < 9; k++) st<NT>(&fB[(size_t)k * g.plane + r0 + i], fo[k]);
+#pragma unroll
+    for (int k = 0; k < 9; k++) st<NT>(&gB[(size_t)k * g.plane + r0 + i], go[k]);
+    return;
+  }
   float dr = 0.f, dT = 0.f;
 #pragma unroll
   for (int k = 0; k < 9; k++) dr += f[k];
@@ -391,7 +460,6 @@ __device__ __forceinline__ void lbm_node32(const Geom &g, const LbmPtrs &p, cons
   const float T = c.T0 + dT;
   flag_nonfinite(p.bad, rho, T);
   const float rhoi = 1.f / rho;
-  float *F = (float *)p.F;
   const float Fy = ld<NT>(&F[r0 + i]);
   const float jx = f[1] - f[3] + f[5] - f[6] - f[7] + f[8];
   const float jy = f[2] - f[4] + f[5] + f[6] - f[7] - f[8];
@@ -408,27 +476,20 @@ __device__ __forceinline__ void lbm_node32(const Geom &g, const LbmPtrs &p, cons
   const float tz = -3.f * v * Fy, tp = 3.f * (1.f - v) * Fy, tm = 3.f * (-1.f - v) * Fy;
   const float d2 = 9.f * v * Fy, d5 = 9.f * (u + v) * Fy, d6 = 9.f * (v - u) * Fy;
   const float fac[9] = {tz, tz, tp + d2, tz, tm + d2, tp + d5, tp + d6, tm + d5, tm + d6};
-  float *fB = (float *)p.fB, *gB = (float *)p.gB;
   float feq[9], geq[9];
   eq_pops32(dr, rho, A, feq);
 #pragma unroll
   for (int k = 0; k < 9; k++)
     st<NT>(&fB[(size_t)k * g.plane + r0 + i], f[k] * c.one_m_om + c.om * feq[k] + c.src[k] * fac[k]);
   eq_pops32(dT, T, A, geq);
-  if (VARIANT == RBN_VARIANT_NUDGING) {
-    float Q = 0.f;
-    if (nv) {
-      Q = c.alpha * (wn * (T - Tb));
-      ((float *)p.nud)[r0 + i] = -Q;
-    }
-#pragma unroll
-    for (int k = 0; k < 9; k++)
-      st<NT>(&gB[(size_t)k * g.plane + r0 + i], q[k] * c.one_m_omg + c.omg * geq[k] - c.bw[k] * Q);
-  } else {
-    // truth run: no nudging source (the shared form's - bw_k * 0 term is left out)
-#pragma unroll
-    for (int k = 0; k < 9; k++) st<NT>(&gB[(size_t)k * g.plane + r0 + i], q[k] * c.one_m_omg + c.omg * geq[k]);
+  float Q = 0.f;
+  if (nv) {
+    Q = c.alpha * (wn * (T - Tb));
+    ((float *)p.nud)[r0 + i] = -Q;
   }
+#pragma unroll
+  for (int k = 0; k < 9; k++)
+    st<NT>(&gB[(size_t)k * g.plane + r0 + i], q[k] * c.one_m_omg + c.omg * geq[k] - c.bw[k] * Q);
 }
 
 // First node of this thread in lattice block L of N.  The lattice blocks enumerate row
