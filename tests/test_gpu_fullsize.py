@@ -60,27 +60,39 @@ def test_fullsize_fp64_bit_exact_against_oracle():
 
 
 def test_fullsize_fp32_two_step_equals_one_step_and_conserves_mass():
+    # one, two and three steps per launch (three: the default at this size) with the
+    # benchmark's particles and records every 20 steps, calls of 17 and 24 steps
     p = _params(rbn.RBN_FP32, 20)
     T0 = conduction_T(p, 1e-3, seed=3)
     parts = random_particles(p, 100_000, 100_000, seed=4)
     runs = []
-    for steps in (1, 2):
+    for steps in (1, 2, 3):
         s = rbn.Solver(p)
         s.set_tuning(lbm_steps=steps)
+        s.collect_records()
         setup(s, p, T0, parts)
         for n in (17, 24):
             s.step(n)
         runs.append(s)
+    d = rbn.Solver(p)
+    assert d.tuning()["lbm_steps"] == 3
+    d.close()
     fa, ga = runs[0].populations()
-    fb, gb = runs[1].populations()
-    assert_equal_arrays(fa, fb, "f")
-    assert_equal_arrays(ga, gb, "g")
-    del fa, ga, fb, gb
-    pa, pb = runs[0].particles(), runs[1].particles()
-    for k in ("x", "y", "u", "v", "xo", "yo", "uo", "vo"):
-        assert_equal_arrays(pa[k], pb[k], "particle " + k)
-    rho = runs[1].fields()["rho"]
+    for other in runs[1:]:
+        fb, gb = other.populations()
+        assert_equal_arrays(fa, fb, "f")
+        assert_equal_arrays(ga, gb, "g")
+        del fb, gb
+        pa, pb = runs[0].particles(), other.particles()
+        for k in ("x", "y", "u", "v", "xo", "yo", "uo", "vo"):
+            assert_equal_arrays(pa[k], pb[k], "particle " + k)
+        assert [r["step"] for r in runs[0].records] == [r["step"] for r in other.records] == [20, 40]
+        for x, y in zip(runs[0].records, other.records):
+            for k in ("x", "y", "u", "v", "temp"):
+                assert_equal_arrays(x[k], y[k], f"record {x['step']} {k}")
+    del fa, ga
+    rho = runs[2].fields()["rho"]
     assert abs(rho.sum() / (NX * NY) - 1.0) < 1e-6
-    _particles_inside(runs[1])
+    _particles_inside(runs[2])
     for s in runs:
         s.close()
