@@ -257,6 +257,77 @@ def test_reflecting_walls():
     assert st["yo"][0] == 1.2 and st["yo"][1] == 19.8
 
 
+def _halo(F):
+    """A node field [nx][ny] with the particle interpolation's ghost layer: node (i, j) at
+    [i][j] (1-based), periodic copies in x and in y."""
+    nx, ny = F.shape
+    H = np.zeros((nx + 2, ny + 2))
+    H[1:-1, 1:-1] = F
+    H[0, 1:-1], H[-1, 1:-1] = F[-1], F[0]
+    H[:, 0], H[:, -1] = H[:, ny], H[:, 1]
+    return H
+
+
+def _interp(H, x, y):
+    i1, j1 = np.trunc(x).astype(int), np.trunc(y).astype(int)
+    ax, ay = x - i1, y - j1
+    return (H[i1, j1] * (1 - ax) * (1 - ay) + H[i1 + 1, j1] * ax * (1 - ay) + H[i1, j1 + 1] * (1 - ax) * ay
+            + H[i1 + 1, j1 + 1] * ax * ay)
+
+
+def test_particle_step_in_a_varying_flow_matches_an_independent_restatement():
+    # One particle update in a random velocity field that changes from the previous step:
+    # bilinear interpolation with periodic ghosts (positions near both x edges and near both
+    # walls), the tracers' Adams-Bashforth step, the inertial particles' Maxey-Riley step with
+    # the fluid acceleration from time and centred space differences (one-sided at the
+    # walls), then the periodic wrap and the wall reflection.
+    p = params(24, 16, relax=0.9, relaxg=0.8)
+    rng = np.random.default_rng(11)
+    f = W * (1 + 0.05 * rng.uniform(-1, 1, (p.nx, p.ny, 9)))
+    g = W * (0.5 + 0.1 * rng.uniform(-1, 1, (p.nx, p.ny, 9)))
+    up_prev = 0.01 * rng.uniform(-1, 1, (p.nx, p.ny))
+    vp_prev = 0.01 * rng.uniform(-1, 1, (p.nx, p.ny))
+    o = Oracle(p)
+    o.set_populations(f, g, np.ones((p.nx, p.ny)), 0.5 * np.ones((p.nx, p.ny)), up_prev, vp_prev)
+    n = 60
+    x = np.concatenate([rng.uniform(0.01, 0.9, 8), rng.uniform(p.nx - 0.9, p.nx, 8), rng.uniform(0.01, p.nx, n - 16)])
+    y = np.concatenate([rng.uniform(1.0, 1.3, 10), rng.uniform(p.ny - 0.3, p.ny - 1e-9, 10),
+                        rng.uniform(1.0, p.ny - 1e-9, n - 20)])
+    tau = np.where(np.arange(n) % 2 == 0, 0.0, rng.uniform(2.0, 30.0, n))
+    beta = rng.uniform(0.0, 0.9, n)
+    u0, v0 = 0.02 * rng.uniform(-1, 1, n), 0.02 * rng.uniform(-1, 1, n)
+    o.init_particles(np.arange(n, dtype=np.int32), x, y, tau_p=tau, beta_p=beta, up=u0, vp=v0)
+    o.step(1)
+    fl, st = o.fields(), o.particles()
+    U, V = fl["u"], fl["v"]
+    # node accelerations: time differences, centred x (periodic) and y differences,
+    # one-sided y differences on the wall rows
+    dtu, dtv = U - up_prev, V - vp_prev
+    dxu, dxv = (np.roll(U, -1, 0) - np.roll(U, 1, 0)) * 0.5, (np.roll(V, -1, 0) - np.roll(V, 1, 0)) * 0.5
+    dyu, dyv = np.empty_like(U), np.empty_like(V)
+    for D, A in ((dyu, U), (dyv, V)):
+        D[:, 1:-1] = (A[:, 2:] - A[:, :-2]) * 0.5
+        D[:, 0] = A[:, 1] - A[:, 0]
+        D[:, -1] = A[:, -1] - A[:, -2]
+    at = {k: _interp(_halo(F), x, y) for k, F in (("u", U), ("v", V), ("dtu", dtu), ("dtv", dtv), ("dxu", dxu),
+                                                  ("dxv", dxv), ("dyu", dyu), ("dyv", dyv))}
+    heavy = tau > 0
+    inv = np.where(heavy, 1.0 / np.where(heavy, tau, 1.0), 0.0)
+    un = np.where(heavy, u0 + beta * (at["dtu"] + at["u"] * at["dxu"] + at["v"] * at["dyu"]) - inv * (u0 - at["u"]),
+                  at["u"])
+    vn = np.where(heavy, v0 + beta * (at["dtv"] + at["u"] * at["dxv"] + at["v"] * at["dyv"]) - inv * (v0 - at["v"]),
+                  at["v"])
+    xn = np.where(heavy, x + un, x + 1.5 * at["u"] - 0.5 * u0)
+    yn = np.where(heavy, y + vn, y + 1.5 * at["v"] - 0.5 * v0)
+    xn = np.where(xn >= p.nx, xn - p.nx, np.where(xn <= 0.0, xn + p.nx, xn))
+    low, high = yn < 1.0, yn >= p.ny
+    yn = np.where(low, yn + 2 * (1 - yn), np.where(high, yn + 2 * (p.ny - yn), yn))
+    vn = np.where(low | high, -vn, vn)
+    for k, a in (("x", xn), ("y", yn), ("u", un), ("v", vn), ("xo", x), ("yo", y), ("uo", u0), ("vo", v0)):
+        assert np.allclose(st[k], a, rtol=0, atol=2e-15), k
+    assert np.any(low | high) and np.any((x < 1.0) & heavy) and np.any((x > p.nx - 1) & ~heavy)
+
+
 def test_record_sampling_and_energy_in_uniform_flow():
     U0, T0 = 0.015, 0.25
     p = params(32, 64, noutconfig=3, Tup=T0, Tdown=T0)
