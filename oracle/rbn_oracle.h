@@ -14,13 +14,15 @@
  * CPU path; every arithmetic expression keeps the reference's operand order so that the
  * FP64 results are reproducible bit for bit.  Compile with -ffp-contract=off.
  *
- * Parity pins (tests/test_oracle_*.py): closed-form equilibrium moments, Guo-source
- * invariants, rest-state and conduction fixed points, Poiseuille profile, bilinear
- * exactness, AB2/Maxey–Riley closed forms, the wrap/reflect map, deposit known answers
- * and the nudging recurrence.  The full-step composition, the read_prtcl state machine
- * across record boundaries and the reference-scratch initialisation (glibc rand/tanh) are
- * "parity unpinned": only a run of the unmodified reference could pin them, and it cannot
- * be built here (it needs h5pcc / parallel HDF5).
+ * Parity pins (tests/test_oracle_pins.py): closed-form equilibrium moments, Guo-source
+ * invariants, rest-state and conduction fixed points, bilinear exactness, AB2/Maxey–Riley
+ * closed forms, the wrap/reflect map, deposit known answers, the nudging recurrence, the
+ * observer state machine's known answers across record boundaries and walls, and
+ * independent numpy restatements of a whole truth-run step, a whole nudged step and a whole
+ * particle step.  The reference-scratch initialisation (glibc rand/tanh) is "parity
+ * unpinned": its arithmetic is glibc's, which the library calls the same way on the host,
+ * and the unmodified reference, which could pin it, cannot be built here (it needs h5pcc /
+ * parallel HDF5).
  */
 #ifndef RBN_ORACLE_H
 #define RBN_ORACLE_H
