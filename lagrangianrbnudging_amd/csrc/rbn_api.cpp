@@ -1378,7 +1378,7 @@ int step_loop(rbn_ctx *c, int nsteps) {
              const rbn::PartJob job = carry ? pending_job(c) : rbn::PartJob{};
              return rbn::launch_lbm3(c->stream, c->prec, c->geom, lp, c->F2, c->uv[s2], c->uv[s3],
                                      moments3 ? c->rho : nullptr, moments3 ? c->T : nullptr, c->consts.data(),
-                                     t.seg3, t.xcd2, t.reverse && ((istep / 3) & 1),
+                                     t.seg3, t.xcd2, t.reverse && ((istep / 3) & 1), t.nt3,
                                      carry ? &job : nullptr);
            }
            if (pair) {
@@ -1546,6 +1546,8 @@ int rbn_set_tuning(rbn_ctx *c, const char *key, int value) {
   } else if (!strcmp(key, "lbm3_seg")) {
     if (value != 0 && (value < 4 || value > 4096)) return fail(c, RBN_EINVAL, "lbm3_seg must be 0 or 4..4096");
     c->tune.seg3 = value;
+  } else if (!strcmp(key, "lbm3_nt")) {
+    c->tune.nt3 = value != 0;
   } else if (!strcmp(key, "lbm2_tile_h")) {
     if (value != 0 && value != 4 && value != 8 && value != 16)
       return fail(c, RBN_EINVAL, "lbm2_tile_h must be 0, 4, 8 or 16");
@@ -1585,6 +1587,7 @@ int rbn_get_tuning(const rbn_ctx *c, const char *key, int *value) {
   else if (!strcmp(key, "lbm_reverse")) *value = c->tune.reverse;
   else if (!strcmp(key, "lbm_steps")) *value = c->tune.steps3 ? 3 : (c->tune.steps2 ? 2 : 1);
   else if (!strcmp(key, "lbm3_seg")) *value = c->tune.seg3;
+  else if (!strcmp(key, "lbm3_nt")) *value = c->tune.nt3;
   else if (!strcmp(key, "lbm2_tile_h")) *value = c->tune.tile_h;
   else if (!strcmp(key, "lbm2_seg")) *value = c->tune.seg;
   else if (!strcmp(key, "lbm2_xcd")) *value = c->tune.xcd2;

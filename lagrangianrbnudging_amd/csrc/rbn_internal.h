@@ -204,6 +204,7 @@ struct LbmTuning {
   int steps2 = 1;   // truth run: two steps per lattice launch (temporal blocking in LDS)
   int steps3 = 0;   // truth run: three steps per lattice launch by row sweeps (with steps2)
   int seg3 = 0;     // three-step sweeps: rows per segment (0: one wave of workgroups)
+  int nt3 = 0;      // three-step sweeps: non-temporal stores of the populations and F
   int tile_h = 0;   // rows of a two-step tile (4, 8, 16; 0: 8 in FP32, 4 in FP64)
   int seg = 0;      // two steps by row sweeps over segments of seg rows (8..64); 0: by tiles
   int xcd2 = 1;     // row sweeps: consecutive strips and segments on one XCD
@@ -262,9 +263,11 @@ int lbm2_tile_rows(int prec, int tile_h);
 // step's velocity slot; Fout receives F(s+3), uv2 / uv3 the velocities of steps s+1 / s+2,
 // rho3 / T3 (or null) the moments of step s+2.  seg: rows per segment (0: the fewest that
 // put every workgroup on the device at once); xcd: consecutive strips and segments on one
-// XCD.  job (or null): particle updates carried as extra workgroups ahead of the strips.
+// XCD; nt: non-temporal stores of the populations and F.  job (or null): particle updates
+// carried as extra workgroups ahead of the strips.
 hipError_t launch_lbm3(hipStream_t s, int prec, const Geom &g, const LbmPtrs &p, void *Fout, void *uv2, void *uv3,
-                       void *rho3, void *T3, const void *step_const, int seg, int xcd, int flip, const PartJob *job);
+                       void *rho3, void *T3, const void *step_const, int seg, int xcd, int flip, int nt,
+                       const PartJob *job);
 // Rows per segment launch_lbm3 uses for tuning seg.
 int lbm3_segment_rows(int prec, const Geom &g, int seg);
 // Two steps of K1 in one launch (rbn_lbm2.hip): p holds the first step's inputs and the second

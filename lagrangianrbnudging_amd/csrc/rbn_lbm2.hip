@@ -97,7 +97,7 @@ struct Sweep3Args {
 
 // Three steps per launch (rbn_lbm3_wave.h): strip sx outputs columns [sx nx / strips,
 // (sx + 1) nx / strips), so that every strip has nearly the same number of live threads.
-template <typename R, int SW>
+template <typename R, int SW, int NT>
 __global__ __launch_bounds__(SW) __attribute__((amdgpu_waves_per_eu(3))) void k_lbm3w(Geom g, Sweep3Args a,
                                                                                     StepConst<R> c,
                                                                                     part::PartSteps<R> ps) {
@@ -112,12 +112,12 @@ __global__ __launch_bounds__(SW) __attribute__((amdgpu_waves_per_eu(3))) void k_
   const int sy = w / a.strips, sx = w - sy * a.strips;
   const int ox0 = (int)((long long)sx * g.nx / a.strips), ox1 = (int)((long long)(sx + 1) * g.nx / a.strips);
   const int ja = sy * a.seg;
-  lbm3::three_step_strip<R, SW>(g, a.p, c, ox0, ox1, ja, min(ja + a.seg, g.ny), L);
+  lbm3::three_step_strip<R, SW, NT>(g, a.p, c, ox0, ox1, ja, min(ja + a.seg, g.ny), L);
 }
 
 template <typename R>
 const void *lbm3_kernel() {
-  return reinterpret_cast<const void *>(&k_lbm3w<R, kStripW>);
+  return reinterpret_cast<const void *>(&k_lbm3w<R, kStripW, 1>);
 }
 
 // Workgroups of k_lbm3w that fit on the device at once (resident on every CU), per device
@@ -137,12 +137,15 @@ int lbm3_slots(int prec) {
 }
 
 template <typename R>
-void dispatch_sweep3(hipStream_t s, const Geom &g, const Sweep3Args &a, const void *step_const,
+void dispatch_sweep3(hipStream_t s, const Geom &g, const Sweep3Args &a, int nt, const void *step_const,
                      const PartJob *job) {
   const StepConst<R> &c = *(const StepConst<R> *)step_const;
   const part::PartSteps<R> ps = part::part_steps<R>(g, job, kStripW);
   const dim3 grid((unsigned)(a.nwg + ps.nblocks)), block(kStripW);
-  hipLaunchKernelGGL((k_lbm3w<R, kStripW>), grid, block, 0, s, g, a, c, ps);
+  if (nt)
+    hipLaunchKernelGGL((k_lbm3w<R, kStripW, 1>), grid, block, 0, s, g, a, c, ps);
+  else
+    hipLaunchKernelGGL((k_lbm3w<R, kStripW, 0>), grid, block, 0, s, g, a, c, ps);
 }
 
 template <typename R>
@@ -226,7 +229,8 @@ int lbm3_segment_rows(int prec, const Geom &g, int seg) {
 }
 
 hipError_t launch_lbm3(hipStream_t s, int prec, const Geom &g, const LbmPtrs &p, void *Fout, void *uv2, void *uv3,
-                       void *rho3, void *T3, const void *step_const, int seg, int xcd, int flip, const PartJob *job) {
+                       void *rho3, void *T3, const void *step_const, int seg, int xcd, int flip, int nt,
+                       const PartJob *job) {
   Sweep3Args a{};
   a.p.fA = p.fA;
   a.p.gA = p.gA;
@@ -246,9 +250,9 @@ hipError_t launch_lbm3(hipStream_t s, int prec, const Geom &g, const LbmPtrs &p,
   a.flip = flip;
   a.xcd = xcd;
   if (prec == RBN_FP64)
-    dispatch_sweep3<double>(s, g, a, step_const, job);
+    dispatch_sweep3<double>(s, g, a, nt, step_const, job);
   else
-    dispatch_sweep3<float>(s, g, a, step_const, job);
+    dispatch_sweep3<float>(s, g, a, nt, step_const, job);
   return hipGetLastError();
 }
 

@@ -169,7 +169,7 @@ __device__ __forceinline__ void finish(const StepConst<R> &c, bool bot, bool top
 // one iteration ahead, in flight across the barriers, measured no faster: the sweep is bound
 // by the latency of its dependent arithmetic and barriers at three waves per SIMD, about 52 %
 // VALU-busy, not by memory latency.)
-template <typename R, int SW>
+template <typename R, int SW, int NT>
 __device__ __forceinline__ void three_step_strip(const Geom &g, const ThreeStepPtrs &a, const StepConst<R> &c,
                                                  int ox0, int ox1, int ja, int jb, Lds3<R, SW> &L) {
   const int t = threadIdx.x;
@@ -234,10 +234,10 @@ __device__ __forceinline__ void three_step_strip(const Geom &g, const ThreeStepP
         flag_nonfinite(a.bad, rho, T);
         const size_t o = (size_t)r2 * pitch + x;
 #pragma unroll
-        for (int k = 0; k < 9; k++) st<1>(&fB[(size_t)k * g.plane + o], fo[k]);
+        for (int k = 0; k < 9; k++) st<NT>(&fB[(size_t)k * g.plane + o], fo[k]);
 #pragma unroll
-        for (int k = 0; k < 9; k++) st<1>(&gB[(size_t)k * g.plane + o], go[k]);
-        st<1>(&((R *)a.Fout)[o], Fn);
+        for (int k = 0; k < 9; k++) st<NT>(&gB[(size_t)k * g.plane + o], go[k]);
+        st<NT>(&((R *)a.Fout)[o], Fn);
         st_uv<0, R>(a.uv[2], o, u, v);
         if (a.rho) ((R *)a.rho)[o] = rho;
         if (a.T) ((R *)a.T)[o] = T;

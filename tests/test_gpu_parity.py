@@ -111,7 +111,8 @@ def test_particle_fuse_fp32_matches_separate_launches(writefull):
                  dict(particle_fuse=1, lbm_steps=1, lbm_xcd=2), dict(particle_fuse=1, lbm_steps=2),
                  dict(particle_fuse=1, lbm_steps=2, lbm2_seg=0, lbm2_tile_h=4),
                  dict(particle_fuse=1, lbm_steps=2, lbm2_seg=16, lbm2_xcd=0), dict(particle_fuse=0, lbm_steps=2),
-                 dict(particle_fuse=1, lbm_steps=3), dict(particle_fuse=1, lbm_steps=3, lbm3_seg=7, lbm2_xcd=0), dict(particle_fuse=0, lbm_steps=3)):
+                 dict(particle_fuse=1, lbm_steps=3), dict(particle_fuse=1, lbm_steps=3, lbm3_seg=7, lbm2_xcd=0),
+                 dict(particle_fuse=1, lbm_steps=3, lbm3_nt=1), dict(particle_fuse=0, lbm_steps=3)):
         g = rbn.Solver(p)
         g.set_tuning(**tune)
         g.collect_records()
@@ -176,7 +177,8 @@ def test_three_step_launch_is_three_steps(prec, shape, seg, xcd):
     # Row sweeps three steps deep: a lattice of 8 columns (the strip's threads wrap around it
     # many times), exactly one strip (252 columns), two uneven
     # strips (253), several strips, segments of 4 rows up to one segment for the whole
-    # lattice, segments that do not divide the rows, both workgroup orders.  Energy diagnostics
+    # lattice, segments that do not divide the rows, both workgroup orders (the hardware order
+    # with non-temporal stores, the other with cached ones).  Energy diagnostics
     # every 7 steps (a diagnostics step is never among the first two of a launch), calls of
     # lengths that leave launches of three, two and one step.
     nx, ny = shape
@@ -186,7 +188,7 @@ def test_three_step_launch_is_three_steps(prec, shape, seg, xcd):
     runs = []
     for steps in (1, 3):
         s = rbn.Solver(p)
-        s.set_tuning(lbm_steps=steps, lbm3_seg=seg, lbm2_xcd=xcd)
+        s.set_tuning(lbm_steps=steps, lbm3_seg=seg, lbm2_xcd=xcd, lbm3_nt=1 - xcd)
         s.collect_diagnostics()
         s.init_fields(T=T0)
         for n in (23, 8, 1, 2, 9, 4):
