@@ -472,6 +472,20 @@ inline int fold_blocks(const DepPtrs &d, int max_blocks) {
   return (int)std::max<long long>(1, std::min<long long>((d.active_cap + kBlock - 1) / kBlock, max_blocks));
 }
 
+// Tile pass of entry k of a batched launch's tile map (two tiles per block, one per half, of
+// members M; k < 0: none).  A half is whole wavefronts, so its entry and its member's
+// geometry and deposit descriptors are wavefront-uniform: they are read once, by scalar
+// loads, into registers, instead of field by field after each of dep_tile's barriers.
+__device__ __forceinline__ void member_tile(const NudgeMember *M, const int4 *map, int k, DepTilePairLds &L,
+                                            bool api) {
+  const int ku = __builtin_amdgcn_readfirstlane(k);
+  const int4 w = map[ku < 0 ? 0 : ku];
+  const int m = __builtin_amdgcn_readfirstlane(w.x), t = __builtin_amdgcn_readfirstlane(w.y);
+  const Geom g = M[m].g;
+  const DepPtrs d = M[m].d;
+  dep_tile(g, d, ku < 0 ? -1 : t, L, api);
+}
+
 // The head of a step's batched deposit: the tile passes of the bilinear members' previous
 // deposit when it was deferred (ntail_tiles map entries `tmap` of members `tail`, two per
 // block in the first blocks; they read only their own set's bins and write their own set),
@@ -483,9 +497,7 @@ __global__ __launch_bounds__(kBlock) void k_nb_head(const NudgeMember *M, const 
   __shared__ DepTilePairLds L;
   const int ntail = (ntail_tiles + 1) / 2;  // blocks of the tile passes, two tiles each
   if ((int)blockIdx.x < ntail) {
-    const int k = half_tile(blockIdx.x, ntail_tiles);
-    const int4 w = tmap[k < 0 ? 0 : k];
-    dep_tile(tail[w.x].g, tail[w.x].d, k < 0 ? -1 : w.y, L, false);
+    member_tile(tail, tmap, half_tile(blockIdx.x, ntail_tiles), L, false);
     return;
   }
   const int4 w = hmap[blockIdx.x - ntail];
@@ -545,9 +557,7 @@ __global__ __launch_bounds__(kBlock) void k_nb_fold(const NudgeMember *M, const 
 // The bilinear members' tile passes (api: as dep_tile's).
 __global__ __launch_bounds__(kBlock) void k_nb_tiles(const NudgeMember *M, const int4 *map, int ntiles, int api) {
   __shared__ DepTilePairLds L;
-  const int k = half_tile(blockIdx.x, ntiles);
-  const int4 w = map[k < 0 ? 0 : k];
-  dep_tile(M[w.x].g, M[w.x].d, k < 0 ? -1 : w.y, L, api != 0);
+  member_tile(M, map, half_tile(blockIdx.x, ntiles), L, api != 0);
 }
 
 }  // namespace
