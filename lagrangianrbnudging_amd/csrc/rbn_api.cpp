@@ -656,8 +656,11 @@ int rbn_create(const rbn_params *p, int device, rbn_ctx **out) {
   // FP32 truth runs of 2^21 nodes and more advance three steps per launch by row sweeps
   // (measured with particles: 2048 x 1024 0.0465 against 0.0489 ms per step, 4096 x 2048
   // 0.156 against 0.175, 8192 x 4096 0.663 against 0.700); smaller lattices give the sweep
-  // too few workgroups (1024 x 512: 0.0212 against 0.0138), and FP64 sweeps hold one wave
-  // per SIMD (4096 x 2048: 0.363 against 0.299).
+  // too few workgroups (1024 x 512: 0.0212 against 0.0138).  FP64 keeps two: its sweep
+  // (128-thread strips, 251 VGPRs, 1.5 waves per SIMD) carries three particle updates at a
+  // cost that cancels its gain (with particles 4096 x 2048 0.3024 against 0.3019 ms per step,
+  // 2048 x 1024 0.0791 against 0.0832, 8192 x 4096 1.340 against 1.314; without particles
+  // 4096 x 2048 0.2589 against 0.2789).
   c->tune.steps3 = c->prec == RBN_FP32 && p->variant == RBN_VARIANT_TRAJECTORIES &&
                    (long long)p->nx * p->ny >= (1LL << 21);
   const size_t fb = g.plane * c->rs;

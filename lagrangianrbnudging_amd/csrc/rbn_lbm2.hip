@@ -95,10 +95,16 @@ struct Sweep3Args {
   int strips, seg, nwg, flip, xcd;
 };
 
+// Threads of a three-step strip: 256 in FP32; 128 in FP64, whose LDS rows are twice as wide
+// (49.9 KB at 128: three workgroups per CU, where 256 threads fit one).
+template <typename R>
+constexpr int strip_w3() { return sizeof(R) == 8 ? 128 : kStripW; }
+
 // Three steps per launch (rbn_lbm3_wave.h): strip sx outputs columns [sx nx / strips,
 // (sx + 1) nx / strips), so that every strip has nearly the same number of live threads.
+// FP32 holds three waves per SIMD (as many as the LDS allows), FP64 two.
 template <typename R, int SW, int NT>
-__global__ __launch_bounds__(SW) __attribute__((amdgpu_waves_per_eu(3))) void k_lbm3w(Geom g, Sweep3Args a,
+__global__ __launch_bounds__(SW) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 8 ? 2 : 3))) void k_lbm3w(Geom g, Sweep3Args a,
                                                                                     StepConst<R> c,
                                                                                     part::PartSteps<R> ps) {
   __shared__ lbm3::Lds3<R, SW> L;
@@ -117,7 +123,7 @@ __global__ __launch_bounds__(SW) __attribute__((amdgpu_waves_per_eu(3))) void k_
 
 template <typename R>
 const void *lbm3_kernel() {
-  return reinterpret_cast<const void *>(&k_lbm3w<R, kStripW, 1>);
+  return reinterpret_cast<const void *>(&k_lbm3w<R, strip_w3<R>(), 1>);
 }
 
 // Workgroups of k_lbm3w that fit on the device at once (resident on every CU), per device
@@ -130,7 +136,8 @@ int lbm3_slots(int prec) {
   if (slot > 0) return slot;
   const void *k = prec == RBN_FP64 ? lbm3_kernel<double>() : lbm3_kernel<float>();
   int per_cu = 0, cus = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, kStripW, 0) != hipSuccess) return 0;
+  const int sw = prec == RBN_FP64 ? strip_w3<double>() : strip_w3<float>();
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, sw, 0) != hipSuccess) return 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
   slot = per_cu * cus;
   return slot;
@@ -140,12 +147,13 @@ template <typename R>
 void dispatch_sweep3(hipStream_t s, const Geom &g, const Sweep3Args &a, int nt, const void *step_const,
                      const PartJob *job) {
   const StepConst<R> &c = *(const StepConst<R> *)step_const;
-  const part::PartSteps<R> ps = part::part_steps<R>(g, job, kStripW);
-  const dim3 grid((unsigned)(a.nwg + ps.nblocks)), block(kStripW);
+  constexpr int SW = strip_w3<R>();
+  const part::PartSteps<R> ps = part::part_steps<R>(g, job, SW);
+  const dim3 grid((unsigned)(a.nwg + ps.nblocks)), block(SW);
   if (nt)
-    hipLaunchKernelGGL((k_lbm3w<R, kStripW, 1>), grid, block, 0, s, g, a, c, ps);
+    hipLaunchKernelGGL((k_lbm3w<R, SW, 1>), grid, block, 0, s, g, a, c, ps);
   else
-    hipLaunchKernelGGL((k_lbm3w<R, kStripW, 0>), grid, block, 0, s, g, a, c, ps);
+    hipLaunchKernelGGL((k_lbm3w<R, SW, 0>), grid, block, 0, s, g, a, c, ps);
 }
 
 template <typename R>
@@ -216,14 +224,17 @@ void dispatch(hipStream_t s, int prec, int nt, int th, const Geom &g, const Lbm2
 
 int lbm2_tile_rows(int prec, int tile_h) { return tile_rows_of(prec, tile_h); }
 
-int lbm3_strips(int nx) { return (nx + kStripW - 5) / (kStripW - 4); }
+int lbm3_strips(int prec, int nx) {
+  const int sw = prec == RBN_FP64 ? strip_w3<double>() : strip_w3<float>();
+  return (nx + sw - 5) / (sw - 4);
+}
 
 int lbm3_segment_rows(int prec, const Geom &g, int seg) {
   if (seg > 0) return min(seg, g.ny);
   // 0: the fewest rows per segment that still put every workgroup on the device at once
   // (one wave of workgroups, no tail), at least 16
   const int slots = lbm3_slots(prec);
-  const int strips = lbm3_strips(g.nx);
+  const int strips = lbm3_strips(prec, g.nx);
   const int nseg = slots > 0 ? max(1, slots / strips) : 1;
   return min(g.ny, max(16, (g.ny + nseg - 1) / nseg));
 }
@@ -245,7 +256,7 @@ hipError_t launch_lbm3(hipStream_t s, int prec, const Geom &g, const LbmPtrs &p,
   a.p.T = T3;
   a.p.bad = p.bad;
   a.seg = lbm3_segment_rows(prec, g, seg);
-  a.strips = lbm3_strips(g.nx);
+  a.strips = lbm3_strips(prec, g.nx);
   a.nwg = a.strips * ((g.ny + a.seg - 1) / a.seg);
   a.flip = flip;
   a.xcd = xcd;
