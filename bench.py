@@ -315,7 +315,7 @@ def nudged_solver(rbn, pt, recs, ids, x, y, T_at, T_first, prec, alpha, deposit)
     q.precision = prec
     q.deposit = deposit
     s = rbn.Solver(q, 0)
-    s.set_observation_source(recs)
+    s.set_observation_table(recs)  # held on the device from the first step
     s.init_fields(T=T_first)
     s.init_observers(ids, x, y, T_at)
     return s

@@ -308,7 +308,17 @@ rbn::ObsPtrs obs_view(int nobs, int *ints, double *d) {
   return o;
 }
 
-rbn::ObsPtrs obs_ptrs(rbn_ctx *c) { return obs_view(c->nobs, c->d_obs_i, c->d_obs_d); }
+rbn::ObsPtrs obs_ptrs(rbn_ctx *c) {
+  rbn::ObsPtrs o = obs_view(c->nobs, c->d_obs_i, c->d_obs_d);
+  if (c->tab_rec >= 0) {
+    const size_t n = (size_t)c->nobs, r = (size_t)c->tab_rec;
+    o.rx = c->d_tab + 3 * r * n;
+    o.ry = o.rx + n;
+    o.rT = o.rx + 2 * n;
+    o.chk = c->d_tabmap + r * n;
+  }
+  return o;
+}
 
 int check_err_flag(rbn_ctx *c) {
   int e[2] = {0, 0}, h = 0;
@@ -484,6 +494,112 @@ int load_observation(rbn_ctx *c, int rstep, hipStream_t s_up) {
 
 int load_observation(rbn_ctx *c, int rstep) { return load_observation(c, rstep, c->stream); }
 
+// ---------------------------------------------------------------- observation table on the device
+// rbn_step over an observation table (obs_device): the caller's table is copied to the
+// device at the first step after it or the observers change, and the observer map of every
+// record is computed there, so that a record becomes current by pointer (obs_ptrs): no host
+// copy, no upload and no map per record (each cost about 28 us of host time at 1e4
+// observers, about 5 us per step at records every 5 steps).  Tables over kTabCap bytes,
+// whose entries per record differ from the observers' count, or whose observers have
+// sparse or repeated ids are served record by record as other providers are.
+
+constexpr size_t kTabCap = 8ull << 30;
+
+void free_obs_table(rbn_ctx *c) {
+  dfree_t(c->d_tab);
+  dfree_t(c->d_tabmap);
+  c->tab_miss.clear();
+  c->tab_rec = -1;
+  c->tab_tried = false;
+}
+
+// Build the device table when it suits; false (and nothing built) when it does not.
+int build_obs_table(rbn_ctx *c, bool *built) {
+  *built = false;
+  c->tab_tried = true;
+  const rbn_ctx::ObsTable &t = c->obs_table;
+  const int n = c->nobs;
+  if (n <= 0 || t.n != n || t.nrec <= 0) return RBN_OK;
+  const size_t N = (size_t)n, R = (size_t)t.nrec;
+  if (R * N * (3 * sizeof(double) + sizeof(int)) > kTabCap) return RBN_OK;
+  const int lo = *std::min_element(c->oid.begin(), c->oid.end());
+  const int hi = *std::max_element(c->oid.begin(), c->oid.end());
+  const long long range = (long long)hi - lo + 1;
+  if (range > 4LL * n + 1024) return RBN_OK;
+  std::vector<int> lookup((size_t)range, -1);
+  for (int i = 0; i < n; i++) {
+    int &l = lookup[(size_t)(c->oid[(size_t)i] - lo)];
+    if (l >= 0) return RBN_OK;  // two observers share an id
+    l = i;
+  }
+  void *p = nullptr;
+  int rc;
+  if ((rc = dalloc(c, &p, R * 3 * N * sizeof(double)))) return rc;
+  c->d_tab = (double *)p;
+  if ((rc = dalloc(c, &p, R * N * sizeof(int)))) {
+    free_obs_table(c);
+    c->tab_tried = true;
+    return rc;
+  }
+  c->d_tabmap = (int *)p;
+  int *ids = nullptr, *dl = nullptr, *miss = nullptr;
+  hipError_t e = hipMalloc(&ids, R * N * sizeof(int));
+  if (e == hipSuccess) e = hipMalloc(&dl, (size_t)range * sizeof(int));
+  if (e == hipSuccess) e = hipMalloc(&miss, R * sizeof(int));
+  const size_t w = N * sizeof(double);
+  if (e == hipSuccess) e = hipMemcpy2DAsync(c->d_tab, 3 * w, t.x, w, w, R, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipMemcpy2DAsync(c->d_tab + N, 3 * w, t.y, w, w, R, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipMemcpy2DAsync(c->d_tab + 2 * N, 3 * w, t.T, w, w, R, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(ids, t.id, R * N * sizeof(int), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(dl, lookup.data(), (size_t)range * sizeof(int), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = rbn::launch_obs_table(c->stream, ids, t.nrec, n, dl, lo, (int)range, c->d_tabmap, miss);
+  c->tab_miss.assign(R, 0);
+  if (e == hipSuccess) e = hipMemcpyAsync(c->tab_miss.data(), miss, R * sizeof(int), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  for (void *q : {(void *)ids, (void *)dl, (void *)miss})
+    if (q) (void)hipFree(q);
+  if (e != hipSuccess) {
+    free_obs_table(c);
+    c->tab_tried = true;
+    return fail(c, RBN_EDEVICE, "observation table upload failed: %s", hipGetErrorString(e));
+  }
+  *built = true;
+  return RBN_OK;
+}
+
+// Make record rstep current for rbn_step: from the device table when it holds it, else as
+// load_observation does (which also reports a record the table lacks).
+int load_observation_dev(rbn_ctx *c, int rstep, hipStream_t s_up) {
+  if (c->d_tab) {
+    const rbn_ctx::ObsTable &t = c->obs_table;
+    const int *it = std::lower_bound(t.steps, t.steps + t.nrec, rstep);
+    if (it != t.steps + t.nrec && *it == rstep) {
+      const int r = (int)(it - t.steps);
+      if (c->tab_miss[(size_t)r] < c->nobs)
+        return fail(c, RBN_ENOTFOUND, "observer id %d missing from record %d", c->oid[(size_t)c->tab_miss[(size_t)r]],
+                    rstep);
+      c->tab_rec = r;
+      return RBN_OK;
+    }
+  }
+  c->tab_rec = -1;
+  return load_observation(c, rstep, s_up);
+}
+
+// End of rbn_step: the current record back in the observer arrays' own buffer, where the
+// adjoint, the ensemble and record-by-record loads expect it.
+int settle_records(rbn_ctx *c) {
+  if (c->tab_rec < 0) return RBN_OK;
+  const size_t n = (size_t)c->nobs, r = (size_t)c->tab_rec;
+  const rbn::ObsPtrs o = obs_view(c->nobs, c->d_obs_i, c->d_obs_d);
+  HIPCHK(c, hipMemcpyAsync((void *)o.rx, c->d_tab + 3 * r * n, 3 * n * sizeof(double), hipMemcpyDeviceToDevice,
+                           c->stream));
+  HIPCHK(c, hipMemcpyAsync((void *)o.chk, c->d_tabmap + r * n, n * sizeof(int), hipMemcpyDeviceToDevice, c->stream));
+  c->tab_rec = -1;
+  c->prev_rid.clear();  // the map in the observer arrays is no longer that of the last fetched record
+  return RBN_OK;
+}
+
 int check_sample_args(rbn_ctx *c, const char *who, int field, int n, const double *x, const double *y,
                       const double *out) {
   int rc = check_ctx(c);
@@ -574,6 +690,7 @@ void rbn_destroy(rbn_ctx *c) {
   dfree_t(c->d_err);
   dfree_t(c->d_obs_i);
   dfree_t(c->d_obs_d);
+  free_obs_table(c);
   dfree_t(c->d_energy);
   dfree_t(c->dep.cnt);
   dfree_t(c->dep.fill);
@@ -1010,6 +1127,7 @@ int rbn_init_observers(rbn_ctx *c, int n, const int *id, const double *x, const 
   if (n > (1 << 24)) return fail(c, RBN_ERANGE, "too many observers");
   dfree_t(c->d_obs_i);
   dfree_t(c->d_obs_d);
+  free_obs_table(c);
   dfree_t(c->dep.list);
   dfree_t(c->dep.active);
   dfree_t(c->dep.bin);
@@ -1061,6 +1179,7 @@ int rbn_init_observers(rbn_ctx *c, int n, const int *id, const double *x, const 
 
 int rbn_set_obs_provider(rbn_ctx *c, rbn_obs_fn fn, void *user) {
   if (!c) return rbn::thread_fail(RBN_EINVAL, "null context");
+  free_obs_table(c);
   c->obs_fn = fn;
   c->obs_user = user;
   return RBN_OK;
@@ -1092,6 +1211,7 @@ int rbn_set_obs_table(rbn_ctx *c, int nrec, int n, const int *steps, const int *
     return fail(c, RBN_EINVAL, "rbn_set_obs_table: bad arguments");
   for (int r = 1; r < nrec; r++)
     if (steps[r] <= steps[r - 1]) return fail(c, RBN_EINVAL, "rbn_set_obs_table: record steps must ascend");
+  free_obs_table(c);  // copied to the device again at the next step
   c->obs_table = {nrec, n, steps, id, x, y, T};
   c->obs_fn = table_provider;
   c->obs_user = &c->obs_table;
@@ -1235,7 +1355,7 @@ int nudged_deposit(rbn_ctx *c, int istep, hipStream_t s, const rbn::DepPtrs &d, 
   const int right = P.fix_q3 ? P.nx - 3 : P.nx - 2;
   const int r = istep % P.ntempread;
   int rc;
-  if (r == 1 && (rc = load_observation(c, istep + P.ntempread - 1, s))) return rc;
+  if (r == 1 && (rc = load_observation_dev(c, istep + P.ntempread - 1, s))) return rc;
   const int branch = r == 1 ? 1 : (r == 0 ? 0 : 2);
   const rbn::ObsPtrs o = obs_ptrs(c);
   return timed(c, KT_DEPOSIT, s,
@@ -1278,6 +1398,10 @@ int step_loop(rbn_ctx *c, int nsteps) {
   const bool npairs = nudged && pairs && !dovl && nsteps > 1;
   if (npairs && (rc = ensure_dep2(c))) return rc;
   if (nsteps > 0) c->at_init = false;
+  if (nudged && nsteps > 0 && c->tune.obs_device && c->obs_user == &c->obs_table && !c->tab_tried) {
+    bool built = false;
+    if ((rc = build_obs_table(c, &built))) return rc;
+  }
   // nudged pairs, bilinear deposit: the step whose observer update (and binning into dep2) the
   // previous two-step launch carried, when the next launch is a two-step one too
   int head_done = -1;
@@ -1322,14 +1446,13 @@ int step_loop(rbn_ctx *c, int nsteps) {
       const bool hmoments = (nd > 0 && (hstep % nd) == 0) || n + 1 == nsteps - 1;
       carry_head = !hmoments && n + 2 < nsteps;
     }
-    const rbn::ObsPtrs o3 = nudged ? obs_ptrs(c) : rbn::ObsPtrs{};
     if (nudged && pair) {
       // the first step's bilinear tile pass rides in the second step's first launch
       const bool bil = c->dep.mode == RBN_DEPOSIT_BILINEAR;
       if (head_done != istep && (rc = nudged_deposit(c, istep, c->stream, c->dep2, nullptr, bil))) return rc;
       if ((rc = nudged_deposit(c, istep2, c->stream, c->dep, bil ? &c->dep2 : nullptr, in_tile))) return rc;
       if (carry_head && hstep % P.ntempread == 1 &&
-          (rc = load_observation(c, hstep + P.ntempread - 1, c->stream)))
+          (rc = load_observation_dev(c, hstep + P.ntempread - 1, c->stream)))
         return rc;
     } else if (nudged && !dovl) {
       if (head_done == istep) return fail(c, RBN_ESTATE, "internal: a carried observer update before a single step");
@@ -1342,6 +1465,8 @@ int step_loop(rbn_ctx *c, int nsteps) {
       HIPCHK(c, hipStreamWaitEvent(c->stream, c->ev_dep[n & 1], 0));
     }
     const rbn::DepPtrs &dset = (dovl && (n & 1)) || (nudged && pair) ? c->dep2 : c->dep;
+    // the observers as the carried observer update reads them: after its record's load
+    const rbn::ObsPtrs o3 = nudged ? obs_ptrs(c) : rbn::ObsPtrs{};
     // A launch writes the velocity slots after cur: K1 of step s overwrites the slot of step
     // s-8, a two-step launch also that of s-7, a three-step one those of s-7 and s-6; pending
     // updates read slots s-4 .. s-1.
@@ -1484,6 +1609,10 @@ int rbn_step(rbn_ctx *c, int nsteps) {
   if (nudged && c->nobs == 0 && !c->d_obs_i) return fail(c, RBN_ESTATE, "nudged run without observers");
   const bool parts = !nudged && P.particle && c->np > 0;
   rc = step_loop(c, nsteps);
+  if (nudged) {
+    const int rs = settle_records(c);
+    if (!rc) rc = rs;
+  }
   // The last step's particle update (particle_fuse) has no lattice launch to ride in.
   const int rf = flush_particles(c);
   if (rc) {
@@ -1567,6 +1696,9 @@ int rbn_set_tuning(rbn_ctx *c, const char *key, int value) {
     c->tune.dep_overlap = value != 0;
   } else if (!strcmp(key, "dep_in_tile")) {
     c->tune.dep_in_tile = value != 0;
+  } else if (!strcmp(key, "obs_device")) {
+    c->tune.obs_device = value != 0;
+    if (!value) free_obs_table(c);
   } else if (!strcmp(key, "particle_overlap")) {
     c->tune.overlap = value != 0;
   } else if (!strcmp(key, "particle_fuse")) {
@@ -1597,6 +1729,7 @@ int rbn_get_tuning(const rbn_ctx *c, const char *key, int *value) {
   else if (!strcmp(key, "lbm2_variant")) *value = c->tune.variant;
   else if (!strcmp(key, "dep_overlap")) *value = c->tune.dep_overlap;
   else if (!strcmp(key, "dep_in_tile")) *value = c->tune.dep_in_tile;
+  else if (!strcmp(key, "obs_device")) *value = c->tune.obs_device;
   else if (!strcmp(key, "particle_overlap")) *value = c->tune.overlap;
   else if (!strcmp(key, "particle_fuse")) *value = c->tune.fuse;
   else if (!strcmp(key, "adj_store_deposit")) *value = c->adj_store_deposit;

@@ -211,6 +211,7 @@ struct LbmTuning {
   int variant = 0;  // row sweeps: 1 pulls one row ahead
   int dep_overlap = 0;  // nudged run: observer update and deposit of step s+1 beside the lattice step s
   int dep_in_tile = 1;  // nudged two-step launches: the second step's bilinear deposit summed in the tiles
+  int obs_device = 1;   // nudged run over an observation table: the table held on the device
 };
 
 // The particle updates of one to three consecutive steps, in step order, carried by the next
@@ -298,6 +299,12 @@ hipError_t launch_particle_init(hipStream_t s, int prec, const Geom &g, const vo
 // the first launch (a 7x7 set is never one: it has no bins, and is ignored); defer: leave
 // this deposit's tile pass (bilinear; a 7x7 deposit always completes) to a later call's
 // tail, which must come before anything reads the deposit.
+// An observation table on the device (nrec records of n entries): map[r n + i] = the first
+// entry of record r whose id (ids[r n + j]) is observer i's (lookup[id - lo] = i for ids in
+// lo .. lo + range - 1, -1 for no observer), INT_MAX when none; first_missing[r] = the first
+// observer record r misses, n when none.
+hipError_t launch_obs_table(hipStream_t s, const int *ids, int nrec, int n, const int *lookup, int lo, int range,
+                            int *map, int *first_missing);
 hipError_t launch_obs_deposit(hipStream_t s, int branch, int ntempread, const Geom &g, int right_limit,
                               const ObsPtrs &o, const DepPtrs &d, const DepPtrs *tail = nullptr,
                               bool defer = false);
@@ -426,6 +433,16 @@ struct rbn_ctx {
   } obs_slot[2];
   int obs_next = 0;
   std::vector<int> prev_rid;  // ids of the record whose observer map is on the device
+  // rbn_step over an observation table held on the device (obs_device): x, y, T of record r
+  // at d_tab + 3 r nobs (the layout of rx, ry, rT), its observer map at d_tabmap + r nobs,
+  // tab_miss[r] its first missing observer (nobs when none); tab_rec: the table record now
+  // current, or -1 (the current record is in the observer arrays' own rx, ry, rT, chk);
+  // tab_tried: built or found unsuitable since the table or the observers last changed.
+  double *d_tab = nullptr;
+  int *d_tabmap = nullptr;
+  std::vector<int> tab_miss;
+  int tab_rec = -1;
+  bool tab_tried = false;
   rbn::DepPtrs dep{};
 
   // diagnostics

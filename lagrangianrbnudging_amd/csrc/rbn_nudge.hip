@@ -41,6 +41,7 @@
 // counter and a 16-byte store each, where per-node atomics on the sums would each move a
 // line of HBM.
 #include <algorithm>
+#include <climits>
 
 #include "rbn_internal.h"
 #include "rbn_nudge_node.h"
@@ -346,6 +347,44 @@ __global__ __launch_bounds__(kBlock) void k_obs_zero(Geom g, ObsPtrs o, int bran
   }
 }
 
+// An observation table on the device: the observer map of every record (map[r n + i] = the
+// first entry of record r with observer i's id, INT_MAX when none) from the records' ids and
+// a lookup of the observers' ids (dense range lo .. lo + range - 1, -1 where no observer).
+__global__ __launch_bounds__(kBlock) void k_tab_fill(int *map, size_t count) {
+  for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < count; e += (size_t)gridDim.x * kBlock)
+    map[e] = INT_MAX;
+}
+
+__global__ __launch_bounds__(kBlock) void k_tab_map(const int *ids, size_t count, int n, const int *lookup, int lo,
+                                                   int range, int *map) {
+  for (size_t e = (size_t)blockIdx.x * kBlock + threadIdx.x; e < count; e += (size_t)gridDim.x * kBlock) {
+    const long long k = (long long)ids[e] - lo;
+    const int i = k >= 0 && k < range ? lookup[k] : -1;
+    if (i < 0) continue;
+    const size_t r = e / (size_t)n;
+    atomicMin(&map[r * (size_t)n + i], (int)(e - r * (size_t)n));
+  }
+}
+
+// Record blockIdx.x: the first observer its map misses (n when none).
+__global__ __launch_bounds__(kBlock) void k_tab_check(const int *map, int n, int *first_missing) {
+  __shared__ int s[kBlock];
+  const int *m = map + (size_t)blockIdx.x * n;
+  int first = n;
+  for (int i = threadIdx.x; i < n; i += kBlock)
+    if (m[i] == INT_MAX) {
+      first = i;
+      break;
+    }
+  s[threadIdx.x] = first;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] = min(s[threadIdx.x], s[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) first_missing[blockIdx.x] = s[0];
+}
+
 __global__ __launch_bounds__(kBlock) void k_dep_tiles(Geom g, DepPtrs d) {
   __shared__ DepTilePairLds L;
   dep_tile(g, d, half_tile(blockIdx.x, dep_tiles(g)), L, true);
@@ -627,6 +666,17 @@ hipError_t launch_nudge_tiles(hipStream_t s, const NudgeMember *members, const N
   if (plan.nm <= 0 || plan.tot[NB_TILES] <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_nb_tiles, dim3((plan.tot[NB_TILES] + 1) / 2), dim3(kBlock), 0, s, members,
                      plan.map + plan.off[NB_TILES], plan.tot[NB_TILES], api ? 1 : 0);
+  return hipGetLastError();
+}
+
+hipError_t launch_obs_table(hipStream_t s, const int *ids, int nrec, int n, const int *lookup, int lo, int range,
+                            int *map, int *first_missing) {
+  if (nrec <= 0 || n <= 0) return hipSuccess;
+  const size_t count = (size_t)nrec * (size_t)n;
+  const unsigned grid = (unsigned)std::min<size_t>((count + kBlock - 1) / kBlock, 16384);
+  hipLaunchKernelGGL(k_tab_fill, dim3(grid), dim3(kBlock), 0, s, map, count);
+  hipLaunchKernelGGL(k_tab_map, dim3(grid), dim3(kBlock), 0, s, ids, count, n, lookup, lo, range, map);
+  hipLaunchKernelGGL(k_tab_check, dim3((unsigned)nrec), dim3(kBlock), 0, s, map, n, first_missing);
   return hipGetLastError();
 }
 

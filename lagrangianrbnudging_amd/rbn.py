@@ -450,7 +450,7 @@ class Solver:
     def tuning(self) -> dict:
         out = {}
         for k in ("lbm_block", "lbm_npt", "lbm_nt", "lbm_ntload", "lbm_uv_keep", "lbm_xcd", "lbm_reverse", "lbm_steps", "lbm3_seg", "lbm3_nt", "lbm2_seg", "lbm2_xcd", "lbm2_variant", "lbm2_tile_h", "particle_overlap", "particle_fuse", "dep_overlap",
-                  "dep_in_tile", "adj_store_deposit"):
+                  "dep_in_tile", "obs_device", "adj_store_deposit"):
             v = C.c_int()
             self._c(self.lib.rbn_get_tuning(self.ctx, k.encode(), C.byref(v)))
             out[k] = v.value

@@ -289,6 +289,21 @@ def test_observation_table_errors():
     with pytest.raises(rbn.RbnError) as e:
         h.step(1)
     assert e.value.code == -10  # RBN_ECALLBACK
+    # an observer missing from a record fails the step that reads that record, with the table
+    # on the device (the default) or record by record
+    bad = {k: v.copy() for k, v in rec.items()}
+    bad["id"] = np.array([0, 0], np.int32)
+    for dev in (1, 0):
+        m = rbn.Solver(pn)
+        m.set_tuning(obs_device=dev)
+        m.init_fields(T=conduction_T(pn, 1e-2, 1))
+        m.init_observers(ids, np.array([3.0, 5.0]), np.array([3.0, 5.0]), np.array([0.2, 0.4]))
+        m.set_observation_table({4: rec, 8: bad})
+        m.step(4)  # reads record 4 only
+        with pytest.raises(rbn.RbnError) as e:
+            m.step(1)  # step 5 needs record 8
+        assert e.value.code == -9 and "observer id 1 missing from record 8" in str(e.value)  # RBN_ENOTFOUND
+        m.close()
 
 
 @pytest.mark.parametrize("prec", [rbn.RBN_FP64, rbn.RBN_FP32])
@@ -341,5 +356,67 @@ def test_two_step_nudged_launches_equal_one_step_launches(prec, deposit):
             x, y = a.nudging(), b.nudging()
             for k in ("nudge_vol", "tempreadint", "nud_term", "weight"):
                 assert_equal_arrays(x[k], y[k], f"after a call of {n}: {k}")
+    for s in sims:
+        s.close()
+
+
+def _nudged_states_equal(a, b, what):
+    fa, ga = a.populations()
+    fb, gb = b.populations()
+    assert_equal_arrays(fa, fb, f"{what}: f")
+    assert_equal_arrays(ga, gb, f"{what}: g")
+    x, y = a.fields(), b.fields()
+    for k in ("rho", "T", "u", "v"):
+        assert_equal_arrays(x[k], y[k], f"{what}: {k}")
+    x, y = a.nudging(), b.nudging()
+    for k in ("nudge_vol", "tempreadint", "nud_term", "weight"):
+        assert_equal_arrays(x[k], y[k], f"{what}: {k}")
+
+
+@pytest.mark.parametrize("prec", [rbn.RBN_FP64, rbn.RBN_FP32])
+@pytest.mark.parametrize("tune", [dict(lbm_steps=2), dict(lbm_steps=1), dict(lbm_steps=1, dep_overlap=1)])
+def test_observation_table_on_the_device_equals_records_fetched_when_needed(prec, tune):
+    # With an observation table on the device (obs_device) a record becomes current by
+    # pointer, its observer map computed for every record when the table is uploaded: bit for
+    # bit the run that fetches, maps and uploads each record on the step that reads it.  Every
+    # record permutes the observers (a new map each time); calls end inside record intervals
+    # and on their edges, after which the
+    # getters and the next call find the record where a plain load leaves it; the table is
+    # replaced between calls by one holding the same records; the last call reaches the end
+    # of the table.  FP64 also against the oracle.
+    p = rb_params(70, 25, 2e5, noutconfig=4)
+    p.variant = rbn.RBN_VARIANT_NUDGING
+    p.ntempread = 4
+    p.alpha = 0.3
+    p.deposit = rbn.RBN_DEPOSIT_BILINEAR
+    p.precision = prec
+    ids, x0, y0, T, recs = synthetic_observations(p.nx, p.ny, 60, 84, 4, seed=38)
+    T0 = conduction_T(p, 1e-2, seed=39)
+    sims = []
+    for dev in (0, 1):
+        s = rbn.Solver(p)
+        s.set_tuning(obs_device=dev, **tune)
+        assert s.tuning()["obs_device"] == dev
+        s.init_fields(T=T0)
+        s.init_observers(ids, x0, y0, T)
+        s.set_observation_table(recs)
+        sims.append(s)
+    o = None
+    if prec == rbn.RBN_FP64:
+        o = Oracle(p)
+        o.init_fields(T=T0)
+        o.init_observers(ids, x0, y0, T)
+        o.set_observation_source(recs)
+    for i, n in enumerate((1, 3, 4, 9, 6, 2, 13, 7, 30, 8)):  # 83 steps: the table's records up to 84
+        if i == 4:
+            for s in sims:
+                s.set_observation_table({k: dict(v) for k, v in recs.items()})
+        for s in sims:
+            s.step(n)
+        _nudged_states_equal(sims[1], sims[0], f"after call {i} of {n} steps")
+        if o is not None:
+            o.step(n)
+    if o is not None:
+        _compare(sims[1], o)
     for s in sims:
         s.close()
