@@ -351,14 +351,17 @@ def run_nudging_sweep(rbn, steps, spinup):
         for alpha in NUD_SWEEP_ALPHA:
             s = nudged_solver(rbn, pt, sub, ids[:n], x[:n], y[:n], T_at[:n], T_first, rbn.RBN_FP32, alpha,
                               rbn.RBN_DEPOSIT_BILINEAR)
+            # the first record interval untimed (it uploads the observation table), the rest
+            # timed: `steps` steps in all
+            s.step(NUD_EVERY)
             s.synchronize()
             t0 = time.perf_counter()
-            s.step(steps)
+            s.step(steps - NUD_EVERY)
             s.synchronize()
             dt = time.perf_counter() - t0
             row.append(float(f"{np.sqrt(np.mean((s.fields()['T'] - T_end) ** 2)):.4g}"))
             if alpha == NUD_ALPHA:
-                ms.append(round(1e3 * dt / steps, 4))
+                ms.append(round(1e3 * dt / (steps - NUD_EVERY), 4))
             s.close()
         l2.append(row)
     return {"steps": steps, "n_obs": list(NUD_SWEEP_OBS), "alpha": list(NUD_SWEEP_ALPHA),
