@@ -643,8 +643,8 @@ MIN_ENS_STEPS = 300
 def launch_min_bytes(precision, record_fraction, particles=True, steps=2):
     """The fewest HBM bytes a lattice launch of `steps` steps can move: every population and
     the force read once (19 reals per node) and written once (19), the velocity of every step
-    written (2 reals each), rho and T written on the launches whose last step writes the
-    moments (2 reals, on steps * record_fraction of the launches), plus the carried particle
+    written (2 reals each), rho and T written on the launches holding a record step (2 reals,
+    on steps * record_fraction of the launches), plus the carried particle
     updates' algorithmic bytes (one per step)."""
     s = 8 if precision == 0 else 4
     b = NX * NY * s * (38 + 2 * steps + 2 * steps * record_fraction)
@@ -775,7 +775,7 @@ def main() -> int:
         # the dominant launch: three steps (where record steps allow) or two; record steps end
         # launches, so three-step launches never write the moments at 20-step records
         spl = 3 if r32["lbm3_timed"] > 0 else 2
-        rf = 0.0 if spl == 3 else r32["record_fraction"]
+        rf = r32["record_fraction"]
         bytes_launch = launch_bytes(1, rf, True, steps=spl)
         min_bytes = launch_min_bytes(1, rf, steps=spl)
         avg = r32[f"lbm{spl}_avg_ms"]

@@ -351,10 +351,12 @@ int rbn_adj_backward(rbn_ctx *ctx, double *dJ_dT0, double *dJ_dalpha);
  *                particles only under particle_fuse; nudged runs without dep_overlap, the
  *                observer updates and deposits of both steps run before the launch (the
  *                deposits depend on the records only) (default 2); 1: one step per launch;
- *                3 (truth runs): three steps per launch by row sweeps wherever the first two
- *                write no moments (the first two steps' sideways-moving populations passed
- *                through LDS, strips of 256 threads overlapping by four columns), else two
- *                or one as with 2 (default 3 for FP32 truth runs of at least 2^21 nodes)
+ *                3 (truth runs): three steps per launch by row sweeps wherever at most one
+ *                of the three writes moments and neither of the first two is a diagnostics
+ *                step, so that a record step may be any of them (the first two steps'
+ *                sideways-moving populations passed through LDS, strips of 256 threads, 128
+ *                in FP64, overlapping by four columns), else two or one as with 2 (default 3
+ *                for FP32 truth runs of at least 2^21 nodes)
  *   "lbm3_seg"   rows per segment of a three-step sweep, 4..4096; 0 (default): the fewest
  *                rows that put every workgroup of the launch on the device at once, at least 16
  *   "lbm3_nt"    1: non-temporal stores of a three-step sweep's populations and force;

@@ -1381,8 +1381,9 @@ int step_loop(rbn_ctx *c, int nsteps) {
   // lbm_steps 2: two truth-run steps per lattice launch, the first of which writes no
   // moments; with particles only when their updates ride in lattice launches.
   const bool pairs = c->tune.steps2 && (!parts || fuse);
-  // lbm_steps 3: three truth-run steps per launch (row sweeps) where the first two write no
-  // moments; a launch of two or one step elsewhere.
+  // lbm_steps 3: three truth-run steps per launch (row sweeps) wherever at most one of them
+  // writes moments and neither of the first two is a diagnostics step; a launch of two or one
+  // step elsewhere.
   const bool triples = pairs && c->tune.steps3 && !nudged;
   hipStream_t ks = overlap ? c->pstream : c->stream;
   // dep_overlap: the observer update and deposit of step s+1 run on pstream (unused by nudged
@@ -1410,7 +1411,18 @@ int step_loop(rbn_ctx *c, int nsteps) {
     const bool record = parts && (istep % P.noutconfig) == 0;
     const bool diag = nd > 0 && (istep % nd) == 0;
     const bool moments = record || diag || n == nsteps - 1;
-    const bool pair = pairs && (!nudged || npairs) && !moments && n + 1 < nsteps;
+    // a three-step launch: at most one of its steps writes moments (the sweep writes those of
+    // that step's level), so a record step may be any of the three; a diagnostics step (the
+    // energy is taken from the latest velocity) or the call's last step only the third
+    bool tri = false;
+    if (triples && n + 2 < nsteps) {
+      const int i2 = istep + 1, i3 = istep + 2;
+      const bool d2 = nd > 0 && (i2 % nd) == 0;
+      const bool m2 = (parts && (i2 % P.noutconfig) == 0) || d2;
+      const bool m3 = (parts && (i3 % P.noutconfig) == 0) || (nd > 0 && (i3 % nd) == 0) || n + 2 == nsteps - 1;
+      tri = !diag && !d2 && (int)moments + (int)m2 + (int)m3 <= 1;
+    }
+    const bool pair = tri || (pairs && (!nudged || npairs) && !moments && n + 1 < nsteps);
     int istep2 = 0;
     bool record2 = false, diag2 = false, moments2 = false;
     if (pair) {
@@ -1420,8 +1432,7 @@ int step_loop(rbn_ctx *c, int nsteps) {
       diag2 = nd > 0 && (istep2 % nd) == 0;
       moments2 = record2 || diag2 || n == nsteps - 1;
     }
-    // a three-step launch: a pair whose second step writes no moments, and a third step
-    const bool triple = triples && pair && !moments2;
+    const bool triple = tri;
     int istep3 = 0;
     bool record3 = false, diag3 = false, moments3 = false;
     if (triple) {
@@ -1431,7 +1442,9 @@ int step_loop(rbn_ctx *c, int nsteps) {
       diag3 = nd > 0 && (istep3 % nd) == 0;
       moments3 = record3 || diag3 || n == nsteps - 1;
     }
-    const bool writes_moments = triple ? moments3 : (pair ? moments2 : moments);
+    // the level of a three-step launch whose moments it writes (-1: none)
+    const int mlev = !triple ? -1 : (moments ? 0 : (moments2 ? 1 : (moments3 ? 2 : -1)));
+    const bool writes_moments = triple ? mlev >= 0 : (pair ? moments2 : moments);
     // bilinear deposit: the second step's deposit summed inside the two-step launch from its
     // bins, except in the call's last launch (its deposit fills the arrays the getters read)
     const bool in_tile = nudged && pair && c->dep.mode == RBN_DEPOSIT_BILINEAR && c->tune.dep_in_tile &&
@@ -1505,7 +1518,7 @@ int step_loop(rbn_ctx *c, int nsteps) {
            if (triple) {
              const rbn::PartJob job = carry ? pending_job(c) : rbn::PartJob{};
              return rbn::launch_lbm3(c->stream, c->prec, c->geom, lp, c->F2, c->uv[s2], c->uv[s3],
-                                     moments3 ? c->rho : nullptr, moments3 ? c->T : nullptr, c->consts.data(),
+                                     mlev >= 0 ? c->rho : nullptr, mlev >= 0 ? c->T : nullptr, mlev, c->consts.data(),
                                      t.seg3, t.xcd2, t.reverse && ((istep / 3) & 1), t.nt3,
                                      carry ? &job : nullptr);
            }
@@ -1561,17 +1574,23 @@ int step_loop(rbn_ctx *c, int nsteps) {
     if (pair && nudged) std::swap(c->nud, c->nud2);
     const int prv = c->cur;
     c->cur = triple ? s3 : (pair ? s2 : s1);
-    c->moments_valid = writes_moments;
+    // rho and T hold the latest step's moments (a three-step launch may write an earlier step's)
+    c->moments_valid = triple ? mlev == 2 : writes_moments;
     if (fuse) {
-      // this launch's particle updates wait for the next launch (only its last step can be a
-      // record step)
+      // this launch's particle updates wait for the next launch; a record step (the last step
+      // of a one- or two-step launch, any step of a three-step one) writes its record then,
+      // interpolating T of that step (no later step of the launch writes T)
+      auto pend = [&](int st, int cur_slot, int prv_slot, bool rec) -> int {
+        rbn_ctx::RecSlot *slot = nullptr;
+        if (rec && (rc = claim_record_slot(c, &slot))) return rc;
+        add_pending(c, st, cur_slot, prv_slot, rec, rec && P.prtcl_writefull, slot);
+        return RBN_OK;
+      };
+      if (pair && (rc = pend(istep, s1, prv, record))) return rc;
+      if (triple && (rc = pend(istep2, s2, s1, record2))) return rc;
       const int rstep = triple ? istep3 : (pair ? istep2 : istep);
       const bool rec = triple ? record3 : (pair ? record2 : record);
-      rbn_ctx::RecSlot *slot = nullptr;
-      if (rec && (rc = claim_record_slot(c, &slot))) return rc;
-      if (pair) add_pending(c, istep, s1, prv, false, false, nullptr);
-      if (triple) add_pending(c, istep2, s2, s1, false, false, nullptr);
-      add_pending(c, rstep, c->cur, triple ? s2 : (pair ? s1 : prv), rec, rec && P.prtcl_writefull, slot);
+      if ((rc = pend(rstep, c->cur, triple ? s2 : (pair ? s1 : prv), rec))) return rc;
     } else if (parts) {
       // K2 of step s; with overlap, on the particle stream alongside K1 of step s+1.
       if (overlap) {

@@ -262,13 +262,13 @@ int lbm2_tile_rows(int prec, int tile_h);
 // Three steps of K1 in one launch by row sweeps (rbn_lbm2.hip, rbn_lbm3_wave.h): p holds the
 // first step's inputs and the third step's outputs (fB, gB), p.F = F(s), p.uv = the first
 // step's velocity slot; Fout receives F(s+3), uv2 / uv3 the velocities of steps s+1 / s+2,
-// rho3 / T3 (or null) the moments of step s+2.  seg: rows per segment (0: the fewest that
+// rho3 / T3 (or null) the moments of step s+mlev (mlev 0, 1 or 2).  seg: rows per segment (0: the fewest that
 // put every workgroup on the device at once); xcd: consecutive strips and segments on one
 // XCD; nt: non-temporal stores of the populations and F.  job (or null): particle updates
 // carried as extra workgroups ahead of the strips.
 hipError_t launch_lbm3(hipStream_t s, int prec, const Geom &g, const LbmPtrs &p, void *Fout, void *uv2, void *uv3,
-                       void *rho3, void *T3, const void *step_const, int seg, int xcd, int flip, int nt,
-                       const PartJob *job);
+                       void *rho3, void *T3, int mlev, const void *step_const, int seg, int xcd, int flip,
+                       int nt, const PartJob *job);
 // Rows per segment launch_lbm3 uses for tuning seg.
 int lbm3_segment_rows(int prec, const Geom &g, int seg);
 // Two steps of K1 in one launch (rbn_lbm2.hip): p holds the first step's inputs and the second

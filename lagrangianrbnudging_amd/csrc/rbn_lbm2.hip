@@ -240,8 +240,8 @@ int lbm3_segment_rows(int prec, const Geom &g, int seg) {
 }
 
 hipError_t launch_lbm3(hipStream_t s, int prec, const Geom &g, const LbmPtrs &p, void *Fout, void *uv2, void *uv3,
-                       void *rho3, void *T3, const void *step_const, int seg, int xcd, int flip, int nt,
-                       const PartJob *job) {
+                       void *rho3, void *T3, int mlev, const void *step_const, int seg, int xcd, int flip,
+                       int nt, const PartJob *job) {
   Sweep3Args a{};
   a.p.fA = p.fA;
   a.p.gA = p.gA;
@@ -254,6 +254,7 @@ hipError_t launch_lbm3(hipStream_t s, int prec, const Geom &g, const LbmPtrs &p,
   a.p.uv[2] = uv3;
   a.p.rho = rho3;
   a.p.T = T3;
+  a.p.mlev = rho3 && T3 ? mlev : -1;
   a.p.bad = p.bad;
   a.seg = lbm3_segment_rows(prec, g, seg);
   a.strips = lbm3_strips(prec, g.nx);

@@ -28,8 +28,9 @@ namespace lbm3 {
 using namespace lbm2;
 
 // The planes of a three-step launch: step s reads fA, gA and F(s); step s+2 writes fB, gB
-// (step s+3's inputs) and F(s+3); uv[u] receives the velocity of step s+u, rho / T (or null)
-// the moments of step s+2; bad (or null) the non-finite flag of step s+2's moments.
+// (step s+3's inputs) and F(s+3); uv[u] receives the velocity of step s+u, rho / T the
+// moments of step s+mlev (mlev 0, 1 or 2; -1: none); bad (or null) the non-finite flag of
+// step s+2's moments.
 struct ThreeStepPtrs {
   const void *fA, *gA;
   void *fB, *gB;
@@ -37,6 +38,7 @@ struct ThreeStepPtrs {
   void *Fout;
   void *uv[3];
   void *rho, *T;
+  int mlev;
   int *bad;
 };
 
@@ -205,7 +207,13 @@ __device__ __forceinline__ void three_step_strip(const Geom &g, const ThreeStepP
       R u, v, rho, T;
       finish(c, r0 == 0, r0 == g.ny - 1, cur.q);
       collide(c, cur.f, cur.q, cur.Fy, C0.f, C0.g, rho, T, u, v, C0.F);
-      if (out && r0 >= ja && r0 < jb) st_uv<0, R>(a.uv[0], (size_t)r0 * pitch + x, u, v);
+      if (out && r0 >= ja && r0 < jb) {
+        st_uv<0, R>(a.uv[0], (size_t)r0 * pitch + x, u, v);
+        if (a.mlev == 0) {
+          ((R *)a.rho)[(size_t)r0 * pitch + x] = rho;
+          ((R *)a.T)[(size_t)r0 * pitch + x] = T;
+        }
+      }
     } else {
       row_zero(C0);
     }
@@ -218,7 +226,13 @@ __device__ __forceinline__ void three_step_strip(const Geom &g, const ThreeStepP
       R f[9], q[9], u, v, rho, T;
       gather_truth<R>(acc, r1 == 0, r1 == g.ny - 1, c, f, q);
       collide(c, f, q, P0.F, C1.f, C1.g, rho, T, u, v, C1.F);
-      if (out && r1 >= ja && r1 < jb) st_uv<0, R>(a.uv[1], (size_t)r1 * pitch + x, u, v);
+      if (out && r1 >= ja && r1 < jb) {
+        st_uv<0, R>(a.uv[1], (size_t)r1 * pitch + x, u, v);
+        if (a.mlev == 1) {
+          ((R *)a.rho)[(size_t)r1 * pitch + x] = rho;
+          ((R *)a.T)[(size_t)r1 * pitch + x] = T;
+        }
+      }
     } else {
       row_zero(C1);
     }
@@ -239,8 +253,10 @@ __device__ __forceinline__ void three_step_strip(const Geom &g, const ThreeStepP
         for (int k = 0; k < 9; k++) st<NT>(&gB[(size_t)k * g.plane + o], go[k]);
         st<NT>(&((R *)a.Fout)[o], Fn);
         st_uv<0, R>(a.uv[2], o, u, v);
-        if (a.rho) ((R *)a.rho)[o] = rho;
-        if (a.T) ((R *)a.T)[o] = T;
+        if (a.mlev == 2) {
+          ((R *)a.rho)[o] = rho;
+          ((R *)a.T)[o] = T;
+        }
       }
     }
     m3 = m3 == 2 ? 0 : m3 + 1;
