@@ -109,7 +109,11 @@ __global__ __launch_bounds__(SW) __attribute__((amdgpu_waves_per_eu(sizeof(R) ==
                                                                                     part::PartSteps<R> ps) {
   __shared__ lbm3::Lds3<R, SW> L;
   if ((int)blockIdx.x < ps.nblocks) {
-    part::run_part_steps(ps, blockIdx.x * SW + threadIdx.x);
+    // 256 particles per workgroup whatever the strip width (FP64: two per thread), so that
+    // the particle workgroups take as many slots as with FP32's 256-thread strips
+    constexpr int PPT = 256 / SW;
+#pragma unroll
+    for (int k = 0; k < PPT; k++) part::run_part_steps(ps, ((int)blockIdx.x * PPT + k) * SW + threadIdx.x);
     return;
   }
   int b = (int)blockIdx.x - ps.nblocks;
@@ -148,7 +152,7 @@ void dispatch_sweep3(hipStream_t s, const Geom &g, const Sweep3Args &a, int nt, 
                      const PartJob *job) {
   const StepConst<R> &c = *(const StepConst<R> *)step_const;
   constexpr int SW = strip_w3<R>();
-  const part::PartSteps<R> ps = part::part_steps<R>(g, job, SW);
+  const part::PartSteps<R> ps = part::part_steps<R>(g, job, 256);
   const dim3 grid((unsigned)(a.nwg + ps.nblocks)), block(SW);
   if (nt)
     hipLaunchKernelGGL((k_lbm3w<R, SW, 1>), grid, block, 0, s, g, a, c, ps);
