@@ -525,21 +525,20 @@ __device__ __forceinline__ void member_tile(const NudgeMember *M, const int4 *ma
   dep_tile(g, d, ku < 0 ? -1 : t, L, api);
 }
 
-// The head of a step's batched deposit: the tile passes of the bilinear members' previous
-// deposit when it was deferred (ntail_tiles map entries `tmap` of members `tail`, two per
-// block in the first blocks; they read only their own set's bins and write their own set),
-// then every member's
-// observer update (bilinear members bin their contributions) and the 7x7 members' zero
-// passes (map entries `hmap`, pass NB_OBS or NB_ZERO).
+// The head of a step's batched deposit: every member's observer update (bilinear members
+// bin their contributions) and the 7x7 members' zero passes (nhead map entries `hmap`, pass
+// NB_OBS or NB_ZERO) in the first blocks, then the tile passes of the bilinear members'
+// previous deposit when it was deferred (ntail_tiles map entries `tmap` of members `tail`,
+// two per block; they read only their own set's bins and write their own set).  The
+// observer blocks go first: they wait on atomics, and the tile passes fill in around them.
 __global__ __launch_bounds__(kBlock) void k_nb_head(const NudgeMember *M, const NudgeMember *tail, const int4 *tmap,
-                                                    int ntail_tiles, const int4 *hmap, int t) {
+                                                    int ntail_tiles, const int4 *hmap, int nhead, int t) {
   __shared__ DepTilePairLds L;
-  const int ntail = (ntail_tiles + 1) / 2;  // blocks of the tile passes, two tiles each
-  if ((int)blockIdx.x < ntail) {
-    member_tile(tail, tmap, half_tile(blockIdx.x, ntail_tiles), L, false);
+  if ((int)blockIdx.x >= nhead) {
+    member_tile(tail, tmap, half_tile(blockIdx.x - nhead, ntail_tiles), L, false);
     return;
   }
-  const int4 w = hmap[blockIdx.x - ntail];
+  const int4 w = hmap[blockIdx.x];
   const NudgeMember &d = M[w.x];
   if (w.w == NB_OBS) {
     // the step's place in the record interval as the single-member loop derives it
@@ -647,8 +646,7 @@ hipError_t launch_nudge_batch(hipStream_t s, const NudgeMember *members, const N
   const int ntail = tail ? tot[NB_TILES] : 0, nhead = tot[NB_OBS] + tot[NB_ZERO];
   if (ntail + nhead > 0)
     hipLaunchKernelGGL(k_nb_head, dim3((ntail + 1) / 2 + nhead), b, 0, s, members, tail ? tail : members, at(NB_TILES),
-                       ntail,
-                       at(NB_OBS), t);
+                       ntail, at(NB_OBS), nhead, t);
   if (tot[NB_CAND] > 0) hipLaunchKernelGGL(k_nb_count, dim3(tot[NB_CAND]), b, 0, s, members, at(NB_CAND));
   if (tot[NB_SCAN] > 0) {  // the 7x7 members' count scan and list fill
     hipLaunchKernelGGL(k_nb_scan_tiles, dim3(tot[NB_SCAN]), b, 0, s, members, at(NB_SCAN));
