@@ -327,20 +327,21 @@ __device__ __forceinline__ int half_tile(int b, int ntiles) {
 // The step's observer update, and per deposit: 7x7, the zero pass (blocks past the
 // observers'; the two touch disjoint arrays); bilinear, each observer's contributions binned
 // by the thread that updated it (no zero pass: the tile pass leaves every array it reads
-// clear).  The first tail_blocks blocks run the tile pass of `tail`, an earlier bilinear
-// deposit whose bins are complete (it reads only its own set's arrays).
+// clear).  The tail_blocks blocks after the observers' run the tile pass of `tail`, an
+// earlier bilinear deposit whose bins are complete (it reads only its own set's arrays); the
+// observer blocks go first, as they wait on atomics and the tile passes fill in around them.
 __global__ __launch_bounds__(kBlock) void k_obs_zero(Geom g, ObsPtrs o, int branch, int ntempread, int tail_blocks,
                                                      int obs_blocks, DepPtrs d, DepPtrs tail) {
   __shared__ DepTilePairLds L;
   const int b = blockIdx.x;
-  if (b < tail_blocks) {
-    dep_tile(g, tail, half_tile(b, dep_tiles(g)), L, true);
-  } else if (b < tail_blocks + obs_blocks) {
-    const int i = (b - tail_blocks) * kBlock + threadIdx.x;
+  if (b < obs_blocks) {
+    const int i = b * kBlock + threadIdx.x;
     if (i < o.n) {
       obs_update_one(o, i, branch, g.nx, g.ny, ntempread);
       if (d.mode == RBN_DEPOSIT_BILINEAR) bin_one(g, d, o, i);
     }
+  } else if (b < obs_blocks + tail_blocks) {
+    dep_tile(g, tail, half_tile(b - obs_blocks, dep_tiles(g)), L, true);
   } else {
     const int zb = tail_blocks + obs_blocks;
     zero_part(g.plane, d, b - zb, gridDim.x - zb);
