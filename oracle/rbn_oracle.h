@@ -19,10 +19,9 @@
  * closed forms, the wrap/reflect map, deposit known answers, the nudging recurrence, the
  * observer state machine's known answers across record boundaries and walls, and
  * independent numpy restatements of a whole truth-run step, a whole nudged step and a whole
- * particle step.  The reference-scratch initialisation (glibc rand/tanh) is "parity
- * unpinned": its arithmetic is glibc's, which the library calls the same way on the host,
- * and the unmodified reference, which could pin it, cannot be built here (it needs h5pcc /
- * parallel HDF5).
+ * particle step, and the reference-scratch initialisation against a numpy restatement of
+ * the jittered tanh front over an independent restatement of glibc's rand() (its published
+ * TYPE_3 additive feedback algorithm, checked against rand()'s known first outputs).
  */
 #ifndef RBN_ORACLE_H
 #define RBN_ORACLE_H

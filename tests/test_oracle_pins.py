@@ -853,3 +853,49 @@ def test_observer_map_takes_the_first_entry_of_a_repeated_id():
     st = o.observer_state()
     assert np.array_equal(st["xint"], np.array([18.0, 18.0, 17.0]))
     assert np.array_equal(st["Tint"], np.array([0.8, 0.8, 0.7]))
+
+
+# ------------------------------------------------------------------ scratch initialisation
+
+
+def _glibc_rand(n, seed=0):
+    """The first n values of glibc's rand() after srand(seed), restated from its published
+    algorithm (random_r.c, TYPE_3: an additive feedback generator of degree 31 and separation
+    3): seed 0 is taken as 1; r[i] = 16807 r[i-1] mod (2^31 - 1) for 0 < i < 31,
+    r[i] = r[i-31] for 31 <= i < 34, then r[i] = r[i-31] + r[i-3] mod 2^32; the first 310
+    sums are discarded and each output is a sum shifted right by one bit."""
+    r = [seed or 1]
+    for i in range(1, 31):
+        r.append((16807 * r[i - 1]) % 2147483647)
+    r += [r[i - 31] for i in range(31, 34)]
+    while len(r) < 344 + n:
+        r.append((r[-31] + r[-3]) & 0xFFFFFFFF)
+    return [x >> 1 for x in r[344:344 + n]]
+
+
+def test_glibc_rand_restatement_known_answers():
+    # glibc's widely published first outputs of rand() with the default seed
+    assert _glibc_rand(5) == [1804289383, 846930886, 1681692777, 1714636915, 1957747793]
+    assert _glibc_rand(3, seed=1) == _glibc_rand(3, seed=0)
+
+
+def test_scratch_initialisation_is_the_jittered_tanh_front():
+    """Trajectories/code/temppart_dump.c:688-701 and Nudging/code/tempvel_nudge.c:582-595 on
+    one rank (me = 0): rand() after srand(0) drawn once per node over the (nx+2) x (ny+2)
+    nodes with halos, i outer, j inner; the front centre ny/2 + (rand/RAND_MAX - 0.5) 2 epstan
+    with epstan = 10; rho = rhom1, u = v = 0.  The loop in j-outer order, or another seed,
+    misses by about 0.9."""
+    nx, ny, tup, tdown = 16, 12, 0.25, 1.5
+    o = Oracle(params(nx, ny, Tup=tup, Tdown=tdown))
+    try:
+        o.init_scratch_reference()
+        fl = o.fields()
+    finally:
+        o.close()
+    rnd = np.array(_glibc_rand((nx + 2) * (ny + 2)), dtype=np.float64).reshape(nx + 2, ny + 2)
+    yc = ny * 1 / 2.0 + ((1.0 / 2147483647) * rnd - 0.5) * 2.0 * 10.0
+    yy = np.arange(ny + 2, dtype=np.float64)[None, :] - 1.0
+    T = tdown + (tup - tdown) * (np.tanh((yy - yc) / 10.0) + 1.0) / 2.0
+    np.testing.assert_allclose(fl["T"], T[1:-1, 1:-1], rtol=0, atol=1e-15)
+    assert np.all(fl["rho"] == params(nx, ny).rhom1)
+    assert np.all(fl["u"] == 0) and np.all(fl["v"] == 0)
